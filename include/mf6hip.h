@@ -1,0 +1,195 @@
+/*
+ * mf6hip.h -- C ABI of libmf6hip.so, the MI355X (gfx950) implementation of
+ * the MODFLOW 6 IMS linear solve and GWF matrix fill.
+ *
+ * Conventions (copied from the reference's own C-ABI, srcbmi/mf6bmi.f90 and
+ * srcbmi/mf6xmi.F90): every entry point returns int 0 on success and a
+ * non-zero status on failure; the message of the last failure is fetched
+ * with mf6hip_last_error (cf. get_last_bmi_error, srcbmi/mf6bmiError.f90:57).
+ * A linear solve that does not converge is NOT a failure: it reports
+ * *icnvg = 0 and the outer loop decides (NumericalSolution.f90:1626-1629).
+ *
+ * Arrays cross the boundary in host memory, in the reference's own layout:
+ * CSR with the diagonal first in each row followed by ascending columns
+ * (src/Utilities/Matrix/SparseMatrix.f90:16, src/Utilities/Sparse.f90:217),
+ * 1-based ia/ja exactly as the Fortran memory manager holds them, FP64
+ * values, int32 indices.  Cell numbering is the DIS node number
+ * icol + ncol*(irow-1) + (ilay-1)*nrow*ncol (src/Utilities/GeomUtil.f90:90).
+ *
+ * One handle = one IMS solution with one structured (DIS) GWF model; calls
+ * on a handle are strictly sequential, as in MF6 (single-threaded host).
+ * The library holds the system resident in HBM between calls.
+ */
+#ifndef MF6HIP_H
+#define MF6HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- IMS settings ------------------------------------------------------ */
+
+/* precision of the device solve (opt-in; stock .ims files get MF6HIP_FP64) */
+#define MF6HIP_FP64 0        /* everything FP64 */
+#define MF6HIP_MIXED 1       /* FP32 ILU0 + FP32 SpMV inside FP64 iterative
+                                refinement; true residual, closure dots and
+                                the head update stay FP64 */
+#define MF6HIP_MIXED_FP16 2  /* as MIXED, ILU0 pivots stored in FP16 */
+
+/* ordering of the device ILU0 (the reference factors in natural order) */
+#define MF6HIP_ORDER_RB 0    /* red-black: colour (k+i+j) mod 2 */
+
+/* Mirrors ImsLinearSettingsType (src/Solution/LinearMethods/
+ * ImsLinearSettings.f90:13-32) field for field, plus the device opt-ins.
+ * Declared interoperable (bind(C)) on the Fortran side, see
+ * modflow6_amd/fortran/mf6hip_module.f90. */
+typedef struct mf6hip_ims_settings {
+  double dvclose;   /* INNER_DVCLOSE */
+  double rclose;    /* INNER_RCLOSE */
+  int icnvgopt;     /* 0 default, 1 STRICT, 2 L2NORM_RCLOSE, 3 RELATIVE, 4 */
+  int iter1;        /* INNER_MAXIMUM */
+  int ilinmeth;     /* 1 CG, 2 BICGSTAB */
+  int iscl;         /* SCALING_METHOD: must be 0 on the device path */
+  int iord;         /* REORDERING_METHOD: must be 0 on the device path */
+  int north;        /* NUMBER_ORTHOGONALIZATIONS */
+  double relax;     /* RELAXATION_FACTOR: 0 -> ILU0, > 0 -> MILU0 */
+  int level;        /* PRECONDITIONER_LEVELS: must be 0 (no ILUT) */
+  double droptol;   /* PRECONDITIONER_DROP_TOLERANCE: must be 0 */
+  int precision;    /* MF6HIP_FP64 / MF6HIP_MIXED / MF6HIP_MIXED_FP16 */
+  int ordering;     /* MF6HIP_ORDER_RB */
+  int nlay, nrow, ncol; /* DIS shape of the (single) model */
+  int inner_fp32_max;   /* MIXED: FP32 iterations per refinement step (0 = auto) */
+  double inner_fp32_rtol; /* MIXED: relative FP32 residual reduction per step (0 = auto) */
+} mf6hip_ims_settings;
+
+/* Per-inner-iteration record, the values ConvergenceSummaryType keeps for
+ * CSV_INNER_OUTPUT (src/Solution/ConvergenceSummary.f90:13-34,
+ * ImsLinearBase.f90:186-197).  Locations are 1-based node numbers. */
+typedef struct mf6hip_cnvg_out {
+  int capacity;     /* in: length of the arrays below (0 = none wanted) */
+  int count;        /* out: records written */
+  double *dvmax;    /* signed largest x change of the iteration */
+  double *rmax;     /* signed largest residual of the iteration */
+  double *alpha;    /* CG / BiCGSTAB alpha */
+  double *omega;    /* BiCGSTAB omega (0 for CG) */
+  int *locdv;
+  int *locr;
+} mf6hip_cnvg_out;
+
+/* --- drop-in linear solve (replaces imslinear_ap) ---------------------- */
+
+/* Create a solver for a neq x neq system with CSR structure ia/ja (1-based,
+ * diagonal first).  Replaces imslinear_ar (src/Solution/LinearMethods/
+ * ImsLinear.f90:111-339) / LinearSolverBaseType%initialize
+ * (src/Solution/LinearSolverBase.f90:32).  The structure must be the
+ * 7-point DIS stencil of nlay x nrow x ncol with every cell present. */
+int mf6hip_create(int neq, int nja, const int *ia, const int *ja,
+                  const mf6hip_ims_settings *settings, void **handle);
+
+/* One linear solve: factor, initial residual, CG or BiCGSTAB until the
+ * IMS convergence test (ImsLinearBase.f90:1101) passes or INNER_MAXIMUM is
+ * reached.  amat/rhs are the system after the sln_ls row fixes
+ * (NumericalSolution.f90:2434-2478); x holds the start heads on entry and the
+ * solution on return.  Replaces imslinear_ap (ImsLinear.f90:617-750) as
+ * called from sln_ls (NumericalSolution.f90:2604-2608), and
+ * LinearSolverBaseType%solve (LinearSolverBase.f90:44). */
+int mf6hip_solve(void *handle, int kstp, int kiter, const double *amat,
+                 const double *rhs, double *x, int *icnvg, int *in_iter,
+                 mf6hip_cnvg_out *out);
+
+/* Release the handle (LinearSolverBaseType%destroy, LinearSolverBase.f90:53). */
+int mf6hip_destroy(void *handle);
+
+/* Copy the last error message (NUL-terminated, truncated to buflen). */
+int mf6hip_last_error(char *buf, int buflen);
+
+/* --- device-resident GWF formulate + solve ----------------------------- */
+
+/* Model arrays of one DIS GWF model, natural node order, host memory; the
+ * library copies them to HBM.  Names follow the reference's NPF / STO /
+ * DIS members (gwf-npf.f90, gwf-sto.f90, Dis.f90). */
+typedef struct mf6hip_gwf_model {
+  int nlay, nrow, ncol;
+  const double *delr;       /* ncol */
+  const double *delc;       /* nrow */
+  const double *top;        /* nodes: cell top (layer 1 TOP, then BOTM above) */
+  const double *bot;        /* nodes: cell bottom */
+  const double *k11;        /* nodes: NPF K */
+  const double *k33;        /* nodes: NPF K33 (K when not given) */
+  const int *icelltype;     /* nodes: NPF ICELLTYPE (after negative->1) */
+  const int *ibound;        /* nodes: IDOMAIN/ibound before CHD (1 or 0) */
+  int inewton;              /* GWF NEWTON */
+  int inewtonur;            /* GWF NEWTON UNDER_RELAXATION */
+  int ivarcv;               /* NPF VARIABLECV */
+  int idewatcv;             /* NPF VARIABLECV DEWATERED */
+  int icellavg;             /* NPF ALTERNATIVE_CELL_AVERAGING (0 = harmonic) */
+  int has_sto;              /* STO package present */
+  const int *iconvert;      /* nodes (STO ICONVERT), may be NULL without STO */
+  const double *ss;         /* nodes (STO SS) */
+  const double *sy;         /* nodes (STO SY) */
+} mf6hip_gwf_model;
+
+/* package types, in the bndlist order MF6 fixes by type (gwf.f90:119-124,
+ * src/Utilities/Idm/ModelPackageInputs.f90:246-310): WEL, DRN, RCH, CHD */
+#define MF6HIP_PKG_WEL 1
+#define MF6HIP_PKG_DRN 2
+#define MF6HIP_PKG_RCH 3
+#define MF6HIP_PKG_CHD 4
+
+/* Upload the model (replaces npf_ar/sto_ar + calc_condsat,
+ * gwf-npf.f90:1748-2037: the saturated conductances are computed on the
+ * device). */
+int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *model);
+
+/* Set one boundary package's period data (bnd_rp).  nodelist is 1-based.
+ * WEL: v1 = q.  DRN: v1 = elevation, v2 = conductance.  RCH: v1 = recharge
+ * rate (nodelist = receiving cells).  CHD: v1 = head.  Packages are applied
+ * in MF6HIP_PKG order regardless of call order; a second call with the same
+ * type replaces that package's data. */
+int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
+                           const int *nodelist, const double *v1,
+                           const double *v2);
+
+/* Heads in / out (natural order, host memory). */
+int mf6hip_gwf_set_x(void *handle, const double *x);
+int mf6hip_gwf_get_x(void *handle, double *x);
+
+/* Start a time step (gwf_ad, gwf.f90:396-421 with chd_ad): xold = x, CHD
+ * cells take their head; iss = 1 for a steady-state period; delt the step
+ * length. */
+int mf6hip_gwf_advance(void *handle, int iss, double delt);
+
+/* sln_buildsystem (NumericalSolution.f90:1941-1991) for the GWF model:
+ * amat = 0, rhs = 0, then npf_cf, bnd_cf, npf_fc, sto_fc, bnd_fc, npf_fn,
+ * sto_fn in gwf_fc order (gwf.f90:446-555), bit-identical to the reference.
+ * If amat_out / rhs_out are not NULL the assembled CSR values (reference
+ * layout, nja) and rhs are copied back for inspection. */
+int mf6hip_gwf_formulate(void *handle, int kiter, double *amat_out,
+                         double *rhs_out);
+
+/* sln_ls (NumericalSolution.f90:2404-2615) on the device-resident system:
+ * xtemp = x, row fixes, the linear solve, then sln_get_dxmax (:3122).
+ * hncg receives the signed largest head change, lrch its 1-based node. */
+int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, int *icnvg,
+                            int *in_iter, double *hncg, int *lrch,
+                            mf6hip_cnvg_out *out);
+
+/* The tail of NumericalSolution%solve (:1722-1760) for one model when the
+ * outer iteration has not converged: sln_calcdx (no under-relaxation,
+ * nonmeth = 0), then npf_nur (gwf-npf.f90:705-741) if Newton
+ * under-relaxation is on.  On return *inewtonur tells whether NUR moved a
+ * head, *dxold_max is sln_maxval of dxold, and, when NUR applied, *hncg /
+ * *lrch are recomputed (sln_get_dxmax). */
+int mf6hip_gwf_outer_update(void *handle, int kiter, int *inewtonur,
+                            double *dxold_max, double *hncg, int *lrch);
+
+/* Timing of the last mf6hip_gwf_linear_solve on the device stream (HIP
+ * events): total milliseconds and the average duration of its dominant
+ * kernel (the SpMV) with its launch count. */
+int mf6hip_last_timing(void *handle, double *solve_ms, double *spmv_avg_ms,
+                       int *spmv_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MF6HIP_H */

@@ -1,0 +1,143 @@
+"""ctypes binding of libmf6hip.so (the C ABI declared in include/mf6hip.h).
+
+The library is built in-tree (``modflow6_amd/libmf6hip.so``) by
+``__graft_entry__.build()`` / ``make -C modflow6_amd/csrc``.  There is no
+fallback: if the library is missing or a call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmf6hip.so")
+
+MF6HIP_FP64 = 0
+MF6HIP_MIXED = 1
+MF6HIP_MIXED_FP16 = 2
+MF6HIP_ORDER_RB = 0
+PKG = {"WEL": 1, "DRN": 2, "RCH": 3, "CHD": 4}
+
+# every entry point declared in include/mf6hip.h
+EXPORTS = [
+    "mf6hip_create", "mf6hip_solve", "mf6hip_destroy", "mf6hip_last_error",
+    "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_x",
+    "mf6hip_gwf_get_x", "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
+    "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
+    "mf6hip_last_timing",
+]
+
+
+class ImsSettingsC(C.Structure):
+    """mf6hip_ims_settings (mirrors ImsLinearSettingsType)."""
+    _fields_ = [("dvclose", C.c_double), ("rclose", C.c_double),
+                ("icnvgopt", C.c_int), ("iter1", C.c_int),
+                ("ilinmeth", C.c_int), ("iscl", C.c_int), ("iord", C.c_int),
+                ("north", C.c_int), ("relax", C.c_double),
+                ("level", C.c_int), ("droptol", C.c_double),
+                ("precision", C.c_int), ("ordering", C.c_int),
+                ("nlay", C.c_int), ("nrow", C.c_int), ("ncol", C.c_int),
+                ("inner_fp32_max", C.c_int),
+                ("inner_fp32_rtol", C.c_double)]
+
+
+class CnvgOutC(C.Structure):
+    _fields_ = [("capacity", C.c_int), ("count", C.c_int),
+                ("dvmax", C.POINTER(C.c_double)),
+                ("rmax", C.POINTER(C.c_double)),
+                ("alpha", C.POINTER(C.c_double)),
+                ("omega", C.POINTER(C.c_double)),
+                ("locdv", C.POINTER(C.c_int)),
+                ("locr", C.POINTER(C.c_int))]
+
+
+class GwfModelC(C.Structure):
+    _fields_ = [("nlay", C.c_int), ("nrow", C.c_int), ("ncol", C.c_int),
+                ("delr", C.POINTER(C.c_double)),
+                ("delc", C.POINTER(C.c_double)),
+                ("top", C.POINTER(C.c_double)),
+                ("bot", C.POINTER(C.c_double)),
+                ("k11", C.POINTER(C.c_double)),
+                ("k33", C.POINTER(C.c_double)),
+                ("icelltype", C.POINTER(C.c_int)),
+                ("ibound", C.POINTER(C.c_int)),
+                ("inewton", C.c_int), ("inewtonur", C.c_int),
+                ("ivarcv", C.c_int), ("idewatcv", C.c_int),
+                ("icellavg", C.c_int), ("has_sto", C.c_int),
+                ("iconvert", C.POINTER(C.c_int)),
+                ("ss", C.POINTER(C.c_double)),
+                ("sy", C.POINTER(C.c_double))]
+
+
+class Mf6HipError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load libmf6hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise Mf6HipError(
+            f"{LIBPATH} not found: build it with __graft_entry__.build() or "
+            "make -C modflow6_amd/csrc")
+    L = C.CDLL(LIBPATH)
+    i, d, vp = C.c_int, C.c_double, C.c_void_p
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    L.mf6hip_create.argtypes = [i, i, ip, ip, C.POINTER(ImsSettingsC),
+                                C.POINTER(vp)]
+    L.mf6hip_solve.argtypes = [vp, i, i, dp, dp, dp, ip, ip,
+                               C.POINTER(CnvgOutC)]
+    L.mf6hip_destroy.argtypes = [vp]
+    L.mf6hip_last_error.argtypes = [C.c_char_p, i]
+    L.mf6hip_gwf_create.argtypes = [vp, C.POINTER(GwfModelC)]
+    L.mf6hip_gwf_set_package.argtypes = [vp, i, i, ip, dp, dp]
+    L.mf6hip_gwf_set_x.argtypes = [vp, dp]
+    L.mf6hip_gwf_get_x.argtypes = [vp, dp]
+    L.mf6hip_gwf_advance.argtypes = [vp, i, d]
+    L.mf6hip_gwf_formulate.argtypes = [vp, i, dp, dp]
+    L.mf6hip_gwf_linear_solve.argtypes = [vp, i, i, ip, ip, dp, ip,
+                                          C.POINTER(CnvgOutC)]
+    L.mf6hip_gwf_outer_update.argtypes = [vp, i, ip, dp, dp, ip]
+    L.mf6hip_last_timing.argtypes = [vp, dp, dp, ip]
+    L.mf6hip_set_timing.argtypes = [vp, i, i]
+    L.mf6hip_apply_op.argtypes = [vp, i, dp, dp]
+    L.mf6hip_load_factor.argtypes = [vp, dp]
+    L.mf6hip_get_system.argtypes = [vp, dp, dp]
+    for nm in EXPORTS + ["mf6hip_set_timing", "mf6hip_apply_op",
+                         "mf6hip_load_factor", "mf6hip_get_system"]:
+        getattr(L, nm).restype = i
+    _LIB = L
+    return L
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    load().mf6hip_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise Mf6HipError(f"{what} failed (status {rc}): {last_error()}")
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))

@@ -1,0 +1,214 @@
+"""The synthetic GWF workloads, as plain input data (what an MF6 deck holds).
+
+Each builder returns a :class:`GwfCase` -- DIS shape and geometry, NPF / STO
+properties, boundary-package period data, TDIS periods and IMS settings --
+that both the device solver (``modflow6_amd.solution``) and the CPU oracle
+consume, so the two always see identical inputs.  All randomness comes from
+``numpy.random.default_rng(seed)``.
+
+Workloads (sizes at scale 1):
+
+* ``steady_small``      3 x 100 x 100 steady confined, iid ln K (sigma 1),
+                        CHD 10 / 0 m on the first / last column, RCH 1e-4,
+                        20 wells at -500 m3/d, CG + ILU0.
+* ``steady_10m``        10 x 1000 x 1000 steady confined, exponential-
+                        covariance ln K (sigma 2, correlation 20 cells),
+                        CHD / RCH as above, 2000 wells at -200 m3/d.
+* ``transient_newton``  5 x 500 x 500 unconfined Newton-Raphson with
+                        under-relaxation, SS 1e-5, SY 0.2, iid ln K (sigma
+                        1.5), RCH U[0, 5e-4], layer-1 drains at top - 1 m
+                        (C = 100), 500 wells at -1000 m3/d, 10 periods x
+                        30 d x 5 steps, BiCGSTAB + ILU0.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import fields
+
+
+@dataclass
+class ImsOptions:
+    """IMS file values (sln-ims.dfn); names follow ImsLinearSettingsType and
+    the NONLINEAR block of NumericalSolution."""
+    outer_dvclose: float = 1e-6
+    outer_maximum: int = 50
+    inner_maximum: int = 5000
+    inner_dvclose: float = 1e-6
+    inner_rclose: float = 1e-6
+    icnvgopt: int = 0
+    linear_acceleration: str = "CG"      # CG | BICGSTAB
+    relaxation_factor: float = 0.0       # > 0 -> MILU0
+    number_orthogonalizations: int = 0
+    under_relaxation: str = "NONE"       # only NONE on the device path
+
+    @property
+    def ilinmeth(self):
+        return 1 if self.linear_acceleration.upper() == "CG" else 2
+
+
+@dataclass
+class Package:
+    """Boundary package period data (0-based node numbers).
+    kind: WEL (v1 = q), DRN (v1 = elev, v2 = cond), RCH (v1 = rate),
+    CHD (v1 = head)."""
+    kind: str
+    nodes: np.ndarray
+    v1: np.ndarray
+    v2: Optional[np.ndarray] = None
+
+
+@dataclass
+class GwfCase:
+    name: str
+    nlay: int
+    nrow: int
+    ncol: int
+    delr: np.ndarray
+    delc: np.ndarray
+    top2d: np.ndarray          # (nrow, ncol)
+    botm: np.ndarray           # (nlay, nrow, ncol)
+    k11: np.ndarray            # (nodes,)
+    k33: np.ndarray            # (nodes,)
+    icelltype: np.ndarray      # (nodes,) int
+    strt: np.ndarray           # (nodes,)
+    packages: List[Package]
+    ims: ImsOptions
+    periods: List[Tuple[float, int, bool]]   # (perlen, nstp, steady)
+    newton: bool = False
+    newton_ur: bool = False
+    iconvert: Optional[np.ndarray] = None
+    ss: Optional[np.ndarray] = None
+    sy: Optional[np.ndarray] = None
+    meta: Dict = field(default_factory=dict)
+
+    @property
+    def nodes(self):
+        return self.nlay * self.nrow * self.ncol
+
+    @property
+    def has_sto(self):
+        return self.ss is not None
+
+    def cell_top_bot(self):
+        nrc = self.nrow * self.ncol
+        bot = self.botm.reshape(-1).astype(np.float64)
+        top = np.empty_like(bot)
+        top[:nrc] = self.top2d.reshape(-1)
+        top[nrc:] = bot[:-nrc]
+        return top, bot
+
+
+def _node(k, i, j, nrow, ncol):
+    return (k * nrow + i) * ncol + j
+
+
+def _chd_columns(nlay, nrow, ncol, hleft, hright):
+    k, i = np.meshgrid(np.arange(nlay), np.arange(nrow), indexing="ij")
+    k = k.reshape(-1)
+    i = i.reshape(-1)
+    left = _node(k, i, 0, nrow, ncol)
+    right = _node(k, i, ncol - 1, nrow, ncol)
+    # list order as an MF6 CHD block would be written: layer by layer,
+    # row by row, first column then last
+    nodes = np.stack([left, right], axis=1).reshape(-1)
+    heads = np.tile([hleft, hright], k.size).astype(np.float64)
+    return Package("CHD", nodes.astype(np.int64), heads)
+
+
+def _wells(rng, nlay, nrow, ncol, count, q, col_lo=1, col_hi=None,
+           layers=None):
+    """`count` unique seeded cells away from the CHD columns."""
+    col_hi = ncol - 1 if col_hi is None else col_hi
+    layers = np.arange(nlay) if layers is None else np.asarray(layers)
+    chosen = set()
+    out = []
+    while len(out) < count:
+        k = int(rng.choice(layers))
+        i = int(rng.integers(0, nrow))
+        j = int(rng.integers(col_lo, col_hi))
+        nd = _node(k, i, j, nrow, ncol)
+        if nd in chosen:
+            continue
+        chosen.add(nd)
+        out.append(nd)
+    return Package("WEL", np.array(out, dtype=np.int64),
+                   np.full(count, q, dtype=np.float64))
+
+
+def _rch_top(nrow, ncol, rate):
+    nodes = np.arange(nrow * ncol, dtype=np.int64)   # layer-1 cells
+    r = np.broadcast_to(np.asarray(rate, dtype=np.float64), (nrow * ncol,))
+    return Package("RCH", nodes, np.ascontiguousarray(r))
+
+
+def steady_small(seed=20260101, nlay=3, nrow=100, ncol=100, nwel=20):
+    rng = np.random.default_rng(seed)
+    nodes = nlay * nrow * ncol
+    k11 = fields.iid_lognormal(nodes, np.log(10.0), 1.0, rng)
+    botm = np.stack([np.full((nrow, ncol), -10.0 * (k + 1))
+                     for k in range(nlay)])
+    pk = [_wells(rng, nlay, nrow, ncol, nwel, -500.0),
+          _rch_top(nrow, ncol, 1e-4),
+          _chd_columns(nlay, nrow, ncol, 10.0, 0.0)]
+    return GwfCase(
+        name=f"steady_small_{nlay}x{nrow}x{ncol}", nlay=nlay, nrow=nrow,
+        ncol=ncol, delr=np.full(ncol, 100.0), delc=np.full(nrow, 100.0),
+        top2d=np.zeros((nrow, ncol)), botm=botm, k11=k11, k33=0.1 * k11,
+        icelltype=np.zeros(nodes, dtype=np.int32), strt=np.zeros(nodes),
+        packages=pk, ims=ImsOptions(), periods=[(1.0, 1, True)],
+        meta=dict(seed=seed))
+
+
+def steady_10m(seed=20260102, nlay=10, nrow=1000, ncol=1000, nwel=2000,
+               corr_len=20.0, sigma=2.0):
+    rng = np.random.default_rng(seed)
+    nodes = nlay * nrow * ncol
+    k11 = fields.correlated_lognormal((nlay, nrow, ncol), np.log(10.0),
+                                      sigma, corr_len, rng).reshape(nodes)
+    botm = np.stack([np.full((nrow, ncol), -10.0 * (k + 1))
+                     for k in range(nlay)])
+    pk = [_wells(rng, nlay, nrow, ncol, nwel, -200.0),
+          _rch_top(nrow, ncol, 1e-4),
+          _chd_columns(nlay, nrow, ncol, 10.0, 0.0)]
+    return GwfCase(
+        name=f"steady_10m_{nlay}x{nrow}x{ncol}", nlay=nlay, nrow=nrow,
+        ncol=ncol, delr=np.full(ncol, 50.0), delc=np.full(nrow, 50.0),
+        top2d=np.zeros((nrow, ncol)), botm=botm, k11=k11, k33=0.1 * k11,
+        icelltype=np.zeros(nodes, dtype=np.int32), strt=np.zeros(nodes),
+        packages=pk, ims=ImsOptions(inner_maximum=20000),
+        periods=[(1.0, 1, True)], meta=dict(seed=seed, corr_len=corr_len,
+                                            sigma=sigma))
+
+
+def transient_newton(seed=20260103, nlay=5, nrow=500, ncol=500, nwel=500,
+                     nper=10, nstp=5, perlen=30.0):
+    rng = np.random.default_rng(seed)
+    nodes = nlay * nrow * ncol
+    k11 = fields.iid_lognormal(nodes, np.log(10.0), 1.5, rng)
+    jj = np.arange(ncol, dtype=np.float64)
+    top_row = 50.0 * (1.0 - jj / (ncol - 1))
+    top2d = np.tile(top_row, (nrow, 1))
+    botm = np.stack([top2d - 10.0 * (k + 1) for k in range(nlay)])
+    rch = rng.uniform(0.0, 5e-4, nrow * ncol)
+    drn_nodes = np.arange(nrow * ncol, dtype=np.int64)
+    drn = Package("DRN", drn_nodes, top2d.reshape(-1) - 1.0,
+                  np.full(nrow * ncol, 100.0))
+    wel = _wells(rng, nlay, nrow, ncol, nwel, -1000.0, col_lo=0,
+                 col_hi=ncol)
+    strt = np.repeat((top2d - 0.5).reshape(1, -1), nlay, axis=0).reshape(-1)
+    ims = ImsOptions(outer_dvclose=1e-5, outer_maximum=200,
+                     inner_maximum=300, inner_dvclose=1e-6,
+                     inner_rclose=1e-2, linear_acceleration="BICGSTAB")
+    return GwfCase(
+        name=f"transient_newton_{nlay}x{nrow}x{ncol}", nlay=nlay, nrow=nrow,
+        ncol=ncol, delr=np.full(ncol, 100.0), delc=np.full(nrow, 100.0),
+        top2d=top2d, botm=botm, k11=k11, k33=k11.copy(),
+        icelltype=np.ones(nodes, dtype=np.int32), strt=strt,
+        packages=[wel, drn, _rch_top(nrow, ncol, rch)], ims=ims,
+        periods=[(perlen, nstp, False)] * nper, newton=True, newton_ur=True,
+        iconvert=np.ones(nodes, dtype=np.int32), ss=np.full(nodes, 1e-5),
+        sy=np.full(nodes, 0.2), meta=dict(seed=seed))

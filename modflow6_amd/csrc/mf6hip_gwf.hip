@@ -1,0 +1,902 @@
+// mf6hip_gwf.hip -- device-resident GWF formulate and the outer-loop pieces.
+//
+// Reference routines (src/Model/GroundWaterFlow, src/Model/ModelUtilities,
+// src/Solution/NumericalSolution.f90):
+//   calc_condsat gwf-npf.f90:1950-2037, npf_cf :444-470 (thksat :775-794),
+//   npf_fc :474-574, npf_fn :578-698, npf_nur :705-741,
+//   hcond / vcond / condmean GwfConductanceUtils.f90:43-284,
+//   sto_fc gwf-sto.f90:226-345, sto_fn :353-445, SsTerms / SyTerms /
+//   SsCapacity / SyCapacity GwfStorageUtils.f90:32-180,
+//   sQuadraticSaturation(Derivative) SmoothingFunctions.f90:275-406,
+//   bnd_fc BoundaryPackage.f90:453-479, wel_cf gwf-wel.f90:296-332,
+//   rch_cf gwf-rch.f90:303-353, drn_cf gwf-drn.f90:340-373,
+//   gwf_fc order gwf.f90:466-555, sln_ls row fixes NumericalSolution.f90:
+//   2434-2478, sln_get_dxmax :3122-3153, sln_calcdx :2912, sln_maxval :2875.
+//
+// The reference scatters each connection's terms into both rows while
+// looping over nodes in ascending order.  Here one thread owns one row and
+// gathers: the diagonal and rhs of a row receive their terms in ascending
+// neighbour order -- the order the scatter loop delivers them -- and every
+// pair term is computed with the reference's argument order (lower node
+// first), so each entry is bit-identical to the reference's (no FMA
+// contraction; build with -ffp-contract=off).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mf6hip_handle.h"
+
+#define CK(x)                         \
+  do {                                \
+    int rc_ = (x);                    \
+    if (rc_ != MF6HIP_OK) return rc_; \
+  } while (0)
+
+static inline int ggrid(int n) {
+  int g = (n + MF6_BLOCK - 1) / MF6_BLOCK;
+  if (g > MF6_MAXBLK) g = MF6_MAXBLK;
+  return g < 1 ? 1 : g;
+}
+
+template <typename T>
+static int galloc(T **p, size_t n) {
+  if (*p) return MF6HIP_OK;
+  if (hipMalloc((void **)p, n * sizeof(T) + 256) != hipSuccess) {
+    mf6hip_set_error("hipMalloc failed (GWF model arrays)");
+    return MF6HIP_EHIP;
+  }
+  return MF6HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stateless helpers (reference statement order)
+// ---------------------------------------------------------------------------
+
+__device__ inline double sqsat(double top, double bot, double x, double eps) {
+  double teps = eps, y;
+  double b = top - bot;
+  if (b > 0.0) {
+    double br;
+    if (x < bot) br = 0.0;
+    else if (x > top) br = 1.0;
+    else br = (x - bot) / b;
+    double av = 1.0 / (1.0 - teps);
+    double bri = 1.0 - br;
+    if (br < teps) y = av * 0.5 * (br * br) / teps;
+    else if (br < (1.0 - teps)) y = av * br + 0.5 * (1.0 - av);
+    else if (br < 1.0) y = 1.0 - ((av * 0.5 * (bri * bri)) / teps);
+    else y = 1.0;
+  } else {
+    y = (x < bot) ? 0.0 : 1.0;
+  }
+  return y;
+}
+
+__device__ inline double sqsat_deriv(double top, double bot, double x, double eps) {
+  double teps = eps, y, br;
+  double b = top - bot;
+  if (x < bot) br = 0.0;
+  else if (x > top) br = 1.0;
+  else br = (x - bot) / b;
+  double av = 1.0 / (1.0 - teps);
+  double bri = 1.0 - br;
+  if (br < teps) y = av * br / teps;
+  else if (br < (1.0 - teps)) y = av;
+  else if (br < 1.0) y = av * bri / teps;
+  else y = 0.0;
+  return y / b;
+}
+
+__device__ inline double logmean(double d1, double d2) {
+  double drat = d2 / d1;
+  if (drat <= 0.995 || drat >= 1.005) return (d2 - d1) / log(drat);
+  return 0.5 * (d1 + d2);
+}
+
+__device__ inline double condmean(double k1, double k2, double thick1, double thick2,
+                                  double cl1, double cl2, double width, int iavg) {
+  double t1 = k1 * thick1;
+  double t2 = k2 * thick2;
+  if (iavg == 0) return (t1 * t2 > 0.0) ? width * t1 * t2 / (t1 * cl2 + t2 * cl1) : 0.0;
+  if (iavg == 1) {
+    double tm = (t1 * t2 > 0.0) ? logmean(t1, t2) : 0.0;
+    return tm * width / (cl1 + cl2);
+  }
+  if (iavg == 2) {
+    double km = (k1 * k2 > 0.0) ? logmean(k1, k2) : 0.0;
+    return km * 0.5 * (thick1 + thick2) * width / (cl1 + cl2);
+  }
+  double denom = (k1 * cl2 + k2 * cl1);
+  double km = denom > 0.0 ? k1 * k2 / denom : 0.0;
+  return km * 0.5 * (thick1 + thick2) * width;
+}
+
+__device__ inline double hcond(int ibdn, int ibdm, int ictn, int ictm, int iupstream,
+                               int icellavg, double condsat, double hn, double hm,
+                               double satn, double satm, double hkn, double hkm,
+                               double topn, double topm, double botn, double botm,
+                               double cln, double clm, double fawidth) {
+  if (ibdn == 0 || ibdm == 0) return 0.0;
+  if (ictn == 0 && ictm == 0) return condsat;
+  if (iupstream == 1) {
+    double sat_up = (hn > hm) ? satn : satm;
+    return sat_up * condsat;
+  }
+  double thksatn = satn * (topn - botn);
+  double thksatm = satm * (topm - botm);
+  return condmean(hkn, hkm, thksatn, thksatm, cln, clm, fawidth, icellavg);
+}
+
+__device__ inline double vcond(int ibdn, int ibdm, int ictn, int ictm, int ivarcv,
+                               int idewatcv, double condsat, double hn, double hm,
+                               double vkn, double vkm, double satn, double satm,
+                               double topn, double topm, double botn, double botm,
+                               double flowarea) {
+  if (ibdn == 0 || ibdm == 0) return 0.0;
+  if (ivarcv == 0) return condsat;
+  if (ictn == 0 && ictm == 0) return condsat;
+  if (hn >= topn && hm >= topm) return condsat;
+  double satntmp = satn, satmtmp = satm;
+  if (idewatcv == 0) {
+    if (botn > botm) satmtmp = 1.0;
+    else satntmp = 1.0;
+  }
+  double bovk1 = satntmp * (topn - botn) * 0.5 / vkn;
+  double bovk2 = satmtmp * (topm - botm) * 0.5 / vkm;
+  double denom = (bovk1 + bovk2);
+  return denom != 0.0 ? flowarea / denom : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// model arrays (position order) and connection geometry
+// ---------------------------------------------------------------------------
+
+struct GwfDev {
+  Layout L;
+  const double *top, *bot, *area, *k11, *k33, *ss, *sy, *delr, *delc, *condsat;
+  const int *icelltype, *iconvert, *ibound;
+  int inewton, ivarcv, idewatcv, icellavg, has_sto;
+  double satomega;
+};
+
+__device__ inline double cell_sat(const GwfDev &g, int p, double h) {
+  // npf_cf: sat of convertible cells; confined cells keep sat = 1
+  if (g.icelltype[p] == 0) return 1.0;
+  if (g.ibound[p] == 0) return 0.0;
+  double s;
+  if (h >= g.top[p]) s = 1.0;
+  else s = (h - g.bot[p]) / (g.top[p] - g.bot[p]);
+  if (g.inewton != 0) s = sqsat(g.top[p], g.bot[p], h, g.satomega);
+  return s;
+}
+
+// geometry of the connection between a (lower node) and b (higher node);
+// dir 0 = right (j+1), 1 = front (i+1), 2 = down (k+1)
+struct Conn {
+  int ihc;
+  double cl1, cl2, hwva;
+};
+
+__device__ inline Conn conn_geom(const GwfDev &g, int dir, int pa, int pb, int ia_, int ja_) {
+  Conn c;
+  if (dir == 0) {
+    c.ihc = 1; c.cl1 = 0.5 * g.delr[ja_]; c.cl2 = 0.5 * g.delr[ja_ + 1]; c.hwva = g.delc[ia_];
+  } else if (dir == 1) {
+    c.ihc = 1; c.cl1 = 0.5 * g.delc[ia_]; c.cl2 = 0.5 * g.delc[ia_ + 1]; c.hwva = g.delr[ja_];
+  } else {
+    c.ihc = 0;
+    c.cl1 = 0.5 * (g.top[pa] - g.bot[pa]);
+    c.cl2 = 0.5 * (g.top[pb] - g.bot[pb]);
+    c.hwva = g.delr[ja_] * g.delc[ia_];
+  }
+  return c;
+}
+
+// npf_fc conductance of the pair (a < b) as the reference computes it
+__device__ inline double pair_cond(const GwfDev &g, int dir, int pa, int pb, int ia_,
+                                   int ja_, double ha, double hb, double csat) {
+  Conn c = conn_geom(g, dir, pa, pb, ia_, ja_);
+  double sata = cell_sat(g, pa, ha), satb = cell_sat(g, pb, hb);
+  if (c.ihc == 0) {
+    return vcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.ivarcv,
+                 g.idewatcv, csat, ha, hb, g.k33[pa], g.k33[pb], sata, satb, g.top[pa],
+                 g.top[pb], g.bot[pa], g.bot[pb], c.hwva);
+  }
+  return hcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.inewton,
+               g.icellavg, csat, ha, hb, sata, satb, g.k11[pa], g.k11[pb], g.top[pa],
+               g.top[pb], g.bot[pa], g.bot[pb], c.cl1, c.cl2, c.hwva);
+}
+
+// calc_condsat (upper connections, saturation 1): plane dir at position of
+// the lower node a
+__global__ void k_condsat(GwfDev g, double *__restrict__ condsat) {
+  const Layout &L = g.L;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    for (int dir = 0; dir < 3; dir++) {
+      int kb = k + (dir == 2), ib = i + (dir == 1), jb = j + (dir == 0);
+      double cs = 0.0;
+      if (kb < L.nlay && ib < L.nrow && jb < L.ncol) {
+        int pb = cell_pos(L, kb, ib, jb);
+        Conn cn = conn_geom(g, dir, p, pb, i, j);
+        double topn = g.top[p], botn = g.bot[p], topm = g.top[pb], botm = g.bot[pb];
+        if (cn.ihc == 0) {
+          cs = vcond(1, 1, 1, 1, 1, 1, 1.0, botn, botm, g.k33[p], g.k33[pb], 1.0, 1.0, topn,
+                     topm, botn, botm, cn.hwva);
+        } else {
+          cs = hcond(1, 1, 1, 1, 0, g.icellavg, 1.0, topn, topm, 1.0, 1.0, g.k11[p],
+                     g.k11[pb], topn, topm, botn, botm, cn.cl1, cn.cl2, cn.hwva);
+        }
+      }
+      condsat[dir * L.NP + p] = cs;
+    }
+  }
+}
+
+// bnd_cf of the WEL / DRN / RCH packages
+__global__ void k_bnd_cf(int kind, int nb, const int *__restrict__ nodes,
+                         const double *__restrict__ v1, const double *__restrict__ v2,
+                         Layout L, const int *__restrict__ ibound,
+                         const double *__restrict__ x, const double *__restrict__ area,
+                         double *__restrict__ hcof, double *__restrict__ brhs) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
+    int p = pos_of_natural(L, nodes[e]);
+    if (kind == MF6HIP_PKG_WEL) {
+      hcof[e] = 0.0;
+      brhs[e] = ibound[p] <= 0 ? 0.0 : -v1[e];
+    } else if (kind == MF6HIP_PKG_DRN) {
+      if (ibound[p] <= 0) {
+        hcof[e] = 0.0;
+        brhs[e] = 0.0;
+      } else {
+        double cdrn = v2[e], drnbot = v1[e];
+        double fact = x[p] <= drnbot ? 0.0 : 1.0;
+        brhs[e] = -fact * cdrn * drnbot;
+        hcof[e] = -fact * cdrn;
+      }
+    } else if (kind == MF6HIP_PKG_RCH) {
+      hcof[e] = 0.0;
+      double rr = -v1[e] * area[p];
+      brhs[e] = ibound[p] <= 0 ? 0.0 : rr;
+    } else {
+      hcof[e] = 0.0;
+      brhs[e] = 0.0;
+    }
+  }
+}
+
+struct BndDev {
+  const int *off, *ent;
+  const double *hcof[5], *brhs[5];
+};
+
+// sln_buildsystem for one model: one thread per row
+__global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, int iss, double delt,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ xold,
+                                                         double *__restrict__ A,
+                                                         double *__restrict__ rhs) {
+  const Layout &L = g.L;
+  const int N = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    const double hcell = x[p];
+    // neighbour table in ascending natural order: up, back, left, right,
+    // front, down.  nb = position, dir = connection plane, lower = pair
+    // (nb, n) has nb first
+    int npos[6];
+    int pres[6];
+    pres[0] = k > 0;
+    pres[1] = i > 0;
+    pres[2] = j > 0;
+    pres[3] = j < L.ncol - 1;
+    pres[4] = i < L.nrow - 1;
+    pres[5] = k < L.nlay - 1;
+    npos[0] = pres[0] ? p - L.HP : -1;
+    npos[5] = pres[5] ? p + L.HP : -1;
+    {
+      const int ob = (1 - c) * L.H + k * L.HP;
+      npos[1] = pres[1] ? ob + hn(l, -L.ncol) : -1;
+      npos[2] = pres[2] ? ob + hn(l, -1) : -1;
+      npos[3] = pres[3] ? ob + hn(l, 1) : -1;
+      npos[4] = pres[4] ? ob + hn(l, L.ncol) : -1;
+    }
+    const int dirs[6] = {2, 1, 0, 0, 1, 2};
+    double off[6];
+    double diag = 0.0;
+    // --- npf_fc
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      off[s] = 0.0;
+      if (!pres[s]) continue;
+      int m = npos[s];
+      int dir = dirs[s];
+      double cond;
+      if (s < 3) {   // m < n: pair (m, n), condsat stored at m
+        int im = i - (s == 1), jm = j - (s == 2);
+        cond = pair_cond(g, dir, m, p, im, jm, x[m], hcell, g.condsat[dir * N + m]);
+      } else {
+        cond = pair_cond(g, dir, p, m, i, j, hcell, x[m], g.condsat[dir * N + p]);
+      }
+      off[s] = off[s] + cond;
+      diag = diag + (-cond);
+    }
+    double b = 0.0;
+    // --- sto_fc
+    if (g.has_sto && !iss && g.ibound[p] >= 1) {
+      double tled = 1.0 / delt;
+      double tp = g.top[p], bt = g.bot[p];
+      double snold, snnew;
+      if (g.iconvert[p] == 0) {
+        snold = 1.0;
+        snnew = 1.0;
+      } else {
+        snold = sqsat(tp, bt, xold[p], 0.0);
+        snnew = sqsat(tp, bt, hcell, 0.0);
+      }
+      double sc1 = g.ss[p] * (tp - bt) * g.area[p];
+      double rho1 = sc1 * tled;
+      double rho1old = rho1;
+      double aterm = -rho1 * snnew;
+      double rhsterm = 0.0;
+      if (g.iconvert[p] != 0) {
+        double tthk = tp - bt;
+        double zold = bt + 0.5 * tthk * snold;
+        double znew = bt + 0.5 * tthk * snnew;
+        rhsterm = -rho1old * snold * (xold[p] - zold) - rho1 * snnew * znew;
+      } else {
+        rhsterm = -rho1old * snold * xold[p];
+      }
+      diag = diag + aterm;
+      b = b + rhsterm;
+      if (g.iconvert[p] != 0) {
+        double rho2 = g.sy[p] * g.area[p] * tled;
+        double rho2old = rho2;
+        double tthk = tp - bt;
+        double at = 0.0, rt;
+        if (snnew < 1.0) {
+          if (snnew > 0.0) {
+            at = -rho2;
+            rt = -rho2old * tthk * snold - rho2 * bt;
+          } else {
+            rt = tthk * (0.0 - rho2old * snold);
+          }
+        } else {
+          rt = tthk * (rho2 * snnew - rho2old * snold);
+        }
+        diag = diag + at;
+        b = b + rt;
+      }
+    }
+    // --- bnd_fc (WEL, DRN, RCH in bndlist order; CHD adds nothing)
+    for (int e = bd.off[p]; e < bd.off[p + 1]; e++) {
+      int code = bd.ent[e];
+      int pk = code >> 28, idx = code & 0x0fffffff;
+      b = b + bd.brhs[pk][idx];
+      diag = diag + bd.hcof[pk][idx];
+    }
+    // --- npf_fn
+    if (g.inewton) {
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        if (!pres[s]) continue;
+        int dir = dirs[s];
+        if (dir == 2 && g.ivarcv == 0) continue;
+        int m = npos[s];
+        int pa, pb_;
+        double csat;
+        if (s < 3) { pa = m; pb_ = p; csat = g.condsat[dir * N + m]; }
+        else { pa = p; pb_ = m; csat = g.condsat[dir * N + p]; }
+        double ha = x[pa], hb = x[pb_];
+        int iups = pb_;
+        if (hb < ha) iups = pa;
+        int idn = (iups == pa) ? pb_ : pa;
+        if (g.icelltype[iups] == 0) continue;
+        double topup = g.top[iups], botup = g.bot[iups];
+        double consterm = -csat * (x[iups] - x[idn]);
+        double derv = sqsat_deriv(topup, botup, x[iups], g.satomega);
+        if (iups == pa) {
+          double term = consterm * derv;
+          if (pa == p) {          // this row is n (upstream)
+            b = b + term * ha;
+            diag = diag + term;
+          } else {                // this row is m
+            b = b - term * ha;
+            if (g.ibound[p] > 0) off[s] = off[s] + (-term);
+          }
+        } else {
+          double term = -consterm * derv;
+          if (pa == p) {          // row n, m upstream
+            b = b + term * hb;
+            if (g.ibound[p] > 0) off[s] = off[s] + term;
+          } else {                // row m (upstream)
+            b = b - term * hb;
+            diag = diag + (-term);
+          }
+        }
+      }
+      // --- sto_fn
+      if (g.has_sto && !iss && g.ibound[p] > 0 && g.iconvert[p] != 0) {
+        double tled = 1.0 / delt;
+        double tp = g.top[p], bt = g.bot[p];
+        double tthk = tp - bt;
+        double h = hcell;
+        double snnew = sqsat(tp, bt, h, 1.0e-6);
+        double rho1 = g.ss[p] * (tp - bt) * g.area[p] * tled;
+        double rho2 = g.sy[p] * g.area[p] * tled;
+        double derv = sqsat_deriv(tp, bt, h, 1.0e-6);
+        double drterm = -rho1 * derv * (h - bt) + rho1 * tthk * snnew * derv;
+        diag = diag + drterm;
+        b = b + drterm * h;
+        if (snnew < 1.0) {
+          if (snnew > 0.0) {
+            double rterm = -rho2 * tthk * snnew;
+            drterm = -rho2 * tthk * derv;
+            diag = diag + (drterm + rho2);
+            b = b - rterm + drterm * h + rho2 * bt;
+          }
+        }
+      }
+    }
+    A[p] = diag;
+#pragma unroll
+    for (int s = 0; s < 6; s++) A[(s + 1) * N + p] = off[s];
+    rhs[p] = b;
+  }
+}
+
+// sln_ls row fixes (NumericalSolution.f90:2434-2478) for one model
+__global__ void __launch_bounds__(MF6_BLOCK) k_row_fixes(Layout L, const int *__restrict__ active,
+                                                         int isymmetric, double *__restrict__ A,
+                                                         double *__restrict__ rhs,
+                                                         const double *__restrict__ x,
+                                                         double *__restrict__ xtemp) {
+  const int N = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    double xn = x[p];
+    xtemp[p] = xn;
+    double b = rhs[p];
+    if (active[p] > 0) {
+      double adiag = fabs(A[p]);
+      if (adiag < 1.0e-15) {
+        A[p] = -1.0;
+        b = b + (-1.0) * xn;
+      }
+      if (isymmetric) {
+        int pres[6] = {k > 0, i > 0, j > 0, j < L.ncol - 1, i < L.nrow - 1, k < L.nlay - 1};
+        const int ob = (1 - c) * L.H + k * L.HP;
+        int npos[6] = {p - L.HP, ob + hn(l, -L.ncol), ob + hn(l, -1), ob + hn(l, 1),
+                       ob + hn(l, L.ncol), p + L.HP};
+        for (int s = 0; s < 6; s++) {
+          if (!pres[s]) continue;
+          int m = npos[s];
+          if (active[m] < 0) {
+            b = b - (A[(s + 1) * N + p] * x[m]);
+            A[(s + 1) * N + p] = 0.0;
+          }
+        }
+      }
+    } else {
+      A[p] = 1.0;
+      for (int s = 1; s < 7; s++) A[s * N + p] = 0.0;
+      b = xn;
+    }
+    rhs[p] = b;
+  }
+}
+
+// gwf_ad: xold = x, CHD cells take their head and become ibound < 0
+__global__ void k_advance(Layout L, const double *__restrict__ x, double *__restrict__ xold,
+                          const int *__restrict__ ibound0, int *__restrict__ ibound) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    xold[p] = x[p];
+    ibound[p] = ibound0[p];
+  }
+}
+
+__global__ void k_chd_set(int nb, const int *__restrict__ nodes, const double *__restrict__ head,
+                          Layout L, double *__restrict__ x, double *__restrict__ xold,
+                          int *__restrict__ ibound) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
+    int p = pos_of_natural(L, nodes[e]);
+    if (ibound[p] != 0) {
+      x[p] = head[e];
+      xold[p] = head[e];
+      ibound[p] = -(e + 1);
+    }
+  }
+}
+
+// sln_get_dxmax partials: signed largest |xa - xb| over active cells
+__global__ void __launch_bounds__(MF6_BLOCK) k_dxmax(Layout L, const int *__restrict__ active,
+                                                     const double *__restrict__ xa,
+                                                     const double *__restrict__ xb,
+                                                     MaxLoc *__restrict__ mpart) {
+  __shared__ MaxLoc mlds[8];
+  MaxLoc m{0.0, -1};
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    if (active[p] < 1) continue;
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    double hd = xa[p] - xb[p];
+    MaxLoc cand{hd, natural(L, k, i, j)};
+    double fa = fabs(m.v), fb = fabs(hd);
+    if (fb > fa || (fb == fa && fb > 0.0 && cand.loc < m.loc)) m = cand;
+  }
+  // block reduce
+  for (int o = 32; o > 0; o >>= 1) {
+    MaxLoc b2;
+    b2.v = __shfl_xor(m.v, o);
+    b2.loc = __shfl_xor(m.loc, o);
+    m = maxloc_pick(m, b2);
+  }
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) mlds[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    MaxLoc r{0.0, -1};
+    for (int t = 0; t < (int)(blockDim.x >> 6); t++) r = maxloc_pick(r, mlds[t]);
+    mpart[blockIdx.x] = r;
+  }
+}
+
+__global__ void k_maxloc_final(const MaxLoc *mpart, int nb, MaxLoc *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  MaxLoc r{0.0, -1};
+  for (int t = 0; t < nb; t++) r = maxloc_pick(r, mpart[t]);
+  *out = r;
+}
+
+// sln_calcdx and npf_nur for one model
+__global__ void k_calcdx(Layout L, const int *__restrict__ active, const double *__restrict__ x,
+                         const double *__restrict__ xtemp, double *__restrict__ dx) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x)
+    dx[p] = active[p] < 1 ? 0.0 : x[p] - xtemp[p];
+}
+
+__global__ void k_nur(Layout L, const int *__restrict__ ibound, const int *__restrict__ icelltype,
+                      const int *__restrict__ ibotnode, const double *__restrict__ bot,
+                      double *__restrict__ x, const double *__restrict__ xtemp,
+                      double *__restrict__ dx, int *flag) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    if (ibound[p] < 1) continue;
+    if (icelltype[p] > 0) {
+      double botm = bot[ibotnode[p]];
+      if (x[p] < botm) {
+        double xx = xtemp[p] * (1.0 - 0.9) + botm * 0.9;
+        x[p] = xx;
+        dx[p] = 0.0;
+        atomicOr(flag, 1);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static GwfDev make_dev(Mf6Handle *h) {
+  GwfDev g;
+  g.L = h->L;
+  g.top = h->top; g.bot = h->bot; g.area = h->area; g.k11 = h->k11; g.k33 = h->k33;
+  g.ss = h->ss; g.sy = h->sy; g.delr = h->delr; g.delc = h->delc; g.condsat = h->condsat;
+  g.icelltype = h->icelltype; g.iconvert = h->iconvert; g.ibound = h->ibound;
+  g.inewton = h->m.inewton; g.ivarcv = h->m.ivarcv; g.idewatcv = h->m.idewatcv;
+  g.icellavg = h->m.icellavg; g.has_sto = h->m.has_sto;
+  g.satomega = h->m.inewton ? 1.0e-6 : 0.0;   // gwf-npf.f90:1394-1396
+  return g;
+}
+
+template <typename T>
+static int upload_pos(Mf6Handle *h, T **dst, const T *src) {
+  const Layout &L = h->L;
+  CK(galloc(dst, (size_t)L.NP));
+  std::vector<T> tmp(L.NP, (T)0);
+  for (int n = 0; n < L.N; n++) tmp[pos_of_natural(L, n)] = src[n];
+  HIPCHK(hipMemcpy(*dst, tmp.data(), sizeof(T) * L.NP, hipMemcpyHostToDevice));
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !m) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  if (m->nlay != L.nlay || m->nrow != L.nrow || m->ncol != L.ncol) {
+    mf6hip_set_error("model shape differs from the solver's DIS shape");
+    return MF6HIP_EINVAL;
+  }
+  if (!m->delr || !m->delc || !m->top || !m->bot || !m->k11 || !m->k33 || !m->icelltype ||
+      !m->ibound) {
+    mf6hip_set_error("model array missing");
+    return MF6HIP_EINVAL;
+  }
+  if (m->has_sto && (!m->iconvert || !m->ss || !m->sy)) {
+    mf6hip_set_error("STO arrays missing");
+    return MF6HIP_EINVAL;
+  }
+  if (m->ivarcv) {
+    mf6hip_set_error("VARIABLECV is not supported on the device path");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  h->m = *m;
+  CK(upload_pos(h, &h->top, m->top));
+  CK(upload_pos(h, &h->bot, m->bot));
+  CK(upload_pos(h, &h->k11, m->k11));
+  CK(upload_pos(h, &h->k33, m->k33));
+  CK(upload_pos(h, &h->icelltype, m->icelltype));
+  CK(upload_pos(h, &h->ibound0, m->ibound));
+  {
+    std::vector<double> area(L.N);
+    for (int n = 0; n < L.N; n++) {
+      int r = n % L.NRC, i = r / L.ncol, j = r - i * L.ncol;
+      area[n] = m->delr[j] * m->delc[i];
+    }
+    CK(upload_pos(h, &h->area, area.data()));
+  }
+  if (m->has_sto) {
+    CK(upload_pos(h, &h->iconvert, m->iconvert));
+    CK(upload_pos(h, &h->ss, m->ss));
+    CK(upload_pos(h, &h->sy, m->sy));
+  }
+  CK(galloc(&h->delr, (size_t)L.ncol));
+  CK(galloc(&h->delc, (size_t)L.nrow));
+  HIPCHK(hipMemcpy(h->delr, m->delr, sizeof(double) * L.ncol, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->delc, m->delc, sizeof(double) * L.nrow, hipMemcpyHostToDevice));
+  CK(galloc(&h->ibound, (size_t)L.NP));
+  HIPCHK(hipMemcpy(h->ibound, h->ibound0, sizeof(int) * L.NP, hipMemcpyDeviceToDevice));
+  CK(galloc(&h->condsat, 3 * (size_t)L.NP));
+  CK(galloc(&h->xold, (size_t)L.NP));
+  CK(galloc(&h->dxold, (size_t)L.NP));
+  HIPCHK(hipMemset(h->condsat, 0, sizeof(double) * 3 * L.NP));
+  HIPCHK(hipMemset(h->xold, 0, sizeof(double) * L.NP));
+  HIPCHK(hipMemset(h->dxold, 0, sizeof(double) * L.NP));
+  // ibotnode (gwf-npf.f90:1905-1939): walk down while the bottom decreases
+  {
+    std::vector<int> ib(L.N);
+    for (int n = 0; n < L.N; n++) {
+      int cur = n;
+      while (cur + L.NRC < L.N && m->bot[cur + L.NRC] < m->bot[cur]) cur += L.NRC;
+      ib[n] = pos_of_natural(L, cur);
+    }
+    CK(upload_pos(h, &h->ibotnode, ib.data()));
+  }
+  h->has_model = true;
+  GwfDev g = make_dev(h);
+  k_condsat<<<ggrid(L.N), MF6_BLOCK, 0, h->stream>>>(g, h->condsat);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
+                                      const int *nodelist, const double *v1,
+                                      const double *v2) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  if (pkgtype < MF6HIP_PKG_WEL || pkgtype > MF6HIP_PKG_CHD) { mf6hip_set_error("unknown package type"); return MF6HIP_EINVAL; }
+  if (nbound < 0 || (nbound > 0 && (!nodelist || !v1))) { mf6hip_set_error("bad package arrays"); return MF6HIP_EINVAL; }
+  if (pkgtype == MF6HIP_PKG_DRN && nbound > 0 && !v2) { mf6hip_set_error("DRN needs conductance (v2)"); return MF6HIP_EINVAL; }
+  if (nbound >= (1 << 28)) { mf6hip_set_error("too many bounds"); return MF6HIP_EUNSUPPORTED; }
+  DevPackage &pk = h->pk[pkgtype];
+  void *old[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs};
+  for (void *b : old)
+    if (b) hipFree(b);
+  pk = DevPackage();
+  pk.nbound = nbound;
+  if (nbound > 0) {
+    std::vector<int> nd(nbound);
+    for (int e = 0; e < nbound; e++) {
+      if (nodelist[e] < 1 || nodelist[e] > h->L.N) { mf6hip_set_error("package node out of range"); return MF6HIP_EINVAL; }
+      nd[e] = nodelist[e] - 1;
+    }
+    CK(galloc(&pk.nodes, (size_t)nbound));
+    CK(galloc(&pk.v1, (size_t)nbound));
+    CK(galloc(&pk.hcof, (size_t)nbound));
+    CK(galloc(&pk.brhs, (size_t)nbound));
+    HIPCHK(hipMemcpy(pk.nodes, nd.data(), sizeof(int) * nbound, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pk.v1, v1, sizeof(double) * nbound, hipMemcpyHostToDevice));
+    if (v2) {
+      CK(galloc(&pk.v2, (size_t)nbound));
+      HIPCHK(hipMemcpy(pk.v2, v2, sizeof(double) * nbound, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemset(pk.hcof, 0, sizeof(double) * nbound));
+    HIPCHK(hipMemset(pk.brhs, 0, sizeof(double) * nbound));
+  }
+  // host copy of the node lists for the per-cell entry table
+  h->bnd_dirty = true;
+  static_assert(MF6HIP_PKG_CHD == 4, "package numbering");
+  // keep nodelists on the host handle for rebuilding the table
+  h->pk_host_nodes[pkgtype].assign(nbound, 0);
+  for (int e = 0; e < nbound; e++) h->pk_host_nodes[pkgtype][e] = nodelist[e] - 1;
+  return MF6HIP_OK;
+}
+
+// per-position list of bound entries in bndlist order (WEL, DRN, RCH; CHD
+// contributes nothing to the matrix) and list order
+static int rebuild_bnd_table(Mf6Handle *h) {
+  const Layout &L = h->L;
+  std::vector<int> cnt(L.NP + 1, 0);
+  auto pos_of = [&](int n) { return pos_of_natural(L, n); };
+  const int order[3] = {MF6HIP_PKG_WEL, MF6HIP_PKG_DRN, MF6HIP_PKG_RCH};
+  for (int t : order)
+    for (int n : h->pk_host_nodes[t]) cnt[pos_of(n) + 1]++;
+  for (int p = 0; p < L.NP; p++) cnt[p + 1] += cnt[p];
+  int tot = cnt[L.NP];
+  std::vector<int> ent(tot > 0 ? tot : 1);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int t : order) {
+    const auto &nd = h->pk_host_nodes[t];
+    for (int e = 0; e < (int)nd.size(); e++) ent[fill[pos_of(nd[e])]++] = (t << 28) | e;
+  }
+  if (h->bnd_off) hipFree(h->bnd_off);
+  if (h->bnd_ent) hipFree(h->bnd_ent);
+  h->bnd_off = nullptr;
+  h->bnd_ent = nullptr;
+  CK(galloc(&h->bnd_off, (size_t)L.NP + 1));
+  CK(galloc(&h->bnd_ent, ent.size()));
+  HIPCHK(hipMemcpy(h->bnd_off, cnt.data(), sizeof(int) * (L.NP + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->bnd_ent, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice));
+  h->nbnd_ent = tot;
+  h->bnd_dirty = false;
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_set_x(void *handle, const double *x) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !x) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  std::vector<double> tmp(L.NP, 0.0);
+  for (int n = 0; n < L.N; n++) tmp[pos_of_natural(L, n)] = x[n];
+  HIPCHK(hipMemcpy(h->x, tmp.data(), sizeof(double) * L.NP, hipMemcpyHostToDevice));
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_get_x(void *handle, double *x) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !x) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  std::vector<double> tmp(L.NP);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(tmp.data(), h->x, sizeof(double) * L.NP, hipMemcpyDeviceToHost));
+  for (int n = 0; n < L.N; n++) x[n] = tmp[pos_of_natural(L, n)];
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_advance(void *handle, int iss, double delt) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  if (!iss && !(delt > 0.0)) {
+    mf6hip_set_error("DETECTED TIME STEP LENGTH OF ZERO.  GWF STORAGE PACKAGE CANNOT BE USED UNLESS DELT IS NON-ZERO.");
+    return MF6HIP_EINVAL;
+  }
+  const Layout &L = h->L;
+  h->iss = iss;
+  h->delt = delt;
+  hipStream_t s = h->stream;
+  k_advance<<<ggrid(L.NP), MF6_BLOCK, 0, s>>>(L, h->x, h->xold, h->ibound0, h->ibound);
+  DevPackage &chd = h->pk[MF6HIP_PKG_CHD];
+  if (chd.nbound > 0)
+    k_chd_set<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, chd.v1, L, h->x,
+                                                      h->xold, h->ibound);
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, double *amat_out, double *rhs_out) {
+  (void)kiter;
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  if (h->bnd_dirty) CK(rebuild_bnd_table(h));
+  for (int t = MF6HIP_PKG_WEL; t <= MF6HIP_PKG_RCH; t++) {
+    DevPackage &pk = h->pk[t];
+    if (pk.nbound > 0)
+      k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(t, pk.nbound, pk.nodes, pk.v1, pk.v2, L,
+                                                      h->ibound, h->x, h->area, pk.hcof,
+                                                      pk.brhs);
+  }
+  BndDev bd;
+  bd.off = h->bnd_off;
+  bd.ent = h->bnd_ent;
+  for (int t = 0; t < 5; t++) {
+    bd.hcof[t] = h->pk[t].hcof;
+    bd.brhs[t] = h->pk[t].brhs;
+  }
+  GwfDev g = make_dev(h);
+  k_formulate<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, bd, h->iss, h->delt, h->x, h->xold, h->A,
+                                               h->b);
+  HIPCHK(hipGetLastError());
+  if (amat_out || rhs_out) CK(mf6hip_copy_system(h, amat_out, rhs_out));
+  return MF6HIP_OK;
+}
+
+int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa, const double *xb,
+                 bool) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  int nb = ggrid(L.N);
+  k_dxmax<<<nb, MF6_BLOCK, 0, s>>>(L, h->ibound, xa, xb, h->d_mpart);
+  MaxLoc *out = (MaxLoc *)(h->d_part + 7 * MF6_MAXBLK);
+  k_maxloc_final<<<1, 64, 0, s>>>(h->d_mpart, nb, out);
+  MaxLoc r;
+  HIPCHK(hipMemcpyAsync(&r, out, sizeof(MaxLoc), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *hncg = r.v;
+  *lrch = r.loc + 1;
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, int *icnvg,
+                                       int *in_iter, double *hncg, int *lrch,
+                                       mf6hip_cnvg_out *out) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model || !icnvg || !in_iter || !hncg || !lrch) {
+    mf6hip_set_error("null argument or no model");
+    return MF6HIP_EINVAL;
+  }
+  const Layout &L = h->L;
+  k_row_fixes<<<ggrid(L.N), MF6_BLOCK, 0, h->stream>>>(L, h->ibound, h->s.ilinmeth == 1, h->A,
+                                                        h->b, h->x, h->xtemp);
+  HIPCHK(hipGetLastError());
+  CK(mf6hip_device_solve(h, kstp, kiter, icnvg, in_iter, out));
+  CK(mf6hip_dxmax(h, hncg, lrch, h->x, h->xtemp, true));
+  return MF6HIP_OK;
+}
+
+static int maxval_host(Mf6Handle *h, double *vmax) {
+  // sln_maxval: sequential normalised scan (NumericalSolution.f90:2875-2905)
+  const Layout &L = h->L;
+  std::vector<double> tmp(L.NP);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(tmp.data(), h->dxold, sizeof(double) * L.NP, hipMemcpyDeviceToHost));
+  std::vector<double> nat(L.N);
+  for (int n = 0; n < L.N; n++) nat[n] = tmp[pos_of_natural(L, n)];
+  double v = nat[0];
+  for (int n = 1; n < L.N; n++) {
+    double d = nat[n];
+    double denom = fabs(v);
+    if (denom == 0.0) denom = 2.220446049250313080847263336181640625e-16;
+    if (fabs(d) / denom > 1.0) v = d;
+  }
+  *vmax = v;
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_outer_update(void *handle, int kiter, int *inewtonur,
+                                       double *dxold_max, double *hncg, int *lrch) {
+  (void)kiter;
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model || !inewtonur || !dxold_max || !hncg || !lrch) {
+    mf6hip_set_error("null argument or no model");
+    return MF6HIP_EINVAL;
+  }
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  k_calcdx<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(L, h->ibound, h->x, h->xtemp, h->dxold);
+  *inewtonur = 0;
+  *dxold_max = 0.0;
+  if (h->m.inewton && h->m.inewtonur) {
+    int *flag = h->d_fail;
+    HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    k_nur<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(L, h->ibound, h->icelltype, h->ibotnode, h->bot, h->x,
+                                           h->xtemp, h->dxold, flag);
+    int hf = 0;
+    HIPCHK(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *inewtonur = hf;
+    if (hf) {
+      CK(maxval_host(h, dxold_max));
+      CK(mf6hip_dxmax(h, hncg, lrch, h->x, h->xtemp, true));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}

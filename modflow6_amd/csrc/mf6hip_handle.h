@@ -1,0 +1,116 @@
+// mf6hip_handle.h -- the per-solution handle behind the C ABI.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/mf6hip.h"
+#include "mf6hip_common.h"
+
+// Status codes (0 = success, as the reference's BMI functions).
+#define MF6HIP_OK 0
+#define MF6HIP_EINVAL 1
+#define MF6HIP_EHIP 2
+#define MF6HIP_EUNSUPPORTED 3
+
+void mf6hip_set_error(const std::string &msg);
+
+#define HIPCHK(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      mf6hip_set_error(std::string("HIP error '") + hipGetErrorString(e_) + \
+                       "' at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+      return MF6HIP_EHIP;                                                   \
+    }                                                                       \
+  } while (0)
+
+// Package data on the device (one per MF6HIP_PKG type).
+struct DevPackage {
+  int nbound = 0;
+  int *nodes = nullptr;    // natural, 0-based
+  double *v1 = nullptr;
+  double *v2 = nullptr;
+  double *hcof = nullptr;  // bnd_cf results
+  double *brhs = nullptr;
+};
+
+struct Mf6Handle {
+  Layout L{};
+  mf6hip_ims_settings s{};
+  int neq = 0, nja = 0;
+  hipStream_t stream = nullptr;
+  int nblk = 0;           // blocks of the reduction kernels
+
+  // CSR structure (0-based row starts) for host <-> plane conversion
+  int *d_ia = nullptr;
+  std::vector<int> h_ia;
+
+  // FP64 system in column red-black layout
+  double *A = nullptr;    // 7 planes
+  double *b = nullptr;    // rhs
+  double *x = nullptr;    // heads
+  double *xtemp = nullptr;
+  double *invpiv = nullptr;   // ILU0 / MILU0 inverse pivots
+  double *invpiv_tmp = nullptr;
+  int *d_fail = nullptr;      // first failing permuted row of the factor
+  // Krylov work vectors (FP64)
+  double *d = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr, *z = nullptr,
+         *zf = nullptr;
+  double *t = nullptr, *v = nullptr, *dhat = nullptr, *phat = nullptr,
+         *qhat = nullptr;
+  // mixed precision (FP32 inner solve)
+  float *A32 = nullptr, *invpiv32 = nullptr;
+  unsigned short *invpiv16 = nullptr;
+  float *r32 = nullptr, *e32 = nullptr, *p32 = nullptr, *p232 = nullptr,
+        *q32 = nullptr, *z32 = nullptr, *zf32 = nullptr;
+  // reductions / state
+  double *d_part = nullptr;   // [8][MF6_MAXBLK]
+  MaxLoc *d_mpart = nullptr;  // [4][MF6_MAXBLK]
+  KState *d_state = nullptr;
+  KState *h_state = nullptr;  // pinned mirror
+  double *d_hist = nullptr;   // [4][hist_cap]  dvmax rmax alpha omega
+  int *d_hloc = nullptr;      // [2][hist_cap]
+  int hist_cap = 0;
+  // scratch for host transfers in natural order
+  double *d_tmp_nat = nullptr;
+  double *d_amat_csr = nullptr;
+
+  // GWF model (device-resident formulate)
+  bool has_model = false;
+  mf6hip_gwf_model m{};
+  double *top = nullptr, *bot = nullptr, *area = nullptr, *k11 = nullptr,
+         *k33 = nullptr, *ss = nullptr, *sy = nullptr;
+  double *delr = nullptr, *delc = nullptr;
+  int *icelltype = nullptr, *iconvert = nullptr;
+  int *ibound = nullptr;      // positions; CHD cells negative
+  int *ibound0 = nullptr;     // as uploaded (before CHD)
+  double *condsat = nullptr;  // 3 planes: right, front, down (per position)
+  double *xold = nullptr;
+  double *dxold = nullptr;
+  int *ibotnode = nullptr;    // position of the lowest connected cell
+  int *bnd_off = nullptr;     // per position: start into bnd_ent (N+1)
+  int *bnd_ent = nullptr;     // entries: (pkg << 28) | index
+  int nbnd_ent = 0;
+  bool bnd_dirty = true;
+  DevPackage pk[5];
+  std::vector<int> pk_host_nodes[5];
+  int iss = 1;
+  double delt = 1.0;
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> spmv_ev;
+  int spmv_sample = 16;
+  double last_solve_ms = 0, last_spmv_ms = 0;
+  int last_spmv_n = 0;
+  bool timing = false;
+  int spmv_calls = 0;
+};
+
+// solver entry points used by the GWF part
+int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg,
+                        int *in_iter, mf6hip_cnvg_out *out);
+int mf6hip_launch_row_fixes(Mf6Handle *h);
+int mf6hip_copy_system(Mf6Handle *h, double *amat_out, double *rhs_out);
+int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa,
+                 const double *xb, bool active_only);

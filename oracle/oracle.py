@@ -1,0 +1,614 @@
+"""ctypes driver of the C oracle (oracle/mf6_oracle.c) -- test infrastructure.
+
+Holds an independent restatement of the pieces of a MODFLOW 6 GWF simulation
+around the hot path: DIS connectivity (Connections.f90:463-700, fillisym /
+filljas :1205-1270), the boundary packages, gwf_ad, the NumericalSolution
+outer loop (NumericalSolution.f90:1287-1327, :1482-1837) and imslinear_ap
+(ImsLinear.f90:617-750).  All arithmetic on the hot path runs in the C
+library; Python only sequences the calls the way the reference does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class IterRec(C.Structure):
+    _fields_ = [("dvmax", C.c_double), ("rmax", C.c_double),
+                ("l2norm", C.c_double), ("alpha", C.c_double),
+                ("omega", C.c_double), ("locdv", C.c_int), ("locr", C.c_int)]
+
+
+def lib():
+    """Load oracle/liboracle.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run make -C oracle")
+        L = C.CDLL(path)
+        d, i = C.c_double, C.c_int
+        L.orc_ddot.restype = d
+        L.orc_ddot.argtypes = [i, f64p, f64p]
+        L.orc_dnrm2.restype = d
+        L.orc_dnrm2.argtypes = [i, f64p]
+        L.orc_amux.argtypes = [i, f64p, f64p, f64p, i32p, i32p]
+        L.orc_is_close.restype = i
+        L.orc_is_close.argtypes = [d, d]
+        L.orc_epfact.restype = d
+        L.orc_epfact.argtypes = [i, i]
+        L.orc_testcnvg.argtypes = [i, C.POINTER(C.c_int), i, d, d, d, d, d, d]
+        L.orc_pccrs.argtypes = [i, i, i32p, i32p, i32p, i32p]
+        L.orc_pcu.restype = i
+        L.orc_pcu.argtypes = [i, i, i, d, f64p, i32p, i32p, f64p, i32p, i32p,
+                              i32p, f64p]
+        L.orc_ilu0a.argtypes = [i, i, f64p, i32p, i32p, f64p, f64p]
+        L.orc_residual.argtypes = [i, f64p, f64p, f64p, f64p, i32p, i32p]
+        L.orc_ims_ap.restype = i
+        L.orc_ims_ap.argtypes = [i, i, i32p, i32p, f64p, f64p, f64p, i, i, i,
+                                 i, i, d, d, d, i, f64p, i32p, i32p, i32p,
+                                 f64p, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                 C.c_void_p]
+        for nm, nargs in (("orc_sqsat", 4), ("orc_sqsat_deriv", 4)):
+            getattr(L, nm).restype = d
+            getattr(L, nm).argtypes = [d] * nargs
+        L.orc_condmean.restype = d
+        L.orc_condmean.argtypes = [d] * 7 + [i]
+        L.orc_hcond.restype = d
+        L.orc_hcond.argtypes = [i] * 7 + [d] * 14
+        L.orc_vcond.restype = d
+        L.orc_vcond.argtypes = [i] * 7 + [d] * 12
+        L.orc_sscapacity.restype = d
+        L.orc_sscapacity.argtypes = [i, d, d, d, d]
+        L.orc_sycapacity.restype = d
+        L.orc_sycapacity.argtypes = [d, d]
+        L.orc_ssterms.argtypes = [i, i, i, d, d, d, d, d, d, d, d,
+                                  C.POINTER(d), C.POINTER(d)]
+        L.orc_syterms.argtypes = [d] * 6 + [C.POINTER(d), C.POINTER(d)]
+        L.orc_calc_condsat.argtypes = [i, i32p, i32p, i32p, i32p, f64p, f64p,
+                                       f64p, f64p, f64p, f64p, f64p, i, f64p]
+        L.orc_npf_cf.argtypes = [i, i, d, i32p, i32p, f64p, f64p, f64p, f64p]
+        L.orc_npf_fc.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
+                                 f64p, f64p, i32p, i32p, i, i, i, i, f64p,
+                                 f64p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_npf_fn.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, i32p,
+                                 i32p, i, f64p, f64p, d, f64p, f64p, f64p]
+        L.orc_sto_fc.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
+                                 f64p, d, f64p, f64p, f64p, f64p]
+        L.orc_sto_fn.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
+                                 f64p, d, f64p, f64p, f64p]
+        L.orc_bnd_fc.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
+        L.orc_wel_cf.argtypes = [i, i32p, f64p, i32p, f64p, f64p]
+        L.orc_rch_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
+        L.orc_drn_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p, f64p]
+        L.orc_sln_ls_fixes.argtypes = [i, i32p, i32p, i32p, i, f64p, f64p,
+                                       f64p, f64p]
+        L.orc_get_dxmax.restype = i
+        L.orc_get_dxmax.argtypes = [i, i32p, f64p, f64p, C.POINTER(d)]
+        L.orc_calcdx.argtypes = [i, i32p, f64p, f64p, f64p]
+        L.orc_maxval.restype = d
+        L.orc_maxval.argtypes = [i, f64p]
+        L.orc_npf_nur.restype = i
+        L.orc_npf_nur.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
+                                  C.POINTER(d), C.POINTER(C.c_int)]
+        L.orc_sln_l2norm.restype = d
+        L.orc_sln_l2norm.argtypes = [i, i32p, i32p, f64p, f64p, f64p, i32p,
+                                     f64p]
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """Load oracle/_ref/libmf6ref.so (the reference's own Fortran), or None
+    when it has not been built (it needs /root/reference)."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libmf6ref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        d, i = C.c_double, C.c_int
+        R.ref_pccrs.argtypes = [i, i, i32p, i32p, i32p, i32p]
+        R.ref_pcu.argtypes = [i, i, i, d, f64p, i32p, i32p, f64p, i32p, i32p]
+        R.ref_ilu0a.argtypes = [i, i, f64p, i32p, i32p, f64p, f64p]
+        R.ref_residual.argtypes = [i, i, f64p, f64p, f64p, f64p, i32p, i32p]
+        R.ref_amux.argtypes = [i, f64p, f64p, f64p, i32p, i32p]
+        R.ref_ddot.restype = d
+        R.ref_ddot.argtypes = [i, f64p, f64p]
+        R.ref_dnrm2.restype = d
+        R.ref_dnrm2.argtypes = [i, f64p]
+        R.ref_ims_ap.argtypes = [i, i, i32p, i32p, f64p, f64p, f64p, i, i, i,
+                                 i, i, d, d, d, i, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(d), f64p, f64p,
+                                 i32p, i32p]
+        R.ref_testcnvg.argtypes = [i, C.POINTER(C.c_int), i, d, d, d, d, d, d]
+        R.ref_epfact.restype = d
+        R.ref_epfact.argtypes = [i, i]
+        R.ref_is_close.restype = i
+        R.ref_is_close.argtypes = [d, d]
+        R.ref_hcond.restype = d
+        R.ref_hcond.argtypes = [i] * 7 + [d] * 14
+        R.ref_vcond.restype = d
+        R.ref_vcond.argtypes = [i] * 7 + [d] * 12
+        R.ref_condmean.restype = d
+        R.ref_condmean.argtypes = [d] * 7 + [i]
+        R.ref_sqsat.restype = d
+        R.ref_sqsat.argtypes = [d] * 4
+        R.ref_sqsat_deriv.restype = d
+        R.ref_sqsat_deriv.argtypes = [d] * 4
+        R.ref_ssterms.argtypes = [i, i, i, d, d, d, d, d, d, d, d,
+                                  C.POINTER(d), C.POINTER(d)]
+        R.ref_syterms.argtypes = [d] * 6 + [C.POINTER(d), C.POINTER(d)]
+        R.ref_sscapacity.restype = d
+        R.ref_sscapacity.argtypes = [i, d, d, d, d]
+        R.ref_sycapacity.restype = d
+        R.ref_sycapacity.argtypes = [d, d]
+        _REF = R
+    return _REF
+
+
+# --------------------------------------------------------------------------
+# DIS connectivity
+# --------------------------------------------------------------------------
+
+@dataclass
+class DisConnections:
+    """Connections%disconnections for a full DIS grid (every cell present).
+
+    Rows: diagonal, then up (k-1), back (i-1), left (j-1), right (j+1),
+    front (i+1), down (k+1) -- ascending columns (Connections.f90:523-603).
+    Symmetric arrays (ihc, cl1, cl2, hwva) are indexed by jas, numbered in
+    the order right, front, down per cell (:620-686).  All 0-based.
+    """
+    nlay: int
+    nrow: int
+    ncol: int
+    ia: np.ndarray
+    ja: np.ndarray
+    jas: np.ndarray
+    isym: np.ndarray
+    ihc: np.ndarray
+    cl1: np.ndarray
+    cl2: np.ndarray
+    hwva: np.ndarray
+    top: np.ndarray
+    bot: np.ndarray
+    area: np.ndarray
+
+    @property
+    def nodes(self):
+        return self.nlay * self.nrow * self.ncol
+
+    @property
+    def nja(self):
+        return int(self.ia[-1])
+
+
+def dis_connections(nlay, nrow, ncol, delr, delc, top2d, botm):
+    """Build DisConnections.  top2d: (nrow, ncol) TOP; botm: (nlay, nrow,
+    ncol) BOTM.  Cell top of layer k > 0 is botm[k-1] (Dis.f90)."""
+    delr = np.asarray(delr, dtype=np.float64)
+    delc = np.asarray(delc, dtype=np.float64)
+    nodes = nlay * nrow * ncol
+    nrc = nrow * ncol
+    n = np.arange(nodes, dtype=np.int64)
+    k = n // nrc
+    i = (n // ncol) % nrow
+    j = n % ncol
+    # neighbour offsets in ascending column order
+    offs = [(-nrc, k > 0), (-ncol, i > 0), (-1, j > 0), (1, j < ncol - 1),
+            (ncol, i < nrow - 1), (nrc, k < nlay - 1)]
+    cnt = 1 + sum(m.astype(np.int64) for _, m in offs)
+    ia = np.zeros(nodes + 1, dtype=np.int64)
+    ia[1:] = np.cumsum(cnt)
+    nja = int(ia[-1])
+    ja = np.empty(nja, dtype=np.int64)
+    pos = ia[:-1].copy()
+    ja[pos] = n
+    slot_pos = {}
+    pos = pos + 1
+    for s, (off, m) in enumerate(offs):
+        idx = np.nonzero(m)[0]
+        ja[pos[idx]] = n[idx] + off
+        slot_pos[s] = (idx, pos[idx].copy())
+        pos[idx] += 1
+    # jas: upper connections numbered per node in ascending order
+    # (filljas, Connections.f90:1235-1262)
+    jas = np.zeros(nja, dtype=np.int64)
+    upper_mask = ja > np.repeat(n, cnt)
+    ups = np.nonzero(upper_mask)[0]
+    jas[ups] = np.arange(ups.size)
+    # isym (fillisym :1205-1232): position of the transpose entry
+    isym = np.empty(nja, dtype=np.int64)
+    isym[ia[:-1]] = ia[:-1]
+    pair = {0: 5, 1: 4, 2: 3, 3: 2, 4: 1, 5: 0}
+    for s in range(6):
+        idx, p = slot_pos[s]
+        tidx, tp = slot_pos[pair[s]]
+        # transpose of (n, n+off) is (n+off, n) found at slot pair[s] of row m
+        lookup = np.full(nodes, -1, dtype=np.int64)
+        lookup[tidx] = tp
+        isym[p] = lookup[n[idx] + offs[s][0]]
+    lower = np.nonzero((~upper_mask) & (ja != np.repeat(n, cnt)))[0]
+    jas[lower] = jas[isym[lower]]
+    # geometry
+    top = np.empty(nodes)
+    bot = np.asarray(botm, dtype=np.float64).reshape(nodes)
+    top[:nrc] = np.asarray(top2d, dtype=np.float64).reshape(nrc)
+    top[nrc:] = bot[:-nrc]
+    area = (delr[j] * delc[i]).astype(np.float64)
+    njas = ups.size
+    ihc = np.zeros(njas, dtype=np.int64)
+    cl1 = np.zeros(njas)
+    cl2 = np.zeros(njas)
+    hwva = np.zeros(njas)
+    # right
+    idx, p = slot_pos[3]
+    js = jas[p]
+    ihc[js] = 1
+    cl1[js] = 0.5 * delr[j[idx]]
+    cl2[js] = 0.5 * delr[j[idx] + 1]
+    hwva[js] = delc[i[idx]]
+    # front
+    idx, p = slot_pos[4]
+    js = jas[p]
+    ihc[js] = 1
+    cl1[js] = 0.5 * delc[i[idx]]
+    cl2[js] = 0.5 * delc[i[idx] + 1]
+    hwva[js] = delr[j[idx]]
+    # down
+    idx, p = slot_pos[5]
+    js = jas[p]
+    m = idx + nrc
+    ihc[js] = 0
+    cl1[js] = 0.5 * (top[idx] - bot[idx])
+    cl2[js] = 0.5 * (top[m] - bot[m])
+    hwva[js] = delr[j[idx]] * delc[i[idx]]
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    return DisConnections(nlay, nrow, ncol, i32(ia), i32(ja), i32(jas),
+                          i32(isym), i32(ihc), cl1, cl2, hwva, top, bot, area)
+
+
+# --------------------------------------------------------------------------
+# IMS linear solve
+# --------------------------------------------------------------------------
+
+@dataclass
+class ImsSettings:
+    dvclose: float = 1e-6
+    rclose: float = 1e-6
+    icnvgopt: int = 0
+    iter1: int = 5000
+    ilinmeth: int = 1
+    north: int = 0
+    relax: float = 0.0
+
+    @property
+    def ipc(self):
+        return 2 if self.relax > 0 else 1
+
+
+class ImsLinear:
+    """imslinear_ar + imslinear_ap on a fixed CSR structure (0-based)."""
+
+    def __init__(self, ia, ja, settings: ImsSettings):
+        self.ia = np.ascontiguousarray(ia, dtype=np.int32)
+        self.ja = np.ascontiguousarray(ja, dtype=np.int32)
+        self.neq = self.ia.size - 1
+        self.nja = int(self.ia[-1])
+        self.s = settings
+        self.iapc = np.zeros(self.neq + 1, dtype=np.int32)
+        self.japc = np.zeros(self.nja, dtype=np.int32)
+        lib().orc_pccrs(self.neq, self.nja, self.ia, self.ja, self.iapc,
+                        self.japc)
+        self.apc = np.zeros(self.nja)   # ImsLinear.f90:317 APC = 0
+        self.iwork = np.zeros(self.neq, dtype=np.int32)
+        self.work = np.zeros(9 * self.neq)
+
+    def apply(self, amat, rhs, x, kstp=1, history=False):
+        """Returns (icnvg, innerit, l2norm0, ilu_stop, hist_list)."""
+        s = self.s
+        icnvg = C.c_int(0)
+        l2 = C.c_double(0.0)
+        stop = C.c_int(-1)
+        hist = (IterRec * max(1, s.iter1))() if history else None
+        innerit = lib().orc_ims_ap(
+            self.neq, self.nja, self.ia, self.ja, amat, rhs, x, s.ilinmeth,
+            s.ipc, s.icnvgopt, s.north, s.iter1, s.dvclose, s.rclose, s.relax,
+            kstp, self.apc, self.iapc, self.japc, self.iwork, self.work,
+            C.byref(icnvg), C.byref(l2), C.byref(stop),
+            C.cast(hist, C.c_void_p) if hist is not None else None)
+        recs = None
+        if history:
+            recs = [(hist[t].dvmax, hist[t].rmax, hist[t].alpha, hist[t].omega,
+                     hist[t].locdv, hist[t].locr, hist[t].l2norm)
+                    for t in range(innerit)]
+        return icnvg.value, innerit, l2.value, stop.value, recs
+
+
+# --------------------------------------------------------------------------
+# Red-black permutation used by the device solver (restated here so that the
+# device factor can be checked against pcilu0 on the permuted matrix)
+# --------------------------------------------------------------------------
+
+def rb_permutation(nlay, nrow, ncol):
+    """new index of natural cell n: colour*(N/2) + (n >> 1), colour =
+    (k+i+j) mod 2; requires ncol even."""
+    if ncol % 2:
+        raise ValueError("red-black split layout needs an even ncol")
+    N = nlay * nrow * ncol
+    n = np.arange(N, dtype=np.int64)
+    k = n // (nrow * ncol)
+    i = (n // ncol) % nrow
+    j = n % ncol
+    c = (k + i + j) & 1
+    return (c * (N // 2) + (n >> 1)).astype(np.int64)
+
+
+def column_rb_permutation(nlay, nrow, ncol):
+    """Ordering of the device ILU0: vertical cell columns coloured by
+    (i + j) mod 2, colour 0 first; within a colour, columns in natural
+    (i, j) order; within a column, layers top to bottom.  Returns
+    new_index[natural]."""
+    nrc = nrow * ncol
+    N = nlay * nrc
+    n = np.arange(N, dtype=np.int64)
+    k = n // nrc
+    l = n % nrc
+    i = l // ncol
+    j = l % ncol
+    c = (i + j) & 1
+    ncol0 = (nrc + 1) // 2          # columns of colour 0
+    # index of the column within its colour, in natural (i, j) order
+    colidx = np.empty(nrc, dtype=np.int64)
+    lc = np.arange(nrc)
+    cc = ((lc // ncol) + (lc % ncol)) & 1
+    for col in (0, 1):
+        sel = np.nonzero(cc == col)[0]
+        colidx[sel] = np.arange(sel.size)
+    return (c * ncol0 * nlay + colidx[l] * nlay + k).astype(np.int64)
+
+
+def permute_csr(ia, ja, a, perm):
+    """CSR of P A P^T (new = perm[old]) with the reference row layout:
+    diagonal first, remaining columns ascending."""
+    N = ia.size - 1
+    inv = np.empty(N, dtype=np.int64)
+    inv[perm] = np.arange(N)
+    cnt = np.diff(ia)
+    nia = np.zeros(N + 1, dtype=np.int64)
+    nia[1:] = np.cumsum(cnt[inv])
+    nja = np.empty_like(ja)
+    na = np.empty_like(a) if a is not None else None
+    src_map = np.empty(ja.size, dtype=np.int64)
+    for newr in range(N):
+        old = inv[newr]
+        s, e = ia[old], ia[old + 1]
+        cols = perm[ja[s:e]]
+        order = np.argsort(cols, kind="stable")
+        cols = cols[order]
+        d = np.nonzero(cols == newr)[0][0]
+        o = np.concatenate(([order[d]], np.delete(order, d)))
+        ns = nia[newr]
+        nja[ns:ns + (e - s)] = perm[ja[s:e]][o]
+        src_map[ns:ns + (e - s)] = s + o
+    if a is not None:
+        na = a[src_map]
+    return (nia.astype(np.int32), nja.astype(np.int32), na, src_map)
+
+
+# --------------------------------------------------------------------------
+# GWF simulation (one model, one IMS solution)
+# --------------------------------------------------------------------------
+
+_PKG_ORDER = {"WEL": 0, "DRN": 1, "RCH": 2, "CHD": 3}   # gwf.f90:119-124
+
+
+class GwfSimulation:
+    """Restated MF6 run of a :class:`modflow6_amd.configs.GwfCase`.
+
+    Mirrors Mf6DoTimestep -> sln_ca (NumericalSolution.f90:1287-1327):
+    gwf_ad, then outer iterations of solve() until |hncg| <= outer_dvclose.
+    """
+
+    def __init__(self, case):
+        self.case = case
+        c = case
+        top, bot = c.cell_top_bot()
+        self.dis = dis_connections(c.nlay, c.nrow, c.ncol, c.delr, c.delc,
+                                   c.top2d, c.botm)
+        d = self.dis
+        self.neq = d.nodes
+        self.k11 = np.ascontiguousarray(c.k11, dtype=np.float64)
+        self.k33 = np.ascontiguousarray(c.k33, dtype=np.float64)
+        self.icelltype = np.ascontiguousarray(c.icelltype, dtype=np.int32)
+        self.ibound = np.ones(self.neq, dtype=np.int32)
+        self.inewton = 1 if c.newton else 0
+        self.satomega = 1e-6 if c.newton else 0.0        # gwf-npf.f90:1394
+        self.condsat = np.zeros(d.hwva.size)
+        lib().orc_calc_condsat(self.neq, d.ia, d.ja, d.jas, d.ihc, d.cl1,
+                               d.cl2, d.hwva, d.top, d.bot, self.k11,
+                               self.k33, 0, self.condsat)
+        self.sat = np.ones(self.neq)                      # preprocess :1850
+        self.x = np.ascontiguousarray(c.strt, dtype=np.float64).copy()
+        self.xold = self.x.copy()
+        self.xtemp = self.x.copy()
+        self.dxold = np.zeros(self.neq)
+        self.amat = np.zeros(d.nja)
+        self.rhs = np.zeros(self.neq)
+        self.pk = sorted(c.packages, key=lambda p: _PKG_ORDER[p.kind])
+        self.pk_hcof = [np.zeros(p.nodes.size) for p in self.pk]
+        self.pk_rhs = [np.zeros(p.nodes.size) for p in self.pk]
+        self.pk_nodes = [np.ascontiguousarray(p.nodes, dtype=np.int32)
+                         for p in self.pk]
+        ims = c.ims
+        self.settings = ImsSettings(ims.inner_dvclose, ims.inner_rclose,
+                                    ims.icnvgopt, ims.inner_maximum,
+                                    ims.ilinmeth,
+                                    ims.number_orthogonalizations,
+                                    ims.relaxation_factor)
+        self.imslinear = ImsLinear(d.ia, d.ja, self.settings)
+        self.isymmetric = 1 if ims.ilinmeth == 1 else 0
+        self.iconvert = (np.ascontiguousarray(c.iconvert, dtype=np.int32)
+                         if c.has_sto else None)
+        self.ss = c.ss
+        self.sy = c.sy
+        self.ibotnode = self._ibotnode() if c.newton_ur else None
+        self.delt = 1.0
+        self.iss = 1
+        self.kstp = 0
+        self.log = []
+
+    def _ibotnode(self):
+        # gwf-npf.f90:1905-1939: lowest vertically connected node
+        d = self.dis
+        nrc = d.nrow * d.ncol
+        n = np.arange(self.neq)
+        k = n // nrc
+        col = n % nrc
+        out = n.copy()
+        for kk in range(d.nlay - 1, -1, -1):
+            pass
+        # walk downward while the next layer's bottom is lower
+        out = n.copy()
+        cur = n.copy()
+        while True:
+            nxt = cur + nrc
+            ok = nxt < self.neq
+            ok[ok] = d.bot[nxt[ok]] < d.bot[cur[ok]]
+            if not ok.any():
+                break
+            cur = np.where(ok, nxt, cur)
+        return np.ascontiguousarray(cur, dtype=np.int32)
+
+    # -- gwf_ad (gwf.f90:396-421) and chd_ad (gwf-chd.f90:175-199)
+    def advance(self, delt, steady):
+        self.delt = delt
+        self.iss = 1 if steady else 0
+        self.xold[:] = self.x
+        for p in self.pk:
+            if p.kind == "CHD":
+                nd = np.asarray(p.nodes)
+                self.x[nd] = p.v1
+                self.xold[nd] = p.v1
+                self.ibound[nd] = -1
+
+    # -- sln_buildsystem (NumericalSolution.f90:1941-1991)
+    def formulate(self, kiter):
+        L = lib()
+        d = self.dis
+        self.amat[:] = 0.0
+        self.rhs[:] = 0.0
+        # gwf_cf: npf_cf, then bnd_cf per package
+        L.orc_npf_cf(self.neq, self.inewton, self.satomega, self.icelltype,
+                     self.ibound, d.top, d.bot, self.x, self.sat)
+        for t, p in enumerate(self.pk):
+            nd = self.pk_nodes[t]
+            if p.kind == "WEL":
+                L.orc_wel_cf(nd.size, nd, np.ascontiguousarray(p.v1),
+                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
+            elif p.kind == "RCH":
+                L.orc_rch_cf(nd.size, nd, np.ascontiguousarray(p.v1), d.area,
+                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
+            elif p.kind == "DRN":
+                L.orc_drn_cf(nd.size, nd, np.ascontiguousarray(p.v1),
+                             np.ascontiguousarray(p.v2), self.ibound, self.x,
+                             self.pk_hcof[t], self.pk_rhs[t])
+            elif p.kind == "CHD":
+                self.pk_hcof[t][:] = 0.0
+                self.pk_rhs[t][:] = 0.0
+        # gwf_fc
+        L.orc_npf_fc(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
+                     d.hwva, self.condsat, self.ibound, self.icelltype,
+                     self.inewton, 0, 1, 0, self.k11, self.k33, d.top, d.bot,
+                     self.sat, self.x, self.amat)
+        if self.case.has_sto and not self.iss:
+            L.orc_sto_fc(self.neq, d.ia, self.ibound, self.iconvert, d.top,
+                         d.bot, d.area, self.ss, self.sy, self.delt,
+                         self.xold, self.x, self.amat, self.rhs)
+        for t, p in enumerate(self.pk):
+            if p.kind == "CHD":
+                continue      # chd_fc is a no-op (gwf-chd.f90:238-250)
+            nd = self.pk_nodes[t]
+            L.orc_bnd_fc(nd.size, nd, self.pk_hcof[t], self.pk_rhs[t], d.ia,
+                         self.amat, self.rhs)
+        if self.inewton:
+            L.orc_npf_fn(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc,
+                         self.condsat, self.ibound, self.icelltype, 0, d.top,
+                         d.bot, self.satomega, self.x, self.amat, self.rhs)
+            if self.case.has_sto and not self.iss:
+                L.orc_sto_fn(self.neq, d.ia, self.ibound, self.iconvert,
+                             d.top, d.bot, d.area, self.ss, self.sy,
+                             self.delt, self.x, self.amat, self.rhs)
+
+    # -- NumericalSolution%solve (:1482-1837), no backtracking / PTC
+    def solve_outer(self, kiter, history=False):
+        L = lib()
+        d = self.dis
+        self.formulate(kiter)
+        L.orc_sln_ls_fixes(self.neq, d.ia, d.ja, self.ibound,
+                           self.isymmetric, self.amat, self.rhs, self.x,
+                           self.xtemp)
+        icnvg, inner, l2n0, stop, recs = self.imslinear.apply(
+            self.amat, self.rhs, self.x, kstp=self.kstp, history=history)
+        hncg = C.c_double(0.0)
+        lrch = L.orc_get_dxmax(self.neq, self.ibound, self.x, self.xtemp,
+                               C.byref(hncg))
+        conv = abs(hncg.value) <= self.case.ims.outer_dvclose
+        rec = dict(kiter=kiter, inner=inner, icnvg_inner=icnvg,
+                   hncg=hncg.value, lrch=lrch, l2norm0=l2n0, hist=recs)
+        if not conv:
+            L.orc_calcdx(self.neq, self.ibound, self.x, self.xtemp, self.dxold)
+            if self.case.newton_ur:
+                dxmax = C.c_double(0.0)
+                locmax = C.c_int(0)
+                inur = L.orc_npf_nur(self.neq, self.ibound, self.icelltype,
+                                     self.ibotnode, d.bot, self.x, self.xtemp,
+                                     self.dxold, C.byref(dxmax),
+                                     C.byref(locmax))
+                if inur:
+                    dxo = L.orc_maxval(self.neq, self.dxold)
+                    rec["nur"] = True
+                    if (abs(dxo) <= self.case.ims.outer_dvclose and
+                            abs(hncg.value) <= self.case.ims.outer_dvclose):
+                        conv = True
+                        lrch = L.orc_get_dxmax(self.neq, self.ibound, self.x,
+                                               self.xtemp, C.byref(hncg))
+                        rec["hncg"] = hncg.value
+        rec["converged"] = conv
+        return conv, rec
+
+    def run_timestep(self, delt, steady, history=False):
+        self.kstp += 1
+        self.advance(delt, steady)
+        recs = []
+        conv = False
+        for kiter in range(1, self.case.ims.outer_maximum + 1):
+            conv, rec = self.solve_outer(kiter, history=history)
+            recs.append(rec)
+            if conv:
+                break
+        self.log.append(recs)
+        return conv, recs
+
+    def run(self, history=False, max_steps=None):
+        nsteps = 0
+        for perlen, nstp, steady in self.case.periods:
+            self.kstp = 0
+            for _ in range(nstp):
+                self.run_timestep(perlen / nstp, steady, history=history)
+                nsteps += 1
+                if max_steps is not None and nsteps >= max_steps:
+                    return self.x
+        return self.x
