@@ -33,10 +33,11 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_precision(case, precision, steps, warmup, label):
+def run_precision(case, precision, steps, warmup, label, replace=0):
     from modflow6_amd import _lib
     from modflow6_amd.solution import GwfSolution
-    gs = GwfSolution(case, precision=precision, timing=True)
+    gs = GwfSolution(case, precision=precision, timing=True,
+                     inner_fp32_max=replace)
     x0 = np.ascontiguousarray(case.strt, dtype=np.float64)
     out = []
     for s in range(warmup + steps):
@@ -125,6 +126,8 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--shape", default="10,1000,1000")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replace", type=int, default=0,
+                    help="mixed: FP32 iterations between residual replacements (0 = 64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
@@ -143,7 +146,8 @@ def main():
     if args.precision in ("both", "fp64"):
         res["fp64"] = run_precision(case, _lib.MF6HIP_FP64, args.steps, args.warmup, "fp64")
     if args.precision in ("both", "mixed"):
-        res["mixed"] = run_precision(case, _lib.MF6HIP_MIXED, args.steps, args.warmup, "mixed")
+        res["mixed"] = run_precision(case, _lib.MF6HIP_MIXED, args.steps, args.warmup, "mixed",
+                                     replace=args.replace)
 
     def mean(lst, key):
         return float(np.mean([r[key] for r in lst]))
@@ -159,14 +163,23 @@ def main():
     if "fp64" in res and "mixed" in res:
         dh = float(np.max(np.abs(res["mixed"][1] - res["fp64"][1])))
         summary["mixed"]["max_head_diff_vs_fp64_m"] = dh
+    # heads against the CPU oracle (FP64 natural-order ILU0-PCG, the
+    # reference algorithm) at 4096 seeded cells of the same solve
+    gold = os.path.join(ROOT, "tests", "golden", "steady_10m_heads_sample.npz")
+    if os.path.exists(gold) and case.nodes == 10_000_000:
+        g = np.load(gold)
+        for k, (recs, heads, amat, rhs) in res.items():
+            summary[k]["max_head_diff_vs_oracle_m"] = float(
+                np.max(np.abs(heads[g["cells"]] - g["heads"])))
     head = "mixed" if "mixed" in summary else "fp64"
     value = summary[head]["s_per_solve"]
 
-    # roofline of the dominant kernel: the CG SpMV with the fused direction
-    # update (q = A p, p = z + beta p) in the 7-plane stencil format:
-    # 8 B per stored coefficient (nnz) + z, p_in read + p_out, q written
-    fp64_spmv_bytes = 8 * nja + 32 * N
-    fp32_spmv_bytes = 4 * nja + 16 * N
+    # roofline of the dominant kernel, k_f_spmv (CG SpMV q = A p): symmetric
+    # stencil planes (diagonal + the upper coefficient of every connection,
+    # N + njas values) + p read + q written
+    njas = (nja - N) // 2
+    fp64_spmv_bytes = 8 * (N + njas) + 2 * 8 * N
+    fp32_spmv_bytes = 4 * (N + njas) + 2 * 4 * N
     if head == "mixed":
         sp_bytes, sp_us = fp32_spmv_bytes, summary["mixed"]["spmv_us"]
     else:
@@ -174,7 +187,7 @@ def main():
     achieved = sp_bytes / (sp_us * 1e-6) / 1e9 if sp_us > 0 else None
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
-                    traffic=None, kernel="k_cg_spmv", bytes_per_launch=sp_bytes,
+                    traffic=None, kernel="k_f_spmv", bytes_per_launch=sp_bytes,
                     avg_launch_us=sp_us)
     if "fp64" in summary:
         f = summary["fp64"]["spmv_us"]

@@ -58,8 +58,13 @@ typedef struct mf6hip_ims_settings {
   int precision;    /* MF6HIP_FP64 / MF6HIP_MIXED / MF6HIP_MIXED_FP16 */
   int ordering;     /* MF6HIP_ORDER_RB */
   int nlay, nrow, ncol; /* DIS shape of the (single) model */
-  int inner_fp32_max;   /* MIXED: FP32 iterations per refinement step (0 = auto) */
-  double inner_fp32_rtol; /* MIXED: relative FP32 residual reduction per step (0 = auto) */
+  int inner_fp32_max;   /* MIXED: FP32 CG iterations between replacements of the
+                           FP32 residual by the FP64 b - A x (0 = 64) */
+  double inner_fp32_rtol; /* MIXED, unsymmetric or unfused path: relative FP32
+                             residual reduction per refinement step (0 = 1e-4) */
+  int kernel_variant;   /* 0: fused thread-per-column CG when the system is
+                           symmetric (default); 1: one kernel per reference
+                           vector pass */
 } mf6hip_ims_settings;
 
 /* Per-inner-iteration record, the values ConvergenceSummaryType keeps for

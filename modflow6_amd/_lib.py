@@ -40,7 +40,8 @@ class ImsSettingsC(C.Structure):
                 ("precision", C.c_int), ("ordering", C.c_int),
                 ("nlay", C.c_int), ("nrow", C.c_int), ("ncol", C.c_int),
                 ("inner_fp32_max", C.c_int),
-                ("inner_fp32_rtol", C.c_double)]
+                ("inner_fp32_rtol", C.c_double),
+                ("kernel_variant", C.c_int)]
 
 
 class CnvgOutC(C.Structure):

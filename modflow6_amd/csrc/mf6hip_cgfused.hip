@@ -1,0 +1,749 @@
+// mf6hip_cgfused.hip -- fused preconditioned CG (ims_base_cg,
+// src/Solution/LinearMethods/ImsLinearBase.f90:30-240) for symmetric
+// systems, one thread per vertical cell column.
+//
+// One CG iteration is five streaming kernels and two one-block scalar
+// kernels instead of the reference's nine vector passes:
+//
+//   pc_red_fwd  d -= alpha q (red cells) ; forward ILU0 sweep of red columns
+//   pc_black    d -= alpha q (black)     ; forward + backward sweep of black
+//               columns ; partial rho = d.z
+//   pc_red_bwd  backward sweep of red columns ; partial rho
+//   s_round     convergence test of the finished iteration (testcnvg,
+//               is_close, rho == 0, INNER_MAXIMUM), rho, beta
+//   pupdate     p = z + beta p ; x += alpha_prev p_prev ; partial max |p|
+//   spmv        q = A p ; partial p.q
+//   s_alpha     alpha = rho / (p.q + sign(DPREC, p.q)) ; deltax = alpha pmax
+//
+// The arithmetic of every vector element is the reference's (x = x + alpha
+// p, d = d - alpha q, p = z + beta p, amux order, ilu0a order); only the
+// point in time at which x and d receive their update moves, and dot
+// products are summed in a different (fixed) order.  Symmetry lets every
+// off-diagonal be read from the upper planes (right, front, down) of the
+// owning cell, so the lower planes are never streamed.
+//
+// The same kernels run in FP32 (matrix, pivots, Krylov vectors) for the
+// mixed-precision solve: FP32 CG with reliable updates -- every R iterations
+// and whenever the FP32 test passes, the FP32 correction is added to the
+// FP64 heads and the FP32 residual is replaced by the FP64 true residual
+// b - A x, keeping the search direction (residual replacement, so the
+// Krylov space is not restarted).
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "mf6hip_handle.h"
+
+#define DPREC_F 2.220446049250313080847263336181640625e-16
+#define DSAME_F (100.0 * DPREC_F)
+
+#define CK(x)                         \
+  do {                                \
+    int rc_ = (x);                    \
+    if (rc_ != MF6HIP_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+template <typename T>
+__device__ inline T wsum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ inline MaxLoc wmax(MaxLoc a) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    MaxLoc b;
+    b.v = __shfl_xor(a.v, o);
+    b.loc = __shfl_xor(a.loc, o);
+    a = maxloc_pick(a, b);
+  }
+  return a;
+}
+
+__device__ inline double bsum(double v, double *lds) {
+  v = wsum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s = s + lds[i];
+  return s;
+}
+
+__device__ inline MaxLoc bmax(MaxLoc a, MaxLoc *lds) {
+  a = wmax(a);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = a;
+  __syncthreads();
+  MaxLoc r{0.0, -1};
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = maxloc_pick(r, lds[i]);
+  return r;
+}
+
+__device__ inline double rsum(const double *part, int nb, double *lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s = s + part[i];
+  return bsum(s, lds);
+}
+
+__device__ inline MaxLoc rmaxl(const MaxLoc *part, int nb, MaxLoc *lds) {
+  MaxLoc a{0.0, -1};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) a = maxloc_pick(a, part[i]);
+  return bmax(a, lds);
+}
+
+__device__ inline int is_close_(double a, double b) {
+  if (a == b) return 1;
+  return fabs(a - b) <= fmax(DSAME_F * fmax(fabs(a), fabs(b)), 0.0);
+}
+
+template <typename TP>
+__device__ inline double ldp(const TP *p, int e) { return (double)p[e]; }
+template <>
+__device__ inline double ldp<unsigned short>(const unsigned short *p, int e) {
+  unsigned short u = p[e];
+  __half h = *reinterpret_cast<__half *>(&u);
+  return (double)__half2float(h);
+}
+
+// fused-CG state beyond KState (kept in the same struct array: index 2)
+struct FState {
+  double apend_x;   // alpha of the x update still to apply
+  double apend_d;   // alpha of the d update still to apply
+  double pmax;      // signed max |p| of the last direction
+  int ploc;
+  int reliable_due;
+};
+
+}  // namespace
+
+// partial slots in part[] / mpart[] (each nblk long)
+enum { P_SS_R = 0, P_SS_B = 1, P_RHO_B = 2, P_RHO_R = 3, P_PQ = 4 };
+enum { M_D_R = 0, M_D_B = 1, M_P = 2 };
+
+// ---------------------------------------------------------------------------
+// precondition kernels
+// ---------------------------------------------------------------------------
+
+template <typename TM, typename TV, typename TP>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__restrict__ A,
+                                                         const TP *__restrict__ piv,
+                                                         TV *__restrict__ d,
+                                                         const TV *__restrict__ q,
+                                                         TV *__restrict__ zf, double *part,
+                                                         MaxLoc *mpart, const KState *st,
+                                                         const FState *fs, int nblk) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  const int NP = L.NP;
+  const TV alpha = (TV)fs->apend_d;
+  double ss = 0.0;
+  MaxLoc md{0.0, -1};
+  for (int h2 = blockIdx.x * blockDim.x + threadIdx.x; h2 < L.HP0;
+       h2 += gridDim.x * blockDim.x) {
+    int l = col_l(L, 0, h2);
+    TV zup = (TV)0;
+    TV lup = (TV)0;   // A(n, up) * piv(up)
+    for (int k = 0; k < L.nlay; k++) {
+      int pos = k * L.HP + h2;
+      TV dv = d[pos];
+      dv = dv - alpha * q[pos];
+      d[pos] = dv;
+      md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
+      ss = ss + (double)dv * (double)dv;
+      TV tv = dv;
+      if (k > 0) tv = tv - lup * zup;
+      zf[pos] = tv;
+      zup = tv;
+      lup = (TV)((TV)A[6 * NP + pos] * (TV)ldp(piv, pos));
+    }
+  }
+  ss = bsum(ss, lds);
+  if (threadIdx.x == 0) part[P_SS_R * nblk + blockIdx.x] = ss;
+  md = bmax(md, mlds);
+  if (threadIdx.x == 0) mpart[M_D_R * nblk + blockIdx.x] = md;
+}
+
+// Black columns: forward sweep (red neighbours + cell above) written to z,
+// then the backward sweep (cell below) over the same column.  Staging the
+// forward values in z instead of registers keeps the register count low
+// and the kernel memory-parallel.
+template <typename TM, typename TV, typename TP>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__restrict__ A,
+                                                       const TP *__restrict__ piv,
+                                                       TV *__restrict__ d,
+                                                       const TV *__restrict__ q,
+                                                       const TV *__restrict__ zf,
+                                                       TV *__restrict__ z, double *part,
+                                                       MaxLoc *mpart, const KState *st,
+                                                       const FState *fs, int nblk) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  const int NP = L.NP;
+  const TV alpha = (TV)fs->apend_d;
+  double ss = 0.0, rho = 0.0;
+  MaxLoc md{0.0, -1};
+  for (int h2 = blockIdx.x * blockDim.x + threadIdx.x; h2 < L.HP1;
+       h2 += gridDim.x * blockDim.x) {
+    int l = col_l(L, 1, h2);
+    int i = l / L.ncol, j = l - i * L.ncol;
+    int ib = i > 0 ? hn(l, -L.ncol) : -1;
+    int il = j > 0 ? hn(l, -1) : -1;
+    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    TV zup = (TV)0, lup = (TV)0;
+    for (int k = 0; k < L.nlay; k++) {
+      int pos = L.H + k * L.HP + h2;
+      int rb = k * L.HP;
+      TV dv = d[pos];
+      dv = dv - alpha * q[pos];
+      d[pos] = dv;
+      md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
+      ss = ss + (double)dv * (double)dv;
+      TV tv = dv;
+      // L(n, r) = A(n, r) piv(r); A(n, back) = front(back), A(n, left) =
+      // right(left), A(n, right) = right(n), A(n, front) = front(n)
+      if (ib >= 0) tv = tv - (TV)((TV)A[5 * NP + rb + ib] * (TV)ldp(piv, rb + ib)) * zf[rb + ib];
+      if (il >= 0) tv = tv - (TV)((TV)A[4 * NP + rb + il] * (TV)ldp(piv, rb + il)) * zf[rb + il];
+      if (ir >= 0) tv = tv - (TV)((TV)A[4 * NP + pos] * (TV)ldp(piv, rb + ir)) * zf[rb + ir];
+      if (ifr >= 0) tv = tv - (TV)((TV)A[5 * NP + pos] * (TV)ldp(piv, rb + ifr)) * zf[rb + ifr];
+      if (k > 0) tv = tv - lup * zup;
+      z[pos] = tv;
+      zup = tv;
+      lup = (TV)((TV)A[6 * NP + pos] * (TV)ldp(piv, pos));
+    }
+    TV zdn = (TV)0;
+    for (int k = L.nlay - 1; k >= 0; k--) {
+      int pos = L.H + k * L.HP + h2;
+      TV tv = z[pos];
+      if (k < L.nlay - 1) tv = tv - (TV)A[6 * NP + pos] * zdn;
+      tv = tv * (TV)ldp(piv, pos);
+      z[pos] = tv;
+      zdn = tv;
+      rho = rho + (double)d[pos] * (double)tv;
+    }
+  }
+  ss = bsum(ss, lds);
+  if (threadIdx.x == 0) part[P_SS_B * nblk + blockIdx.x] = ss;
+  rho = bsum(rho, lds);
+  if (threadIdx.x == 0) part[P_RHO_B * nblk + blockIdx.x] = rho;
+  md = bmax(md, mlds);
+  if (threadIdx.x == 0) mpart[M_D_B * nblk + blockIdx.x] = md;
+}
+
+template <typename TM, typename TV, typename TP>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__restrict__ A,
+                                                         const TP *__restrict__ piv,
+                                                         const TV *__restrict__ d,
+                                                         const TV *__restrict__ zf,
+                                                         TV *__restrict__ z, double *part,
+                                                         const KState *st, int nblk) {
+  __shared__ double lds[8];
+  if (st->stop) return;
+  const int NP = L.NP;
+  double rho = 0.0;
+  for (int h2 = blockIdx.x * blockDim.x + threadIdx.x; h2 < L.HP0;
+       h2 += gridDim.x * blockDim.x) {
+    int l = col_l(L, 0, h2);
+    int i = l / L.ncol, j = l - i * L.ncol;
+    int ib = i > 0 ? hn(l, -L.ncol) : -1;
+    int il = j > 0 ? hn(l, -1) : -1;
+    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    TV zdn = (TV)0;
+    for (int k = L.nlay - 1; k >= 0; k--) {
+      int pos = k * L.HP + h2;
+      int bb = L.H + k * L.HP;
+      TV tv = zf[pos];
+      if (k < L.nlay - 1) tv = tv - (TV)A[6 * NP + pos] * zdn;
+      if (ib >= 0) tv = tv - (TV)A[5 * NP + bb + ib] * z[bb + ib];
+      if (il >= 0) tv = tv - (TV)A[4 * NP + bb + il] * z[bb + il];
+      if (ir >= 0) tv = tv - (TV)A[4 * NP + pos] * z[bb + ir];
+      if (ifr >= 0) tv = tv - (TV)A[5 * NP + pos] * z[bb + ifr];
+      tv = tv * (TV)ldp(piv, pos);
+      z[pos] = tv;
+      zdn = tv;
+      rho = rho + (double)d[pos] * (double)tv;
+    }
+  }
+  rho = bsum(rho, lds);
+  if (threadIdx.x == 0) part[P_RHO_R * nblk + blockIdx.x] = rho;
+}
+
+// ---------------------------------------------------------------------------
+// direction update with the deferred head update, then the SpMV
+// ---------------------------------------------------------------------------
+
+// p = z (first iteration) or z + beta p_prev ; x += alpha_prev p_prev ;
+// partial max |p| (signed, first occurrence) for deltax = alpha pmax
+template <typename TV, typename TX>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__restrict__ z,
+                                                         const TV *__restrict__ pin,
+                                                         TV *__restrict__ pout, TX *__restrict__ x,
+                                                         MaxLoc *mpart, const KState *st,
+                                                         const FState *fs, int nblk) {
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  const bool first = st->iter == 0;
+  const TV beta = (TV)st->beta;
+  const TV ax = (TV)fs->apend_x;
+  MaxLoc mp{0.0, -1};
+  const int nthreads = 2 * L.HP0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nthreads;
+       t += gridDim.x * blockDim.x) {
+    int c = t & 1;
+    int h2 = t >> 1;
+    if (h2 >= ncols_of(L, c)) continue;
+    int l = col_l(L, c, h2);
+    const int own = c * L.H + h2;
+    for (int k = 0; k < L.nlay; k++) {
+      int pos = own + k * L.HP;
+      TV pv;
+      if (first) {
+        pv = z[pos];
+      } else {
+        TV po = pin[pos];
+        pv = z[pos] + beta * po;
+        x[pos] = (TX)(x[pos] + (TX)(ax * po));
+      }
+      pout[pos] = pv;
+      mp = maxloc_pick(mp, MaxLoc{(double)pv, k * L.NRC + l});
+    }
+  }
+  mp = bmax(mp, mlds);
+  if (threadIdx.x == 0) mpart[M_P * nblk + blockIdx.x] = mp;
+}
+
+// q = A p (amux order, symmetric planes), partial p.q
+template <typename TM, typename TV>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__restrict__ A,
+                                                      const TV *__restrict__ p,
+                                                      TV *__restrict__ q, double *part,
+                                                      const KState *st, int nblk) {
+  __shared__ double lds[8];
+  if (st->stop) return;
+  const int NP = L.NP;
+  double s = 0.0;
+  // adjacent threads take the red and the black column of the same index,
+  // so the neighbour values one colour gathers from the other are the ones
+  // its partner thread streams
+  const int nthreads = 2 * L.HP0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nthreads;
+       t += gridDim.x * blockDim.x) {
+    int c = t & 1;
+    int h2 = t >> 1;
+    if (h2 >= ncols_of(L, c)) continue;
+    int l = col_l(L, c, h2);
+    int i = l / L.ncol, j = l - i * L.ncol;
+    int ib = i > 0 ? hn(l, -L.ncol) : -1;
+    int il = j > 0 ? hn(l, -1) : -1;
+    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    const int own = c * L.H + h2;
+    const int oth = (1 - c) * L.H;
+    TV pprev = (TV)0;
+    TV pcur = p[own];
+    TV aup = (TV)0;
+    for (int k = 0; k < L.nlay; k++) {
+      int pos = own + k * L.HP;
+      int ob = oth + k * L.HP;
+      TV pnext = (k + 1 < L.nlay) ? p[pos + L.HP] : (TV)0;
+      TV adn = (TV)A[6 * NP + pos];
+      TV tq = (TV)0;
+      tq = tq + (TV)A[pos] * pcur;
+      if (k > 0) tq = tq + aup * pprev;
+      if (ib >= 0) tq = tq + (TV)A[5 * NP + ob + ib] * p[ob + ib];
+      if (il >= 0) tq = tq + (TV)A[4 * NP + ob + il] * p[ob + il];
+      if (ir >= 0) tq = tq + (TV)A[4 * NP + pos] * p[ob + ir];
+      if (ifr >= 0) tq = tq + (TV)A[5 * NP + pos] * p[ob + ifr];
+      if (k < L.nlay - 1) tq = tq + adn * pnext;
+      q[pos] = tq;
+      s = s + (double)pcur * (double)tq;
+      aup = adn;
+      pprev = pcur;
+      pcur = pnext;
+    }
+  }
+  s = bsum(s, lds);
+  if (threadIdx.x == 0) part[P_PQ * nblk + blockIdx.x] = s;
+}
+
+// x += apend_x p (the last iteration's head update), then no update pending
+template <typename TV, typename TX>
+__global__ void k_f_flush(int np, TX *__restrict__ x, const TV *__restrict__ p,
+                          const FState *fs) {
+  const TV ax = (TV)fs->apend_x;
+  if (ax == (TV)0) return;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < np; e += gridDim.x * blockDim.x)
+    x[e] = (TX)(x[e] + (TX)(ax * p[e]));
+}
+
+// ---------------------------------------------------------------------------
+// scalar kernels
+// ---------------------------------------------------------------------------
+
+__device__ inline void testcnvg_(int icnvgopt, int *icnvg, int iiter, double dvmax, double rmax,
+                                 double rmax0, double epfact, double dvclose, double rclose) {
+  if (icnvgopt == 0) {
+    if (fabs(dvmax) <= dvclose && fabs(rmax) <= rclose) *icnvg = 1;
+  } else if (icnvgopt == 1) {
+    if (fabs(dvmax) <= dvclose && fabs(rmax) <= rclose) *icnvg = iiter == 1 ? 1 : -1;
+  } else if (icnvgopt == 2) {
+    if (fabs(dvmax) <= dvclose || rmax <= rclose) *icnvg = 1;
+    else if (rmax <= rmax0 * epfact) *icnvg = -1;
+  } else if (icnvgopt == 3) {
+    if (fabs(dvmax) <= dvclose) *icnvg = 1;
+    else if (rmax <= rmax0 * rclose) *icnvg = -1;
+  } else if (icnvgopt == 4) {
+    if (fabs(dvmax) <= dvclose && rmax <= rclose) *icnvg = 1;
+    else if (rmax <= rmax0 * epfact) *icnvg = -1;
+  }
+}
+
+// End of a round: test the iteration whose residual the preconditioner
+// kernels just produced, then rho / beta of the next iteration.
+__global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
+                          int nblk, double *hist, int *hloc) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  double ssr = rsum(part + P_SS_R * nblk, nblk, lds);
+  double ssb = rsum(part + P_SS_B * nblk, nblk, lds);
+  double rb = rsum(part + P_RHO_B * nblk, nblk, lds);
+  double rr = rsum(part + P_RHO_R * nblk, nblk, lds);
+  MaxLoc mdr = rmaxl(mpart + M_D_R * nblk, nblk, mlds);
+  MaxLoc mdb = rmaxl(mpart + M_D_B * nblk, nblk, mlds);
+  if (threadIdx.x != 0) return;
+  MaxLoc md = maxloc_pick(mdr, mdb);
+  double rho_new = rb + rr;
+  int it = st->iter;   // iteration whose residual was just formed (0: initial)
+  fs->apend_d = 0.0;
+  if (it > 0) {
+    double l2norm = sqrt(ssr + ssb);
+    double deltax = st->alpha * fs->pmax;
+    st->deltax = deltax;
+    st->xloc = fs->ploc;
+    st->rmax = md.v;
+    st->rloc = md.loc;
+    st->l2norm = l2norm;
+    if (it <= st->hist_cap) {
+      int cap = st->hist_cap;
+      hist[it - 1] = deltax;
+      hist[cap + it - 1] = md.v;
+      hist[2 * cap + it - 1] = st->alpha;
+      hist[3 * cap + it - 1] = 0.0;
+      hloc[it - 1] = fs->ploc;
+      hloc[cap + it - 1] = md.loc;
+    }
+    int o = st->icnvgopt;
+    double rcnvg = (o == 2 || o == 3 || o == 4) ? l2norm : md.v;
+    int icnvg = st->icnvg;
+    testcnvg_(o, &icnvg, it, deltax, rcnvg, st->l2norm0, st->epfact, st->dvclose, st->rclose);
+    if (rcnvg == 0.0) icnvg = 1;
+    st->icnvg = icnvg;
+    if (icnvg != 0) { st->stop = 1; return; }
+    if (is_close_(st->rho, st->rho0)) { st->stop = 2; return; }
+    if (st->rho == 0.0) { st->stop = 2; return; }
+    st->rho0 = st->rho;
+    if (it >= st->itmax) { st->stop = 3; return; }
+  }
+  st->rho = rho_new;
+  if (it > 0) st->beta = rho_new / st->rho0;
+}
+
+__global__ void k_f_alpha(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
+                          int nblk) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  double pq = rsum(part + P_PQ * nblk, nblk, lds);
+  MaxLoc mp = rmaxl(mpart + M_P * nblk, nblk, mlds);
+  if (threadIdx.x != 0) return;
+  double den = pq;
+  den = den + copysign(DPREC_F, den);
+  st->pq = pq;
+  st->alpha = st->rho / den;
+  st->iter = st->iter + 1;
+  fs->apend_x = st->alpha;
+  fs->apend_d = st->alpha;
+  fs->pmax = mp.v;
+  fs->ploc = mp.loc;
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: reliable update (residual replacement)
+// ---------------------------------------------------------------------------
+
+// x64 += xhat (FP32 correction), xhat = 0
+__global__ void k_r_accumulate(int np, double *__restrict__ x, float *__restrict__ xhat) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < np; e += gridDim.x * blockDim.x) {
+    x[e] = x[e] + (double)xhat[e];
+    xhat[e] = 0.0f;
+  }
+}
+
+// d32 = b - A x (FP64 row sums, amux order), partial max |r|
+__global__ void __launch_bounds__(MF6_BLOCK) k_r_residual(Layout L, const double *__restrict__ A,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ b,
+                                                          float *__restrict__ d32,
+                                                          MaxLoc *mpart) {
+  __shared__ MaxLoc mlds[8];
+  const int NP = L.NP;
+  MaxLoc m{0.0, -1};
+  const int ncolumns = L.HP0 + L.HP1;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncolumns;
+       t += gridDim.x * blockDim.x) {
+    int c = t >= L.HP0;
+    int h2 = t - c * L.HP0;
+    int l = col_l(L, c, h2);
+    int i = l / L.ncol, j = l - i * L.ncol;
+    int ib = i > 0 ? hn(l, -L.ncol) : -1;
+    int il = j > 0 ? hn(l, -1) : -1;
+    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    const int own = c * L.H + h2;
+    const int oth = (1 - c) * L.H;
+    for (int k = 0; k < L.nlay; k++) {
+      int pos = own + k * L.HP;
+      int ob = oth + k * L.HP;
+      double t2 = 0.0;
+      t2 = t2 + A[pos] * x[pos];
+      if (k > 0) t2 = t2 + A[NP + pos] * x[pos - L.HP];
+      if (ib >= 0) t2 = t2 + A[2 * NP + pos] * x[ob + ib];
+      if (il >= 0) t2 = t2 + A[3 * NP + pos] * x[ob + il];
+      if (ir >= 0) t2 = t2 + A[4 * NP + pos] * x[ob + ir];
+      if (ifr >= 0) t2 = t2 + A[5 * NP + pos] * x[ob + ifr];
+      if (k < L.nlay - 1) t2 = t2 + A[6 * NP + pos] * x[pos + L.HP];
+      double r = b[pos] - t2;
+      d32[pos] = (float)r;
+      m = maxloc_pick(m, MaxLoc{r, k * L.NRC + l});
+    }
+  }
+  m = bmax(m, mlds);
+  if (threadIdx.x == 0) mpart[blockIdx.x] = m;
+}
+
+// after a replacement: no pending d / x update; a pending FP32 test result
+// is re-examined on the true residual
+__global__ void k_r_after(KState *st, FState *fs, const MaxLoc *mpart, int nblk, double *out) {
+  __shared__ MaxLoc mlds[8];
+  MaxLoc m = rmaxl(mpart, nblk, mlds);
+  if (threadIdx.x != 0) return;
+  fs->apend_d = 0.0;
+  fs->apend_x = 0.0;
+  out[0] = m.v;
+  out[1] = (double)m.loc;
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+
+static inline int gridc(int n) {
+  int g = (n + MF6_BLOCK - 1) / MF6_BLOCK;
+  if (g > MF6_MAXBLK) g = MF6_MAXBLK;
+  return g < 1 ? 1 : g;
+}
+
+template <typename TM, typename TV, typename TP>
+static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV *q, TV *zf,
+                      TV *z, KState *st, FState *fs, int nb) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  k_f_red_fwd<TM, TV, TP><<<nb, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
+                                                   st, fs, nb);
+  k_f_black<TM, TV, TP><<<nb, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
+                                                 st, fs, nb);
+  k_f_red_bwd<TM, TV, TP><<<nb, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb);
+}
+
+static int read_states(Mf6Handle *h) {
+  HIPCHK(hipMemcpyAsync(h->h_state, h->d_state, 3 * sizeof(KState), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return MF6HIP_OK;
+}
+
+// Fused FP64 CG on the device system.  Requires: d = b - A x in h->d,
+// l2norm0 in state[0], q zeroed, A symmetric.
+int mf6hip_run_cg_fused(Mf6Handle *h) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  KState *st = h->d_state;
+  FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
+  HIPCHK(hipMemsetAsync(fs, 0, sizeof(FState), s));
+  HIPCHK(hipMemsetAsync(h->q, 0, sizeof(double) * L.NP, s));
+  // one grid size for every kernel of a round, so that each partial array
+  // has the same number of slots for the round kernel to reduce
+  const int nbc = gridc(L.HP0);
+  const int nbs = gridc(2 * L.HP0);
+  const int nb = nbs > nbc ? nbs : nbc;
+  double *pbuf[2] = {h->p, h->p2};
+  int batch = 4;
+  int round = 0;
+  // round 0: preconditioner of the initial residual
+  while (true) {
+    for (int it = 0; it < batch; it++, round++) {
+      launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs,
+                                        nb);
+      k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, h->d_hist,
+                                        h->d_hloc);
+      double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
+      k_f_pupdate<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->z, pin, pout, h->x, h->d_mpart,
+                                                           st, fs, nb);
+      int slot = -1;
+      if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
+          2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
+        slot = h->last_spmv_n++;
+        (void)hipEventRecord(h->spmv_ev[2 * slot], s);
+      }
+      k_f_spmv<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->A, pout, h->q, h->d_part, st, nb);
+      if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
+      k_f_alpha<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
+    }
+    HIPCHK(hipGetLastError());
+    CK(read_states(h));
+    if (h->h_state->stop) break;
+    if (batch < 64) batch *= 2;
+  }
+  // apply the pending head update of the last iteration: the direction of
+  // iteration m lives in pbuf[m & 1]
+  int m = h->h_state->iter;
+  if (m > 0) {
+    k_f_flush<double, double><<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->x, pbuf[m & 1], fs);
+    HIPCHK(hipGetLastError());
+  }
+  return MF6HIP_OK;
+}
+
+// Mixed precision: FP32 fused CG with reliable updates.  Requires the FP32
+// planes (A32) and pivots; x (FP64) holds the start heads, b the rhs.
+template <typename TP>
+static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  KState *st = h->d_state;
+  FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
+  const int nbc = gridc(L.HP0);
+  const int nbs = gridc(2 * L.HP0);
+  const int nb = nbs > nbc ? nbs : nbc;
+  float *pbuf[2] = {h->p32, h->p232};
+  double *rout = h->d_part + 7 * MF6_MAXBLK;
+  const int R = h->s.inner_fp32_max > 0 ? h->s.inner_fp32_max : 64;   // iterations between replacements
+  HIPCHK(hipMemsetAsync(fs, 0, sizeof(FState), s));
+  HIPCHK(hipMemsetAsync(h->q32, 0, sizeof(float) * L.NP, s));
+  HIPCHK(hipMemsetAsync(h->e32, 0, sizeof(float) * L.NP, s));
+  // true residual into d32 (r32)
+  k_r_residual<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->r32, h->d_mpart);
+  k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout);
+  int round = 0, last_replace = 0;
+  int total_budget = h->s.iter1;
+  bool done = false;
+  *icnvg = 0;
+  while (!done) {
+    int batch = 8;
+    while (true) {
+      for (int it = 0; it < batch; it++, round++) {
+        launch_pc<float, float, TP>(h, h->A32, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs,
+                                    nb);
+        k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, h->d_hist,
+                                          h->d_hloc);
+        float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
+        k_f_pupdate<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->z32, pin, pout, h->e32,
+                                                           h->d_mpart, st, fs, nb);
+        int slot = -1;
+        if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
+            2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
+          slot = h->last_spmv_n++;
+          (void)hipEventRecord(h->spmv_ev[2 * slot], s);
+        }
+        k_f_spmv<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->A32, pout, h->q32, h->d_part, st,
+                                                        nb);
+        if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
+        k_f_alpha<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
+      }
+      HIPCHK(hipGetLastError());
+      CK(read_states(h));
+      if (h->h_state->stop) break;
+      if (h->h_state->iter - last_replace >= R) break;
+      if (batch < 32) batch *= 2;
+    }
+    KState hs = h->h_state[0];
+    int m = hs.iter;
+    // replace the residual: x += xhat + pending alpha p ; d32 = b - A x
+    // the head update of iteration m (direction p_m) is still pending, and
+    // so is alpha q for d when no test ran yet: fold both into the FP64
+    // heads and recompute the residual from them
+    if (m > 0)
+      k_f_flush<float, float><<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->e32, pbuf[m & 1], fs);
+    round = m;
+    k_r_accumulate<<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->x, h->e32);
+    k_r_residual<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->r32, h->d_mpart);
+    k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout);
+    double hr[2];
+    HIPCHK(hipMemcpyAsync(hr, rout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    last_replace = m;
+    if (hs.stop != 0) {
+      bool true_ok = fabs(hr[0]) <= h->s.rclose;
+      if (hs.stop == 1 && true_ok) { *icnvg = 1; done = true; break; }
+      if (hs.stop == 3 || m >= total_budget) { done = true; break; }
+      if (hs.stop == 2 && true_ok && fabs(hs.deltax) <= h->s.dvclose) { *icnvg = 1; done = true; break; }
+      if (hs.stop == 2) { done = true; break; }
+      // FP32 test passed but the true residual did not: continue from the
+      // replaced residual, re-testing this iteration with it
+      h->h_state[0].stop = 0;
+      h->h_state[0].icnvg = 0;
+      h->h_state[0].iter = m;
+      HIPCHK(hipMemcpyAsync(st, h->h_state, sizeof(KState), hipMemcpyHostToDevice, s));
+    }
+  }
+  *iters = h->h_state[0].iter;
+  return MF6HIP_OK;
+}
+
+int mf6hip_run_cg_reliable(Mf6Handle *h, int *icnvg, int *iters) {
+  if (h->s.precision == MF6HIP_MIXED) return run_cg_reliable<float>(h, h->invpiv32, icnvg, iters);
+  return run_cg_reliable<unsigned short>(h, h->invpiv16, icnvg, iters);
+}
+
+// bitwise symmetry of the planes: A(n, up) == A(up, n), A(n, back) ==
+// A(back, n), A(n, left) == A(left, n)
+__global__ void k_check_sym(Layout L, const double *__restrict__ A, int *flag) {
+  const int NP = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < NP; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    const int ob = (1 - c) * L.H + k * L.HP;
+    bool ok = true;
+    if (k > 0) ok = ok && (A[NP + p] == A[6 * NP + p - L.HP]);
+    if (i > 0) ok = ok && (A[2 * NP + p] == A[5 * NP + ob + hn(l, -L.ncol)]);
+    if (j > 0) ok = ok && (A[3 * NP + p] == A[4 * NP + ob + hn(l, -1)]);
+    if (!ok) atomicOr(flag, 1);
+  }
+}
+
+int mf6hip_is_symmetric(Mf6Handle *h, int *sym) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  int *flag = h->d_fail + 1;
+  HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  k_check_sym<<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L, h->A, flag);
+  int hf = 0;
+  HIPCHK(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *sym = hf == 0;
+  return MF6HIP_OK;
+}

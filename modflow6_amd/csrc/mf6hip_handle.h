@@ -112,5 +112,8 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg,
                         int *in_iter, mf6hip_cnvg_out *out);
 int mf6hip_launch_row_fixes(Mf6Handle *h);
 int mf6hip_copy_system(Mf6Handle *h, double *amat_out, double *rhs_out);
+int mf6hip_run_cg_fused(Mf6Handle *h);
+int mf6hip_run_cg_reliable(Mf6Handle *h, int *icnvg, int *iters);
+int mf6hip_is_symmetric(Mf6Handle *h, int *sym);
 int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa,
                  const double *xb, bool active_only);

@@ -1217,21 +1217,31 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
   h->spmv_calls = 0;
   HIPCHK(hipEventRecord(h->ev0, s));
   CK(ilu_factor(h));
+  int sym = 0;
+  if (h->s.ilinmeth == 1 && h->s.north == 0) CK(mf6hip_is_symmetric(h, &sym));
   if (h->s.precision != MF6HIP_FP64) {
     k_convert<double, float><<<grid_for(MF6_NSLOT * L.NP), MF6_BLOCK, 0, s>>>(
         MF6_NSLOT * L.NP, h->A, h->A32);
     CK(init_state(h, kstp));
-    int rc;
-    if (h->s.precision == MF6HIP_MIXED) rc = run_mixed_cg<float>(h, h->invpiv32);
-    else rc = run_mixed_cg<unsigned short>(h, h->invpiv16);
-    if (rc) return rc;
+    if (sym && h->s.icnvgopt == 0 && h->s.kernel_variant == 0) {
+      int ic = 0, its = 0;
+      CK(mf6hip_run_cg_reliable(h, &ic, &its));
+      h->h_state->icnvg = ic;
+      h->h_state->iter = its;
+    } else {
+      int rc;
+      if (h->s.precision == MF6HIP_MIXED) rc = run_mixed_cg<float>(h, h->invpiv32);
+      else rc = run_mixed_cg<unsigned short>(h, h->invpiv16);
+      if (rc) return rc;
+    }
   } else {
     k_residual<double, double><<<g, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->d, nullptr);
     CK(init_state(h, kstp));
     CK(compute_l2norm0(h));
     CK(read_state(h));
     if (!h->h_state->stop) {
-      if (h->s.ilinmeth == 1) CK(run_cg(h));
+      if (h->s.ilinmeth == 1 && sym && h->s.kernel_variant == 0) CK(mf6hip_run_cg_fused(h));
+      else if (h->s.ilinmeth == 1) CK(run_cg(h));
       else CK(run_bcgs(h));
     }
   }
@@ -1294,6 +1304,7 @@ static int check_settings(const mf6hip_ims_settings *s) {
   if (s->precision < 0 || s->precision > 2) { mf6hip_set_error("unknown precision mode"); return MF6HIP_EINVAL; }
   if (s->precision != MF6HIP_FP64 && s->ilinmeth != 1) { mf6hip_set_error("mixed precision is implemented for CG only"); return MF6HIP_EUNSUPPORTED; }
   if (s->ordering != MF6HIP_ORDER_RB) { mf6hip_set_error("unknown ordering"); return MF6HIP_EINVAL; }
+  if (s->kernel_variant < 0 || s->kernel_variant > 1) { mf6hip_set_error("unknown kernel variant"); return MF6HIP_EINVAL; }
   if (s->nlay < 1 || s->nrow < 1 || s->ncol < 1) { mf6hip_set_error("DIS shape nlay/nrow/ncol required"); return MF6HIP_EINVAL; }
   if (s->nlay > 32) { mf6hip_set_error("more than 32 layers is not supported on the device path"); return MF6HIP_EUNSUPPORTED; }
   return MF6HIP_OK;
@@ -1334,7 +1345,7 @@ static int alloc_solver(Mf6Handle *h) {
   CK(dalloc(&h->xtemp, N));
   CK(dalloc(&h->invpiv, N));
   CK(dalloc(&h->invpiv_tmp, N));
-  CK(dalloc(&h->d_fail, 1));
+  CK(dalloc(&h->d_fail, 4));
   CK(dalloc(&h->d, N));
   CK(dalloc(&h->zf, N));
   CK(dalloc(&h->z, N));
@@ -1364,7 +1375,7 @@ static int alloc_solver(Mf6Handle *h) {
   }
   CK(dalloc(&h->d_part, 8 * (size_t)MF6_MAXBLK));
   CK(dalloc(&h->d_mpart, 4 * (size_t)MF6_MAXBLK));
-  CK(dalloc(&h->d_state, 2));
+  CK(dalloc(&h->d_state, 4));
   h->hist_cap = h->s.iter1 > 0 ? h->s.iter1 : 1;
   CK(dalloc(&h->d_hist, 4 * (size_t)h->hist_cap));
   CK(dalloc(&h->d_hloc, 2 * (size_t)h->hist_cap));
@@ -1422,7 +1433,7 @@ extern "C" int mf6hip_create(int neq, int nja, const int *ia, const int *ja,
     if (!rc) rc = dalloc(&h->d_amat_csr, (size_t)nja);
   }
   if (!rc) {
-    if (hipHostMalloc((void **)&h->h_state, 2 * sizeof(KState), hipHostMallocDefault) != hipSuccess) rc = MF6HIP_EHIP;
+    if (hipHostMalloc((void **)&h->h_state, 4 * sizeof(KState), hipHostMallocDefault) != hipSuccess) rc = MF6HIP_EHIP;
   }
   if (!rc) {
     hipEventCreate(&h->ev0);

@@ -33,6 +33,7 @@ module mf6hip_module
     integer(c_int) :: ncol
     integer(c_int) :: inner_fp32_max
     real(c_double) :: inner_fp32_rtol
+    integer(c_int) :: kernel_variant
   end type mf6hip_ims_settings
 
   public :: mf6hip_create, mf6hip_solve, mf6hip_destroy, mf6hip_last_error

@@ -37,13 +37,15 @@ class ImsLinearSettings:
     ordering: int = _lib.MF6HIP_ORDER_RB
     inner_fp32_max: int = 0
     inner_fp32_rtol: float = 0.0
+    kernel_variant: int = 0      # 0 fused CG (symmetric systems), 1 unfused
 
     def to_c(self, nlay, nrow, ncol):
         return _lib.ImsSettingsC(
             self.dvclose, self.rclose, self.icnvgopt, self.iter1,
             self.ilinmeth, self.iscl, self.iord, self.north, self.relax,
             self.level, self.droptol, self.precision, self.ordering, nlay,
-            nrow, ncol, self.inner_fp32_max, self.inner_fp32_rtol)
+            nrow, ncol, self.inner_fp32_max, self.inner_fp32_rtol,
+            self.kernel_variant)
 
 
 def dis_structure(nlay, nrow, ncol):

@@ -46,7 +46,7 @@ class GwfSolution:
     """One IMS solution holding one DIS GWF model (a GwfCase)."""
 
     def __init__(self, case, precision=_lib.MF6HIP_FP64, timing=False,
-                 inner_fp32_rtol=0.0, inner_fp32_max=0):
+                 inner_fp32_rtol=0.0, inner_fp32_max=0, kernel_variant=0):
         self.case = c = case
         if c.ims.under_relaxation.upper() != "NONE":
             raise _lib.Mf6HipError("UNDER_RELAXATION is not supported on the "
@@ -59,7 +59,8 @@ class GwfSolution:
             icnvgopt=c.ims.icnvgopt, iter1=c.ims.inner_maximum,
             ilinmeth=c.ims.ilinmeth, north=c.ims.number_orthogonalizations,
             relax=c.ims.relaxation_factor, precision=precision,
-            inner_fp32_rtol=inner_fp32_rtol, inner_fp32_max=inner_fp32_max)
+            inner_fp32_rtol=inner_fp32_rtol, inner_fp32_max=inner_fp32_max,
+            kernel_variant=kernel_variant)
         self.solver = HipLinearSolver().initialize(ia, ja, self.settings,
                                                    c.nlay, c.nrow, c.ncol)
         self.h = self.solver.handle

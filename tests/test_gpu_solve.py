@@ -107,3 +107,29 @@ def test_mixed_precision_steady_small(precision):
           "max |dh|", np.max(np.abs(hd - ho)))
     assert gs.log[0][-1].converged
     assert np.max(np.abs(hd - ho)) < 10 * case.ims.inner_dvclose
+
+
+@pytest.mark.parametrize("shape", [(3, 100, 100), (1, 51, 51), (5, 33, 17)])
+def test_fused_cg_matches_unfused(shape):
+    """The fused thread-per-column CG and the one-kernel-per-pass CG run the
+    same iteration; only the time at which x / d are updated and the dot
+    summation order differ."""
+    nlay, nrow, ncol = shape
+    case = configs.steady_small(seed=9, nlay=nlay, nrow=nrow, ncol=ncol,
+                                nwel=min(20, nrow * (ncol - 2) // 4))
+    sim, amat, rhs, x0 = oracle_system(case)
+    res = {}
+    for variant in (0, 1):
+        sv = device_solver(case, kernel_variant=variant)
+        xd = x0.copy()
+        rec = ConvergenceRecord(case.ims.inner_maximum)
+        icn, it = sv.solve(1, amat, rhs, xd, kstp=1, record=rec)
+        sv.destroy()
+        res[variant] = (icn, it, xd, rec)
+    print("fused", res[0][1], "unfused", res[1][1])
+    assert res[0][0] == 1 and res[1][0] == 1
+    assert abs(res[0][1] - res[1][1]) <= max(3, int(0.02 * res[1][1]))
+    assert np.max(np.abs(res[0][2] - res[1][2])) < 10 * case.ims.inner_dvclose
+    for t in range(5):
+        assert abs(res[0][3].dvmax[t] - res[1][3].dvmax[t]) <= 1e-9 * abs(res[1][3].dvmax[t])
+        assert abs(res[0][3].rmax[t] - res[1][3].rmax[t]) <= 1e-9 * abs(res[1][3].rmax[t])
