@@ -31,13 +31,17 @@ extern "C" {
 
 /* precision of the device solve (opt-in; stock .ims files get MF6HIP_FP64) */
 #define MF6HIP_FP64 0        /* everything FP64 */
-#define MF6HIP_MIXED 1       /* FP32 ILU0 + FP32 SpMV inside FP64 iterative
-                                refinement; true residual, closure dots and
-                                the head update stay FP64 */
+#define MF6HIP_MIXED 1       /* FP32 ILU0 + FP32 SpMV: CG with the FP32
+                                residual replaced by the FP64 b - A x every
+                                inner_fp32_max iterations (symmetric systems),
+                                or FP64 iterative refinement around FP32
+                                inner solves; heads, true residual and the
+                                convergence test stay FP64 */
 #define MF6HIP_MIXED_FP16 2  /* as MIXED, ILU0 pivots stored in FP16 */
 
 /* ordering of the device ILU0 (the reference factors in natural order) */
-#define MF6HIP_ORDER_RB 0    /* red-black: colour (k+i+j) mod 2 */
+#define MF6HIP_ORDER_RB 0    /* column red-black: vertical cell columns
+                                coloured (irow + icol) mod 2 */
 
 /* Mirrors ImsLinearSettingsType (src/Solution/LinearMethods/
  * ImsLinearSettings.f90:13-32) field for field, plus the device opt-ins.
@@ -193,6 +197,31 @@ int mf6hip_gwf_outer_update(void *handle, int kiter, int *inewtonur,
  * kernel (the SpMV) with its launch count. */
 int mf6hip_last_timing(void *handle, double *solve_ms, double *spmv_avg_ms,
                        int *spmv_launches);
+
+/* --- diagnostics (no counterpart in the reference; used by the parity
+ * tests and the benchmark to reach single operators) --------------------- */
+
+/* enable (1) / disable (0) per-solve HIP-event timing; sample = number of
+ * SpMV launches timed individually per solve (at most 256) */
+int mf6hip_set_timing(void *handle, int enable, int sample);
+
+/* One device operator on host vectors in natural node order:
+ * op 0: y = A x (amux order, src/Utilities/Libraries/sparsekit/
+ *       sparsekit.f90:1-59);
+ * op 1: y = M^-1 x, the ILU0/MILU0 apply (ims_base_ilu0a, src/Solution/
+ *       LinearMethods/ImsLinearBase.f90:1049-1092) of the factor on the
+ *       device, in the device ordering;
+ * op 2: y = the inverse pivots of that factor (x unused).
+ * A and the factor are those of the last mf6hip_solve / mf6hip_load_factor. */
+int mf6hip_apply_op(void *handle, int op, const double *x, double *y);
+
+/* Upload amat (reference CSR layout) and factor it (ims_base_pcilu0,
+ * ImsLinearBase.f90:928-1042) on the device without solving. */
+int mf6hip_load_factor(void *handle, const double *amat);
+
+/* Copy the device-resident system of the last solve back (reference CSR
+ * layout, after the sln_ls row fixes); either pointer may be NULL. */
+int mf6hip_get_system(void *handle, double *amat_out, double *rhs_out);
 
 #ifdef __cplusplus
 }

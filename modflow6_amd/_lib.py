@@ -26,7 +26,8 @@ EXPORTS = [
     "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_x",
     "mf6hip_gwf_get_x", "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
     "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
-    "mf6hip_last_timing",
+    "mf6hip_last_timing", "mf6hip_set_timing", "mf6hip_apply_op",
+    "mf6hip_load_factor", "mf6hip_get_system",
 ]
 
 
@@ -112,8 +113,7 @@ def load():
     L.mf6hip_apply_op.argtypes = [vp, i, dp, dp]
     L.mf6hip_load_factor.argtypes = [vp, dp]
     L.mf6hip_get_system.argtypes = [vp, dp, dp]
-    for nm in EXPORTS + ["mf6hip_set_timing", "mf6hip_apply_op",
-                         "mf6hip_load_factor", "mf6hip_get_system"]:
+    for nm in EXPORTS:
         getattr(L, nm).restype = i
     _LIB = L
     return L

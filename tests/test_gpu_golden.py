@@ -1,0 +1,49 @@
+"""Device path (formulate + linear solve + Newton under-relaxation, all on the
+GPU) on the reference's own autotest models: closed-form heads of a 1-D CHD
+line (test_par_gwf01.py:200-211), test_gwf_newton01.py (heads 8 and 7) and
+the golden heads of test_gwf_newton_under_relaxation.py (results.hds.cmp,
+compared with np.allclose as that test does)."""
+import os
+
+import numpy as np
+import pytest
+
+from modflow6_amd.solution import GwfSolution
+from oracle import oracle as O
+from tests.cases import GOLD, bump_case, line_case, newton01_case, read_ulasav
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_linear_heads_between_two_chd():
+    case = line_case()
+    gs = GwfSolution(case)
+    h = gs.run()
+    gs.destroy()
+    exact = 1.0 + 9.0 * np.arange(case.ncol) / (case.ncol - 1)
+    assert np.max(np.abs(h - exact)) < 1e-9
+
+
+def test_device_newton01_closed_form():
+    gs = GwfSolution(newton01_case())
+    h = gs.run().reshape(2, 3, 3)
+    gs.destroy()
+    assert np.allclose(h[0], 8.0, rtol=0, atol=1e-7)
+    assert np.allclose(h[1], 7.0, rtol=0, atol=1e-12)
+
+
+def test_device_bump_newton_under_relaxation_golden():
+    case = bump_case()
+    gs = GwfSolution(case)
+    h = gs.run().reshape(51, 51)
+    recs = gs.log[0]
+    gs.destroy()
+    sim = O.GwfSimulation(case)
+    ho = sim.run().reshape(51, 51)
+    print("device outer iterations", len(recs), "oracle", len(sim.log[0]),
+          "NUR engaged", sum(r.nur for r in recs))
+    base = read_ulasav(os.path.join(GOLD, "ex-gwf-bump", "results.hds.cmp"))
+    assert recs[-1].converged
+    assert any(r.nur for r in recs)
+    assert np.allclose(base, h)
+    assert np.max(np.abs(h - ho)) < 1e-6
