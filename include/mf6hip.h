@@ -168,29 +168,71 @@ int mf6hip_gwf_get_x(void *handle, double *x);
  * length. */
 int mf6hip_gwf_advance(void *handle, int iss, double delt);
 
-/* sln_buildsystem (NumericalSolution.f90:1941-1991) for the GWF model:
- * amat = 0, rhs = 0, then npf_cf, bnd_cf, npf_fc, sto_fc, bnd_fc, npf_fn,
- * sto_fn in gwf_fc order (gwf.f90:446-555), bit-identical to the reference.
- * If amat_out / rhs_out are not NULL the assembled CSR values (reference
- * layout, nja) and rhs are copied back for inspection. */
-int mf6hip_gwf_formulate(void *handle, int kiter, double *amat_out,
+/* sln_buildsystem(kiter, inewton) (NumericalSolution.f90:1941-1991) for the
+ * GWF model: amat = 0, rhs = 0, then npf_cf, bnd_cf, npf_fc, sto_fc, bnd_fc
+ * and, when inewton = 1, npf_fn and sto_fn, in gwf_fc order (gwf.f90:446-555,
+ * inwtflag), bit-identical to the reference.  Backtracking refills with
+ * inewton = 0 (:2698).  If amat_out / rhs_out are not NULL the assembled CSR
+ * values (reference layout, nja) and rhs are copied back for inspection. */
+int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double *amat_out,
                          double *rhs_out);
 
+/* r = A x - b of the current device system with inactive rows zeroed
+ * (sln_calc_residual, NumericalSolution.f90:2966-2982): *l2norm = ||r||_2
+ * (sln_l2norm, :2855-2871) and *rvmax = max |r(n)| / (area (top - bot))
+ * over cells with ibound >= 1, the reciprocal pseudo-time step of gwf_ptc
+ * (gwf.f90:625-679) before its zero-residual protection. */
+int mf6hip_gwf_residual_norms(void *handle, double *l2norm, double *rvmax);
+
+/* sln_backtracking_xupdate (NumericalSolution.f90:2784-2843): *btflag = 1
+ * when breduc * max|x - xtemp| (active cells) >= dvclose, and then
+ * x = xtemp + breduc (x - xtemp) on active cells. */
+int mf6hip_gwf_backtrack(void *handle, double breduc, double dvclose,
+                         int *btflag);
+
+/* Pseudo-transient continuation of sln_ls (NumericalSolution.f90:2480-2573).
+ * The caller fills iptc and ptcf from sln_calc_ptc / gwf_ptc for this outer
+ * iteration and keeps ptcdel / l2norm0 between outer iterations. */
+typedef struct mf6hip_ptc {
+  int iptc;          /* in: PTC requested (gwf_ptc); out: 1 if PTC terms were
+                        added (after NO_PTC and the l2norm test) */
+  int iallowptc;     /* 1 allowed, 0 NO_PTC ALL, -1 NO_PTC FIRST */
+  int iptcopt;       /* 0: ptcdel = 1 / ptcf ; 1: ||b|| / ||r|| (DEV_PTC_OPTION) */
+  int kstp, kper;
+  double ptcf;       /* in: reciprocal pseudo-time step from gwf_ptc */
+  double ptcdel0;    /* DEV_PTC_DEL0 (0 = not set) */
+  double ptcexp;     /* DEV_PTC_EXPONENT (1) */
+  double ptcdel;     /* in/out state */
+  double l2norm0;    /* in/out state */
+} mf6hip_ptc;
+
 /* sln_ls (NumericalSolution.f90:2404-2615) on the device-resident system:
- * xtemp = x, row fixes, the linear solve, then sln_get_dxmax (:3122).
- * hncg receives the signed largest head change, lrch its 1-based node. */
-int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, int *icnvg,
-                            int *in_iter, double *hncg, int *lrch,
+ * xtemp = x, row fixes, pseudo-transient continuation when ptc is not NULL,
+ * the linear solve, then sln_get_dxmax (:3122).  hncg receives the signed
+ * largest head change, lrch its 1-based node. */
+int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, mf6hip_ptc *ptc,
+                            int *icnvg, int *in_iter, double *hncg, int *lrch,
                             mf6hip_cnvg_out *out);
 
+/* IMS NONLINEAR under-relaxation (sln_underrelax, NumericalSolution.f90
+ * :2989-3114): nonmeth 1 SIMPLE, 2 COOLEY, 3 DBD. */
+typedef struct mf6hip_underrelax {
+  int nonmeth;
+  double gamma, theta, akappa, amomentum;
+  double bigch;      /* in: hncg of this outer iteration */
+  double bigchold;   /* in/out: Cooley state */
+  double relaxold;   /* in/out: Cooley state */
+} mf6hip_underrelax;
+
 /* The tail of NumericalSolution%solve (:1722-1760) for one model when the
- * outer iteration has not converged: sln_calcdx (no under-relaxation,
- * nonmeth = 0), then npf_nur (gwf-npf.f90:705-741) if Newton
- * under-relaxation is on.  On return *inewtonur tells whether NUR moved a
- * head, *dxold_max is sln_maxval of dxold, and, when NUR applied, *hncg /
- * *lrch are recomputed (sln_get_dxmax). */
-int mf6hip_gwf_outer_update(void *handle, int kiter, int *inewtonur,
-                            double *dxold_max, double *hncg, int *lrch);
+ * outer iteration has not converged: sln_underrelax when ur is not NULL and
+ * ur->nonmeth > 0, else sln_calcdx; then npf_nur (gwf-npf.f90:705-741) if
+ * Newton under-relaxation is on.  On return *inewtonur tells whether NUR
+ * moved a head, *dxold_max is sln_maxval of dxold, and, when NUR applied,
+ * *hncg / *lrch are recomputed (sln_get_dxmax). */
+int mf6hip_gwf_outer_update(void *handle, int kiter, mf6hip_underrelax *ur,
+                            int *inewtonur, double *dxold_max, double *hncg,
+                            int *lrch);
 
 /* Timing of the last mf6hip_gwf_linear_solve on the device stream (HIP
  * events): total milliseconds and the average duration of its dominant

@@ -26,6 +26,7 @@ EXPORTS = [
     "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_x",
     "mf6hip_gwf_get_x", "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
     "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
+    "mf6hip_gwf_residual_norms", "mf6hip_gwf_backtrack",
     "mf6hip_last_timing", "mf6hip_set_timing", "mf6hip_apply_op",
     "mf6hip_load_factor", "mf6hip_get_system",
 ]
@@ -73,6 +74,23 @@ class GwfModelC(C.Structure):
                 ("sy", C.POINTER(C.c_double))]
 
 
+class PtcC(C.Structure):
+    """mf6hip_ptc: pseudo-transient continuation control and state."""
+    _fields_ = [("iptc", C.c_int), ("iallowptc", C.c_int),
+                ("iptcopt", C.c_int), ("kstp", C.c_int), ("kper", C.c_int),
+                ("ptcf", C.c_double), ("ptcdel0", C.c_double),
+                ("ptcexp", C.c_double), ("ptcdel", C.c_double),
+                ("l2norm0", C.c_double)]
+
+
+class UnderRelaxC(C.Structure):
+    """mf6hip_underrelax: IMS under-relaxation settings and Cooley state."""
+    _fields_ = [("nonmeth", C.c_int), ("gamma", C.c_double),
+                ("theta", C.c_double), ("akappa", C.c_double),
+                ("amomentum", C.c_double), ("bigch", C.c_double),
+                ("bigchold", C.c_double), ("relaxold", C.c_double)]
+
+
 class Mf6HipError(RuntimeError):
     pass
 
@@ -104,10 +122,13 @@ def load():
     L.mf6hip_gwf_set_x.argtypes = [vp, dp]
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
     L.mf6hip_gwf_advance.argtypes = [vp, i, d]
-    L.mf6hip_gwf_formulate.argtypes = [vp, i, dp, dp]
-    L.mf6hip_gwf_linear_solve.argtypes = [vp, i, i, ip, ip, dp, ip,
-                                          C.POINTER(CnvgOutC)]
-    L.mf6hip_gwf_outer_update.argtypes = [vp, i, ip, dp, dp, ip]
+    L.mf6hip_gwf_formulate.argtypes = [vp, i, i, dp, dp]
+    L.mf6hip_gwf_residual_norms.argtypes = [vp, dp, dp]
+    L.mf6hip_gwf_backtrack.argtypes = [vp, d, d, ip]
+    L.mf6hip_gwf_linear_solve.argtypes = [vp, i, i, C.POINTER(PtcC), ip, ip,
+                                          dp, ip, C.POINTER(CnvgOutC)]
+    L.mf6hip_gwf_outer_update.argtypes = [vp, i, C.POINTER(UnderRelaxC), ip,
+                                          dp, dp, ip]
     L.mf6hip_last_timing.argtypes = [vp, dp, dp, ip]
     L.mf6hip_set_timing.argtypes = [vp, i, i]
     L.mf6hip_apply_op.argtypes = [vp, i, dp, dp]

@@ -43,11 +43,42 @@ class ImsOptions:
     linear_acceleration: str = "CG"      # CG | BICGSTAB
     relaxation_factor: float = 0.0       # > 0 -> MILU0
     number_orthogonalizations: int = 0
-    under_relaxation: str = "NONE"       # only NONE on the device path
+    # NONLINEAR block (SIMPLE complexity defaults, sln_setouter
+    # NumericalSolution.f90:2623-2671)
+    under_relaxation: str = "NONE"       # NONE | SIMPLE | COOLEY | DBD
+    under_relaxation_gamma: float = 1.0
+    under_relaxation_theta: float = 1.0
+    under_relaxation_kappa: float = 0.0
+    under_relaxation_momentum: float = 0.0
+    backtracking_number: int = 0
+    backtracking_tolerance: float = 0.0
+    backtracking_reduction_factor: float = 0.0
+    backtracking_residual_limit: float = 0.0
+    # OPTIONS block: NO_PTC [ALL | FIRST]; None = PTC allowed (the default,
+    # active for steady-state Newton models); DEV_PTC_* options
+    no_ptc: Optional[str] = None
+    ptc_option: int = 0
+    ptc_exponent: float = 1.0
+    ptc_del0: float = 0.0
 
     @property
     def ilinmeth(self):
         return 1 if self.linear_acceleration.upper() == "CG" else 2
+
+    @property
+    def nonmeth(self):
+        return {"NONE": 0, "SIMPLE": 1, "COOLEY": 2, "DBD": 3}[self.under_relaxation.upper()]
+
+    @property
+    def iallowptc(self):
+        if self.no_ptc is None:
+            return 1
+        return -1 if self.no_ptc.upper() == "FIRST" else 0
+
+    @property
+    def theta(self):
+        # sln_ar clamps THETA below 1e-3 (NumericalSolution.f90:868-870)
+        return max(self.under_relaxation_theta, 1e-3)
 
 
 @dataclass

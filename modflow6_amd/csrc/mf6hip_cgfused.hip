@@ -142,6 +142,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__r
                                                          TV *__restrict__ zf, double *part,
                                                          MaxLoc *mpart, const KState *st,
                                                          const FState *fs, int nblk) {
+  const int lb = xcd_block();
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__r
   const TV alpha = (TV)fs->apend_d;
   double ss = 0.0;
   MaxLoc md{0.0, -1};
-  for (int h2 = blockIdx.x * blockDim.x + threadIdx.x; h2 < L.HP0;
+  for (int h2 = lb * blockDim.x + threadIdx.x; h2 < L.HP0;
        h2 += gridDim.x * blockDim.x) {
     int l = col_l(L, 0, h2);
     TV zup = (TV)0;
@@ -169,9 +170,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__r
     }
   }
   ss = bsum(ss, lds);
-  if (threadIdx.x == 0) part[P_SS_R * nblk + blockIdx.x] = ss;
+  if (threadIdx.x == 0) part[P_SS_R * nblk + lb] = ss;
   md = bmax(md, mlds);
-  if (threadIdx.x == 0) mpart[M_D_R * nblk + blockIdx.x] = md;
+  if (threadIdx.x == 0) mpart[M_D_R * nblk + lb] = md;
 }
 
 // Black columns: forward sweep (red neighbours + cell above) written to z,
@@ -187,6 +188,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
                                                        TV *__restrict__ z, double *part,
                                                        MaxLoc *mpart, const KState *st,
                                                        const FState *fs, int nblk) {
+  const int lb = xcd_block();
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
@@ -194,7 +196,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
   const TV alpha = (TV)fs->apend_d;
   double ss = 0.0, rho = 0.0;
   MaxLoc md{0.0, -1};
-  for (int h2 = blockIdx.x * blockDim.x + threadIdx.x; h2 < L.HP1;
+  for (int h2 = lb * blockDim.x + threadIdx.x; h2 < L.HP1;
        h2 += gridDim.x * blockDim.x) {
     int l = col_l(L, 1, h2);
     int i = l / L.ncol, j = l - i * L.ncol;
@@ -235,11 +237,11 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
     }
   }
   ss = bsum(ss, lds);
-  if (threadIdx.x == 0) part[P_SS_B * nblk + blockIdx.x] = ss;
+  if (threadIdx.x == 0) part[P_SS_B * nblk + lb] = ss;
   rho = bsum(rho, lds);
-  if (threadIdx.x == 0) part[P_RHO_B * nblk + blockIdx.x] = rho;
+  if (threadIdx.x == 0) part[P_RHO_B * nblk + lb] = rho;
   md = bmax(md, mlds);
-  if (threadIdx.x == 0) mpart[M_D_B * nblk + blockIdx.x] = md;
+  if (threadIdx.x == 0) mpart[M_D_B * nblk + lb] = md;
 }
 
 template <typename TM, typename TV, typename TP>
@@ -249,11 +251,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
                                                          const TV *__restrict__ zf,
                                                          TV *__restrict__ z, double *part,
                                                          const KState *st, int nblk) {
+  const int lb = xcd_block();
   __shared__ double lds[8];
   if (st->stop) return;
   const int NP = L.NP;
   double rho = 0.0;
-  for (int h2 = blockIdx.x * blockDim.x + threadIdx.x; h2 < L.HP0;
+  for (int h2 = lb * blockDim.x + threadIdx.x; h2 < L.HP0;
        h2 += gridDim.x * blockDim.x) {
     int l = col_l(L, 0, h2);
     int i = l / L.ncol, j = l - i * L.ncol;
@@ -278,7 +281,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
     }
   }
   rho = bsum(rho, lds);
-  if (threadIdx.x == 0) part[P_RHO_R * nblk + blockIdx.x] = rho;
+  if (threadIdx.x == 0) part[P_RHO_R * nblk + lb] = rho;
 }
 
 // ---------------------------------------------------------------------------
@@ -293,6 +296,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
                                                          TV *__restrict__ pout, TX *__restrict__ x,
                                                          MaxLoc *mpart, const KState *st,
                                                          const FState *fs, int nblk) {
+  const int lb = xcd_block();
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
   const bool first = st->iter == 0;
@@ -300,7 +304,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
   const TV ax = (TV)fs->apend_x;
   MaxLoc mp{0.0, -1};
   const int nthreads = 2 * L.HP0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nthreads;
+  for (int t = lb * blockDim.x + threadIdx.x; t < nthreads;
        t += gridDim.x * blockDim.x) {
     int c = t & 1;
     int h2 = t >> 1;
@@ -322,7 +326,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
     }
   }
   mp = bmax(mp, mlds);
-  if (threadIdx.x == 0) mpart[M_P * nblk + blockIdx.x] = mp;
+  if (threadIdx.x == 0) mpart[M_P * nblk + lb] = mp;
 }
 
 // q = A p (amux order, symmetric planes), partial p.q
@@ -331,6 +335,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
                                                       const TV *__restrict__ p,
                                                       TV *__restrict__ q, double *part,
                                                       const KState *st, int nblk) {
+  const int lb = xcd_block();
   __shared__ double lds[8];
   if (st->stop) return;
   const int NP = L.NP;
@@ -339,7 +344,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
   // so the neighbour values one colour gathers from the other are the ones
   // its partner thread streams
   const int nthreads = 2 * L.HP0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nthreads;
+  for (int t = lb * blockDim.x + threadIdx.x; t < nthreads;
        t += gridDim.x * blockDim.x) {
     int c = t & 1;
     int h2 = t >> 1;
@@ -376,7 +381,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
     }
   }
   s = bsum(s, lds);
-  if (threadIdx.x == 0) part[P_PQ * nblk + blockIdx.x] = s;
+  if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
 }
 
 // x += apend_x p (the last iteration's head update), then no update pending
@@ -415,17 +420,44 @@ __device__ inline void testcnvg_(int icnvgopt, int *icnvg, int iiter, double dvm
 // kernels just produced, then rho / beta of the next iteration.
 __global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
                           int nblk, double *hist, int *hloc) {
-  __shared__ double lds[8];
-  __shared__ MaxLoc mlds[8];
+  __shared__ double lds[4][MF6_BLOCK / 64];
+  __shared__ MaxLoc mlds[2][MF6_BLOCK / 64];
   if (st->stop) return;
-  double ssr = rsum(part + P_SS_R * nblk, nblk, lds);
-  double ssb = rsum(part + P_SS_B * nblk, nblk, lds);
-  double rb = rsum(part + P_RHO_B * nblk, nblk, lds);
-  double rr = rsum(part + P_RHO_R * nblk, nblk, lds);
-  MaxLoc mdr = rmaxl(mpart + M_D_R * nblk, nblk, mlds);
-  MaxLoc mdb = rmaxl(mpart + M_D_B * nblk, nblk, mlds);
+  // the six reductions in one pass over the partials (independent loads in
+  // flight together); per slot the summation order is that of rsum/rmaxl
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  MaxLoc mm[2] = {{0.0, -1}, {0.0, -1}};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    a[0] = a[0] + part[P_SS_R * nblk + i];
+    a[1] = a[1] + part[P_SS_B * nblk + i];
+    a[2] = a[2] + part[P_RHO_B * nblk + i];
+    a[3] = a[3] + part[P_RHO_R * nblk + i];
+    mm[0] = maxloc_pick(mm[0], mpart[M_D_R * nblk + i]);
+    mm[1] = maxloc_pick(mm[1], mpart[M_D_B * nblk + i]);
+  }
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    a[q] = wsum(a[q]);
+    if (ln == 0) lds[q][w] = a[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    mm[q] = wmax(mm[q]);
+    if (ln == 0) mlds[q][w] = mm[q];
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  MaxLoc md = maxloc_pick(mdr, mdb);
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
+  MaxLoc mt[2] = {{0.0, -1}, {0.0, -1}};
+  for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) tot[q] = tot[q] + lds[q][v];
+    mt[0] = maxloc_pick(mt[0], mlds[0][v]);
+    mt[1] = maxloc_pick(mt[1], mlds[1][v]);
+  }
+  const double ssr = tot[0], ssb = tot[1], rb = tot[2], rr = tot[3];
+  MaxLoc md = maxloc_pick(mt[0], mt[1]);
   double rho_new = rb + rr;
   int it = st->iter;   // iteration whose residual was just formed (0: initial)
   fs->apend_d = 0.0;
@@ -464,12 +496,27 @@ __global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxL
 
 __global__ void k_f_alpha(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
                           int nblk) {
-  __shared__ double lds[8];
-  __shared__ MaxLoc mlds[8];
+  __shared__ double lds[MF6_BLOCK / 64];
+  __shared__ MaxLoc mlds[MF6_BLOCK / 64];
   if (st->stop) return;
-  double pq = rsum(part + P_PQ * nblk, nblk, lds);
-  MaxLoc mp = rmaxl(mpart + M_P * nblk, nblk, mlds);
+  double a = 0.0;
+  MaxLoc mm{0.0, -1};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    a = a + part[P_PQ * nblk + i];
+    mm = maxloc_pick(mm, mpart[M_P * nblk + i]);
+  }
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  a = wsum(a);
+  mm = wmax(mm);
+  if (ln == 0) { lds[w] = a; mlds[w] = mm; }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  double pq = 0.0;
+  MaxLoc mp{0.0, -1};
+  for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
+    pq = pq + lds[v];
+    mp = maxloc_pick(mp, mlds[v]);
+  }
   double den = pq;
   den = den + copysign(DPREC_F, den);
   st->pq = pq;
@@ -499,11 +546,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_r_residual(Layout L, const double
                                                           const double *__restrict__ b,
                                                           float *__restrict__ d32,
                                                           MaxLoc *mpart) {
+  const int lb = xcd_block();
   __shared__ MaxLoc mlds[8];
   const int NP = L.NP;
   MaxLoc m{0.0, -1};
   const int ncolumns = L.HP0 + L.HP1;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncolumns;
+  for (int t = lb * blockDim.x + threadIdx.x; t < ncolumns;
        t += gridDim.x * blockDim.x) {
     int c = t >= L.HP0;
     int h2 = t - c * L.HP0;
@@ -532,7 +580,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_r_residual(Layout L, const double
     }
   }
   m = bmax(m, mlds);
-  if (threadIdx.x == 0) mpart[blockIdx.x] = m;
+  if (threadIdx.x == 0) mpart[lb] = m;
 }
 
 // after a replacement: no pending d / x update; a pending FP32 test result

@@ -134,4 +134,16 @@ struct KState {
 };
 
 #define MF6_BLOCK 256
-#define MF6_MAXBLK 2048
+#define MF6_MAXBLK 8192
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b mod 8), and each XCD has its own L2.  Stencil kernels read the
+// columns one grid row away (+-ncol/2 positions, a few workgroups away), so
+// they map workgroups to work in XCD-contiguous ranges: XCD x takes the
+// logical blocks [x*q + min(x, r), ...) of a grid of G = 8q + r workgroups.
+// Any G gives a bijection of [0, G).
+__device__ inline int xcd_block() {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b & 7, m = b >> 3, q = G >> 3, r = G & 7;
+  return x * q + (x < r ? x : r) + m;
+}

@@ -158,6 +158,7 @@ struct GwfDev {
   const double *top, *bot, *area, *k11, *k33, *ss, *sy, *delr, *delc, *condsat;
   const int *icelltype, *iconvert, *ibound;
   int inewton, ivarcv, idewatcv, icellavg, has_sto;
+  int fn;   // add the Newton terms (npf_fn, sto_fn): gwf_fc inwtflag
   double satomega;
 };
 
@@ -380,7 +381,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, in
       diag = diag + bd.hcof[pk][idx];
     }
     // --- npf_fn
-    if (g.inewton) {
+    if (g.fn) {
 #pragma unroll
       for (int s = 0; s < 6; s++) {
         if (!pres[s]) continue;
@@ -546,11 +547,170 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_dxmax(Layout L, const int *__rest
   }
 }
 
-__global__ void k_maxloc_final(const MaxLoc *mpart, int nb, MaxLoc *out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one block: the block partials' maximum.  maxloc_pick orders candidates by
+// |v|, then by the smaller location, so the result does not depend on the
+// reduction order (the reference's first-occurrence rule).
+__global__ void __launch_bounds__(MF6_BLOCK) k_maxloc_final(const MaxLoc *mpart, int nb,
+                                                            MaxLoc *out) {
+  __shared__ MaxLoc mlds[MF6_BLOCK / 64];
+  MaxLoc m{0.0, -1};
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) m = maxloc_pick(m, mpart[t]);
+  for (int o = 32; o > 0; o >>= 1) {
+    MaxLoc b2;
+    b2.v = __shfl_xor(m.v, o);
+    b2.loc = __shfl_xor(m.loc, o);
+    m = maxloc_pick(m, b2);
+  }
+  if ((threadIdx.x & 63) == 0) mlds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   MaxLoc r{0.0, -1};
-  for (int t = 0; t < nb; t++) r = maxloc_pick(r, mpart[t]);
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = maxloc_pick(r, mlds[w]);
   *out = r;
+}
+
+// r = A x - b of the current system (amux slot order), inactive rows zero
+// (sln_calc_residual): per-block partials of sum r^2, of max |r| / cell
+// volume over ibound >= 1 (gwf_ptc) and of sum b^2 over active rows (the
+// bnorm of DEV_PTC_OPTION).
+__global__ void __launch_bounds__(MF6_BLOCK) k_resid_norms(
+    Layout L, const double *__restrict__ A, const double *__restrict__ x,
+    const double *__restrict__ b, const int *__restrict__ ibound,
+    const double *__restrict__ area, const double *__restrict__ top,
+    const double *__restrict__ bot, double *part, int nblk) {
+  __shared__ double lds[3][MF6_BLOCK / 64];
+  const int NP = L.NP;
+  double ss = 0.0, bb = 0.0, mx = 0.0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < NP; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    if (ibound[p] < 1) continue;
+    const int ob = (1 - c) * L.H + k * L.HP;
+    double t = 0.0;
+    t = t + A[p] * x[p];
+    if (k > 0) t = t + A[NP + p] * x[p - L.HP];
+    if (i > 0) t = t + A[2 * NP + p] * x[ob + hn(l, -L.ncol)];
+    if (j > 0) t = t + A[3 * NP + p] * x[ob + hn(l, -1)];
+    if (j < L.ncol - 1) t = t + A[4 * NP + p] * x[ob + hn(l, 1)];
+    if (i < L.nrow - 1) t = t + A[5 * NP + p] * x[ob + hn(l, L.ncol)];
+    if (k < L.nlay - 1) t = t + A[6 * NP + p] * x[p + L.HP];
+    double r = t + (-1.0) * b[p];
+    ss = ss + r * r;
+    bb = bb + b[p] * b[p];
+    double rv = fabs(r) / (area[p] * (top[p] - bot[p]));
+    if (rv > mx) mx = rv;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ss = ss + __shfl_xor(ss, o);
+    bb = bb + __shfl_xor(bb, o);
+    mx = fmax(mx, __shfl_xor(mx, o));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { lds[0][w] = ss; lds[1][w] = bb; lds[2][w] = mx; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
+    a0 = a0 + lds[0][v];
+    a1 = a1 + lds[1][v];
+    a2 = fmax(a2, lds[2][v]);
+  }
+  part[blockIdx.x] = a0;
+  part[nblk + blockIdx.x] = a1;
+  part[2 * nblk + blockIdx.x] = a2;
+}
+
+__global__ void __launch_bounds__(MF6_BLOCK) k_resid_final(const double *part, int nblk,
+                                                           double *out) {
+  __shared__ double lds[3][MF6_BLOCK / 64];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int t = threadIdx.x; t < nblk; t += blockDim.x) {
+    a0 = a0 + part[t];
+    a1 = a1 + part[nblk + t];
+    a2 = fmax(a2, part[2 * nblk + t]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 = a0 + __shfl_xor(a0, o);
+    a1 = a1 + __shfl_xor(a1, o);
+    a2 = fmax(a2, __shfl_xor(a2, o));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { lds[0][w] = a0; lds[1][w] = a1; lds[2][w] = a2; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
+    s0 = s0 + lds[0][v];
+    s1 = s1 + lds[1][v];
+    s2 = fmax(s2, lds[2][v]);
+  }
+  out[0] = sqrt(s0);
+  out[1] = sqrt(s1);
+  out[2] = s2;
+}
+
+// PTC terms of sln_ls (NumericalSolution.f90:2548-2558): diag -= ptcval,
+// rhs -= ptcval x on active rows
+__global__ void k_ptc_apply(Layout L, const int *__restrict__ ibound, double ptcval,
+                            double *__restrict__ A, double *__restrict__ b,
+                            const double *__restrict__ x) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    if (ibound[p] < 1) continue;
+    A[p] = A[p] + (-ptcval);
+    b[p] = b[p] - ptcval * x[p];
+  }
+}
+
+// apply_backtracking (NumericalSolution.f90:2830-2843)
+__global__ void k_bt_apply(Layout L, const int *__restrict__ ibound, double breduc,
+                           double *__restrict__ x, const double *__restrict__ xtemp) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    if (ibound[p] < 1) continue;
+    double delx = breduc * (x[p] - xtemp[p]);
+    x[p] = xtemp[p] + delx;
+  }
+}
+
+// sln_underrelax SIMPLE (gamma) and COOLEY (relax) (:3008-3061)
+__global__ void k_ur_relax(Layout L, const int *__restrict__ ibound, double relax,
+                           double *__restrict__ x, const double *__restrict__ xtemp,
+                           double *__restrict__ dxold) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    if (ibound[p] < 1) continue;
+    double delx = x[p] - xtemp[p];
+    dxold[p] = delx;
+    x[p] = xtemp[p] + relax * delx;
+  }
+}
+
+// sln_underrelax delta-bar-delta (:3064-3111)
+__global__ void k_ur_dbd(Layout L, const int *__restrict__ ibound, int kiter, double theta,
+                         double akappa, double gamma, double amomentum,
+                         double *__restrict__ x, const double *__restrict__ xtemp,
+                         double *__restrict__ dxold, double *__restrict__ wsave,
+                         double *__restrict__ hchold, double *__restrict__ deold) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    if (ibound[p] < 1) continue;
+    double delx = x[p] - xtemp[p];
+    if (kiter == 1) {
+      wsave[p] = 1.0;
+      hchold[p] = 1.0e-20;
+      deold[p] = 0.0;
+    }
+    double ww = wsave[p];
+    if (deold[p] * delx < 0.0) ww = theta * wsave[p];
+    else ww = wsave[p] + akappa;
+    if (ww > 1.0) ww = 1.0;
+    wsave[p] = ww;
+    if (kiter == 1) hchold[p] = delx;
+    else hchold[p] = (1.0 - gamma) * delx + gamma * hchold[p];
+    deold[p] = delx;
+    dxold[p] = delx;
+    double amom = 0.0;
+    if (kiter > 4) amom = amomentum;
+    delx = delx * ww + amom * hchold[p];
+    x[p] = xtemp[p] + delx;
+  }
 }
 
 // sln_calcdx and npf_nur for one model
@@ -591,6 +751,7 @@ static GwfDev make_dev(Mf6Handle *h) {
   g.inewton = h->m.inewton; g.ivarcv = h->m.ivarcv; g.idewatcv = h->m.idewatcv;
   g.icellavg = h->m.icellavg; g.has_sto = h->m.has_sto;
   g.satomega = h->m.inewton ? 1.0e-6 : 0.0;   // gwf-npf.f90:1394-1396
+  g.fn = h->m.inewton;
   return g;
 }
 
@@ -789,10 +950,12 @@ extern "C" int mf6hip_gwf_advance(void *handle, int iss, double delt) {
   return MF6HIP_OK;
 }
 
-extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, double *amat_out, double *rhs_out) {
+extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double *amat_out,
+                                    double *rhs_out) {
   (void)kiter;
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  if (inewton != 0 && inewton != 1) { mf6hip_set_error("inewton must be 0 or 1"); return MF6HIP_EINVAL; }
   const Layout &L = h->L;
   hipStream_t s = h->stream;
   if (h->bnd_dirty) CK(rebuild_bnd_table(h));
@@ -811,6 +974,7 @@ extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, double *amat_out, d
     bd.brhs[t] = h->pk[t].brhs;
   }
   GwfDev g = make_dev(h);
+  g.fn = h->m.inewton && inewton;
   k_formulate<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, bd, h->iss, h->delt, h->x, h->xold, h->A,
                                                h->b);
   HIPCHK(hipGetLastError());
@@ -825,7 +989,7 @@ int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa, const 
   int nb = ggrid(L.N);
   k_dxmax<<<nb, MF6_BLOCK, 0, s>>>(L, h->ibound, xa, xb, h->d_mpart);
   MaxLoc *out = (MaxLoc *)(h->d_part + 7 * MF6_MAXBLK);
-  k_maxloc_final<<<1, 64, 0, s>>>(h->d_mpart, nb, out);
+  k_maxloc_final<<<1, MF6_BLOCK, 0, s>>>(h->d_mpart, nb, out);
   MaxLoc r;
   HIPCHK(hipMemcpyAsync(&r, out, sizeof(MaxLoc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -834,8 +998,95 @@ int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa, const 
   return MF6HIP_OK;
 }
 
-extern "C" int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, int *icnvg,
-                                       int *in_iter, double *hncg, int *lrch,
+// r = A x - b norms of the current device system: out[0] = ||r||_2 (active
+// rows), out[1] = ||b||_2 (active rows), out[2] = max |r| / volume
+static int resid_norms(Mf6Handle *h, double out[3]) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  const int nb = ggrid(L.NP);
+  double *dout = h->d_part + 7 * MF6_MAXBLK;
+  k_resid_norms<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->ibound, h->area, h->top, h->bot,
+                                         h->d_part, nb);
+  k_resid_final<<<1, MF6_BLOCK, 0, s>>>(h->d_part, nb, dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_residual_norms(void *handle, double *l2norm, double *rvmax) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model || !l2norm || !rvmax) {
+    mf6hip_set_error("null argument or no model");
+    return MF6HIP_EINVAL;
+  }
+  double o[3];
+  CK(resid_norms(h, o));
+  *l2norm = o[0];
+  *rvmax = o[2];
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_backtrack(void *handle, double breduc, double dvclose, int *btflag) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model || !btflag) {
+    mf6hip_set_error("null argument or no model");
+    return MF6HIP_EINVAL;
+  }
+  double dmax;
+  int loc;
+  CK(mf6hip_dxmax(h, &dmax, &loc, h->x, h->xtemp, true));
+  *btflag = breduc * fabs(dmax) >= dvclose ? 1 : 0;
+  if (*btflag) {
+    const Layout &L = h->L;
+    k_bt_apply<<<ggrid(L.NP), MF6_BLOCK, 0, h->stream>>>(L, h->ibound, breduc, h->x, h->xtemp);
+    HIPCHK(hipGetLastError());
+  }
+  return MF6HIP_OK;
+}
+
+// the PTC block of sln_ls (NumericalSolution.f90:2480-2573) on the fixed system
+static int apply_ptc(Mf6Handle *h, int kiter, mf6hip_ptc *pt) {
+  int iallowptc = pt->iallowptc < 0 ? (pt->kper > 1 ? 1 : 0) : pt->iallowptc;
+  int iptct = pt->iptc * iallowptc;
+  double o[3] = {0.0, 0.0, 0.0};
+  double l2norm = 0.0;
+  if (iptct != 0) {
+    CK(resid_norms(h, o));
+    l2norm = o[0];
+    if (kiter == 1) {
+      if (pt->kper > 1 || pt->kstp > 1) {
+        if (l2norm <= pt->l2norm0) pt->iptc = 0;
+      }
+    } else {
+      double a = l2norm, b = pt->l2norm0;   // is_close, MathUtil.f90:45-86
+      bool same = a == b ||
+                  fabs(a - b) <= fmax(100.0 * 2.220446049250313080847263336181640625e-16 *
+                                          fmax(fabs(a), fabs(b)), 0.0);
+      if (same) pt->iptc = 0;
+    }
+  }
+  iptct = pt->iptc * iallowptc;
+  pt->iptc = iptct;   // out: 1 when the PTC terms are added
+  if (iptct == 0) return MF6HIP_OK;
+  if (kiter == 1) {
+    if (pt->ptcdel0 > 0.0) pt->ptcdel = pt->ptcdel0;
+    else if (pt->iptcopt == 0) pt->ptcdel = 1.0 / pt->ptcf;
+    else pt->ptcdel = o[1] / l2norm;
+  } else {
+    if (l2norm > 0.0) pt->ptcdel = pt->ptcdel * pow(pt->l2norm0 / l2norm, pt->ptcexp);
+    else pt->ptcdel = 0.0;
+  }
+  double ptcval = pt->ptcdel > 0.0 ? 1.0 / pt->ptcdel : 1.0;
+  const Layout &L = h->L;
+  k_ptc_apply<<<ggrid(L.NP), MF6_BLOCK, 0, h->stream>>>(L, h->ibound, ptcval, h->A, h->b, h->x);
+  HIPCHK(hipGetLastError());
+  pt->l2norm0 = l2norm;
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, mf6hip_ptc *ptc,
+                                       int *icnvg, int *in_iter, double *hncg, int *lrch,
                                        mf6hip_cnvg_out *out) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !h->has_model || !icnvg || !in_iter || !hncg || !lrch) {
@@ -846,6 +1097,7 @@ extern "C" int mf6hip_gwf_linear_solve(void *handle, int kstp, int kiter, int *i
   k_row_fixes<<<ggrid(L.N), MF6_BLOCK, 0, h->stream>>>(L, h->ibound, h->s.ilinmeth == 1, h->A,
                                                         h->b, h->x, h->xtemp);
   HIPCHK(hipGetLastError());
+  if (ptc) CK(apply_ptc(h, kiter, ptc));
   CK(mf6hip_device_solve(h, kstp, kiter, icnvg, in_iter, out));
   CK(mf6hip_dxmax(h, hncg, lrch, h->x, h->xtemp, true));
   return MF6HIP_OK;
@@ -870,9 +1122,9 @@ static int maxval_host(Mf6Handle *h, double *vmax) {
   return MF6HIP_OK;
 }
 
-extern "C" int mf6hip_gwf_outer_update(void *handle, int kiter, int *inewtonur,
-                                       double *dxold_max, double *hncg, int *lrch) {
-  (void)kiter;
+extern "C" int mf6hip_gwf_outer_update(void *handle, int kiter, mf6hip_underrelax *ur,
+                                       int *inewtonur, double *dxold_max, double *hncg,
+                                       int *lrch) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !h->has_model || !inewtonur || !dxold_max || !hncg || !lrch) {
     mf6hip_set_error("null argument or no model");
@@ -880,7 +1132,46 @@ extern "C" int mf6hip_gwf_outer_update(void *handle, int kiter, int *inewtonur,
   }
   const Layout &L = h->L;
   hipStream_t s = h->stream;
-  k_calcdx<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(L, h->ibound, h->x, h->xtemp, h->dxold);
+  const int nonmeth = ur ? ur->nonmeth : 0;
+  if (nonmeth < 0 || nonmeth > 3) { mf6hip_set_error("nonmeth must be 0..3"); return MF6HIP_EINVAL; }
+  if (nonmeth == 1) {
+    k_ur_relax<<<ggrid(L.NP), MF6_BLOCK, 0, s>>>(L, h->ibound, ur->gamma, h->x, h->xtemp,
+                                                 h->dxold);
+  } else if (nonmeth == 2) {
+    // Cooley (NumericalSolution.f90:3023-3061): scalar recurrence on the host
+    double relax;
+    double bigch = ur->bigch;
+    if (kiter == 1) {
+      relax = 1.0;
+      ur->relaxold = 1.0;
+      ur->bigchold = bigch;
+    } else {
+      double es = bigch / (ur->bigchold * ur->relaxold);
+      double aes = fabs(es);
+      if (es < -1.0) relax = 0.5 / aes;
+      else relax = (3.0 + es) / (3.0 + aes);
+    }
+    ur->relaxold = relax;
+    ur->bigchold = (1.0 - ur->gamma) * bigch + ur->gamma * ur->bigchold;
+    if (relax < 1.0)
+      k_ur_relax<<<ggrid(L.NP), MF6_BLOCK, 0, s>>>(L, h->ibound, relax, h->x, h->xtemp,
+                                                   h->dxold);
+  } else if (nonmeth == 3) {
+    if (!h->wsave) {
+      CK(galloc(&h->wsave, (size_t)L.NP));
+      CK(galloc(&h->hchold, (size_t)L.NP));
+      CK(galloc(&h->deold, (size_t)L.NP));
+      HIPCHK(hipMemsetAsync(h->wsave, 0, sizeof(double) * L.NP, s));
+      HIPCHK(hipMemsetAsync(h->hchold, 0, sizeof(double) * L.NP, s));
+      HIPCHK(hipMemsetAsync(h->deold, 0, sizeof(double) * L.NP, s));
+    }
+    double theta = ur->theta < 1.0e-3 ? 1.0e-3 : ur->theta;   // sln_ar :868-870
+    k_ur_dbd<<<ggrid(L.NP), MF6_BLOCK, 0, s>>>(L, h->ibound, kiter, theta, ur->akappa,
+                                               ur->gamma, ur->amomentum, h->x, h->xtemp,
+                                               h->dxold, h->wsave, h->hchold, h->deold);
+  } else {
+    k_calcdx<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(L, h->ibound, h->x, h->xtemp, h->dxold);
+  }
   *inewtonur = 0;
   *dxold_max = 0.0;
   if (h->m.inewton && h->m.inewtonur) {

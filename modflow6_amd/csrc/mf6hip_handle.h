@@ -87,6 +87,7 @@ struct Mf6Handle {
   double *condsat = nullptr;  // 3 planes: right, front, down (per position)
   double *xold = nullptr;
   double *dxold = nullptr;
+  double *wsave = nullptr, *hchold = nullptr, *deold = nullptr;   // DBD state
   int *ibotnode = nullptr;    // position of the lowest connected cell
   int *bnd_off = nullptr;     // per position: start into bnd_ent (N+1)
   int *bnd_ent = nullptr;     // entries: (pkg << 28) | index

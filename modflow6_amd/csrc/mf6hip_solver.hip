@@ -1482,7 +1482,7 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->d_hist, h->d_hloc, h->d_tmp_nat, h->d_amat_csr, h->top, h->bot,
                   h->area, h->k11, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
-                  h->ibotnode, h->bnd_off, h->bnd_ent};
+                  h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int t = 0; t < 5; t++) {

@@ -36,7 +36,67 @@ module mf6hip_module
     integer(c_int) :: kernel_variant
   end type mf6hip_ims_settings
 
+  integer(c_int), parameter, public :: MF6HIP_PKG_WEL = 1
+  integer(c_int), parameter, public :: MF6HIP_PKG_DRN = 2
+  integer(c_int), parameter, public :: MF6HIP_PKG_RCH = 3
+  integer(c_int), parameter, public :: MF6HIP_PKG_CHD = 4
+
+  !> mf6hip_gwf_model: DIS/NPF/STO arrays of one GWF model (c_loc of the
+  !! memory-manager arrays; natural node order)
+  type, bind(C), public :: mf6hip_gwf_model
+    integer(c_int) :: nlay
+    integer(c_int) :: nrow
+    integer(c_int) :: ncol
+    type(c_ptr) :: delr
+    type(c_ptr) :: delc
+    type(c_ptr) :: top
+    type(c_ptr) :: bot
+    type(c_ptr) :: k11
+    type(c_ptr) :: k33
+    type(c_ptr) :: icelltype
+    type(c_ptr) :: ibound
+    integer(c_int) :: inewton
+    integer(c_int) :: inewtonur
+    integer(c_int) :: ivarcv
+    integer(c_int) :: idewatcv
+    integer(c_int) :: icellavg
+    integer(c_int) :: has_sto
+    type(c_ptr) :: iconvert
+    type(c_ptr) :: ss
+    type(c_ptr) :: sy
+  end type mf6hip_gwf_model
+
+  !> mf6hip_ptc: pseudo-transient continuation control and state
+  type, bind(C), public :: mf6hip_ptc
+    integer(c_int) :: iptc
+    integer(c_int) :: iallowptc
+    integer(c_int) :: iptcopt
+    integer(c_int) :: kstp
+    integer(c_int) :: kper
+    real(c_double) :: ptcf
+    real(c_double) :: ptcdel0
+    real(c_double) :: ptcexp
+    real(c_double) :: ptcdel
+    real(c_double) :: l2norm0
+  end type mf6hip_ptc
+
+  !> mf6hip_underrelax: NONLINEAR under-relaxation settings and Cooley state
+  type, bind(C), public :: mf6hip_underrelax
+    integer(c_int) :: nonmeth
+    real(c_double) :: gamma
+    real(c_double) :: theta
+    real(c_double) :: akappa
+    real(c_double) :: amomentum
+    real(c_double) :: bigch
+    real(c_double) :: bigchold
+    real(c_double) :: relaxold
+  end type mf6hip_underrelax
+
   public :: mf6hip_create, mf6hip_solve, mf6hip_destroy, mf6hip_last_error
+  public :: mf6hip_gwf_create, mf6hip_gwf_set_package, mf6hip_gwf_set_x, &
+            mf6hip_gwf_get_x, mf6hip_gwf_advance, mf6hip_gwf_formulate, &
+            mf6hip_gwf_residual_norms, mf6hip_gwf_backtrack, &
+            mf6hip_gwf_linear_solve, mf6hip_gwf_outer_update
 
   interface
     function mf6hip_create(neq, nja, ia, ja, settings, handle) result(rc) &
@@ -72,6 +132,100 @@ module mf6hip_module
       import :: c_int, c_char
       character(kind=c_char), intent(out) :: buf(*)
       integer(c_int), value :: buflen
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_create(handle, model) result(rc) &
+      bind(C, name='mf6hip_gwf_create')
+      import :: c_int, c_ptr, mf6hip_gwf_model
+      type(c_ptr), value :: handle
+      type(mf6hip_gwf_model), intent(in) :: model
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_set_package(handle, pkgtype, nbound, nodelist, v1, &
+                                    v2) result(rc) &
+      bind(C, name='mf6hip_gwf_set_package')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: pkgtype, nbound
+      type(c_ptr), value :: nodelist, v1, v2
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_set_x(handle, x) result(rc) &
+      bind(C, name='mf6hip_gwf_set_x')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: x(*)
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_get_x(handle, x) result(rc) &
+      bind(C, name='mf6hip_gwf_get_x')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: x(*)
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_advance(handle, iss, delt) result(rc) &
+      bind(C, name='mf6hip_gwf_advance')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: iss
+      real(c_double), value :: delt
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_formulate(handle, kiter, inewton, amat_out, rhs_out) &
+      result(rc) bind(C, name='mf6hip_gwf_formulate')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: kiter, inewton
+      type(c_ptr), value :: amat_out, rhs_out
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_residual_norms(handle, l2norm, rvmax) result(rc) &
+      bind(C, name='mf6hip_gwf_residual_norms')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: l2norm, rvmax
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_backtrack(handle, breduc, dvclose, btflag) result(rc) &
+      bind(C, name='mf6hip_gwf_backtrack')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), value :: breduc, dvclose
+      integer(c_int), intent(out) :: btflag
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_linear_solve(handle, kstp, kiter, ptc, icnvg, in_iter, &
+                                     hncg, lrch, out) result(rc) &
+      bind(C, name='mf6hip_gwf_linear_solve')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: kstp, kiter
+      type(c_ptr), value :: ptc
+      integer(c_int), intent(out) :: icnvg, in_iter, lrch
+      real(c_double), intent(out) :: hncg
+      type(c_ptr), value :: out
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_outer_update(handle, kiter, ur, inewtonur, dxold_max, &
+                                     hncg, lrch) result(rc) &
+      bind(C, name='mf6hip_gwf_outer_update')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: kiter
+      type(c_ptr), value :: ur
+      integer(c_int), intent(out) :: inewtonur, lrch
+      real(c_double), intent(out) :: dxold_max, hncg
       integer(c_int) :: rc
     end function
   end interface

@@ -13,9 +13,13 @@ structured GWF model whose matrix fill and linear solve run on the GPU:
                           (:1287-1327)
 
 Heads stay in HBM between calls; only scalars cross to the host each outer
-iteration.  Under-relaxation (NONLINEAR UNDER_RELAXATION), backtracking and
-pseudo-transient continuation are not part of the device path and are
-rejected.
+iteration.  The NONLINEAR block runs on the device as well: backtracking
+(sln_backtracking :2680-2776, ``mf6hip_gwf_backtrack`` +
+``mf6hip_gwf_residual_norms``), pseudo-transient continuation for
+steady-state Newton models (sln_calc_ptc :2936-2962 / gwf_ptc
+gwf.f90:625-679, applied inside ``mf6hip_gwf_linear_solve`` as sln_ls
+:2480-2573 does) and SIMPLE / COOLEY / DBD under-relaxation (sln_underrelax
+:2989-3114, in ``mf6hip_gwf_outer_update``).
 """
 from __future__ import annotations
 
@@ -40,6 +44,8 @@ class OuterRecord:
     converged: bool
     nur: bool = False
     solve_ms: float = 0.0
+    nbt: int = 0
+    iptc: int = 0
 
 
 class GwfSolution:
@@ -48,9 +54,6 @@ class GwfSolution:
     def __init__(self, case, precision=_lib.MF6HIP_FP64, timing=False,
                  inner_fp32_rtol=0.0, inner_fp32_max=0, kernel_variant=0):
         self.case = c = case
-        if c.ims.under_relaxation.upper() != "NONE":
-            raise _lib.Mf6HipError("UNDER_RELAXATION is not supported on the "
-                                   "device path")
         L = _lib.load()
         self.L = L
         ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
@@ -108,30 +111,96 @@ class GwfSolution:
         if timing:
             _lib.check(L.mf6hip_set_timing(self.h, 1, 16), "mf6hip_set_timing")
         self.kstp = 0
+        self.kper = 0
+        self.iss = 1
+        self.delt = 1.0
         self.log: List[List[OuterRecord]] = []
+        ims = c.ims
+        self.ur = _lib.UnderRelaxC(ims.nonmeth, ims.under_relaxation_gamma,
+                                   ims.theta, ims.under_relaxation_kappa,
+                                   ims.under_relaxation_momentum, 0.0, 0.0, 0.0)
+        self.ptc = _lib.PtcC(0, ims.iallowptc, ims.ptc_option, 0, 0, 0.0,
+                             ims.ptc_del0, ims.ptc_exponent, 0.0, 0.0)
+        self.res_prev = 0.0
+        self.res_new = 0.0
 
     # gwf_ad + chd_ad
     def prepare_timestep(self, delt, steady):
-        _lib.check(self.L.mf6hip_gwf_advance(self.h, 1 if steady else 0,
-                                             float(delt)),
+        self.iss = 1 if steady else 0
+        self.delt = float(delt)
+        _lib.check(self.L.mf6hip_gwf_advance(self.h, self.iss, self.delt),
                    "mf6hip_gwf_advance")
 
-    def formulate(self, kiter, want_system=False):
+    def formulate(self, kiter, want_system=False, inewton=1):
         amat = rhs = None
         if want_system:
             amat = np.zeros(self.solver.nja)
             rhs = np.zeros(self.solver.neq)
-        _lib.check(self.L.mf6hip_gwf_formulate(self.h, kiter, _lib.dptr(amat),
-                                               _lib.dptr(rhs)),
+        _lib.check(self.L.mf6hip_gwf_formulate(self.h, kiter, inewton,
+                                               _lib.dptr(amat), _lib.dptr(rhs)),
                    "mf6hip_gwf_formulate")
         return amat, rhs
 
-    def linear_solve(self, kiter, record=None):
+    def residual_norms(self):
+        l2, rv = C.c_double(0.0), C.c_double(0.0)
+        _lib.check(self.L.mf6hip_gwf_residual_norms(self.h, C.byref(l2),
+                                                    C.byref(rv)),
+                   "mf6hip_gwf_residual_norms")
+        return l2.value, rv.value
+
+    # sln_backtracking (NumericalSolution.f90:2680-2776)
+    def backtracking(self, kiter):
+        ims = self.case.ims
+        self.formulate(kiter, inewton=0)
+        if kiter == 1:
+            self.res_prev = self.residual_norms()[0]
+        else:
+            self.res_new = self.residual_norms()[0]
+        nbt = 0
+        if kiter > 1:
+            if self.res_new > self.res_prev * ims.backtracking_tolerance:
+                for nb in range(1, ims.backtracking_number + 1):
+                    flag = C.c_int(0)
+                    _lib.check(self.L.mf6hip_gwf_backtrack(
+                        self.h, ims.backtracking_reduction_factor,
+                        ims.outer_dvclose, C.byref(flag)), "mf6hip_gwf_backtrack")
+                    if flag.value == 0:
+                        break
+                    nbt = nb
+                    self.formulate(kiter, inewton=0)
+                    self.res_new = self.residual_norms()[0]
+                    if nb == ims.backtracking_number:
+                        break
+                    if self.res_new < self.res_prev * ims.backtracking_tolerance:
+                        break
+                    if self.res_new < ims.backtracking_residual_limit:
+                        break
+            self.res_prev = self.res_new
+        return nbt
+
+    # sln_calc_ptc -> gwf_ptc (gwf.f90:625-679)
+    def calc_ptc(self):
+        iptc, ptcf = 0, 0.0
+        if self.iss > 0 and self.case.newton:
+            iptc = 1
+            ptcf = self.residual_norms()[1]
+            if ptcf == 0.0:
+                ptcf = 1.0 / (self.delt * 10.0)
+        return iptc, ptcf
+
+    def linear_solve(self, kiter, record=None, iptc=0, ptcf=0.0):
         icnvg, inner, lrch = C.c_int(0), C.c_int(0), C.c_int(0)
         hncg = C.c_double(0.0)
         out = record.to_c() if record is not None else None
+        ptc = None
+        if iptc:
+            self.ptc.iptc = iptc
+            self.ptc.ptcf = ptcf
+            self.ptc.kstp = self.kstp
+            self.ptc.kper = self.kper
+            ptc = C.byref(self.ptc)
         _lib.check(self.L.mf6hip_gwf_linear_solve(
-            self.h, self.kstp, kiter, C.byref(icnvg), C.byref(inner),
+            self.h, self.kstp, kiter, ptc, C.byref(icnvg), C.byref(inner),
             C.byref(hncg), C.byref(lrch),
             C.byref(out) if out is not None else None),
             "mf6hip_gwf_linear_solve")
@@ -148,18 +217,25 @@ class GwfSolution:
 
     # NumericalSolution%solve (:1482-1837)
     def solve(self, kiter, record=None):
+        ims = self.case.ims
+        nbt = 0
+        if ims.backtracking_number > 0:
+            nbt = self.backtracking(kiter)
         self.formulate(kiter)
-        icnvg, inner, hncg, lrch = self.linear_solve(kiter, record)
+        iptc, ptcf = self.calc_ptc()
+        icnvg, inner, hncg, lrch = self.linear_solve(kiter, record, iptc, ptcf)
         solve_ms = self.last_timing()[0]
-        dvclose = self.case.ims.outer_dvclose
+        dvclose = ims.outer_dvclose
         conv = abs(hncg) <= dvclose
         rec = OuterRecord(kiter, inner, icnvg, hncg, lrch, conv,
-                          solve_ms=solve_ms)
+                          solve_ms=solve_ms, nbt=nbt,
+                          iptc=self.ptc.iptc if iptc else 0)
         if not conv:
             inur, dxo, h2, l2 = C.c_int(0), C.c_double(0), C.c_double(0), C.c_int(0)
+            self.ur.bigch = hncg
             _lib.check(self.L.mf6hip_gwf_outer_update(
-                self.h, kiter, C.byref(inur), C.byref(dxo), C.byref(h2),
-                C.byref(l2)), "mf6hip_gwf_outer_update")
+                self.h, kiter, C.byref(self.ur), C.byref(inur), C.byref(dxo),
+                C.byref(h2), C.byref(l2)), "mf6hip_gwf_outer_update")
             if inur.value:
                 rec.nur = True
                 # sln_nur_has_converged (:3205-3218)
@@ -185,7 +261,9 @@ class GwfSolution:
 
     def run(self, max_steps=None):
         nsteps = 0
+        self.kper = 0
         for perlen, nstp, steady in self.case.periods:
+            self.kper += 1
             self.kstp = 0
             for _ in range(nstp):
                 self.run_timestep(perlen / nstp, steady)

@@ -1117,3 +1117,111 @@ double orc_sln_l2norm(int neq, const int *ia, const int *ja,
   for (int n = 0; n < neq; n++) s = s + r[n] * r[n];
   return sqrt(s);
 }
+
+/* NumericalSolution.f90:2800-2826 get_backtracking_flag: 1 when
+ * breduc * max|x - xtemp| over active cells >= dvclose. */
+int orc_bt_flag(int neq, const int *active, const double *x,
+                const double *xtemp, double breduc, double dvclose)
+{
+  double dmax = 0.0;
+  for (int n = 0; n < neq; n++) {
+    if (active[n] < 1) continue;
+    double a = fabs(x[n] - xtemp[n]);
+    if (a > dmax) dmax = a;
+  }
+  return breduc * dmax >= dvclose ? 1 : 0;
+}
+
+/* NumericalSolution.f90:2830-2843 apply_backtracking */
+void orc_bt_apply(int neq, const int *active, double *x, const double *xtemp,
+                  double breduc)
+{
+  for (int n = 0; n < neq; n++) {
+    if (active[n] < 1) continue;
+    double delx = breduc * (x[n] - xtemp[n]);
+    x[n] = xtemp[n] + delx;
+  }
+}
+
+/* gwf.f90:625-679 gwf_ptc on the residual r = A x - b of the formulated
+ * system (sln_calc_ptc, NumericalSolution.f90:2936-2962).  The maximum cell
+ * volume is area * (top - bot) (head at the cell top).  Returns iptc. */
+int orc_gwf_ptc(int nodes, int iss, int inewton, const int *ibound,
+                const double *area, const double *top, const double *bot,
+                const double *resid, double delt, double *ptcf)
+{
+  int iptct = (iss > 0 && inewton > 0) ? inewton : 0;
+  *ptcf = DZERO;
+  if (iptct > 0) {
+    for (int n = 0; n < nodes; n++) {
+      if (ibound[n] < 1) continue;
+      double v = area[n] * (top[n] - bot[n]);
+      double r = fabs(resid[n]) / v;
+      if (r > *ptcf) *ptcf = r;
+    }
+    if (*ptcf == DZERO) *ptcf = DONE / (delt * 10.0);
+  }
+  return iptct > 0 ? 1 : 0;
+}
+
+/* Pseudo-transient continuation state of NumericalSolutionType. */
+typedef struct {
+  int iallowptc;     /* 1 allowed, 0 NO_PTC ALL, -1 NO_PTC FIRST */
+  int iptcopt;       /* 0: ptcdel = 1/ptcf ; 1: bnorm / l2norm */
+  double ptcdel0, ptcexp;
+  double ptcdel, l2norm0;
+} orc_ptc_state;
+
+/* NumericalSolution.f90:2480-2573: the PTC block of sln_ls, on the system
+ * after the row fixes.  Returns 1 when PTC terms were added (iptc after the
+ * l2norm test times iallowptc); r is neq work. */
+int orc_sln_ptc(orc_ptc_state *st, int neq, const int *ia, const int *ja,
+                double *amat, double *rhs, const double *x,
+                const int *active, int kiter, int kstp, int kper, int iptc,
+                double ptcf, double *r)
+{
+  int iallowptc;
+  if (st->iallowptc < 0) iallowptc = kper > 1 ? 1 : 0;
+  else iallowptc = st->iallowptc;
+  int iptct = iptc * iallowptc;
+  double l2norm = 0.0;
+  if (iptct != 0) {
+    l2norm = orc_sln_l2norm(neq, ia, ja, amat, x, rhs, active, r);
+    if (kiter == 1) {
+      if (kper > 1 || kstp > 1) {
+        if (l2norm <= st->l2norm0) iptc = 0;
+      }
+    } else {
+      if (orc_is_close(l2norm, st->l2norm0)) iptc = 0;
+    }
+  }
+  iptct = iptc * iallowptc;
+  if (iptct == 0) return 0;
+  {
+    if (kiter == 1) {
+      if (st->ptcdel0 > DZERO) {
+        st->ptcdel = st->ptcdel0;
+      } else if (st->iptcopt == 0) {
+        st->ptcdel = DONE / ptcf;
+      } else {
+        double bnorm = DZERO;
+        for (int n = 0; n < neq; n++)
+          if (active[n] > 0) bnorm = bnorm + rhs[n] * rhs[n];
+        bnorm = sqrt(bnorm);
+        st->ptcdel = bnorm / l2norm;
+      }
+    } else {
+      if (l2norm > DZERO) st->ptcdel = st->ptcdel * pow(st->l2norm0 / l2norm, st->ptcexp);
+      else st->ptcdel = DZERO;
+    }
+    double ptcval = st->ptcdel > DZERO ? DONE / st->ptcdel : DONE;
+    for (int n = 0; n < neq; n++) {
+      if (active[n] > 0) {
+        amat[ia[n]] = amat[ia[n]] + (-ptcval);
+        rhs[n] = rhs[n] - ptcval * x[n];
+      }
+    }
+    st->l2norm0 = l2norm;
+  }
+  return 1;
+}

@@ -23,6 +23,21 @@ i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
+class UrState(C.Structure):
+    """orc_ur_state: under-relaxation settings and Cooley state."""
+    _fields_ = [("nonmeth", C.c_int), ("gamma", C.c_double),
+                ("theta", C.c_double), ("akappa", C.c_double),
+                ("amomentum", C.c_double), ("bigch", C.c_double),
+                ("bigchold", C.c_double), ("relaxold", C.c_double)]
+
+
+class PtcState(C.Structure):
+    """orc_ptc_state: pseudo-transient continuation settings and state."""
+    _fields_ = [("iallowptc", C.c_int), ("iptcopt", C.c_int),
+                ("ptcdel0", C.c_double), ("ptcexp", C.c_double),
+                ("ptcdel", C.c_double), ("l2norm0", C.c_double)]
+
+
 class IterRec(C.Structure):
     _fields_ = [("dvmax", C.c_double), ("rmax", C.c_double),
                 ("l2norm", C.c_double), ("alpha", C.c_double),
@@ -105,6 +120,17 @@ def lib():
         L.orc_sln_l2norm.restype = d
         L.orc_sln_l2norm.argtypes = [i, i32p, i32p, f64p, f64p, f64p, i32p,
                                      f64p]
+        L.orc_underrelax.argtypes = [C.POINTER(UrState), i, d, i, i32p, f64p,
+                                     f64p, f64p, f64p, f64p, f64p]
+        L.orc_bt_flag.restype = i
+        L.orc_bt_flag.argtypes = [i, i32p, f64p, f64p, d, d]
+        L.orc_bt_apply.argtypes = [i, i32p, f64p, f64p, d]
+        L.orc_gwf_ptc.restype = i
+        L.orc_gwf_ptc.argtypes = [i, i, i, i32p, f64p, f64p, f64p, f64p, d,
+                                  C.POINTER(d)]
+        L.orc_sln_ptc.restype = i
+        L.orc_sln_ptc.argtypes = [C.POINTER(PtcState), i, i32p, i32p, f64p,
+                                  f64p, f64p, i32p, i, i, i, i, d, f64p]
         _LIB = L
     return _LIB
 
@@ -468,7 +494,20 @@ class GwfSimulation:
         self.delt = 1.0
         self.iss = 1
         self.kstp = 0
+        self.kper = 0
         self.log = []
+        # NONLINEAR block state (NumericalSolution.f90:81-130)
+        self.ur = UrState(ims.nonmeth, ims.under_relaxation_gamma, ims.theta,
+                          ims.under_relaxation_kappa,
+                          ims.under_relaxation_momentum, 0.0, 0.0, 0.0)
+        self.wsave = np.zeros(self.neq)
+        self.hchold = np.zeros(self.neq)
+        self.deold = np.zeros(self.neq)
+        self.res_prev = 0.0
+        self.res_new = 0.0
+        self.ptc = PtcState(ims.iallowptc, ims.ptc_option, ims.ptc_del0,
+                            ims.ptc_exponent, 0.0, 0.0)
+        self.work = np.zeros(self.neq)
 
     def _ibotnode(self):
         # gwf-npf.f90:1905-1939: lowest vertically connected node
@@ -504,8 +543,9 @@ class GwfSimulation:
                 self.xold[nd] = p.v1
                 self.ibound[nd] = -1
 
-    # -- sln_buildsystem (NumericalSolution.f90:1941-1991)
-    def formulate(self, kiter):
+    # -- sln_buildsystem (NumericalSolution.f90:1941-1991); inewton = 0
+    #    leaves out the Newton terms (gwf_fc inwtflag, gwf.f90:466-555)
+    def formulate(self, kiter, inewton=1):
         L = lib()
         d = self.dis
         self.amat[:] = 0.0
@@ -543,7 +583,7 @@ class GwfSimulation:
             nd = self.pk_nodes[t]
             L.orc_bnd_fc(nd.size, nd, self.pk_hcof[t], self.pk_rhs[t], d.ia,
                          self.amat, self.rhs)
-        if self.inewton:
+        if self.inewton and inewton:
             L.orc_npf_fn(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc,
                          self.condsat, self.ibound, self.icelltype, 0, d.top,
                          d.bot, self.satomega, self.x, self.amat, self.rhs)
@@ -552,14 +592,72 @@ class GwfSimulation:
                              d.top, d.bot, d.area, self.ss, self.sy,
                              self.delt, self.x, self.amat, self.rhs)
 
-    # -- NumericalSolution%solve (:1482-1837), no backtracking / PTC
+    # -- sln_l2norm (:2855-2871) of the current system
+    def l2norm(self):
+        d = self.dis
+        return lib().orc_sln_l2norm(self.neq, d.ia, d.ja, self.amat, self.x,
+                                    self.rhs, self.ibound, self.work)
+
+    # -- sln_backtracking (:2680-2776)
+    def backtracking(self, kiter):
+        L = lib()
+        ims = self.case.ims
+        self.formulate(kiter, inewton=0)
+        if kiter == 1:
+            self.res_prev = self.l2norm()
+        else:
+            self.res_new = self.l2norm()
+        nbt = 0
+        if kiter > 1:
+            if self.res_new > self.res_prev * ims.backtracking_tolerance:
+                for nb in range(1, ims.backtracking_number + 1):
+                    # sln_backtracking_xupdate (:2784-2796)
+                    btflag = L.orc_bt_flag(self.neq, self.ibound, self.x,
+                                           self.xtemp,
+                                           ims.backtracking_reduction_factor,
+                                           ims.outer_dvclose)
+                    if btflag == 0:
+                        break
+                    L.orc_bt_apply(self.neq, self.ibound, self.x, self.xtemp,
+                                   ims.backtracking_reduction_factor)
+                    nbt = nb
+                    self.formulate(kiter, inewton=0)
+                    self.res_new = self.l2norm()
+                    if nb == ims.backtracking_number:
+                        break
+                    if self.res_new < self.res_prev * ims.backtracking_tolerance:
+                        break
+                    if self.res_new < ims.backtracking_residual_limit:
+                        break
+            self.res_prev = self.res_new
+        return nbt
+
+    # -- sln_calc_ptc (:2936-2962) -> gwf_ptc (gwf.f90:625-679)
+    def calc_ptc(self):
+        d = self.dis
+        self.l2norm()        # fills self.work with r = A x - b (active)
+        ptcf = C.c_double(0.0)
+        iptc = lib().orc_gwf_ptc(self.neq, self.iss, self.inewton,
+                                 self.ibound, d.area, d.top, d.bot,
+                                 self.work, self.delt, C.byref(ptcf))
+        return iptc, ptcf.value
+
+    # -- NumericalSolution%solve (:1482-1837)
     def solve_outer(self, kiter, history=False):
         L = lib()
         d = self.dis
+        ims = self.case.ims
+        nbt = 0
+        if ims.backtracking_number > 0:
+            nbt = self.backtracking(kiter)
         self.formulate(kiter)
+        iptc, ptcf = self.calc_ptc()
         L.orc_sln_ls_fixes(self.neq, d.ia, d.ja, self.ibound,
                            self.isymmetric, self.amat, self.rhs, self.x,
                            self.xtemp)
+        iptc = L.orc_sln_ptc(C.byref(self.ptc), self.neq, d.ia, d.ja,
+                             self.amat, self.rhs, self.x, self.ibound, kiter,
+                             self.kstp, self.kper, iptc, ptcf, self.work)
         icnvg, inner, l2n0, stop, recs = self.imslinear.apply(
             self.amat, self.rhs, self.x, kstp=self.kstp, history=history)
         hncg = C.c_double(0.0)
@@ -567,9 +665,16 @@ class GwfSimulation:
                                C.byref(hncg))
         conv = abs(hncg.value) <= self.case.ims.outer_dvclose
         rec = dict(kiter=kiter, inner=inner, icnvg_inner=icnvg,
-                   hncg=hncg.value, lrch=lrch, l2norm0=l2n0, hist=recs)
+                   hncg=hncg.value, lrch=lrch, l2norm0=l2n0, hist=recs,
+                   iptc=iptc, nbt=nbt)
         if not conv:
-            L.orc_calcdx(self.neq, self.ibound, self.x, self.xtemp, self.dxold)
+            if ims.nonmeth > 0:
+                L.orc_underrelax(C.byref(self.ur), kiter, hncg.value, self.neq,
+                                 self.ibound, self.x, self.xtemp, self.dxold,
+                                 self.wsave, self.hchold, self.deold)
+            else:
+                L.orc_calcdx(self.neq, self.ibound, self.x, self.xtemp,
+                             self.dxold)
             if self.case.newton_ur:
                 dxmax = C.c_double(0.0)
                 locmax = C.c_int(0)
@@ -604,7 +709,9 @@ class GwfSimulation:
 
     def run(self, history=False, max_steps=None):
         nsteps = 0
+        self.kper = 0
         for perlen, nstp, steady in self.case.periods:
+            self.kper += 1
             self.kstp = 0
             for _ in range(nstp):
                 self.run_timestep(perlen / nstp, steady, history=history)

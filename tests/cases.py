@@ -62,10 +62,36 @@ def bump_case():
     # IMS of the reference test (SIMPLE complexity + its overrides)
     ims = ImsOptions(outer_maximum=75, outer_dvclose=1e-8, inner_maximum=100,
                      inner_dvclose=1e-9, inner_rclose=1e-3,
-                     linear_acceleration="BICGSTAB")
+                     linear_acceleration="BICGSTAB", no_ptc="ALL")
     return GwfCase(
         name="bump", nlay=1, nrow=nrow, ncol=ncol, delr=np.full(ncol, 100.0 / ncol),
         delc=np.full(nrow, 100.0 / nrow), top2d=np.full((nrow, ncol), 25.0),
         botm=botm, k11=np.ones(nodes), k33=np.ones(nodes),
         icelltype=np.ones(nodes, np.int32), strt=np.full(nodes, 7.5), packages=[chd],
         ims=ims, periods=[(1.0, 1, True)], newton=True, newton_ur=True)
+
+
+# NONLINEAR / OPTIONS variants of the bump model; every one converges to the
+# same heads (results.hds.cmp), through different outer-iteration paths
+BUMP_NONLINEAR = {
+    "ptc": dict(no_ptc=None),
+    "simple": dict(under_relaxation="SIMPLE", under_relaxation_gamma=0.7),
+    "cooley": dict(under_relaxation="COOLEY", under_relaxation_gamma=0.3),
+    "dbd": dict(under_relaxation="DBD", under_relaxation_theta=0.7,
+                under_relaxation_kappa=0.1, under_relaxation_gamma=0.2,
+                under_relaxation_momentum=0.001),
+    # the COMPLEX complexity preset (sln_setouter, NumericalSolution.f90
+    # :2656-2669) with PTC allowed
+    "complex": dict(under_relaxation="DBD", under_relaxation_theta=0.8,
+                    under_relaxation_kappa=1e-4, under_relaxation_gamma=0.0,
+                    backtracking_number=20, backtracking_tolerance=1.05,
+                    backtracking_reduction_factor=0.1,
+                    backtracking_residual_limit=0.002, no_ptc=None),
+}
+
+
+def bump_variant(name):
+    import dataclasses
+    c = bump_case()
+    c.ims = dataclasses.replace(c.ims, **BUMP_NONLINEAR[name])
+    return c

@@ -39,7 +39,11 @@ STRUCTS = {
     "mf6hip_ims_settings": _lib.ImsSettingsC,
     "mf6hip_cnvg_out": _lib.CnvgOutC,
     "mf6hip_gwf_model": _lib.GwfModelC,
+    "mf6hip_ptc": _lib.PtcC,
+    "mf6hip_underrelax": _lib.UnderRelaxC,
 }
+FORTRAN_TYPES = ["mf6hip_ims_settings", "mf6hip_gwf_model", "mf6hip_ptc",
+                 "mf6hip_underrelax"]
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
@@ -68,15 +72,17 @@ def test_struct_layout_matches_header(tmp_path):
 
 
 @pytest.mark.skipif(shutil.which("flang") is None, reason="needs flang")
-def test_fortran_settings_type_matches_header(tmp_path):
-    """The bind(C) derived type in the Fortran module has the C struct's
+@pytest.mark.parametrize("tname", FORTRAN_TYPES)
+def test_fortran_bind_c_types_match_header(tname, tmp_path):
+    """The bind(C) derived types of the Fortran module have the C structs'
     size and field offsets (checked through c_loc differences)."""
+    cls = STRUCTS[tname]
     mod = os.path.join(ROOT, "modflow6_amd", "fortran", "mf6hip_module.f90")
     prog = tmp_path / "chk.f90"
-    fields = [f for f, _ in _lib.ImsSettingsC._fields_]
+    fields = [f for f, _ in cls._fields_]
     body = ["program chk", "  use iso_c_binding", "  use mf6hip_module",
-            "  implicit none", "  type(mf6hip_ims_settings), target :: s",
-            "  type(mf6hip_ims_settings), target :: a(2)",
+            "  implicit none", f"  type({tname}), target :: s",
+            f"  type({tname}), target :: a(2)",
             "  integer(c_intptr_t) :: b",
             "  b = transfer(c_loc(s), b)",
             "  print *, 'size', transfer(c_loc(a(2)), b) - transfer(c_loc(a(1)), b)"]
@@ -89,9 +95,9 @@ def test_fortran_settings_type_matches_header(tmp_path):
                           cwd=tmp_path)
     out = subprocess.check_output([str(exe)], text=True)
     got = {ln.split()[0]: int(ln.split()[1]) for ln in out.strip().split("\n")}
-    assert got["size"] == C.sizeof(_lib.ImsSettingsC)
+    assert got["size"] == C.sizeof(cls)
     for f in fields:
-        assert got[f] == getattr(_lib.ImsSettingsC, f).offset, f
+        assert got[f] == getattr(cls, f).offset, f
 
 
 def _settings(**kw):
@@ -148,6 +154,9 @@ def test_null_arguments_rejected():
     assert L.mf6hip_solve(None, 1, 1, None, None, None, None, None, None) != 0
     assert L.mf6hip_gwf_create(None, None) == 1
     assert L.mf6hip_gwf_set_x(None, None) == 1
+    assert L.mf6hip_gwf_formulate(None, 1, 1, None, None) == 1
+    assert L.mf6hip_gwf_backtrack(None, 0.5, 1e-3, None) == 1
+    assert L.mf6hip_gwf_residual_norms(None, None, None) == 1
     assert L.mf6hip_destroy(None) == 0
     buf = C.create_string_buffer(4)
     assert L.mf6hip_last_error(buf, 4) == 0 and len(buf.value) <= 3

@@ -2,7 +2,8 @@
 GPU) on the reference's own autotest models: closed-form heads of a 1-D CHD
 line (test_par_gwf01.py:200-211), test_gwf_newton01.py (heads 8 and 7) and
 the golden heads of test_gwf_newton_under_relaxation.py (results.hds.cmp,
-compared with np.allclose as that test does)."""
+compared with np.allclose as that test does), also with the IMS NONLINEAR
+controls (backtracking, pseudo-transient continuation, under-relaxation)."""
 import os
 
 import numpy as np
@@ -10,7 +11,8 @@ import pytest
 
 from modflow6_amd.solution import GwfSolution
 from oracle import oracle as O
-from tests.cases import GOLD, bump_case, line_case, newton01_case, read_ulasav
+from tests.cases import (BUMP_NONLINEAR, GOLD, bump_case, bump_variant, line_case,
+                         newton01_case, read_ulasav)
 
 pytestmark = pytest.mark.gpu
 
@@ -47,3 +49,30 @@ def test_device_bump_newton_under_relaxation_golden():
     assert any(r.nur for r in recs)
     assert np.allclose(base, h)
     assert np.max(np.abs(h - ho)) < 1e-6
+
+
+@pytest.mark.parametrize("name", sorted(BUMP_NONLINEAR))
+def test_device_nonlinear_controls(name):
+    """Backtracking, PTC and SIMPLE / COOLEY / DBD under-relaxation on the
+    device: the golden heads, engaged as in the oracle, and the oracle's
+    heads to 1e-6 m."""
+    case = bump_variant(name)
+    gs = GwfSolution(case)
+    h = gs.run().reshape(51, 51)
+    recs = gs.log[0]
+    gs.destroy()
+    sim = O.GwfSimulation(case)
+    ho = sim.run().reshape(51, 51)
+    orecs = sim.log[0]
+    print(name, "device outer", len(recs), "oracle", len(orecs),
+          "bt", sum(r.nbt for r in recs), sum(r["nbt"] for r in orecs),
+          "ptc", sum(r.iptc for r in recs), sum(r["iptc"] for r in orecs))
+    base = read_ulasav(os.path.join(GOLD, "ex-gwf-bump", "results.hds.cmp"))
+    assert recs[-1].converged
+    assert np.allclose(base, h)
+    assert np.max(np.abs(h - ho)) < 1e-6
+    assert abs(len(recs) - len(orecs)) <= max(2, len(orecs) // 10)
+    if name in ("ptc", "complex"):
+        assert sum(r.iptc for r in recs) > 0
+    if name == "complex":
+        assert sum(r.nbt for r in recs) > 0

@@ -15,10 +15,12 @@ invariants that a dropped term, a wrong sign or a transposed operand break.
 import os
 
 import numpy as np
+import pytest
 
 from modflow6_amd import configs
 from oracle import oracle as O
-from tests.cases import GOLD, bump_case, line_case, newton01_case, read_ulasav
+from tests.cases import (BUMP_NONLINEAR, GOLD, bump_case, bump_variant, line_case,
+                         newton01_case, read_ulasav)
 
 
 def test_linear_heads_between_two_chd():
@@ -137,3 +139,43 @@ def test_cg_exact_on_small_spd_system():
     icn, it, *_ = ims.apply(sim.amat, sim.rhs.copy(), sim.x, kstp=1)
     exact = 1.0 + 9.0 * np.arange(case.ncol) / (case.ncol - 1)
     assert it <= 2 and np.max(np.abs(sim.x - exact)) < 1e-12
+
+
+@pytest.mark.parametrize("name", sorted(BUMP_NONLINEAR))
+def test_nonlinear_controls_reach_the_golden_heads(name):
+    """Under-relaxation, backtracking and PTC change the outer-iteration
+    path, not the fixed point: each variant reproduces results.hds.cmp, and
+    each actually engages (a dropped update would leave the path of the
+    plain run unchanged)."""
+    sim = O.GwfSimulation(bump_variant(name))
+    h = sim.run().reshape(51, 51)
+    recs = sim.log[0]
+    base = read_ulasav(os.path.join(GOLD, "ex-gwf-bump", "results.hds.cmp"))
+    assert recs[-1]["converged"]
+    assert np.allclose(base, h)
+    plain = O.GwfSimulation(bump_case())
+    plain.run()
+    if name in ("ptc", "complex"):
+        assert sum(r["iptc"] for r in recs) > 0
+    if name == "complex":
+        assert sum(r["nbt"] for r in recs) > 0
+    if name != "cooley":
+        assert len(recs) != len(plain.log[0])
+
+
+def test_simple_under_relaxation_step():
+    """One outer iteration with SIMPLE under-relaxation lands at
+    xtemp + gamma (x_lin - xtemp), x_lin the plain linear update."""
+    a = O.GwfSimulation(bump_variant("simple"))
+    b = O.GwfSimulation(bump_case())
+    for s in (a, b):
+        s.kper, s.kstp = 1, 1
+        s.advance(1.0, True)
+        s.solve_outer(1)
+    g = 0.7
+    x_lin = b.x.copy()
+    # NUR may have moved cells of b below their bottom; compare elsewhere
+    moved = np.abs(b.dxold) == 0
+    exp = a.xtemp + g * (x_lin - a.xtemp)
+    keep = ~moved & (a.ibound > 0)
+    assert np.allclose(a.x[keep], exp[keep], rtol=0, atol=1e-12)
