@@ -69,6 +69,9 @@ typedef struct mf6hip_ims_settings {
   int kernel_variant;   /* 0: fused thread-per-column CG when the system is
                            symmetric (default); 1: one kernel per reference
                            vector pass */
+  int preconditioner;   /* 0: ILU0 / MILU0 (IMS); 1: ILU0 followed by the
+                           learned correction of an attached network, in
+                           flexible CG (CG only) */
 } mf6hip_ims_settings;
 
 /* Per-inner-iteration record, the values ConvergenceSummaryType keeps for
@@ -239,6 +242,43 @@ int mf6hip_gwf_outer_update(void *handle, int kiter, mf6hip_underrelax *ur,
  * kernel (the SpMV) with its launch count. */
 int mf6hip_last_timing(void *handle, double *solve_ms, double *spmv_avg_ms,
                        int *spmv_launches);
+
+/* --- learned preconditioner (no counterpart in the reference) ----------
+ *
+ * A 4-level 3D U-Net (channels c0 = 16, c1..c3 in {16, 32, 48, 64}; 1x2x2
+ * pooling, so nrow and ncol must be divisible by 8) with BF16 weights and
+ * activations and FP32 accumulation on MFMA, trained on the device on
+ * (residual, error) pairs.  Inputs are nin planes of nlay*nrow*ncol FP32
+ * values in DIS node order; parameters are one FP32 vector: per 3x3x3
+ * convolution (e0a e0b e1a e1b e2a e2b e3a e3b d2 d1 d0) W[27][cin][cout]
+ * then b[cout], then the 1x1x1 output layer w[c0], b. */
+int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, const int *channels,
+                       void **net);
+int mf6hip_unet_destroy(void *net);
+int mf6hip_unet_num_params(void *net, long long *n);
+int mf6hip_unet_set_params(void *net, const float *params);
+int mf6hip_unet_get_params(void *net, float *params);
+/* y = network(x) */
+int mf6hip_unet_forward(void *net, const float *x, float *y);
+/* relative squared error sum (y - t)^2 / sum t^2 of one sample, and its
+ * gradient with respect to the parameters */
+int mf6hip_unet_grad(void *net, const float *x, const float *t, float *loss,
+                     float *grad);
+/* store n training samples on the device: [n][nin + 1][nodes] FP32 planes
+ * (inputs, then the target); kept in BF16 */
+int mf6hip_unet_set_data(void *net, int n, const float *samples);
+/* Adam: nsteps steps over batches of stored samples in the given order
+ * (order has nsteps * batch entries); loss[nsteps] gets each step's mean */
+int mf6hip_unet_train(void *net, int nsteps, int batch, const int *order,
+                      float lr, float *loss);
+/* Attach a trained network (3 inputs) to a solver created with
+ * preconditioner = 1; features = 2 planes (standardised ln K and
+ * ln(K33 / K)) in DIS node order; fcg_m = directions the flexible CG keeps
+ * (1..64).  net = NULL detaches (the flexible CG then runs on ILU0). */
+int mf6hip_attach_unet(void *handle, void *net, const float *features, int fcg_m);
+/* Iterations of the last solve that used the learned correction (the rest
+ * ran on the ILU0 fallback). */
+int mf6hip_last_learned_iterations(void *handle, int *n);
 
 /* --- diagnostics (no counterpart in the reference; used by the parity
  * tests and the benchmark to reach single operators) --------------------- */

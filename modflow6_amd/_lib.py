@@ -29,6 +29,10 @@ EXPORTS = [
     "mf6hip_gwf_residual_norms", "mf6hip_gwf_backtrack",
     "mf6hip_last_timing", "mf6hip_set_timing", "mf6hip_apply_op",
     "mf6hip_load_factor", "mf6hip_get_system",
+    "mf6hip_unet_create", "mf6hip_unet_destroy", "mf6hip_unet_num_params",
+    "mf6hip_unet_set_params", "mf6hip_unet_get_params", "mf6hip_unet_forward",
+    "mf6hip_unet_grad", "mf6hip_unet_set_data", "mf6hip_unet_train",
+    "mf6hip_attach_unet", "mf6hip_last_learned_iterations",
 ]
 
 
@@ -43,7 +47,8 @@ class ImsSettingsC(C.Structure):
                 ("nlay", C.c_int), ("nrow", C.c_int), ("ncol", C.c_int),
                 ("inner_fp32_max", C.c_int),
                 ("inner_fp32_rtol", C.c_double),
-                ("kernel_variant", C.c_int)]
+                ("kernel_variant", C.c_int),
+                ("preconditioner", C.c_int)]
 
 
 class CnvgOutC(C.Structure):
@@ -134,6 +139,18 @@ def load():
     L.mf6hip_apply_op.argtypes = [vp, i, dp, dp]
     L.mf6hip_load_factor.argtypes = [vp, dp]
     L.mf6hip_get_system.argtypes = [vp, dp, dp]
+    fp = C.POINTER(C.c_float)
+    L.mf6hip_unet_create.argtypes = [i, i, i, i, ip, C.POINTER(vp)]
+    L.mf6hip_unet_destroy.argtypes = [vp]
+    L.mf6hip_unet_num_params.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.mf6hip_unet_set_params.argtypes = [vp, fp]
+    L.mf6hip_unet_get_params.argtypes = [vp, fp]
+    L.mf6hip_unet_forward.argtypes = [vp, fp, fp]
+    L.mf6hip_unet_grad.argtypes = [vp, fp, fp, fp, fp]
+    L.mf6hip_unet_set_data.argtypes = [vp, i, fp]
+    L.mf6hip_unet_train.argtypes = [vp, i, i, ip, C.c_float, fp]
+    L.mf6hip_attach_unet.argtypes = [vp, vp, fp, i]
+    L.mf6hip_last_learned_iterations.argtypes = [vp, ip]
     for nm in EXPORTS:
         getattr(L, nm).restype = i
     _LIB = L
@@ -156,6 +173,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def iptr(a):

@@ -58,6 +58,12 @@ struct Mf6Handle {
          *zf = nullptr;
   double *t = nullptr, *v = nullptr, *dhat = nullptr, *phat = nullptr,
          *qhat = nullptr;
+  // learned preconditioner (flexible CG)
+  void *unet = nullptr;          // attached network (not owned)
+  float *unet_feat = nullptr;    // [2][N] features, natural order
+  int learned_iters = 0;         // iterations of the last solve that used it
+  int fcg_m = 8;                 // flexible CG: directions kept
+  double *fcg_store = nullptr;   // [2][fcg_m][NP]: directions p_j and A p_j
   // mixed precision (FP32 inner solve)
   float *A32 = nullptr, *invpiv32 = nullptr;
   unsigned short *invpiv16 = nullptr;
@@ -118,3 +124,13 @@ int mf6hip_run_cg_reliable(Mf6Handle *h, int *icnvg, int *iters);
 int mf6hip_is_symmetric(Mf6Handle *h, int *sym);
 int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa,
                  const double *xb, bool active_only);
+
+// solver pieces shared with the flexible CG (mf6hip_fcg.hip)
+int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z);
+int mf6hip_spmv64(Mf6Handle *h, const double *x, double *y);
+int mf6hip_run_fcg(Mf6Handle *h);
+// network entry points used by the solver (mf6hip_unet.hip)
+int mf6hip_unet_apply_device(void *net, hipStream_t s);
+unsigned short *mf6hip_unet_input(void *net);
+float *mf6hip_unet_output(void *net);
+int mf6hip_unet_dims(void *net, int *nlay, int *nrow, int *ncol, int *nin);

@@ -942,6 +942,18 @@ static void launch_spmv(Mf6Handle *h, const TM *A, const TV *x, TV *y) {
   k_spmv<TM, TV><<<grid_for(h->L.NP), MF6_BLOCK, 0, h->stream>>>(h->L, A, x, y);
 }
 
+int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z) {
+  launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, r, h->zf, z, nullptr);
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
+
+int mf6hip_spmv64(Mf6Handle *h, const double *x, double *y) {
+  launch_spmv<double, double>(h, h->A, x, y);
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
+
 static void timed_spmv_begin(Mf6Handle *h, int &slot) {
   slot = -1;
   if (!h->timing) return;
@@ -1240,7 +1252,8 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
     CK(compute_l2norm0(h));
     CK(read_state(h));
     if (!h->h_state->stop) {
-      if (h->s.ilinmeth == 1 && sym && h->s.kernel_variant == 0) CK(mf6hip_run_cg_fused(h));
+      if (h->s.preconditioner == 1) CK(mf6hip_run_fcg(h));
+      else if (h->s.ilinmeth == 1 && sym && h->s.kernel_variant == 0) CK(mf6hip_run_cg_fused(h));
       else if (h->s.ilinmeth == 1) CK(run_cg(h));
       else CK(run_bcgs(h));
     }
@@ -1305,6 +1318,11 @@ static int check_settings(const mf6hip_ims_settings *s) {
   if (s->precision != MF6HIP_FP64 && s->ilinmeth != 1) { mf6hip_set_error("mixed precision is implemented for CG only"); return MF6HIP_EUNSUPPORTED; }
   if (s->ordering != MF6HIP_ORDER_RB) { mf6hip_set_error("unknown ordering"); return MF6HIP_EINVAL; }
   if (s->kernel_variant < 0 || s->kernel_variant > 1) { mf6hip_set_error("unknown kernel variant"); return MF6HIP_EINVAL; }
+  if (s->preconditioner < 0 || s->preconditioner > 1) { mf6hip_set_error("unknown preconditioner"); return MF6HIP_EINVAL; }
+  if (s->preconditioner == 1 && (s->ilinmeth != 1 || s->precision != MF6HIP_FP64)) {
+    mf6hip_set_error("the learned preconditioner runs in FP64 flexible CG (LINEAR_ACCELERATION CG)");
+    return MF6HIP_EUNSUPPORTED;
+  }
   if (s->nlay < 1 || s->nrow < 1 || s->ncol < 1) { mf6hip_set_error("DIS shape nlay/nrow/ncol required"); return MF6HIP_EINVAL; }
   if (s->nlay > 32) { mf6hip_set_error("more than 32 layers is not supported on the device path"); return MF6HIP_EUNSUPPORTED; }
   return MF6HIP_OK;
@@ -1354,7 +1372,8 @@ static int alloc_solver(Mf6Handle *h) {
   if (h->s.precision == MF6HIP_FP64) {
     CK(dalloc(&h->p, N));
     CK(dalloc(&h->p2, N));
-    if (h->s.ilinmeth == 2) {
+    // BiCGSTAB vectors; flexible CG uses t, v, dhat, phat as well
+    if (h->s.ilinmeth == 2 || h->s.preconditioner == 1) {
       CK(dalloc(&h->t, N));
       CK(dalloc(&h->v, N));
       CK(dalloc(&h->dhat, N));
@@ -1482,7 +1501,8 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->d_hist, h->d_hloc, h->d_tmp_nat, h->d_amat_csr, h->top, h->bot,
                   h->area, h->k11, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
-                  h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent};
+                  h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
+                  h->unet_feat, h->fcg_store};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int t = 0; t < 5; t++) {

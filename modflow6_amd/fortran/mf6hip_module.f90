@@ -34,6 +34,7 @@ module mf6hip_module
     integer(c_int) :: inner_fp32_max
     real(c_double) :: inner_fp32_rtol
     integer(c_int) :: kernel_variant
+    integer(c_int) :: preconditioner
   end type mf6hip_ims_settings
 
   integer(c_int), parameter, public :: MF6HIP_PKG_WEL = 1

@@ -38,6 +38,7 @@ class ImsLinearSettings:
     inner_fp32_max: int = 0
     inner_fp32_rtol: float = 0.0
     kernel_variant: int = 0      # 0 fused CG (symmetric systems), 1 unfused
+    preconditioner: int = 0      # 0 ILU0/MILU0, 1 ILU0 + learned correction (FCG)
 
     def to_c(self, nlay, nrow, ncol):
         return _lib.ImsSettingsC(
@@ -45,7 +46,7 @@ class ImsLinearSettings:
             self.ilinmeth, self.iscl, self.iord, self.north, self.relax,
             self.level, self.droptol, self.precision, self.ordering, nlay,
             nrow, ncol, self.inner_fp32_max, self.inner_fp32_rtol,
-            self.kernel_variant)
+            self.kernel_variant, self.preconditioner)
 
 
 def dis_structure(nlay, nrow, ncol):
@@ -111,9 +112,11 @@ class HipLinearSolver:
     def initialize(self, ia, ja, settings: ImsLinearSettings, nlay, nrow,
                    ncol):
         L = _lib.load()
+        self.L = L
         self.neq = ia.size - 1
         self.nja = ja.size
         self.settings = settings
+        self._structure = (ia, ja, nlay, nrow, ncol)
         s = settings.to_c(nlay, nrow, ncol)
         _lib.check(L.mf6hip_create(self.neq, self.nja, _lib.iptr(ia),
                                    _lib.iptr(ja), C.byref(s),
@@ -136,6 +139,26 @@ class HipLinearSolver:
         self.iteration_number = inner.value
         self.is_converged = icnvg.value
         return icnvg.value, inner.value
+
+    def reconfigure(self):
+        """Recreate the handle with self.settings (same structure)."""
+        self.destroy()
+        ia, ja, nlay, nrow, ncol = self._structure
+        return self.initialize(ia, ja, self.settings, nlay, nrow, ncol)
+
+    def attach_unet(self, net, feat, fcg_m=8):
+        """Attach a trained network (learned.UNet) with its two feature
+        planes; used when settings.preconditioner == 1."""
+        f = None if feat is None else np.ascontiguousarray(feat, dtype=np.float32)
+        _lib.check(self.L.mf6hip_attach_unet(self.handle, net.h if net else None,
+                                             _lib.fptr(f), int(fcg_m)),
+                   "mf6hip_attach_unet")
+
+    def learned_iterations(self):
+        n = C.c_int(0)
+        _lib.check(self.L.mf6hip_last_learned_iterations(self.handle, C.byref(n)),
+                   "mf6hip_last_learned_iterations")
+        return n.value
 
     def print_summary(self):
         return (f"{self.name}: HIP IMS linear solver, {self.neq} equations, "

@@ -1,0 +1,159 @@
+"""The learned-preconditioner network on the GPU (BF16 MFMA forward and
+backward, Adam) against the CPU oracle (oracle/unet_oracle.py), and the
+flexible-CG solve with the network attached.
+
+Tolerances: the device rounds weights, inputs and every activation and
+activation gradient to BF16 (8 significant bits) and accumulates in FP32,
+so outputs and gradients are compared by relative L2 error (forward 3e-2,
+gradients 1e-1 per layer) and cosine similarity (0.999 / 0.99)."""
+import numpy as np
+import pytest
+
+from modflow6_amd import _lib, configs
+from modflow6_amd.ims import ImsLinearSettings
+from modflow6_amd.learned import UNet, features, init_params, layer_shapes
+from oracle import oracle as O
+from oracle import unet_oracle as U
+from tests.helpers import device_solver, oracle_system
+
+pytestmark = pytest.mark.gpu
+
+
+def to_oracle(flat, nin, ch):
+    out, o = [], 0
+    for a, b in layer_shapes(nin, ch):
+        w = flat[o:o + 27 * a * b].reshape(27, a, b).astype(np.float64); o += 27 * a * b
+        bb = flat[o:o + b].astype(np.float64); o += b
+        out.append((w, bb))
+    wo = flat[o:o + ch[0]].reshape(ch[0], 1).astype(np.float64); o += ch[0]
+    out.append((wo, flat[o:o + 1].astype(np.float64)))
+    return out
+
+
+def from_oracle(grads):
+    return np.concatenate([np.concatenate([w.ravel(), b.ravel()]) for w, b in grads])
+
+
+def bf16(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def cos(a, b):
+    return float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+
+
+SHAPES = [((2, 16, 16), (16, 16, 32, 32)), ((3, 16, 24), (16, 32, 64, 64))]
+
+
+def sample(shape, seed):
+    rng = np.random.default_rng(seed)
+    D, H, W = shape
+    x = rng.standard_normal((3, D * H * W)).astype(np.float32)
+    t = rng.standard_normal(D * H * W).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("shape,ch", SHAPES)
+def test_unet_forward_matches_oracle(shape, ch):
+    D, H, W = shape
+    p = init_params(3, ch, np.random.default_rng(3))
+    p[::7] += 0.05          # non-zero biases too
+    net = UNet(D, H, W, 3, ch, params=p)
+    x, _ = sample(shape, 4)
+    y = net.forward(x)
+    net.destroy()
+    ps = to_oracle(bf16(p), 3, ch)
+    xo = bf16(x).astype(np.float64).T.reshape(D, H, W, 3)
+    yo = U.forward(ps, xo).reshape(-1)
+    print("forward rel", rel(y, yo), "cos", cos(y, yo))
+    assert rel(y, yo) < 3e-2 and cos(y, yo) > 0.999
+
+
+@pytest.mark.parametrize("shape,ch", SHAPES)
+def test_unet_gradients_match_oracle(shape, ch):
+    D, H, W = shape
+    p = init_params(3, ch, np.random.default_rng(5))
+    net = UNet(D, H, W, 3, ch, params=p)
+    x, t = sample(shape, 6)
+    loss, g = net.grad(x, t)
+    net.destroy()
+    ps = to_oracle(bf16(p), 3, ch)
+    xo = bf16(x).astype(np.float64).T.reshape(D, H, W, 3)
+    to = bf16(t).astype(np.float64).reshape(D, H, W, 1)
+    lo, go = U.loss_and_grads(ps, xo, to)
+    assert abs(loss - lo) <= 3e-2 * lo
+    o = 0
+    for k, (w, b) in enumerate(go):
+        nw, nb = w.size, b.size
+        gw, gb = g[o:o + nw], g[o + nw:o + nw + nb]
+        o += nw + nb
+        print(f"layer {k}: weight rel {rel(gw, w.ravel()):.3e} cos {cos(gw, w.ravel()):.5f}")
+        assert rel(gw, w.ravel()) < 1e-1 and cos(gw, w.ravel()) > 0.99
+        assert rel(gb, b.ravel()) < 1e-1
+    assert o == g.size
+
+
+def test_unet_training_reduces_loss():
+    shape, ch = (2, 16, 16), (16, 16, 32, 32)
+    D, H, W = shape
+    rng = np.random.default_rng(7)
+    # targets a smooth function of the inputs: a learnable map
+    n = 8
+    data = np.zeros((n, 4, D * H * W), dtype=np.float32)
+    for i in range(n):
+        x = rng.standard_normal((3, D, H, W))
+        data[i, :3] = x.reshape(3, -1)
+        tt = 0.5 * x[0] + np.roll(x[1], 1, axis=2) - 0.3 * x[2]
+        data[i, 3] = tt.reshape(-1)
+    net = UNet(D, H, W, 3, ch, seed=1)
+    net.set_data(data)
+    loss = net.train(nsteps=120, batch=2, lr=2e-3, seed=0)
+    net.destroy()
+    print("loss first/last", loss[:3], loss[-3:])
+    assert np.all(np.isfinite(loss))
+    assert loss[-10:].mean() < 0.5 * loss[:10].mean()
+
+
+def test_fcg_with_untrained_network_converges():
+    """A random network must not break the solve: the energy-optimal step
+    and the fallback keep flexible CG converging to the ILU0-PCG heads."""
+    case = configs.steady_small(nlay=2, nrow=32, ncol=32, nwel=5)
+    sim, amat, rhs, x0 = oracle_system(case)
+    ref = device_solver(case)
+    xr = x0.copy()
+    ic, it_ref = ref.solve(1, amat, rhs, xr)
+    ref.destroy()
+    sv = device_solver(case, preconditioner=1)
+    net = UNet(case.nlay, case.nrow, case.ncol, 3, (16, 16, 32, 32), seed=2)
+    sv.attach_unet(net, features(case))
+    xl = x0.copy()
+    icl, it_l = sv.solve(1, amat, rhs, xl)
+    nl = sv.learned_iterations()
+    sv.destroy()
+    net.destroy()
+    print("ILU0-PCG", it_ref, "FCG+net", it_l, "learned iterations", nl)
+    assert ic == 1 and icl == 1
+    assert np.max(np.abs(xl - xr)) < 10 * case.ims.inner_dvclose
+
+
+def test_fcg_without_network_matches_pcg():
+    case = configs.steady_small(nlay=2, nrow=32, ncol=32, nwel=5)
+    sim, amat, rhs, x0 = oracle_system(case)
+    ref = device_solver(case)
+    xr = x0.copy()
+    ic, it_ref = ref.solve(1, amat, rhs, xr)
+    ref.destroy()
+    sv = device_solver(case, preconditioner=1)
+    xl = x0.copy()
+    icl, it_l = sv.solve(1, amat, rhs, xl)
+    sv.destroy()
+    print("ILU0-PCG", it_ref, "ILU0-FCG", it_l)
+    assert icl == 1 and abs(it_l - it_ref) <= max(3, it_ref // 20)
+    assert np.max(np.abs(xl - xr)) < 10 * case.ims.inner_dvclose
