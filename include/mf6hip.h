@@ -280,6 +280,32 @@ int mf6hip_attach_unet(void *handle, void *net, const float *features, int fcg_m
  * ran on the ILU0 fallback). */
 int mf6hip_last_learned_iterations(void *handle, int *n);
 
+/* --- ensembles of independent systems (no counterpart in the reference:
+ * MF6 solves one model per solution; an ensemble is many IMS solves of
+ * different DIS models at once) ------------------------------------------
+ *
+ * Mixed-precision CG (FP32 ILU0 / SpMV, FP64 heads and residual
+ * replacement) for every system in each kernel launch; the iteration
+ * sequence is captured as a HIP graph and replayed until every system has
+ * stopped.  Systems must use CG, ILU0 / MILU0 and ICNVGOPT 0. */
+int mf6hip_batch_create(int capacity, void **batch);
+/* register a system: settings carry its DIS shape and IMS values */
+int mf6hip_batch_add(void *batch, const mf6hip_ims_settings *settings);
+/* allocate device storage for every registered system */
+int mf6hip_batch_finalize(void *batch);
+/* system data in the reference layout (1-based DIS CSR, amat after the
+ * sln_ls row fixes, rhs, start heads) */
+int mf6hip_batch_set_system(void *batch, int isys, const int *ia, int nja,
+                            const double *amat, const double *rhs,
+                            const double *x0);
+/* pick columns per thread and the persistent grid by timing short replays;
+ * report receives a text summary */
+int mf6hip_batch_autotune(void *batch, char *report, int report_len);
+/* solve every system; icnvg / iters per system, total device time */
+int mf6hip_batch_solve(void *batch, int *icnvg, int *iters, double *solve_ms);
+int mf6hip_batch_get_x(void *batch, int isys, double *x);
+int mf6hip_batch_destroy(void *batch);
+
 /* --- diagnostics (no counterpart in the reference; used by the parity
  * tests and the benchmark to reach single operators) --------------------- */
 

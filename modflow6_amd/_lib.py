@@ -33,6 +33,9 @@ EXPORTS = [
     "mf6hip_unet_set_params", "mf6hip_unet_get_params", "mf6hip_unet_forward",
     "mf6hip_unet_grad", "mf6hip_unet_set_data", "mf6hip_unet_train",
     "mf6hip_attach_unet", "mf6hip_last_learned_iterations",
+    "mf6hip_batch_create", "mf6hip_batch_add", "mf6hip_batch_finalize",
+    "mf6hip_batch_set_system", "mf6hip_batch_autotune", "mf6hip_batch_solve",
+    "mf6hip_batch_get_x", "mf6hip_batch_destroy",
 ]
 
 
@@ -151,6 +154,14 @@ def load():
     L.mf6hip_unet_train.argtypes = [vp, i, i, ip, C.c_float, fp]
     L.mf6hip_attach_unet.argtypes = [vp, vp, fp, i]
     L.mf6hip_last_learned_iterations.argtypes = [vp, ip]
+    L.mf6hip_batch_create.argtypes = [i, C.POINTER(vp)]
+    L.mf6hip_batch_add.argtypes = [vp, C.POINTER(ImsSettingsC)]
+    L.mf6hip_batch_finalize.argtypes = [vp]
+    L.mf6hip_batch_set_system.argtypes = [vp, i, ip, i, dp, dp, dp]
+    L.mf6hip_batch_autotune.argtypes = [vp, C.c_char_p, i]
+    L.mf6hip_batch_solve.argtypes = [vp, ip, ip, dp]
+    L.mf6hip_batch_get_x.argtypes = [vp, i, dp]
+    L.mf6hip_batch_destroy.argtypes = [vp]
     for nm in EXPORTS:
         getattr(L, nm).restype = i
     _LIB = L

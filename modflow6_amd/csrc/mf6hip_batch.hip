@@ -1,0 +1,984 @@
+// mf6hip_batch.hip -- an ensemble of independent steady systems (uneven DIS
+// shapes) solved together: mixed-precision PCG with reliable updates (FP32
+// matrix, pivots and Krylov vectors; FP64 heads and FP64 residual
+// replacement), every kernel launch covering all systems, the iteration
+// sequence captured once as a HIP graph and replayed until every system has
+// stopped.
+//
+// Work is cut into tiles of columns: a tile is (system, first column) of
+// 256 * cpt columns (cpt = columns per thread); red, black and interleaved
+// (both colours, thread t -> colour t & 1) tile lists exist for the three
+// kinds of stencil kernels.  Kernels walk the tile lists with a persistent
+// grid (gridDim blocks, tile = block, block + gridDim, ...), skip systems
+// that have stopped, and write one reduction partial per tile; one block per
+// system then reduces its contiguous tile range.  cpt and the grid size are
+// chosen by an autotuner that times a short replay of each candidate.
+//
+// Per system the arithmetic is that of the single-system fused CG
+// (ims_base_cg, src/Solution/LinearMethods/ImsLinearBase.f90:30-240, on the
+// column red-black ordering), including the IMS convergence test.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mf6hip_handle.h"
+
+namespace {
+
+#define BCK(x)                        \
+  do {                                \
+    int rc_ = (x);                    \
+    if (rc_ != MF6HIP_OK) return rc_; \
+  } while (0)
+
+struct BSys {
+  Layout L;
+  long long aoff;   // first of the 7 planes (each NP long)
+  long long voff;   // first vector entry
+  int tr0, ntr;     // red tiles
+  int tb0, ntb;     // black tiles
+  int ti0, nti;     // interleaved tiles
+};
+
+struct BFState {
+  double apend_x, apend_d, pmax;
+  int ploc, pcur;
+};
+
+struct Tile {
+  int sys, base;
+};
+
+enum { P_SS_R = 0, P_SS_B = 1, P_RHO_B = 2, P_RHO_R = 3, P_PQ = 4 };
+enum { M_D_R = 0, M_D_B = 1, M_P = 2, M_TRUE = 3 };
+
+template <typename T>
+__device__ inline T wsum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ inline MaxLoc wmax(MaxLoc a) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    MaxLoc b;
+    b.v = __shfl_xor(a.v, o);
+    b.loc = __shfl_xor(a.loc, o);
+    a = maxloc_pick(a, b);
+  }
+  return a;
+}
+
+// block sum / max, result valid in thread 0
+__device__ inline double bsum(double v, double *lds) {
+  v = wsum(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s = s + lds[i];
+  return s;
+}
+
+__device__ inline MaxLoc bmax(MaxLoc a, MaxLoc *lds) {
+  a = wmax(a);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = a;
+  __syncthreads();
+  MaxLoc r{0.0, -1};
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = maxloc_pick(r, lds[i]);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// stencil kernels (FP32)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(MF6_BLOCK) kb_red_fwd(const BSys *sys, const Tile *tiles,
+                                                        int ntiles, int cpt, const float *A32,
+                                                        const float *piv, float *d,
+                                                        const float *q, float *zf, double *part,
+                                                        MaxLoc *mpart, int T, const KState *st,
+                                                        const BFState *fs) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (st[s].stop) continue;
+    const Layout L = sys[s].L;
+    const float *A = A32 + sys[s].aoff;
+    const float *pv = piv + sys[s].voff;
+    float *dd = d + sys[s].voff;
+    const float *qq = q + sys[s].voff;
+    float *zz = zf + sys[s].voff;
+    const int NP = L.NP;
+    const float alpha = (float)fs[s].apend_d;
+    double ss = 0.0;
+    MaxLoc md{0.0, -1};
+    for (int c = 0; c < cpt; c++) {
+      const int h2 = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      if (h2 >= L.HP0) continue;
+      const int l = col_l(L, 0, h2);
+      float zup = 0.f, lup = 0.f;
+      for (int k = 0; k < L.nlay; k++) {
+        const int pos = k * L.HP + h2;
+        float dv = dd[pos];
+        dv = dv - alpha * qq[pos];
+        dd[pos] = dv;
+        md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
+        ss = ss + (double)dv * (double)dv;
+        float tv = dv;
+        if (k > 0) tv = tv - lup * zup;
+        zz[pos] = tv;
+        zup = tv;
+        lup = A[6 * NP + pos] * pv[pos];
+      }
+    }
+    ss = bsum(ss, lds);
+    if (threadIdx.x == 0) part[P_SS_R * T + tile] = ss;
+    md = bmax(md, mlds);
+    if (threadIdx.x == 0) mpart[M_D_R * T + tile] = md;
+  }
+}
+
+__global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Tile *tiles,
+                                                      int ntiles, int cpt, const float *A32,
+                                                      const float *piv, float *d, const float *q,
+                                                      const float *zf, float *z, double *part,
+                                                      MaxLoc *mpart, int T, const KState *st,
+                                                      const BFState *fs) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (st[s].stop) continue;
+    const Layout L = sys[s].L;
+    const float *A = A32 + sys[s].aoff;
+    const float *pv = piv + sys[s].voff;
+    float *dd = d + sys[s].voff;
+    const float *qq = q + sys[s].voff;
+    const float *zr = zf + sys[s].voff;
+    float *zz = z + sys[s].voff;
+    const int NP = L.NP;
+    const float alpha = (float)fs[s].apend_d;
+    double ss = 0.0, rho = 0.0;
+    MaxLoc md{0.0, -1};
+    for (int c = 0; c < cpt; c++) {
+      const int h2 = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      if (h2 >= L.HP1) continue;
+      const int l = col_l(L, 1, h2);
+      const int i = l / L.ncol, j = l - i * L.ncol;
+      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
+      const int il = j > 0 ? hn(l, -1) : -1;
+      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      float zup = 0.f, lup = 0.f;
+      for (int k = 0; k < L.nlay; k++) {
+        const int pos = L.H + k * L.HP + h2;
+        const int rb = k * L.HP;
+        float dv = dd[pos];
+        dv = dv - alpha * qq[pos];
+        dd[pos] = dv;
+        md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
+        ss = ss + (double)dv * (double)dv;
+        float tv = dv;
+        if (ib >= 0) tv = tv - (A[5 * NP + rb + ib] * pv[rb + ib]) * zr[rb + ib];
+        if (il >= 0) tv = tv - (A[4 * NP + rb + il] * pv[rb + il]) * zr[rb + il];
+        if (ir >= 0) tv = tv - (A[4 * NP + pos] * pv[rb + ir]) * zr[rb + ir];
+        if (ifr >= 0) tv = tv - (A[5 * NP + pos] * pv[rb + ifr]) * zr[rb + ifr];
+        if (k > 0) tv = tv - lup * zup;
+        zz[pos] = tv;
+        zup = tv;
+        lup = A[6 * NP + pos] * pv[pos];
+      }
+      float zdn = 0.f;
+      for (int k = L.nlay - 1; k >= 0; k--) {
+        const int pos = L.H + k * L.HP + h2;
+        float tv = zz[pos];
+        if (k < L.nlay - 1) tv = tv - A[6 * NP + pos] * zdn;
+        tv = tv * pv[pos];
+        zz[pos] = tv;
+        zdn = tv;
+        rho = rho + (double)dd[pos] * (double)tv;
+      }
+    }
+    ss = bsum(ss, lds);
+    if (threadIdx.x == 0) part[P_SS_B * T + tile] = ss;
+    rho = bsum(rho, lds);
+    if (threadIdx.x == 0) part[P_RHO_B * T + tile] = rho;
+    md = bmax(md, mlds);
+    if (threadIdx.x == 0) mpart[M_D_B * T + tile] = md;
+  }
+}
+
+__global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const Tile *tiles,
+                                                        int ntiles, int cpt, const float *A32,
+                                                        const float *piv, const float *d,
+                                                        const float *zf, float *z, double *part,
+                                                        int T, const KState *st) {
+  __shared__ double lds[8];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (st[s].stop) continue;
+    const Layout L = sys[s].L;
+    const float *A = A32 + sys[s].aoff;
+    const float *pv = piv + sys[s].voff;
+    const float *dd = d + sys[s].voff;
+    const float *zr = zf + sys[s].voff;
+    float *zz = z + sys[s].voff;
+    const int NP = L.NP;
+    double rho = 0.0;
+    for (int c = 0; c < cpt; c++) {
+      const int h2 = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      if (h2 >= L.HP0) continue;
+      const int l = col_l(L, 0, h2);
+      const int i = l / L.ncol, j = l - i * L.ncol;
+      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
+      const int il = j > 0 ? hn(l, -1) : -1;
+      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      float zdn = 0.f;
+      for (int k = L.nlay - 1; k >= 0; k--) {
+        const int pos = k * L.HP + h2;
+        const int bb = L.H + k * L.HP;
+        float tv = zr[pos];
+        if (k < L.nlay - 1) tv = tv - A[6 * NP + pos] * zdn;
+        if (ib >= 0) tv = tv - A[5 * NP + bb + ib] * zz[bb + ib];
+        if (il >= 0) tv = tv - A[4 * NP + bb + il] * zz[bb + il];
+        if (ir >= 0) tv = tv - A[4 * NP + pos] * zz[bb + ir];
+        if (ifr >= 0) tv = tv - A[5 * NP + pos] * zz[bb + ifr];
+        tv = tv * pv[pos];
+        zz[pos] = tv;
+        zdn = tv;
+        rho = rho + (double)dd[pos] * (double)tv;
+      }
+    }
+    rho = bsum(rho, lds);
+    if (threadIdx.x == 0) part[P_RHO_R * T + tile] = rho;
+  }
+}
+
+// p_out = z (first iteration) or z + beta p_in ; e += apend_x p_in ; max |p|
+__global__ void __launch_bounds__(MF6_BLOCK) kb_pupdate(const BSys *sys, const Tile *tiles,
+                                                        int ntiles, int cpt, const float *z,
+                                                        float *p0, float *p1, float *e,
+                                                        MaxLoc *mpart, int T, const KState *st,
+                                                        const BFState *fs) {
+  __shared__ MaxLoc mlds[8];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (st[s].stop) continue;
+    const Layout L = sys[s].L;
+    const long long vo = sys[s].voff;
+    const float *pin = (fs[s].pcur ? p1 : p0) + vo;
+    float *pout = (fs[s].pcur ? p0 : p1) + vo;
+    const float *zz = z + vo;
+    float *ee = e + vo;
+    const bool first = st[s].iter == 0;
+    const float beta = (float)st[s].beta;
+    const float ax = (float)fs[s].apend_x;
+    MaxLoc mp{0.0, -1};
+    for (int c = 0; c < cpt; c++) {
+      const int t = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      const int cc = t & 1, h2 = t >> 1;
+      if (h2 >= ncols_of(L, cc)) continue;
+      const int l = col_l(L, cc, h2);
+      const int own = cc * L.H + h2;
+      for (int k = 0; k < L.nlay; k++) {
+        const int pos = own + k * L.HP;
+        float pv;
+        if (first) {
+          pv = zz[pos];
+        } else {
+          const float po = pin[pos];
+          pv = zz[pos] + beta * po;
+          ee[pos] = ee[pos] + ax * po;
+        }
+        pout[pos] = pv;
+        mp = maxloc_pick(mp, MaxLoc{(double)pv, k * L.NRC + l});
+      }
+    }
+    mp = bmax(mp, mlds);
+    if (threadIdx.x == 0) mpart[M_P * T + tile] = mp;
+  }
+}
+
+// q = A p (symmetric planes), partial p.q
+__global__ void __launch_bounds__(MF6_BLOCK) kb_spmv(const BSys *sys, const Tile *tiles,
+                                                     int ntiles, int cpt, const float *A32,
+                                                     const float *p0, const float *p1, float *q,
+                                                     double *part, int T, const KState *st,
+                                                     const BFState *fs) {
+  __shared__ double lds[8];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (st[s].stop) continue;
+    const Layout L = sys[s].L;
+    const long long vo = sys[s].voff;
+    const float *A = A32 + sys[s].aoff;
+    const float *p = (fs[s].pcur ? p0 : p1) + vo;   // the direction just formed
+    float *qq = q + vo;
+    const int NP = L.NP;
+    double sum = 0.0;
+    for (int c = 0; c < cpt; c++) {
+      const int t = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      const int cc = t & 1, h2 = t >> 1;
+      if (h2 >= ncols_of(L, cc)) continue;
+      const int l = col_l(L, cc, h2);
+      const int i = l / L.ncol, j = l - i * L.ncol;
+      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
+      const int il = j > 0 ? hn(l, -1) : -1;
+      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      const int own = cc * L.H + h2;
+      const int oth = (1 - cc) * L.H;
+      float pprev = 0.f, pcur = p[own], aup = 0.f;
+      for (int k = 0; k < L.nlay; k++) {
+        const int pos = own + k * L.HP;
+        const int ob = oth + k * L.HP;
+        const float pnext = (k + 1 < L.nlay) ? p[pos + L.HP] : 0.f;
+        const float adn = A[6 * NP + pos];
+        float tq = 0.f;
+        tq = tq + A[pos] * pcur;
+        if (k > 0) tq = tq + aup * pprev;
+        if (ib >= 0) tq = tq + A[5 * NP + ob + ib] * p[ob + ib];
+        if (il >= 0) tq = tq + A[4 * NP + ob + il] * p[ob + il];
+        if (ir >= 0) tq = tq + A[4 * NP + pos] * p[ob + ir];
+        if (ifr >= 0) tq = tq + A[5 * NP + pos] * p[ob + ifr];
+        if (k < L.nlay - 1) tq = tq + adn * pnext;
+        qq[pos] = tq;
+        sum = sum + (double)pcur * (double)tq;
+        aup = adn;
+        pprev = pcur;
+        pcur = pnext;
+      }
+    }
+    sum = bsum(sum, lds);
+    if (threadIdx.x == 0) part[P_PQ * T + tile] = sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-system scalar kernels (one block per system)
+// ---------------------------------------------------------------------------
+
+__device__ inline void testcnvg0(int *icnvg, double dvmax, double rmax, double dvclose,
+                                 double rclose) {
+  if (fabs(dvmax) <= dvclose && fabs(rmax) <= rclose) *icnvg = 1;
+}
+
+__device__ inline int is_close_b(double a, double b) {
+  if (a == b) return 1;
+  return fabs(a - b) <= fmax(100.0 * 2.220446049250313080847263336181640625e-16 *
+                                 fmax(fabs(a), fabs(b)), 0.0);
+}
+
+__global__ void __launch_bounds__(MF6_BLOCK) kb_round(const BSys *sys, KState *st, BFState *fs,
+                                                      const double *part, const MaxLoc *mpart,
+                                                      int T) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  const int s = blockIdx.x;
+  if (st[s].stop) return;
+  const BSys S = sys[s];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  MaxLoc mm[2] = {{0.0, -1}, {0.0, -1}};
+  for (int i = threadIdx.x; i < S.ntr; i += blockDim.x) {
+    a[0] = a[0] + part[P_SS_R * T + S.tr0 + i];
+    a[3] = a[3] + part[P_RHO_R * T + S.tr0 + i];
+    mm[0] = maxloc_pick(mm[0], mpart[M_D_R * T + S.tr0 + i]);
+  }
+  for (int i = threadIdx.x; i < S.ntb; i += blockDim.x) {
+    a[1] = a[1] + part[P_SS_B * T + S.tb0 + i];
+    a[2] = a[2] + part[P_RHO_B * T + S.tb0 + i];
+    mm[1] = maxloc_pick(mm[1], mpart[M_D_B * T + S.tb0 + i]);
+  }
+  const double ssr = bsum(a[0], lds);
+  const double ssb = bsum(a[1], lds);
+  const double rb = bsum(a[2], lds);
+  const double rr = bsum(a[3], lds);
+  const MaxLoc mdr = bmax(mm[0], mlds);
+  const MaxLoc mdb = bmax(mm[1], mlds);
+  if (threadIdx.x != 0) return;
+  KState &k = st[s];
+  BFState &f = fs[s];
+  const MaxLoc md = maxloc_pick(mdr, mdb);
+  const double rho_new = rb + rr;
+  const int it = k.iter;
+  f.apend_d = 0.0;
+  if (it > 0) {
+    const double deltax = k.alpha * f.pmax;
+    k.deltax = deltax;
+    k.xloc = f.ploc;
+    k.rmax = md.v;
+    k.rloc = md.loc;
+    k.l2norm = sqrt(ssr + ssb);
+    int icnvg = k.icnvg;
+    testcnvg0(&icnvg, deltax, md.v, k.dvclose, k.rclose);
+    if (md.v == 0.0) icnvg = 1;
+    k.icnvg = icnvg;
+    if (icnvg != 0) { k.stop = 1; return; }
+    if (is_close_b(k.rho, k.rho0)) { k.stop = 2; return; }
+    if (k.rho == 0.0) { k.stop = 2; return; }
+    k.rho0 = k.rho;
+    if (it >= k.itmax) { k.stop = 3; return; }
+  }
+  k.rho = rho_new;
+  if (it > 0) k.beta = rho_new / k.rho0;
+}
+
+__global__ void __launch_bounds__(MF6_BLOCK) kb_alpha(const BSys *sys, KState *st, BFState *fs,
+                                                      const double *part, const MaxLoc *mpart,
+                                                      int T) {
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  const int s = blockIdx.x;
+  if (st[s].stop) return;
+  const BSys S = sys[s];
+  double a = 0.0;
+  MaxLoc mm{0.0, -1};
+  for (int i = threadIdx.x; i < S.nti; i += blockDim.x) {
+    a = a + part[P_PQ * T + S.ti0 + i];
+    mm = maxloc_pick(mm, mpart[M_P * T + S.ti0 + i]);
+  }
+  const double pq = bsum(a, lds);
+  const MaxLoc mp = bmax(mm, mlds);
+  if (threadIdx.x != 0) return;
+  KState &k = st[s];
+  BFState &f = fs[s];
+  double den = pq;
+  den = den + copysign(2.220446049250313080847263336181640625e-16, den);
+  k.pq = pq;
+  k.alpha = k.rho / den;
+  k.iter = k.iter + 1;
+  f.apend_x = k.alpha;
+  f.apend_d = k.alpha;
+  f.pmax = mp.v;
+  f.ploc = mp.loc;
+  f.pcur = 1 - f.pcur;   // the direction just formed is the current one
+}
+
+// ---------------------------------------------------------------------------
+// residual replacement (every system, stopped or not)
+// ---------------------------------------------------------------------------
+
+// e += apend_x p_cur (the pending head update), x64 += e, e = 0
+__global__ void kb_fold(const BSys *sys, const Tile *tiles, int ntiles, int cpt, const float *p0,
+                        const float *p1, float *e, double *x, BFState *fs, const int *active) {
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (!active[s]) continue;
+    const Layout L = sys[s].L;
+    const long long vo = sys[s].voff;
+    const float *p = (fs[s].pcur ? p1 : p0) + vo;
+    const float ax = (float)fs[s].apend_x;
+    for (int c = 0; c < cpt; c++) {
+      const int t = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      const int cc = t & 1, h2 = t >> 1;
+      if (h2 >= ncols_of(L, cc)) continue;
+      const int own = cc * L.H + h2;
+      for (int k = 0; k < L.nlay; k++) {
+        const long long pos = vo + own + k * L.HP;
+        float ev = e[pos];
+        if (ax != 0.f) ev = ev + ax * p[own + k * L.HP];
+        x[pos] = x[pos] + (double)ev;
+        e[pos] = 0.f;
+      }
+    }
+  }
+}
+
+// d32 = b - A64 x64 (FP64, amux order) ; partial max |r| per tile
+__global__ void __launch_bounds__(MF6_BLOCK) kb_residual(const BSys *sys, const Tile *tiles,
+                                                         int ntiles, int cpt, const double *A64,
+                                                         const double *x, const double *b,
+                                                         float *d32, MaxLoc *mpart, int T,
+                                                         const int *active) {
+  __shared__ MaxLoc mlds[8];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tiles[tile].sys;
+    if (!active[s]) continue;
+    const Layout L = sys[s].L;
+    const long long vo = sys[s].voff;
+    const double *A = A64 + sys[s].aoff;
+    const double *xx = x + vo;
+    const int NP = L.NP;
+    MaxLoc m{0.0, -1};
+    for (int c = 0; c < cpt; c++) {
+      const int t = tiles[tile].base + c * blockDim.x + threadIdx.x;
+      const int cc = t & 1, h2 = t >> 1;
+      if (h2 >= ncols_of(L, cc)) continue;
+      const int l = col_l(L, cc, h2);
+      const int i = l / L.ncol, j = l - i * L.ncol;
+      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
+      const int il = j > 0 ? hn(l, -1) : -1;
+      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
+      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      const int own = cc * L.H + h2;
+      const int oth = (1 - cc) * L.H;
+      for (int k = 0; k < L.nlay; k++) {
+        const int pos = own + k * L.HP;
+        const int ob = oth + k * L.HP;
+        double t2 = 0.0;
+        t2 = t2 + A[pos] * xx[pos];
+        if (k > 0) t2 = t2 + A[NP + pos] * xx[pos - L.HP];
+        if (ib >= 0) t2 = t2 + A[2 * NP + pos] * xx[ob + ib];
+        if (il >= 0) t2 = t2 + A[3 * NP + pos] * xx[ob + il];
+        if (ir >= 0) t2 = t2 + A[4 * NP + pos] * xx[ob + ir];
+        if (ifr >= 0) t2 = t2 + A[5 * NP + pos] * xx[ob + ifr];
+        if (k < L.nlay - 1) t2 = t2 + A[6 * NP + pos] * xx[pos + L.HP];
+        const double r = b[vo + pos] - t2;
+        d32[vo + pos] = (float)r;
+        m = maxloc_pick(m, MaxLoc{r, k * L.NRC + l});
+      }
+    }
+    m = bmax(m, mlds);
+    if (threadIdx.x == 0) mpart[M_TRUE * T + tile] = m;
+  }
+}
+
+// per system: no pending update; the true max |r| into out[s]
+__global__ void __launch_bounds__(MF6_BLOCK) kb_after(const BSys *sys, BFState *fs,
+                                                      const MaxLoc *mpart, int T, double *out,
+                                                      const int *active) {
+  __shared__ MaxLoc mlds[8];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const BSys S = sys[s];
+  MaxLoc m{0.0, -1};
+  for (int i = threadIdx.x; i < S.nti; i += blockDim.x)
+    m = maxloc_pick(m, mpart[M_TRUE * T + S.ti0 + i]);
+  m = bmax(m, mlds);
+  if (threadIdx.x != 0) return;
+  fs[s].apend_d = 0.0;
+  fs[s].apend_x = 0.0;
+  out[s] = m.v;
+}
+
+__global__ void kb_convert(long long n, const double *a, float *b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct BatchHandle {
+  int nsys = 0, cap = 0;
+  std::vector<BSys> sys;
+  std::vector<mf6hip_ims_settings> set;
+  long long ptot = 0, vtot = 0;        // plane / vector lengths
+  double *A64 = nullptr, *b64 = nullptr, *x64 = nullptr, *x0 = nullptr, *piv64 = nullptr;
+  float *A32 = nullptr, *piv32 = nullptr, *d32 = nullptr, *z32 = nullptr, *zf32 = nullptr,
+        *q32 = nullptr, *p0 = nullptr, *p1 = nullptr, *e32 = nullptr;
+  BSys *dsys = nullptr;
+  Tile *dtr = nullptr, *dtb = nullptr, *dti = nullptr;
+  int ntr = 0, ntb = 0, nti = 0, T = 0;
+  double *part = nullptr;
+  MaxLoc *mpart = nullptr;
+  KState *st = nullptr;
+  BFState *fs = nullptr;
+  int *active = nullptr;
+  double *rtrue = nullptr;
+  KState *h_st = nullptr;
+  double *h_rtrue = nullptr;
+  int cpt = 1, grid = 0, R = 32;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool finalized = false;
+  double last_ms = 0.0;
+  std::vector<int> iters, icnvg;
+  std::string tune;
+};
+
+namespace {
+
+void batch_free_graph(BatchHandle *b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+template <typename T>
+int balloc(T **p, size_t n) {
+  if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+    mf6hip_set_error("hipMalloc failed (ensemble)");
+    return MF6HIP_EHIP;
+  }
+  if (hipMemset(*p, 0, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+    mf6hip_set_error("hipMemset failed (ensemble)");
+    return MF6HIP_EHIP;
+  }
+  return MF6HIP_OK;
+}
+
+// tile lists for a columns-per-thread value
+int build_tiles(BatchHandle *b, int cpt) {
+  std::vector<Tile> tr, tb, ti;
+  const int w = MF6_BLOCK * cpt;
+  for (int s = 0; s < b->nsys; s++) {
+    BSys &S = b->sys[s];
+    S.tr0 = (int)tr.size();
+    for (int h = 0; h < S.L.HP0; h += w) tr.push_back(Tile{s, h});
+    S.ntr = (int)tr.size() - S.tr0;
+    S.tb0 = (int)tb.size();
+    for (int h = 0; h < S.L.HP1; h += w) tb.push_back(Tile{s, h});
+    S.ntb = (int)tb.size() - S.tb0;
+    S.ti0 = (int)ti.size();
+    for (int h = 0; h < 2 * S.L.HP0; h += w) ti.push_back(Tile{s, h});
+    S.nti = (int)ti.size() - S.ti0;
+  }
+  b->ntr = (int)tr.size();
+  b->ntb = (int)tb.size();
+  b->nti = (int)ti.size();
+  int T = b->ntr;
+  if (b->ntb > T) T = b->ntb;
+  if (b->nti > T) T = b->nti;
+  if (b->dtr) hipFree(b->dtr);
+  if (b->dtb) hipFree(b->dtb);
+  if (b->dti) hipFree(b->dti);
+  if (b->part) hipFree(b->part);
+  if (b->mpart) hipFree(b->mpart);
+  b->dtr = b->dtb = b->dti = nullptr;
+  b->part = nullptr;
+  b->mpart = nullptr;
+  b->T = T;
+  BCK(balloc(&b->dtr, tr.size()));
+  BCK(balloc(&b->dtb, tb.size()));
+  BCK(balloc(&b->dti, ti.size()));
+  BCK(balloc(&b->part, 5 * (size_t)T));
+  BCK(balloc(&b->mpart, 4 * (size_t)T));
+  HIPCHK(hipMemcpy(b->dtr, tr.data(), sizeof(Tile) * tr.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dtb, tb.data(), sizeof(Tile) * tb.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dti, ti.data(), sizeof(Tile) * ti.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dsys, b->sys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice));
+  b->cpt = cpt;
+  return MF6HIP_OK;
+}
+
+// one CG iteration for every running system
+void launch_iteration(BatchHandle *b) {
+  hipStream_t s = b->stream;
+  const int g = b->grid, T = b->T, c = b->cpt;
+  kb_red_fwd<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtr, b->ntr, c, b->A32, b->piv32, b->d32, b->q32,
+                                     b->zf32, b->part, b->mpart, T, b->st, b->fs);
+  kb_black<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtb, b->ntb, c, b->A32, b->piv32, b->d32, b->q32,
+                                   b->zf32, b->z32, b->part, b->mpart, T, b->st, b->fs);
+  kb_red_bwd<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtr, b->ntr, c, b->A32, b->piv32, b->d32,
+                                     b->zf32, b->z32, b->part, T, b->st);
+  kb_round<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->st, b->fs, b->part, b->mpart, T);
+  kb_pupdate<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, c, b->z32, b->p0, b->p1, b->e32,
+                                     b->mpart, T, b->st, b->fs);
+  kb_spmv<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, c, b->A32, b->p0, b->p1, b->q32,
+                                  b->part, T, b->st, b->fs);
+  kb_alpha<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->st, b->fs, b->part, b->mpart, T);
+}
+
+// fold the FP32 correction into the FP64 heads and replace the residual
+void launch_replacement(BatchHandle *b) {
+  hipStream_t s = b->stream;
+  const int g = b->grid;
+  kb_fold<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, b->cpt, b->p0, b->p1, b->e32, b->x64,
+                                  b->fs, b->active);
+  kb_residual<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, b->cpt, b->A64, b->x64, b->b64,
+                                      b->d32, b->mpart, b->T, b->active);
+  kb_after<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->fs, b->mpart, b->T, b->rtrue, b->active);
+}
+
+int capture(BatchHandle *b) {
+  batch_free_graph(b);
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < b->R; i++) launch_iteration(b);
+  launch_replacement(b);
+  hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+  if (e != hipSuccess) { mf6hip_set_error("graph capture failed"); return MF6HIP_EHIP; }
+  HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  return MF6HIP_OK;
+}
+
+// initial state of every system: true residual, KState from the settings
+int reset_states(BatchHandle *b) {
+  for (int s = 0; s < b->nsys; s++) {
+    KState &k = b->h_st[s];
+    memset(&k, 0, sizeof(KState));
+    k.dvclose = b->set[s].dvclose;
+    k.rclose = b->set[s].rclose;
+    k.itmax = b->set[s].iter1;
+    k.icnvgopt = 0;
+    k.epfact = 1.0;
+  }
+  HIPCHK(hipMemcpyAsync(b->st, b->h_st, sizeof(KState) * b->nsys, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemsetAsync(b->fs, 0, sizeof(BFState) * b->nsys, b->stream));
+  HIPCHK(hipMemsetAsync(b->q32, 0, sizeof(float) * b->vtot, b->stream));
+  HIPCHK(hipMemsetAsync(b->e32, 0, sizeof(float) * b->vtot, b->stream));
+  std::vector<int> act(b->nsys, 1);
+  HIPCHK(hipMemcpyAsync(b->active, act.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice, b->stream));
+  launch_replacement(b);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return MF6HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mf6hip_batch_create(int capacity, void **handle) {
+  if (!handle || capacity < 1) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
+  BatchHandle *b = new BatchHandle();
+  b->cap = capacity;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    mf6hip_set_error("hipStreamCreate failed");
+    return MF6HIP_EHIP;
+  }
+  *handle = b;
+  return MF6HIP_OK;
+}
+
+// Register a system (shape + settings); storage is allocated by finalize.
+extern "C" int mf6hip_batch_add(void *handle, const mf6hip_ims_settings *settings) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b || !settings) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  if (b->finalized) { mf6hip_set_error("ensemble already finalized"); return MF6HIP_EINVAL; }
+  if (b->nsys >= b->cap) { mf6hip_set_error("ensemble capacity exceeded"); return MF6HIP_EINVAL; }
+  const mf6hip_ims_settings &st = *settings;
+  if (st.ilinmeth != 1 || st.iscl || st.iord || st.level > 0 || st.droptol > 0.0 ||
+      st.icnvgopt != 0 || st.north != 0) {
+    mf6hip_set_error("ensemble systems must use CG + ILU0/MILU0 with the default convergence test");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  if (st.nlay < 1 || st.nlay > 32 || st.nrow < 1 || st.ncol < 1 ||
+      (long long)st.nlay * st.nrow * st.ncol > (1LL << 28)) {
+    mf6hip_set_error("bad DIS shape");
+    return MF6HIP_EINVAL;
+  }
+  BSys S{};
+  S.L = make_layout(st.nlay, st.nrow, st.ncol);
+  S.aoff = b->ptot;
+  S.voff = b->vtot;
+  b->ptot += (long long)MF6_NSLOT * S.L.NP;
+  b->vtot += S.L.NP;
+  b->sys.push_back(S);
+  b->set.push_back(st);
+  b->nsys++;
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_batch_finalize(void *handle) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b || b->nsys < 1) { mf6hip_set_error("empty ensemble"); return MF6HIP_EINVAL; }
+  if (b->finalized) return MF6HIP_OK;
+  BCK(balloc(&b->A64, b->ptot));
+  BCK(balloc(&b->A32, b->ptot));
+  const size_t v = b->vtot;
+  BCK(balloc(&b->b64, v));
+  BCK(balloc(&b->x64, v));
+  BCK(balloc(&b->x0, v));
+  BCK(balloc(&b->piv64, v));
+  BCK(balloc(&b->piv32, v));
+  BCK(balloc(&b->d32, v));
+  BCK(balloc(&b->z32, v));
+  BCK(balloc(&b->zf32, v));
+  BCK(balloc(&b->q32, v));
+  BCK(balloc(&b->p0, v));
+  BCK(balloc(&b->p1, v));
+  BCK(balloc(&b->e32, v));
+  BCK(balloc(&b->dsys, b->nsys));
+  BCK(balloc(&b->st, b->nsys));
+  BCK(balloc(&b->fs, b->nsys));
+  BCK(balloc(&b->active, b->nsys));
+  BCK(balloc(&b->rtrue, b->nsys));
+  HIPCHK(hipHostMalloc((void **)&b->h_st, sizeof(KState) * b->nsys, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&b->h_rtrue, sizeof(double) * b->nsys, hipHostMallocDefault));
+  BCK(build_tiles(b, 1));
+  int ncu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0)
+    ncu = prop.multiProcessorCount;
+  b->grid = 8 * ncu;
+  b->finalized = true;
+  return MF6HIP_OK;
+}
+
+// System data in the reference layout (host): 1-based DIS CSR, amat after
+// the sln_ls row fixes, rhs, start heads.
+extern "C" int mf6hip_batch_set_system(void *handle, int isys, const int *ia, int nja,
+                                       const double *amat, const double *rhs, const double *x0) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b || !b->finalized || !ia || !amat || !rhs || !x0) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
+  if (isys < 0 || isys >= b->nsys) { mf6hip_set_error("system index out of range"); return MF6HIP_EINVAL; }
+  const BSys &S = b->sys[isys];
+  if (ia[S.L.N] - 1 != nja) { mf6hip_set_error("IA / NJA mismatch"); return MF6HIP_EINVAL; }
+  BCK(mf6hip_stage_system(S.L, ia, nja, amat, rhs, x0, b->set[isys].relax, b->A64 + S.aoff,
+                          b->b64 + S.voff, b->x64 + S.voff, b->piv64 + S.voff, b->stream));
+  HIPCHK(hipMemcpyAsync(b->x0 + S.voff, b->x64 + S.voff, sizeof(double) * S.L.NP,
+                        hipMemcpyDeviceToDevice, b->stream));
+  const long long np = (long long)MF6_NSLOT * S.L.NP;
+  kb_convert<<<1024, 256, 0, b->stream>>>(np, b->A64 + S.aoff, b->A32 + S.aoff);
+  kb_convert<<<256, 256, 0, b->stream>>>(S.L.NP, b->piv64 + S.voff, b->piv32 + S.voff);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return MF6HIP_OK;
+}
+
+// Autotune (columns per thread, persistent grid) on the loaded systems:
+// time one replay of R iterations per candidate from the start state.
+extern "C" int mf6hip_batch_autotune(void *handle, char *report, int report_len) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b || !b->finalized) { mf6hip_set_error("ensemble not finalized"); return MF6HIP_EINVAL; }
+  int ncu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0)
+    ncu = prop.multiProcessorCount;
+  // the trial replays start from the staged start heads
+  const double *xsave = b->x0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int cpts[2] = {1, 2};
+  const int gmul[3] = {4, 8, 16};
+  float best = 1e30f;
+  int bc = 1, bg = 8 * ncu;
+  std::string rep;
+  for (int ci = 0; ci < 2; ci++) {
+    BCK(build_tiles(b, cpts[ci]));
+    for (int gi = 0; gi < 3; gi++) {
+      b->grid = gmul[gi] * ncu;
+      HIPCHK(hipMemcpy(b->x64, xsave, sizeof(double) * b->vtot, hipMemcpyDeviceToDevice));
+      BCK(reset_states(b));
+      BCK(capture(b));
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));   // warm
+      HIPCHK(hipStreamSynchronize(b->stream));
+      HIPCHK(hipMemcpy(b->x64, xsave, sizeof(double) * b->vtot, hipMemcpyDeviceToDevice));
+      BCK(reset_states(b));
+      HIPCHK(hipEventRecord(e0, b->stream));
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+      HIPCHK(hipEventRecord(e1, b->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      rep += "cpt " + std::to_string(cpts[ci]) + " grid " + std::to_string(b->grid) + ": " +
+             std::to_string(ms) + " ms; ";
+      if (ms < best) { best = ms; bc = cpts[ci]; bg = b->grid; }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  BCK(build_tiles(b, bc));
+  b->grid = bg;
+  HIPCHK(hipMemcpy(b->x64, xsave, sizeof(double) * b->vtot, hipMemcpyDeviceToDevice));
+  batch_free_graph(b);
+  rep += "chosen cpt " + std::to_string(bc) + " grid " + std::to_string(bg);
+  b->tune = rep;
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return MF6HIP_OK;
+}
+
+// Solve every system: replays of the captured graph (R iterations + the
+// FP64 residual replacement) until each system has converged (FP32 test
+// passed and the FP64 true residual within RCLOSE) or reached INNER_MAXIMUM.
+extern "C" int mf6hip_batch_solve(void *handle, int *icnvg, int *iters, double *solve_ms) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b || !b->finalized || !icnvg || !iters) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, b->stream));
+  // every solve starts from the staged start heads
+  HIPCHK(hipMemcpyAsync(b->x64, b->x0, sizeof(double) * b->vtot, hipMemcpyDeviceToDevice,
+                        b->stream));
+  BCK(reset_states(b));
+  BCK(capture(b));
+  std::vector<int> done(b->nsys, 0), act(b->nsys, 1);
+  for (int s = 0; s < b->nsys; s++) { icnvg[s] = 0; iters[s] = 0; }
+  int ndone = 0, replays = 0;
+  const int max_replays = 1000000;
+  while (ndone < b->nsys && replays < max_replays) {
+    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    replays++;
+    HIPCHK(hipMemcpyAsync(b->h_st, b->st, sizeof(KState) * b->nsys, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(b->h_rtrue, b->rtrue, sizeof(double) * b->nsys, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    bool changed = false;
+    for (int s = 0; s < b->nsys; s++) {
+      if (done[s]) continue;
+      KState &k = b->h_st[s];
+      const bool true_ok = fabs(b->h_rtrue[s]) <= b->set[s].rclose;
+      if (k.stop == 0 && k.iter < k.itmax) continue;
+      if (k.stop == 1 && true_ok) { icnvg[s] = 1; done[s] = 1; }
+      else if (k.stop == 2 && true_ok && fabs(k.deltax) <= b->set[s].dvclose) { icnvg[s] = 1; done[s] = 1; }
+      else if (k.stop == 2 || k.stop == 3 || k.iter >= k.itmax) { done[s] = 1; }
+      else {
+        // FP32 test passed, the true residual not: continue from it
+        k.stop = 0;
+        k.icnvg = 0;
+        changed = true;
+      }
+      if (done[s]) {
+        iters[s] = k.iter;
+        ndone++;
+        act[s] = 0;
+        k.stop = k.stop ? k.stop : 3;
+        changed = true;
+      }
+    }
+    if (changed) {
+      HIPCHK(hipMemcpyAsync(b->st, b->h_st, sizeof(KState) * b->nsys, hipMemcpyHostToDevice, b->stream));
+      HIPCHK(hipMemcpyAsync(b->active, act.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice, b->stream));
+    }
+  }
+  for (int s = 0; s < b->nsys; s++)
+    if (!done[s]) iters[s] = b->h_st[s].iter;
+  HIPCHK(hipEventRecord(e1, b->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  b->last_ms = ms;
+  if (solve_ms) *solve_ms = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return MF6HIP_OK;
+}
+
+// Heads of one system (natural order, host).
+extern "C" int mf6hip_batch_get_x(void *handle, int isys, double *x) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b || !b->finalized || !x) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
+  if (isys < 0 || isys >= b->nsys) { mf6hip_set_error("system index out of range"); return MF6HIP_EINVAL; }
+  const Layout &L = b->sys[isys].L;
+  std::vector<double> tmp(L.NP);
+  HIPCHK(hipMemcpy(tmp.data(), b->x64 + b->sys[isys].voff, sizeof(double) * L.NP, hipMemcpyDeviceToHost));
+  for (int n = 0; n < L.N; n++) x[n] = tmp[pos_of_natural(L, n)];
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_batch_destroy(void *handle) {
+  BatchHandle *b = (BatchHandle *)handle;
+  if (!b) return MF6HIP_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_free_graph(b);
+  void *bufs[] = {b->A64, b->b64, b->x64, b->x0, b->piv64, b->A32, b->piv32, b->d32, b->z32, b->zf32,
+                  b->q32, b->p0, b->p1, b->e32, b->dsys, b->dtr, b->dtb, b->dti, b->part,
+                  b->mpart, b->st, b->fs, b->active, b->rtrue};
+  for (void *p : bufs)
+    if (p) hipFree(p);
+  if (b->h_st) hipHostFree(b->h_st);
+  if (b->h_rtrue) hipHostFree(b->h_rtrue);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return MF6HIP_OK;
+}

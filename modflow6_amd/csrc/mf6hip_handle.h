@@ -129,6 +129,9 @@ int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa,
 int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z);
 int mf6hip_spmv64(Mf6Handle *h, const double *x, double *y);
 int mf6hip_run_fcg(Mf6Handle *h);
+int mf6hip_stage_system(const Layout &L, const int *ia1, int nja, const double *amat,
+                        const double *rhs, const double *x0, double relax, double *A64,
+                        double *b64, double *x64, double *piv64, hipStream_t s);
 // network entry points used by the solver (mf6hip_unet.hip)
 int mf6hip_unet_apply_device(void *net, hipStream_t s);
 unsigned short *mf6hip_unet_input(void *net);

@@ -942,6 +942,59 @@ static void launch_spmv(Mf6Handle *h, const TM *A, const TV *x, TV *y) {
   k_spmv<TM, TV><<<grid_for(h->L.NP), MF6_BLOCK, 0, h->stream>>>(h->L, A, x, y);
 }
 
+// One system of an ensemble into device planes: CSR (reference layout,
+// host) -> 7 planes, rhs / start heads -> positions, ILU0 / MILU0 factor.
+// The plane and vector ranges must be zeroed by the caller (padding).
+int mf6hip_stage_system(const Layout &L, const int *ia1, int nja, const double *amat,
+                        const double *rhs, const double *x0, double relax, double *A64,
+                        double *b64, double *x64, double *piv64, hipStream_t s) {
+  int *dia = nullptr, *dfail = nullptr;
+  double *dcsr = nullptr, *dvec = nullptr, *dtmp = nullptr;
+  std::vector<int> ia0(L.N + 1);
+  for (int n = 0; n <= L.N; n++) ia0[n] = ia1[n] - 1;
+  int rc = MF6HIP_OK;
+  if (hipMalloc((void **)&dia, sizeof(int) * (L.N + 1)) != hipSuccess ||
+      hipMalloc((void **)&dfail, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&dcsr, sizeof(double) * nja) != hipSuccess ||
+      hipMalloc((void **)&dvec, sizeof(double) * L.N) != hipSuccess ||
+      hipMalloc((void **)&dtmp, sizeof(double) * L.NP) != hipSuccess) {
+    mf6hip_set_error("hipMalloc failed (ensemble staging)");
+    rc = MF6HIP_EHIP;
+  }
+  const int big = 0x7fffffff;
+  if (!rc && (hipMemcpyAsync(dia, ia0.data(), sizeof(int) * (L.N + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+              hipMemcpyAsync(dcsr, amat, sizeof(double) * nja, hipMemcpyHostToDevice, s) != hipSuccess ||
+              hipMemcpyAsync(dfail, &big, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+              hipMemsetAsync(dtmp, 0, sizeof(double) * L.NP, s) != hipSuccess))
+    rc = MF6HIP_EHIP;
+  if (!rc) {
+    k_csr_to_planes<<<grid_for(L.N), MF6_BLOCK, 0, s>>>(L, dia, dcsr, A64);
+    if (hipMemcpyAsync(dvec, rhs, sizeof(double) * L.N, hipMemcpyHostToDevice, s) != hipSuccess) rc = MF6HIP_EHIP;
+  }
+  if (!rc) {
+    k_nat_to_pos<double, double><<<grid_for(L.N), MF6_BLOCK, 0, s>>>(L, dvec, b64);
+    // the vector staging buffer is reused: wait for the rhs conversion
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpyAsync(dvec, x0, sizeof(double) * L.N, hipMemcpyHostToDevice, s) != hipSuccess)
+      rc = MF6HIP_EHIP;
+  }
+  if (!rc) {
+    k_nat_to_pos<double, double><<<grid_for(L.N), MF6_BLOCK, 0, s>>>(L, dvec, x64);
+    int g = (L.HP + MF6_BLOCK - 1) / MF6_BLOCK;
+    k_ilu_factor_red<<<g, MF6_BLOCK, 0, s>>>(L, A64, relax, dtmp, dfail);
+    k_ilu_factor_black<<<g, MF6_BLOCK, 0, s>>>(L, A64, relax, dtmp, dfail);
+    k_ilu_commit<<<grid_for(L.NP), MF6_BLOCK, 0, s>>>(L, dtmp, piv64, dfail);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = MF6HIP_EHIP;
+  }
+  if (rc == MF6HIP_EHIP) mf6hip_set_error("ensemble staging failed");
+  if (dia) hipFree(dia);
+  if (dfail) hipFree(dfail);
+  if (dcsr) hipFree(dcsr);
+  if (dvec) hipFree(dvec);
+  if (dtmp) hipFree(dtmp);
+  return rc;
+}
+
 int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z) {
   launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, r, h->zf, z, nullptr);
   HIPCHK(hipGetLastError());
