@@ -418,8 +418,10 @@ __device__ inline void testcnvg_(int icnvgopt, int *icnvg, int iiter, double dvm
 
 // End of a round: test the iteration whose residual the preconditioner
 // kernels just produced, then rho / beta of the next iteration.
+// Colour partials sit at part[slot * nblk + b] for b < ncol (the colour
+// kernels' grid).
 __global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
-                          int nblk, double *hist, int *hloc) {
+                          int nblk, int ncol, double *hist, int *hloc) {
   __shared__ double lds[4][MF6_BLOCK / 64];
   __shared__ MaxLoc mlds[2][MF6_BLOCK / 64];
   if (st->stop) return;
@@ -427,7 +429,7 @@ __global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxL
   // flight together); per slot the summation order is that of rsum/rmaxl
   double a[4] = {0.0, 0.0, 0.0, 0.0};
   MaxLoc mm[2] = {{0.0, -1}, {0.0, -1}};
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+  for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
     a[0] = a[0] + part[P_SS_R * nblk + i];
     a[1] = a[1] + part[P_SS_B * nblk + i];
     a[2] = a[2] + part[P_RHO_B * nblk + i];
@@ -605,16 +607,20 @@ static inline int gridc(int n) {
   return g < 1 ? 1 : g;
 }
 
+// The colour kernels run on nbc blocks (one thread per column of one
+// colour); their partials use the slot stride nb of the interleaved kernels.
+// A grid sized for both colours would leave the XCDs that receive the upper
+// half of the logical blocks idle.
 template <typename TM, typename TV, typename TP>
 static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV *q, TV *zf,
-                      TV *z, KState *st, FState *fs, int nb) {
+                      TV *z, KState *st, FState *fs, int nbc, int nb) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
-  k_f_red_fwd<TM, TV, TP><<<nb, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
-                                                   st, fs, nb);
-  k_f_black<TM, TV, TP><<<nb, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
-                                                 st, fs, nb);
-  k_f_red_bwd<TM, TV, TP><<<nb, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb);
+  k_f_red_fwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
+                                                    st, fs, nb);
+  k_f_black<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
+                                                  st, fs, nb);
+  k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb);
 }
 
 static int read_states(Mf6Handle *h) {
@@ -645,8 +651,8 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
   while (true) {
     for (int it = 0; it < batch; it++, round++) {
       launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs,
-                                        nb);
-      k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, h->d_hist,
+                                        nbc, nb);
+      k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                         h->d_hloc);
       double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
       k_f_pupdate<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->z, pin, pout, h->x, h->d_mpart,
@@ -705,8 +711,8 @@ static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters
     while (true) {
       for (int it = 0; it < batch; it++, round++) {
         launch_pc<float, float, TP>(h, h->A32, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs,
-                                    nb);
-        k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, h->d_hist,
+                                    nbc, nb);
+        k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                           h->d_hloc);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
         k_f_pupdate<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->z32, pin, pout, h->e32,
