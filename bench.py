@@ -205,7 +205,8 @@ def ensemble_workload(args):
 def learned_workload(args):
     from modflow6_amd import learned
     summary, params = learned.run_workload(ntrain=args.ntrain, ntest=args.ntest,
-                                           epochs=args.epochs, log=log)
+                                           epochs=args.epochs, scale_up=args.scale_up,
+                                           scale_up_iters=args.scale_up_iters, log=log)
     os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
     np.save(os.path.join(ROOT, "gpurun_out", "unet_params.npy"), params)
     print(json.dumps(summary))
@@ -229,6 +230,10 @@ def main():
     ap.add_argument("--ntrain", type=int, default=64)
     ap.add_argument("--ntest", type=int, default=16)
     ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--scale-up", action="store_true",
+                    help="learned: also solve the 10 x 1000 x 1000 model with the network")
+    ap.add_argument("--scale-up-iters", type=int, default=20000,
+                    help="learned: iteration cap of the 10M-cell flexible-CG solve")
     ap.add_argument("--nsys", type=int, default=256, help="ensemble: realisations")
     ap.add_argument("--check", type=int, default=16,
                     help="ensemble: realisations whose true residual is checked")

@@ -282,7 +282,7 @@ def solve_pair(case, net=None, fcg_m=8, iter1=None):
 
 
 def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=6, batch=4, lr=1e-3,
-                 fcg_m=8, scale_up=True, log=print):
+                 fcg_m=8, scale_up=True, scale_up_iters=20000, log=print):
     """Train on `ntrain` seeded fields, evaluate on `ntest` unseen seeds, and
     apply the trained network to the 10 x 1000 x 1000 steady model."""
     t0 = time.perf_counter()
@@ -327,7 +327,7 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=6, batch=4, l
     if scale_up:
         case = configs.steady_10m()
         big = UNet(case.nlay, case.nrow, case.ncol, nin=3, channels=CHANNELS, params=params)
-        r = solve_pair(case, big, fcg_m, iter1=20000)
+        r = solve_pair(case, big, fcg_m, iter1=scale_up_iters)
         big.destroy()
         summary["scale_up_10m"] = r
         log(f"10M cells: PCG {r['pcg']['iterations']} it {r['pcg']['s']:.2f} s, FCG+net "
