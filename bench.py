@@ -206,7 +206,9 @@ def learned_workload(args):
     from modflow6_amd import learned
     summary, params = learned.run_workload(ntrain=args.ntrain, ntest=args.ntest,
                                            epochs=args.epochs, scale_up=args.scale_up,
-                                           scale_up_iters=args.scale_up_iters, log=log)
+                                           scale_up_iters=args.scale_up_iters,
+                                           nn_every=[int(v) for v in args.nn_every.split(",")],
+                                           log=log)
     os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
     np.save(os.path.join(ROOT, "gpurun_out", "unet_params.npy"), params)
     print(json.dumps(summary))
@@ -229,7 +231,9 @@ def main():
                          "learned preconditioner")
     ap.add_argument("--ntrain", type=int, default=64)
     ap.add_argument("--ntest", type=int, default=16)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--nn-every", default="1,2,4",
+                    help="learned: network application periods evaluated (first is headline)")
     ap.add_argument("--scale-up", action="store_true",
                     help="learned: also solve the 10 x 1000 x 1000 model with the network")
     ap.add_argument("--scale-up-iters", type=int, default=20000,

@@ -274,8 +274,10 @@ int mf6hip_unet_train(void *net, int nsteps, int batch, const int *order,
 /* Attach a trained network (3 inputs) to a solver created with
  * preconditioner = 1; features = 2 planes (standardised ln K and
  * ln(K33 / K)) in DIS node order; fcg_m = directions the flexible CG keeps
- * (1..64).  net = NULL detaches (the flexible CG then runs on ILU0). */
-int mf6hip_attach_unet(void *handle, void *net, const float *features, int fcg_m);
+ * (1..64); the network corrects every nn_every-th iteration's ILU0 step.
+ * net = NULL detaches (the flexible CG then runs on ILU0). */
+int mf6hip_attach_unet(void *handle, void *net, const float *features, int fcg_m,
+                       int nn_every);
 /* Iterations of the last solve that used the learned correction (the rest
  * ran on the ILU0 fallback). */
 int mf6hip_last_learned_iterations(void *handle, int *n);

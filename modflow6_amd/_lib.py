@@ -152,7 +152,7 @@ def load():
     L.mf6hip_unet_grad.argtypes = [vp, fp, fp, fp, fp]
     L.mf6hip_unet_set_data.argtypes = [vp, i, fp]
     L.mf6hip_unet_train.argtypes = [vp, i, i, ip, C.c_float, fp]
-    L.mf6hip_attach_unet.argtypes = [vp, vp, fp, i]
+    L.mf6hip_attach_unet.argtypes = [vp, vp, fp, i, i]
     L.mf6hip_last_learned_iterations.argtypes = [vp, ip]
     L.mf6hip_batch_create.argtypes = [i, C.POINTER(vp)]
     L.mf6hip_batch_add.argtypes = [vp, C.POINTER(ImsSettingsC)]

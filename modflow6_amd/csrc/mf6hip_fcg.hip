@@ -11,10 +11,11 @@
 // (ImsLinearBase.f90:1101-1146) is applied to every iteration with the signed
 // largest head change alpha p and residual; when it passes, the FP64 true
 // residual b - A x is recomputed and must pass it too (else the iteration
-// continues from the true residual).  Fallback: the ILU0-only step replaces
-// the learned one for the rest of the solve when z.r changes sign against
-// the ILU0 step (not a descent direction) or when ten learned iterations
-// reduce ||r||_2 by less than 10 %.
+// continues from the true residual).  The network runs on every nn_every-th
+// iteration (ILU0 alone on the others).  Fallback: the ILU0-only step
+// replaces the learned one for the rest of the solve when z.r changes sign
+// against the ILU0 step (not a descent direction) or when ten applications
+// of the network (10 nn_every iterations) reduce ||r||_2 by less than 10 %.
 #include <cmath>
 #include <vector>
 
@@ -301,7 +302,8 @@ int mf6hip_run_fcg(Mf6Handle *h) {
     it++;
     int used = 0;
     double zr_ilu = 0.0;
-    FCK(precondition(h, r, z, learned, &used, &zr_ilu));
+    const bool use_net = learned && ((it - 1) % h->nn_every == 0);
+    FCK(precondition(h, r, z, use_net, &used, &zr_ilu));
     double zr;
     FCK(dots(h, 1, z, r, nullptr, nullptr, nullptr, nullptr, &zr));
     if (used && !(zr * zr_ilu > 0.0)) {
@@ -366,10 +368,10 @@ int mf6hip_run_fcg(Mf6Handle *h) {
       for (double v : tmp) tmax = fmax(tmax, fabs(v));
       if (tmax <= h->s.rclose || h->s.icnvgopt == 3) { icnvg = 1; break; }
     }
-    // stall check of the learned step over windows of ten iterations
+    // stall check of the learned step over windows of ten applications
     if (learned) {
       if (window == 0) rn_window = rnorm;
-      if (++window == 10) {
+      if (++window == 10 * h->nn_every) {
         if (!(rnorm < 0.9 * rn_window)) learned = false;
         window = 0;
       }
@@ -382,10 +384,13 @@ int mf6hip_run_fcg(Mf6Handle *h) {
   return MF6HIP_OK;
 }
 
-extern "C" int mf6hip_attach_unet(void *handle, void *net, const float *features, int fcg_m) {
+extern "C" int mf6hip_attach_unet(void *handle, void *net, const float *features, int fcg_m,
+                                  int nn_every) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h) { mf6hip_set_error("null handle"); return MF6HIP_EINVAL; }
   if (fcg_m < 1 || fcg_m > 64) { mf6hip_set_error("fcg_m must be 1..64"); return MF6HIP_EINVAL; }
+  if (nn_every < 1) { mf6hip_set_error("nn_every must be >= 1"); return MF6HIP_EINVAL; }
+  h->nn_every = nn_every;
   if (fcg_m != h->fcg_m && h->fcg_store) {
     hipFree(h->fcg_store);
     h->fcg_store = nullptr;

@@ -63,6 +63,7 @@ struct Mf6Handle {
   float *unet_feat = nullptr;    // [2][N] features, natural order
   int learned_iters = 0;         // iterations of the last solve that used it
   int fcg_m = 8;                 // flexible CG: directions kept
+  int nn_every = 1;              // flexible CG: network on every nn_every-th iteration
   double *fcg_store = nullptr;   // [2][fcg_m][NP]: directions p_j and A p_j
   // mixed precision (FP32 inner solve)
   float *A32 = nullptr, *invpiv32 = nullptr;

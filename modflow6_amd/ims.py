@@ -146,12 +146,12 @@ class HipLinearSolver:
         ia, ja, nlay, nrow, ncol = self._structure
         return self.initialize(ia, ja, self.settings, nlay, nrow, ncol)
 
-    def attach_unet(self, net, feat, fcg_m=8):
+    def attach_unet(self, net, feat, fcg_m=8, nn_every=1):
         """Attach a trained network (learned.UNet) with its two feature
         planes; used when settings.preconditioner == 1."""
         f = None if feat is None else np.ascontiguousarray(feat, dtype=np.float32)
         _lib.check(self.L.mf6hip_attach_unet(self.handle, net.h if net else None,
-                                             _lib.fptr(f), int(fcg_m)),
+                                             _lib.fptr(f), int(fcg_m), int(nn_every)),
                    "mf6hip_attach_unet")
 
     def learned_iterations(self):

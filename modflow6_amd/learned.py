@@ -249,7 +249,7 @@ def make_dataset(seeds, shape=(16, 256, 256), iterates=(25, 50, 100, 200), log=p
     return np.stack(out)
 
 
-def solve_pair(case, net=None, fcg_m=8, iter1=None):
+def solve_pair(case, net=None, fcg_m=8, iter1=None, nn_every=1):
     """One steady solve of `case`: FP64 ILU0-PCG (the device fused CG) and
     flexible CG with the network (when given).  Returns a dict of iteration
     counts, device times and head differences."""
@@ -267,10 +267,13 @@ def solve_pair(case, net=None, fcg_m=8, iter1=None):
                                       _settings(case, preconditioner=1,
                                                 iter1=iter1 or case.ims.inner_maximum),
                                       case.nlay, case.nrow, case.ncol)
-    for name, n in (("fcg_ilu0", None), ("fcg_learned", net)):
-        if name == "fcg_learned" and net is None:
-            continue
-        sv.attach_unet(n, sy.feat if n is not None else None, fcg_m)
+    periods = [nn_every] if isinstance(nn_every, int) else list(nn_every)
+    runs = [("fcg_ilu0", None, 1)]
+    if net is not None:
+        runs += [("fcg_learned" if k == periods[0] else f"fcg_learned_every{k}", net, k)
+                 for k in periods]
+    for name, n, k in runs:
+        sv.attach_unet(n, sy.feat if n is not None else None, fcg_m, k)
         x = sy.x0.copy()
         t0 = time.perf_counter()
         ic, it = sv.solve(1, sy.amat, sy.rhs, x)
@@ -281,8 +284,8 @@ def solve_pair(case, net=None, fcg_m=8, iter1=None):
     return res
 
 
-def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=6, batch=4, lr=1e-3,
-                 fcg_m=8, scale_up=True, scale_up_iters=20000, log=print):
+def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, lr=1e-3,
+                 fcg_m=8, scale_up=True, scale_up_iters=20000, nn_every=1, log=print):
     """Train on `ntrain` seeded fields, evaluate on `ntest` unseen seeds, and
     apply the trained network to the 10 x 1000 x 1000 steady model."""
     t0 = time.perf_counter()
@@ -297,7 +300,9 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=6, batch=4, l
     chunk = max(1, 64 // batch)
     for s0 in range(0, nsteps, chunk):
         n = min(chunk, nsteps - s0)
-        losses.extend(net.train(n, batch, lr * (0.5 if s0 > nsteps // 2 else 1.0), seed=s0))
+        # cosine decay of the learning rate to 5 % over the run
+        lr_s = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * s0 / nsteps)))
+        losses.extend(net.train(n, batch, lr_s, seed=s0))
         if (s0 // chunk) % 8 == 0:
             log(f"train: step {s0 + n}/{nsteps}, loss {np.mean(losses[-n:]):.4f}, "
                 f"{time.perf_counter() - t1:.0f} s")
@@ -305,16 +310,17 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=6, batch=4, l
     params = net.get_params()
     evals = []
     for seed in test_seeds:
-        r = solve_pair(learned_case(seed, *shape), net, fcg_m)
+        r = solve_pair(learned_case(seed, *shape), net, fcg_m, nn_every=nn_every)
         evals.append(r)
         log(f"test seed {seed}: PCG {r['pcg']['iterations']} it {r['pcg']['s']:.3f} s, "
-            f"FCG+ILU0 {r['fcg_ilu0']['iterations']} it, FCG+net "
-            f"{r['fcg_learned']['iterations']} it {r['fcg_learned']['s']:.3f} s "
-            f"({r['fcg_learned']['learned_iterations']} learned)")
+            f"FCG+ILU0 {r['fcg_ilu0']['iterations']} it, " + ", ".join(
+                f"{k} {v['iterations']} it {v['s']:.3f} s ({v['learned_iterations']} learned)"
+                for k, v in r.items() if k.startswith("fcg_learned")))
     net.destroy()
     summary = dict(
         train_fields=ntrain, test_fields=ntest, samples=int(data.shape[0]),
         shape=list(shape), epochs=epochs, batch=batch, train_steps=nsteps, train_s=train_s,
+        nn_every=nn_every,
         loss_first=float(np.mean(losses[:10])), loss_last=float(np.mean(losses[-10:])),
         pcg_iterations=float(np.mean([e["pcg"]["iterations"] for e in evals])),
         fcg_ilu0_iterations=float(np.mean([e["fcg_ilu0"]["iterations"] for e in evals])),
@@ -324,10 +330,16 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=6, batch=4, l
         max_head_diff_vs_pcg_m=float(max(e["fcg_learned"]["max_head_diff_vs_pcg_m"]
                                          for e in evals)),
         all_converged=all(e["fcg_learned"]["converged"] for e in evals))
+    for k in evals[0]:
+        if k.startswith("fcg_learned_every"):
+            summary[k] = dict(
+                iterations=float(np.mean([e[k]["iterations"] for e in evals])),
+                s=float(np.mean([e[k]["s"] for e in evals])),
+                all_converged=all(e[k]["converged"] for e in evals))
     if scale_up:
         case = configs.steady_10m()
         big = UNet(case.nlay, case.nrow, case.ncol, nin=3, channels=CHANNELS, params=params)
-        r = solve_pair(case, big, fcg_m, iter1=scale_up_iters)
+        r = solve_pair(case, big, fcg_m, iter1=scale_up_iters, nn_every=nn_every)
         big.destroy()
         summary["scale_up_10m"] = r
         log(f"10M cells: PCG {r['pcg']['iterations']} it {r['pcg']['s']:.2f} s, FCG+net "
