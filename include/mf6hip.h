@@ -245,14 +245,16 @@ int mf6hip_last_timing(void *handle, double *solve_ms, double *spmv_avg_ms,
 
 /* --- learned preconditioner (no counterpart in the reference) ----------
  *
- * A 4-level 3D U-Net (channels c0 = 16, c1..c3 in {16, 32, 48, 64}; 1x2x2
- * pooling, so nrow and ncol must be divisible by 8) with BF16 weights and
+ * A 3D U-Net of nlev levels (2..8; channels c0 = 16, c1.. in {16, 32, 48,
+ * 64}) with 1x2x2 mean pooling (level l+1 has ceil(nrow_l / 2) rows and
+ * ceil(ncol_l / 2) columns) and nearest upsampling, BF16 weights and
  * activations and FP32 accumulation on MFMA, trained on the device on
  * (residual, error) pairs.  Inputs are nin planes of nlay*nrow*ncol FP32
  * values in DIS node order; parameters are one FP32 vector: per 3x3x3
- * convolution (e0a e0b e1a e1b e2a e2b e3a e3b d2 d1 d0) W[27][cin][cout]
- * then b[cout], then the 1x1x1 output layer w[c0], b. */
-int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, const int *channels,
+ * convolution (encoder e0a e0b e1a e1b ... two per level, then decoder
+ * d_{nlev-2} ... d0) W[27][cin][cout] then b[cout], then the 1x1x1 output
+ * layer w[c0], b. */
+int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, int nlev, const int *channels,
                        void **net);
 int mf6hip_unet_destroy(void *net);
 int mf6hip_unet_num_params(void *net, long long *n);

@@ -143,7 +143,7 @@ def load():
     L.mf6hip_load_factor.argtypes = [vp, dp]
     L.mf6hip_get_system.argtypes = [vp, dp, dp]
     fp = C.POINTER(C.c_float)
-    L.mf6hip_unet_create.argtypes = [i, i, i, i, ip, C.POINTER(vp)]
+    L.mf6hip_unet_create.argtypes = [i, i, i, i, i, ip, C.POINTER(vp)]
     L.mf6hip_unet_destroy.argtypes = [vp]
     L.mf6hip_unet_num_params.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.mf6hip_unet_set_params.argtypes = [vp, fp]

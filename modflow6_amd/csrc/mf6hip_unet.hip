@@ -247,34 +247,47 @@ __global__ void k_bgrad(int V, const bf16 *__restrict__ dy, int ldd, int doff, i
 // elementwise pieces
 // ---------------------------------------------------------------------------
 
-// pooled[d, h, w, c] = mean of the 2x2 block (h, w halved); c < C
-__global__ void k_pool(Grid3 gout, const bf16 *__restrict__ x, int ldx, int xoff, int C,
+// Pooling and upsampling between a fine grid gf and its coarse grid gc with
+// gc.H = ceil(gf.H / 2), gc.W = ceil(gf.W / 2): coarse cell (h, w) covers the
+// fine cells (2h + a, 2w + b), a, b in {0, 1}, that exist (1, 2 or 4 cells
+// on the last row / column of an odd extent).
+__device__ __forceinline__ int pool_count(const Grid3 &gf, int hc, int wc) {
+  const int nh = (2 * hc + 1 < gf.H) ? 2 : 1, nw = (2 * wc + 1 < gf.W) ? 2 : 1;
+  return nh * nw;
+}
+
+// pooled[d, h, w, c] = mean of the fine cells of coarse cell (h, w); c < C
+__global__ void k_pool(Grid3 gc, Grid3 gf, const bf16 *__restrict__ x, int ldx, int xoff, int C,
                        bf16 *__restrict__ y, int ldy, int yoff) {
-  const int n = gout.V() * C;
-  const int Win = gout.W * 2, Hin = gout.H * 2;
+  const int n = gc.V() * C;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int c = i % C, v = i / C;
-    const int w = v % gout.W, h = (v / gout.W) % gout.H, d = v / (gout.W * gout.H);
-    const size_t b0 = ((size_t)(d * Hin + 2 * h) * Win + 2 * w);
-    float s = bf2f(x[b0 * ldx + xoff + c]) + bf2f(x[(b0 + 1) * ldx + xoff + c]);
-    s = s + (bf2f(x[(b0 + Win) * ldx + xoff + c]) + bf2f(x[(b0 + Win + 1) * ldx + xoff + c]));
-    y[(size_t)v * ldy + yoff + c] = f2bf(0.25f * s);
+    const int w = v % gc.W, h = (v / gc.W) % gc.H, d = v / (gc.W * gc.H);
+    const bool hr = 2 * h + 1 < gf.H, wr = 2 * w + 1 < gf.W;
+    const size_t b0 = ((size_t)(d * gf.H + 2 * h) * gf.W + 2 * w);
+    float s = bf2f(x[b0 * ldx + xoff + c]);
+    if (wr) s = s + bf2f(x[(b0 + 1) * ldx + xoff + c]);
+    if (hr) {
+      float t = bf2f(x[(b0 + gf.W) * ldx + xoff + c]);
+      if (wr) t = t + bf2f(x[(b0 + gf.W + 1) * ldx + xoff + c]);
+      s = s + t;
+    }
+    y[(size_t)v * ldy + yoff + c] = f2bf(s / (float)((hr ? 2 : 1) * (wr ? 2 : 1)));
   }
 }
 
-// dx (fine grid) (+)= 0.25 dy (coarse), the pooling gradient; then masked by
-// (h > 0) when mask != 0
-__global__ void k_pool_bwd(Grid3 gin, const bf16 *__restrict__ dy, int ldd, int doff, int C,
-                           const bf16 *__restrict__ add, int lda, int aoff,
+// dx (fine grid) (+)= dy (coarse) / count, the pooling gradient; then masked
+// by (mask > 0) when mask != 0
+__global__ void k_pool_bwd(Grid3 gf, Grid3 gc, const bf16 *__restrict__ dy, int ldd, int doff,
+                           int C, const bf16 *__restrict__ add, int lda, int aoff,
                            const bf16 *__restrict__ mask, int ldm, int moff,
                            bf16 *__restrict__ dx, int ldx, int xoff) {
-  const int n = gin.V() * C;
-  const int Wc = gin.W / 2, Hc = gin.H / 2;
+  const int n = gf.V() * C;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int c = i % C, v = i / C;
-    const int w = v % gin.W, h = (v / gin.W) % gin.H, d = v / (gin.W * gin.H);
-    const size_t vc = (size_t)(d * Hc + h / 2) * Wc + w / 2;
-    float g = 0.25f * bf2f(dy[vc * ldd + doff + c]);
+    const int w = v % gf.W, h = (v / gf.W) % gf.H, d = v / (gf.W * gf.H);
+    const size_t vc = (size_t)(d * gc.H + h / 2) * gc.W + w / 2;
+    float g = bf2f(dy[vc * ldd + doff + c]) / (float)pool_count(gf, h / 2, w / 2);
     if (add) g = bf2f(add[(size_t)v * lda + aoff + c]) + g;
     if (mask && !(bf2f(mask[(size_t)v * ldm + moff + c]) > 0.f)) g = 0.f;
     dx[(size_t)v * ldx + xoff + c] = f2bf(g);
@@ -282,31 +295,36 @@ __global__ void k_pool_bwd(Grid3 gin, const bf16 *__restrict__ dy, int ldd, int 
 }
 
 // nearest 1x2x2 upsampling of a coarse tensor into a fine slice
-__global__ void k_up(Grid3 gout, const bf16 *__restrict__ x, int ldx, int xoff, int C,
+__global__ void k_up(Grid3 gf, Grid3 gc, const bf16 *__restrict__ x, int ldx, int xoff, int C,
                      bf16 *__restrict__ y, int ldy, int yoff) {
-  const int n = gout.V() * C;
-  const int Wc = gout.W / 2, Hc = gout.H / 2;
+  const int n = gf.V() * C;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int c = i % C, v = i / C;
-    const int w = v % gout.W, h = (v / gout.W) % gout.H, d = v / (gout.W * gout.H);
-    const size_t vc = (size_t)(d * Hc + h / 2) * Wc + w / 2;
+    const int w = v % gf.W, h = (v / gf.W) % gf.H, d = v / (gf.W * gf.H);
+    const size_t vc = (size_t)(d * gc.H + h / 2) * gc.W + w / 2;
     y[(size_t)v * ldy + yoff + c] = x[vc * ldx + xoff + c];
   }
 }
 
-// coarse gradient of the upsampling: sum of the 2x2 fine block, masked
-__global__ void k_up_bwd(Grid3 gout, const bf16 *__restrict__ dy, int ldd, int doff, int C,
-                         const bf16 *__restrict__ add, int lda, int aoff,
+// coarse gradient of the upsampling: sum over the fine cells of the coarse
+// cell, masked
+__global__ void k_up_bwd(Grid3 gc, Grid3 gf, const bf16 *__restrict__ dy, int ldd, int doff,
+                         int C, const bf16 *__restrict__ add, int lda, int aoff,
                          const bf16 *__restrict__ mask, int ldm, int moff,
                          bf16 *__restrict__ dx, int ldx, int xoff) {
-  const int n = gout.V() * C;
-  const int Wf = gout.W * 2, Hf = gout.H * 2;
+  const int n = gc.V() * C;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int c = i % C, v = i / C;
-    const int w = v % gout.W, h = (v / gout.W) % gout.H, d = v / (gout.W * gout.H);
-    const size_t b0 = ((size_t)(d * Hf + 2 * h) * Wf + 2 * w);
-    float s = bf2f(dy[b0 * ldd + doff + c]) + bf2f(dy[(b0 + 1) * ldd + doff + c]);
-    s = s + (bf2f(dy[(b0 + Wf) * ldd + doff + c]) + bf2f(dy[(b0 + Wf + 1) * ldd + doff + c]));
+    const int w = v % gc.W, h = (v / gc.W) % gc.H, d = v / (gc.W * gc.H);
+    const bool hr = 2 * h + 1 < gf.H, wr = 2 * w + 1 < gf.W;
+    const size_t b0 = ((size_t)(d * gf.H + 2 * h) * gf.W + 2 * w);
+    float s = bf2f(dy[b0 * ldd + doff + c]);
+    if (wr) s = s + bf2f(dy[(b0 + 1) * ldd + doff + c]);
+    if (hr) {
+      float t = bf2f(dy[(b0 + gf.W) * ldd + doff + c]);
+      if (wr) t = t + bf2f(dy[(b0 + gf.W + 1) * ldd + doff + c]);
+      s = s + t;
+    }
     if (add) s = bf2f(add[(size_t)v * lda + aoff + c]) + s;
     if (mask && !(bf2f(mask[(size_t)v * ldm + moff + c]) > 0.f)) s = 0.f;
     dx[(size_t)v * ldx + xoff + c] = f2bf(s);
@@ -454,30 +472,42 @@ struct UConv {
   s4 *wt = nullptr;     // data-gradient fragments
 };
 
+// levels 0..nlev-1; conv layers in the order e0a e0b e1a e1b ... (two per
+// level, encoder), then the decoder d_{nlev-2} ... d0, one per level
+constexpr int MAXLEV = 8;
+constexpr int MAXCONV = 3 * MAXLEV - 1;
+
 struct UNetHandle {
   int nin = 3;
-  int ch[4];
-  Grid3 g[4];
-  UConv conv[11];       // e0a e0b e1a e1b e2a e2b e3a e3b d2 d1 d0
+  int nlev = 4;
+  int ch[MAXLEV];
+  Grid3 g[MAXLEV];
+  UConv conv[MAXCONV];
   size_t wo_off, bo_off, nparam;
   float *p = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
   float *part = nullptr;   // split-K and block partials
   size_t part_n = 0;
   // activations (bf16) per level
   bf16 *in0 = nullptr;                 // level-0 input, 16 channels
-  bf16 *pin[4] = {};                   // pooled inputs (levels 1..3)
-  bf16 *ha[4] = {};                    // first conv output
-  bf16 *cat[3] = {};                   // decoder inputs [up | skip]
-  bf16 *hb3 = nullptr;                 // bottom output
-  bf16 *u[3] = {};                     // decoder outputs
+  bf16 *pin[MAXLEV] = {};              // pooled inputs (levels 1..nlev-1)
+  bf16 *ha[MAXLEV] = {};               // first conv output
+  bf16 *cat[MAXLEV] = {};              // decoder inputs [up | skip] (levels < nlev-1)
+  bf16 *hbot = nullptr;                // bottom level output
+  bf16 *u[MAXLEV] = {};                // decoder outputs (levels < nlev-1)
   float *y = nullptr;                  // fp32 output
   float *lossv = nullptr;              // [2]
   // gradients
-  bf16 *gu[3] = {}, *gcat[3] = {}, *ghb[4] = {}, *gha[4] = {}, *gpin[4] = {};
+  bf16 *gu[MAXLEV] = {}, *gcat[MAXLEV] = {}, *ghb[MAXLEV] = {}, *gha[MAXLEV] = {},
+       *gpin[MAXLEV] = {};
   bf16 *data = nullptr;                // training samples [nsamples][V][4]
   int nsamples = 0;
   int adam_t = 0;
   hipStream_t stream = nullptr;
+  int nconv() const { return 3 * nlev - 1; }
+  // decoder conv of level l (< nlev - 1)
+  int dconv(int l) const { return 2 * nlev + (nlev - 2 - l); }
+  // channels of the upsampled part of cat[l]
+  int cup(int l) const { return ch[l + 1]; }
 };
 
 namespace {
@@ -560,7 +590,7 @@ int conv_wgrad(UNetHandle *u, const Grid3 &g, const UConv &L, const bf16 *x, int
 
 static int u_pack(UNetHandle *u) {
   hipStream_t s = u->stream;
-  for (int i = 0; i < 11; i++) {
+  for (int i = 0; i < u->nconv(); i++) {
     UConv &L = u->conv[i];
     const int nf = 27 * (L.cin / 16) * (L.cout / 16) * 64;
     k_pack_w<<<nblk(nf), 256, 0, s>>>(u->p + L.w_off, L.cin_real, L.cin, L.cout, 0, L.wp);
@@ -574,35 +604,36 @@ static int u_pack(UNetHandle *u) {
 static int u_forward(UNetHandle *u) {
   hipStream_t s = u->stream;
   const int *c = u->ch;
-  // channel count of the upsampled part of cat[l]
-  const int cup[3] = {c[1], c[2], c[3]};
-  for (int l = 0; l < 4; l++) {
+  const int nl = u->nlev, bot = nl - 1;
+  for (int l = 0; l < nl; l++) {
     const Grid3 &g = u->g[l];
     const bf16 *xin;
     int ldx, cin;
     if (l == 0) { xin = u->in0; ldx = 16; cin = 16; }
     else {
-      // pool hb_{l-1} (in cat[l-1] at offset cup[l-1]) into pin[l]
-      const int cprev = c[l - 1];
-      k_pool<<<nblk((size_t)g.V() * cprev), 256, 0, s>>>(g, u->cat[l - 1], cup[l - 1] + cprev,
-                                                          cup[l - 1], cprev, u->pin[l], cprev, 0);
+      // pool hb_{l-1} (in cat[l-1] at offset cup(l-1)) into pin[l]
+      const int cprev = c[l - 1], cu = u->cup(l - 1);
+      k_pool<<<nblk((size_t)g.V() * cprev), 256, 0, s>>>(g, u->g[l - 1], u->cat[l - 1],
+                                                          cu + cprev, cu, cprev, u->pin[l],
+                                                          cprev, 0);
       xin = u->pin[l]; ldx = cprev; cin = cprev;
     }
     const UConv &A = u->conv[2 * l], &B = u->conv[2 * l + 1];
     UCK(launch_conv(u, g, xin, ldx, 0, cin, A.wp, A.cout, u->p + A.b_off, 1, u->ha[l], c[l], 0,
                     nullptr, 0, 0));
-    bf16 *hb = l < 3 ? u->cat[l] : u->hb3;
-    const int ldh = l < 3 ? cup[l] + c[l] : c[3];
-    const int offh = l < 3 ? cup[l] : 0;
+    bf16 *hb = l < bot ? u->cat[l] : u->hbot;
+    const int ldh = l < bot ? u->cup(l) + c[l] : c[bot];
+    const int offh = l < bot ? u->cup(l) : 0;
     UCK(launch_conv(u, g, u->ha[l], c[l], 0, c[l], B.wp, B.cout, u->p + B.b_off, 1, hb, ldh, offh,
                     nullptr, 0, 0));
   }
-  for (int l = 2; l >= 0; l--) {
+  for (int l = bot - 1; l >= 0; l--) {
     const Grid3 &g = u->g[l];
-    const int ldc = cup[l] + c[l];
-    const bf16 *src = l == 2 ? u->hb3 : u->u[l + 1];
-    k_up<<<nblk((size_t)g.V() * cup[l]), 256, 0, s>>>(g, src, cup[l], 0, cup[l], u->cat[l], ldc, 0);
-    const UConv &D = u->conv[8 + (2 - l)];
+    const int cu = u->cup(l), ldc = cu + c[l];
+    const bf16 *src = l == bot - 1 ? u->hbot : u->u[l + 1];
+    k_up<<<nblk((size_t)g.V() * cu), 256, 0, s>>>(g, u->g[l + 1], src, cu, 0, cu, u->cat[l],
+                                                   ldc, 0);
+    const UConv &D = u->conv[u->dconv(l)];
     UCK(launch_conv(u, g, u->cat[l], ldc, 0, ldc, D.wp, D.cout, u->p + D.b_off, 1, u->u[l], c[l],
                     0, nullptr, 0, 0));
   }
@@ -618,7 +649,7 @@ static int u_forward(UNetHandle *u) {
 static int u_backward(UNetHandle *u, const bf16 *tgt, int ldt, int toff, int accumulate) {
   hipStream_t s = u->stream;
   const int *c = u->ch;
-  const int cup[3] = {c[1], c[2], c[3]};
+  const int nl = u->nlev, bot = nl - 1;
   const Grid3 &g0 = u->g[0];
   const int V0 = g0.V();
   const int nb = nblk(V0) > 1024 ? 1024 : nblk(V0);
@@ -628,36 +659,32 @@ static int u_backward(UNetHandle *u, const bf16 *tgt, int ldt, int toff, int acc
                                u->p + u->wo_off, u->gu[0], u->part);
   k_colsum<<<1, 256, 0, s>>>(u->part, nb, c[0] + 1, u->grad + u->wo_off, accumulate);
   // decoder, top to bottom
-  for (int l = 0; l < 3; l++) {
+  for (int l = 0; l < bot; l++) {
     const Grid3 &g = u->g[l];
-    const int ldc = cup[l] + c[l];
-    const UConv &D = u->conv[8 + (2 - l)];
+    const int cu = u->cup(l), ldc = cu + c[l];
+    const UConv &D = u->conv[u->dconv(l)];
     UCK(conv_wgrad(u, g, D, u->cat[l], ldc, 0, u->gu[l], c[l], 0, accumulate));
     UCK(launch_conv(u, g, u->gu[l], c[l], 0, c[l], D.wt, ldc, nullptr, 0, u->gcat[l], ldc, 0,
                     nullptr, 0, 0));
     const Grid3 &gc = u->g[l + 1];
-    if (l < 2) {
-      // gradient of u[l+1] through the upsampling, masked by u[l+1] > 0
-      k_up_bwd<<<nblk((size_t)gc.V() * cup[l]), 256, 0, s>>>(gc, u->gcat[l], ldc, 0, cup[l],
-                                                             nullptr, 0, 0, u->u[l + 1], c[l + 1],
-                                                             0, u->gu[l + 1], c[l + 1], 0);
-    } else {
-      // gradient of hb3, masked by hb3 > 0
-      k_up_bwd<<<nblk((size_t)gc.V() * cup[l]), 256, 0, s>>>(gc, u->gcat[l], ldc, 0, cup[l],
-                                                             nullptr, 0, 0, u->hb3, c[3], 0,
-                                                             u->ghb[3], c[3], 0);
-    }
+    // gradient of the coarser decoder output (or of the bottom output)
+    // through the upsampling, masked by that output > 0
+    const bf16 *msk = l < bot - 1 ? u->u[l + 1] : u->hbot;
+    bf16 *dst = l < bot - 1 ? u->gu[l + 1] : u->ghb[bot];
+    k_up_bwd<<<nblk((size_t)gc.V() * cu), 256, 0, s>>>(gc, g, u->gcat[l], ldc, 0, cu, nullptr,
+                                                        0, 0, msk, cu, 0, dst, cu, 0);
   }
   // encoder, bottom to top
-  for (int l = 3; l >= 0; l--) {
+  for (int l = bot; l >= 0; l--) {
     const Grid3 &g = u->g[l];
-    if (l < 3) {
+    if (l < bot) {
       // hb_l gradient: skip part of gcat[l] + pooled gradient from level l+1,
       // masked by hb_l > 0
-      const int ldc = cup[l] + c[l];
-      k_pool_bwd<<<nblk((size_t)g.V() * c[l]), 256, 0, s>>>(g, u->gpin[l + 1], c[l], 0, c[l],
-                                                             u->gcat[l], ldc, cup[l], u->cat[l],
-                                                             ldc, cup[l], u->ghb[l], c[l], 0);
+      const int cu = u->cup(l), ldc = cu + c[l];
+      k_pool_bwd<<<nblk((size_t)g.V() * c[l]), 256, 0, s>>>(g, u->g[l + 1], u->gpin[l + 1], c[l],
+                                                             0, c[l], u->gcat[l], ldc, cu,
+                                                             u->cat[l], ldc, cu, u->ghb[l], c[l],
+                                                             0);
     }
     const UConv &A = u->conv[2 * l], &B = u->conv[2 * l + 1];
     UCK(conv_wgrad(u, g, B, u->ha[l], c[l], 0, u->ghb[l], c[l], 0, accumulate));
@@ -687,20 +714,16 @@ static int u_adam(UNetHandle *u, float lr, float gscale) {
 static void u_free(UNetHandle *u) {
   if (!u) return;
   if (u->stream) hipStreamSynchronize(u->stream);
-  void *bufs[] = {u->p, u->grad, u->m, u->v, u->part, u->in0, u->hb3, u->y, u->lossv, u->data};
+  void *bufs[] = {u->p, u->grad, u->m, u->v, u->part, u->in0, u->hbot, u->y, u->lossv, u->data};
   for (void *b : bufs)
     if (b) hipFree(b);
-  for (int l = 0; l < 4; l++) {
-    void *b2[] = {u->pin[l], u->ha[l], u->ghb[l], u->gha[l], u->gpin[l]};
+  for (int l = 0; l < MAXLEV; l++) {
+    void *b2[] = {u->pin[l], u->ha[l], u->ghb[l], u->gha[l], u->gpin[l],
+                  u->cat[l], u->u[l], u->gu[l], u->gcat[l]};
     for (void *b : b2)
       if (b) hipFree(b);
   }
-  for (int l = 0; l < 3; l++) {
-    void *b3[] = {u->cat[l], u->u[l], u->gu[l], u->gcat[l]};
-    for (void *b : b3)
-      if (b) hipFree(b);
-  }
-  for (int i = 0; i < 11; i++) {
+  for (int i = 0; i < MAXCONV; i++) {
     if (u->conv[i].wp) hipFree(u->conv[i].wp);
     if (u->conv[i].wt) hipFree(u->conv[i].wt);
   }
@@ -708,16 +731,20 @@ static void u_free(UNetHandle *u) {
   delete u;
 }
 
-extern "C" int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, const int *channels,
-                                  void **handle) {
+extern "C" int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, int nlev,
+                                  const int *channels, void **handle) {
   if (!handle || !channels) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
   *handle = nullptr;
-  if (nlay < 1 || nrow < 8 || ncol < 8 || nrow % 8 || ncol % 8) {
-    mf6hip_set_error("the learned preconditioner needs nrow and ncol divisible by 8");
+  if (nlev < 2 || nlev > MAXLEV) {
+    mf6hip_set_error("the network has 2.." + std::to_string(MAXLEV) + " levels");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  if (nlay < 1 || nrow < 2 || ncol < 2) {
+    mf6hip_set_error("the learned preconditioner needs nrow and ncol >= 2");
     return MF6HIP_EUNSUPPORTED;
   }
   if (nin < 1 || nin > 16) { mf6hip_set_error("1..16 input channels"); return MF6HIP_EINVAL; }
-  for (int l = 0; l < 4; l++)
+  for (int l = 0; l < nlev; l++)
     if (channels[l] < 16 || channels[l] > 64 || channels[l] % 16) {
       mf6hip_set_error("channels must be 16, 32, 48 or 64");
       return MF6HIP_EUNSUPPORTED;
@@ -726,24 +753,29 @@ extern "C" int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, const i
   if ((long long)nlay * nrow * ncol > (1LL << 28)) { mf6hip_set_error("grid too large"); return MF6HIP_EUNSUPPORTED; }
   UNetHandle *u = new UNetHandle();
   u->nin = nin;
-  for (int l = 0; l < 4; l++) {
+  u->nlev = nlev;
+  for (int l = 0; l < nlev; l++) {
     u->ch[l] = channels[l];
-    u->g[l] = Grid3{nlay, nrow >> l, ncol >> l};
+    u->g[l] = l == 0 ? Grid3{nlay, nrow, ncol}
+                     : Grid3{nlay, (u->g[l - 1].H + 1) / 2, (u->g[l - 1].W + 1) / 2};
   }
   const int *c = u->ch;
-  const int cup[3] = {c[1], c[2], c[3]};
-  const int cin_pad[11] = {16, c[0], c[0], c[1], c[1], c[2], c[2], c[3],
-                           cup[2] + c[2], cup[1] + c[1], cup[0] + c[0]};
-  const int cin_real[11] = {nin, c[0], c[0], c[1], c[1], c[2], c[2], c[3],
-                            cup[2] + c[2], cup[1] + c[1], cup[0] + c[0]};
-  const int cout[11] = {c[0], c[0], c[1], c[1], c[2], c[2], c[3], c[3], c[2], c[1], c[0]};
+  const int bot = nlev - 1;
   size_t off = 0;
   int rc = MF6HIP_OK;
-  for (int i = 0; i < 11; i++) {
+  for (int i = 0; i < u->nconv(); i++) {
     UConv &L = u->conv[i];
-    L.cin = cin_pad[i];
-    L.cin_real = cin_real[i];
-    L.cout = cout[i];
+    if (i < 2 * nlev) {
+      const int l = i / 2;
+      L.cin = i == 0 ? 16 : (i % 2 ? c[l] : c[l - 1]);
+      L.cin_real = i == 0 ? nin : L.cin;
+      L.cout = c[l];
+    } else {
+      const int l = bot - 1 - (i - 2 * nlev);   // decoder level
+      L.cin = u->cup(l) + c[l];
+      L.cin_real = L.cin;
+      L.cout = c[l];
+    }
     L.w_off = off;
     off += (size_t)27 * L.cin_real * L.cout;
     L.b_off = off;
@@ -763,7 +795,7 @@ extern "C" int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, const i
   if (!rc) rc = ualloc(&u->v, off);
   // partials: the largest split-K buffer (2048 / tiles splits, at least 1)
   size_t pn = 1 << 20;
-  for (int i = 0; i < 11; i++) {
+  for (int i = 0; i < u->nconv(); i++) {
     const UConv &L = u->conv[i];
     const size_t n = (size_t)27 * L.cin * L.cout;
     const int tiles = 27 * (L.cin / 16) * (L.cout / 16);
@@ -777,20 +809,20 @@ extern "C" int mf6hip_unet_create(int nlay, int nrow, int ncol, int nin, const i
   if (!rc) rc = ualloc(&u->in0, V0 * 16);
   if (!rc) rc = ualloc(&u->y, V0);
   if (!rc) rc = ualloc(&u->lossv, 4);
-  for (int l = 0; l < 4 && !rc; l++) {
+  for (int l = 0; l < nlev && !rc; l++) {
     const size_t V = u->g[l].V();
     if (l > 0) rc = ualloc(&u->pin[l], V * c[l - 1]);
     if (!rc) rc = ualloc(&u->ha[l], V * c[l]);
     if (!rc) rc = ualloc(&u->ghb[l], V * c[l]);
     if (!rc) rc = ualloc(&u->gha[l], V * c[l]);
     if (!rc && l > 0) rc = ualloc(&u->gpin[l], V * c[l - 1]);
-    if (!rc && l < 3) {
-      rc = ualloc(&u->cat[l], V * (cup[l] + c[l]));
+    if (!rc && l < bot) {
+      rc = ualloc(&u->cat[l], V * (u->cup(l) + c[l]));
       if (!rc) rc = ualloc(&u->u[l], V * c[l]);
       if (!rc) rc = ualloc(&u->gu[l], V * c[l]);
-      if (!rc) rc = ualloc(&u->gcat[l], V * (cup[l] + c[l]));
+      if (!rc) rc = ualloc(&u->gcat[l], V * (u->cup(l) + c[l]));
     }
-    if (!rc && l == 3) rc = ualloc(&u->hb3, V * c[3]);
+    if (!rc && l == bot) rc = ualloc(&u->hbot, V * c[bot]);
   }
   if (!rc && hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking) != hipSuccess) {
     mf6hip_set_error("hipStreamCreate failed");

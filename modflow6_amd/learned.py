@@ -26,7 +26,9 @@ import numpy as np
 from . import _lib, configs, fields
 from .ims import HipLinearSolver, ImsLinearSettings, dis_structure
 
-CHANNELS = (16, 32, 64, 64)
+# six levels: the coarsest grid of a 256 x 256 layer is 8 x 8, so the
+# receptive field spans the model
+CHANNELS = (16, 32, 64, 64, 64, 64)
 
 
 def standardise(a):
@@ -69,8 +71,9 @@ class UNet:
                  seed=0):
         self.L = L = _lib.load()
         self.h = C.c_void_p()
-        ch = (C.c_int * 4)(*channels)
-        _lib.check(L.mf6hip_unet_create(nlay, nrow, ncol, nin, ch, C.byref(self.h)),
+        ch = (C.c_int * len(channels))(*channels)
+        _lib.check(L.mf6hip_unet_create(nlay, nrow, ncol, nin, len(channels), ch,
+                                        C.byref(self.h)),
                    "mf6hip_unet_create")
         n = C.c_longlong(0)
         _lib.check(L.mf6hip_unet_num_params(self.h, C.byref(n)), "mf6hip_unet_num_params")
@@ -132,9 +135,13 @@ class UNet:
 
 
 def layer_shapes(nin, ch):
-    c0, c1, c2, c3 = ch
-    return [(nin, c0), (c0, c0), (c0, c1), (c1, c1), (c1, c2), (c2, c2),
-            (c2, c3), (c3, c3), (c3 + c2, c2), (c2 + c1, c1), (c1 + c0, c0)]
+    """(cin, cout) of the 3x3x3 convolutions in parameter order: two per
+    level (encoder), then the decoder from the second-coarsest level up."""
+    enc = []
+    for l, c in enumerate(ch):
+        enc += [(nin if l == 0 else ch[l - 1], c), (c, c)]
+    dec = [(ch[l + 1] + ch[l], ch[l]) for l in range(len(ch) - 2, -1, -1)]
+    return enc + dec
 
 
 def init_params(nin, ch, rng):

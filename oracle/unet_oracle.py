@@ -8,13 +8,14 @@ mathematics in plain numpy, independently of the HIP implementation:
 * conv3(x, W, b): 3x3x3 cross-correlation with zero padding 1,
   y[d,h,w,o] = b[o] + sum_{t=(i,j,k)} sum_c x[d+i-1, h+j-1, w+k-1, c] W[t,c,o]
   with tap index t = 9 i + 3 j + k;
-* pool: mean over 1x2x2 blocks (H, W halved); up: nearest 1x2x2 repeat;
-* levels 0..3 with channels ch[0..3]:
+* pool: mean over 1x2x2 blocks (H, W -> ceil(H / 2), ceil(W / 2); a block
+  on the last row / column of an odd extent averages the 1 or 2 cells it
+  has); up: nearest 1x2x2 repeat cropped to the fine extent;
+* levels 0..L-1 with channels ch[0..L-1]:
     a0 = relu(conv(relu(conv(in))))            encoder, level 0
-    a1 = relu(conv(relu(conv(pool(a0)))))      level 1, likewise a2, a3
-    u2 = relu(conv(cat(up(a3), a2)))           decoder (cat: up channels first)
-    u1 = relu(conv(cat(up(u2), a1)))
-    u0 = relu(conv(cat(up(u1), a0)))
+    a_l = relu(conv(relu(conv(pool(a_{l-1})))))  levels 1..L-1
+    u_{L-2} = relu(conv(cat(up(a_{L-1}), a_{L-2})))   decoder (cat: up first)
+    u_l = relu(conv(cat(up(u_{l+1}), a_l)))    l = L-3..0
     y  = u0 . w_out + b_out                    1x1x1 output conv
 * loss = sum (y - t)^2 / sum t^2 (relative squared error of the batch).
 
@@ -24,14 +25,14 @@ from __future__ import annotations
 
 import numpy as np
 
-# (name, cin_source, cout_index); conv layers in evaluation order
+# (name, cin, cout) of the conv layers in parameter order
 def layer_shapes(cin, ch):
-    c0, c1, c2, c3 = ch
-    return [("e0a", cin, c0), ("e0b", c0, c0),
-            ("e1a", c0, c1), ("e1b", c1, c1),
-            ("e2a", c1, c2), ("e2b", c2, c2),
-            ("e3a", c2, c3), ("e3b", c3, c3),
-            ("d2", c3 + c2, c2), ("d1", c2 + c1, c1), ("d0", c1 + c0, c0)]
+    L = len(ch)
+    out = []
+    for l in range(L):
+        out += [(f"e{l}a", cin if l == 0 else ch[l - 1], ch[l]), (f"e{l}b", ch[l], ch[l])]
+    out += [(f"d{l}", ch[l + 1] + ch[l], ch[l]) for l in range(L - 2, -1, -1)]
+    return out
 
 
 def init_params(cin, ch, rng):
@@ -69,22 +70,50 @@ def conv3_bwd(x, w, dy):
     return dxp[1:-1, 1:-1, 1:-1], dw, dy.reshape(-1, dy.shape[-1]).sum(0)
 
 
+def _blocks(H, W):
+    """(a, b, rows, cols): the fine cells (2h + a, 2w + b) of each coarse
+    cell that exist, as slices of the fine and coarse grids."""
+    Hc, Wc = (H + 1) // 2, (W + 1) // 2
+    for a in (0, 1):
+        for b in (0, 1):
+            nh, nw = len(range(a, H, 2)), len(range(b, W, 2))
+            yield (slice(a, None, 2), slice(b, None, 2)), (slice(0, nh), slice(0, nw)), (Hc, Wc)
+
+
+def _count(H, W):
+    Hc, Wc = (H + 1) // 2, (W + 1) // 2
+    n = np.zeros((Hc, Wc))
+    for _, (rc, cc), _ in _blocks(H, W):
+        n[rc, cc] += 1.0
+    return n[None, :, :, None]
+
+
 def pool(x):
     D, H, W, C = x.shape
-    return x.reshape(D, H // 2, 2, W // 2, 2, C).mean(axis=(2, 4))
+    s = np.zeros((D, (H + 1) // 2, (W + 1) // 2, C))
+    for (rf, cf), (rc, cc), _ in _blocks(H, W):
+        s[:, rc, cc] += x[:, rf, cf]
+    return s / _count(H, W)
 
 
-def pool_bwd(dy):
-    return np.repeat(np.repeat(dy, 2, axis=1), 2, axis=2) * 0.25
+def pool_bwd(dy, H, W):
+    g = dy / _count(H, W)
+    dx = np.zeros((dy.shape[0], H, W, dy.shape[-1]))
+    for (rf, cf), (rc, cc), _ in _blocks(H, W):
+        dx[:, rf, cf] = g[:, rc, cc]
+    return dx
 
 
-def up(x):
-    return np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)
+def up(x, H, W):
+    return np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)[:, :H, :W]
 
 
 def up_bwd(dy):
     D, H, W, C = dy.shape
-    return dy.reshape(D, H // 2, 2, W // 2, 2, C).sum(axis=(2, 4))
+    s = np.zeros((D, (H + 1) // 2, (W + 1) // 2, C))
+    for (rf, cf), (rc, cc), _ in _blocks(H, W):
+        s[:, rc, cc] += dy[:, rf, cf]
+    return s
 
 
 def relu(x):
@@ -92,17 +121,19 @@ def relu(x):
 
 
 def _named(params, cin):
-    ch = [p[0].shape[2] for p in params[0:8:2]]
+    nlev = len(params) // 3
+    ch = [p[0].shape[2] for p in params[0:2 * nlev:2]]
     names = [n for n, _, _ in layer_shapes(cin, ch)]
     return ch, names, dict(zip(names, params[:-1]))
 
 
 def forward(params, x, keep=False):
-    _, _, P = _named(params, x.shape[-1])
+    ch, _, P = _named(params, x.shape[-1])
+    nlev = len(ch)
     st = {}
     h = x
     acts = []
-    for lvl in range(4):
+    for lvl in range(nlev):
         if lvl > 0:
             h = pool(h)
         st[f"in{lvl}"] = h
@@ -111,9 +142,10 @@ def forward(params, x, keep=False):
         st[f"h{lvl}a"], st[f"h{lvl}b"] = ha, hb
         acts.append(hb)
         h = hb
-    u = acts[3]
-    for lvl in (2, 1, 0):
-        cat = np.concatenate([up(u), acts[lvl]], axis=-1)
+    u = acts[-1]
+    for lvl in range(nlev - 2, -1, -1):
+        a = acts[lvl]
+        cat = np.concatenate([up(u, a.shape[1], a.shape[2]), a], axis=-1)
         st[f"cat{lvl}"] = cat
         u = relu(conv3(cat, *P[f"d{lvl}"]))
         st[f"u{lvl}"] = u
@@ -134,17 +166,18 @@ def loss_and_grads(params, x, t):
     u0 = st["u0"]
     g_out = (u0.reshape(-1, u0.shape[-1]).T @ dy.reshape(-1, 1), dy.sum(axis=(0, 1, 2)))
     du = dy @ wo.T
-    dacts = {lvl: 0.0 for lvl in range(4)}
-    for lvl in (0, 1, 2):
+    nlev = len(ch)
+    dacts = {lvl: 0.0 for lvl in range(nlev)}
+    for lvl in range(nlev - 1):
         du = du * (st[f"u{lvl}"] > 0)
         dcat, dw, db = conv3_bwd(st[f"cat{lvl}"], P[f"d{lvl}"][0], du)
         G[f"d{lvl}"] = (dw, db)
         cu = dcat.shape[-1] - ch[lvl]
         dacts[lvl] = dacts[lvl] + dcat[..., cu:]
         du = up_bwd(dcat[..., :cu])
-    dacts[3] = dacts[3] + du
+    dacts[nlev - 1] = dacts[nlev - 1] + du
     dh = None
-    for lvl in (3, 2, 1, 0):
+    for lvl in range(nlev - 1, -1, -1):
         g = dacts[lvl] if dh is None else dacts[lvl] + dh
         g = g * (st[f"h{lvl}b"] > 0)
         dha, dw, db = conv3_bwd(st[f"h{lvl}a"], P[f"e{lvl}b"][0], g)
@@ -152,6 +185,8 @@ def loss_and_grads(params, x, t):
         dha = dha * (st[f"h{lvl}a"] > 0)
         din, dw, db = conv3_bwd(st[f"in{lvl}"], P[f"e{lvl}a"][0], dha)
         G[f"e{lvl}a"] = (dw, db)
-        dh = pool_bwd(din) if lvl > 0 else din
+        if lvl > 0:
+            hp = st[f"h{lvl - 1}b"]
+            dh = pool_bwd(din, hp.shape[1], hp.shape[2])
     grads = [G[n] for n in names] + [g_out]
     return loss, grads

@@ -50,7 +50,11 @@ def cos(a, b):
     return float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
 
 
-SHAPES = [((2, 16, 16), (16, 16, 32, 32)), ((3, 16, 24), (16, 32, 64, 64))]
+# four levels on even extents; five and six levels with odd extents on the
+# way down (13 -> 7 -> 4 -> 2 -> 1 rows), so the partial pooling windows and
+# the cropped upsampling are covered
+SHAPES = [((2, 16, 16), (16, 16, 32, 32)), ((3, 16, 24), (16, 32, 64, 64)),
+          ((2, 13, 21), (16, 32, 32, 48, 64)), ((2, 40, 36), (16, 16, 32, 32, 64, 64))]
 
 
 def sample(shape, seed):

@@ -12,26 +12,35 @@ F = torch.nn.functional
 
 
 def torch_net(tp, xt):
+    nlev = len(tp) // 3
+
     def conv(h, w, b):
         return F.conv3d(h, w.reshape(3, 3, 3, w.shape[1], w.shape[2]).permute(4, 3, 0, 1, 2),
                         b, padding=1)
     h = xt
     acts = []
-    for lvl in range(4):
+    for lvl in range(nlev):
         if lvl > 0:
-            h = F.avg_pool3d(h, (1, 2, 2))
+            # ceil_mode: a window on the last row / column of an odd extent
+            # averages the cells it covers
+            h = F.avg_pool3d(h, (1, 2, 2), ceil_mode=True)
         h = F.relu(conv(h, *tp[2 * lvl]))
         h = F.relu(conv(h, *tp[2 * lvl + 1]))
         acts.append(h)
-    u = acts[3]
-    for lvl, idx in ((2, 8), (1, 9), (0, 10)):
-        cat = torch.cat([F.interpolate(u, scale_factor=(1, 2, 2), mode="nearest"), acts[lvl]], 1)
-        u = F.relu(conv(cat, *tp[idx]))
-    wo, bo = tp[11]
+    u = acts[-1]
+    for i, lvl in enumerate(range(nlev - 2, -1, -1)):
+        a = acts[lvl]
+        up = u.repeat_interleave(2, dim=3).repeat_interleave(2, dim=4)
+        cat = torch.cat([up[:, :, :, :a.shape[3], :a.shape[4]], a], 1)
+        u = F.relu(conv(cat, *tp[2 * nlev + i]))
+    wo, bo = tp[-1]
     return torch.einsum("ncdhw,co->nodhw", u, wo) + bo.reshape(1, -1, 1, 1, 1)
 
 
-@pytest.mark.parametrize("shape,ch", [((2, 8, 8), (4, 6, 8, 8)), ((3, 16, 8), (16, 16, 32, 32))])
+@pytest.mark.parametrize("shape,ch", [((2, 8, 8), (4, 6, 8, 8)), ((3, 16, 8), (16, 16, 32, 32)),
+                                      ((2, 13, 11), (4, 6, 8, 8, 8)),
+                                      ((2, 21, 9), (4, 4, 6, 6, 8)),
+                                      ((1, 40, 36), (4, 4, 4, 6, 6, 8))])
 def test_unet_oracle_matches_torch_autograd(shape, ch):
     rng = np.random.default_rng(1)
     ps = U.init_params(3, ch, rng)
@@ -50,6 +59,17 @@ def test_unet_oracle_matches_torch_autograd(shape, ch):
     for (gw, gb), (w, b) in zip(grads, tp):
         assert np.max(np.abs(gw - w.grad.numpy())) <= 1e-12 * max(1.0, np.max(np.abs(gw)))
         assert np.max(np.abs(gb - b.grad.numpy())) <= 1e-12 * max(1.0, np.max(np.abs(gb)))
+
+
+def test_pool_of_odd_extent_averages_the_cells_it_covers():
+    """Hand-worked 1 x 3 x 3 case: corner block of 4, edge blocks of 2, the
+    last corner of 1; the pooling gradient spreads dy / count back."""
+    x = np.arange(9.0).reshape(1, 3, 3, 1)
+    p = U.pool(x)[0, :, :, 0]
+    assert np.array_equal(p, [[(0 + 1 + 3 + 4) / 4, (2 + 5) / 2], [(6 + 7) / 2, 8.0]])
+    dx = U.pool_bwd(np.ones((1, 2, 2, 1)), 3, 3)[0, :, :, 0]
+    assert np.array_equal(dx, [[0.25, 0.25, 0.5], [0.25, 0.25, 0.5], [0.5, 0.5, 1.0]])
+    assert np.array_equal(U.up_bwd(np.ones((1, 3, 3, 1)))[0, :, :, 0], [[4, 2], [2, 1]])
 
 
 def test_unet_oracle_gradient_finite_difference():
