@@ -1,21 +1,27 @@
 // mf6hip_fcg.hip -- flexible preconditioned CG with the learned correction.
 //
-// Preconditioner (variable, hence flexible CG with one stored direction):
+// Preconditioner (variable, hence flexible CG):
 //   z  = ILU0(r)                      the device ILU0 of the IMS solve
 //   t  = r - A z                      what the ILU0 step leaves
 //   y  = s * net([t / s, features])   s = rms(t): the network sees unit-scale
 //                                     residuals; its output is rescaled
 //   z += w y,  w = y.t / y.A y        energy-optimal step along y
 // FCG(m) (Notay 2000): p = z - sum_j (z.q_j / p_j.q_j) p_j over the last m
-// directions, alpha = p.r / p.q, x += alpha p, r -= alpha q.  The IMS convergence test
-// (ImsLinearBase.f90:1101-1146) is applied to every iteration with the signed
-// largest head change alpha p and residual; when it passes, the FP64 true
-// residual b - A x is recomputed and must pass it too (else the iteration
-// continues from the true residual).  The network runs on every nn_every-th
-// iteration (ILU0 alone on the others).  Fallback: the ILU0-only step
-// replaces the learned one for the rest of the solve when z.r changes sign
-// against the ILU0 step (not a descent direction) or when ten applications
-// of the network (10 nn_every iterations) reduce ||r||_2 by less than 10 %.
+// directions, alpha = p.r / p.q, x += alpha p, r -= alpha q.  The IMS
+// convergence test (ImsLinearBase.f90:1101-1146) is applied to every
+// iteration with the signed largest head change alpha p and residual; when
+// it passes, the FP64 true residual b - A x is recomputed and must pass it
+// too (else the iteration continues from the true residual).  The network
+// runs on every nn_every-th iteration (ILU0 alone on the others).
+// Fallback: the ILU0-only step replaces the learned one for the rest of the
+// solve when z.r changes sign against the ILU0 step (not a descent
+// direction) or when ten applications of the network (10 nn_every
+// iterations) reduce ||r||_2 by less than 10 %.
+//
+// Every scalar of the iteration lives in a device FcgState: the host
+// enqueues the kernels of a group of iterations without waiting and reads
+// the state once per group; once the solve has stopped (converged) the
+// state-changing kernels of the rest of the group return at once.
 #include <cmath>
 #include <vector>
 
@@ -23,15 +29,64 @@
 
 namespace {
 
-__global__ void k_fcg_sub(int n, const double *__restrict__ r, double *__restrict__ t) {
+constexpr int FCG_MAXM = 64;   // directions kept, at most
+constexpr int FCG_NDOT = 8;    // dot products per pass
+constexpr int FCG_GROUP = 8;   // iterations enqueued between state reads
+
+struct FcgState {
+  double tt, zr_ilu, yt, yay, yr, w;   // learned step
+  double pr;                           // p.r
+  double rnorm, dvmax, rmax;
+  double rn_window;
+  double c[FCG_MAXM];                  // z.q_j, by ring slot
+  double pqs[FCG_MAXM];                // p_j.q_j, by ring slot
+  int learned;                         // network steps still allowed
+  int cand;                            // the IMS test passed: check b - A x
+  int icnvg, iter, learned_iters, window;
+};
+
+struct FcgSlots {
+  int n;
+  int slot[FCG_MAXM];
+};
+
+struct FcgTest {
+  double dvclose, rclose;
+  int icnvgopt, nn_every;
+};
+
+__device__ inline double wave_sum(double s) {
+  for (int o = 32; o > 0; o >>= 1) s = s + __shfl_xor(s, o);
+  return s;
+}
+
+__global__ void k_fcg_init(FcgState *fs, KState *ks, int learned) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  fs->learned = learned;
+  fs->cand = 0;
+  fs->icnvg = 0;
+  fs->iter = 0;
+  fs->learned_iters = 0;
+  fs->window = 0;
+  fs->rn_window = -1.0;
+  for (int j = 0; j < FCG_MAXM; j++) { fs->c[j] = 0.0; fs->pqs[j] = 0.0; }
+  ks->stop = 0;
+}
+
+// t = r - t
+__global__ void k_fcg_sub(int n, const double *__restrict__ r, double *__restrict__ t,
+                          const KState *ks) {
+  if (ks->stop) return;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
     t[e] = r[e] - t[e];
 }
 
-// network input (BF16, 16 channels, natural order): ch0 = t / s, ch1..2 =
-// features, ch3..15 = 0
-__global__ void k_fcg_nn_input(Layout L, const double *__restrict__ t, double inv_s,
+// network input (BF16, 16 channels, natural order): ch0 = t / s with
+// s = rms(t) = sqrt(tt / N), ch1..2 = features, ch3..15 = 0
+__global__ void k_fcg_nn_input(Layout L, const double *__restrict__ t, const FcgState *fs,
                                const float *__restrict__ feat, unsigned short *__restrict__ in0) {
+  const double tt = fs->tt;
+  const double inv_s = tt > 0.0 ? sqrt((double)L.N / tt) : 0.0;
   const int n16 = L.N * 16;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += gridDim.x * blockDim.x) {
     const int c = i & 15, n = i >> 4;
@@ -45,25 +100,86 @@ __global__ void k_fcg_nn_input(Layout L, const double *__restrict__ t, double in
 }
 
 // yv (positions) = s * y (natural order); padding slots 0
-__global__ void k_fcg_nn_out(Layout L, const float *__restrict__ y, double s,
+__global__ void k_fcg_nn_out(Layout L, const float *__restrict__ y, const FcgState *fs,
                              double *__restrict__ yv) {
+  const double tt = fs->tt;
+  const double sc = tt > 0.0 ? sqrt(tt / (double)L.N) : 0.0;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.NP; e += gridDim.x * blockDim.x) {
     int c, k, i, j, l;
-    yv[e] = pos_to_cell(L, e, c, k, i, j, l) ? s * (double)y[natural(L, k, i, j)] : 0.0;
+    yv[e] = pos_to_cell(L, e, c, k, i, j, l) ? sc * (double)y[natural(L, k, i, j)] : 0.0;
   }
 }
 
+// the weight of the learned step: w = y.t / y.A y, or 0 when the network is
+// off, the step is degenerate, or z.r would change sign against the ILU0
+// step (then the network is switched off for the rest of the solve)
+__global__ void k_fcg_decide(FcgState *fs, const KState *ks) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double w = 0.0;
+  if (!ks->stop && fs->learned && fs->tt > 0.0 && fs->yay != 0.0 && isfinite(fs->yt) &&
+      isfinite(fs->yay)) {
+    w = fs->yt / fs->yay;
+    const double zr = fs->zr_ilu + w * fs->yr;
+    if (!(zr * fs->zr_ilu > 0.0)) {
+      w = 0.0;
+      fs->learned = 0;
+    }
+  }
+  fs->w = w;
+  if (w != 0.0) fs->learned_iters++;
+}
+
 // z += w y
-__global__ void k_fcg_axpy(int n, double w, const double *__restrict__ y, double *__restrict__ z) {
+__global__ void k_fcg_axpy_w(int n, const FcgState *fs, const double *__restrict__ y,
+                             double *__restrict__ z) {
+  const double w = fs->w;
+  if (w == 0.0) return;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
     z[e] = z[e] + w * y[e];
 }
 
-// up to 3 dot products in one pass: part[k * nb + block]
+// up to FCG_NDOT dot products a . b_k in one pass (b_k = B + slot_k * ld,
+// or the explicit pointers when B is null): part[k * gridDim.x + block]
+struct DotArgs {
+  int n;
+  const double *b[FCG_NDOT];
+};
+
+__global__ void __launch_bounds__(MF6_BLOCK) k_fcg_mdots(int n, const double *__restrict__ a,
+                                                         DotArgs d, double *part,
+                                                         const KState *ks) {
+  if (ks->stop) return;
+  __shared__ double lds[FCG_NDOT][MF6_BLOCK / 64];
+  double s[FCG_NDOT];
+#pragma unroll
+  for (int k = 0; k < FCG_NDOT; k++) s[k] = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const double av = a[e];
+#pragma unroll
+    for (int k = 0; k < FCG_NDOT; k++)
+      if (k < d.n) s[k] = s[k] + av * d.b[k][e];
+  }
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < FCG_NDOT; k++) {
+    const double t = wave_sum(s[k]);
+    if ((threadIdx.x & 63) == 0) lds[k][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < d.n) {
+    double t = 0.0;
+    for (int v = 0; v < (int)(blockDim.x >> 6); v++) t = t + lds[threadIdx.x][v];
+    part[threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// pairs of vectors: part[k * gridDim.x + block] = sum a_k . b_k, k < 3
 __global__ void __launch_bounds__(MF6_BLOCK) k_fcg_dots(int n, int ndot, const double *a0,
                                                         const double *b0, const double *a1,
                                                         const double *b1, const double *a2,
-                                                        const double *b2, double *part) {
+                                                        const double *b2, double *part,
+                                                        const KState *ks) {
+  if (ks->stop) return;
   __shared__ double lds[3][MF6_BLOCK / 64];
   double s[3] = {0.0, 0.0, 0.0};
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
@@ -71,53 +187,75 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_fcg_dots(int n, int ndot, const d
     if (ndot > 1) s[1] = s[1] + a1[e] * b1[e];
     if (ndot > 2) s[2] = s[2] + a2[e] * b2[e];
   }
-  for (int k = 0; k < 3; k++)
-    for (int o = 32; o > 0; o >>= 1) s[k] = s[k] + __shfl_xor(s[k], o);
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < 3; k++) lds[k][w] = s[k];
+  for (int k = 0; k < 3; k++) {
+    const double t = wave_sum(s[k]);
+    if ((threadIdx.x & 63) == 0) lds[k][w] = t;
+  }
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 0; k < ndot; k++) {
-      double t = 0.0;
-      for (int v = 0; v < (int)(blockDim.x >> 6); v++) t = t + lds[k][v];
-      part[k * gridDim.x + blockIdx.x] = t;
-    }
+  if (threadIdx.x < ndot) {
+    double t = 0.0;
+    for (int v = 0; v < (int)(blockDim.x >> 6); v++) t = t + lds[threadIdx.x][v];
+    part[threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
 }
 
-__global__ void k_fcg_sum(const double *part, int nb, int ndot, double *out) {
-  __shared__ double lds[MF6_BLOCK];
+// out_k = sum of the nb partials of dot k (fixed order), k < ndot
+struct SumOut {
+  double *o[FCG_NDOT];
+};
+
+__global__ void __launch_bounds__(MF6_BLOCK) k_fcg_sum(const double *part, int nb, int ndot,
+                                                       SumOut out, const KState *ks) {
+  if (ks->stop) return;
+  __shared__ double lds[MF6_BLOCK / 64];
   for (int k = 0; k < ndot; k++) {
     double s = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s = s + part[k * nb + i];
-    lds[threadIdx.x] = s;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int i = 0; i < (int)blockDim.x; i++) t = t + lds[i];
-      out[k] = t;
+      for (int v = 0; v < (int)(blockDim.x >> 6); v++) t = t + lds[v];
+      *out.o[k] = t;
     }
     __syncthreads();
   }
 }
 
-// p = z + beta p_prev
-__global__ void k_fcg_dir(int n, const double *__restrict__ z, double beta,
-                          const double *__restrict__ pp, double *__restrict__ p) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-    p[e] = beta == 0.0 ? z[e] : z[e] + beta * pp[e];
+// p = z - sum_j (c_j / pq_j) P_j over the stored slots
+__global__ void k_fcg_dir(int n, const double *__restrict__ z, const double *__restrict__ P,
+                          size_t ld, FcgSlots sl, const FcgState *fs, double *__restrict__ p,
+                          const KState *ks) {
+  if (ks->stop) return;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    double v = z[e];
+    for (int j = 0; j < sl.n; j++) {
+      const int k = sl.slot[j];
+      const double pq = fs->pqs[k];
+      if (pq != 0.0) v = v - (fs->c[k] / pq) * P[(size_t)k * ld + e];
+    }
+    p[e] = v;
+  }
 }
 
-// x += alpha p ; r -= alpha q ; partials: sum r^2, signed max |alpha p| and
-// max |r| with natural locations (first occurrence on ties)
-__global__ void __launch_bounds__(MF6_BLOCK) k_fcg_update(Layout L, double alpha,
+// x += alpha p ; r -= alpha q with alpha = p.r / p.q ; partials: sum r^2,
+// signed max |alpha p| and max |r| with natural locations (first occurrence
+// on ties)
+__global__ void __launch_bounds__(MF6_BLOCK) k_fcg_update(Layout L, const FcgState *fs, int head,
                                                           double *__restrict__ x,
                                                           const double *__restrict__ p,
                                                           double *__restrict__ r,
                                                           const double *__restrict__ q,
-                                                          double *part, MaxLoc *mpart) {
+                                                          double *part, MaxLoc *mpart,
+                                                          const KState *ks) {
+  if (ks->stop) return;
   __shared__ double lds[MF6_BLOCK / 64];
   __shared__ MaxLoc ml[2][MF6_BLOCK / 64];
+  double den = fs->pqs[head];
+  den = den + copysign(2.220446049250313080847263336181640625e-16, den);
+  const double alpha = fs->pr / den;
   double ss = 0.0;
   MaxLoc mx{0.0, -1}, mr{0.0, -1};
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.NP; e += gridDim.x * blockDim.x) {
@@ -155,9 +293,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_fcg_update(Layout L, double alpha
   }
 }
 
+// end of an iteration: norms, the IMS test (-> cand), the stall check of the
+// learned step
 __global__ void __launch_bounds__(MF6_BLOCK) k_fcg_upd_final(const double *part,
                                                              const MaxLoc *mpart, int nb,
-                                                             double *out) {
+                                                             FcgState *fs, KState *ks,
+                                                             FcgTest tst) {
+  if (ks->stop) return;
   __shared__ double lds[MF6_BLOCK / 64];
   __shared__ MaxLoc ml[2][MF6_BLOCK / 64];
   double t = 0.0;
@@ -184,15 +326,38 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_fcg_upd_final(const double *part,
     ra = maxloc_pick(ra, ml[0][v]);
     rb = maxloc_pick(rb, ml[1][v]);
   }
-  out[0] = s;
-  out[1] = ra.v;
-  out[2] = rb.v;
+  const double rnorm = sqrt(s), dvmax = ra.v, rmax = rb.v;
+  fs->rnorm = rnorm;
+  fs->dvmax = dvmax;
+  fs->rmax = rmax;
+  fs->iter++;
+  // IMS test on the signed maxima (ICNVGOPT 0 / 1), or with the L2 norm
+  bool conv = fabs(dvmax) <= tst.dvclose && fabs(rmax) <= tst.rclose;
+  if (tst.icnvgopt == 2 || tst.icnvgopt == 3 || tst.icnvgopt == 4)
+    conv = fabs(dvmax) <= tst.dvclose && (tst.icnvgopt == 3 || rnorm <= tst.rclose);
+  fs->cand = conv ? 1 : 0;
+  // stall check of the learned step over windows of ten applications
+  if (fs->learned) {
+    if (fs->window == 0) fs->rn_window = rnorm;
+    if (++fs->window == 10 * tst.nn_every) {
+      if (!(rnorm < 0.9 * fs->rn_window)) fs->learned = 0;
+      fs->window = 0;
+    }
+  }
 }
 
-// r = b - A x (amux order), the FP64 true residual
-__global__ void k_fcg_resid(Layout L, const double *__restrict__ A, const double *__restrict__ x,
-                            const double *__restrict__ b, double *__restrict__ r) {
+// r = b - A x (amux order), the FP64 true residual, when the IMS test passed;
+// partials of max |r|
+__global__ void __launch_bounds__(MF6_BLOCK) k_fcg_resid(Layout L, const double *__restrict__ A,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ b,
+                                                         double *__restrict__ r,
+                                                         const FcgState *fs, const KState *ks,
+                                                         double *part) {
+  if (ks->stop || !fs->cand) return;
+  __shared__ double lds[MF6_BLOCK / 64];
   const int NP = L.NP;
+  double m = 0.0;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NP; e += gridDim.x * blockDim.x) {
     int c, k, i, j, l;
     if (!pos_to_cell(L, e, c, k, i, j, l)) { r[e] = 0.0; continue; }
@@ -205,7 +370,39 @@ __global__ void k_fcg_resid(Layout L, const double *__restrict__ A, const double
     if (j < L.ncol - 1) t = t + A[4 * NP + e] * x[ob + hn(l, 1)];
     if (i < L.nrow - 1) t = t + A[5 * NP + e] * x[ob + hn(l, L.ncol)];
     if (k < L.nlay - 1) t = t + A[6 * NP + e] * x[e + L.HP];
-    r[e] = b[e] - t;
+    const double rv = b[e] - t;
+    r[e] = rv;
+    m = fmax(m, fabs(rv));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int v = 0; v < (int)(blockDim.x >> 6); v++) t = fmax(t, lds[v]);
+    part[blockIdx.x] = t;
+  }
+}
+
+// the true residual decides: stop when max |b - A x| <= rclose (or the
+// test ignores the residual)
+__global__ void __launch_bounds__(MF6_BLOCK) k_fcg_resid_final(const double *part, int nb,
+                                                               FcgState *fs, KState *ks,
+                                                               FcgTest tst) {
+  if (ks->stop || !fs->cand) return;
+  __shared__ double lds[MF6_BLOCK / 64];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, part[i]);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int v = 0; v < (int)(blockDim.x >> 6); v++) t = fmax(t, lds[v]);
+  fs->cand = 0;
+  if (t <= tst.rclose || tst.icnvgopt == 3) {
+    fs->icnvg = 1;
+    ks->stop = 1;
   }
 }
 
@@ -223,51 +420,40 @@ inline int gsz(int n) {
     if (rc_ != MF6HIP_OK) return rc_; \
   } while (0)
 
-// dots of up to 3 pairs (host result)
-static int dots(Mf6Handle *h, int ndot, const double *a0, const double *b0, const double *a1,
-                const double *b1, const double *a2, const double *b2, double *out) {
+// device dots of up to 3 pairs into state fields
+static void dots(Mf6Handle *h, double *part, int ndot, const double *a0, const double *b0,
+                 const double *a1, const double *b1, const double *a2, const double *b2,
+                 double *o0, double *o1, double *o2) {
   const int nb = gsz(h->L.NP);
   hipStream_t s = h->stream;
-  double *dout = h->d_part + 7 * MF6_MAXBLK;
-  k_fcg_dots<<<nb, MF6_BLOCK, 0, s>>>(h->L.NP, ndot, a0, b0, a1, b1, a2, b2, h->d_part);
-  k_fcg_sum<<<1, MF6_BLOCK, 0, s>>>(h->d_part, nb, ndot, dout);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * ndot, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return MF6HIP_OK;
+  k_fcg_dots<<<nb, MF6_BLOCK, 0, s>>>(h->L.NP, ndot, a0, b0, a1, b1, a2, b2, part, h->d_state);
+  SumOut so{};
+  so.o[0] = o0;
+  so.o[1] = o1;
+  so.o[2] = o2;
+  k_fcg_sum<<<1, MF6_BLOCK, 0, s>>>(part, nb, ndot, so, h->d_state);
 }
 
-// z = ILU0(r) + w y, y = s net(t / s), t = r - A ILU0(r), with the
-// energy-optimal weight w = (y.t) / (y.A y) (the step along y that
-// minimises the A-norm of the error left by ILU0, so a poor prediction costs
-// iterations but cannot push the iterate away); *used = 1 when the
-// network ran, *zr_ilu = ILU0(r).r (the sign a descent direction must keep)
-static int precondition(Mf6Handle *h, const double *r, double *z, bool learned, int *used,
-                        double *zr_ilu) {
+// z = ILU0(r), plus the learned step when `learned` (enqueued only; the
+// weight is decided on the device)
+static int precondition(Mf6Handle *h, FcgState *fs, double *part, const double *r, double *z,
+                        bool learned) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
-  *used = 0;
-  FCK(mf6hip_ilu_apply64(h, r, z));
+  FCK(mf6hip_ilu_apply64(h, r, z, h->d_state));
   if (!learned) return MF6HIP_OK;
   double *t = h->v, *yv = h->dhat, *ay = h->phat;
   FCK(mf6hip_spmv64(h, z, t));
-  k_fcg_sub<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, r, t);
-  double d2[2];
-  FCK(dots(h, 2, t, t, z, r, nullptr, nullptr, d2));
-  *zr_ilu = d2[1];
-  if (!(d2[0] > 0.0)) return MF6HIP_OK;
-  const double sc = sqrt(d2[0] / L.N);
-  k_fcg_nn_input<<<gsz(L.N * 16), MF6_BLOCK, 0, s>>>(L, t, 1.0 / sc, h->unet_feat,
+  k_fcg_sub<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, r, t, h->d_state);
+  dots(h, part, 2, t, t, z, r, nullptr, nullptr, &fs->tt, &fs->zr_ilu, nullptr);
+  k_fcg_nn_input<<<gsz(L.N * 16), MF6_BLOCK, 0, s>>>(L, t, fs, h->unet_feat,
                                                      mf6hip_unet_input(h->unet));
   FCK(mf6hip_unet_apply_device(h->unet, s));
-  k_fcg_nn_out<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L, mf6hip_unet_output(h->unet), sc, yv);
+  k_fcg_nn_out<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L, mf6hip_unet_output(h->unet), fs, yv);
   FCK(mf6hip_spmv64(h, yv, ay));
-  double w2[2];
-  FCK(dots(h, 2, yv, t, yv, ay, nullptr, nullptr, w2));
-  if (w2[1] != 0.0 && std::isfinite(w2[0]) && std::isfinite(w2[1])) {
-    k_fcg_axpy<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, w2[0] / w2[1], yv, z);
-    *used = 1;
-  }
+  dots(h, part, 3, yv, t, yv, ay, yv, r, &fs->yt, &fs->yay, &fs->yr);
+  k_fcg_decide<<<1, 64, 0, s>>>(fs, h->d_state);
+  k_fcg_axpy_w<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, fs, yv, z);
   HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }
@@ -281,105 +467,73 @@ int mf6hip_run_fcg(Mf6Handle *h) {
   KState *hs = h->h_state;
   const int m = h->fcg_m;
   if (!h->fcg_store) {
-    if (hipMalloc((void **)&h->fcg_store, sizeof(double) * 2 * (size_t)m * L.NP) != hipSuccess) {
+    const size_t bytes = sizeof(double) * 2 * (size_t)m * L.NP + sizeof(FcgState) +
+                         sizeof(double) * FCG_NDOT * 2048;
+    if (hipMalloc((void **)&h->fcg_store, bytes) != hipSuccess) {
       mf6hip_set_error("hipMalloc failed (flexible CG directions)");
       return MF6HIP_EHIP;
     }
   }
-  double *r = h->d, *z = h->z, *p = h->p, *q = h->q;
+  double *r = h->d, *z = h->z;
   double *P = h->fcg_store, *Q = h->fcg_store + (size_t)m * L.NP;
-  std::vector<double> pqs(m, 0.0);
-  int nstored = 0, head = 0;
+  FcgState *fs = (FcgState *)(h->fcg_store + 2 * (size_t)m * L.NP);
+  double *part = (double *)(fs + 1);
   const int nb = gsz(L.NP);
-  double *dout = h->d_part + 7 * MF6_MAXBLK;
+  FcgTest tst{h->s.dvclose, h->s.rclose, h->s.icnvgopt, h->nn_every};
   bool learned = h->unet != nullptr;
-  h->learned_iters = 0;
-  double rn_window = -1.0;
-  int window = 0;
-  int it = 0, icnvg = 0;
+  k_fcg_init<<<1, 64, 0, s>>>(fs, h->d_state, learned ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  int nstored = 0, head = 0, it = 0;
   const int itmax = h->s.iter1;
+  FcgState hf{};
   while (it < itmax) {
-    it++;
-    int used = 0;
-    double zr_ilu = 0.0;
-    const bool use_net = learned && ((it - 1) % h->nn_every == 0);
-    FCK(precondition(h, r, z, use_net, &used, &zr_ilu));
-    double zr;
-    FCK(dots(h, 1, z, r, nullptr, nullptr, nullptr, nullptr, &zr));
-    if (used && !(zr * zr_ilu > 0.0)) {
-      // z.r changed sign against the ILU0 step (M is definite with the
-      // sign of A): fall back to the ILU0 step for good
-      learned = false;
-      FCK(precondition(h, r, z, false, &used, &zr_ilu));
-    }
-    if (used) h->learned_iters++;
-    // p = z - sum_j (z.q_j / p_j.q_j) p_j over the stored directions
-    std::vector<double> c(nstored, 0.0);
-    for (int j0 = 0; j0 < nstored; j0 += 3) {
-      const int nd = nstored - j0 < 3 ? nstored - j0 : 3;
-      const double *qj[3] = {nullptr, nullptr, nullptr};
-      for (int k = 0; k < nd; k++) qj[k] = Q + (size_t)((head - 1 - j0 - k + 2 * m) % m) * L.NP;
-      FCK(dots(h, nd, z, qj[0], z, qj[1], z, qj[2], c.data() + j0));
-    }
-    k_fcg_dir<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, z, 0.0, z, p);
-    for (int j = 0; j < nstored; j++) {
-      const int slot = (head - 1 - j + 2 * m) % m;
-      if (pqs[slot] != 0.0)
-        k_fcg_axpy<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, -c[j] / pqs[slot],
-                                                   P + (size_t)slot * L.NP, p);
-    }
-    FCK(mf6hip_spmv64(h, p, q));
-    double pq[2];
-    FCK(dots(h, 2, p, q, p, r, nullptr, nullptr, pq));
-    double den = pq[0];
-    den = den + copysign(2.220446049250313080847263336181640625e-16, den);
-    const double alpha = pq[1] / den;
-    k_fcg_update<<<nb, MF6_BLOCK, 0, s>>>(L, alpha, h->x, p, r, q, h->d_part, h->d_mpart);
-    k_fcg_upd_final<<<1, MF6_BLOCK, 0, s>>>(h->d_part, h->d_mpart, nb, dout);
-    HIPCHK(hipGetLastError());
-    // store the direction in the ring (the oldest one is replaced)
-    HIPCHK(hipMemcpyAsync(P + (size_t)head * L.NP, p, sizeof(double) * L.NP,
-                          hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(Q + (size_t)head * L.NP, q, sizeof(double) * L.NP,
-                          hipMemcpyDeviceToDevice, s));
-    pqs[head] = pq[0];
-    head = (head + 1) % m;
-    if (nstored < m) nstored++;
-    double u3[3];
-    HIPCHK(hipMemcpyAsync(u3, dout, sizeof(u3), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const double rnorm = sqrt(u3[0]), dvmax = u3[1], rmax = u3[2];
-    hs->deltax = dvmax;
-    hs->rmax = rmax;
-    hs->l2norm = rnorm;
-    // IMS test on the signed maxima (ICNVGOPT 0 / 1), or with the L2 norm
-    bool conv = fabs(dvmax) <= h->s.dvclose && fabs(rmax) <= h->s.rclose;
-    if (h->s.icnvgopt == 2 || h->s.icnvgopt == 3 || h->s.icnvgopt == 4) {
-      conv = fabs(dvmax) <= h->s.dvclose && (h->s.icnvgopt == 3 || rnorm <= h->s.rclose);
-    }
-    if (conv) {
-      // the FP64 true residual is the judge
-      k_fcg_resid<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, r);
-      HIPCHK(hipGetLastError());
-      std::vector<double> tmp(L.NP);
-      HIPCHK(hipMemcpyAsync(tmp.data(), r, sizeof(double) * L.NP, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      double tmax = 0.0;
-      for (double v : tmp) tmax = fmax(tmax, fabs(v));
-      if (tmax <= h->s.rclose || h->s.icnvgopt == 3) { icnvg = 1; break; }
-    }
-    // stall check of the learned step over windows of ten applications
-    if (learned) {
-      if (window == 0) rn_window = rnorm;
-      if (++window == 10 * h->nn_every) {
-        if (!(rnorm < 0.9 * rn_window)) learned = false;
-        window = 0;
+    const int ng = itmax - it < FCG_GROUP ? itmax - it : FCG_GROUP;
+    for (int g = 0; g < ng; g++) {
+      it++;
+      const bool use_net = learned && ((it - 1) % h->nn_every == 0);
+      FCK(precondition(h, fs, part, r, z, use_net));
+      // c_j = z.q_j over the stored slots (newest first), FCG_NDOT a pass
+      FcgSlots sl{};
+      sl.n = nstored;
+      for (int j = 0; j < nstored; j++) sl.slot[j] = (head - 1 - j + 2 * m) % m;
+      for (int j0 = 0; j0 < nstored; j0 += FCG_NDOT) {
+        const int nd = nstored - j0 < FCG_NDOT ? nstored - j0 : FCG_NDOT;
+        DotArgs da{};
+        da.n = nd;
+        SumOut so{};
+        for (int k = 0; k < nd; k++) {
+          da.b[k] = Q + (size_t)sl.slot[j0 + k] * L.NP;
+          so.o[k] = &fs->c[sl.slot[j0 + k]];
+        }
+        k_fcg_mdots<<<nb, MF6_BLOCK, 0, s>>>(L.NP, z, da, part, h->d_state);
+        k_fcg_sum<<<1, MF6_BLOCK, 0, s>>>(part, nb, nd, so, h->d_state);
       }
+      // the new direction goes straight into the ring slot `head`
+      double *p = P + (size_t)head * L.NP, *q = Q + (size_t)head * L.NP;
+      k_fcg_dir<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, z, P, (size_t)L.NP, sl, fs, p, h->d_state);
+      FCK(mf6hip_spmv64(h, p, q));
+      dots(h, part, 2, p, q, p, r, nullptr, nullptr, &fs->pqs[head], &fs->pr, nullptr);
+      k_fcg_update<<<nb, MF6_BLOCK, 0, s>>>(L, fs, head, h->x, p, r, q, h->d_part, h->d_mpart,
+                                            h->d_state);
+      k_fcg_upd_final<<<1, MF6_BLOCK, 0, s>>>(h->d_part, h->d_mpart, nb, fs, h->d_state, tst);
+      k_fcg_resid<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, r, fs, h->d_state, part);
+      k_fcg_resid_final<<<1, MF6_BLOCK, 0, s>>>(part, nb, fs, h->d_state, tst);
+      HIPCHK(hipGetLastError());
+      head = (head + 1) % m;
+      if (nstored < m) nstored++;
     }
+    HIPCHK(hipMemcpyAsync(&hf, fs, sizeof(FcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hf.icnvg) break;
+    learned = learned && hf.learned;
   }
-  hs->icnvg = icnvg;
-  hs->iter = it;
+  hs->icnvg = hf.icnvg;
+  hs->iter = hf.iter;
+  hs->deltax = hf.dvmax;
+  hs->rmax = hf.rmax;
+  hs->l2norm = hf.rnorm;
   hs->stop = 1;
+  h->learned_iters = hf.learned_iters;
   HIPCHK(hipMemcpyAsync(h->d_state, hs, sizeof(KState), hipMemcpyHostToDevice, s));
   return MF6HIP_OK;
 }

@@ -127,7 +127,8 @@ int mf6hip_dxmax(Mf6Handle *h, double *hncg, int *lrch, const double *xa,
                  const double *xb, bool active_only);
 
 // solver pieces shared with the flexible CG (mf6hip_fcg.hip)
-int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z);
+// z = ILU0(r); the kernels return at once when st is given and st->stop is set
+int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z, const KState *st);
 int mf6hip_spmv64(Mf6Handle *h, const double *x, double *y);
 int mf6hip_run_fcg(Mf6Handle *h);
 int mf6hip_stage_system(const Layout &L, const int *ia1, int nja, const double *amat,

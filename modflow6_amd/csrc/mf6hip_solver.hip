@@ -995,8 +995,8 @@ int mf6hip_stage_system(const Layout &L, const int *ia1, int nja, const double *
   return rc;
 }
 
-int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z) {
-  launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, r, h->zf, z, nullptr);
+int mf6hip_ilu_apply64(Mf6Handle *h, const double *r, double *z, const KState *st) {
+  launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, r, h->zf, z, st);
   HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }
