@@ -538,7 +538,9 @@ int launch_conv(UNetHandle *u, const Grid3 &g, const bf16 *x, int ldx, int xoff,
     case 2: k_conv3<2><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
     case 3: k_conv3<3><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
     case 4: k_conv3<4><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
+    case 5: k_conv3<5><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
     case 6: k_conv3<6><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
+    case 7: k_conv3<7><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
     case 8: k_conv3<8><<<blocks, 256, 0, s>>>(g, x, ldx, xoff, cin, wp, bias, relu, y, ldy, yoff, mask, ldm, moff, ntiles); break;
     default:
       mf6hip_set_error("unsupported channel count " + std::to_string(cout));
