@@ -127,7 +127,15 @@ def _named(params, cin):
     return ch, names, dict(zip(names, params[:-1]))
 
 
-def forward(params, x, keep=False):
+def _identity(a):
+    return a
+
+
+def forward(params, x, keep=False, rnd=None):
+    """Output y (D, H, W, 1).  rnd, when given, is applied to every stored
+    activation (pooled inputs, conv outputs): with a BF16 rounding it
+    restates a device that keeps activations in BF16."""
+    R = rnd or _identity
     ch, _, P = _named(params, x.shape[-1])
     nlev = len(ch)
     st = {}
@@ -135,10 +143,10 @@ def forward(params, x, keep=False):
     acts = []
     for lvl in range(nlev):
         if lvl > 0:
-            h = pool(h)
+            h = R(pool(h))
         st[f"in{lvl}"] = h
-        za = conv3(h, *P[f"e{lvl}a"]); ha = relu(za)
-        zb = conv3(ha, *P[f"e{lvl}b"]); hb = relu(zb)
+        za = conv3(h, *P[f"e{lvl}a"]); ha = R(relu(za))
+        zb = conv3(ha, *P[f"e{lvl}b"]); hb = R(relu(zb))
         st[f"h{lvl}a"], st[f"h{lvl}b"] = ha, hb
         acts.append(hb)
         h = hb
@@ -147,17 +155,22 @@ def forward(params, x, keep=False):
         a = acts[lvl]
         cat = np.concatenate([up(u, a.shape[1], a.shape[2]), a], axis=-1)
         st[f"cat{lvl}"] = cat
-        u = relu(conv3(cat, *P[f"d{lvl}"]))
+        u = R(relu(conv3(cat, *P[f"d{lvl}"])))
         st[f"u{lvl}"] = u
     wo, bo = params[-1]
     y = u @ wo + bo
     return (y, st) if keep else y
 
 
-def loss_and_grads(params, x, t):
-    """Relative squared error of one sample and its parameter gradients."""
+def loss_and_grads(params, x, t, rnd=None):
+    """Relative squared error of one sample and its parameter gradients.
+    rnd, when given, is applied to the activations (as in forward) and to
+    every stored activation gradient (the gradients of the decoder outputs,
+    of the concatenations, of both convolutions of each encoder level and of
+    the pooled inputs)."""
+    R = rnd or _identity
     ch, names, P = _named(params, x.shape[-1])
-    y, st = forward(params, x, keep=True)
+    y, st = forward(params, x, keep=True, rnd=rnd)
     den = np.sum(t * t)
     loss = np.sum((y - t) ** 2) / den
     dy = 2.0 * (y - t) / den
@@ -169,8 +182,9 @@ def loss_and_grads(params, x, t):
     nlev = len(ch)
     dacts = {lvl: 0.0 for lvl in range(nlev)}
     for lvl in range(nlev - 1):
-        du = du * (st[f"u{lvl}"] > 0)
+        du = R(du * (st[f"u{lvl}"] > 0))
         dcat, dw, db = conv3_bwd(st[f"cat{lvl}"], P[f"d{lvl}"][0], du)
+        dcat = R(dcat)
         G[f"d{lvl}"] = (dw, db)
         cu = dcat.shape[-1] - ch[lvl]
         dacts[lvl] = dacts[lvl] + dcat[..., cu:]
@@ -179,14 +193,14 @@ def loss_and_grads(params, x, t):
     dh = None
     for lvl in range(nlev - 1, -1, -1):
         g = dacts[lvl] if dh is None else dacts[lvl] + dh
-        g = g * (st[f"h{lvl}b"] > 0)
+        g = R(g * (st[f"h{lvl}b"] > 0))
         dha, dw, db = conv3_bwd(st[f"h{lvl}a"], P[f"e{lvl}b"][0], g)
         G[f"e{lvl}b"] = (dw, db)
-        dha = dha * (st[f"h{lvl}a"] > 0)
+        dha = R(dha * (st[f"h{lvl}a"] > 0))
         din, dw, db = conv3_bwd(st[f"in{lvl}"], P[f"e{lvl}a"][0], dha)
         G[f"e{lvl}a"] = (dw, db)
         if lvl > 0:
             hp = st[f"h{lvl - 1}b"]
-            dh = pool_bwd(din, hp.shape[1], hp.shape[2])
+            dh = pool_bwd(R(din), hp.shape[1], hp.shape[2])
     grads = [G[n] for n in names] + [g_out]
     return loss, grads

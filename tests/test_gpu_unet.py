@@ -2,11 +2,14 @@
 backward, Adam) against the CPU oracle (oracle/unet_oracle.py), and the
 flexible-CG solve with the network attached.
 
-Tolerances: the device rounds weights, inputs and every activation and
-activation gradient to BF16 (8 significant bits) and accumulates in FP32,
-so outputs and gradients are compared by relative L2 error (forward 3e-2,
-weight gradients 1e-1 per layer, bias gradients 2.5e-1) and cosine similarity
-(0.999 / 0.99 / 0.97)."""
+The device keeps conv weights, activations and activation gradients in BF16
+and accumulates in FP32 (biases and the output layer stay FP32).  The oracle
+restates that with a BF16 rounding at the same storage points (rnd=), so the
+device must match it closely: relative L2 1e-2 (forward) and 2e-2
+(gradients), where the remaining difference is FP32 summation order
+flipping an occasional BF16 rounding.  Against the exact (unrounded)
+oracle the BF16 storage itself costs 3-9 % per layer, checked only as a
+sanity bound (cosine 0.98)."""
 import numpy as np
 import pytest
 
@@ -20,10 +23,13 @@ from tests.helpers import device_solver, oracle_system
 pytestmark = pytest.mark.gpu
 
 
-def to_oracle(flat, nin, ch):
+def to_oracle(flat, nin, ch, wround=None):
+    """Flat parameters -> oracle (W, b) pairs; wround rounds the conv
+    weights only (the device packs those to BF16)."""
+    wr = wround or (lambda a: a)
     out, o = [], 0
     for a, b in layer_shapes(nin, ch):
-        w = flat[o:o + 27 * a * b].reshape(27, a, b).astype(np.float64); o += 27 * a * b
+        w = wr(flat[o:o + 27 * a * b]).reshape(27, a, b).astype(np.float64); o += 27 * a * b
         bb = flat[o:o + b].astype(np.float64); o += b
         out.append((w, bb))
     wo = flat[o:o + ch[0]].reshape(ch[0], 1).astype(np.float64); o += ch[0]
@@ -40,6 +46,11 @@ def bf16(a):
     u = a.view(np.uint32).astype(np.uint64)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
     return r.astype(np.uint32).view(np.float32)
+
+
+def bf16d(a):
+    """BF16 rounding (via FP32, nearest-even) of a float64 array, as float64."""
+    return bf16(a).astype(np.float64)
 
 
 def rel(a, b):
@@ -74,11 +85,12 @@ def test_unet_forward_matches_oracle(shape, ch):
     x, _ = sample(shape, 4)
     y = net.forward(x)
     net.destroy()
-    ps = to_oracle(bf16(p), 3, ch)
     xo = bf16(x).astype(np.float64).T.reshape(D, H, W, 3)
-    yo = U.forward(ps, xo).reshape(-1)
-    print("forward rel", rel(y, yo), "cos", cos(y, yo))
-    assert rel(y, yo) < 3e-2 and cos(y, yo) > 0.999
+    yr = U.forward(to_oracle(p, 3, ch, bf16d), xo, rnd=bf16d).reshape(-1)
+    ye = U.forward(to_oracle(p, 3, ch), xo).reshape(-1)
+    print("forward rel (BF16 oracle)", rel(y, yr), "cos", cos(y, yr), "| exact", rel(y, ye))
+    assert rel(y, yr) < 1e-2 and cos(y, yr) > 0.9999
+    assert cos(y, ye) > 0.98
 
 
 @pytest.mark.parametrize("shape,ch", SHAPES)
@@ -89,21 +101,21 @@ def test_unet_gradients_match_oracle(shape, ch):
     x, t = sample(shape, 6)
     loss, g = net.grad(x, t)
     net.destroy()
-    ps = to_oracle(bf16(p), 3, ch)
     xo = bf16(x).astype(np.float64).T.reshape(D, H, W, 3)
     to = bf16(t).astype(np.float64).reshape(D, H, W, 1)
-    lo, go = U.loss_and_grads(ps, xo, to)
-    assert abs(loss - lo) <= 3e-2 * lo
+    lr_, gr = U.loss_and_grads(to_oracle(p, 3, ch, bf16d), xo, to, rnd=bf16d)
+    le, ge = U.loss_and_grads(to_oracle(p, 3, ch), xo, to)
+    assert abs(loss - lr_) <= 1e-3 * lr_
     o = 0
-    for k, (w, b) in enumerate(go):
+    for k, ((w, b), (we, be)) in enumerate(zip(gr, ge)):
         nw, nb = w.size, b.size
         gw, gb = g[o:o + nw], g[o + nw:o + nw + nb]
         o += nw + nb
-        print(f"layer {k}: weight rel {rel(gw, w.ravel()):.3e} cos {cos(gw, w.ravel()):.5f}")
-        assert rel(gw, w.ravel()) < 1e-1 and cos(gw, w.ravel()) > 0.99
-        # a bias gradient sums BF16 activation gradients of both signs over
-        # every voxel, so cancellation amplifies their rounding
-        assert rel(gb, b.ravel()) < 2.5e-1 and cos(gb, b.ravel()) > 0.97
+        print(f"layer {k}: weight rel {rel(gw, w.ravel()):.3e} bias rel {rel(gb, b.ravel()):.3e}"
+              f" | exact: weight rel {rel(gw, we.ravel()):.3e} cos {cos(gw, we.ravel()):.5f}")
+        assert rel(gw, w.ravel()) < 2e-2 and cos(gw, w.ravel()) > 0.9998
+        assert rel(gb, b.ravel()) < 2e-2
+        assert cos(gw, we.ravel()) > 0.98
     assert o == g.size
 
 
