@@ -333,6 +333,7 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, 
         fcg_ilu0_iterations=float(np.mean([e["fcg_ilu0"]["iterations"] for e in evals])),
         fcg_learned_iterations=float(np.mean([e["fcg_learned"]["iterations"] for e in evals])),
         pcg_s=float(np.mean([e["pcg"]["s"] for e in evals])),
+        fcg_ilu0_s=float(np.mean([e["fcg_ilu0"]["s"] for e in evals])),
         fcg_learned_s=float(np.mean([e["fcg_learned"]["s"] for e in evals])),
         max_head_diff_vs_pcg_m=float(max(e["fcg_learned"]["max_head_diff_vs_pcg_m"]
                                          for e in evals)),
