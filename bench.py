@@ -73,6 +73,20 @@ def true_residual(case, amat, rhs, x):
     return float(np.max(np.abs(rhs - ax)))
 
 
+def pmc_traffic():
+    """HBM bytes per launch of the FP32 SpMV at 10 M cells from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by the factors
+    measured on a copy kernel of known size in the same runs)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed_steady_10m.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"HBM traffic per launch = .* = ([0-9.]+) MB", open(path).read())
+    if not m:
+        return None, None
+    return float(m.group(1)) * 1e6, "profiles/hbm_traffic_mixed_steady_10m.md"
+
+
 def cpu_baseline(case, budget_s=30.0):
     """The reference's own ims_base_cg (oracle/_ref, flang -O2, 1 core) on
     the full system: factor + initial residual, then CG iterations timed
@@ -171,7 +185,7 @@ def ensemble_workload(args):
     cells = sum(c.nodes for c in cases)
     log(f"ensemble: {len(cases)} realisations, {cells} cells, built in "
         f"{time.perf_counter() - t0:.0f} s")
-    ens = Ensemble(cases, log=log)
+    ens = Ensemble(cases, log=log, keep_systems=True)
     tune = ens.autotune()
     log("autotune: " + tune)
     runs = []
@@ -184,14 +198,15 @@ def ensemble_workload(args):
             runs.append(ms)
     resid = []
     for i, c in enumerate(cases[:args.check]):
-        from modflow6_amd.ensemble import fixed_system
-        amat, rhs, _ = fixed_system(c)
+        amat, rhs, _ = ens.systems[i]
         x = ens.heads(i)
         ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
         rows = np.repeat(np.arange(c.nodes), np.diff(ia))
         ax = np.bincount(rows, weights=amat * x[ja - 1], minlength=c.nodes)
         resid.append(float(np.max(np.abs(rhs - ax))))
     ens.destroy()
+    from modflow6_amd.ensemble import solve_one_by_one
+    one_s, one_its = solve_one_by_one(cases, ens.systems, log=log)
     ms = float(np.mean(runs))
     print(json.dumps(dict(workload=f"ensemble of {len(cases)} steady realisations "
                                    f"(nlay 2-8, n 64-512), {cells} cells",
@@ -199,7 +214,9 @@ def ensemble_workload(args):
                           iterations_mean=float(iters.mean()), iterations_max=int(iters.max()),
                           converged=int(icnvg.sum()), systems=len(cases), autotune=tune,
                           max_true_residual_checked=max(resid) if resid else None,
-                          checked_systems=len(resid))))
+                          checked_systems=len(resid),
+                          one_by_one_s=one_s, one_by_one_iterations_mean=float(np.mean(one_its)),
+                          speedup_vs_one_by_one=one_s / (ms * 1e-3))))
 
 
 def learned_workload(args):
@@ -304,10 +321,21 @@ def main():
     else:
         sp_bytes, sp_us = fp64_spmv_bytes, summary["fp64"]["spmv_us"]
     achieved = sp_bytes / (sp_us * 1e-6) / 1e9 if sp_us > 0 else None
+    # CSR-equivalent bytes of the same product (values + int32 column
+    # indices + row pointers + x + y), FP64 and FP32 value forms
+    csr64 = 12 * nja + 20 * N + 4
+    csr32 = 8 * nja + 12 * N + 4
+    traffic, traffic_src = None, None
+    if head == "mixed" and N == 10_000_000:
+        traffic, traffic_src = pmc_traffic()
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
-                    traffic=None, kernel="k_f_spmv", bytes_per_launch=sp_bytes,
-                    avg_launch_us=sp_us)
+                    traffic=traffic, traffic_source=traffic_src,
+                    kernel="k_f_spmv", bytes_per_launch=sp_bytes,
+                    avg_launch_us=sp_us,
+                    csr_equivalent_bytes=csr32 if head == "mixed" else csr64,
+                    csr_equivalent_GBs=((csr32 if head == "mixed" else csr64) /
+                                        (sp_us * 1e-6) / 1e9) if sp_us > 0 else None)
     if "fp64" in summary:
         f = summary["fp64"]["spmv_us"]
         roofline["fp64_spmv_GBs"] = fp64_spmv_bytes / (f * 1e-6) / 1e9 if f > 0 else None

@@ -62,9 +62,10 @@ def fixed_system(case):
 class Ensemble:
     """ctypes handle of one device ensemble."""
 
-    def __init__(self, cases, log=None):
+    def __init__(self, cases, log=None, keep_systems=False):
         self.L = L = _lib.load()
         self.cases = list(cases)
+        self.systems = [] if keep_systems else None
         self.h = C.c_void_p()
         _lib.check(L.mf6hip_batch_create(len(self.cases), C.byref(self.h)),
                    "mf6hip_batch_create")
@@ -78,6 +79,8 @@ class Ensemble:
         t0 = time.perf_counter()
         for i, c in enumerate(self.cases):
             amat, rhs, x0 = fixed_system(c)
+            if self.systems is not None:
+                self.systems.append((amat, rhs, x0))
             ia, _ = dis_structure(c.nlay, c.nrow, c.ncol)
             _lib.check(L.mf6hip_batch_set_system(self.h, i, _lib.iptr(ia), amat.size,
                                                  _lib.dptr(amat), _lib.dptr(rhs),
@@ -109,3 +112,29 @@ class Ensemble:
         if self.h:
             self.L.mf6hip_batch_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def solve_one_by_one(cases, systems, log=None):
+    """The same realisations solved one at a time by the single-model device
+    solver (same mixed-precision fused CG, one handle per realisation, created
+    outside the timing).  Returns (seconds summed over the solves,
+    iterations)."""
+    from .ims import HipLinearSolver
+    total, its = 0.0, []
+    for i, (c, (amat, rhs, x0)) in enumerate(zip(cases, systems)):
+        s = ImsLinearSettings(dvclose=c.ims.inner_dvclose, rclose=c.ims.inner_rclose,
+                              iter1=c.ims.inner_maximum, ilinmeth=1,
+                              precision=_lib.MF6HIP_MIXED)
+        ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
+        sv = HipLinearSolver().initialize(ia, ja, s, c.nlay, c.nrow, c.ncol)
+        x = x0.copy()
+        sv.solve(1, amat, rhs, x)           # warm: first call builds the factor path
+        x = x0.copy()
+        t0 = time.perf_counter()
+        _, it = sv.solve(1, amat, rhs, x)
+        total += time.perf_counter() - t0
+        its.append(it)
+        sv.destroy()
+        if log and ((i + 1) % 64 == 0 or i + 1 == len(cases)):
+            log(f"one by one: {i + 1}/{len(cases)} solved, {total:.2f} s of solves")
+    return total, its

@@ -199,9 +199,10 @@ def apply_op(sv: HipLinearSolver, op, x):
     return y
 
 
-def training_pairs(sy: DeviceSystem, iterates=(25, 50, 100, 200)):
+def training_pairs(sy: DeviceSystem, iterates=(0, 25, 50, 100, 200)):
     """Samples [len(iterates)][4][nodes]: t / s, ln K, ln(K33/K), (e - z) / s
-    from FP64 ILU0-PCG iterates x_k (k = iterates) of the device solver."""
+    from FP64 ILU0-PCG iterates x_k (k = iterates; x_0 = the start heads) of
+    the device solver."""
     sv = sy.solver
     tight = sv.settings
     xs = sy.x0.copy()
@@ -211,11 +212,12 @@ def training_pairs(sy: DeviceSystem, iterates=(25, 50, 100, 200)):
     sv.solve(1, sy.amat, sy.rhs, xs)
     out = []
     for k in iterates:
-        sv.settings = ImsLinearSettings(**{**tight.__dict__, "dvclose": 0.0, "rclose": 1e-300,
-                                           "iter1": int(k)})
-        sv.reconfigure()
         xk = sy.x0.copy()
-        sv.solve(1, sy.amat, sy.rhs, xk)
+        if k > 0:
+            sv.settings = ImsLinearSettings(**{**tight.__dict__, "dvclose": 0.0,
+                                               "rclose": 1e-300, "iter1": int(k)})
+            sv.reconfigure()
+            sv.solve(1, sy.amat, sy.rhs, xk)
         rk = sy.rhs - apply_op(sv, 0, xk)
         ek = xs - xk
         z = apply_op(sv, 1, rk)
@@ -241,7 +243,7 @@ def _settings(case, **kw):
     return ImsLinearSettings(**base)
 
 
-def make_dataset(seeds, shape=(16, 256, 256), iterates=(25, 50, 100, 200), log=print):
+def make_dataset(seeds, shape=(16, 256, 256), iterates=(0, 25, 50, 100, 200), log=print):
     """Training samples from FP64 ILU0-PCG iterates on seeded fields."""
     out = []
     t0 = time.perf_counter()
