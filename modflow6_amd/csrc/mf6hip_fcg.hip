@@ -1,11 +1,14 @@
 // mf6hip_fcg.hip -- flexible preconditioned CG with the learned correction.
 //
-// Preconditioner (variable, hence flexible CG):
-//   z  = ILU0(r)                      the device ILU0 of the IMS solve
+// Preconditioner (variable, hence flexible CG), a two-level cycle with the
+// network as the coarse correction:
+//   z  = ILU0(r)                      pre-smoothing: the device ILU0
 //   t  = r - A z                      what the ILU0 step leaves
 //   y  = s * net([t / s, features])   s = rms(t): the network sees unit-scale
 //                                     residuals; its output is rescaled
 //   z += w y,  w = y.t / y.A y        energy-optimal step along y
+//   z += ILU0(r - A z)                post-smoothing: damps the rough part of
+//                                     the correction (BF16 noise of y)
 // FCG(m) (Notay 2000): p = z - sum_j (z.q_j / p_j.q_j) p_j over the last m
 // directions, alpha = p.r / p.q, x += alpha p, r -= alpha q.  The IMS
 // convergence test (ImsLinearBase.f90:1101-1146) is applied to every
@@ -35,6 +38,7 @@ constexpr int FCG_GROUP = 8;   // iterations enqueued between state reads
 
 struct FcgState {
   double tt, zr_ilu, yt, yay, yr, w;   // learned step
+  double zr_l;                         // z.r of the smoothed learned step
   double pr;                           // p.r
   double rnorm, dvmax, rmax;
   double rn_window;
@@ -111,31 +115,52 @@ __global__ void k_fcg_nn_out(Layout L, const float *__restrict__ y, const FcgSta
 }
 
 // the weight of the learned step: w = y.t / y.A y, or 0 when the network is
-// off, the step is degenerate, or z.r would change sign against the ILU0
-// step (then the network is switched off for the rest of the solve)
+// off or the step is degenerate
 __global__ void k_fcg_decide(FcgState *fs, const KState *ks) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double w = 0.0;
   if (!ks->stop && fs->learned && fs->tt > 0.0 && fs->yay != 0.0 && isfinite(fs->yt) &&
-      isfinite(fs->yay)) {
+      isfinite(fs->yay))
     w = fs->yt / fs->yay;
-    const double zr = fs->zr_ilu + w * fs->yr;
-    if (!(zr * fs->zr_ilu > 0.0)) {
-      w = 0.0;
-      fs->learned = 0;
-    }
-  }
   fs->w = w;
-  if (w != 0.0) fs->learned_iters++;
 }
 
-// z += w y
-__global__ void k_fcg_axpy_w(int n, const FcgState *fs, const double *__restrict__ y,
-                             double *__restrict__ z) {
+// zl = z + w y
+__global__ void k_fcg_axpy_w(int n, const FcgState *fs, const double *__restrict__ z,
+                             const double *__restrict__ y, double *__restrict__ zl) {
   const double w = fs->w;
   if (w == 0.0) return;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-    z[e] = z[e] + w * y[e];
+    zl[e] = z[e] + w * y[e];
+}
+
+// zl += zs (post-smoothing correction)
+__global__ void k_fcg_add(int n, const FcgState *fs, const double *__restrict__ zs,
+                          double *__restrict__ zl) {
+  if (fs->w == 0.0) return;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    zl[e] = zl[e] + zs[e];
+}
+
+// keep the learned z (zl.r has the sign of the ILU0 step's z.r: a descent
+// direction) or fall back to the ILU0 step for the rest of the solve
+__global__ void k_fcg_accept(FcgState *fs, const KState *ks) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ks->stop || fs->w == 0.0) { fs->w = 0.0; return; }
+  if (fs->zr_l * fs->zr_ilu > 0.0 && isfinite(fs->zr_l)) {
+    fs->learned_iters++;
+  } else {
+    fs->w = 0.0;
+    fs->learned = 0;
+  }
+}
+
+// z = zl when the learned step was accepted
+__global__ void k_fcg_select(int n, const FcgState *fs, const double *__restrict__ zl,
+                             double *__restrict__ z) {
+  if (fs->w == 0.0) return;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    z[e] = zl[e];
 }
 
 // up to FCG_NDOT dot products a . b_k in one pass (b_k = B + slot_k * ld,
@@ -442,7 +467,7 @@ static int precondition(Mf6Handle *h, FcgState *fs, double *part, const double *
   hipStream_t s = h->stream;
   FCK(mf6hip_ilu_apply64(h, r, z, h->d_state));
   if (!learned) return MF6HIP_OK;
-  double *t = h->v, *yv = h->dhat, *ay = h->phat;
+  double *t = h->v, *yv = h->dhat, *ay = h->phat, *zl = h->qhat;
   FCK(mf6hip_spmv64(h, z, t));
   k_fcg_sub<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, r, t, h->d_state);
   dots(h, part, 2, t, t, z, r, nullptr, nullptr, &fs->tt, &fs->zr_ilu, nullptr);
@@ -451,9 +476,17 @@ static int precondition(Mf6Handle *h, FcgState *fs, double *part, const double *
   FCK(mf6hip_unet_apply_device(h->unet, s));
   k_fcg_nn_out<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L, mf6hip_unet_output(h->unet), fs, yv);
   FCK(mf6hip_spmv64(h, yv, ay));
-  dots(h, part, 3, yv, t, yv, ay, yv, r, &fs->yt, &fs->yay, &fs->yr);
+  dots(h, part, 2, yv, t, yv, ay, nullptr, nullptr, &fs->yt, &fs->yay, nullptr);
   k_fcg_decide<<<1, 64, 0, s>>>(fs, h->d_state);
-  k_fcg_axpy_w<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, fs, yv, z);
+  k_fcg_axpy_w<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, fs, z, yv, zl);
+  // post-smoothing: zl += ILU0(r - A zl)
+  FCK(mf6hip_spmv64(h, zl, t));
+  k_fcg_sub<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, r, t, h->d_state);
+  FCK(mf6hip_ilu_apply64(h, t, yv, h->d_state));
+  k_fcg_add<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, fs, yv, zl);
+  dots(h, part, 1, zl, r, nullptr, nullptr, nullptr, nullptr, &fs->zr_l, nullptr, nullptr);
+  k_fcg_accept<<<1, 64, 0, s>>>(fs, h->d_state);
+  k_fcg_select<<<gsz(L.NP), MF6_BLOCK, 0, s>>>(L.NP, fs, zl, z);
   HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }
