@@ -329,12 +329,16 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
   if (threadIdx.x == 0) mpart[M_P * nblk + lb] = mp;
 }
 
-// q = A p (amux order, symmetric planes), partial p.q
-template <typename TM, typename TV>
+// q = A p (amux order, symmetric planes), partial p.q.  The layer walk is
+// unrolled over a compile-time bound NLMAX in chunks of four layers whose
+// loads are all issued before their arithmetic (more loads in flight per
+// thread than the runtime-bounded loop gets).
+template <typename TM, typename TV, int NLMAX>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__restrict__ A,
                                                       const TV *__restrict__ p,
                                                       TV *__restrict__ q, double *part,
                                                       const KState *st, int nblk) {
+  constexpr int CH = 4;
   const int lb = xcd_block();
   __shared__ double lds[8];
   if (st->stop) return;
@@ -351,37 +355,76 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
     if (h2 >= ncols_of(L, c)) continue;
     int l = col_l(L, c, h2);
     int i = l / L.ncol, j = l - i * L.ncol;
-    int ib = i > 0 ? hn(l, -L.ncol) : -1;
-    int il = j > 0 ? hn(l, -1) : -1;
-    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+    const int ib = hb ? hn(l, -L.ncol) : 0;
+    const int il = hl ? hn(l, -1) : 0;
+    const int ir = hr ? hn(l, 1) : 0;
+    const int ifr = hf ? hn(l, L.ncol) : 0;
     const int own = c * L.H + h2;
     const int oth = (1 - c) * L.H;
     TV pprev = (TV)0;
     TV pcur = p[own];
     TV aup = (TV)0;
-    for (int k = 0; k < L.nlay; k++) {
-      int pos = own + k * L.HP;
-      int ob = oth + k * L.HP;
-      TV pnext = (k + 1 < L.nlay) ? p[pos + L.HP] : (TV)0;
-      TV adn = (TV)A[6 * NP + pos];
-      TV tq = (TV)0;
-      tq = tq + (TV)A[pos] * pcur;
-      if (k > 0) tq = tq + aup * pprev;
-      if (ib >= 0) tq = tq + (TV)A[5 * NP + ob + ib] * p[ob + ib];
-      if (il >= 0) tq = tq + (TV)A[4 * NP + ob + il] * p[ob + il];
-      if (ir >= 0) tq = tq + (TV)A[4 * NP + pos] * p[ob + ir];
-      if (ifr >= 0) tq = tq + (TV)A[5 * NP + pos] * p[ob + ifr];
-      if (k < L.nlay - 1) tq = tq + adn * pnext;
-      q[pos] = tq;
-      s = s + (double)pcur * (double)tq;
-      aup = adn;
-      pprev = pcur;
-      pcur = pnext;
+#pragma unroll
+    for (int k0 = 0; k0 < NLMAX; k0 += CH) {
+      if (k0 >= L.nlay) break;
+      TV a0[CH], a4[CH], a5[CH], a6[CH], ab[CH], al[CH], pb[CH], pl[CH], pr[CH], pf[CH],
+          pn[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int k = k0 + u;
+        if (k < L.nlay) {
+          const int pos = own + k * L.HP, ob = oth + k * L.HP;
+          const bool dn = k < L.nlay - 1;
+          a0[u] = (TV)A[pos];
+          a6[u] = dn ? (TV)A[6 * NP + pos] : (TV)0;
+          pn[u] = dn ? p[pos + L.HP] : (TV)0;
+          ab[u] = hb ? (TV)A[5 * NP + ob + ib] : (TV)0;
+          pb[u] = hb ? p[ob + ib] : (TV)0;
+          al[u] = hl ? (TV)A[4 * NP + ob + il] : (TV)0;
+          pl[u] = hl ? p[ob + il] : (TV)0;
+          a4[u] = hr ? (TV)A[4 * NP + pos] : (TV)0;
+          pr[u] = hr ? p[ob + ir] : (TV)0;
+          a5[u] = hf ? (TV)A[5 * NP + pos] : (TV)0;
+          pf[u] = hf ? p[ob + ifr] : (TV)0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int k = k0 + u;
+        if (k < L.nlay) {
+          TV tq = (TV)0;
+          tq = tq + a0[u] * pcur;
+          if (k > 0) tq = tq + aup * pprev;
+          if (hb) tq = tq + ab[u] * pb[u];
+          if (hl) tq = tq + al[u] * pl[u];
+          if (hr) tq = tq + a4[u] * pr[u];
+          if (hf) tq = tq + a5[u] * pf[u];
+          if (k < L.nlay - 1) tq = tq + a6[u] * pn[u];
+          q[own + k * L.HP] = tq;
+          s = s + (double)pcur * (double)tq;
+          aup = a6[u];
+          pprev = pcur;
+          pcur = pn[u];
+        }
+      }
     }
   }
   s = bsum(s, lds);
   if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
+}
+
+template <typename TM, typename TV>
+static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
+                          TV *q, double *part, const KState *st) {
+  if (L.nlay <= 4)
+    k_f_spmv<TM, TV, 4><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+  else if (L.nlay <= 8)
+    k_f_spmv<TM, TV, 8><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+  else if (L.nlay <= 16)
+    k_f_spmv<TM, TV, 16><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+  else
+    k_f_spmv<TM, TV, 32><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
 }
 
 // x += apend_x p (the last iteration's head update), then no update pending
@@ -420,10 +463,17 @@ __device__ inline void testcnvg_(int icnvgopt, int *icnvg, int iiter, double dvm
 // kernels just produced, then rho / beta of the next iteration.
 // Colour partials sit at part[slot * nblk + b] for b < ncol (the colour
 // kernels' grid).
-__global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
-                          int nblk, int ncol, double *hist, int *hloc) {
-  __shared__ double lds[4][MF6_BLOCK / 64];
-  __shared__ MaxLoc mlds[2][MF6_BLOCK / 64];
+// The one-block scalar kernels run MF6_RBLOCK threads, so each thread adds
+// only a few of the (thousands of) block partials: their loads come from
+// other XCDs' writes and the loop over them is latency-bound.
+#define MF6_RBLOCK 1024
+
+__global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
+                                                        const double *part,
+                                                        const MaxLoc *mpart, int nblk,
+                                                        int ncol, double *hist, int *hloc) {
+  __shared__ double lds[4][MF6_RBLOCK / 64];
+  __shared__ MaxLoc mlds[2][MF6_RBLOCK / 64];
   if (st->stop) return;
   // the six reductions in one pass over the partials (independent loads in
   // flight together); per slot the summation order is that of rsum/rmaxl
@@ -496,10 +546,11 @@ __global__ void k_f_round(KState *st, FState *fs, const double *part, const MaxL
   if (it > 0) st->beta = rho_new / st->rho0;
 }
 
-__global__ void k_f_alpha(KState *st, FState *fs, const double *part, const MaxLoc *mpart,
-                          int nblk) {
-  __shared__ double lds[MF6_BLOCK / 64];
-  __shared__ MaxLoc mlds[MF6_BLOCK / 64];
+__global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
+                                                        const double *part,
+                                                        const MaxLoc *mpart, int nblk) {
+  __shared__ double lds[MF6_RBLOCK / 64];
+  __shared__ MaxLoc mlds[MF6_RBLOCK / 64];
   if (st->stop) return;
   double a = 0.0;
   MaxLoc mm{0.0, -1};
@@ -652,7 +703,7 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
     for (int it = 0; it < batch; it++, round++) {
       launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs,
                                         nbc, nb);
-      k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
+      k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                         h->d_hloc);
       double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
       k_f_pupdate<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->z, pin, pout, h->x, h->d_mpart,
@@ -663,9 +714,9 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
         slot = h->last_spmv_n++;
         (void)hipEventRecord(h->spmv_ev[2 * slot], s);
       }
-      k_f_spmv<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->A, pout, h->q, h->d_part, st, nb);
+      launch_f_spmv<double, double>(L, nb, s, h->A, pout, h->q, h->d_part, st);
       if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
-      k_f_alpha<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
+      k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
     }
     HIPCHK(hipGetLastError());
     CK(read_states(h));
@@ -712,7 +763,7 @@ static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters
       for (int it = 0; it < batch; it++, round++) {
         launch_pc<float, float, TP>(h, h->A32, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs,
                                     nbc, nb);
-        k_f_round<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
+        k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                           h->d_hloc);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
         k_f_pupdate<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->z32, pin, pout, h->e32,
@@ -723,10 +774,9 @@ static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters
           slot = h->last_spmv_n++;
           (void)hipEventRecord(h->spmv_ev[2 * slot], s);
         }
-        k_f_spmv<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->A32, pout, h->q32, h->d_part, st,
-                                                        nb);
+        launch_f_spmv<float, float>(L, nb, s, h->A32, pout, h->q32, h->d_part, st);
         if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
-        k_f_alpha<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
+        k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
       }
       HIPCHK(hipGetLastError());
       CK(read_states(h));
