@@ -474,7 +474,17 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
                                                         int ncol, double *hist, int *hloc) {
   __shared__ double lds[4][MF6_RBLOCK / 64];
   __shared__ MaxLoc mlds[2][MF6_RBLOCK / 64];
-  if (st->stop) return;
+  // the state is read before the partials (and the stop flag is only acted
+  // on after them), so its load latency overlaps theirs
+  const int stop = st->stop;
+  KState S;
+  double pmax = 0.0;
+  int ploc = -1;
+  if (threadIdx.x == 0) {
+    S = *st;
+    pmax = fs->pmax;
+    ploc = fs->ploc;
+  }
   // the six reductions in one pass over the partials (independent loads in
   // flight together); per slot the summation order is that of rsum/rmaxl
   double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -499,7 +509,7 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
     if (ln == 0) mlds[q][w] = mm[q];
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || stop) return;
   double tot[4] = {0.0, 0.0, 0.0, 0.0};
   MaxLoc mt[2] = {{0.0, -1}, {0.0, -1}};
   for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
@@ -511,39 +521,39 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
   const double ssr = tot[0], ssb = tot[1], rb = tot[2], rr = tot[3];
   MaxLoc md = maxloc_pick(mt[0], mt[1]);
   double rho_new = rb + rr;
-  int it = st->iter;   // iteration whose residual was just formed (0: initial)
+  int it = S.iter;   // iteration whose residual was just formed (0: initial)
   fs->apend_d = 0.0;
   if (it > 0) {
     double l2norm = sqrt(ssr + ssb);
-    double deltax = st->alpha * fs->pmax;
+    double deltax = S.alpha * pmax;
     st->deltax = deltax;
-    st->xloc = fs->ploc;
+    st->xloc = ploc;
     st->rmax = md.v;
     st->rloc = md.loc;
     st->l2norm = l2norm;
-    if (it <= st->hist_cap) {
-      int cap = st->hist_cap;
+    if (it <= S.hist_cap) {
+      int cap = S.hist_cap;
       hist[it - 1] = deltax;
       hist[cap + it - 1] = md.v;
-      hist[2 * cap + it - 1] = st->alpha;
+      hist[2 * cap + it - 1] = S.alpha;
       hist[3 * cap + it - 1] = 0.0;
-      hloc[it - 1] = fs->ploc;
+      hloc[it - 1] = ploc;
       hloc[cap + it - 1] = md.loc;
     }
-    int o = st->icnvgopt;
+    int o = S.icnvgopt;
     double rcnvg = (o == 2 || o == 3 || o == 4) ? l2norm : md.v;
-    int icnvg = st->icnvg;
-    testcnvg_(o, &icnvg, it, deltax, rcnvg, st->l2norm0, st->epfact, st->dvclose, st->rclose);
+    int icnvg = S.icnvg;
+    testcnvg_(o, &icnvg, it, deltax, rcnvg, S.l2norm0, S.epfact, S.dvclose, S.rclose);
     if (rcnvg == 0.0) icnvg = 1;
     st->icnvg = icnvg;
     if (icnvg != 0) { st->stop = 1; return; }
-    if (is_close_(st->rho, st->rho0)) { st->stop = 2; return; }
-    if (st->rho == 0.0) { st->stop = 2; return; }
-    st->rho0 = st->rho;
-    if (it >= st->itmax) { st->stop = 3; return; }
+    if (is_close_(S.rho, S.rho0)) { st->stop = 2; return; }
+    if (S.rho == 0.0) { st->stop = 2; return; }
+    st->rho0 = S.rho;
+    if (it >= S.itmax) { st->stop = 3; return; }
   }
   st->rho = rho_new;
-  if (it > 0) st->beta = rho_new / st->rho0;
+  if (it > 0) st->beta = rho_new / S.rho;   // rho0 was just set to the old rho
 }
 
 __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
@@ -551,7 +561,13 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
                                                         const MaxLoc *mpart, int nblk) {
   __shared__ double lds[MF6_RBLOCK / 64];
   __shared__ MaxLoc mlds[MF6_RBLOCK / 64];
-  if (st->stop) return;
+  const int stop = st->stop;
+  double rho = 0.0;
+  int iter = 0;
+  if (threadIdx.x == 0) {
+    rho = st->rho;
+    iter = st->iter;
+  }
   double a = 0.0;
   MaxLoc mm{0.0, -1};
   for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
@@ -563,7 +579,7 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
   mm = wmax(mm);
   if (ln == 0) { lds[w] = a; mlds[w] = mm; }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || stop) return;
   double pq = 0.0;
   MaxLoc mp{0.0, -1};
   for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
@@ -573,8 +589,8 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
   double den = pq;
   den = den + copysign(DPREC_F, den);
   st->pq = pq;
-  st->alpha = st->rho / den;
-  st->iter = st->iter + 1;
+  st->alpha = rho / den;
+  st->iter = iter + 1;
   fs->apend_x = st->alpha;
   fs->apend_d = st->alpha;
   fs->pmax = mp.v;
