@@ -236,7 +236,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--precision", default="both",
+                    choices=["both", "fp64", "mixed", "mixed16"],
+                    help="both: FP64, mixed (FP32 factor) and mixed16 (FP16 factor)")
     ap.add_argument("--shape", default="10,1000,1000")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--replace", type=int, default=0,
@@ -284,6 +286,11 @@ def main():
     if args.precision in ("both", "mixed"):
         res["mixed"] = run_precision(case, _lib.MF6HIP_MIXED, args.steps, args.warmup, "mixed",
                                      replace=args.replace)
+    if args.precision in ("both", "mixed16"):
+        # the FP16 factor variant: ILU0 planes in FP16 (power-of-two scaled),
+        # SpMV and Krylov vectors FP32, FP64 residual replacement
+        res["mixed16"] = run_precision(case, _lib.MF6HIP_MIXED_FP16, args.steps, args.warmup,
+                                       "mixed16", replace=args.replace)
 
     def mean(lst, key):
         return float(np.mean([r[key] for r in lst]))
@@ -296,9 +303,10 @@ def main():
                           spmv_us=mean(recs, "spmv_ms") * 1e3,
                           true_residual_inf=true_residual(case, amat, rhs, heads),
                           converged=bool(recs[-1]["icnvg"]))
-    if "fp64" in res and "mixed" in res:
-        dh = float(np.max(np.abs(res["mixed"][1] - res["fp64"][1])))
-        summary["mixed"]["max_head_diff_vs_fp64_m"] = dh
+    for mk in ("mixed", "mixed16"):
+        if "fp64" in res and mk in res:
+            dh = float(np.max(np.abs(res[mk][1] - res["fp64"][1])))
+            summary[mk]["max_head_diff_vs_fp64_m"] = dh
     # heads against the CPU oracle (FP64 natural-order ILU0-PCG, the
     # reference algorithm) at 4096 seeded cells of the same solve
     gold = os.path.join(ROOT, "tests", "golden", "steady_10m_heads_sample.npz")
@@ -371,6 +379,9 @@ def main():
     }
     if "fp64" in summary and "mixed" in summary:
         line["speedup_mixed_vs_fp64"] = summary["fp64"]["s_per_solve"] / summary["mixed"]["s_per_solve"]
+    if "fp64" in summary and "mixed16" in summary:
+        line["speedup_mixed16_vs_fp64"] = (summary["fp64"]["s_per_solve"] /
+                                           summary["mixed16"]["s_per_solve"])
     print(json.dumps(line))
 
 

@@ -165,6 +165,14 @@ int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
 /* Heads in / out (natural order, host memory). */
 int mf6hip_gwf_set_x(void *handle, const double *x);
 int mf6hip_gwf_get_x(void *handle, double *x);
+/* The same exchange with a device buffer of N doubles in DIS node order (a
+ * coupler's GPU array, e.g. a torch tensor): no host copy.  Replaces reading
+ * / writing SLN_1/X through get_value_ptr (srcbmi/mf6bmi.f90:707) between
+ * prepare_solve and finalize_solve (srcbmi/mf6xmi.F90:221-340) when the
+ * coupler's data already lives on the GPU.  Both return after the copy has
+ * completed on the library's stream. */
+int mf6hip_gwf_set_x_device(void *handle, const double *x_dev);
+int mf6hip_gwf_get_x_device(void *handle, double *x_dev);
 
 /* Start a time step (gwf_ad, gwf.f90:396-421 with chd_ad): xold = x, CHD
  * cells take their head; iss = 1 for a steady-state period; delt the step

@@ -24,7 +24,8 @@ PKG = {"WEL": 1, "DRN": 2, "RCH": 3, "CHD": 4}
 EXPORTS = [
     "mf6hip_create", "mf6hip_solve", "mf6hip_destroy", "mf6hip_last_error",
     "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_x",
-    "mf6hip_gwf_get_x", "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
+    "mf6hip_gwf_get_x", "mf6hip_gwf_set_x_device", "mf6hip_gwf_get_x_device",
+    "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
     "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
     "mf6hip_gwf_residual_norms", "mf6hip_gwf_backtrack",
     "mf6hip_last_timing", "mf6hip_set_timing", "mf6hip_apply_op",
@@ -129,6 +130,8 @@ def load():
     L.mf6hip_gwf_set_package.argtypes = [vp, i, i, ip, dp, dp]
     L.mf6hip_gwf_set_x.argtypes = [vp, dp]
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
+    L.mf6hip_gwf_set_x_device.argtypes = [vp, vp]
+    L.mf6hip_gwf_get_x_device.argtypes = [vp, vp]
     L.mf6hip_gwf_advance.argtypes = [vp, i, d]
     L.mf6hip_gwf_formulate.argtypes = [vp, i, i, dp, dp]
     L.mf6hip_gwf_residual_norms.argtypes = [vp, dp, dp]

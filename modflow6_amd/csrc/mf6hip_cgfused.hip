@@ -175,6 +175,11 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__r
   if (threadIdx.x == 0) mpart[M_D_R * nblk + lb] = md;
 }
 
+// The preconditioner planes may be stored scaled by sc = 2^-k with the
+// inverse pivots scaled by 2^k (FP16 planes): the forward products A * piv
+// are then unscaled, and each backward sweep starts from its forward value
+// times sc; sc = 1 otherwise (multiplying by 1 changes nothing).
+//
 // Black columns: forward sweep (red neighbours + cell above) written to z,
 // then the backward sweep (cell below) over the same column.  Staging the
 // forward values in z instead of registers keeps the register count low
@@ -187,7 +192,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
                                                        const TV *__restrict__ zf,
                                                        TV *__restrict__ z, double *part,
                                                        MaxLoc *mpart, const KState *st,
-                                                       const FState *fs, int nblk) {
+                                                       const FState *fs, int nblk, TV sc) {
   const int lb = xcd_block();
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
@@ -228,7 +233,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
     TV zdn = (TV)0;
     for (int k = L.nlay - 1; k >= 0; k--) {
       int pos = L.H + k * L.HP + h2;
-      TV tv = z[pos];
+      TV tv = z[pos] * sc;
       if (k < L.nlay - 1) tv = tv - (TV)A[6 * NP + pos] * zdn;
       tv = tv * (TV)ldp(piv, pos);
       z[pos] = tv;
@@ -250,7 +255,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
                                                          const TV *__restrict__ d,
                                                          const TV *__restrict__ zf,
                                                          TV *__restrict__ z, double *part,
-                                                         const KState *st, int nblk) {
+                                                         const KState *st, int nblk, TV sc) {
   const int lb = xcd_block();
   __shared__ double lds[8];
   if (st->stop) return;
@@ -268,7 +273,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
     for (int k = L.nlay - 1; k >= 0; k--) {
       int pos = k * L.HP + h2;
       int bb = L.H + k * L.HP;
-      TV tv = zf[pos];
+      TV tv = zf[pos] * sc;
       if (k < L.nlay - 1) tv = tv - (TV)A[6 * NP + pos] * zdn;
       if (ib >= 0) tv = tv - (TV)A[5 * NP + bb + ib] * z[bb + ib];
       if (il >= 0) tv = tv - (TV)A[4 * NP + bb + il] * z[bb + il];
@@ -680,14 +685,14 @@ static inline int gridc(int n) {
 // half of the logical blocks idle.
 template <typename TM, typename TV, typename TP>
 static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV *q, TV *zf,
-                      TV *z, KState *st, FState *fs, int nbc, int nb) {
+                      TV *z, KState *st, FState *fs, int nbc, int nb, TV sc = (TV)1) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
   k_f_red_fwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
                                                     st, fs, nb);
   k_f_black<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
-                                                  st, fs, nb);
-  k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb);
+                                                  st, fs, nb, sc);
+  k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb, sc);
 }
 
 static int read_states(Mf6Handle *h) {
@@ -751,8 +756,9 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
 
 // Mixed precision: FP32 fused CG with reliable updates.  Requires the FP32
 // planes (A32) and pivots; x (FP64) holds the start heads, b the rhs.
-template <typename TP>
-static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters) {
+template <typename TMP, typename TP>
+static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float sc, int *icnvg,
+                           int *iters) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
   KState *st = h->d_state;
@@ -777,8 +783,8 @@ static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters
     int batch = 8;
     while (true) {
       for (int it = 0; it < batch; it++, round++) {
-        launch_pc<float, float, TP>(h, h->A32, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs,
-                                    nbc, nb);
+        launch_pc<TMP, float, TP>(h, Apc, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs, nbc,
+                                  nb, sc);
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                           h->d_hloc);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
@@ -835,8 +841,10 @@ static int run_cg_reliable(Mf6Handle *h, const TP *piv32, int *icnvg, int *iters
 }
 
 int mf6hip_run_cg_reliable(Mf6Handle *h, int *icnvg, int *iters) {
-  if (h->s.precision == MF6HIP_MIXED) return run_cg_reliable<float>(h, h->invpiv32, icnvg, iters);
-  return run_cg_reliable<unsigned short>(h, h->invpiv16, icnvg, iters);
+  if (h->s.precision == MF6HIP_MIXED)
+    return run_cg_reliable<float, float>(h, h->A32, h->invpiv32, 1.0f, icnvg, iters);
+  // MIXED_FP16: FP16 preconditioner planes (scaled), FP32 SpMV planes
+  return run_cg_reliable<_Float16, float>(h, h->A16, h->invpiv32s, h->pc_scale, icnvg, iters);
 }
 
 // bitwise symmetry of the planes: A(n, up) == A(up, n), A(n, back) ==

@@ -930,6 +930,39 @@ extern "C" int mf6hip_gwf_get_x(void *handle, double *x) {
   return MF6HIP_OK;
 }
 
+namespace {
+// x_nat[n] = x[pos(n)]
+__global__ void k_x_to_natural(Layout L, const double *__restrict__ x, double *__restrict__ xn) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < L.N; n += gridDim.x * blockDim.x)
+    xn[n] = x[pos_of_natural(L, n)];
+}
+// x[pos(n)] = x_nat[n] (padding positions untouched)
+__global__ void k_x_from_natural(Layout L, const double *__restrict__ xn, double *__restrict__ x) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < L.N; n += gridDim.x * blockDim.x)
+    x[pos_of_natural(L, n)] = xn[n];
+}
+}  // namespace
+
+extern "C" int mf6hip_gwf_set_x_device(void *handle, const double *x_dev) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !x_dev) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  k_x_from_natural<<<ggrid(L.N), MF6_BLOCK, 0, h->stream>>>(L, x_dev, h->x);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return MF6HIP_OK;
+}
+
+extern "C" int mf6hip_gwf_get_x_device(void *handle, double *x_dev) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !x_dev) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  k_x_to_natural<<<ggrid(L.N), MF6_BLOCK, 0, h->stream>>>(L, h->x, x_dev);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return MF6HIP_OK;
+}
+
 extern "C" int mf6hip_gwf_advance(void *handle, int iss, double delt) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }

@@ -68,6 +68,13 @@ struct Mf6Handle {
   // mixed precision (FP32 inner solve)
   float *A32 = nullptr, *invpiv32 = nullptr;
   unsigned short *invpiv16 = nullptr;
+  // MIXED_FP16 fused CG: ILU0 planes in FP16 scaled by pc_scale = 2^-k, with
+  // the FP32 inverse pivots scaled by 2^k (invpiv32s); pc_scale keeps the
+  // largest off-diagonal below 2^14
+  _Float16 *A16 = nullptr;
+  float *invpiv32s = nullptr;
+  float pc_scale = 1.0f;
+  double *d_amax = nullptr;
   float *r32 = nullptr, *e32 = nullptr, *p32 = nullptr, *p232 = nullptr,
         *q32 = nullptr, *z32 = nullptr, *zf32 = nullptr;
   // reductions / state

@@ -169,6 +169,30 @@ __global__ void k_pos_to_nat(Layout L, const TI *__restrict__ in, TO *__restrict
   }
 }
 
+// max |x| over n values, as the bit pattern of a non-negative double (whose
+// integer order is the value order)
+__global__ void k_maxabs_bits(int n, const double *__restrict__ x, unsigned long long *out) {
+  double m = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    m = fmax(m, fabs(x[e]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// y = (FP16)(x * scale) (scale a power of two: exact before the rounding)
+__global__ void k_planes_to_half(int n, const double *__restrict__ x, double scale,
+                                 _Float16 *__restrict__ y) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    y[e] = (_Float16)(x[e] * scale);
+}
+
+// y = (float)(x * scale)
+__global__ void k_scale_to_float(int n, const double *__restrict__ x, double scale,
+                                 float *__restrict__ y) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    y[e] = (float)(x[e] * scale);
+}
+
 template <typename TI, typename TO>
 __global__ void k_convert(int n, const TI *__restrict__ in, TO *__restrict__ out) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
@@ -914,6 +938,25 @@ static int ilu_factor(Mf6Handle *h) {
     k_convert<double, float><<<grid_for(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->invpiv, h->invpiv32);
   } else if (h->s.precision == MF6HIP_MIXED_FP16) {
     k_to_half<<<grid_for(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->invpiv, h->invpiv16);
+    // scaled FP16 planes for the fused CG's preconditioner: 2^-k with k the
+    // smallest integer that brings max |off-diagonal| to 2^14 or below
+    HIPCHK(hipMemsetAsync(h->d_amax, 0, sizeof(double), s));
+    k_maxabs_bits<<<grid_for(3 * L.NP), MF6_BLOCK, 0, s>>>(3 * L.NP, h->A + 4 * (size_t)L.NP,
+                                                          (unsigned long long *)h->d_amax);
+    double amax = 0.0;
+    HIPCHK(hipMemcpyAsync(&amax, h->d_amax, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int k = 0;
+    if (amax > 0.0 && std::isfinite(amax)) {
+      k = (int)std::ceil(std::log2(amax / 16384.0));
+      if (k < -60) k = -60;
+      if (k > 60) k = 60;
+    }
+    h->pc_scale = std::ldexp(1.0f, -k);
+    k_planes_to_half<<<grid_for(3 * L.NP), MF6_BLOCK, 0, s>>>(
+        3 * L.NP, h->A + 4 * (size_t)L.NP, std::ldexp(1.0, -k), h->A16 + 4 * (size_t)L.NP);
+    k_scale_to_float<<<grid_for(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->invpiv, std::ldexp(1.0, k),
+                                                          h->invpiv32s);
   }
   HIPCHK(hipGetLastError());
   return MF6HIP_OK;
@@ -1436,7 +1479,12 @@ static int alloc_solver(Mf6Handle *h) {
   } else {
     CK(dalloc(&h->A32, MF6_NSLOT * N));
     if (h->s.precision == MF6HIP_MIXED) CK(dalloc(&h->invpiv32, N));
-    else CK(dalloc(&h->invpiv16, N));
+    else {
+      CK(dalloc(&h->invpiv16, N));
+      CK(dalloc(&h->A16, MF6_NSLOT * N));
+      CK(dalloc(&h->invpiv32s, N));
+      CK(dalloc(&h->d_amax, 1));
+    }
     CK(dalloc(&h->r32, N));
     CK(dalloc(&h->e32, N));
     CK(dalloc(&h->p32, N));
@@ -1555,7 +1603,7 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->area, h->k11, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
-                  h->unet_feat, h->fcg_store};
+                  h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int t = 0; t < 5; t++) {
