@@ -315,7 +315,7 @@ def main():
         for k, (recs, heads, amat, rhs) in res.items():
             summary[k]["max_head_diff_vs_oracle_m"] = float(
                 np.max(np.abs(heads[g["cells"]] - g["heads"])))
-    head = "mixed" if "mixed" in summary else "fp64"
+    head = next(k for k in ("mixed", "mixed16", "fp64") if k in summary)
     value = summary[head]["s_per_solve"]
 
     # roofline of the dominant kernel, k_f_spmv (CG SpMV q = A p): symmetric
@@ -324,8 +324,9 @@ def main():
     njas = (nja - N) // 2
     fp64_spmv_bytes = 8 * (N + njas) + 2 * 8 * N
     fp32_spmv_bytes = 4 * (N + njas) + 2 * 4 * N
-    if head == "mixed":
-        sp_bytes, sp_us = fp32_spmv_bytes, summary["mixed"]["spmv_us"]
+    f32 = head in ("mixed", "mixed16")
+    if f32:
+        sp_bytes, sp_us = fp32_spmv_bytes, summary[head]["spmv_us"]
     else:
         sp_bytes, sp_us = fp64_spmv_bytes, summary["fp64"]["spmv_us"]
     achieved = sp_bytes / (sp_us * 1e-6) / 1e9 if sp_us > 0 else None
@@ -334,15 +335,15 @@ def main():
     csr64 = 12 * nja + 20 * N + 4
     csr32 = 8 * nja + 12 * N + 4
     traffic, traffic_src = None, None
-    if head == "mixed" and N == 10_000_000:
+    if f32 and N == 10_000_000:
         traffic, traffic_src = pmc_traffic()
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=traffic, traffic_source=traffic_src,
                     kernel="k_f_spmv", bytes_per_launch=sp_bytes,
                     avg_launch_us=sp_us,
-                    csr_equivalent_bytes=csr32 if head == "mixed" else csr64,
-                    csr_equivalent_GBs=((csr32 if head == "mixed" else csr64) /
+                    csr_equivalent_bytes=csr32 if f32 else csr64,
+                    csr_equivalent_GBs=((csr32 if f32 else csr64) /
                                         (sp_us * 1e-6) / 1e9) if sp_us > 0 else None)
     if "fp64" in summary:
         f = summary["fp64"]["spmv_us"]
@@ -367,7 +368,7 @@ def main():
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": value * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-in-f64" if head == "mixed" else "f64",
+        "dtype": "f32-in-f64" if f32 else "f64",
         "data": "synthetic (seeded lognormal K, CHD/RCH/WEL)",
         "config": {"workload": f"steady confined GWF {nlay}x{nrow}x{ncol} DIS "
                                f"({N} cells, {nja} nonzeros), CG + column red-black ILU0, "
