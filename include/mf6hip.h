@@ -188,6 +188,17 @@ int mf6hip_gwf_advance(void *handle, int iss, double delt);
 int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double *amat_out,
                          double *rhs_out);
 
+/* Budget of the current heads: gwf_cq + gwf_bd (gwf.f90:741-800), i.e.
+ * npf_cq / qcalc (gwf-npf.f90:745-862), sto_cq (gwf-sto.f90:447-564),
+ * bnd_cq (BoundaryPackage.f90:545-619), csr_diagsum (Sparse.f90:262-281)
+ * and calc_chd_rate (gwf-chd.f90:264-320).  flowja (may be NULL): nja values
+ * in the CSR order of mf6hip_create, positive into the cell, the diagonal the
+ * cell's flow residual.  terms[12]: (rate in, rate out) of STO-SS, STO-SY,
+ * WEL, DRN, RCH, CHD (zeros for an absent term); pct_discrepancy as
+ * Budget.f90:259-267.  The conductances use the sat of the last
+ * mf6hip_gwf_formulate, as npf_cq uses the sat of the last npf_cf. */
+int mf6hip_gwf_budget(void *handle, double *flowja, double *terms, double *pct_discrepancy);
+
 /* r = A x - b of the current device system with inactive rows zeroed
  * (sln_calc_residual, NumericalSolution.f90:2966-2982): *l2norm = ||r||_2
  * (sln_l2norm, :2855-2871) and *rvmax = max |r(n)| / (area (top - bot))

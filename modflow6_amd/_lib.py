@@ -27,7 +27,7 @@ EXPORTS = [
     "mf6hip_gwf_get_x", "mf6hip_gwf_set_x_device", "mf6hip_gwf_get_x_device",
     "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
     "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
-    "mf6hip_gwf_residual_norms", "mf6hip_gwf_backtrack",
+    "mf6hip_gwf_residual_norms", "mf6hip_gwf_backtrack", "mf6hip_gwf_budget",
     "mf6hip_last_timing", "mf6hip_set_timing", "mf6hip_apply_op",
     "mf6hip_load_factor", "mf6hip_get_system",
     "mf6hip_unet_create", "mf6hip_unet_destroy", "mf6hip_unet_num_params",
@@ -131,6 +131,7 @@ def load():
     L.mf6hip_gwf_set_x.argtypes = [vp, dp]
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
     L.mf6hip_gwf_set_x_device.argtypes = [vp, vp]
+    L.mf6hip_gwf_budget.argtypes = [vp, dp, dp, C.POINTER(C.c_double)]
     L.mf6hip_gwf_get_x_device.argtypes = [vp, vp]
     L.mf6hip_gwf_advance.argtypes = [vp, i, d]
     L.mf6hip_gwf_formulate.argtypes = [vp, i, i, dp, dp]

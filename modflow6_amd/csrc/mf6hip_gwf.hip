@@ -156,6 +156,7 @@ __device__ inline double vcond(int ibdn, int ibdm, int ictn, int ictm, int ivarc
 struct GwfDev {
   Layout L;
   const double *top, *bot, *area, *k11, *k33, *ss, *sy, *delr, *delc, *condsat;
+  double *sat;   // npf_cf's sat, written by the formulate (read by the budget)
   const int *icelltype, *iconvert, *ibound;
   int inewton, ivarcv, idewatcv, icellavg, has_sto;
   int fn;   // add the Newton terms (npf_fn, sto_fn): gwf_fc inwtflag
@@ -200,6 +201,22 @@ __device__ inline double pair_cond(const GwfDev &g, int dir, int pa, int pb, int
                                    int ja_, double ha, double hb, double csat) {
   Conn c = conn_geom(g, dir, pa, pb, ia_, ja_);
   double sata = cell_sat(g, pa, ha), satb = cell_sat(g, pb, hb);
+  if (c.ihc == 0) {
+    return vcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.ivarcv,
+                 g.idewatcv, csat, ha, hb, g.k33[pa], g.k33[pb], sata, satb, g.top[pa],
+                 g.top[pb], g.bot[pa], g.bot[pb], c.hwva);
+  }
+  return hcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.inewton,
+               g.icellavg, csat, ha, hb, sata, satb, g.k11[pa], g.k11[pb], g.top[pa],
+               g.top[pb], g.bot[pa], g.bot[pb], c.cl1, c.cl2, c.hwva);
+}
+
+// qcalc's conductance (gwf-npf.f90:797-862): npf_fc's formula with the sat of
+// the last npf_cf and the heads of the budget
+__device__ inline double pair_cond_sat(const GwfDev &g, int dir, int pa, int pb, int ia_,
+                                       int ja_, double ha, double hb, double sata,
+                                       double satb, double csat) {
+  Conn c = conn_geom(g, dir, pa, pb, ia_, ja_);
   if (c.ihc == 0) {
     return vcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.ivarcv,
                  g.idewatcv, csat, ha, hb, g.k33[pa], g.k33[pb], sata, satb, g.top[pa],
@@ -286,6 +303,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, in
     int c, k, i, j, l;
     if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
     const double hcell = x[p];
+    g.sat[p] = cell_sat(g, p, hcell);   // npf_cf (gwf-npf.f90:444-470)
     // neighbour table in ascending natural order: up, back, left, right,
     // front, down.  nb = position, dir = connection plane, lower = pair
     // (nb, n) has nb first
@@ -739,6 +757,211 @@ __global__ void k_nur(Layout L, const int *__restrict__ ibound, const int *__res
 }
 
 // ---------------------------------------------------------------------------
+// budget: gwf_cq + gwf_bd (gwf.f90:741-800)
+// ---------------------------------------------------------------------------
+
+// bnd_cq_simrate (BoundaryPackage.f90:583-619) without mover: the rate into
+// the model of every bound, hcof h - rhs for active cells
+__global__ void k_bnd_rate(int nb, const int *__restrict__ nodes, Layout L,
+                           const int *__restrict__ ibound, const double *__restrict__ x,
+                           const double *__restrict__ hcof, const double *__restrict__ brhs,
+                           double *__restrict__ rate) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
+    const int p = pos_of_natural(L, nodes[e]);
+    rate[e] = ibound[p] > 0 ? hcof[e] * x[p] - brhs[e] : 0.0;
+  }
+}
+
+struct BndRates {
+  const int *off, *ent;
+  const double *rate[5];
+};
+
+// One thread per cell: the face flows of its row in CSR order (npf_cq /
+// qcalc: qnm = cond (hm - hn) computed for the lower node first, the upper
+// node's entry its negative), then the diagonal in the reference's order of
+// accumulation: storage (sto_cq: SsTerms then SyTerms rates), the package
+// rates in bndlist order (bnd_cq), the face flows in CSR order
+// (csr_diagsum) and, for a CHD cell, its CHD rate (calc_chd_rate: minus the
+// inflow from its neighbours).  strg receives the STO-SS / STO-SY rates.
+__global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
+    GwfDev g, BndRates br, const int *__restrict__ ia, const int *__restrict__ chd_cell,
+    int iss, double delt, const double *__restrict__ x, const double *__restrict__ xold,
+    double *__restrict__ flowja, double *__restrict__ strg) {
+  const Layout &L = g.L;
+  const int N = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < L.NP; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    const int n = natural(L, k, i, j);
+    const double hcell = x[p];
+    int npos[6], pres[6];
+    pres[0] = k > 0;
+    pres[1] = i > 0;
+    pres[2] = j > 0;
+    pres[3] = j < L.ncol - 1;
+    pres[4] = i < L.nrow - 1;
+    pres[5] = k < L.nlay - 1;
+    npos[0] = pres[0] ? p - L.HP : -1;
+    npos[5] = pres[5] ? p + L.HP : -1;
+    {
+      const int ob = (1 - c) * L.H + k * L.HP;
+      npos[1] = pres[1] ? ob + hn(l, -L.ncol) : -1;
+      npos[2] = pres[2] ? ob + hn(l, -1) : -1;
+      npos[3] = pres[3] ? ob + hn(l, 1) : -1;
+      npos[4] = pres[4] ? ob + hn(l, L.ncol) : -1;
+    }
+    const int dirs[6] = {2, 1, 0, 0, 1, 2};
+    double q[6];
+    int ipos = ia[n] + 1;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      q[s] = 0.0;
+      if (!pres[s]) continue;
+      const int m = npos[s];
+      const int dir = dirs[s];
+      if (s < 3) {   // m < n: qcalc(m, n) and this entry is its negative
+        const int im = i - (s == 1), jm = j - (s == 2);
+        const double cond = pair_cond_sat(g, dir, m, p, im, jm, x[m], hcell, g.sat[m],
+                                          g.sat[p], g.condsat[dir * N + m]);
+        q[s] = -(cond * (hcell - x[m]));
+      } else {
+        const double cond = pair_cond_sat(g, dir, p, m, i, j, hcell, x[m], g.sat[p],
+                                          g.sat[m], g.condsat[dir * N + p]);
+        q[s] = cond * (x[m] - hcell);
+      }
+      flowja[ipos++] = q[s];
+    }
+    double diag = 0.0;
+    // --- sto_cq
+    double rss = 0.0, rsy = 0.0;
+    if (g.has_sto && !iss && g.ibound[p] > 0) {
+      const double tled = 1.0 / delt;
+      const double tp = g.top[p], bt = g.bot[p];
+      double snold, snnew;
+      if (g.iconvert[p] == 0) {
+        snold = 1.0;
+        snnew = 1.0;
+      } else {
+        snold = sqsat(tp, bt, xold[p], 0.0);
+        snnew = sqsat(tp, bt, hcell, 0.0);
+      }
+      const double sc1 = g.ss[p] * (tp - bt) * g.area[p];
+      const double rho1 = sc1 * tled;
+      const double rho1old = rho1;
+      const double aterm = -rho1 * snnew;
+      double rhsterm;
+      if (g.iconvert[p] != 0) {
+        const double tthk = tp - bt;
+        const double zold = bt + 0.5 * tthk * snold;
+        const double znew = bt + 0.5 * tthk * snnew;
+        rhsterm = -rho1old * snold * (xold[p] - zold) - rho1 * snnew * znew;
+      } else {
+        rhsterm = -rho1old * snold * xold[p];
+      }
+      rss = aterm * hcell - rhsterm;
+      diag = diag + rss;
+      if (g.iconvert[p] != 0) {
+        const double rho2 = g.sy[p] * g.area[p] * tled;
+        const double rho2old = rho2;
+        const double tthk = tp - bt;
+        rsy = rho2old * tthk * snold - rho2 * tthk * snnew;
+      }
+      diag = diag + rsy;
+    }
+    strg[p] = rss;
+    strg[N + p] = rsy;
+    // --- bnd_cq (WEL, DRN, RCH in bndlist order)
+    for (int e = br.off[p]; e < br.off[p + 1]; e++) {
+      const int code = br.ent[e];
+      diag = diag + br.rate[code >> 28][code & 0x0fffffff];
+    }
+    // --- csr_diagsum
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+      if (pres[s]) diag = diag + q[s];
+    // --- calc_chd_rate
+    if (chd_cell[p]) {
+      double rate = 0.0;
+#pragma unroll
+      for (int s = 0; s < 6; s++)
+        if (pres[s]) rate = rate - q[s];
+      diag = diag + rate;
+    }
+    flowja[ia[n]] = diag;
+  }
+}
+
+// calc_chd_rate's in / out split per CHD bound (over active neighbours; the
+// flows of the cell's row in CSR order: up, back, left, right, front, down)
+__global__ void k_chd_inout(int nb, const int *__restrict__ nodes, const int *__restrict__ ia,
+                            Layout L, const int *__restrict__ ibound,
+                            const double *__restrict__ flowja, double *__restrict__ rin,
+                            double *__restrict__ rout) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
+    const int n = nodes[e];
+    const int k = n / L.NRC, r = n - k * L.NRC, i = r / L.ncol, j = r - i * L.ncol;
+    const int kk[6] = {k - 1, k, k, k, k, k + 1};
+    const int ii[6] = {i, i - 1, i, i, i + 1, i};
+    const int jj[6] = {j, j, j - 1, j + 1, j, j};
+    double a = 0.0, b = 0.0;
+    int ip = ia[n] + 1;
+    for (int s = 0; s < 6; s++) {
+      if (kk[s] < 0 || kk[s] >= L.nlay || ii[s] < 0 || ii[s] >= L.nrow || jj[s] < 0 ||
+          jj[s] >= L.ncol)
+        continue;
+      const double qv = flowja[ip++];
+      if (ibound[cell_pos(L, kk[s], ii[s], jj[s])] > 0) {
+        if (qv < 0.0) a = a - qv;
+        else b = b + qv;
+      }
+    }
+    rin[e] = a;
+    rout[e] = b;
+  }
+}
+
+// flag[pos(node)] = 1 for the listed nodes
+__global__ void k_mark_cells(int nb, const int *__restrict__ nodes, Layout L,
+                             int *__restrict__ flag) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x)
+    flag[pos_of_natural(L, nodes[e])] = 1;
+}
+
+// rate_accumulator (Budget.f90:631-648) over n values with a fixed-shape
+// reduction: part[2 b] = sum of positives, part[2 b + 1] = - sum of negatives
+__global__ void __launch_bounds__(MF6_BLOCK) k_rate_acc(int n, const double *__restrict__ f,
+                                                        double *part) {
+  __shared__ double lds[2][MF6_BLOCK / 64];
+  double a = 0.0, b = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const double v = f[e];
+    if (v < 0.0) b = b - v;
+    else a = a + v;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = a + __shfl_xor(a, o);
+    b = b + __shfl_xor(b, o);
+  }
+  if ((threadIdx.x & 63) == 0) { lds[0][threadIdx.x >> 6] = a; lds[1][threadIdx.x >> 6] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) { sa = sa + lds[0][w]; sb = sb + lds[1][w]; }
+    part[2 * blockIdx.x] = sa;
+    part[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+__global__ void k_rate_final(const double *part, int nb, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < nb; i++) { a = a + part[2 * i]; b = b + part[2 * i + 1]; }
+  out[0] = a;
+  out[1] = b;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -747,6 +970,7 @@ static GwfDev make_dev(Mf6Handle *h) {
   g.L = h->L;
   g.top = h->top; g.bot = h->bot; g.area = h->area; g.k11 = h->k11; g.k33 = h->k33;
   g.ss = h->ss; g.sy = h->sy; g.delr = h->delr; g.delc = h->delc; g.condsat = h->condsat;
+  g.sat = h->sat;
   g.icelltype = h->icelltype; g.iconvert = h->iconvert; g.ibound = h->ibound;
   g.inewton = h->m.inewton; g.ivarcv = h->m.ivarcv; g.idewatcv = h->m.idewatcv;
   g.icellavg = h->m.icellavg; g.has_sto = h->m.has_sto;
@@ -813,6 +1037,12 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   CK(galloc(&h->ibound, (size_t)L.NP));
   HIPCHK(hipMemcpy(h->ibound, h->ibound0, sizeof(int) * L.NP, hipMemcpyDeviceToDevice));
   CK(galloc(&h->condsat, 3 * (size_t)L.NP));
+  CK(galloc(&h->sat, (size_t)L.NP));
+  {
+    // preprocess (gwf-npf.f90:1850): sat = 1 until the first npf_cf
+    std::vector<double> one(L.NP, 1.0);
+    HIPCHK(hipMemcpy(h->sat, one.data(), sizeof(double) * L.NP, hipMemcpyHostToDevice));
+  }
   CK(galloc(&h->xold, (size_t)L.NP));
   CK(galloc(&h->dxold, (size_t)L.NP));
   HIPCHK(hipMemset(h->condsat, 0, sizeof(double) * 3 * L.NP));
@@ -1222,5 +1452,96 @@ extern "C" int mf6hip_gwf_outer_update(void *handle, int kiter, mf6hip_underrela
     }
   }
   HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
+
+// gwf_cq + gwf_bd (gwf.f90:741-800) of the current heads: flowja in the CSR
+// order of mf6hip_create (optional), the budget terms (rate in, rate out) of
+// STO-SS, STO-SY, WEL, DRN, RCH, CHD and the percent discrepancy
+// (Budget.f90:259-267).
+extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms,
+                                 double *pct_discrepancy) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  if (!terms || !pct_discrepancy) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  if (h->bnd_dirty) CK(rebuild_bnd_table(h));
+  const int pkorder[3] = {MF6HIP_PKG_WEL, MF6HIP_PKG_DRN, MF6HIP_PKG_RCH};
+  size_t need = 0;
+  for (int t : pkorder) need += (size_t)h->pk[t].nbound;
+  need += 2 * (size_t)h->pk[MF6HIP_PKG_CHD].nbound;
+  CK(galloc(&h->flowja, (size_t)h->nja));
+  CK(galloc(&h->strg, 2 * (size_t)L.NP));
+  CK(galloc(&h->chd_flag, (size_t)L.NP));
+  if (need > h->bsim_n) {
+    if (h->bsim) hipFree(h->bsim);
+    h->bsim = nullptr;
+    CK(galloc(&h->bsim, need));
+    h->bsim_n = need;
+  }
+  // bnd_cf with the current heads, then the rates (bnd_cq)
+  BndRates br;
+  br.off = h->bnd_off;
+  br.ent = h->bnd_ent;
+  for (int t = 0; t < 5; t++) br.rate[t] = nullptr;
+  double *seg = h->bsim;
+  const double *segs[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int q = 0; q < 3; q++) {
+    const int t = pkorder[q];
+    DevPackage &pk = h->pk[t];
+    segs[q] = seg;
+    if (pk.nbound > 0) {
+      k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(t, pk.nbound, pk.nodes, pk.v1, pk.v2, L,
+                                                      h->ibound, h->x, h->area, pk.hcof,
+                                                      pk.brhs);
+      k_bnd_rate<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(pk.nbound, pk.nodes, L, h->ibound, h->x,
+                                                        pk.hcof, pk.brhs, seg);
+    }
+    br.rate[t] = seg;
+    seg += pk.nbound;
+  }
+  DevPackage &chd = h->pk[MF6HIP_PKG_CHD];
+  double *chd_in = seg, *chd_out = seg + chd.nbound;
+  HIPCHK(hipMemsetAsync(h->chd_flag, 0, sizeof(int) * L.NP, s));
+  if (chd.nbound > 0)
+    k_mark_cells<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, L, h->chd_flag);
+  GwfDev g = make_dev(h);
+  k_budget_cells<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, br, h->d_ia, h->chd_flag, h->iss, h->delt,
+                                                  h->x, h->xold, h->flowja, h->strg);
+  if (chd.nbound > 0)
+    k_chd_inout<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, h->d_ia, L,
+                                                       h->ibound, h->flowja, chd_in, chd_out);
+  HIPCHK(hipGetLastError());
+  // rate_accumulator of each term: STO-SS, STO-SY, WEL, DRN, RCH, then the
+  // CHD in and out lists (all non-negative: their "in" sums)
+  const double *arr[7] = {h->strg, h->strg + L.NP, segs[0], segs[1], segs[2], chd_in, chd_out};
+  const int len[7] = {L.NP, L.NP, h->pk[MF6HIP_PKG_WEL].nbound, h->pk[MF6HIP_PKG_DRN].nbound,
+                      h->pk[MF6HIP_PKG_RCH].nbound, chd.nbound, chd.nbound};
+  double *dout = h->d_part + 7 * MF6_MAXBLK;
+  for (int t = 0; t < 7; t++) {
+    if (len[t] == 0) continue;
+    const int nb = ggrid(len[t]) > 1024 ? 1024 : ggrid(len[t]);
+    k_rate_acc<<<nb, MF6_BLOCK, 0, s>>>(len[t], arr[t], h->d_part);
+    k_rate_final<<<1, 64, 0, s>>>(h->d_part, nb, dout + 2 * t);
+  }
+  HIPCHK(hipGetLastError());
+  double acc[14] = {0.0};
+  HIPCHK(hipMemcpyAsync(acc, dout, sizeof(acc), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int t = 0; t < 7; t++)
+    if (len[t] == 0) { acc[2 * t] = 0.0; acc[2 * t + 1] = 0.0; }
+  for (int t = 0; t < 5; t++) { terms[2 * t] = acc[2 * t]; terms[2 * t + 1] = acc[2 * t + 1]; }
+  terms[10] = acc[10];   // CHD in = sum of ratechdin
+  terms[11] = acc[12];   // CHD out = sum of ratechdout
+  double totin = 0.0, totout = 0.0;
+  for (int t = 0; t < 6; t++) {
+    totin = totin + terms[2 * t];
+    totout = totout + terms[2 * t + 1];
+  }
+  const double avg = (totin + totout) / 2.0;
+  *pct_discrepancy = avg != 0.0 ? 100.0 * (totin - totout) / avg : 0.0;
+  if (flowja_out)
+    HIPCHK(hipMemcpy(flowja_out, h->flowja, sizeof(double) * h->nja, hipMemcpyDeviceToHost));
   return MF6HIP_OK;
 }

@@ -99,6 +99,13 @@ struct Mf6Handle {
   int *ibound = nullptr;      // positions; CHD cells negative
   int *ibound0 = nullptr;     // as uploaded (before CHD)
   double *condsat = nullptr;  // 3 planes: right, front, down (per position)
+  double *sat = nullptr;      // npf sat of the last formulate (per position)
+  // budget (gwf_cq + gwf_bd) work arrays, allocated on first use
+  double *flowja = nullptr;   // nja, CSR order
+  double *strg = nullptr;     // [2][NP]: STO-SS, STO-SY rates
+  double *bsim = nullptr;     // per-bound rates of WEL, DRN, RCH, then CHD in / out
+  size_t bsim_n = 0;
+  int *chd_flag = nullptr;    // per position: a CHD cell
   double *xold = nullptr;
   double *dxold = nullptr;
   double *wsave = nullptr, *hchold = nullptr, *deold = nullptr;   // DBD state

@@ -1603,7 +1603,8 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->area, h->k11, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
-                  h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax};
+                  h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax, h->sat,
+                  h->flowja, h->strg, h->bsim, h->chd_flag};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int t = 0; t < 5; t++) {

@@ -97,7 +97,8 @@ module mf6hip_module
   public :: mf6hip_gwf_create, mf6hip_gwf_set_package, mf6hip_gwf_set_x, &
             mf6hip_gwf_get_x, mf6hip_gwf_advance, mf6hip_gwf_formulate, &
             mf6hip_gwf_residual_norms, mf6hip_gwf_backtrack, &
-            mf6hip_gwf_linear_solve, mf6hip_gwf_outer_update
+            mf6hip_gwf_linear_solve, mf6hip_gwf_outer_update, &
+            mf6hip_gwf_budget, mf6hip_gwf_set_x_device, mf6hip_gwf_get_x_device
 
   interface
     function mf6hip_create(neq, nja, ia, ja, settings, handle) result(rc) &
@@ -185,6 +186,32 @@ module mf6hip_module
       type(c_ptr), value :: handle
       integer(c_int), value :: kiter, inewton
       type(c_ptr), value :: amat_out, rhs_out
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_budget(handle, flowja, terms, pct) result(rc) &
+      bind(C, name='mf6hip_gwf_budget')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: flowja(*)
+      real(c_double), intent(out) :: terms(12)
+      real(c_double), intent(out) :: pct
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_set_x_device(handle, x_dev) result(rc) &
+      bind(C, name='mf6hip_gwf_set_x_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      type(c_ptr), value :: x_dev
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_get_x_device(handle, x_dev) result(rc) &
+      bind(C, name='mf6hip_gwf_get_x_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      type(c_ptr), value :: x_dev
       integer(c_int) :: rc
     end function
 

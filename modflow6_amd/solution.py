@@ -272,6 +272,24 @@ class GwfSolution:
                     return self.heads()
         return self.heads()
 
+    BUDGET_TERMS = ("STO-SS", "STO-SY", "WEL", "DRN", "RCH", "CHD")
+
+    def budget(self, want_flowja=True):
+        """gwf_cq + gwf_bd of the current heads on the device: flowja (CSR
+        order, positive into the cell, diagonal = flow residual), the budget
+        terms {name: (rate in, rate out)} and the percent discrepancy."""
+        c = self.case
+        nja = c.nodes + 2 * (c.nlay * (c.nrow * (c.ncol - 1) + (c.nrow - 1) * c.ncol)
+                             + (c.nlay - 1) * c.nrow * c.ncol)
+        fj = np.zeros(nja) if want_flowja else None
+        terms = np.zeros(12)
+        pct = C.c_double(0.0)
+        _lib.check(self.L.mf6hip_gwf_budget(self.h, _lib.dptr(fj), _lib.dptr(terms),
+                                            C.byref(pct)), "mf6hip_gwf_budget")
+        return dict(flowja=fj, pct_discrepancy=pct.value,
+                    terms={k: (terms[2 * t], terms[2 * t + 1])
+                           for t, k in enumerate(self.BUDGET_TERMS)})
+
     def heads(self):
         x = np.zeros(self.case.nodes)
         _lib.check(self.L.mf6hip_gwf_get_x(self.h, _lib.dptr(x)),

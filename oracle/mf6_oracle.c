@@ -890,6 +890,158 @@ void orc_bnd_fc(int nbound, const int *nodelist, const double *hcof,
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* GWF budget: gwf_cq + gwf_bd (gwf.f90:741-800)                        */
+/* ------------------------------------------------------------------ */
+
+/* gwf-npf.f90:745-771 npf_cq with sgwf_npf_qcalc (:797-862), no XT3D and no
+ * perched flow: qnm = cond(n, m) (hm - hn) for m > n, flowja(ipos) = qnm,
+ * flowja(isym) = -qnm (positive into the cell).  The conductance is the one
+ * npf_fc computes from the same arguments (sat from the last npf_cf). */
+void orc_npf_cq(int nodes, const int *ia, const int *ja, const int *jas,
+                const int *isym, const int *ihc, const double *cl1,
+                const double *cl2, const double *hwva, const double *condsat,
+                const int *ibound, const int *icelltype, int inewton,
+                int ivarcv, int idewatcv, int icellavg, const double *k11,
+                const double *k33, const double *top, const double *bot,
+                const double *sat, const double *hnew, double *flowja)
+{
+  for (int n = 0; n < nodes; n++) {
+    for (int ii = ia[n] + 1; ii < ia[n + 1]; ii++) {
+      int m = ja[ii];
+      if (m < n) continue;
+      int jj = jas[ii];
+      int ihcj = ihc[jj];
+      double hyn = ihcj == 0 ? k33[n] : k11[n];
+      double hym = ihcj == 0 ? k33[m] : k11[m];
+      double cond;
+      if (ihcj == 0) {
+        cond = orc_vcond(ibound[n], ibound[m], icelltype[n], icelltype[m],
+                         inewton, ivarcv, idewatcv, condsat[jj], hnew[n],
+                         hnew[m], hyn, hym, sat[n], sat[m], top[n], top[m],
+                         bot[n], bot[m], hwva[jj]);
+      } else {
+        cond = orc_hcond(ibound[n], ibound[m], icelltype[n], icelltype[m],
+                         inewton, ihcj, icellavg, condsat[jj], hnew[n],
+                         hnew[m], sat[n], sat[m], hyn, hym, top[n], top[m],
+                         bot[n], bot[m], cl1[jj], cl2[jj], hwva[jj]);
+      }
+      double qnm = cond * (hnew[m] - hnew[n]);
+      flowja[ii] = qnm;
+      flowja[isym[ii]] = -qnm;
+    }
+  }
+}
+
+/* gwf-sto.f90:447-564 sto_cq (transient only; istor_coef, iorig_ss,
+ * iconf_ss and integratechanges 0, STO satomega 0) with the rates of SsTerms
+ * (GwfStorageUtils.f90:32-91: aterm hnew - rhsterm) and SyTerms (:98-137:
+ * rho2old tthk snold - rho2 tthk snnew), each added to the diagonal. */
+void orc_sto_cq(int nodes, const int *ia, const int *ibound,
+                const int *iconvert, const double *top, const double *bot,
+                const double *area, const double *ss, const double *sy,
+                double delt, const double *hold, const double *hnew,
+                double *flowja, double *strgss, double *strgsy)
+{
+  for (int n = 0; n < nodes; n++) { strgss[n] = DZERO; strgsy[n] = DZERO; }
+  double tled = DONE / delt;
+  for (int n = 0; n < nodes; n++) {
+    if (ibound[n] <= 0) continue;
+    double tp = top[n], bt = bot[n];
+    double snold, snnew;
+    if (iconvert[n] == 0) { snold = DONE; snnew = DONE; }
+    else {
+      snold = orc_sqsat(tp, bt, hold[n], 0.0);
+      snnew = orc_sqsat(tp, bt, hnew[n], 0.0);
+    }
+    double sc1 = orc_sscapacity(0, tp, bt, area[n], ss[n]);
+    double rho1 = sc1 * tled;
+    double rho1old = rho1;
+    double aterm, rhsterm;
+    orc_ssterms(iconvert[n], 0, 0, tp, bt, rho1, rho1old, snnew, snold,
+                hnew[n], hold[n], &aterm, &rhsterm);
+    double rate = aterm * hnew[n] - rhsterm;
+    strgss[n] = rate;
+    int idiag = ia[n];
+    flowja[idiag] = flowja[idiag] + rate;
+    rate = DZERO;
+    if (iconvert[n] != 0) {
+      double sc2 = orc_sycapacity(area[n], sy[n]);
+      double rho2 = sc2 * tled;
+      double rho2old = rho2;
+      double tthk = tp - bt;
+      rate = rho2old * tthk * snold - rho2 * tthk * snnew;
+    }
+    strgsy[n] = rate;
+    flowja[idiag] = flowja[idiag] + rate;
+  }
+}
+
+/* BoundaryPackage.f90:583-619 bnd_cq_simrate without mover: rate hcof h -
+ * rhs into active cells, added to the diagonal; simvals per boundary. */
+void orc_bnd_cq(int nbound, const int *nodelist, const double *hcof,
+                const double *brhs, const int *ibound, const double *hnew,
+                const int *ia, double *flowja, double *simvals)
+{
+  for (int i = 0; i < nbound; i++) {
+    int node = nodelist[i];
+    double rrate = DZERO;
+    if (node >= 0) {
+      int idiag = ia[node];
+      if (ibound[node] > 0) rrate = hcof[i] * hnew[node] - brhs[i];
+      flowja[idiag] = flowja[idiag] + rrate;
+    }
+    simvals[i] = rrate;
+  }
+}
+
+/* Sparse.f90:262-281 csr_diagsum */
+void orc_csr_diagsum(int nodes, const int *ia, double *flowja)
+{
+  for (int n = 0; n < nodes; n++) {
+    int iposdiag = ia[n];
+    for (int ipos = ia[n] + 1; ipos < ia[n + 1]; ipos++)
+      flowja[iposdiag] = flowja[iposdiag] + flowja[ipos];
+  }
+}
+
+/* gwf-chd.f90:264-320 calc_chd_rate: the CHD rate of a cell is minus the
+ * flow into it from its neighbours; in / out split over active neighbours */
+void orc_chd_rate(int nbound, const int *nodelist, const int *ia,
+                  const int *ja, const int *ibound, double *flowja,
+                  double *simvals, double *ratein, double *rateout)
+{
+  for (int i = 0; i < nbound; i++) {
+    int node = nodelist[i];
+    int idiag = ia[node];
+    double rate = DZERO, rin = DZERO, rout = DZERO;
+    for (int ipos = ia[node] + 1; ipos < ia[node + 1]; ipos++) {
+      double q = flowja[ipos];
+      rate = rate - q;
+      int n2 = ja[ipos];
+      if (ibound[n2] > 0) {
+        if (q < DZERO) rin = rin - q;
+        else rout = rout + q;
+      }
+    }
+    simvals[i] = rate;
+    ratein[i] = rin;
+    rateout[i] = rout;
+    flowja[idiag] = flowja[idiag] + rate;
+  }
+}
+
+/* Budget.f90:631-648 rate_accumulator */
+void orc_rate_accumulator(int n, const double *flow, double *rin, double *rout)
+{
+  *rin = DZERO;
+  *rout = DZERO;
+  for (int i = 0; i < n; i++) {
+    if (flow[i] < DZERO) *rout = *rout - flow[i];
+    else *rin = *rin + flow[i];
+  }
+}
+
 /* gwf-wel.f90:296-332 wel_cf without AUTO_FLOW_REDUCE */
 void orc_wel_cf(int nbound, const int *nodelist, const double *q,
                 const int *ibound, double *hcof, double *brhs)

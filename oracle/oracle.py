@@ -104,6 +104,17 @@ def lib():
         L.orc_sto_fn.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
                                  f64p, d, f64p, f64p, f64p]
         L.orc_bnd_fc.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
+        L.orc_npf_cq.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
+                                 f64p, f64p, i32p, i32p, i, i, i, i, f64p,
+                                 f64p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_sto_cq.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
+                                 f64p, d, f64p, f64p, f64p, f64p, f64p]
+        L.orc_bnd_cq.argtypes = [i, i32p, f64p, f64p, i32p, f64p, i32p, f64p,
+                                 f64p]
+        L.orc_csr_diagsum.argtypes = [i, i32p, f64p]
+        L.orc_chd_rate.argtypes = [i, i32p, i32p, i32p, i32p, f64p, f64p,
+                                   f64p, f64p]
+        L.orc_rate_accumulator.argtypes = [i, f64p, C.POINTER(d), C.POINTER(d)]
         L.orc_wel_cf.argtypes = [i, i32p, f64p, i32p, f64p, f64p]
         L.orc_rch_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
         L.orc_drn_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p, f64p]
@@ -591,6 +602,78 @@ class GwfSimulation:
                 L.orc_sto_fn(self.neq, d.ia, self.ibound, self.iconvert,
                              d.top, d.bot, d.area, self.ss, self.sy,
                              self.delt, self.x, self.amat, self.rhs)
+
+    # -- gwf_cq + gwf_bd (gwf.f90:741-800): flowja (positive into the cell;
+    #    the diagonal ends as the cell's flow residual) and the model budget
+    #    terms (rate in, rate out) in the order STO-SS, STO-SY, then the
+    #    packages (WEL, DRN, RCH, CHD); percent discrepancy as Budget.f90:
+    #    259-267 computes it
+    def budget(self):
+        L = lib()
+        d = self.dis
+        flowja = np.zeros(d.nja)
+        L.orc_npf_cq(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
+                     d.hwva, self.condsat, self.ibound, self.icelltype,
+                     self.inewton, 0, 1, 0, self.k11, self.k33, d.top, d.bot,
+                     self.sat, self.x, flowja)
+        terms = {}
+        strgss = np.zeros(self.neq)
+        strgsy = np.zeros(self.neq)
+        if self.case.has_sto and not self.iss:
+            L.orc_sto_cq(self.neq, d.ia, self.ibound, self.iconvert, d.top,
+                         d.bot, d.area, self.ss, self.sy, self.delt,
+                         self.xold, self.x, flowja, strgss, strgsy)
+        if self.case.has_sto:
+            terms["STO-SS"] = self._accumulate(strgss)
+            if np.any(np.asarray(self.iconvert) != 0):
+                terms["STO-SY"] = self._accumulate(strgsy)
+        sim = {}
+        for t, p in enumerate(self.pk):
+            nd = self.pk_nodes[t]
+            if p.kind == "CHD":
+                continue
+            # gwf_cq calls bnd_cf first (heads-dependent terms, DRN)
+            if p.kind == "WEL":
+                L.orc_wel_cf(nd.size, nd, np.ascontiguousarray(p.v1),
+                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
+            elif p.kind == "RCH":
+                L.orc_rch_cf(nd.size, nd, np.ascontiguousarray(p.v1), d.area,
+                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
+            elif p.kind == "DRN":
+                L.orc_drn_cf(nd.size, nd, np.ascontiguousarray(p.v1),
+                             np.ascontiguousarray(p.v2), self.ibound, self.x,
+                             self.pk_hcof[t], self.pk_rhs[t])
+            sv = np.zeros(nd.size)
+            L.orc_bnd_cq(nd.size, nd, self.pk_hcof[t], self.pk_rhs[t],
+                         self.ibound, self.x, d.ia, flowja, sv)
+            sim[p.kind] = sv
+        L.orc_csr_diagsum(self.neq, d.ia, flowja)
+        for t, p in enumerate(self.pk):
+            nd = self.pk_nodes[t]
+            if p.kind == "CHD":
+                sv, rin, rout = (np.zeros(nd.size) for _ in range(3))
+                L.orc_chd_rate(nd.size, nd, d.ia, d.ja, self.ibound, flowja, sv,
+                               rin, rout)
+                sim["CHD"] = sv
+                terms["CHD"] = (self._accumulate(rin)[0], self._accumulate(rout)[0])
+            else:
+                terms[p.kind] = self._accumulate(sim[p.kind])
+        totin = 0.0
+        totout = 0.0
+        for rin, rout in terms.values():
+            totin += rin
+            totout += rout
+        avg = (totin + totout) / 2.0
+        pct = 100.0 * (totin - totout) / avg if avg != 0.0 else 0.0
+        return dict(flowja=flowja, terms=terms, pct_discrepancy=pct,
+                    strgss=strgss, strgsy=strgsy, simvals=sim)
+
+    @staticmethod
+    def _accumulate(flow):
+        rin, rout = C.c_double(), C.c_double()
+        f = np.ascontiguousarray(flow, dtype=np.float64)
+        lib().orc_rate_accumulator(f.size, f, C.byref(rin), C.byref(rout))
+        return rin.value, rout.value
 
     # -- sln_l2norm (:2855-2871) of the current system
     def l2norm(self):

@@ -1,0 +1,83 @@
+"""Device budget (mf6hip_gwf_budget: gwf_cq + gwf_bd) against the oracle's
+restatement (oracle.GwfSimulation.budget).
+
+The device is given the oracle's heads (and formulates at the oracle's last
+formulate heads, so both hold the same npf sat): flowja must then be
+bit-identical (same conductance functions, same accumulation order of the
+diagonal) and the budget terms equal to 1e-12 relative (the device sums
+them with fixed-shape parallel reductions, the oracle sequentially).  A
+device-only run must close its own budget."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from modflow6_amd import configs
+from modflow6_amd.solution import GwfSolution
+from oracle import oracle as O
+from tests.cases import bump_case, line_case, newton01_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _transient():
+    return configs.transient_newton(nlay=3, nrow=24, ncol=30, nwel=8, nper=1, nstp=2)
+
+
+CASES = {
+    "line": line_case,
+    "newton01": newton01_case,
+    "bump": bump_case,
+    "steady_small": lambda: configs.steady_small(nlay=3, nrow=20, ncol=24, nwel=6),
+    "transient_newton": _transient,
+}
+
+
+def _oracle_first_step(case):
+    sim = O.GwfSimulation(case)
+    perlen, nstp, steady = case.periods[0]
+    sim.kper = 1
+    conv, _ = sim.run_timestep(perlen / nstp, steady)
+    assert conv
+    return sim, perlen / nstp, steady
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_budget_matches_oracle(name):
+    case = CASES[name]()
+    sim, delt, steady = _oracle_first_step(case)
+    ob = sim.budget()
+    gs = GwfSolution(case)
+    gs.kper, gs.kstp = 1, 1
+    gs.prepare_timestep(delt, steady)
+    import ctypes as C
+    from modflow6_amd import _lib
+    _lib.check(gs.L.mf6hip_gwf_set_x(gs.h, _lib.dptr(np.ascontiguousarray(sim.xtemp))), "set_x")
+    gs.formulate(2)
+    _lib.check(gs.L.mf6hip_gwf_set_x(gs.h, _lib.dptr(np.ascontiguousarray(sim.x))), "set_x")
+    db = gs.budget()
+    gs.destroy()
+    fo, fd = ob["flowja"], db["flowja"]
+    bad = np.nonzero(fo.view(np.int64) != fd.view(np.int64))[0]
+    assert bad.size == 0, (bad[:10], fo[bad[:10]], fd[bad[:10]])
+    scale = max(max(v) for v in ob["terms"].values())
+    for k, (rin, rout) in db["terms"].items():
+        oin, oout = ob["terms"].get(k, (0.0, 0.0))
+        assert abs(rin - oin) <= 1e-12 * scale and abs(rout - oout) <= 1e-12 * scale, k
+    assert abs(db["pct_discrepancy"] - ob["pct_discrepancy"]) <= 1e-9
+
+
+def test_device_run_closes_its_budget():
+    """A transient Newton step solved and budgeted on the device: the terms
+    balance (percent discrepancy below 1e-3 %) and the storage term is
+    present."""
+    case = _transient()
+    gs = GwfSolution(case)
+    gs.kper = 1
+    perlen, nstp, steady = case.periods[0]
+    gs.run_timestep(perlen / nstp, steady)
+    b = gs.budget(want_flowja=False)
+    gs.destroy()
+    assert abs(b["pct_discrepancy"]) < 1e-3
+    assert b["terms"]["STO-SS"] != (0.0, 0.0)
+    assert b["terms"]["RCH"][0] > 0.0 and b["terms"]["WEL"][1] > 0.0

@@ -118,3 +118,30 @@ def test_device_backtrack_and_residual_norm():
     assert abs(l2 - lo) <= 1e-12 * lo
     vol = d.area * (d.top - d.bot)
     assert abs(rv - np.max(np.abs(r[active]) / vol[active])) <= 1e-12 * rv
+
+
+def test_device_head_exchange_with_device_buffers():
+    """mf6hip_gwf_get_x_device / set_x_device move heads between the model and
+    a caller's GPU buffer (DIS node order) without a host copy: bit-identical
+    to the host exchange, and a solve started from heads set this way ends
+    where the host-set start ends."""
+    import torch
+    from modflow6_amd import _lib
+    case = bump_case()
+    gs = GwfSolution(case)
+    gs.kper, gs.kstp = 1, 1
+    gs.prepare_timestep(1.0, True)
+    gs.solve(1)
+    xh = gs.heads()
+    buf = torch.full((case.nodes,), float("nan"), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _lib.check(gs.L.mf6hip_gwf_get_x_device(gs.h, buf.data_ptr()), "get_x_device")
+    xd = buf.cpu().numpy()
+    assert np.array_equal(xd.view(np.int64), xh.view(np.int64))
+    # perturb on the device, set back, read through the host path
+    buf.add_(0.25)
+    torch.cuda.synchronize()
+    _lib.check(gs.L.mf6hip_gwf_set_x_device(gs.h, buf.data_ptr()), "set_x_device")
+    xs = gs.heads()
+    assert np.array_equal(xs.view(np.int64), (xh + 0.25).view(np.int64))
+    gs.destroy()
