@@ -419,9 +419,134 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
   if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
 }
 
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+// The same product with two columns of one colour per thread (positions h2,
+// h2 + 1, h2 even): the own-plane values and p of the pair are one 2-wide
+// load, which halves the load instructions of the streams.  Needs a layout
+// whose pairs never straddle a grid row: ncol and ncol / 2 even (then HP0 =
+// HP1 is even as well).  Same amux order per cell; the p.q partial adds the
+// pair's two cells per layer.
+template <typename TV, int NLMAX>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__restrict__ A,
+                                                       const TV *__restrict__ p,
+                                                       TV *__restrict__ q, double *part,
+                                                       const KState *st, int nblk) {
+  using V2 = typename Vec2<TV>::type;
+  constexpr int CH = 4;
+  const int lb = xcd_block();
+  __shared__ double lds[8];
+  if (st->stop) return;
+  const int NP = L.NP;
+  double s = 0.0;
+  const int npair = L.HP0;   // HP0 / 2 pairs per colour, interleaved
+  for (int t = lb * blockDim.x + threadIdx.x; t < npair; t += gridDim.x * blockDim.x) {
+    const int c = t & 1;
+    const int h2 = (t >> 1) * 2;
+    const int l0 = col_l(L, c, h2), l1 = col_l(L, c, h2 + 1);
+    const int i = l0 / L.ncol, j0 = l0 - i * L.ncol, j1 = l1 - i * L.ncol;
+    const bool hb = i > 0, hf = i < L.nrow - 1;
+    const bool hl0 = j0 > 0, hr0 = j0 < L.ncol - 1, hl1 = j1 > 0, hr1 = j1 < L.ncol - 1;
+    const int ib0 = hb ? hn(l0, -L.ncol) : 0, ib1 = hb ? hn(l1, -L.ncol) : 0;
+    const int if0 = hf ? hn(l0, L.ncol) : 0, if1 = hf ? hn(l1, L.ncol) : 0;
+    const int il0 = hl0 ? hn(l0, -1) : 0, il1 = hl1 ? hn(l1, -1) : 0;
+    const int ir0 = hr0 ? hn(l0, 1) : 0, ir1 = hr1 ? hn(l1, 1) : 0;
+    const int own = c * L.H + h2, oth = (1 - c) * L.H;
+    V2 pcur = *reinterpret_cast<const V2 *>(p + own);
+    V2 pprev{(TV)0, (TV)0}, aup{(TV)0, (TV)0};
+#pragma unroll
+    for (int k0 = 0; k0 < NLMAX; k0 += CH) {
+      if (k0 >= L.nlay) break;
+      V2 a0[CH], a4[CH], a5[CH], a6[CH], pn[CH];
+      TV gb0[CH], gb1[CH], pb0[CH], pb1[CH], gl0[CH], gl1[CH], pl0[CH], pl1[CH], pr0[CH],
+          pr1[CH], pf0[CH], pf1[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int k = k0 + u;
+        if (k < L.nlay) {
+          const int pos = own + k * L.HP, ob = oth + k * L.HP;
+          const bool dn = k + 1 < L.nlay;
+          a0[u] = *reinterpret_cast<const V2 *>(A + pos);
+          a4[u] = *reinterpret_cast<const V2 *>(A + 4 * NP + pos);
+          a5[u] = *reinterpret_cast<const V2 *>(A + 5 * NP + pos);
+          a6[u] = dn ? *reinterpret_cast<const V2 *>(A + 6 * NP + pos) : V2{(TV)0, (TV)0};
+          pn[u] = dn ? *reinterpret_cast<const V2 *>(p + pos + L.HP) : V2{(TV)0, (TV)0};
+          gb0[u] = hb ? A[5 * NP + ob + ib0] : (TV)0;
+          pb0[u] = hb ? p[ob + ib0] : (TV)0;
+          gb1[u] = hb ? A[5 * NP + ob + ib1] : (TV)0;
+          pb1[u] = hb ? p[ob + ib1] : (TV)0;
+          gl0[u] = hl0 ? A[4 * NP + ob + il0] : (TV)0;
+          pl0[u] = hl0 ? p[ob + il0] : (TV)0;
+          gl1[u] = hl1 ? A[4 * NP + ob + il1] : (TV)0;
+          pl1[u] = hl1 ? p[ob + il1] : (TV)0;
+          pr0[u] = hr0 ? p[ob + ir0] : (TV)0;
+          pr1[u] = hr1 ? p[ob + ir1] : (TV)0;
+          pf0[u] = hf ? p[ob + if0] : (TV)0;
+          pf1[u] = hf ? p[ob + if1] : (TV)0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int k = k0 + u;
+        if (k < L.nlay) {
+          TV t0 = (TV)0, t1 = (TV)0;
+          t0 = t0 + a0[u].x * pcur.x;
+          t1 = t1 + a0[u].y * pcur.y;
+          if (k > 0) {
+            t0 = t0 + aup.x * pprev.x;
+            t1 = t1 + aup.y * pprev.y;
+          }
+          if (hb) {
+            t0 = t0 + gb0[u] * pb0[u];
+            t1 = t1 + gb1[u] * pb1[u];
+          }
+          if (hl0) t0 = t0 + gl0[u] * pl0[u];
+          if (hl1) t1 = t1 + gl1[u] * pl1[u];
+          if (hr0) t0 = t0 + a4[u].x * pr0[u];
+          if (hr1) t1 = t1 + a4[u].y * pr1[u];
+          if (hf) {
+            t0 = t0 + a5[u].x * pf0[u];
+            t1 = t1 + a5[u].y * pf1[u];
+          }
+          if (k + 1 < L.nlay) {
+            t0 = t0 + a6[u].x * pn[u].x;
+            t1 = t1 + a6[u].y * pn[u].y;
+          }
+          *reinterpret_cast<V2 *>(q + own + k * L.HP) = V2{t0, t1};
+          s = s + (double)pcur.x * (double)t0;
+          s = s + (double)pcur.y * (double)t1;
+          aup = a6[u];
+          pprev = pcur;
+          pcur = pn[u];
+        }
+      }
+    }
+  }
+  s = bsum(s, lds);
+  if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
+}
+
+// pairs of one colour never straddle a grid row
+static inline bool spmv_pairable(const Layout &L) {
+  return !L.odd && (L.ncol / 2) % 2 == 0 && L.HP0 == L.HP1 && L.HP0 % 2 == 0;
+}
+
 template <typename TM, typename TV>
 static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
                           TV *q, double *part, const KState *st) {
+  if (spmv_pairable(L)) {
+    if (L.nlay <= 4)
+      k_f_spmv2<TV, 4><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    else if (L.nlay <= 8)
+      k_f_spmv2<TV, 8><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    else if (L.nlay <= 16)
+      k_f_spmv2<TV, 16><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    else
+      k_f_spmv2<TV, 32><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    return;
+  }
   if (L.nlay <= 4)
     k_f_spmv<TM, TV, 4><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
   else if (L.nlay <= 8)
