@@ -429,6 +429,10 @@ template <> struct Vec2<double> { using type = double2; };
 // whose pairs never straddle a grid row: ncol and ncol / 2 even (then HP0 =
 // HP1 is even as well).  Same amux order per cell; the p.q partial adds the
 // pair's two cells per layer.
+// It runs on its own grid (half the columns of k_f_spmv: a grid sized for
+// the single-column kernel would leave the XCDs that receive the upper half
+// of the logical blocks idle); slots gridDim.x .. nblk-1 of its p.q partials
+// (read by k_f_alpha over nblk slots) are cleared.
 template <typename TV, int NLMAX>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__restrict__ A,
                                                        const TV *__restrict__ p,
@@ -525,7 +529,10 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
     }
   }
   s = bsum(s, lds);
-  if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
+  if (threadIdx.x == 0) {
+    part[P_PQ * nblk + lb] = s;
+    for (int e = (int)gridDim.x + lb; e < nblk; e += (int)gridDim.x) part[P_PQ * nblk + e] = 0.0;
+  }
 }
 
 // pairs of one colour never straddle a grid row
@@ -537,14 +544,17 @@ template <typename TM, typename TV>
 static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
                           TV *q, double *part, const KState *st) {
   if (spmv_pairable(L)) {
+    int g2 = (L.HP0 + MF6_BLOCK - 1) / MF6_BLOCK;
+    if (g2 > nb) g2 = nb;
+    if (g2 < 1) g2 = 1;
     if (L.nlay <= 4)
-      k_f_spmv2<TV, 4><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 4><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
     else if (L.nlay <= 8)
-      k_f_spmv2<TV, 8><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 8><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
     else if (L.nlay <= 16)
-      k_f_spmv2<TV, 16><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 16><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
     else
-      k_f_spmv2<TV, 32><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 32><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
     return;
   }
   if (L.nlay <= 4)
