@@ -334,6 +334,86 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
   if (threadIdx.x == 0) mpart[M_P * nblk + lb] = mp;
 }
 
+// The same update with two columns of one colour per thread (positions
+// h2, h2 + 1, h2 even; 2-wide loads), on its own grid, for layouts whose
+// pairs never straddle a grid row; slots gridDim.x .. nblk-1 of its max |p|
+// partials are cleared.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+template <typename TV, typename TX>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__restrict__ z,
+                                                          const TV *__restrict__ pin,
+                                                          TV *__restrict__ pout,
+                                                          TX *__restrict__ x, MaxLoc *mpart,
+                                                          const KState *st, const FState *fs,
+                                                          int nblk) {
+  using V2 = typename Vec2<TV>::type;
+  using X2 = typename Vec2<TX>::type;
+  const int lb = xcd_block();
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  const bool first = st->iter == 0;
+  const TV beta = (TV)st->beta;
+  const TV ax = (TV)fs->apend_x;
+  MaxLoc mp{0.0, -1};
+  const int npair = L.HP0;
+  for (int t = lb * blockDim.x + threadIdx.x; t < npair; t += gridDim.x * blockDim.x) {
+    const int c = t & 1;
+    const int h2 = (t >> 1) * 2;
+    const int l0 = col_l(L, c, h2), l1 = col_l(L, c, h2 + 1);
+    const int own = c * L.H + h2;
+    for (int k = 0; k < L.nlay; k++) {
+      const int pos = own + k * L.HP;
+      const V2 zv = *reinterpret_cast<const V2 *>(z + pos);
+      V2 pv;
+      if (first) {
+        pv = zv;
+      } else {
+        const V2 po = *reinterpret_cast<const V2 *>(pin + pos);
+        pv.x = zv.x + beta * po.x;
+        pv.y = zv.y + beta * po.y;
+        X2 xv = *reinterpret_cast<const X2 *>(x + pos);
+        xv.x = (TX)(xv.x + (TX)(ax * po.x));
+        xv.y = (TX)(xv.y + (TX)(ax * po.y));
+        *reinterpret_cast<X2 *>(x + pos) = xv;
+      }
+      *reinterpret_cast<V2 *>(pout + pos) = pv;
+      mp = maxloc_pick(mp, MaxLoc{(double)pv.x, k * L.NRC + l0});
+      mp = maxloc_pick(mp, MaxLoc{(double)pv.y, k * L.NRC + l1});
+    }
+  }
+  mp = bmax(mp, mlds);
+  if (threadIdx.x == 0) {
+    mpart[M_P * nblk + lb] = mp;
+    for (int e = (int)gridDim.x + lb; e < nblk; e += (int)gridDim.x)
+      mpart[M_P * nblk + e] = MaxLoc{0.0, -1};
+  }
+}
+
+// pairs of one colour never straddle a grid row
+static inline bool spmv_pairable(const Layout &L) {
+  return !L.odd && (L.ncol / 2) % 2 == 0 && L.HP0 == L.HP1 && L.HP0 % 2 == 0;
+}
+
+static inline int pair_grid(const Layout &L, int nb) {
+  int g2 = (L.HP0 + MF6_BLOCK - 1) / MF6_BLOCK;
+  if (g2 > nb) g2 = nb;
+  return g2 < 1 ? 1 : g2;
+}
+
+template <typename TV, typename TX>
+static void launch_f_pupdate(const Layout &L, int nb, hipStream_t s, const TV *z, const TV *pin,
+                             TV *pout, TX *x, MaxLoc *mpart, const KState *st,
+                             const FState *fs) {
+  if (spmv_pairable(L))
+    k_f_pupdate2<TV, TX><<<pair_grid(L, nb), MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st,
+                                                               fs, nb);
+  else
+    k_f_pupdate<TV, TX><<<nb, MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st, fs, nb);
+}
+
 // q = A p (amux order, symmetric planes), partial p.q.  The layer walk is
 // unrolled over a compile-time bound NLMAX in chunks of four layers whose
 // loads are all issued before their arithmetic (more loads in flight per
@@ -418,10 +498,6 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
   s = bsum(s, lds);
   if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
 }
-
-template <typename T> struct Vec2;
-template <> struct Vec2<float> { using type = float2; };
-template <> struct Vec2<double> { using type = double2; };
 
 // The same product with two columns of one colour per thread (positions h2,
 // h2 + 1, h2 even): the own-plane values and p of the pair are one 2-wide
@@ -535,18 +611,11 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
   }
 }
 
-// pairs of one colour never straddle a grid row
-static inline bool spmv_pairable(const Layout &L) {
-  return !L.odd && (L.ncol / 2) % 2 == 0 && L.HP0 == L.HP1 && L.HP0 % 2 == 0;
-}
-
 template <typename TM, typename TV>
 static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
                           TV *q, double *part, const KState *st) {
   if (spmv_pairable(L)) {
-    int g2 = (L.HP0 + MF6_BLOCK - 1) / MF6_BLOCK;
-    if (g2 > nb) g2 = nb;
-    if (g2 < 1) g2 = 1;
+    const int g2 = pair_grid(L, nb);
     if (L.nlay <= 4)
       k_f_spmv2<TV, 4><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
     else if (L.nlay <= 8)
@@ -862,8 +931,7 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
       k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                         h->d_hloc);
       double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
-      k_f_pupdate<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->z, pin, pout, h->x, h->d_mpart,
-                                                           st, fs, nb);
+      launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs);
       int slot = -1;
       if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
           2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
@@ -923,8 +991,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                           h->d_hloc);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
-        k_f_pupdate<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->z32, pin, pout, h->e32,
-                                                           h->d_mpart, st, fs, nb);
+        launch_f_pupdate<float, float>(L, nb, s, h->z32, pin, pout, h->e32, h->d_mpart, st, fs);
         int slot = -1;
         if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
             2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
