@@ -289,181 +289,6 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
   if (threadIdx.x == 0) part[P_RHO_R * nblk + lb] = rho;
 }
 
-template <typename T> using vec2p = T __attribute__((ext_vector_type(2)));
-
-// Pair forms of the black and red backward sweeps: two columns of one colour
-// per thread (positions h2, h2 + 1, h2 even), the own-plane values and
-// vectors as 2-wide loads, the other colour's values gathered per cell; the
-// same statement order per cell as k_f_black / k_f_red_bwd.  They run on
-// their own grid (half the columns) and clear their partial slots from
-// gridDim.x up to ncolb, the colour kernels' slot count.
-template <typename TM, typename TV, typename TP>
-__global__ void __launch_bounds__(MF6_BLOCK) k_f_black2(Layout L, const TM *__restrict__ A,
-                                                        const TP *__restrict__ piv,
-                                                        TV *__restrict__ d,
-                                                        const TV *__restrict__ q,
-                                                        const TV *__restrict__ zf,
-                                                        TV *__restrict__ z, double *part,
-                                                        MaxLoc *mpart, const KState *st,
-                                                        const FState *fs, int nblk, int ncolb,
-                                                        TV sc) {
-  using V = vec2p<TV>;
-  using VM = vec2p<TM>;
-  using VP = vec2p<TP>;
-  const int lb = xcd_block();
-  __shared__ double lds[8];
-  __shared__ MaxLoc mlds[8];
-  if (st->stop) return;
-  const int NP = L.NP;
-  const TV alpha = (TV)fs->apend_d;
-  double ss = 0.0, rho = 0.0;
-  MaxLoc md{0.0, -1};
-  const int npair = L.HP1 / 2;
-  for (int t = lb * blockDim.x + threadIdx.x; t < npair; t += gridDim.x * blockDim.x) {
-    const int h2 = 2 * t;
-    int lc[2], ib[2], il[2], ir[2], ifr[2];
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-      const int l = col_l(L, 1, h2 + e);
-      const int i = l / L.ncol, j = l - i * L.ncol;
-      lc[e] = l;
-      ib[e] = i > 0 ? hn(l, -L.ncol) : -1;
-      il[e] = j > 0 ? hn(l, -1) : -1;
-      ir[e] = j < L.ncol - 1 ? hn(l, 1) : -1;
-      ifr[e] = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
-    }
-    TV zup[2] = {(TV)0, (TV)0}, lup[2] = {(TV)0, (TV)0};
-    for (int k = 0; k < L.nlay; k++) {
-      const int pos = L.H + k * L.HP + h2;
-      const int rb = k * L.HP;
-      V dv = *reinterpret_cast<const V *>(d + pos);
-      const V qv = *reinterpret_cast<const V *>(q + pos);
-      const VM a4 = *reinterpret_cast<const VM *>(A + 4 * NP + pos);
-      const VM a5 = *reinterpret_cast<const VM *>(A + 5 * NP + pos);
-      const VM a6 = *reinterpret_cast<const VM *>(A + 6 * NP + pos);
-      const VP pv = *reinterpret_cast<const VP *>(piv + pos);
-      dv.x = dv.x - alpha * qv.x;
-      dv.y = dv.y - alpha * qv.y;
-      *reinterpret_cast<V *>(d + pos) = dv;
-      V out;
-#pragma unroll
-      for (int e = 0; e < 2; e++) {
-        const TV dve = e ? dv.y : dv.x;
-        md = maxloc_pick(md, MaxLoc{(double)dve, k * L.NRC + lc[e]});
-        ss = ss + (double)dve * (double)dve;
-        TV tv = dve;
-        if (ib[e] >= 0) tv = tv - (TV)((TV)A[5 * NP + rb + ib[e]] * (TV)ldp(piv, rb + ib[e])) * zf[rb + ib[e]];
-        if (il[e] >= 0) tv = tv - (TV)((TV)A[4 * NP + rb + il[e]] * (TV)ldp(piv, rb + il[e])) * zf[rb + il[e]];
-        if (ir[e] >= 0) tv = tv - (TV)((TV)(e ? a4.y : a4.x) * (TV)ldp(piv, rb + ir[e])) * zf[rb + ir[e]];
-        if (ifr[e] >= 0) tv = tv - (TV)((TV)(e ? a5.y : a5.x) * (TV)ldp(piv, rb + ifr[e])) * zf[rb + ifr[e]];
-        if (k > 0) tv = tv - lup[e] * zup[e];
-        zup[e] = tv;
-        lup[e] = (TV)((TV)(e ? a6.y : a6.x) * (TV)(e ? pv.y : pv.x));
-        if (e) out.y = tv;
-        else out.x = tv;
-      }
-      *reinterpret_cast<V *>(z + pos) = out;
-    }
-    TV zdn[2] = {(TV)0, (TV)0};
-    for (int k = L.nlay - 1; k >= 0; k--) {
-      const int pos = L.H + k * L.HP + h2;
-      const V zv = *reinterpret_cast<const V *>(z + pos);
-      const VM a6 = *reinterpret_cast<const VM *>(A + 6 * NP + pos);
-      const VP pv = *reinterpret_cast<const VP *>(piv + pos);
-      const V dv = *reinterpret_cast<const V *>(d + pos);
-      V out;
-#pragma unroll
-      for (int e = 0; e < 2; e++) {
-        TV tv = (e ? zv.y : zv.x) * sc;
-        if (k < L.nlay - 1) tv = tv - (TV)(e ? a6.y : a6.x) * zdn[e];
-        tv = tv * (TV)(e ? pv.y : pv.x);
-        zdn[e] = tv;
-        rho = rho + (double)(e ? dv.y : dv.x) * (double)tv;
-        if (e) out.y = tv;
-        else out.x = tv;
-      }
-      *reinterpret_cast<V *>(z + pos) = out;
-    }
-  }
-  ss = bsum(ss, lds);
-  rho = bsum(rho, lds);
-  md = bmax(md, mlds);
-  if (threadIdx.x == 0) {
-    part[P_SS_B * nblk + lb] = ss;
-    part[P_RHO_B * nblk + lb] = rho;
-    mpart[M_D_B * nblk + lb] = md;
-    for (int e = (int)gridDim.x + lb; e < ncolb; e += (int)gridDim.x) {
-      part[P_SS_B * nblk + e] = 0.0;
-      part[P_RHO_B * nblk + e] = 0.0;
-      mpart[M_D_B * nblk + e] = MaxLoc{0.0, -1};
-    }
-  }
-}
-
-template <typename TM, typename TV, typename TP>
-__global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd2(Layout L, const TM *__restrict__ A,
-                                                          const TP *__restrict__ piv,
-                                                          const TV *__restrict__ d,
-                                                          const TV *__restrict__ zf,
-                                                          TV *__restrict__ z, double *part,
-                                                          const KState *st, int nblk, int ncolb,
-                                                          TV sc) {
-  using V = vec2p<TV>;
-  using VM = vec2p<TM>;
-  using VP = vec2p<TP>;
-  const int lb = xcd_block();
-  __shared__ double lds[8];
-  if (st->stop) return;
-  const int NP = L.NP;
-  double rho = 0.0;
-  const int npair = L.HP0 / 2;
-  for (int t = lb * blockDim.x + threadIdx.x; t < npair; t += gridDim.x * blockDim.x) {
-    const int h2 = 2 * t;
-    int ib[2], il[2], ir[2], ifr[2];
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-      const int l = col_l(L, 0, h2 + e);
-      const int i = l / L.ncol, j = l - i * L.ncol;
-      ib[e] = i > 0 ? hn(l, -L.ncol) : -1;
-      il[e] = j > 0 ? hn(l, -1) : -1;
-      ir[e] = j < L.ncol - 1 ? hn(l, 1) : -1;
-      ifr[e] = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
-    }
-    TV zdn[2] = {(TV)0, (TV)0};
-    for (int k = L.nlay - 1; k >= 0; k--) {
-      const int pos = k * L.HP + h2;
-      const int bb = L.H + k * L.HP;
-      const V zfv = *reinterpret_cast<const V *>(zf + pos);
-      const VM a4 = *reinterpret_cast<const VM *>(A + 4 * NP + pos);
-      const VM a5 = *reinterpret_cast<const VM *>(A + 5 * NP + pos);
-      const VM a6 = *reinterpret_cast<const VM *>(A + 6 * NP + pos);
-      const VP pv = *reinterpret_cast<const VP *>(piv + pos);
-      const V dv = *reinterpret_cast<const V *>(d + pos);
-      V out;
-#pragma unroll
-      for (int e = 0; e < 2; e++) {
-        TV tv = (e ? zfv.y : zfv.x) * sc;
-        if (k < L.nlay - 1) tv = tv - (TV)(e ? a6.y : a6.x) * zdn[e];
-        if (ib[e] >= 0) tv = tv - (TV)A[5 * NP + bb + ib[e]] * z[bb + ib[e]];
-        if (il[e] >= 0) tv = tv - (TV)A[4 * NP + bb + il[e]] * z[bb + il[e]];
-        if (ir[e] >= 0) tv = tv - (TV)(e ? a4.y : a4.x) * z[bb + ir[e]];
-        if (ifr[e] >= 0) tv = tv - (TV)(e ? a5.y : a5.x) * z[bb + ifr[e]];
-        tv = tv * (TV)(e ? pv.y : pv.x);
-        zdn[e] = tv;
-        rho = rho + (double)(e ? dv.y : dv.x) * (double)tv;
-        if (e) out.y = tv;
-        else out.x = tv;
-      }
-      *reinterpret_cast<V *>(z + pos) = out;
-    }
-  }
-  rho = bsum(rho, lds);
-  if (threadIdx.x == 0) {
-    part[P_RHO_R * nblk + lb] = rho;
-    for (int e = (int)gridDim.x + lb; e < ncolb; e += (int)gridDim.x) part[P_RHO_R * nblk + e] = 0.0;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // direction update with the deferred head update, then the SpMV
 // ---------------------------------------------------------------------------
@@ -513,8 +338,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
 // h2, h2 + 1, h2 even; 2-wide loads), on its own grid, for layouts whose
 // pairs never straddle a grid row; slots gridDim.x .. nblk-1 of its max |p|
 // partials are cleared.
-template <typename T> using vec2 = T __attribute__((ext_vector_type(2)));
-template <typename T> struct Vec2 { using type = vec2<T>; };
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
 
 template <typename TV, typename TX>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__restrict__ z,
@@ -709,7 +535,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
     const int ir0 = hr0 ? hn(l0, 1) : 0, ir1 = hr1 ? hn(l1, 1) : 0;
     const int own = c * L.H + h2, oth = (1 - c) * L.H;
     V2 pcur = *reinterpret_cast<const V2 *>(p + own);
-    V2 pprev = V2((TV)0), aup = V2((TV)0);
+    V2 pprev{(TV)0, (TV)0}, aup{(TV)0, (TV)0};
 #pragma unroll
     for (int k0 = 0; k0 < NLMAX; k0 += CH) {
       if (k0 >= L.nlay) break;
@@ -725,8 +551,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
           a0[u] = *reinterpret_cast<const V2 *>(A + pos);
           a4[u] = *reinterpret_cast<const V2 *>(A + 4 * NP + pos);
           a5[u] = *reinterpret_cast<const V2 *>(A + 5 * NP + pos);
-          a6[u] = dn ? *reinterpret_cast<const V2 *>(A + 6 * NP + pos) : V2((TV)0);
-          pn[u] = dn ? *reinterpret_cast<const V2 *>(p + pos + L.HP) : V2((TV)0);
+          a6[u] = dn ? *reinterpret_cast<const V2 *>(A + 6 * NP + pos) : V2{(TV)0, (TV)0};
+          pn[u] = dn ? *reinterpret_cast<const V2 *>(p + pos + L.HP) : V2{(TV)0, (TV)0};
           gb0[u] = hb ? A[5 * NP + ob + ib0] : (TV)0;
           pb0[u] = hb ? p[ob + ib0] : (TV)0;
           gb1[u] = hb ? A[5 * NP + ob + ib1] : (TV)0;
@@ -768,10 +594,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
             t0 = t0 + a6[u].x * pn[u].x;
             t1 = t1 + a6[u].y * pn[u].y;
           }
-          V2 tq;
-          tq.x = t0;
-          tq.y = t1;
-          *reinterpret_cast<V2 *>(q + own + k * L.HP) = tq;
+          *reinterpret_cast<V2 *>(q + own + k * L.HP) = V2{t0, t1};
           s = s + (double)pcur.x * (double)t0;
           s = s + (double)pcur.y * (double)t1;
           aup = a6[u];
@@ -1071,16 +894,6 @@ static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV 
   hipStream_t s = h->stream;
   k_f_red_fwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
                                                     st, fs, nb);
-  if (spmv_pairable(L)) {
-    int g = (L.HP0 / 2 + MF6_BLOCK - 1) / MF6_BLOCK;
-    if (g > nbc) g = nbc;
-    if (g < 1) g = 1;
-    k_f_black2<TM, TV, TP><<<g, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part,
-                                                   h->d_mpart, st, fs, nb, nbc, sc);
-    k_f_red_bwd2<TM, TV, TP><<<g, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb,
-                                                     nbc, sc);
-    return;
-  }
   k_f_black<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
                                                   st, fs, nb, sc);
   k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb, sc);
