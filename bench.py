@@ -2,7 +2,10 @@
 """Benchmark: seconds to INNER_RCLOSE 1e-6 per linear solve of the 10M-cell
 steady confined GWF model (10 x 1000 x 1000 DIS, exponential-covariance
 ln K with sigma 2 / correlation 20 cells, CHD + RCH + 2000 wells), FP64
-ILU0-PCG vs the mixed-precision solve, on one MI355X.
+ILU0-PCG vs the mixed-precision solves (FP32 ILU0 factor, and the FP16
+factor variant), on one MI355X; the headline value is the faster mixed
+solve that meets the accuracy bar (FP64 true residual <= 1e-6, heads within
+1e-5 m of the oracle's sample).
 
 A step = one NumericalSolution outer iteration of the hot path from the
 start heads: device formulate (npf/sto/bnd fill) + sln_ls (row fixes, ILU0
@@ -315,7 +318,16 @@ def main():
         for k, (recs, heads, amat, rhs) in res.items():
             summary[k]["max_head_diff_vs_oracle_m"] = float(
                 np.max(np.abs(heads[g["cells"]] - g["heads"])))
-    head = next(k for k in ("mixed", "mixed16", "fp64") if k in summary)
+    # headline: the faster mixed variant (FP32 or FP16 factor) whose solve
+    # converged with the FP64 true residual within rclose and heads within
+    # 1e-5 m of the oracle's sample; FP64 when no mixed variant ran
+    def valid(k):
+        m = summary[k]
+        return (m["converged"] and m["true_residual_inf"] <= 1e-6 and
+                m.get("max_head_diff_vs_oracle_m", 0.0) <= 1e-5)
+    mixed = [k for k in ("mixed", "mixed16") if k in summary and valid(k)]
+    head = (min(mixed, key=lambda k: summary[k]["s_per_solve"]) if mixed else
+            next(k for k in ("mixed", "mixed16", "fp64") if k in summary))
     value = summary[head]["s_per_solve"]
 
     # roofline of the dominant kernel, k_f_spmv (CG SpMV q = A p): symmetric
@@ -383,6 +395,8 @@ def main():
     if "fp64" in summary and "mixed16" in summary:
         line["speedup_mixed16_vs_fp64"] = (summary["fp64"]["s_per_solve"] /
                                            summary["mixed16"]["s_per_solve"])
+    if "fp64" in summary and head != "fp64":
+        line["speedup_headline_vs_fp64"] = summary["fp64"]["s_per_solve"] / value
     print(json.dumps(line))
 
 
