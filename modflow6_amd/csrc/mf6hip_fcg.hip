@@ -125,13 +125,13 @@ __global__ void k_fcg_decide(FcgState *fs, const KState *ks) {
   fs->w = w;
 }
 
-// zl = z + w y
+// zl = z + w y (zl = z when the step was declined: the post-smoothing that
+// follows then works on defined values even though its result is dropped)
 __global__ void k_fcg_axpy_w(int n, const FcgState *fs, const double *__restrict__ z,
                              const double *__restrict__ y, double *__restrict__ zl) {
   const double w = fs->w;
-  if (w == 0.0) return;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-    zl[e] = z[e] + w * y[e];
+    zl[e] = w == 0.0 ? z[e] : z[e] + w * y[e];
 }
 
 // zl += zs (post-smoothing correction)
