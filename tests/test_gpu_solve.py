@@ -95,9 +95,15 @@ def test_simulation_transient_newton_parity():
     assert np.max(np.abs(hd - ho)) < 10 * case.ims.inner_dvclose * 10
 
 
+@pytest.mark.parametrize("shape", [(3, 100, 100), (2, 33, 17), (2, 30, 30)])
 @pytest.mark.parametrize("precision", [_lib.MF6HIP_MIXED, _lib.MF6HIP_MIXED_FP16])
-def test_mixed_precision_steady_small(precision):
-    case = configs.steady_small()
+def test_mixed_precision_steady_small(precision, shape):
+    """Mixed solves on a layout where the fused kernels take two columns per
+    thread (100 x 100) and on two where they take one (odd ncol; ncol / 2
+    odd)."""
+    nlay, nrow, ncol = shape
+    case = configs.steady_small(nlay=nlay, nrow=nrow, ncol=ncol,
+                                nwel=min(20, nrow * (ncol - 2) // 4))
     sim = O.GwfSimulation(case)
     ho = sim.run()
     gs = GwfSolution(case, precision=precision)
