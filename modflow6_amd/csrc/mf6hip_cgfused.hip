@@ -906,12 +906,64 @@ static int read_states(Mf6Handle *h) {
   return MF6HIP_OK;
 }
 
-// Fused FP64 CG on the device system.  Requires: d = b - A x in h->d,
-// l2norm0 in state[0], q zeroed, A symmetric.
-int mf6hip_run_cg_fused(Mf6Handle *h) {
+// Enqueue n fused FP64 CG iterations starting at an even round (the two
+// direction buffers alternate; n even keeps the parity).
+static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
   KState *st = h->d_state;
+  FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
+  double *pbuf[2] = {h->p, h->p2};
+  for (int round = 0; round < n; round++) {
+    launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs, nbc,
+                                      nb);
+    k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
+                                      h->d_hloc);
+    double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
+    launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs);
+    int slot = -1;
+    if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
+        2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
+      slot = h->last_spmv_n++;
+      (void)hipEventRecord(h->spmv_ev[2 * slot], s);
+    }
+    launch_f_spmv<double, double>(L, nb, s, h->A, pout, h->q, h->d_part, st);
+    if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
+    k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
+  }
+}
+
+// Capture FUSED_GRAPH_ITERS iterations as a graph (once per handle; the
+// buffers and grid sizes are fixed for the handle's life).  The graph holds
+// kernel launches only.
+constexpr int FUSED_GRAPH_ITERS = 8;
+
+static bool fused64_graph(Mf6Handle *h, int nbc, int nb) {
+  if (h->cg_graph) return true;
+  if (h->cg_graph_failed) return false;
+  hipStream_t s = h->stream;
+  hipGraph_t g = nullptr;
+  bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    enqueue_fused64(h, FUSED_GRAPH_ITERS, nbc, nb);
+    ok = hipStreamEndCapture(s, &g) == hipSuccess && g != nullptr;
+  }
+  if (ok) ok = hipGraphInstantiate(&h->cg_graph, g, nullptr, nullptr, 0) == hipSuccess;
+  if (g) (void)hipGraphDestroy(g);
+  if (!ok) {
+    h->cg_graph = nullptr;
+    h->cg_graph_failed = true;
+    (void)hipGetLastError();
+  }
+  return ok;
+}
+
+// Fused FP64 CG on the device system.  Requires: d = b - A x in h->d,
+// l2norm0 in state[0], q zeroed, A symmetric.  Without the per-launch
+// timing the iterations are replayed from a captured graph.
+int mf6hip_run_cg_fused(Mf6Handle *h) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
   FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
   HIPCHK(hipMemsetAsync(fs, 0, sizeof(FState), s));
   HIPCHK(hipMemsetAsync(h->q, 0, sizeof(double) * L.NP, s));
@@ -921,26 +973,14 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
   const int nbs = gridc(2 * L.HP0);
   const int nb = nbs > nbc ? nbs : nbc;
   double *pbuf[2] = {h->p, h->p2};
-  int batch = 4;
-  int round = 0;
-  // round 0: preconditioner of the initial residual
+  const bool graph = !h->timing && fused64_graph(h, nbc, nb);
+  int batch = graph ? FUSED_GRAPH_ITERS : 4;
+  // round 0: preconditioner of the initial residual; batches are even
   while (true) {
-    for (int it = 0; it < batch; it++, round++) {
-      launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs,
-                                        nbc, nb);
-      k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                        h->d_hloc);
-      double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
-      launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs);
-      int slot = -1;
-      if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
-          2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
-        slot = h->last_spmv_n++;
-        (void)hipEventRecord(h->spmv_ev[2 * slot], s);
-      }
-      launch_f_spmv<double, double>(L, nb, s, h->A, pout, h->q, h->d_part, st);
-      if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
-      k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
+    if (graph) {
+      for (int r = 0; r < batch / FUSED_GRAPH_ITERS; r++) HIPCHK(hipGraphLaunch(h->cg_graph, s));
+    } else {
+      enqueue_fused64(h, batch, nbc, nb);
     }
     HIPCHK(hipGetLastError());
     CK(read_states(h));

@@ -126,6 +126,10 @@ struct Mf6Handle {
   double last_solve_ms = 0, last_spmv_ms = 0;
   int last_spmv_n = 0;
   bool timing = false;
+  // fused FP64 CG: eight iterations captured once per handle as a HIP graph
+  // (used when the per-launch SpMV timing is off)
+  hipGraphExec_t cg_graph = nullptr;
+  bool cg_graph_failed = false;
   int spmv_calls = 0;
 };
 

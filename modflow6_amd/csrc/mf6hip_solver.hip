@@ -1607,6 +1607,7 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->flowja, h->strg, h->bsim, h->chd_flag};
   for (void *b : bufs)
     if (b) hipFree(b);
+  if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
   for (int t = 0; t < 5; t++) {
     DevPackage &pk = h->pk[t];
     void *pb[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs};

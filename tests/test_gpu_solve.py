@@ -139,3 +139,27 @@ def test_fused_cg_matches_unfused(shape):
     for t in range(5):
         assert abs(res[0][3].dvmax[t] - res[1][3].dvmax[t]) <= 1e-9 * abs(res[1][3].dvmax[t])
         assert abs(res[0][3].rmax[t] - res[1][3].rmax[t]) <= 1e-9 * abs(res[1][3].rmax[t])
+
+
+def test_fused_cg_graph_replay_matches_eager_launches():
+    """Without the per-launch timing the fused FP64 CG replays captured
+    eight-iteration graphs; with it, it launches every kernel.  The same
+    kernels run in the same order, so heads and iteration counts agree bit
+    for bit, on the first and on a repeated run of the same handle (the
+    graph is reused; host synchronisations sit between the replays)."""
+    case = configs.steady_small(nlay=3, nrow=40, ncol=40, nwel=8)
+    eager = GwfSolution(case, timing=True)
+    he = eager.run()
+    inner_e = [r.inner for r in eager.log[0]]
+    eager.destroy()
+    gs = GwfSolution(case)
+    for rep in range(2):
+        if rep:
+            from modflow6_amd import _lib
+            _lib.check(gs.L.mf6hip_gwf_set_x(gs.h, _lib.dptr(np.ascontiguousarray(case.strt))),
+                       "set_x")
+            gs.log.clear()
+        hg = gs.run()
+        assert [r.inner for r in gs.log[0]] == inner_e
+        assert np.array_equal(hg.view(np.int64), he.view(np.int64))
+    gs.destroy()
