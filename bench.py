@@ -36,10 +36,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_precision(case, precision, steps, warmup, label, replace=0):
+def run_precision(case, precision, steps, warmup, label, replace=0, timing=True):
     from modflow6_amd import _lib
     from modflow6_amd.solution import GwfSolution
-    gs = GwfSolution(case, precision=precision, timing=True,
+    gs = GwfSolution(case, precision=precision, timing=timing,
                      inner_fp32_max=replace)
     x0 = np.ascontiguousarray(case.strt, dtype=np.float64)
     out = []
@@ -252,6 +252,9 @@ def main():
                     help="10m: the headline model (--shape); small: the 3 x 100 x 100 "
                          "model the reference runs on a CPU")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="no per-launch SpMV events (the FP64 fused CG then replays "
+                         "captured graphs; no SpMV time is reported)")
     ap.add_argument("--replace", type=int, default=0,
                     help="mixed: FP32 iterations between residual replacements (0 = 64)")
     ap.add_argument("--workload", default="steady",
@@ -298,7 +301,8 @@ def main():
 
     res = {}
     if args.precision in ("both", "fp64"):
-        res["fp64"] = run_precision(case, _lib.MF6HIP_FP64, args.steps, args.warmup, "fp64")
+        res["fp64"] = run_precision(case, _lib.MF6HIP_FP64, args.steps, args.warmup, "fp64",
+                                    timing=not args.no_timing)
     if args.precision in ("both", "mixed"):
         res["mixed"] = run_precision(case, _lib.MF6HIP_MIXED, args.steps, args.warmup, "mixed",
                                      replace=args.replace)
