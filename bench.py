@@ -335,6 +335,12 @@ def main():
         for k, (recs, heads, amat, rhs) in res.items():
             summary[k]["max_head_diff_vs_oracle_m"] = float(
                 np.max(np.abs(heads[g["cells"]] - g["heads"])))
+    gold_small = os.path.join(ROOT, "tests", "golden", "steady_small_oracle.npz")
+    if args.case == "small" and os.path.exists(gold_small):
+        g = np.load(gold_small)
+        for k, (recs, heads, amat, rhs) in res.items():
+            summary[k]["max_head_diff_vs_oracle_m"] = float(np.max(np.abs(heads - g["heads"])))
+            summary[k]["oracle_iterations"] = int(g["iterations"])
     # headline: the faster mixed variant (FP32 or FP16 factor) whose solve
     # converged with the FP64 true residual within rclose and heads within
     # 1e-5 m of the oracle's sample; FP64 when no mixed variant ran

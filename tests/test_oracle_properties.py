@@ -179,3 +179,20 @@ def test_simple_under_relaxation_step():
     exp = a.xtemp + g * (x_lin - a.xtemp)
     keep = ~moved & (a.ibound > 0)
     assert np.allclose(a.x[keep], exp[keep], rtol=0, atol=1e-12)
+
+
+def test_small_model_golden_heads_current():
+    """tests/golden/steady_small_oracle.npz (the bench's head reference for
+    the 3 x 100 x 100 model) is what the oracle computes today."""
+    g = np.load(os.path.join(GOLD, "steady_small_oracle.npz"))
+    case = configs.steady_small()
+    sim = O.GwfSimulation(case)
+    sim.kstp = 1
+    sim.advance(1.0, True)
+    sim.formulate(1)
+    d = sim.dis
+    O.lib().orc_sln_ls_fixes(sim.neq, d.ia, d.ja, sim.ibound, 1, sim.amat,
+                             sim.rhs, sim.x, sim.xtemp)
+    icn, it, _, _, _ = sim.imslinear.apply(sim.amat, sim.rhs, sim.x, kstp=1)
+    assert icn == 1 and it == int(g["iterations"])
+    assert np.array_equal(sim.x.view(np.int64), g["heads"].view(np.int64))
