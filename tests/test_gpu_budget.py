@@ -81,3 +81,26 @@ def test_device_run_closes_its_budget():
     assert abs(b["pct_discrepancy"]) < 1e-3
     assert b["terms"]["STO-SS"] != (0.0, 0.0)
     assert b["terms"]["RCH"][0] > 0.0 and b["terms"]["WEL"][1] > 0.0
+
+
+@pytest.mark.parametrize("name", ["steady_small", "transient_newton", "bump"])
+def test_device_run_budget_within_1e6_of_oracle_run(name):
+    """The device and the oracle each solve the first time step on their own
+    (heads agree to within 10 inner_dvclose); their budget terms then agree
+    within 1e-6 of the largest term, the cell-budget bar of the comparison
+    with the reference run."""
+    case = CASES[name]()
+    sim, delt, steady = _oracle_first_step(case)
+    ob = sim.budget()
+    gs = GwfSolution(case)
+    gs.kper = 1
+    gs.run_timestep(delt, steady)
+    hd = gs.heads()
+    db = gs.budget(want_flowja=False)
+    gs.destroy()
+    assert np.max(np.abs(hd - sim.x)) < 10 * case.ims.inner_dvclose
+    scale = max(max(v) for v in ob["terms"].values())
+    for k, (rin, rout) in db["terms"].items():
+        oin, oout = ob["terms"].get(k, (0.0, 0.0))
+        assert abs(rin - oin) <= 1e-6 * scale and abs(rout - oout) <= 1e-6 * scale, \
+            (k, rin, oin, rout, oout)
