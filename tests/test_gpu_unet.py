@@ -183,3 +183,37 @@ def test_fcg_without_network_matches_pcg():
     print("ILU0-PCG", it_ref, "ILU0-FCG", it_l)
     assert icl == 1 and abs(it_l - it_ref) <= max(3, it_ref // 20)
     assert np.max(np.abs(xl - xr)) < 10 * case.ims.inner_dvclose
+
+
+def test_trained_network_gain_over_flexible_cg_without_it():
+    """What the trained network gains: a 6-level network trained on the
+    device on ILU0-PCG (residual, error) pairs of eight seeded 4 x 64 x 64
+    fields, applied to three unseen fields inside flexible CG, against the
+    same flexible CG without it.  The gain (ILU0-only iterations / learned
+    iterations) is printed; at a tight tolerance it is below one (a
+    correction must be accurate to ~0.1 % per cell to pay, a BF16 network
+    is not), so the test holds the properties that make the method safe:
+    every solve converges to the ILU0-PCG heads, training lowers the loss,
+    and the stall fallback bounds the cost to under three times the
+    iterations without the network."""
+    from modflow6_amd import learned as LN
+    shape = (4, 64, 64)
+    data = LN.make_dataset(list(range(300, 308)), shape, log=lambda *a: None)
+    net = LN.UNet(*shape, nin=3, channels=LN.CHANNELS, seed=0)
+    net.set_data(data)
+    losses = net.train(240, 4, 1e-3, seed=0)
+    assert np.all(np.isfinite(losses))
+    assert np.mean(losses[-20:]) < np.mean(losses[:5])
+    gains = []
+    for seed in (900, 901, 902):
+        r = LN.solve_pair(LN.learned_case(seed, *shape), net, fcg_m=8)
+        assert r["pcg"]["converged"] and r["fcg_ilu0"]["converged"]
+        assert r["fcg_learned"]["converged"]
+        assert r["fcg_learned"]["max_head_diff_vs_pcg_m"] < 1e-5
+        base, lrn = r["fcg_ilu0"]["iterations"], r["fcg_learned"]["iterations"]
+        gains.append(base / lrn)
+        print(f"seed {seed}: PCG {r['pcg']['iterations']}, FCG+ILU0 {base}, FCG+net {lrn} "
+              f"({r['fcg_learned']['learned_iterations']} learned), gain {base / lrn:.2f}")
+        assert lrn < 3 * base
+    net.destroy()
+    print("iteration gain of the trained network:", [f"{g:.2f}" for g in gains])
