@@ -199,6 +199,26 @@ int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double *amat_out,
  * mf6hip_gwf_formulate, as npf_cq uses the sat of the last npf_cf. */
 int mf6hip_gwf_budget(void *handle, double *flowja, double *terms, double *pct_discrepancy);
 
+/* Flow terms of mf6hip_gwf_budget, in the order of its terms[] pairs. */
+#define MF6HIP_TERM_STO_SS 0
+#define MF6HIP_TERM_STO_SY 1
+#define MF6HIP_TERM_WEL 2
+#define MF6HIP_TERM_DRN 3
+#define MF6HIP_TERM_RCH 4
+#define MF6HIP_TERM_CHD 5
+
+/* One flow term of the last mf6hip_gwf_budget, as the reference saves it to
+ * the cell-by-cell budget file (gwf_ot_flow, gwf.f90:911-958): STO-SS /
+ * STO-SY are the storage rates of every cell in natural node order
+ * (sto%strgss / strgsy, written by sto_save_model_flows, gwf-sto.f90:602-640);
+ * WEL / DRN / RCH / CHD are the package's simvals in bound order (bnd_cq,
+ * BoundaryPackage.f90:545-619; CHD: calc_chd_rate, gwf-chd.f90:264-320),
+ * written by save_print_model_flows (BoundaryPackage.f90:1753-1902).
+ * *count receives the number of values (nodes, or the package's nbound at
+ * that budget); out may be NULL to query the count only.  MF6HIP_EINVAL if
+ * no budget has been computed on the handle. */
+int mf6hip_gwf_flow_term(void *handle, int term, double *out, int *count);
+
 /* r = A x - b of the current device system with inactive rows zeroed
  * (sln_calc_residual, NumericalSolution.f90:2966-2982): *l2norm = ||r||_2
  * (sln_l2norm, :2855-2871) and *rvmax = max |r(n)| / (area (top - bot))

@@ -19,6 +19,8 @@ MF6HIP_MIXED = 1
 MF6HIP_MIXED_FP16 = 2
 MF6HIP_ORDER_RB = 0
 PKG = {"WEL": 1, "DRN": 2, "RCH": 3, "CHD": 4}
+# MF6HIP_TERM_*: the flow terms of mf6hip_gwf_budget / mf6hip_gwf_flow_term
+TERM = {"STO-SS": 0, "STO-SY": 1, "WEL": 2, "DRN": 3, "RCH": 4, "CHD": 5}
 
 # every entry point declared in include/mf6hip.h
 EXPORTS = [
@@ -28,6 +30,7 @@ EXPORTS = [
     "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
     "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
     "mf6hip_gwf_residual_norms", "mf6hip_gwf_backtrack", "mf6hip_gwf_budget",
+    "mf6hip_gwf_flow_term",
     "mf6hip_last_timing", "mf6hip_set_timing", "mf6hip_apply_op",
     "mf6hip_load_factor", "mf6hip_get_system",
     "mf6hip_unet_create", "mf6hip_unet_destroy", "mf6hip_unet_num_params",
@@ -132,6 +135,7 @@ def load():
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
     L.mf6hip_gwf_set_x_device.argtypes = [vp, vp]
     L.mf6hip_gwf_budget.argtypes = [vp, dp, dp, C.POINTER(C.c_double)]
+    L.mf6hip_gwf_flow_term.argtypes = [vp, i, dp, ip]
     L.mf6hip_gwf_get_x_device.argtypes = [vp, vp]
     L.mf6hip_gwf_advance.argtypes = [vp, i, d]
     L.mf6hip_gwf_formulate.argtypes = [vp, i, i, dp, dp]

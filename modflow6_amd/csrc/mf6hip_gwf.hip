@@ -892,25 +892,27 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
   }
 }
 
-// calc_chd_rate's in / out split per CHD bound (over active neighbours; the
-// flows of the cell's row in CSR order: up, back, left, right, front, down)
+// calc_chd_rate per CHD bound: the rate (minus the flows of the cell's row,
+// every neighbour) and its in / out split (active neighbours only); the
+// row's flows in CSR order: up, back, left, right, front, down
 __global__ void k_chd_inout(int nb, const int *__restrict__ nodes, const int *__restrict__ ia,
                             Layout L, const int *__restrict__ ibound,
                             const double *__restrict__ flowja, double *__restrict__ rin,
-                            double *__restrict__ rout) {
+                            double *__restrict__ rout, double *__restrict__ rsim) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
     const int n = nodes[e];
     const int k = n / L.NRC, r = n - k * L.NRC, i = r / L.ncol, j = r - i * L.ncol;
     const int kk[6] = {k - 1, k, k, k, k, k + 1};
     const int ii[6] = {i, i - 1, i, i, i + 1, i};
     const int jj[6] = {j, j, j - 1, j + 1, j, j};
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, rate = 0.0;
     int ip = ia[n] + 1;
     for (int s = 0; s < 6; s++) {
       if (kk[s] < 0 || kk[s] >= L.nlay || ii[s] < 0 || ii[s] >= L.nrow || jj[s] < 0 ||
           jj[s] >= L.ncol)
         continue;
       const double qv = flowja[ip++];
+      rate = rate - qv;
       if (ibound[cell_pos(L, kk[s], ii[s], jj[s])] > 0) {
         if (qv < 0.0) a = a - qv;
         else b = b + qv;
@@ -918,6 +920,7 @@ __global__ void k_chd_inout(int nb, const int *__restrict__ nodes, const int *__
     }
     rin[e] = a;
     rout[e] = b;
+    rsim[e] = rate;
   }
 }
 
@@ -1470,7 +1473,7 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
   const int pkorder[3] = {MF6HIP_PKG_WEL, MF6HIP_PKG_DRN, MF6HIP_PKG_RCH};
   size_t need = 0;
   for (int t : pkorder) need += (size_t)h->pk[t].nbound;
-  need += 2 * (size_t)h->pk[MF6HIP_PKG_CHD].nbound;
+  need += 3 * (size_t)h->pk[MF6HIP_PKG_CHD].nbound;
   CK(galloc(&h->flowja, (size_t)h->nja));
   CK(galloc(&h->strg, 2 * (size_t)L.NP));
   CK(galloc(&h->chd_flag, (size_t)L.NP));
@@ -1491,6 +1494,8 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
     const int t = pkorder[q];
     DevPackage &pk = h->pk[t];
     segs[q] = seg;
+    h->bsim_seg_off[q] = (size_t)(seg - h->bsim);
+    h->bsim_seg_len[q] = pk.nbound;
     if (pk.nbound > 0) {
       k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(t, pk.nbound, pk.nodes, pk.v1, pk.v2, L,
                                                       h->ibound, h->x, h->area, pk.hcof,
@@ -1502,7 +1507,9 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
     seg += pk.nbound;
   }
   DevPackage &chd = h->pk[MF6HIP_PKG_CHD];
-  double *chd_in = seg, *chd_out = seg + chd.nbound;
+  double *chd_in = seg, *chd_out = seg + chd.nbound, *chd_sim = seg + 2 * (size_t)chd.nbound;
+  h->bsim_seg_off[3] = (size_t)(chd_sim - h->bsim);
+  h->bsim_seg_len[3] = chd.nbound;
   HIPCHK(hipMemsetAsync(h->chd_flag, 0, sizeof(int) * L.NP, s));
   if (chd.nbound > 0)
     k_mark_cells<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, L, h->chd_flag);
@@ -1511,7 +1518,8 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
                                                   h->x, h->xold, h->flowja, h->strg);
   if (chd.nbound > 0)
     k_chd_inout<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, h->d_ia, L,
-                                                       h->ibound, h->flowja, chd_in, chd_out);
+                                                       h->ibound, h->flowja, chd_in, chd_out,
+                                                       chd_sim);
   HIPCHK(hipGetLastError());
   // rate_accumulator of each term: STO-SS, STO-SY, WEL, DRN, RCH, then the
   // CHD in and out lists (all non-negative: their "in" sums)
@@ -1543,5 +1551,42 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
   *pct_discrepancy = avg != 0.0 ? 100.0 * (totin - totout) / avg : 0.0;
   if (flowja_out)
     HIPCHK(hipMemcpy(flowja_out, h->flowja, sizeof(double) * h->nja, hipMemcpyDeviceToHost));
+  h->budget_done = 1;
+  return MF6HIP_OK;
+}
+
+// One flow term of the last mf6hip_gwf_budget: the STO-SS / STO-SY rate of
+// every cell (natural order; sto%strgss / strgsy, gwf-sto.f90:447-564) or a
+// package's simvals in bound order (bnd_cq, BoundaryPackage.f90:545-619;
+// CHD: calc_chd_rate, gwf-chd.f90:264-320).  *count receives the number of
+// values; out may be NULL to query it.
+extern "C" int mf6hip_gwf_flow_term(void *handle, int term, double *out, int *count) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  if (!count) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
+  if (term < MF6HIP_TERM_STO_SS || term > MF6HIP_TERM_CHD) {
+    mf6hip_set_error("unknown flow term");
+    return MF6HIP_EINVAL;
+  }
+  if (!h->budget_done) {
+    mf6hip_set_error("no budget has been computed on this handle");
+    return MF6HIP_EINVAL;
+  }
+  const Layout &L = h->L;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (term <= MF6HIP_TERM_STO_SY) {
+    *count = L.N;
+    if (!out) return MF6HIP_OK;
+    std::vector<double> tmp(L.NP);
+    HIPCHK(hipMemcpy(tmp.data(), h->strg + (size_t)(term - MF6HIP_TERM_STO_SS) * L.NP,
+                     sizeof(double) * L.NP, hipMemcpyDeviceToHost));
+    for (int n = 0; n < L.N; n++) out[n] = tmp[pos_of_natural(L, n)];
+    return MF6HIP_OK;
+  }
+  const int q = term - MF6HIP_TERM_WEL;
+  *count = h->bsim_seg_len[q];
+  if (out && *count > 0)
+    HIPCHK(hipMemcpy(out, h->bsim + h->bsim_seg_off[q], sizeof(double) * (size_t)*count,
+                     hipMemcpyDeviceToHost));
   return MF6HIP_OK;
 }

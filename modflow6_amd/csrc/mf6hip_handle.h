@@ -103,8 +103,13 @@ struct Mf6Handle {
   // budget (gwf_cq + gwf_bd) work arrays, allocated on first use
   double *flowja = nullptr;   // nja, CSR order
   double *strg = nullptr;     // [2][NP]: STO-SS, STO-SY rates
-  double *bsim = nullptr;     // per-bound rates of WEL, DRN, RCH, then CHD in / out
+  double *bsim = nullptr;     // per-bound rates of WEL, DRN, RCH, then CHD in / out / rate
   size_t bsim_n = 0;
+  // the per-bound segments of bsim at the last budget (WEL, DRN, RCH, CHD
+  // rate): offset and length; budget_done once a budget has run
+  size_t bsim_seg_off[4] = {0, 0, 0, 0};
+  int bsim_seg_len[4] = {0, 0, 0, 0};
+  int budget_done = 0;
   int *chd_flag = nullptr;    // per position: a CHD cell
   double *xold = nullptr;
   double *dxold = nullptr;

@@ -41,6 +41,12 @@ module mf6hip_module
   integer(c_int), parameter, public :: MF6HIP_PKG_DRN = 2
   integer(c_int), parameter, public :: MF6HIP_PKG_RCH = 3
   integer(c_int), parameter, public :: MF6HIP_PKG_CHD = 4
+  integer(c_int), parameter, public :: MF6HIP_TERM_STO_SS = 0
+  integer(c_int), parameter, public :: MF6HIP_TERM_STO_SY = 1
+  integer(c_int), parameter, public :: MF6HIP_TERM_WEL = 2
+  integer(c_int), parameter, public :: MF6HIP_TERM_DRN = 3
+  integer(c_int), parameter, public :: MF6HIP_TERM_RCH = 4
+  integer(c_int), parameter, public :: MF6HIP_TERM_CHD = 5
 
   !> mf6hip_gwf_model: DIS/NPF/STO arrays of one GWF model (c_loc of the
   !! memory-manager arrays; natural node order)
@@ -98,7 +104,8 @@ module mf6hip_module
             mf6hip_gwf_get_x, mf6hip_gwf_advance, mf6hip_gwf_formulate, &
             mf6hip_gwf_residual_norms, mf6hip_gwf_backtrack, &
             mf6hip_gwf_linear_solve, mf6hip_gwf_outer_update, &
-            mf6hip_gwf_budget, mf6hip_gwf_set_x_device, mf6hip_gwf_get_x_device
+            mf6hip_gwf_budget, mf6hip_gwf_set_x_device, mf6hip_gwf_get_x_device, &
+            mf6hip_gwf_flow_term
 
   interface
     function mf6hip_create(neq, nja, ia, ja, settings, handle) result(rc) &
@@ -196,6 +203,16 @@ module mf6hip_module
       real(c_double), intent(out) :: flowja(*)
       real(c_double), intent(out) :: terms(12)
       real(c_double), intent(out) :: pct
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_flow_term(handle, term, values, count) result(rc) &
+      bind(C, name='mf6hip_gwf_flow_term')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: term
+      type(c_ptr), value :: values
+      integer(c_int), intent(out) :: count
       integer(c_int) :: rc
     end function
 

@@ -30,7 +30,7 @@ from typing import List
 
 import numpy as np
 
-from . import _lib
+from . import _lib, binaryfile
 from .ims import HipLinearSolver, ImsLinearSettings, dis_structure
 
 
@@ -112,6 +112,8 @@ class GwfSolution:
             _lib.check(L.mf6hip_set_timing(self.h, 1, 16), "mf6hip_set_timing")
         self.kstp = 0
         self.kper = 0
+        self.pertim = 0.0
+        self.totim = 0.0
         self.iss = 1
         self.delt = 1.0
         self.log: List[List[OuterRecord]] = []
@@ -257,16 +259,31 @@ class GwfSolution:
     def run_timestep(self, delt, steady):
         self.kstp += 1
         self.prepare_timestep(delt, steady)
+        self.pertim += delt
+        self.totim += delt
         return self.sln_ca()
 
-    def run(self, max_steps=None):
+    def run(self, max_steps=None, head_file=None, budget_file=None, **budget_options):
+        """Every stress period and time step of the case (equal steps).
+        head_file / budget_file (binary files open for writing, or None)
+        receive the heads and the cell-by-cell flows after every step, as
+        OC's SAVE HEAD ALL / SAVE BUDGET ALL with SAVE_FLOWS on every
+        package write them; budget_options go to budget_records."""
         nsteps = 0
         self.kper = 0
+        self.totim = 0.0
         for perlen, nstp, steady in self.case.periods:
             self.kper += 1
             self.kstp = 0
-            for _ in range(nstp):
+            self.pertim = 0.0
+            for istp in range(nstp):
                 self.run_timestep(perlen / nstp, steady)
+                if istp == nstp - 1:
+                    self.pertim = float(perlen)   # end of period (tdis.f90:196-199)
+                if budget_file is not None:
+                    budget_file.write(self.budget_records(**budget_options))
+                if head_file is not None:
+                    head_file.write(self.head_records())
                 nsteps += 1
                 if max_steps is not None and nsteps >= max_steps:
                     return self.heads()
@@ -289,6 +306,45 @@ class GwfSolution:
         return dict(flowja=fj, pct_discrepancy=pct.value,
                     terms={k: (terms[2 * t], terms[2 * t + 1])
                            for t, k in enumerate(self.BUDGET_TERMS)})
+
+    def flow_term(self, name):
+        """One flow term of the last budget(): STO-SS / STO-SY rates per cell
+        (natural order) or a package's simvals in bound order."""
+        n = C.c_int(0)
+        t = _lib.TERM[name]
+        _lib.check(self.L.mf6hip_gwf_flow_term(self.h, t, None, C.byref(n)),
+                   "mf6hip_gwf_flow_term")
+        out = np.zeros(n.value)
+        _lib.check(self.L.mf6hip_gwf_flow_term(self.h, t, _lib.dptr(out), C.byref(n)),
+                   "mf6hip_gwf_flow_term")
+        return out
+
+    def head_records(self):
+        """Head file records (ulasav per layer) of the current heads."""
+        c = self.case
+        return binaryfile.head_records(self.heads(), c.nlay, c.nrow, c.ncol, self.kstp,
+                                       self.kper, self.pertim, self.totim)
+
+    def budget_records(self, model_name="GWF", package_names=None, rch_readasarrays=False):
+        """Budget file records (gwf_ot_flow order) of a budget of the current
+        heads; package_names maps a package kind to its name (default
+        "<KIND>-1", the reference's unnamed-package name)."""
+        c = self.case
+        b = self.budget(want_flowja=True)
+        strgss = strgsy = None
+        if c.has_sto:
+            strgss = self.flow_term("STO-SS")
+            if c.iconvert is not None and np.any(np.asarray(c.iconvert) != 0):
+                strgsy = self.flow_term("STO-SY")
+        names = package_names or {}
+        pk = []
+        for p in c.packages:
+            kind = "RCHA" if (p.kind == "RCH" and rch_readasarrays) else p.kind
+            pk.append((kind, names.get(p.kind, f"{kind}-1"), np.asarray(p.nodes),
+                       self.flow_term(p.kind)))
+        return binaryfile.budget_records(self.kstp, self.kper, self.delt, self.pertim,
+                                         self.totim, c.nlay, c.nrow, c.ncol, b["flowja"],
+                                         strgss, strgsy, pk, model_name)
 
     def heads(self):
         x = np.zeros(self.case.nodes)

@@ -191,6 +191,11 @@ def ref_lib():
         R.ref_sscapacity.argtypes = [i, d, d, d, d]
         R.ref_sycapacity.restype = d
         R.ref_sycapacity.argtypes = [d, d]
+        cp = C.c_char_p
+        R.ref_ulasav.argtypes = [cp, i, i, f64p, cp, i, i, d, d, i, i, i]
+        R.ref_ubdsv1.argtypes = [cp, i, i, i, i, cp, f64p, i, i, i, d, d, d]
+        R.ref_ubdsv06_list.argtypes = [cp, i, i, i, i, cp, cp, cp, cp, cp, i, i, i,
+                                       i, i32p, i32p, f64p, d, d, d]
         _REF = R
     return _REF
 

@@ -10,7 +10,7 @@
 ! Index convention: arrays cross this boundary 1-based, exactly as the
 ! reference stores them (ia/ja/iapc/japc hold Fortran positions).
 module mf6ref_shim
-  use iso_c_binding, only: c_int, c_double
+  use iso_c_binding, only: c_int, c_double, c_char
   use KindModule, only: DP, I4B
   use IMSLinearBaseModule, only: ims_base_pccrs, ims_base_pcu, &
                                  ims_base_ilu0a, ims_base_cg, ims_base_bcgs, &
@@ -22,6 +22,7 @@ module mf6ref_shim
   use SmoothingModule, only: sQuadraticSaturation, &
                              sQuadraticSaturationDerivative
   use GwfStorageUtilsModule, only: SsTerms, SyTerms, SsCapacity, SyCapacity
+  use InputOutputModule, only: ulasav, ubdsv1, ubdsv06, ubdsvd
   implicit none
   private
 
@@ -295,5 +296,94 @@ contains
     real(c_double) :: s
     s = SyCapacity(area, sy)
   end function ref_sycapacity
+
+  ! --- binary output: the reference's own record writers on a unit opened
+  ! as the reference opens binary output (OpenSpec.f90:17-18:
+  ! FORM = 'UNFORMATTED', ACCESS = 'STREAM'); append /= 0 adds to the file
+
+  function open_stream(path, plen, append) result(iu)
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), intent(in) :: plen, append
+    integer(I4B) :: iu
+    character(len=plen) :: fname
+    integer(I4B) :: i
+    do i = 1, plen
+      fname(i:i) = path(i)
+    end do
+    if (append /= 0) then
+      open (newunit=iu, file=fname, form='UNFORMATTED', access='STREAM', &
+            status='OLD', position='APPEND')
+    else
+      open (newunit=iu, file=fname, form='UNFORMATTED', access='STREAM', &
+            status='REPLACE')
+    end if
+  end function open_stream
+
+  function text16(c) result(t)
+    character(kind=c_char), intent(in) :: c(16)
+    character(len=16) :: t
+    integer(I4B) :: i
+    do i = 1, 16
+      t(i:i) = c(i)
+    end do
+  end function text16
+
+  ! InputOutput.f90:924 ulasav
+  subroutine ref_ulasav(path, plen, append, buf, text, kstp, kper, pertim, &
+                        totim, ncol, nrow, ilay) bind(C, name='ref_ulasav')
+    character(kind=c_char), intent(in) :: path(*), text(16)
+    integer(c_int), value :: plen, append, kstp, kper, ncol, nrow, ilay
+    real(c_double), value :: pertim, totim
+    real(c_double), intent(in) :: buf(ncol, nrow)
+    character(len=16) :: t
+    integer(I4B) :: iu
+    real(DP) :: pt, tt
+    t = text16(text)
+    pt = pertim
+    tt = totim
+    iu = open_stream(path, plen, append)
+    call ulasav(buf, t, kstp, kper, pt, tt, ncol, nrow, ilay, iu)
+    close (iu)
+  end subroutine ref_ulasav
+
+  ! InputOutput.f90:944 ubdsv1
+  subroutine ref_ubdsv1(path, plen, append, kstp, kper, text, buff, ncol, nrow, &
+                        nlay, delt, pertim, totim) bind(C, name='ref_ubdsv1')
+    character(kind=c_char), intent(in) :: path(*), text(16)
+    integer(c_int), value :: plen, append, kstp, kper, ncol, nrow, nlay
+    real(c_double), value :: delt, pertim, totim
+    real(c_double), intent(in) :: buff(ncol * nrow * nlay)
+    integer(I4B) :: iu
+    iu = open_stream(path, plen, append)
+    call ubdsv1(kstp, kper, text16(text), iu, buff, ncol, nrow, nlay, 0, &
+                delt, pertim, totim)
+    close (iu)
+  end subroutine ref_ubdsv1
+
+  ! InputOutput.f90:979 ubdsv06 header, then one :1053 ubdsvd entry per
+  ! list item (no auxiliary variables)
+  subroutine ref_ubdsv06_list(path, plen, append, kstp, kper, text, model1, &
+                              package1, model2, package2, ncol, nrow, nlay, &
+                              nlist, nodes, nodes2, q, delt, pertim, totim) &
+    bind(C, name='ref_ubdsv06_list')
+    character(kind=c_char), intent(in) :: path(*), text(16), model1(16), &
+                                          package1(16), model2(16), package2(16)
+    integer(c_int), value :: plen, append, kstp, kper, ncol, nrow, nlay, nlist
+    integer(c_int), intent(in) :: nodes(nlist), nodes2(nlist)
+    real(c_double), intent(in) :: q(nlist)
+    real(c_double), value :: delt, pertim, totim
+    character(len=16), dimension(0) :: auxtxt
+    real(DP), dimension(1) :: aux
+    integer(I4B) :: iu, i
+    aux = 0.0_DP
+    iu = open_stream(path, plen, append)
+    call ubdsv06(kstp, kper, text16(text), text16(model1), text16(package1), &
+                 text16(model2), text16(package2), iu, 0, auxtxt, ncol, nrow, &
+                 nlay, nlist, 0, delt, pertim, totim)
+    do i = 1, nlist
+      call ubdsvd(iu, nodes(i), nodes2(i), q(i), 0, aux)
+    end do
+    close (iu)
+  end subroutine ref_ubdsv06_list
 
 end module mf6ref_shim

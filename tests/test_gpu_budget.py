@@ -56,10 +56,23 @@ def test_device_budget_matches_oracle(name):
     gs.formulate(2)
     _lib.check(gs.L.mf6hip_gwf_set_x(gs.h, _lib.dptr(np.ascontiguousarray(sim.x))), "set_x")
     db = gs.budget()
+    terms = {}
+    if case.has_sto:
+        terms["STO-SS"] = gs.flow_term("STO-SS")
+        terms["STO-SY"] = gs.flow_term("STO-SY")
+    for p in case.packages:
+        terms[p.kind] = gs.flow_term(p.kind)
     gs.destroy()
     fo, fd = ob["flowja"], db["flowja"]
     bad = np.nonzero(fo.view(np.int64) != fd.view(np.int64))[0]
     assert bad.size == 0, (bad[:10], fo[bad[:10]], fd[bad[:10]])
+    # the per-cell storage rates and per-bound package rates the budget file
+    # holds: bit-identical as well
+    want = dict(ob["simvals"])
+    want["STO-SS"], want["STO-SY"] = ob["strgss"], ob["strgsy"]
+    for k, v in terms.items():
+        assert v.shape == want[k].shape, k
+        assert np.array_equal(v.view(np.int64), want[k].view(np.int64)), k
     scale = max(max(v) for v in ob["terms"].values())
     for k, (rin, rout) in db["terms"].items():
         oin, oout = ob["terms"].get(k, (0.0, 0.0))
@@ -104,3 +117,41 @@ def test_device_run_budget_within_1e6_of_oracle_run(name):
         oin, oout = ob["terms"].get(k, (0.0, 0.0))
         assert abs(rin - oin) <= 1e-6 * scale and abs(rout - oout) <= 1e-6 * scale, \
             (k, rin, oin, rout, oout)
+
+
+def test_device_run_writes_head_and_budget_files(tmp_path):
+    """run() with a head and a budget file (OC SAVE HEAD ALL / SAVE BUDGET
+    ALL): one save per time step with the step's kstp, kper, pertim and
+    totim; the last save holds the final heads and the budget of them (the
+    arrays themselves are pinned to the oracle bit for bit above, the
+    record bytes to the reference's writers in test_binaryfile)."""
+    from modflow6_amd import binaryfile as B
+    case = _transient()
+    gs = GwfSolution(case)
+    with open(tmp_path / "m.hds", "wb") as fh, open(tmp_path / "m.cbc", "wb") as fb:
+        gs.run(head_file=fh, budget_file=fb)
+    hd = gs.heads()
+    db = gs.budget()
+    again = gs.budget_records()
+    gs.destroy()
+    perlen, nstp, _ = case.periods[0]
+    delt = perlen / nstp
+    hrec = B.read_heads(str(tmp_path / "m.hds"))
+    assert len(hrec) == nstp * case.nlay
+    for s in range(nstp):
+        for k in range(case.nlay):
+            kstp, kper, pertim, totim, text, ilay, a = hrec[s * case.nlay + k]
+            assert (kstp, kper, ilay, text.strip()) == (s + 1, 1, k + 1, "HEAD")
+            assert pertim == (perlen if s == nstp - 1 else delt * (s + 1))
+            assert totim == delt * (s + 1)
+    last = np.stack([r[6] for r in hrec[-case.nlay:]]).ravel()
+    assert np.array_equal(last, hd)
+    brec = B.read_budget(str(tmp_path / "m.cbc"))
+    per_save = 2 + 1 + len(case.packages)   # STO-SS, STO-SY, FLOW-JA-FACE, lists
+    assert len(brec) == nstp * per_save
+    fj = [r for r in brec if r["text"].strip() == "FLOW-JA-FACE"]
+    assert [r["kstp"] for r in fj] == list(range(1, nstp + 1))
+    assert np.array_equal(fj[-1]["data"], db["flowja"])
+    # the last save is the budget of the final heads, record for record
+    got = open(tmp_path / "m.cbc", "rb").read()
+    assert got[-len(again):] == again
