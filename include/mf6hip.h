@@ -71,7 +71,12 @@ typedef struct mf6hip_ims_settings {
                            vector pass */
   int preconditioner;   /* 0: ILU0 / MILU0 (IMS); 1: ILU0 followed by the
                            learned correction of an attached network, in
-                           flexible CG (CG only) */
+                           flexible CG (CG only); 2: two-level, ILU0 / MILU0
+                           plus an additive coarse correction on column
+                           aggregates (symmetric systems, CG, fused kernels,
+                           any precision) */
+  int coarse_size;      /* preconditioner 2: most coarse unknowns (aggregates
+                           of ha x ha columns, all layers); 0 = 4096 */
 } mf6hip_ims_settings;
 
 /* Per-inner-iteration record, the values ConvergenceSummaryType keeps for

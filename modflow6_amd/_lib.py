@@ -55,7 +55,8 @@ class ImsSettingsC(C.Structure):
                 ("inner_fp32_max", C.c_int),
                 ("inner_fp32_rtol", C.c_double),
                 ("kernel_variant", C.c_int),
-                ("preconditioner", C.c_int)]
+                ("preconditioner", C.c_int),
+                ("coarse_size", C.c_int)]
 
 
 class CnvgOutC(C.Structure):

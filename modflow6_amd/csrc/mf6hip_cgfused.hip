@@ -122,12 +122,13 @@ struct FState {
   double pmax;      // signed max |p| of the last direction
   int ploc;
   int reliable_due;
+  int restart;      // the next round starts a new direction (beta = 0)
 };
 
 }  // namespace
 
 // partial slots in part[] / mpart[] (each nblk long)
-enum { P_SS_R = 0, P_SS_B = 1, P_RHO_B = 2, P_RHO_R = 3, P_PQ = 4 };
+enum { P_SS_R = 0, P_SS_B = 1, P_RHO_B = 2, P_RHO_R = 3, P_PQ = 4, P_RHO_C = 5 };
 enum { M_D_R = 0, M_D_B = 1, M_P = 2 };
 
 // ---------------------------------------------------------------------------
@@ -294,13 +295,19 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
 // ---------------------------------------------------------------------------
 
 // p = z (first iteration) or z + beta p_prev ; x += alpha_prev p_prev ;
-// partial max |p| (signed, first occurrence) for deltax = alpha pmax
+// partial max |p| (signed, first occurrence) for deltax = alpha pmax.
+// With the two-level preconditioner (yc not null) z is first completed by
+// the prolongation of the coarse correction: z + yc[aggregate] on the
+// coupled cells (bit k of the column's cmask).
 template <typename TV, typename TX>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__restrict__ z,
                                                          const TV *__restrict__ pin,
                                                          TV *__restrict__ pout, TX *__restrict__ x,
                                                          MaxLoc *mpart, const KState *st,
-                                                         const FState *fs, int nblk) {
+                                                         const FState *fs, int nblk,
+                                                         const TV *__restrict__ yc,
+                                                         const unsigned *__restrict__ cmask,
+                                                         TwoLevel T) {
   const int lb = xcd_block();
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
@@ -316,14 +323,18 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
     if (h2 >= ncols_of(L, c)) continue;
     int l = col_l(L, c, h2);
     const int own = c * L.H + h2;
+    const unsigned cm = yc ? cmask[c * L.HP + h2] : 0u;
+    const TV yv = cm ? yc[tl_agg(T, L, l)] : (TV)0;
     for (int k = 0; k < L.nlay; k++) {
       int pos = own + k * L.HP;
+      TV zv = z[pos];
+      if ((cm >> k) & 1u) zv = zv + yv;
       TV pv;
       if (first) {
-        pv = z[pos];
+        pv = zv;
       } else {
         TV po = pin[pos];
-        pv = z[pos] + beta * po;
+        pv = zv + beta * po;
         x[pos] = (TX)(x[pos] + (TX)(ax * po));
       }
       pout[pos] = pv;
@@ -348,7 +359,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__
                                                           TV *__restrict__ pout,
                                                           TX *__restrict__ x, MaxLoc *mpart,
                                                           const KState *st, const FState *fs,
-                                                          int nblk) {
+                                                          int nblk, const TV *__restrict__ yc,
+                                                          const unsigned *__restrict__ cmask,
+                                                          TwoLevel T) {
   using V2 = typename Vec2<TV>::type;
   using X2 = typename Vec2<TX>::type;
   const int lb = xcd_block();
@@ -364,9 +377,15 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__
     const int h2 = (t >> 1) * 2;
     const int l0 = col_l(L, c, h2), l1 = col_l(L, c, h2 + 1);
     const int own = c * L.H + h2;
+    const unsigned cm0 = yc ? cmask[c * L.HP + h2] : 0u;
+    const unsigned cm1 = yc ? cmask[c * L.HP + h2 + 1] : 0u;
+    const TV y0 = cm0 ? yc[tl_agg(T, L, l0)] : (TV)0;
+    const TV y1 = cm1 ? yc[tl_agg(T, L, l1)] : (TV)0;
     for (int k = 0; k < L.nlay; k++) {
       const int pos = own + k * L.HP;
-      const V2 zv = *reinterpret_cast<const V2 *>(z + pos);
+      V2 zv = *reinterpret_cast<const V2 *>(z + pos);
+      if ((cm0 >> k) & 1u) zv.x = zv.x + y0;
+      if ((cm1 >> k) & 1u) zv.y = zv.y + y1;
       V2 pv;
       if (first) {
         pv = zv;
@@ -406,12 +425,14 @@ static inline int pair_grid(const Layout &L, int nb) {
 template <typename TV, typename TX>
 static void launch_f_pupdate(const Layout &L, int nb, hipStream_t s, const TV *z, const TV *pin,
                              TV *pout, TX *x, MaxLoc *mpart, const KState *st,
-                             const FState *fs) {
+                             const FState *fs, const TV *yc = nullptr,
+                             const unsigned *cmask = nullptr, TwoLevel T = TwoLevel{}) {
   if (spmv_pairable(L))
     k_f_pupdate2<TV, TX><<<pair_grid(L, nb), MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st,
-                                                               fs, nb);
+                                                               fs, nb, yc, cmask, T);
   else
-    k_f_pupdate<TV, TX><<<nb, MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st, fs, nb);
+    k_f_pupdate<TV, TX><<<nb, MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st, fs, nb, yc,
+                                                 cmask, T);
 }
 
 // q = A p (amux order, symmetric planes), partial p.q.  The layer walk is
@@ -680,7 +701,8 @@ __device__ inline void testcnvg_(int icnvgopt, int *icnvg, int iiter, double dvm
 __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
                                                         const double *part,
                                                         const MaxLoc *mpart, int nblk,
-                                                        int ncol, double *hist, int *hloc) {
+                                                        int ncol, double *hist, int *hloc,
+                                                        int ncoarse = 0) {
   __shared__ double lds[4][MF6_RBLOCK / 64];
   __shared__ MaxLoc mlds[2][MF6_RBLOCK / 64];
   // the state is read before the partials (and the stop flag is only acted
@@ -706,6 +728,8 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
     mm[0] = maxloc_pick(mm[0], mpart[M_D_R * nblk + i]);
     mm[1] = maxloc_pick(mm[1], mpart[M_D_B * nblk + i]);
   }
+  // two-level preconditioner: rho gains rc.yc (partials of the coarse GEMV)
+  for (int i = threadIdx.x; i < ncoarse; i += blockDim.x) a[3] = a[3] + part[P_RHO_C * nblk + i];
   const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -763,6 +787,10 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
   }
   st->rho = rho_new;
   if (it > 0) st->beta = rho_new / S.rho;   // rho0 was just set to the old rho
+  if (fs->restart) {
+    st->beta = 0.0;
+    fs->restart = 0;
+  }
 }
 
 __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
@@ -862,13 +890,16 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_r_residual(Layout L, const double
 }
 
 // after a replacement: no pending d / x update; a pending FP32 test result
-// is re-examined on the true residual
-__global__ void k_r_after(KState *st, FState *fs, const MaxLoc *mpart, int nblk, double *out) {
+// is re-examined on the true residual.  restart: the next direction is the
+// preconditioned true residual alone (beta = 0).
+__global__ void k_r_after(KState *st, FState *fs, const MaxLoc *mpart, int nblk, double *out,
+                          int restart) {
   __shared__ MaxLoc mlds[8];
   MaxLoc m = rmaxl(mpart, nblk, mlds);
   if (threadIdx.x != 0) return;
   fs->apend_d = 0.0;
   fs->apend_x = 0.0;
+  fs->restart = restart;
   out[0] = m.v;
   out[1] = (double)m.loc;
 }
@@ -914,13 +945,16 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
   KState *st = h->d_state;
   FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
   double *pbuf[2] = {h->p, h->p2};
+  const bool tl = h->s.preconditioner == 2;
   for (int round = 0; round < n; round++) {
     launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs, nbc,
                                       nb);
+    if (tl) mf6hip_tl_enqueue<double>(h, h->d, h->tl_yc64, h->d_part + P_RHO_C * nb, nb, st);
     k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                      h->d_hloc);
+                                      h->d_hloc, tl ? nb : 0);
     double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
-    launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs);
+    launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs,
+                                     tl ? h->tl_yc64 : nullptr, h->tl_cmask, h->tl);
     int slot = -1;
     if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
         2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
@@ -1011,16 +1045,22 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
   const int nb = nbs > nbc ? nbs : nbc;
   float *pbuf[2] = {h->p32, h->p232};
   double *rout = h->d_part + 7 * MF6_MAXBLK;
-  const int R = h->s.inner_fp32_max > 0 ? h->s.inner_fp32_max : 64;   // iterations between replacements
+  // iterations between replacements.  With the two-level preconditioner
+  // the direction restarts at every replacement (keeping it, as the
+  // one-level solve does, lets the FP32 recurrences stall against the
+  // coarse correction), so replacements come more often.
+  const int R = h->s.inner_fp32_max > 0 ? h->s.inner_fp32_max
+                                        : (h->s.preconditioner == 2 ? 32 : 64);
   HIPCHK(hipMemsetAsync(fs, 0, sizeof(FState), s));
   HIPCHK(hipMemsetAsync(h->q32, 0, sizeof(float) * L.NP, s));
   HIPCHK(hipMemsetAsync(h->e32, 0, sizeof(float) * L.NP, s));
   // true residual into d32 (r32)
   k_r_residual<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->r32, h->d_mpart);
-  k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout);
+  k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout, 0);
   int round = 0, last_replace = 0;
   int total_budget = h->s.iter1;
   bool done = false;
+  const bool tl = h->s.preconditioner == 2;
   *icnvg = 0;
   while (!done) {
     int batch = 8;
@@ -1028,10 +1068,13 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
       for (int it = 0; it < batch; it++, round++) {
         launch_pc<TMP, float, TP>(h, Apc, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs, nbc,
                                   nb, sc);
+        if (tl)
+          mf6hip_tl_enqueue<float>(h, h->r32, h->tl_yc32, h->d_part + P_RHO_C * nb, nb, st);
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                          h->d_hloc);
+                                          h->d_hloc, tl ? nb : 0);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
-        launch_f_pupdate<float, float>(L, nb, s, h->z32, pin, pout, h->e32, h->d_mpart, st, fs);
+        launch_f_pupdate<float, float>(L, nb, s, h->z32, pin, pout, h->e32, h->d_mpart, st, fs,
+                                       tl ? h->tl_yc32 : nullptr, h->tl_cmask, h->tl);
         int slot = -1;
         if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
             2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
@@ -1059,7 +1102,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
     round = m;
     k_r_accumulate<<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->x, h->e32);
     k_r_residual<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->r32, h->d_mpart);
-    k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout);
+    k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout, tl ? 1 : 0);
     double hr[2];
     HIPCHK(hipMemcpyAsync(hr, rout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

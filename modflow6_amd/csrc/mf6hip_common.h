@@ -136,6 +136,19 @@ struct KState {
 #define MF6_BLOCK 256
 #define MF6_MAXBLK 8192
 
+// Coarse space of the two-level preconditioner: aggregates of ha x ha grid
+// columns (all layers) on an nai x naj aggregate grid; nc = nai naj coarse
+// unknowns, dense rows of ld (a multiple of 64) entries.
+struct TwoLevel {
+  int ha, nai, naj, nc, ld;
+};
+
+// aggregate of the grid column l = i ncol + j
+__host__ __device__ inline int tl_agg(const TwoLevel &T, const Layout &L, int l) {
+  const int i = l / L.ncol, j = l - i * L.ncol;
+  return (i / T.ha) * T.naj + j / T.ha;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b mod 8), and each XCD has its own L2.  Stencil kernels read the
 // columns one grid row away (+-ncol/2 positions, a few workgroups away), so

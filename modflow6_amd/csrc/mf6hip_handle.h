@@ -135,8 +135,26 @@ struct Mf6Handle {
   // (used when the per-launch SpMV timing is off)
   hipGraphExec_t cg_graph = nullptr;
   bool cg_graph_failed = false;
+  // two-level preconditioner (preconditioner = 2): coarse shape, per-column
+  // coupled-cell masks, dense coarse matrix / Cholesky workspace G, the
+  // inverse (FP64 for the FP64 solve, FP32 for the mixed ones), restricted
+  // residual and coarse correction
+  TwoLevel tl{};
+  unsigned *tl_cmask = nullptr;
+  double *tl_G = nullptr, *tl_A64 = nullptr, *tl_rc = nullptr, *tl_yc64 = nullptr;
+  float *tl_A32 = nullptr, *tl_yc32 = nullptr;
+  int *tl_info = nullptr;
+  void *tl_blas = nullptr;    // rocblas_handle
   int spmv_calls = 0;
 };
+
+// two-level preconditioner (mf6hip_coarse.hip)
+int mf6hip_tl_setup(Mf6Handle *h);
+void mf6hip_tl_free(Mf6Handle *h);
+template <typename TV>
+void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslot,
+                       const KState *st);
+int mf6hip_tl_apply64(Mf6Handle *h, const double *r, double *z);
 
 // solver entry points used by the GWF part
 int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg,

@@ -1327,6 +1327,13 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
   CK(ilu_factor(h));
   int sym = 0;
   if (h->s.ilinmeth == 1 && h->s.north == 0) CK(mf6hip_is_symmetric(h, &sym));
+  if (h->s.preconditioner == 2) {
+    if (!sym) {
+      mf6hip_set_error("the two-level preconditioner needs a symmetric system (CG)");
+      return MF6HIP_EUNSUPPORTED;
+    }
+    CK(mf6hip_tl_setup(h));
+  }
   if (h->s.precision != MF6HIP_FP64) {
     k_convert<double, float><<<grid_for(MF6_NSLOT * L.NP), MF6_BLOCK, 0, s>>>(
         MF6_NSLOT * L.NP, h->A, h->A32);
@@ -1414,7 +1421,13 @@ static int check_settings(const mf6hip_ims_settings *s) {
   if (s->precision != MF6HIP_FP64 && s->ilinmeth != 1) { mf6hip_set_error("mixed precision is implemented for CG only"); return MF6HIP_EUNSUPPORTED; }
   if (s->ordering != MF6HIP_ORDER_RB) { mf6hip_set_error("unknown ordering"); return MF6HIP_EINVAL; }
   if (s->kernel_variant < 0 || s->kernel_variant > 1) { mf6hip_set_error("unknown kernel variant"); return MF6HIP_EINVAL; }
-  if (s->preconditioner < 0 || s->preconditioner > 1) { mf6hip_set_error("unknown preconditioner"); return MF6HIP_EINVAL; }
+  if (s->preconditioner < 0 || s->preconditioner > 2) { mf6hip_set_error("unknown preconditioner"); return MF6HIP_EINVAL; }
+  if (s->preconditioner == 2 && (s->ilinmeth != 1 || s->kernel_variant != 0 || s->north != 0 ||
+                                 (s->precision != MF6HIP_FP64 && s->icnvgopt != 0))) {
+    mf6hip_set_error("the two-level preconditioner runs in the fused CG (LINEAR_ACCELERATION CG, no orthogonalization; mixed precision with ICNVGOPT 0)");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  if (s->coarse_size < 0) { mf6hip_set_error("coarse_size must be >= 0"); return MF6HIP_EINVAL; }
   if (s->preconditioner == 1 && (s->ilinmeth != 1 || s->precision != MF6HIP_FP64)) {
     mf6hip_set_error("the learned preconditioner runs in FP64 flexible CG (LINEAR_ACCELERATION CG)");
     return MF6HIP_EUNSUPPORTED;
@@ -1608,6 +1621,7 @@ extern "C" int mf6hip_destroy(void *handle) {
   for (void *b : bufs)
     if (b) hipFree(b);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
+  mf6hip_tl_free(h);
   for (int t = 0; t < 5; t++) {
     DevPackage &pk = h->pk[t];
     void *pb[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs};
@@ -1644,7 +1658,9 @@ extern "C" int mf6hip_set_timing(void *handle, int enable, int sample) {
 
 // Debug / parity helpers: run single kernels on host vectors (natural order).
 // op 0: y = A x (SpMV of the last system), op 1: y = M^-1 x (ILU apply of the
-// last factor), op 2: y = inverse pivots (x ignored).
+// last factor), op 2: y = inverse pivots (x ignored), op 3: y = M^-1 x of the
+// two-level preconditioner (ILU0 plus the coarse correction; needs
+// preconditioner = 2 and a system loaded by mf6hip_load_factor).
 extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !y || (op != 2 && !x)) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
@@ -1659,7 +1675,14 @@ extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y)
   if (op == 0) launch_spmv<double, double>(h, h->A, tx, ty);
   else if (op == 1) launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, tx, h->zf, ty, nullptr);
   else if (op == 2) HIPCHK(hipMemcpyAsync(ty, h->invpiv, sizeof(double) * L.NP, hipMemcpyDeviceToDevice, s));
-  else { mf6hip_set_error("unknown op"); return MF6HIP_EINVAL; }
+  else if (op == 3) {
+    if (h->s.preconditioner != 2 || !h->tl_cmask) {
+      mf6hip_set_error("op 3 needs preconditioner = 2 and a loaded system");
+      return MF6HIP_EINVAL;
+    }
+    launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, tx, h->zf, ty, nullptr);
+    CK(mf6hip_tl_apply64(h, tx, ty));
+  } else { mf6hip_set_error("unknown op"); return MF6HIP_EINVAL; }
   k_pos_to_nat<double, double><<<g, MF6_BLOCK, 0, s>>>(L, ty, h->d_tmp_nat);
   HIPCHK(hipMemcpyAsync(y, h->d_tmp_nat, sizeof(double) * L.N, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1677,6 +1700,7 @@ extern "C" int mf6hip_load_factor(void *handle, const double *amat) {
   HIPCHK(hipMemcpyAsync(h->d_amat_csr, amat, sizeof(double) * h->nja, hipMemcpyHostToDevice, s));
   k_csr_to_planes<<<grid_for(L.N), MF6_BLOCK, 0, s>>>(L, h->d_ia, h->d_amat_csr, h->A);
   CK(ilu_factor(h));
+  if (h->s.preconditioner == 2) CK(mf6hip_tl_setup(h));
   HIPCHK(hipStreamSynchronize(s));
   return MF6HIP_OK;
 }

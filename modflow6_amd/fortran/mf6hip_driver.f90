@@ -51,6 +51,7 @@ program mf6hip_driver
   s%inner_fp32_rtol = 0.0_c_double
   s%kernel_variant = 0
   s%preconditioner = 0
+  s%coarse_size = 0
 
   rc = mf6hip_create(neq, nja, ia, ja, s, h)
   if (rc /= 0) call fail(rc)

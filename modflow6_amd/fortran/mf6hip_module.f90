@@ -35,6 +35,7 @@ module mf6hip_module
     real(c_double) :: inner_fp32_rtol
     integer(c_int) :: kernel_variant
     integer(c_int) :: preconditioner
+    integer(c_int) :: coarse_size
   end type mf6hip_ims_settings
 
   integer(c_int), parameter, public :: MF6HIP_PKG_WEL = 1

@@ -38,7 +38,9 @@ class ImsLinearSettings:
     inner_fp32_max: int = 0
     inner_fp32_rtol: float = 0.0
     kernel_variant: int = 0      # 0 fused CG (symmetric systems), 1 unfused
-    preconditioner: int = 0      # 0 ILU0/MILU0, 1 ILU0 + learned correction (FCG)
+    preconditioner: int = 0      # 0 ILU0/MILU0, 1 ILU0 + learned correction (FCG),
+                                 # 2 two-level (ILU0 + aggregation coarse correction)
+    coarse_size: int = 0         # preconditioner 2: most coarse unknowns (0 = 4096)
 
     def to_c(self, nlay, nrow, ncol):
         return _lib.ImsSettingsC(
@@ -46,7 +48,7 @@ class ImsLinearSettings:
             self.ilinmeth, self.iscl, self.iord, self.north, self.relax,
             self.level, self.droptol, self.precision, self.ordering, nlay,
             nrow, ncol, self.inner_fp32_max, self.inner_fp32_rtol,
-            self.kernel_variant, self.preconditioner)
+            self.kernel_variant, self.preconditioner, self.coarse_size)
 
 
 def dis_structure(nlay, nrow, ncol):

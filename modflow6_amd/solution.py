@@ -52,7 +52,8 @@ class GwfSolution:
     """One IMS solution holding one DIS GWF model (a GwfCase)."""
 
     def __init__(self, case, precision=_lib.MF6HIP_FP64, timing=False,
-                 inner_fp32_rtol=0.0, inner_fp32_max=0, kernel_variant=0):
+                 inner_fp32_rtol=0.0, inner_fp32_max=0, kernel_variant=0,
+                 preconditioner=0, coarse_size=0):
         self.case = c = case
         L = _lib.load()
         self.L = L
@@ -63,7 +64,8 @@ class GwfSolution:
             ilinmeth=c.ims.ilinmeth, north=c.ims.number_orthogonalizations,
             relax=c.ims.relaxation_factor, precision=precision,
             inner_fp32_rtol=inner_fp32_rtol, inner_fp32_max=inner_fp32_max,
-            kernel_variant=kernel_variant)
+            kernel_variant=kernel_variant, preconditioner=preconditioner,
+            coarse_size=coarse_size)
         self.solver = HipLinearSolver().initialize(ia, ja, self.settings,
                                                    c.nlay, c.nrow, c.ncol)
         self.h = self.solver.handle
