@@ -3,9 +3,11 @@
 steady confined GWF model (10 x 1000 x 1000 DIS, exponential-covariance
 ln K with sigma 2 / correlation 20 cells, CHD + RCH + 2000 wells), FP64
 ILU0-PCG vs the mixed-precision solves (FP32 ILU0 factor, and the FP16
-factor variant), on one MI355X; the headline value is the faster mixed
-solve that meets the accuracy bar (FP64 true residual <= 1e-6, heads within
-1e-5 m of the oracle's sample).
+factor variant), each also with the two-level preconditioner (ILU0 plus an
+aggregation coarse correction; methods *_2l), on one MI355X; the headline
+value is the fastest mixed solve that meets the accuracy bar (FP64 true
+residual <= 1e-6, heads within 1e-5 m of the oracle's sample), and the
+speedups are against FP64 ILU0-PCG.
 
 A step = one NumericalSolution outer iteration of the hot path from the
 start heads: device formulate (npf/sto/bnd fill) + sln_ls (row fixes, ILU0
@@ -36,11 +38,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_precision(case, precision, steps, warmup, label, replace=0, timing=True):
+def run_precision(case, precision, steps, warmup, label, replace=0, timing=True,
+                  preconditioner=0, coarse_size=0):
     from modflow6_amd import _lib
     from modflow6_amd.solution import GwfSolution
     gs = GwfSolution(case, precision=precision, timing=timing,
-                     inner_fp32_max=replace)
+                     inner_fp32_max=replace, preconditioner=preconditioner,
+                     coarse_size=coarse_size)
     x0 = np.ascontiguousarray(case.strt, dtype=np.float64)
     out = []
     for s in range(warmup + steps):
@@ -245,8 +249,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="both",
-                    choices=["both", "fp64", "mixed", "mixed16"],
-                    help="both: FP64, mixed (FP32 factor) and mixed16 (FP16 factor)")
+                    help="comma list of fp64, mixed (FP32 factor), mixed16 (FP16 "
+                         "factor), fp64_2l, mixed_2l, mixed16_2l (two-level); both: "
+                         "fp64, mixed, mixed16, fp64_2l, mixed16_2l")
+    ap.add_argument("--coarse-size", type=int, default=0,
+                    help="two-level: most coarse unknowns (0 = 4096)")
     ap.add_argument("--shape", default="10,1000,1000")
     ap.add_argument("--case", default="10m", choices=["10m", "small"],
                     help="10m: the headline model (--shape); small: the 3 x 100 x 100 "
@@ -299,18 +306,23 @@ def main():
     N = case.nodes
     nja = N + 2 * (nlay * (nrow * (ncol - 1) + (nrow - 1) * ncol) + (nlay - 1) * nrow * ncol)
 
+    methods = (["fp64", "mixed", "mixed16", "fp64_2l", "mixed16_2l"]
+               if args.precision == "both" else args.precision.split(","))
+    prec = {"fp64": _lib.MF6HIP_FP64, "mixed": _lib.MF6HIP_MIXED,
+            "mixed16": _lib.MF6HIP_MIXED_FP16}
     res = {}
-    if args.precision in ("both", "fp64"):
-        res["fp64"] = run_precision(case, _lib.MF6HIP_FP64, args.steps, args.warmup, "fp64",
-                                    timing=not args.no_timing)
-    if args.precision in ("both", "mixed"):
-        res["mixed"] = run_precision(case, _lib.MF6HIP_MIXED, args.steps, args.warmup, "mixed",
-                                     replace=args.replace)
-    if args.precision in ("both", "mixed16"):
-        # the FP16 factor variant: ILU0 planes in FP16 (power-of-two scaled),
-        # SpMV and Krylov vectors FP32, FP64 residual replacement
-        res["mixed16"] = run_precision(case, _lib.MF6HIP_MIXED_FP16, args.steps, args.warmup,
-                                       "mixed16", replace=args.replace)
+    for m in methods:
+        base = m[:-3] if m.endswith("_2l") else m
+        if base not in prec:
+            ap.error(f"unknown method {m}")
+        # mixed16: ILU0 planes in FP16 (power-of-two scaled), SpMV and Krylov
+        # vectors FP32, FP64 residual replacement; *_2l: plus the coarse
+        # correction (preconditioner 2)
+        res[m] = run_precision(case, prec[base], args.steps, args.warmup, m,
+                               replace=args.replace,
+                               timing=not args.no_timing if base == "fp64" else True,
+                               preconditioner=2 if m.endswith("_2l") else 0,
+                               coarse_size=args.coarse_size)
 
     def mean(lst, key):
         return float(np.mean([r[key] for r in lst]))
@@ -323,7 +335,7 @@ def main():
                           spmv_us=mean(recs, "spmv_ms") * 1e3,
                           true_residual_inf=true_residual(case, amat, rhs, heads),
                           converged=bool(recs[-1]["icnvg"]))
-    for mk in ("mixed", "mixed16"):
+    for mk in ("mixed", "mixed16", "fp64_2l", "mixed_2l", "mixed16_2l"):
         if "fp64" in res and mk in res:
             dh = float(np.max(np.abs(res[mk][1] - res["fp64"][1])))
             summary[mk]["max_head_diff_vs_fp64_m"] = dh
@@ -348,9 +360,11 @@ def main():
         m = summary[k]
         return (m["converged"] and m["true_residual_inf"] <= 1e-6 and
                 m.get("max_head_diff_vs_oracle_m", 0.0) <= 1e-5)
-    mixed = [k for k in ("mixed", "mixed16") if k in summary and valid(k)]
+    mixed = [k for k in ("mixed", "mixed16", "mixed_2l", "mixed16_2l")
+             if k in summary and valid(k)]
     head = (min(mixed, key=lambda k: summary[k]["s_per_solve"]) if mixed else
-            next(k for k in ("mixed", "mixed16", "fp64") if k in summary))
+            next(k for k in ("mixed", "mixed16", "mixed_2l", "mixed16_2l", "fp64",
+                             "fp64_2l") if k in summary))
     value = summary[head]["s_per_solve"]
 
     # roofline of the dominant kernel, k_f_spmv (CG SpMV q = A p): symmetric
@@ -359,11 +373,11 @@ def main():
     njas = (nja - N) // 2
     fp64_spmv_bytes = 8 * (N + njas) + 2 * 8 * N
     fp32_spmv_bytes = 4 * (N + njas) + 2 * 4 * N
-    f32 = head in ("mixed", "mixed16")
+    f32 = not head.startswith("fp64")
     if f32:
         sp_bytes, sp_us = fp32_spmv_bytes, summary[head]["spmv_us"]
     else:
-        sp_bytes, sp_us = fp64_spmv_bytes, summary["fp64"]["spmv_us"]
+        sp_bytes, sp_us = fp64_spmv_bytes, summary[head]["spmv_us"]
     achieved = sp_bytes / (sp_us * 1e-6) / 1e9 if sp_us > 0 else None
     # CSR-equivalent bytes of the same product (values + int32 column
     # indices + row pointers + x + y), FP64 and FP32 value forms
@@ -410,7 +424,8 @@ def main():
         "dtype": "f32-in-f64" if f32 else "f64",
         "data": "synthetic (seeded lognormal K, CHD/RCH/WEL)",
         "config": {"workload": f"steady confined GWF {nlay}x{nrow}x{ncol} DIS "
-                               f"({N} cells, {nja} nonzeros), CG + column red-black ILU0, "
+                               f"({N} cells, {nja} nonzeros), CG + column red-black ILU0"
+                               f"{' + aggregation coarse correction' if head.endswith('_2l') else ''}, "
                                f"rclose = dvclose = 1e-6",
                    "solver": head},
         "methods": summary,
@@ -422,6 +437,10 @@ def main():
     if "fp64" in summary and "mixed16" in summary:
         line["speedup_mixed16_vs_fp64"] = (summary["fp64"]["s_per_solve"] /
                                            summary["mixed16"]["s_per_solve"])
+    for mk in ("fp64_2l", "mixed_2l", "mixed16_2l"):
+        if "fp64" in summary and mk in summary:
+            line[f"speedup_{mk}_vs_fp64"] = (summary["fp64"]["s_per_solve"] /
+                                             summary[mk]["s_per_solve"])
     if "fp64" in summary and head != "fp64":
         line["speedup_headline_vs_fp64"] = summary["fp64"]["s_per_solve"] / value
     print(json.dumps(line))
