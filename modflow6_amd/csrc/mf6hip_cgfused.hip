@@ -193,7 +193,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
                                                        const TV *__restrict__ zf,
                                                        TV *__restrict__ z, double *part,
                                                        MaxLoc *mpart, const KState *st,
-                                                       const FState *fs, int nblk, TV sc) {
+                                                       const FState *fs, int nblk, TV sc,
+                                                       const unsigned *__restrict__ cmask,
+                                                       double *__restrict__ colsum) {
   const int lb = xcd_block();
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
@@ -211,12 +213,15 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
     int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
     int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
     TV zup = (TV)0, lup = (TV)0;
+    const unsigned cm = colsum ? cmask[L.HP + h2] : 0u;
+    double cs = 0.0;
     for (int k = 0; k < L.nlay; k++) {
       int pos = L.H + k * L.HP + h2;
       int rb = k * L.HP;
       TV dv = d[pos];
       dv = dv - alpha * q[pos];
       d[pos] = dv;
+      if ((cm >> k) & 1u) cs = cs + (double)dv;
       md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
       ss = ss + (double)dv * (double)dv;
       TV tv = dv;
@@ -231,6 +236,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
       zup = tv;
       lup = (TV)((TV)A[6 * NP + pos] * (TV)ldp(piv, pos));
     }
+    if (colsum) colsum[l] = cs;
     TV zdn = (TV)0;
     for (int k = L.nlay - 1; k >= 0; k--) {
       int pos = L.H + k * L.HP + h2;
@@ -256,7 +262,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
                                                          const TV *__restrict__ d,
                                                          const TV *__restrict__ zf,
                                                          TV *__restrict__ z, double *part,
-                                                         const KState *st, int nblk, TV sc) {
+                                                         const KState *st, int nblk, TV sc,
+                                                         const unsigned *__restrict__ cmask,
+                                                         double *__restrict__ colsum) {
   const int lb = xcd_block();
   __shared__ double lds[8];
   if (st->stop) return;
@@ -271,6 +279,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
     int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
     int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
     TV zdn = (TV)0;
+    const unsigned cm = colsum ? cmask[h2] : 0u;
+    double cs = 0.0;
     for (int k = L.nlay - 1; k >= 0; k--) {
       int pos = k * L.HP + h2;
       int bb = L.H + k * L.HP;
@@ -283,8 +293,11 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
       tv = tv * (TV)ldp(piv, pos);
       z[pos] = tv;
       zdn = tv;
-      rho = rho + (double)d[pos] * (double)tv;
+      const TV dv = d[pos];
+      rho = rho + (double)dv * (double)tv;
+      if ((cm >> k) & 1u) cs = cs + (double)dv;
     }
+    if (colsum) colsum[l] = cs;
   }
   rho = bsum(rho, lds);
   if (threadIdx.x == 0) part[P_RHO_R * nblk + lb] = rho;
@@ -918,16 +931,23 @@ static inline int gridc(int n) {
 // colour); their partials use the slot stride nb of the interleaved kernels.
 // A grid sized for both colours would leave the XCDs that receive the upper
 // half of the logical blocks idle.
+// With the two-level preconditioner the black and red backward kernels also
+// write each column's sum of the residual over its coupled cells (colsum,
+// natural column order), the first stage of the restriction.
 template <typename TM, typename TV, typename TP>
 static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV *q, TV *zf,
                       TV *z, KState *st, FState *fs, int nbc, int nb, TV sc = (TV)1) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
+  const bool tl = h->s.preconditioner == 2;
+  const unsigned *cm = tl ? h->tl_cmask : nullptr;
+  double *cs = tl ? h->tl_colsum : nullptr;
   k_f_red_fwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
                                                     st, fs, nb);
   k_f_black<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
-                                                  st, fs, nb, sc);
-  k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb, sc);
+                                                  st, fs, nb, sc, cm, cs);
+  k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb, sc,
+                                                    cm, cs);
 }
 
 static int read_states(Mf6Handle *h) {
@@ -949,7 +969,9 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
   for (int round = 0; round < n; round++) {
     launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs, nbc,
                                       nb);
-    if (tl) mf6hip_tl_enqueue<double>(h, h->d, h->tl_yc64, h->d_part + P_RHO_C * nb, nb, st);
+    if (tl)
+      mf6hip_tl_enqueue<double>(h, (const double *)nullptr, h->tl_yc64, h->d_part + P_RHO_C * nb,
+                                nb, st);
     k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                       h->d_hloc, tl ? nb : 0);
     double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
@@ -1069,7 +1091,8 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
         launch_pc<TMP, float, TP>(h, Apc, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs, nbc,
                                   nb, sc);
         if (tl)
-          mf6hip_tl_enqueue<float>(h, h->r32, h->tl_yc32, h->d_part + P_RHO_C * nb, nb, st);
+          mf6hip_tl_enqueue<float>(h, (const float *)nullptr, h->tl_yc32,
+                                   h->d_part + P_RHO_C * nb, nb, st);
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
                                           h->d_hloc, tl ? nb : 0);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];

@@ -20,7 +20,9 @@
 //   rocSOLVER     dpotrf + dpotri: (-Ac)^-1, FP64
 //   k_tl_finish   Ainv = -(-Ac)^-1 as a full row-major matrix, FP64 and FP32
 // Per CG iteration (after the preconditioner kernels, before the round):
-//   k_tl_restrict one block per aggregate: rc = P^T d
+//   the black / red backward preconditioner kernels write each column's sum
+//   of the residual over its coupled cells (colsum)
+//   k_tl_gather   one block per aggregate: rc = P^T d from colsum
 //   k_tl_gemv     one wavefront per coarse row: yc = Ainv rc, partial rc.yc
 // and the direction update adds the prolongation: p = z + P yc + beta p.
 // rho = d.(z + P yc) = d.z + rc.yc, so the round kernel adds the partials of
@@ -172,33 +174,74 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_tl_restrict(Layout L, const TV *_
   }
 }
 
-// yc = Ainv rc, one wavefront per row (4-wide loads; ld is a multiple of 4);
-// part[b] = this block's sum of rc[I] yc[I]; slots gridDim.x .. nslot-1
-// are cleared
+// rc[I] = sum of colsum over the columns of aggregate I (fixed order)
+__global__ void __launch_bounds__(MF6_BLOCK) k_tl_gather(Layout L, const double *__restrict__ colsum,
+                                                        TwoLevel T, double *__restrict__ rc,
+                                                        const KState *st) {
+  __shared__ double lds[MF6_BLOCK / 64];
+  if (st && st->stop) return;
+  for (int I = blockIdx.x; I < T.nc; I += gridDim.x) {
+    const int ai = I / T.naj, aj = I - ai * T.naj;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < T.ha * T.ha; t += blockDim.x) {
+      const int i = ai * T.ha + t / T.ha, j = aj * T.ha + t % T.ha;
+      if (i < L.nrow && j < L.ncol) s = s + colsum[i * L.ncol + j];
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) rc[I] = s;
+  }
+}
+
+template <typename TA> struct Row4;
+template <> struct Row4<float> {
+  float4 v;
+  __device__ static Row4 load(const float *p) { return Row4{*reinterpret_cast<const float4 *>(p)}; }
+  __device__ double dot(const double *x) const {
+    return (double)v.x * x[0] + (double)v.y * x[1] + (double)v.z * x[2] + (double)v.w * x[3];
+  }
+};
+template <> struct Row4<double> {
+  double2 a, b;
+  __device__ static Row4 load(const double *p) {
+    return Row4{reinterpret_cast<const double2 *>(p)[0], reinterpret_cast<const double2 *>(p)[1]};
+  }
+  __device__ double dot(const double *x) const {
+    return a.x * x[0] + a.y * x[1] + b.x * x[2] + b.y * x[3];
+  }
+};
+
+// yc = Ainv rc, one wavefront per row with 16-byte (FP32) / 32-byte (FP64)
+// row loads; rc is staged in LDS (ld doubles, zero beyond nc; the row
+// entries beyond nc are zero as well).  part[b] = this block's sum of
+// rc[I] yc[I]; slots gridDim.x .. nslot-1 are cleared.
 template <typename TA, typename TV>
 __global__ void __launch_bounds__(MF6_BLOCK) k_tl_gemv(TwoLevel T, const TA *__restrict__ Ainv,
                                                       const double *__restrict__ rc,
                                                       TV *__restrict__ yc, double *part,
                                                       int nslot, const KState *st) {
+  extern __shared__ double xs[];
   __shared__ double lds[MF6_BLOCK / 64];
   if (st && st->stop) return;
+  for (int c = threadIdx.x; c < T.ld; c += blockDim.x) xs[c] = c < T.nc ? rc[c] : 0.0;
+  __syncthreads();
   const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   double acc = 0.0;
   for (int I = blockIdx.x * wpb + w; I < T.nc; I += gridDim.x * wpb) {
     const TA *row = Ainv + (size_t)I * T.ld;
-    double s = 0.0;
-    for (int c = 4 * ln; c < T.nc; c += 256) {
-      const TA a0 = row[c], a1 = row[c + 1], a2 = row[c + 2], a3 = row[c + 3];
-      s = s + (double)a0 * rc[c];
-      if (c + 1 < T.nc) s = s + (double)a1 * rc[c + 1];
-      if (c + 2 < T.nc) s = s + (double)a2 * rc[c + 2];
-      if (c + 3 < T.nc) s = s + (double)a3 * rc[c + 3];
+    double s0 = 0.0, s1 = 0.0;
+    int c = 4 * ln;
+    for (; c + 256 < T.ld; c += 512) {
+      const Row4<TA> r0 = Row4<TA>::load(row + c);
+      const Row4<TA> r1 = Row4<TA>::load(row + c + 256);
+      s0 = s0 + r0.dot(xs + c);
+      s1 = s1 + r1.dot(xs + c + 256);
     }
-    s = wsum64(s);
+    if (c < T.ld) s0 = s0 + Row4<TA>::load(row + c).dot(xs + c);
+    const double sr = wsum64(s0 + s1);
     if (ln == 0) {
-      yc[I] = (TV)s;
-      acc = acc + rc[I] * (double)(TV)s;
+      yc[I] = (TV)sr;
+      acc = acc + xs[I] * (double)(TV)sr;
     }
   }
   acc = block_sum(acc, lds);
@@ -263,7 +306,8 @@ int mf6hip_tl_alloc(Mf6Handle *h) {
       hipMalloc((void **)&h->tl_rc, sizeof(double) * h->tl.ld) != hipSuccess ||
       hipMalloc((void **)&h->tl_yc64, sizeof(double) * h->tl.ld) != hipSuccess ||
       hipMalloc((void **)&h->tl_yc32, sizeof(float) * h->tl.ld) != hipSuccess ||
-      hipMalloc((void **)&h->tl_info, sizeof(int)) != hipSuccess) {
+      hipMalloc((void **)&h->tl_info, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&h->tl_colsum, sizeof(double) * (size_t)L.NRC) != hipSuccess) {
     mf6hip_set_error("hipMalloc failed (two-level preconditioner)");
     return MF6HIP_EHIP;
   }
@@ -293,13 +337,14 @@ void mf6hip_tl_free(Mf6Handle *h) {
   if (h->tl_blas) rocblas_destroy_handle((rocblas_handle)h->tl_blas);
   h->tl_blas = nullptr;
   void *bufs[] = {h->tl_cmask, h->tl_G, h->tl_rc, h->tl_yc64, h->tl_yc32, h->tl_info,
-                  h->tl_A64, h->tl_A32};
+                  h->tl_A64, h->tl_A32, h->tl_colsum};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tl_cmask = nullptr;
   h->tl_G = h->tl_A64 = h->tl_rc = h->tl_yc64 = nullptr;
   h->tl_A32 = h->tl_yc32 = nullptr;
   h->tl_info = nullptr;
+  h->tl_colsum = nullptr;
 }
 
 // Coarse operator of the current FP64 planes: mask, -Ac, its inverse.
@@ -337,21 +382,28 @@ int mf6hip_tl_setup(Mf6Handle *h) {
   return MF6HIP_OK;
 }
 
-// rc = P^T d ; yc = Ainv rc ; partial rc.yc into part[0 .. nslot)
+// rc = P^T d (from the preconditioner kernels' column sums when d is null,
+// else from d itself) ; yc = Ainv rc ; partial rc.yc into part[0 .. nslot)
 template <typename TV>
 void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslot,
                        const KState *st) {
   const TwoLevel &T = h->tl;
   hipStream_t s = h->stream;
   int gr = T.nc < 4096 ? T.nc : 4096;
-  k_tl_restrict<TV><<<gr, MF6_BLOCK, 0, s>>>(h->L, d, h->tl_cmask, T, h->tl_rc, st);
+  if (d)
+    k_tl_restrict<TV><<<gr, MF6_BLOCK, 0, s>>>(h->L, d, h->tl_cmask, T, h->tl_rc, st);
+  else
+    k_tl_gather<<<gr, MF6_BLOCK, 0, s>>>(h->L, h->tl_colsum, T, h->tl_rc, st);
   int gg = (T.nc + 3) / 4;
   if (gg > nslot) gg = nslot;
   if (gg < 1) gg = 1;
+  const size_t shm = sizeof(double) * (size_t)T.ld;
   if (sizeof(TV) == sizeof(double))
-    k_tl_gemv<double, TV><<<gg, MF6_BLOCK, 0, s>>>(T, h->tl_A64, h->tl_rc, yc, part, nslot, st);
+    k_tl_gemv<double, TV><<<gg, MF6_BLOCK, shm, s>>>(T, h->tl_A64, h->tl_rc, yc, part, nslot,
+                                                     st);
   else
-    k_tl_gemv<float, TV><<<gg, MF6_BLOCK, 0, s>>>(T, h->tl_A32, h->tl_rc, yc, part, nslot, st);
+    k_tl_gemv<float, TV><<<gg, MF6_BLOCK, shm, s>>>(T, h->tl_A32, h->tl_rc, yc, part, nslot,
+                                                    st);
 }
 
 template void mf6hip_tl_enqueue<double>(Mf6Handle *, const double *, double *, double *, int,

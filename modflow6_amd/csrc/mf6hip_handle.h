@@ -144,6 +144,7 @@ struct Mf6Handle {
   double *tl_G = nullptr, *tl_A64 = nullptr, *tl_rc = nullptr, *tl_yc64 = nullptr;
   float *tl_A32 = nullptr, *tl_yc32 = nullptr;
   int *tl_info = nullptr;
+  double *tl_colsum = nullptr;   // per grid column: residual summed over coupled cells
   void *tl_blas = nullptr;    // rocblas_handle
   int spmv_calls = 0;
 };

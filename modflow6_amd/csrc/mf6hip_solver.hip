@@ -1427,7 +1427,7 @@ static int check_settings(const mf6hip_ims_settings *s) {
     mf6hip_set_error("the two-level preconditioner runs in the fused CG (LINEAR_ACCELERATION CG, no orthogonalization; mixed precision with ICNVGOPT 0)");
     return MF6HIP_EUNSUPPORTED;
   }
-  if (s->coarse_size < 0) { mf6hip_set_error("coarse_size must be >= 0"); return MF6HIP_EINVAL; }
+  if (s->coarse_size < 0 || s->coarse_size > 6144) { mf6hip_set_error("coarse_size must be in [0, 6144] (the coarse GEMV stages the coarse vector in LDS)"); return MF6HIP_EINVAL; }
   if (s->preconditioner == 1 && (s->ilinmeth != 1 || s->precision != MF6HIP_FP64)) {
     mf6hip_set_error("the learned preconditioner runs in FP64 flexible CG (LINEAR_ACCELERATION CG)");
     return MF6HIP_EUNSUPPORTED;
