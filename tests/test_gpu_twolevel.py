@@ -124,3 +124,23 @@ def test_twolevel_rejects_unsymmetric_paths():
     case = configs.steady_small(nlay=2, nrow=6, ncol=8, nwel=2)
     with pytest.raises(RuntimeError):
         device_solver(case, preconditioner=2, kernel_variant=1)
+
+
+@pytest.mark.parametrize("precision", [_lib.MF6HIP_FP64, _lib.MF6HIP_MIXED_FP16])
+def test_twolevel_outer_loop_matches_oracle(precision):
+    """Whole steady simulation on the device-resident model (formulate, row
+    fixes, two-level solve, outer updates each outer iteration) against the
+    oracle's run of the reference algorithm."""
+    from modflow6_amd.solution import GwfSolution
+    case = configs.steady_10m(nlay=3, nrow=48, ncol=50, nwel=6)
+    ho = O.GwfSimulation(case).run()
+    gs = GwfSolution(case, precision=precision, preconditioner=2, coarse_size=100)
+    try:
+        hd = gs.run()
+        log = gs.log[0]
+    finally:
+        gs.destroy()
+    err = np.max(np.abs(hd - ho))
+    print("outer/inner", [(r.kiter, r.inner) for r in log], "max |h - h_oracle|", err)
+    assert log[-1].converged
+    assert err < 10 * case.ims.inner_dvclose
