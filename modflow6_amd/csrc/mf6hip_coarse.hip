@@ -14,11 +14,16 @@
 // fixed linear operator: plain PCG applies.
 //
 // Setup (once per linear solve, the matrix changes every outer iteration):
-//   k_tl_cmask    per column: bit k set when cell (k, column) is coupled
-//   k_tl_build    one block per aggregate: its row of -Ac (five nonzeros on
-//                 the aggregate grid) into a dense symmetric matrix
-//   rocSOLVER     dpotrf + dpotri: (-Ac)^-1, FP64
-//   k_tl_finish   Ainv = -(-Ac)^-1 as a full row-major matrix, FP64 and FP32
+//   k_tl_cmask     per column: bit k set when cell (k, column) is coupled
+//   k_tl_build     one block per aggregate: its row of S = -Ac (diagonal,
+//                  west and north couplings on the aggregate grid)
+//   k_tl_band_chol S = L L^T in one workgroup: S is banded (bandwidth naj,
+//                  the aggregate grid's width), so L is too; the trailing
+//                  window of the right-looking factorisation lives in LDS
+//   k_tl_band_inv  one wavefront per column j of S^-1: banded forward and
+//                  backward substitution of e_j, the band held across lanes
+//   k_tl_finish_rows  Ainv = -S^-1, FP64 or FP32
+// (aggregate grids wider than 63 take rocSOLVER's dense dpotrf + dpotri)
 // Per CG iteration (after the preconditioner kernels, before the round):
 //   the black / red backward preconditioner kernels write each column's sum
 //   of the residual over its coupled cells (colsum)
@@ -82,13 +87,14 @@ __global__ void k_tl_cmask(Layout L, const double *__restrict__ A, unsigned *__r
   }
 }
 
-// Row I of -Ac into the dense symmetric matrix G (leading dimension ld): the
-// diagonal (cell diagonals plus the couplings inside the aggregate) and the
-// couplings to the four neighbouring aggregates.  An aggregate without a
-// coupled cell gets a unit row (its restricted residual is always zero).
+// Row I of S = -Ac, compact: S3[3 I] = S(I, I) (cell diagonals plus the
+// couplings inside the aggregate), S3[3 I + 1] = S(I, I - 1) (west),
+// S3[3 I + 2] = S(I, I - naj) (north); zero where the neighbour does not
+// exist.  An aggregate without a coupled cell gets a unit row (its
+// restricted residual is always zero).
 __global__ void __launch_bounds__(MF6_BLOCK) k_tl_build(Layout L, const double *__restrict__ A,
                                                        const unsigned *__restrict__ cmask,
-                                                       TwoLevel T, double *__restrict__ G) {
+                                                       TwoLevel T, double *__restrict__ S3) {
   __shared__ double lds[MF6_BLOCK / 64];
   const int I = blockIdx.x;
   if (I >= T.nc) return;
@@ -122,16 +128,168 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_tl_build(Layout L, const double *
   ss = block_sum(ss, lds);
   cnt = block_sum(cnt, lds);
   if (threadIdx.x != 0) return;
+  (void)se;
+  (void)ss;
+  S3[3 * I] = cnt == 0.0 ? 1.0 : -sd;
+  S3[3 * I + 1] = (cnt != 0.0 && aj > 0) ? -sw : 0.0;
+  S3[3 * I + 2] = (cnt != 0.0 && ai > 0) ? -sn : 0.0;
+}
+
+// lower-band entry S(i, l), l <= i, from the compact rows
+__device__ inline double s_entry(const TwoLevel &T, const double *S3, int i, int l) {
+  double v = 0.0;
+  if (l == i) v = S3[3 * i];
+  if (l == i - 1) v = v + S3[3 * i + 1];
+  if (l == i - T.naj) v = v + S3[3 * i + 2];
+  return v;
+}
+
+// dense symmetric S (row-major, leading dimension ld; zeroed beforehand),
+// for rocSOLVER
+__global__ void k_tl_expand(TwoLevel T, const double *__restrict__ S3, double *__restrict__ G) {
   const size_t ld = (size_t)T.ld;
-  if (cnt == 0.0) {
-    G[(size_t)I * ld + I] = 1.0;
-    return;
+  for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < T.nc; I += gridDim.x * blockDim.x) {
+    G[(size_t)I * ld + I] = S3[3 * I];
+    if (I % T.naj > 0) {
+      G[(size_t)I * ld + I - 1] = S3[3 * I + 1];
+      G[(size_t)(I - 1) * ld + I] = S3[3 * I + 1];
+    }
+    if (I >= T.naj) {
+      G[(size_t)I * ld + I - T.naj] = S3[3 * I + 2];
+      G[(size_t)(I - T.naj) * ld + I] = S3[3 * I + 2];
+    }
   }
-  G[(size_t)I * ld + I] = -sd;
-  if (aj > 0) G[(size_t)I * ld + (I - 1)] = -sw;
-  if (aj < T.naj - 1) G[(size_t)I * ld + (I + 1)] = -se;
-  if (ai > 0) G[(size_t)I * ld + (I - T.naj)] = -sn;
-  if (ai < T.nai - 1) G[(size_t)I * ld + (I + T.naj)] = -ss;
+}
+
+// S = L L^T for S banded with bandwidth b = naj <= 63, in one workgroup.
+// Right-looking: step j takes the pivot, scales column j (rows j+1 .. j+b)
+// and subtracts its outer product from the trailing (b+1) x (b+1) window,
+// which lives in LDS with rows and columns taken modulo 64; the row j+b+1
+// enters the window from S as row j leaves it.  Lc[j (b+1) + t] = L(j+t, j).
+// info[0] = 1 + the first column whose pivot is not positive, 0 if none.
+constexpr int TL_CHOL_THREADS = 512;
+__global__ void __launch_bounds__(TL_CHOL_THREADS) k_tl_band_chol(TwoLevel T,
+                                                                  const double *__restrict__ S3,
+                                                                  double *__restrict__ Lc,
+                                                                  int *info) {
+  __shared__ double W[64][65];
+  __shared__ double col[64];
+  __shared__ int bad;
+  const int n = T.nc, b = T.naj, B1 = b + 1;
+  for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) W[e >> 6][e & 63] = 0.0;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < B1 * B1; e += blockDim.x) {
+    const int i = e / B1, l = e - i * B1;
+    if (i < n && l <= i) W[i & 63][l & 63] = s_entry(T, S3, i, l);
+  }
+  // this thread's (t1, t2) pairs of the update, 1 <= t2 <= t1 <= b
+  constexpr int MAXP = (63 * 64 / 2 + TL_CHOL_THREADS - 1) / TL_CHOL_THREADS;
+  int pt1[MAXP], pt2[MAXP];
+  const int npairs = b * B1 / 2;
+#pragma unroll
+  for (int q = 0; q < MAXP; q++) {
+    const int p = (int)threadIdx.x + q * TL_CHOL_THREADS;
+    int t1 = 0, t2 = 0;
+    if (p < npairs) {
+      t1 = (int)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+      while (t1 * (t1 - 1) / 2 > p) t1--;
+      while ((t1 + 1) * t1 / 2 <= p) t1++;
+      t2 = p - t1 * (t1 - 1) / 2 + 1;
+    }
+    pt1[q] = t1;
+    pt2[q] = t2;
+  }
+  __syncthreads();
+  for (int j = 0; j < n; j++) {
+    if (threadIdx.x == 0) {
+      double w = W[j & 63][j & 63];
+      if (!(w > 0.0)) {
+        if (!bad) bad = j + 1;
+        w = 1.0;
+      }
+      const double d = sqrt(w);
+      col[0] = d;
+      Lc[(size_t)j * B1] = d;
+    }
+    __syncthreads();
+    const int t = (int)threadIdx.x + 1;
+    if (t <= b) {
+      const double v = j + t < n ? W[(j + t) & 63][j & 63] / col[0] : 0.0;
+      col[t] = v;
+      if (j + t < n) Lc[(size_t)j * B1 + t] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXP; q++) {
+      const int t1 = pt1[q], t2 = pt2[q];
+      if (t1 > 0 && j + t1 < n) W[(j + t1) & 63][(j + t2) & 63] -= col[t1] * col[t2];
+    }
+    const int inew = j + B1;
+    if (inew < n && (int)threadIdx.x < B1) {
+      const int l = j + 1 + (int)threadIdx.x;
+      W[inew & 63][l & 63] = s_entry(T, S3, inew, l);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) info[0] = bad;
+}
+
+__device__ inline double wsum_all(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Row j of X = S^-1 (= column j, S symmetric) by banded substitution of e_j:
+// forward L y = e_j from row j on (lane t holds the partial y_{m+t}), then
+// backward L^T x = y over every row (lane t holds x_{i+t}); one wavefront
+// per j, four per block.  y is kept in X's row j between the two passes
+// (written and read back by lane 0).
+__global__ void __launch_bounds__(256) k_tl_band_inv(TwoLevel T, const double *__restrict__ Lc,
+                                                    double *__restrict__ X) {
+  const int ln = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (j >= T.nc) return;
+  const int n = T.nc, b = T.naj, B1 = b + 1;
+  double *xr = X + (size_t)j * T.ld;
+  const bool band = ln >= 1 && ln <= b;
+  double w = ln == 0 ? 1.0 : 0.0;
+  for (int m = j; m < n; m++) {
+    const double ym = __shfl(w, 0) / Lc[(size_t)m * B1];
+    if (ln == 0) xr[m] = ym;
+    const double lv = (band && m + ln < n) ? Lc[(size_t)m * B1 + ln] : 0.0;
+    w = w - lv * ym;
+    w = __shfl_down(w, 1);
+    if (ln == 63) w = 0.0;
+  }
+  w = 0.0;
+  for (int i = n - 1; i >= 0; i--) {
+    const double lv = (band && i + ln < n) ? Lc[(size_t)i * B1 + ln] : 0.0;
+    const double sdot = wsum_all(lv * w);
+    double yi = 0.0;
+    if (ln == 0 && i >= j) yi = xr[i];
+    yi = __shfl(yi, 0);
+    const double xi = (yi - sdot) / Lc[(size_t)i * B1];
+    if (ln == 0) xr[i] = xi;
+    w = __shfl_up(w, 1);
+    if (ln == 1) w = xi;
+    if (ln == 0) w = 0.0;
+  }
+  for (int c = n + ln; c < T.ld; c += 64) xr[c] = 0.0;
+}
+
+// Ainv = -X row by row (zero beyond nc), FP64 and / or FP32
+__global__ void k_tl_finish_rows(TwoLevel T, const double *__restrict__ X,
+                                 double *__restrict__ A64, float *__restrict__ A32) {
+  const size_t n = (size_t)T.nc * T.ld;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % T.ld);
+    const double v = c < T.nc ? -X[e] : 0.0;
+    if (A64) A64[e] = v;
+    if (A32) A32[e] = (float)v;
+  }
 }
 
 // Ainv = -(G^-1) from the lower triangle potri left in G (column-major:
@@ -307,7 +465,9 @@ int mf6hip_tl_alloc(Mf6Handle *h) {
       hipMalloc((void **)&h->tl_yc64, sizeof(double) * h->tl.ld) != hipSuccess ||
       hipMalloc((void **)&h->tl_yc32, sizeof(float) * h->tl.ld) != hipSuccess ||
       hipMalloc((void **)&h->tl_info, sizeof(int)) != hipSuccess ||
-      hipMalloc((void **)&h->tl_colsum, sizeof(double) * (size_t)L.NRC) != hipSuccess) {
+      hipMalloc((void **)&h->tl_colsum, sizeof(double) * (size_t)L.NRC) != hipSuccess ||
+      hipMalloc((void **)&h->tl_S3, sizeof(double) * 3 * (size_t)h->tl.nc) != hipSuccess ||
+      hipMalloc((void **)&h->tl_Lc, sizeof(double) * 64 * (size_t)h->tl.nc) != hipSuccess) {
     mf6hip_set_error("hipMalloc failed (two-level preconditioner)");
     return MF6HIP_EHIP;
   }
@@ -337,7 +497,7 @@ void mf6hip_tl_free(Mf6Handle *h) {
   if (h->tl_blas) rocblas_destroy_handle((rocblas_handle)h->tl_blas);
   h->tl_blas = nullptr;
   void *bufs[] = {h->tl_cmask, h->tl_G, h->tl_rc, h->tl_yc64, h->tl_yc32, h->tl_info,
-                  h->tl_A64, h->tl_A32, h->tl_colsum};
+                  h->tl_A64, h->tl_A32, h->tl_colsum, h->tl_S3, h->tl_Lc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tl_cmask = nullptr;
@@ -345,40 +505,58 @@ void mf6hip_tl_free(Mf6Handle *h) {
   h->tl_A32 = h->tl_yc32 = nullptr;
   h->tl_info = nullptr;
   h->tl_colsum = nullptr;
+  h->tl_S3 = h->tl_Lc = nullptr;
 }
 
-// Coarse operator of the current FP64 planes: mask, -Ac, its inverse.
+// Coarse operator of the current FP64 planes: mask, S = -Ac, its inverse.
 int mf6hip_tl_setup(Mf6Handle *h) {
   CK(mf6hip_tl_alloc(h));
   const Layout &L = h->L;
   const TwoLevel &T = h->tl;
   hipStream_t s = h->stream;
   k_tl_cmask<<<tl_grid(2 * (size_t)L.HP), MF6_BLOCK, 0, s>>>(L, h->A, h->tl_cmask);
-  HIPCHK(hipMemsetAsync(h->tl_G, 0, sizeof(double) * (size_t)T.nc * T.ld, s));
-  k_tl_build<<<T.nc, MF6_BLOCK, 0, s>>>(L, h->A, h->tl_cmask, T, h->tl_G);
+  k_tl_build<<<T.nc, MF6_BLOCK, 0, s>>>(L, h->A, h->tl_cmask, T, h->tl_S3);
   HIPCHK(hipGetLastError());
-  rocblas_handle rb = (rocblas_handle)h->tl_blas;
-  if (rocsolver_dpotrf(rb, rocblas_fill_lower, T.nc, h->tl_G, T.ld, h->tl_info) !=
-      rocblas_status_success) {
-    mf6hip_set_error("rocsolver_dpotrf failed");
-    return MF6HIP_EHIP;
-  }
   int info = 0;
-  HIPCHK(hipMemcpyAsync(&info, h->tl_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (T.naj <= 63) {
+    k_tl_band_chol<<<1, TL_CHOL_THREADS, 0, s>>>(T, h->tl_S3, h->tl_Lc, h->tl_info);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&info, h->tl_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info == 0) {
+      k_tl_band_inv<<<(T.nc + 3) / 4, 256, 0, s>>>(T, h->tl_Lc, h->tl_G);
+      k_tl_finish_rows<<<tl_grid((size_t)T.nc * T.ld), MF6_BLOCK, 0, s>>>(T, h->tl_G, h->tl_A64,
+                                                                         h->tl_A32);
+      HIPCHK(hipGetLastError());
+    }
+  } else {
+    HIPCHK(hipMemsetAsync(h->tl_G, 0, sizeof(double) * (size_t)T.nc * T.ld, s));
+    k_tl_expand<<<tl_grid(T.nc), MF6_BLOCK, 0, s>>>(T, h->tl_S3, h->tl_G);
+    HIPCHK(hipGetLastError());
+    rocblas_handle rb = (rocblas_handle)h->tl_blas;
+    if (rocsolver_dpotrf(rb, rocblas_fill_lower, T.nc, h->tl_G, T.ld, h->tl_info) !=
+        rocblas_status_success) {
+      mf6hip_set_error("rocsolver_dpotrf failed");
+      return MF6HIP_EHIP;
+    }
+    HIPCHK(hipMemcpyAsync(&info, h->tl_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info == 0) {
+      if (rocsolver_dpotri(rb, rocblas_fill_lower, T.nc, h->tl_G, T.ld, h->tl_info) !=
+          rocblas_status_success) {
+        mf6hip_set_error("rocsolver_dpotri failed");
+        return MF6HIP_EHIP;
+      }
+      k_tl_finish<<<tl_grid((size_t)T.nc * T.ld), MF6_BLOCK, 0, s>>>(T, h->tl_G, h->tl_A64,
+                                                                    h->tl_A32);
+      HIPCHK(hipGetLastError());
+    }
+  }
   if (info != 0) {
     mf6hip_set_error("two-level preconditioner: the coarse matrix is not definite (leading minor " +
                      std::to_string(info) + "); the system must be symmetric definite");
     return MF6HIP_EINVAL;
   }
-  if (rocsolver_dpotri(rb, rocblas_fill_lower, T.nc, h->tl_G, T.ld, h->tl_info) !=
-      rocblas_status_success) {
-    mf6hip_set_error("rocsolver_dpotri failed");
-    return MF6HIP_EHIP;
-  }
-  k_tl_finish<<<tl_grid((size_t)T.nc * T.ld), MF6_BLOCK, 0, s>>>(T, h->tl_G, h->tl_A64,
-                                                                h->tl_A32);
-  HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }
 
