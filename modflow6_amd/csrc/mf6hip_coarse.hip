@@ -165,12 +165,17 @@ __global__ void k_tl_expand(TwoLevel T, const double *__restrict__ S3, double *_
 // Right-looking: step j takes the pivot, scales column j (rows j+1 .. j+b)
 // and subtracts its outer product from the trailing (b+1) x (b+1) window,
 // which lives in LDS with rows and columns taken modulo 64; the row j+b+1
-// enters the window from S as row j leaves it.  Lc[j (b+1) + t] = L(j+t, j).
+// enters the window from S as row j leaves it.  Lc[j (b+1) + t] = L(j+t, j)
+// (by columns), Lr[i (b+1) + t] = L(i, i-t) (by rows), Lc[j (b+1)] holding
+// the reciprocal pivot 1 / L(j, j) for the substitutions.  (A variant with
+// one barrier per column, forming column j+1 during step j's update, was
+// slower: the step is bound by the window's LDS traffic, not the barriers.)
 // info[0] = 1 + the first column whose pivot is not positive, 0 if none.
 constexpr int TL_CHOL_THREADS = 512;
 __global__ void __launch_bounds__(TL_CHOL_THREADS) k_tl_band_chol(TwoLevel T,
                                                                   const double *__restrict__ S3,
                                                                   double *__restrict__ Lc,
+                                                                  double *__restrict__ Lr,
                                                                   int *info) {
   __shared__ double W[64][65];
   __shared__ double col[64];
@@ -210,14 +215,17 @@ __global__ void __launch_bounds__(TL_CHOL_THREADS) k_tl_band_chol(TwoLevel T,
       }
       const double d = sqrt(w);
       col[0] = d;
-      Lc[(size_t)j * B1] = d;
+      Lc[(size_t)j * B1] = 1.0 / d;
     }
     __syncthreads();
     const int t = (int)threadIdx.x + 1;
     if (t <= b) {
       const double v = j + t < n ? W[(j + t) & 63][j & 63] / col[0] : 0.0;
       col[t] = v;
-      if (j + t < n) Lc[(size_t)j * B1 + t] = v;
+      if (j + t < n) {
+        Lc[(size_t)j * B1 + t] = v;
+        Lr[(size_t)(j + t) * B1 + t] = v;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -241,40 +249,113 @@ __device__ inline double wsum_all(double v) {
   return v;
 }
 
-// Row j of X = S^-1 (= column j, S symmetric) by banded substitution of e_j:
-// forward L y = e_j from row j on (lane t holds the partial y_{m+t}), then
-// backward L^T x = y over every row (lane t holds x_{i+t}); one wavefront
-// per j, four per block.  y is kept in X's row j between the two passes
-// (written and read back by lane 0).
+// lane `lane`'s value in every lane (two 32-bit readlanes; lane is uniform)
+__device__ inline double readlane64(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Row j of X = S^-1 (= column j, S symmetric) by banded substitution of e_j,
+// both passes in axpy form with the band spread over the lanes: row r lives
+// in lane r mod 64, so a step reads its pivot row with one readlane and no
+// value moves between lanes.
+//   forward  L y = e_j from row j on: step m takes y_m from lane m mod 64,
+//            lanes of rows m+1 .. m+b subtract L(row, m) y_m (column m of L
+//            from Lc), and lane m mod 64 then holds row m + 64 (zero);
+//   backward L^T x = y from the last row: step i takes x_i, lanes of rows
+//            i-b .. i-1 subtract L(i, row) x_i (row i of L from Lr), and the
+//            row i-1-b enters (its lane holds a finished row, or row i when
+//            b = 63) with y_{i-1-b} from the forward pass.
+// One wavefront per j, four per block; y is kept in X's row j between the
+// passes.  The steps run in chunks of TL_CH whose band values are loaded
+// together, one chunk ahead (the loads are L2 hits whose latency is many
+// steps long).
+constexpr int TL_CH = 8;
 __global__ void __launch_bounds__(256) k_tl_band_inv(TwoLevel T, const double *__restrict__ Lc,
+                                                    const double *__restrict__ Lr,
                                                     double *__restrict__ X) {
   const int ln = threadIdx.x & 63;
   const int j = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (j >= T.nc) return;
   const int n = T.nc, b = T.naj, B1 = b + 1;
   double *xr = X + (size_t)j * T.ld;
-  const bool band = ln >= 1 && ln <= b;
-  double w = ln == 0 ? 1.0 : 0.0;
-  for (int m = j; m < n; m++) {
-    const double ym = __shfl(w, 0) / Lc[(size_t)m * B1];
-    if (ln == 0) xr[m] = ym;
-    const double lv = (band && m + ln < n) ? Lc[(size_t)m * B1 + ln] : 0.0;
-    w = w - lv * ym;
-    w = __shfl_down(w, 1);
-    if (ln == 63) w = 0.0;
+  auto lcol = [&](int m) {
+    const int t = (ln - m) & 63;
+    return (m < n && t >= 1 && t <= b && m + t < n) ? Lc[(size_t)m * B1 + t] : 0.0;
+  };
+  auto rpiv = [&](int m) { return (m >= 0 && m < n) ? Lc[(size_t)m * B1] : 1.0; };
+  double w = ln == (j & 63) ? 1.0 : 0.0;
+  double lv[TL_CH], rd[TL_CH], lvn[TL_CH], rdn[TL_CH];
+#pragma unroll
+  for (int u = 0; u < TL_CH; u++) {
+    lv[u] = lcol(j + u);
+    rd[u] = rpiv(j + u);
   }
-  w = 0.0;
-  for (int i = n - 1; i >= 0; i--) {
-    const double lv = (band && i + ln < n) ? Lc[(size_t)i * B1 + ln] : 0.0;
-    const double sdot = wsum_all(lv * w);
-    double yi = 0.0;
-    if (ln == 0 && i >= j) yi = xr[i];
-    yi = __shfl(yi, 0);
-    const double xi = (yi - sdot) / Lc[(size_t)i * B1];
-    if (ln == 0) xr[i] = xi;
-    w = __shfl_up(w, 1);
-    if (ln == 1) w = xi;
-    if (ln == 0) w = 0.0;
+  for (int m0 = j; m0 < n; m0 += TL_CH) {
+#pragma unroll
+    for (int u = 0; u < TL_CH; u++) {
+      lvn[u] = lcol(m0 + TL_CH + u);
+      rdn[u] = rpiv(m0 + TL_CH + u);
+    }
+#pragma unroll
+    for (int u = 0; u < TL_CH; u++) {
+      const int m = m0 + u;
+      if (m < n) {
+        const double ym = readlane64(w, m & 63) * rd[u];
+        if (ln == 0) xr[m] = ym;
+        w = w - lv[u] * ym;
+        if (ln == (m & 63)) w = 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TL_CH; u++) {
+      lv[u] = lvn[u];
+      rd[u] = rdn[u];
+    }
+  }
+  // the backward pass reads, in every lane, the y that lane 0 stored
+  __threadfence_block();
+  auto lrow = [&](int i) {
+    const int t = (i - ln) & 63;
+    return (i >= 0 && t >= 1 && t <= b && i - t >= 0) ? Lr[(size_t)i * B1 + t] : 0.0;
+  };
+  auto yraw = [&](int m) { return (m >= j && m >= 0) ? xr[m] : 0.0; };
+  {
+    const int t = (n - 1 - ln) & 63;
+    w = t <= b ? yraw(n - 1 - t) : 0.0;
+  }
+  double yi[TL_CH], yin[TL_CH];
+#pragma unroll
+  for (int u = 0; u < TL_CH; u++) {
+    lv[u] = lrow(n - 1 - u);
+    rd[u] = rpiv(n - 1 - u);
+    yi[u] = yraw(n - 2 - b - u);
+  }
+  for (int i0 = n - 1; i0 >= 0; i0 -= TL_CH) {
+#pragma unroll
+    for (int u = 0; u < TL_CH; u++) {
+      lvn[u] = lrow(i0 - TL_CH - u);
+      rdn[u] = rpiv(i0 - TL_CH - u);
+      yin[u] = yraw(i0 - TL_CH - 1 - b - u);
+    }
+#pragma unroll
+    for (int u = 0; u < TL_CH; u++) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        const double xi = readlane64(w, i & 63) * rd[u];
+        if (ln == 0) xr[i] = xi;
+        w = w - lv[u] * xi;
+        if (ln == ((i - 1 - b) & 63)) w = yi[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TL_CH; u++) {
+      lv[u] = lvn[u];
+      rd[u] = rdn[u];
+      yi[u] = yin[u];
+    }
   }
   for (int c = n + ln; c < T.ld; c += 64) xr[c] = 0.0;
 }
@@ -467,7 +548,8 @@ int mf6hip_tl_alloc(Mf6Handle *h) {
       hipMalloc((void **)&h->tl_info, sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&h->tl_colsum, sizeof(double) * (size_t)L.NRC) != hipSuccess ||
       hipMalloc((void **)&h->tl_S3, sizeof(double) * 3 * (size_t)h->tl.nc) != hipSuccess ||
-      hipMalloc((void **)&h->tl_Lc, sizeof(double) * 64 * (size_t)h->tl.nc) != hipSuccess) {
+      hipMalloc((void **)&h->tl_Lc, sizeof(double) * 64 * (size_t)h->tl.nc) != hipSuccess ||
+      hipMalloc((void **)&h->tl_Lr, sizeof(double) * 64 * (size_t)h->tl.nc) != hipSuccess) {
     mf6hip_set_error("hipMalloc failed (two-level preconditioner)");
     return MF6HIP_EHIP;
   }
@@ -497,7 +579,7 @@ void mf6hip_tl_free(Mf6Handle *h) {
   if (h->tl_blas) rocblas_destroy_handle((rocblas_handle)h->tl_blas);
   h->tl_blas = nullptr;
   void *bufs[] = {h->tl_cmask, h->tl_G, h->tl_rc, h->tl_yc64, h->tl_yc32, h->tl_info,
-                  h->tl_A64, h->tl_A32, h->tl_colsum, h->tl_S3, h->tl_Lc};
+                  h->tl_A64, h->tl_A32, h->tl_colsum, h->tl_S3, h->tl_Lc, h->tl_Lr};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tl_cmask = nullptr;
@@ -505,7 +587,7 @@ void mf6hip_tl_free(Mf6Handle *h) {
   h->tl_A32 = h->tl_yc32 = nullptr;
   h->tl_info = nullptr;
   h->tl_colsum = nullptr;
-  h->tl_S3 = h->tl_Lc = nullptr;
+  h->tl_S3 = h->tl_Lc = h->tl_Lr = nullptr;
 }
 
 // Coarse operator of the current FP64 planes: mask, S = -Ac, its inverse.
@@ -519,12 +601,12 @@ int mf6hip_tl_setup(Mf6Handle *h) {
   HIPCHK(hipGetLastError());
   int info = 0;
   if (T.naj <= 63) {
-    k_tl_band_chol<<<1, TL_CHOL_THREADS, 0, s>>>(T, h->tl_S3, h->tl_Lc, h->tl_info);
+    k_tl_band_chol<<<1, TL_CHOL_THREADS, 0, s>>>(T, h->tl_S3, h->tl_Lc, h->tl_Lr, h->tl_info);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&info, h->tl_info, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (info == 0) {
-      k_tl_band_inv<<<(T.nc + 3) / 4, 256, 0, s>>>(T, h->tl_Lc, h->tl_G);
+      k_tl_band_inv<<<(T.nc + 3) / 4, 256, 0, s>>>(T, h->tl_Lc, h->tl_Lr, h->tl_G);
       k_tl_finish_rows<<<tl_grid((size_t)T.nc * T.ld), MF6_BLOCK, 0, s>>>(T, h->tl_G, h->tl_A64,
                                                                          h->tl_A32);
       HIPCHK(hipGetLastError());

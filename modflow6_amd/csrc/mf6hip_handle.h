@@ -146,7 +146,8 @@ struct Mf6Handle {
   int *tl_info = nullptr;
   double *tl_colsum = nullptr;   // per grid column: residual summed over coupled cells
   double *tl_S3 = nullptr;       // compact rows of S = -Ac (diagonal, west, north)
-  double *tl_Lc = nullptr;       // banded Cholesky factor of S, nc x (naj + 1)
+  double *tl_Lc = nullptr;       // banded Cholesky factor of S by columns, nc x (naj + 1)
+  double *tl_Lr = nullptr;       // the same factor by rows
   void *tl_blas = nullptr;    // rocblas_handle
   int spmv_calls = 0;
 };
