@@ -217,8 +217,21 @@ def ensemble_workload(args):
         ax = np.bincount(rows, weights=amat * x[ja - 1], minlength=c.nodes)
         resid.append(float(np.max(np.abs(rhs - ax))))
     ens.destroy()
+    from modflow6_amd import _lib
     from modflow6_amd.ensemble import solve_one_by_one
-    one_s, one_its = solve_one_by_one(cases, ens.systems, log=log)
+    one_s, one_its, _ = solve_one_by_one(cases, ens.systems, log=log)
+    # the same systems one at a time with the two-level preconditioner
+    # (FP16-factor mixed CG + aggregation coarse correction)
+    tl_s, tl_its, tl_heads = solve_one_by_one(cases, ens.systems, log=log,
+                                              precision=_lib.MF6HIP_MIXED_FP16,
+                                              preconditioner=2, keep=args.check)
+    tl_resid = []
+    for i, c in enumerate(cases[:args.check]):
+        amat, rhs, _ = ens.systems[i]
+        ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
+        rows = np.repeat(np.arange(c.nodes), np.diff(ia))
+        ax = np.bincount(rows, weights=amat * tl_heads[i][ja - 1], minlength=c.nodes)
+        tl_resid.append(float(np.max(np.abs(rhs - ax))))
     ms = float(np.mean(runs))
     print(json.dumps(dict(workload=f"ensemble of {len(cases)} steady realisations "
                                    f"(nlay 2-8, n 64-512), {cells} cells",
@@ -228,7 +241,11 @@ def ensemble_workload(args):
                           max_true_residual_checked=max(resid) if resid else None,
                           checked_systems=len(resid),
                           one_by_one_s=one_s, one_by_one_iterations_mean=float(np.mean(one_its)),
-                          speedup_vs_one_by_one=one_s / (ms * 1e-3))))
+                          speedup_vs_one_by_one=one_s / (ms * 1e-3),
+                          two_level_one_by_one_s=tl_s,
+                          two_level_iterations_mean=float(np.mean(tl_its)),
+                          two_level_iterations_max=int(np.max(tl_its)),
+                          two_level_max_true_residual_checked=max(tl_resid) if tl_resid else None)))
 
 
 def learned_workload(args):

@@ -520,13 +520,16 @@ inline int tl_grid(size_t n) {
 }  // namespace
 
 // Aggregate width and grid: the smallest ha whose aggregate grid has at most
-// target coarse unknowns.
+// target coarse unknowns and, where that costs at most a doubling of ha, at
+// most 63 aggregates per row (the bandwidth the banded setup handles).
 TwoLevel mf6hip_tl_shape(const Layout &L, int target) {
   if (target <= 0) target = 4096;
   TwoLevel T{};
   int ha = (int)std::ceil(std::sqrt((double)L.NRC / (double)target));
   if (ha < 1) ha = 1;
   while (((L.nrow + ha - 1) / ha) * ((L.ncol + ha - 1) / ha) > target) ha++;
+  const int ha0 = ha;
+  while ((L.ncol + ha - 1) / ha > 63 && ha < 2 * ha0) ha++;
   T.ha = ha;
   T.nai = (L.nrow + ha - 1) / ha;
   T.naj = (L.ncol + ha - 1) / ha;

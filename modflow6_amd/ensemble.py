@@ -114,17 +114,26 @@ class Ensemble:
             self.h = C.c_void_p()
 
 
-def solve_one_by_one(cases, systems, log=None):
+def coarse_size_for(case):
+    """Coarse size of the two-level preconditioner for one realisation: one
+    aggregate per 16 grid columns, at most 4096."""
+    return int(min(4096, max(16, case.nrow * case.ncol // 16)))
+
+
+def solve_one_by_one(cases, systems, log=None, precision=_lib.MF6HIP_MIXED,
+                     preconditioner=0, keep=0):
     """The same realisations solved one at a time by the single-model device
-    solver (same mixed-precision fused CG, one handle per realisation, created
-    outside the timing).  Returns (seconds summed over the solves,
-    iterations)."""
+    solver (the fused CG, one handle per realisation, created outside the
+    timing; preconditioner 2 with coarse_size_for each realisation).
+    Returns (seconds summed over the solves, iterations, heads of the first
+    `keep` realisations)."""
     from .ims import HipLinearSolver
-    total, its = 0.0, []
+    total, its, heads = 0.0, [], []
     for i, (c, (amat, rhs, x0)) in enumerate(zip(cases, systems)):
         s = ImsLinearSettings(dvclose=c.ims.inner_dvclose, rclose=c.ims.inner_rclose,
                               iter1=c.ims.inner_maximum, ilinmeth=1,
-                              precision=_lib.MF6HIP_MIXED)
+                              precision=precision, preconditioner=preconditioner,
+                              coarse_size=coarse_size_for(c) if preconditioner == 2 else 0)
         ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
         sv = HipLinearSolver().initialize(ia, ja, s, c.nlay, c.nrow, c.ncol)
         x = x0.copy()
@@ -134,7 +143,9 @@ def solve_one_by_one(cases, systems, log=None):
         _, it = sv.solve(1, amat, rhs, x)
         total += time.perf_counter() - t0
         its.append(it)
+        if i < keep:
+            heads.append(x.copy())
         sv.destroy()
         if log and ((i + 1) % 64 == 0 or i + 1 == len(cases)):
             log(f"one by one: {i + 1}/{len(cases)} solved, {total:.2f} s of solves")
-    return total, its
+    return total, its, heads

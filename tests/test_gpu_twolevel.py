@@ -24,10 +24,14 @@ pytestmark = pytest.mark.gpu
 
 
 def coarse_shape(nrow, ncol, target):
-    """Smallest aggregate width ha with ceil(nrow/ha) ceil(ncol/ha) <= target."""
+    """Smallest aggregate width ha with ceil(nrow/ha) ceil(ncol/ha) <= target,
+    widened (at most to twice that) until ceil(ncol/ha) <= 63."""
     target = target or 4096
     ha = max(1, int(np.ceil(np.sqrt(nrow * ncol / target))))
     while -(-nrow // ha) * -(-ncol // ha) > target:
+        ha += 1
+    ha0 = ha
+    while -(-ncol // ha) > 63 and ha < 2 * ha0:
         ha += 1
     return ha, -(-nrow // ha), -(-ncol // ha)
 
