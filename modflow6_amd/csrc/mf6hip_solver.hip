@@ -171,12 +171,19 @@ __global__ void k_pos_to_nat(Layout L, const TI *__restrict__ in, TO *__restrict
 
 // max |x| over n values, as the bit pattern of a non-negative double (whose
 // integer order is the value order)
-__global__ void k_maxabs_bits(int n, const double *__restrict__ x, unsigned long long *out) {
+// (one atomic per block: reduced over the block in LDS first)
+__global__ void __launch_bounds__(MF6_BLOCK) k_maxabs_bits(int n, const double *__restrict__ x,
+                                                           unsigned long long *out) {
+  __shared__ double lds[MF6_BLOCK / 64];
   double m = 0.0;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
     m = fmax(m, fabs(x[e]));
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, lds[w]);
+  if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
 // y = (FP16)(x * scale) (scale a power of two: exact before the rounding)
@@ -941,8 +948,8 @@ static int ilu_factor(Mf6Handle *h) {
     // scaled FP16 planes for the fused CG's preconditioner: 2^-k with k the
     // smallest integer that brings max |off-diagonal| to 2^14 or below
     HIPCHK(hipMemsetAsync(h->d_amax, 0, sizeof(double), s));
-    k_maxabs_bits<<<grid_for(3 * L.NP), MF6_BLOCK, 0, s>>>(3 * L.NP, h->A + 4 * (size_t)L.NP,
-                                                          (unsigned long long *)h->d_amax);
+    k_maxabs_bits<<<grid_for(3 * L.NP) < 2048 ? grid_for(3 * L.NP) : 2048, MF6_BLOCK, 0, s>>>(
+        3 * L.NP, h->A + 4 * (size_t)L.NP, (unsigned long long *)h->d_amax);
     double amax = 0.0;
     HIPCHK(hipMemcpyAsync(&amax, h->d_amax, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
