@@ -64,6 +64,8 @@ CASES = {
     "steady_small": (lambda: configs.steady_small(nlay=3, nrow=20, ncol=24, nwel=6), 40),
     # odd ncol (single-column direction update), aggregates cut by the edge
     "steady_odd": (lambda: configs.steady_small(nlay=2, nrow=13, ncol=17, nwel=3), 12),
+    # an aggregate grid 150 wide: the dense (rocSOLVER) coarse factorisation
+    "steady_wide": (lambda: configs.steady_small(nlay=2, nrow=4, ncol=300, nwel=3), 1200),
 }
 
 
@@ -74,6 +76,8 @@ def test_coarse_operator_matches_dense_solve(name):
     sim, amat, rhs, x0 = oracle_system(case)
     d = sim.dis
     Q, ha = coarse_operator(case, amat, d.ia, d.ja, target)
+    if name == "steady_wide":
+        assert -(-case.ncol // ha) > 63     # wider than the banded setup takes
     sv = device_solver(case, preconditioner=2, coarse_size=target)
     try:
         load_factor(sv, amat)
