@@ -255,8 +255,9 @@ def learned_workload(args):
                                            scale_up_iters=args.scale_up_iters,
                                            nn_every=[int(v) for v in args.nn_every.split(",")],
                                            log=log)
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    np.save(os.path.join(ROOT, "gpurun_out", "unet_params.npy"), params)
+    if args.params_out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.params_out)), exist_ok=True)
+        np.save(args.params_out, params)
     print(json.dumps(summary))
 
 
@@ -295,6 +296,8 @@ def main():
                     help="learned: also solve the 10 x 1000 x 1000 model with the network")
     ap.add_argument("--scale-up-iters", type=int, default=20000,
                     help="learned: iteration cap of the 10M-cell flexible-CG solve")
+    ap.add_argument("--params-out", default="",
+                    help="learned: file (.npy) that receives the trained parameters")
     ap.add_argument("--nsys", type=int, default=256, help="ensemble: realisations")
     ap.add_argument("--check", type=int, default=16,
                     help="ensemble: realisations whose true residual is checked")
