@@ -409,7 +409,8 @@ def main():
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=traffic, traffic_source=traffic_src,
-                    kernel="k_f_spmv", bytes_per_launch=sp_bytes,
+                    kernel=("k_f_spmv2" if ncol % 4 == 0 and (nrow * ncol // 2) % 2 == 0
+                            else "k_f_spmv"), bytes_per_launch=sp_bytes,
                     avg_launch_us=sp_us,
                     csr_equivalent_bytes=csr32 if f32 else csr64,
                     csr_equivalent_GBs=((csr32 if f32 else csr64) /
