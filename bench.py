@@ -219,6 +219,27 @@ def ensemble_workload(args):
     ens.destroy()
     from modflow6_amd import _lib
     from modflow6_amd.ensemble import solve_one_by_one
+    # the same systems batched with the two-level preconditioner
+    ens2 = Ensemble(cases, log=log, preconditioner=2, systems=ens.systems)
+    tune2 = ens2.autotune()
+    log("autotune (two-level): " + tune2)
+    runs2 = []
+    for s in range(args.warmup + args.steps):
+        ic2, it2, ms2 = ens2.solve()
+        log(f"two-level ensemble solve {s}{' (warmup)' if s < args.warmup else ''}: "
+            f"{ms2:.1f} ms, iterations min/mean/max {it2.min()}/{it2.mean():.0f}/{it2.max()}, "
+            f"converged {int(ic2.sum())}/{len(cases)}")
+        if s >= args.warmup:
+            runs2.append(ms2)
+    resid2 = []
+    for i, c in enumerate(cases[:args.check]):
+        amat, rhs, _ = ens.systems[i]
+        x = ens2.heads(i)
+        ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
+        rows = np.repeat(np.arange(c.nodes), np.diff(ia))
+        ax = np.bincount(rows, weights=amat * x[ja - 1], minlength=c.nodes)
+        resid2.append(float(np.max(np.abs(rhs - ax))))
+    ens2.destroy()
     one_s, one_its, _ = solve_one_by_one(cases, ens.systems, log=log)
     # the same systems one at a time with the two-level preconditioner
     # (FP16-factor mixed CG + aggregation coarse correction)
@@ -242,6 +263,12 @@ def ensemble_workload(args):
                           checked_systems=len(resid),
                           one_by_one_s=one_s, one_by_one_iterations_mean=float(np.mean(one_its)),
                           speedup_vs_one_by_one=one_s / (ms * 1e-3),
+                          two_level_batched_ms=float(np.mean(runs2)),
+                          two_level_batched_iterations_mean=float(it2.mean()),
+                          two_level_batched_iterations_max=int(it2.max()),
+                          two_level_batched_converged=int(ic2.sum()),
+                          two_level_batched_max_true_residual_checked=max(resid2) if resid2 else None,
+                          two_level_autotune=tune2,
                           two_level_one_by_one_s=tl_s,
                           two_level_iterations_mean=float(np.mean(tl_its)),
                           two_level_iterations_max=int(np.max(tl_its)),
@@ -271,7 +298,7 @@ def main():
                          "factor), fp64_2l, mixed_2l, mixed16_2l (two-level); both: "
                          "fp64, mixed, mixed16, fp64_2l, mixed16_2l")
     ap.add_argument("--coarse-size", type=int, default=0,
-                    help="two-level: most coarse unknowns (0 = 4096)")
+                    help="two-level: most coarse unknowns (0 = sqrt(2 N) within [16, 4096])")
     ap.add_argument("--shape", default="10,1000,1000")
     ap.add_argument("--case", default="10m", choices=["10m", "small"],
                     help="10m: the headline model (--shape); small: the 3 x 100 x 100 "

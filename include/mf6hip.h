@@ -76,8 +76,8 @@ typedef struct mf6hip_ims_settings {
                            aggregates (symmetric systems, CG, fused kernels,
                            any precision) */
   int coarse_size;      /* preconditioner 2: most coarse unknowns (aggregates
-                           of ha x ha columns, all layers); 0 = 4096, at
-                           most 6144 */
+                           of ha x ha columns, all layers); 0 = sqrt(2 N)
+                           within [16, 4096]; at most 6144 */
 } mf6hip_ims_settings;
 
 /* Per-inner-iteration record, the values ConvergenceSummaryType keeps for

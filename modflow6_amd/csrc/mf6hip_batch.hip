@@ -39,18 +39,26 @@ struct BSys {
   int tr0, ntr;     // red tiles
   int tb0, ntb;     // black tiles
   int ti0, nti;     // interleaved tiles
+  // two-level preconditioner: coarse shape; offsets of the per-column
+  // coupled-cell masks (2 HP), the per-grid-column residual sums (NRC), the
+  // coarse vectors (ld) and the FP32 coarse inverse (nc ld); the system's
+  // coarse GEMV blocks tg0 .. tg0 + ntg - 1
+  TwoLevel tl;
+  long long cmo, cso, cvo, aio;
+  int tg0, ntg;
 };
 
 struct BFState {
   double apend_x, apend_d, pmax;
   int ploc, pcur;
+  int restart;      // the next round starts a new direction (beta = 0)
 };
 
 struct Tile {
   int sys, base;
 };
 
-enum { P_SS_R = 0, P_SS_B = 1, P_RHO_B = 2, P_RHO_R = 3, P_PQ = 4 };
+enum { P_SS_R = 0, P_SS_B = 1, P_RHO_B = 2, P_RHO_R = 3, P_PQ = 4, P_RHO_C = 5 };
 enum { M_D_R = 0, M_D_B = 1, M_P = 2, M_TRUE = 3 };
 
 template <typename T>
@@ -152,7 +160,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Til
                                                       const float *piv, float *d, const float *q,
                                                       const float *zf, float *z, double *part,
                                                       MaxLoc *mpart, int T, const KState *st,
-                                                      const BFState *fs) {
+                                                      const BFState *fs,
+                                                      const unsigned *cmask, double *colsum) {
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -161,6 +170,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Til
     const Layout L = sys[s].L;
     const float *A = A32 + sys[s].aoff;
     const float *pv = piv + sys[s].voff;
+    const unsigned *cms = colsum ? cmask + sys[s].cmo : nullptr;
+    double *css = colsum ? colsum + sys[s].cso : nullptr;
     float *dd = d + sys[s].voff;
     const float *qq = q + sys[s].voff;
     const float *zr = zf + sys[s].voff;
@@ -179,12 +190,15 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Til
       const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
       const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
       float zup = 0.f, lup = 0.f;
+      const unsigned cm = css ? cms[L.HP + h2] : 0u;
+      double cs = 0.0;
       for (int k = 0; k < L.nlay; k++) {
         const int pos = L.H + k * L.HP + h2;
         const int rb = k * L.HP;
         float dv = dd[pos];
         dv = dv - alpha * qq[pos];
         dd[pos] = dv;
+        if ((cm >> k) & 1u) cs = cs + (double)dv;
         md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
         ss = ss + (double)dv * (double)dv;
         float tv = dv;
@@ -197,6 +211,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Til
         zup = tv;
         lup = A[6 * NP + pos] * pv[pos];
       }
+      if (css) css[l] = cs;
       float zdn = 0.f;
       for (int k = L.nlay - 1; k >= 0; k--) {
         const int pos = L.H + k * L.HP + h2;
@@ -221,7 +236,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const T
                                                         int ntiles, int cpt, const float *A32,
                                                         const float *piv, const float *d,
                                                         const float *zf, float *z, double *part,
-                                                        int T, const KState *st) {
+                                                        int T, const KState *st,
+                                                        const unsigned *cmask, double *colsum) {
   __shared__ double lds[8];
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int s = tiles[tile].sys;
@@ -229,6 +245,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const T
     const Layout L = sys[s].L;
     const float *A = A32 + sys[s].aoff;
     const float *pv = piv + sys[s].voff;
+    const unsigned *cms = colsum ? cmask + sys[s].cmo : nullptr;
+    double *css = colsum ? colsum + sys[s].cso : nullptr;
     const float *dd = d + sys[s].voff;
     const float *zr = zf + sys[s].voff;
     float *zz = z + sys[s].voff;
@@ -244,6 +262,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const T
       const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
       const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
       float zdn = 0.f;
+      const unsigned cm = css ? cms[h2] : 0u;
+      double cs = 0.0;
       for (int k = L.nlay - 1; k >= 0; k--) {
         const int pos = k * L.HP + h2;
         const int bb = L.H + k * L.HP;
@@ -256,8 +276,11 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const T
         tv = tv * pv[pos];
         zz[pos] = tv;
         zdn = tv;
-        rho = rho + (double)dd[pos] * (double)tv;
+        const float dv = dd[pos];
+        rho = rho + (double)dv * (double)tv;
+        if ((cm >> k) & 1u) cs = cs + (double)dv;
       }
+      if (css) css[l] = cs;
     }
     rho = bsum(rho, lds);
     if (threadIdx.x == 0) part[P_RHO_R * T + tile] = rho;
@@ -269,7 +292,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_pupdate(const BSys *sys, const T
                                                         int ntiles, int cpt, const float *z,
                                                         float *p0, float *p1, float *e,
                                                         MaxLoc *mpart, int T, const KState *st,
-                                                        const BFState *fs) {
+                                                        const BFState *fs, const unsigned *cmask,
+                                                        const float *yc) {
   __shared__ MaxLoc mlds[8];
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int s = tiles[tile].sys;
@@ -290,14 +314,19 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_pupdate(const BSys *sys, const T
       if (h2 >= ncols_of(L, cc)) continue;
       const int l = col_l(L, cc, h2);
       const int own = cc * L.H + h2;
+      // two-level: z completed by the prolongation of the coarse correction
+      const unsigned cm = yc ? cmask[sys[s].cmo + cc * L.HP + h2] : 0u;
+      const float yv = cm ? yc[sys[s].cvo + tl_agg(sys[s].tl, L, l)] : 0.f;
       for (int k = 0; k < L.nlay; k++) {
         const int pos = own + k * L.HP;
+        float zv = zz[pos];
+        if ((cm >> k) & 1u) zv = zv + yv;
         float pv;
         if (first) {
-          pv = zz[pos];
+          pv = zv;
         } else {
           const float po = pin[pos];
-          pv = zz[pos] + beta * po;
+          pv = zv + beta * po;
           ee[pos] = ee[pos] + ax * po;
         }
         pout[pos] = pv;
@@ -399,6 +428,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_round(const BSys *sys, KState *s
     a[2] = a[2] + part[P_RHO_B * T + S.tb0 + i];
     mm[1] = maxloc_pick(mm[1], mpart[M_D_B * T + S.tb0 + i]);
   }
+  // two-level: rho gains rc.yc (partials of the system's coarse GEMV blocks)
+  for (int i = threadIdx.x; i < S.ntg; i += blockDim.x) a[3] = a[3] + part[P_RHO_C * T + S.tg0 + i];
   const double ssr = bsum(a[0], lds);
   const double ssb = bsum(a[1], lds);
   const double rb = bsum(a[2], lds);
@@ -431,6 +462,10 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_round(const BSys *sys, KState *s
   }
   k.rho = rho_new;
   if (it > 0) k.beta = rho_new / k.rho0;
+  if (f.restart) {
+    k.beta = 0.0;
+    f.restart = 0;
+  }
 }
 
 __global__ void __launch_bounds__(MF6_BLOCK) kb_alpha(const BSys *sys, KState *st, BFState *fs,
@@ -546,7 +581,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_residual(const BSys *sys, const 
 // per system: no pending update; the true max |r| into out[s]
 __global__ void __launch_bounds__(MF6_BLOCK) kb_after(const BSys *sys, BFState *fs,
                                                       const MaxLoc *mpart, int T, double *out,
-                                                      const int *active) {
+                                                      const int *active, int restart) {
   __shared__ MaxLoc mlds[8];
   const int s = blockIdx.x;
   if (!active[s]) return;
@@ -558,7 +593,79 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_after(const BSys *sys, BFState *
   if (threadIdx.x != 0) return;
   fs[s].apend_d = 0.0;
   fs[s].apend_x = 0.0;
+  fs[s].restart = restart;
   out[s] = m.v;
+}
+
+// two-level: tasks of four coarse rows (aggregates) of one system, in
+// system order; a system's tasks are tg0 .. tg0 + ntg - 1
+struct GTask {
+  int sys, row0;
+};
+
+// rc of every running system: one wavefront per aggregate sums the column
+// sums of its ha x ha columns (fixed order)
+__global__ void __launch_bounds__(MF6_BLOCK) kb_tl_gather(const BSys *sys, const GTask *tasks,
+                                                          int ntasks, const double *colsum,
+                                                          double *rc, const KState *st) {
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
+    const int s = tasks[task].sys;
+    if (st[s].stop) continue;
+    const BSys &S = sys[s];
+    const TwoLevel &Tl = S.tl;
+    const int I = tasks[task].row0 + w;
+    if (I >= Tl.nc) continue;
+    const int ai = I / Tl.naj, aj = I - ai * Tl.naj;
+    const double *cs = colsum + S.cso;
+    double v = 0.0;
+    for (int t = ln; t < Tl.ha * Tl.ha; t += 64) {
+      const int i = ai * Tl.ha + t / Tl.ha, j = aj * Tl.ha + t % Tl.ha;
+      if (i < S.L.nrow && j < S.L.ncol) v = v + cs[i * S.L.ncol + j];
+    }
+    v = wsum(v);
+    if (ln == 0) rc[S.cvo + I] = v;
+  }
+}
+
+// yc = Ainv rc for the task's four rows (one wavefront per row, 16-byte row
+// loads, rc staged in LDS: ld doubles, zero beyond nc) and the task's
+// partial of rc.yc
+__global__ void __launch_bounds__(MF6_BLOCK) kb_tl_gemv(const BSys *sys, const GTask *tasks,
+                                                        int ntasks, const double *rc,
+                                                        const float *Ainv, float *yc,
+                                                        double *part, int T, const KState *st) {
+  extern __shared__ double xs[];
+  __shared__ double lds[8];
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
+    const int s = tasks[task].sys;
+    if (st[s].stop) continue;
+    const BSys &S = sys[s];
+    const TwoLevel &Tl = S.tl;
+    __syncthreads();   // the previous task's xs is no longer read
+    for (int c = threadIdx.x; c < Tl.ld; c += blockDim.x)
+      xs[c] = c < Tl.nc ? rc[S.cvo + c] : 0.0;
+    __syncthreads();
+    double acc = 0.0;
+    const int I = tasks[task].row0 + w;
+    if (I < Tl.nc) {
+      const float *row = Ainv + S.aio + (size_t)I * Tl.ld;
+      double s0 = 0.0;
+      for (int c = 4 * ln; c < Tl.ld; c += 256) {
+        const float4 a = *reinterpret_cast<const float4 *>(row + c);
+        s0 = s0 + (double)a.x * xs[c] + (double)a.y * xs[c + 1] + (double)a.z * xs[c + 2] +
+             (double)a.w * xs[c + 3];
+      }
+      s0 = wsum(s0);
+      if (ln == 0) {
+        yc[S.cvo + I] = (float)s0;
+        acc = xs[I] * (double)(float)s0;
+      }
+    }
+    acc = bsum(acc, lds);
+    if (threadIdx.x == 0) part[P_RHO_C * T + task] = acc;
+  }
 }
 
 __global__ void kb_convert(long long n, const double *a, float *b) {
@@ -593,6 +700,18 @@ struct BatchHandle {
   KState *h_st = nullptr;
   double *h_rtrue = nullptr;
   int cpt = 1, grid = 0, R = 32;
+  // two-level preconditioner (every system, or none)
+  bool tl = false;
+  long long cmtot = 0, cstot = 0, cvtot = 0, aitot = 0;
+  int maxnc = 0, maxld = 0;
+  unsigned *tl_cmask = nullptr;
+  double *tl_colsum = nullptr, *tl_rc = nullptr;
+  float *tl_yc = nullptr, *tl_A32 = nullptr;
+  double *tl_S3 = nullptr, *tl_Lc = nullptr, *tl_Lr = nullptr, *tl_G = nullptr;
+  int *tl_dinfo = nullptr;
+  GTask *dtg = nullptr;
+  int ntg = 0;
+  double tl_setup_ms = 0.0;   // coarse setups at staging, summed
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -646,6 +765,7 @@ int build_tiles(BatchHandle *b, int cpt) {
   int T = b->ntr;
   if (b->ntb > T) T = b->ntb;
   if (b->nti > T) T = b->nti;
+  if (b->ntg > T) T = b->ntg;
   if (b->dtr) hipFree(b->dtr);
   if (b->dtb) hipFree(b->dtb);
   if (b->dti) hipFree(b->dti);
@@ -658,7 +778,7 @@ int build_tiles(BatchHandle *b, int cpt) {
   BCK(balloc(&b->dtr, tr.size()));
   BCK(balloc(&b->dtb, tb.size()));
   BCK(balloc(&b->dti, ti.size()));
-  BCK(balloc(&b->part, 5 * (size_t)T));
+  BCK(balloc(&b->part, 6 * (size_t)T));
   BCK(balloc(&b->mpart, 4 * (size_t)T));
   HIPCHK(hipMemcpy(b->dtr, tr.data(), sizeof(Tile) * tr.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->dtb, tb.data(), sizeof(Tile) * tb.size(), hipMemcpyHostToDevice));
@@ -674,13 +794,21 @@ void launch_iteration(BatchHandle *b) {
   const int g = b->grid, T = b->T, c = b->cpt;
   kb_red_fwd<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtr, b->ntr, c, b->A32, b->piv32, b->d32, b->q32,
                                      b->zf32, b->part, b->mpart, T, b->st, b->fs);
+  double *cs = b->tl ? b->tl_colsum : nullptr;
   kb_black<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtb, b->ntb, c, b->A32, b->piv32, b->d32, b->q32,
-                                   b->zf32, b->z32, b->part, b->mpart, T, b->st, b->fs);
+                                   b->zf32, b->z32, b->part, b->mpart, T, b->st, b->fs,
+                                   b->tl_cmask, cs);
   kb_red_bwd<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtr, b->ntr, c, b->A32, b->piv32, b->d32,
-                                     b->zf32, b->z32, b->part, T, b->st);
+                                     b->zf32, b->z32, b->part, T, b->st, b->tl_cmask, cs);
+  if (b->tl) {
+    kb_tl_gather<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dtg, b->ntg, b->tl_colsum, b->tl_rc, b->st);
+    kb_tl_gemv<<<g, MF6_BLOCK, sizeof(double) * (size_t)b->maxld, s>>>(
+        b->dsys, b->dtg, b->ntg, b->tl_rc, b->tl_A32, b->tl_yc, b->part, T, b->st);
+  }
   kb_round<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->st, b->fs, b->part, b->mpart, T);
   kb_pupdate<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, c, b->z32, b->p0, b->p1, b->e32,
-                                     b->mpart, T, b->st, b->fs);
+                                     b->mpart, T, b->st, b->fs, b->tl_cmask,
+                                     b->tl ? b->tl_yc : nullptr);
   kb_spmv<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, c, b->A32, b->p0, b->p1, b->q32,
                                   b->part, T, b->st, b->fs);
   kb_alpha<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->st, b->fs, b->part, b->mpart, T);
@@ -694,7 +822,9 @@ void launch_replacement(BatchHandle *b) {
                                   b->fs, b->active);
   kb_residual<<<g, MF6_BLOCK, 0, s>>>(b->dsys, b->dti, b->nti, b->cpt, b->A64, b->x64, b->b64,
                                       b->d32, b->mpart, b->T, b->active);
-  kb_after<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->fs, b->mpart, b->T, b->rtrue, b->active);
+  // with the coarse correction the direction restarts at each replacement
+  kb_after<<<b->nsys, MF6_BLOCK, 0, s>>>(b->dsys, b->fs, b->mpart, b->T, b->rtrue, b->active,
+                                         b->tl ? 1 : 0);
 }
 
 int capture(BatchHandle *b) {
@@ -763,10 +893,40 @@ extern "C" int mf6hip_batch_add(void *handle, const mf6hip_ims_settings *setting
     mf6hip_set_error("bad DIS shape");
     return MF6HIP_EINVAL;
   }
+  if (st.preconditioner != 0 && st.preconditioner != 2) {
+    mf6hip_set_error("ensemble systems use preconditioner 0 (ILU0) or 2 (two-level)");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  if (b->nsys > 0 && (st.preconditioner == 2) != b->tl) {
+    mf6hip_set_error("ensemble systems must all use the same preconditioner");
+    return MF6HIP_EINVAL;
+  }
+  if (st.coarse_size < 0 || st.coarse_size > 6144) {
+    mf6hip_set_error("coarse_size must be in [0, 6144]");
+    return MF6HIP_EINVAL;
+  }
   BSys S{};
   S.L = make_layout(st.nlay, st.nrow, st.ncol);
   S.aoff = b->ptot;
   S.voff = b->vtot;
+  b->tl = st.preconditioner == 2;
+  if (b->tl) {
+    S.tl = mf6hip_tl_shape(S.L, st.coarse_size);
+    if (S.tl.naj > 63) {
+      mf6hip_set_error("ensemble two-level: the aggregate grid is wider than 63 (raise coarse_size's aggregates)");
+      return MF6HIP_EUNSUPPORTED;
+    }
+    S.cmo = b->cmtot;
+    S.cso = b->cstot;
+    S.cvo = b->cvtot;
+    S.aio = b->aitot;
+    b->cmtot += 2 * (long long)S.L.HP;
+    b->cstot += S.L.NRC;
+    b->cvtot += S.tl.ld;
+    b->aitot += (long long)S.tl.nc * S.tl.ld;
+    if (S.tl.nc > b->maxnc) b->maxnc = S.tl.nc;
+    if (S.tl.ld > b->maxld) b->maxld = S.tl.ld;
+  }
   b->ptot += (long long)MF6_NSLOT * S.L.NP;
   b->vtot += S.L.NP;
   b->sys.push_back(S);
@@ -801,6 +961,28 @@ extern "C" int mf6hip_batch_finalize(void *handle) {
   BCK(balloc(&b->rtrue, b->nsys));
   HIPCHK(hipHostMalloc((void **)&b->h_st, sizeof(KState) * b->nsys, hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&b->h_rtrue, sizeof(double) * b->nsys, hipHostMallocDefault));
+  if (b->tl) {
+    std::vector<GTask> tg;
+    for (int s = 0; s < b->nsys; s++) {
+      BSys &S = b->sys[s];
+      S.tg0 = (int)tg.size();
+      for (int r = 0; r < S.tl.nc; r += 4) tg.push_back(GTask{s, r});
+      S.ntg = (int)tg.size() - S.tg0;
+    }
+    b->ntg = (int)tg.size();
+    BCK(balloc(&b->dtg, tg.size()));
+    HIPCHK(hipMemcpy(b->dtg, tg.data(), sizeof(GTask) * tg.size(), hipMemcpyHostToDevice));
+    BCK(balloc(&b->tl_cmask, b->cmtot));
+    BCK(balloc(&b->tl_colsum, b->cstot));
+    BCK(balloc(&b->tl_rc, b->cvtot));
+    BCK(balloc(&b->tl_yc, b->cvtot));
+    BCK(balloc(&b->tl_A32, b->aitot));
+    BCK(balloc(&b->tl_S3, 3 * (size_t)b->maxnc));
+    BCK(balloc(&b->tl_Lc, 64 * (size_t)b->maxnc));
+    BCK(balloc(&b->tl_Lr, 64 * (size_t)b->maxnc));
+    BCK(balloc(&b->tl_G, (size_t)b->maxnc * b->maxld));
+    BCK(balloc(&b->tl_dinfo, 1));
+  }
   BCK(build_tiles(b, 1));
   int ncu = 256;
   hipDeviceProp_t prop;
@@ -829,6 +1011,29 @@ extern "C" int mf6hip_batch_set_system(void *handle, int isys, const int *ia, in
   kb_convert<<<256, 256, 0, b->stream>>>(S.L.NP, b->piv64 + S.voff, b->piv32 + S.voff);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->tl) {
+    // the coarse operator of this system, with the ILU0 factor (staging)
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, b->stream));
+    int info = 0;
+    BCK(mf6hip_tl_setup_slice(S.L, b->A64 + S.aoff, S.tl, b->tl_cmask + S.cmo, b->tl_S3,
+                              b->tl_Lc, b->tl_Lr, b->tl_G, b->tl_A32 + S.aio, b->tl_dinfo,
+                              &info, b->stream));
+    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    b->tl_setup_ms += ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (info != 0) {
+      mf6hip_set_error("ensemble two-level: the coarse matrix of system " + std::to_string(isys) +
+                       " is not definite");
+      return MF6HIP_EINVAL;
+    }
+  }
   return MF6HIP_OK;
 }
 
@@ -880,6 +1085,7 @@ extern "C" int mf6hip_batch_autotune(void *handle, char *report, int report_len)
   HIPCHK(hipMemcpy(b->x64, xsave, sizeof(double) * b->vtot, hipMemcpyDeviceToDevice));
   batch_free_graph(b);
   rep += "chosen cpt " + std::to_string(bc) + " grid " + std::to_string(bg);
+  if (b->tl) rep += "; coarse setups at staging " + std::to_string(b->tl_setup_ms) + " ms";
   b->tune = rep;
   if (report && report_len > 0) {
     strncpy(report, rep.c_str(), report_len - 1);
@@ -973,7 +1179,9 @@ extern "C" int mf6hip_batch_destroy(void *handle) {
   batch_free_graph(b);
   void *bufs[] = {b->A64, b->b64, b->x64, b->x0, b->piv64, b->A32, b->piv32, b->d32, b->z32, b->zf32,
                   b->q32, b->p0, b->p1, b->e32, b->dsys, b->dtr, b->dtb, b->dti, b->part,
-                  b->mpart, b->st, b->fs, b->active, b->rtrue};
+                  b->mpart, b->st, b->fs, b->active, b->rtrue, b->dtg, b->tl_cmask,
+                  b->tl_colsum, b->tl_rc, b->tl_yc, b->tl_A32, b->tl_S3, b->tl_Lc, b->tl_Lr,
+                  b->tl_G, b->tl_dinfo};
   for (void *p : bufs)
     if (p) hipFree(p);
   if (b->h_st) hipHostFree(b->h_st);

@@ -521,9 +521,15 @@ inline int tl_grid(size_t n) {
 
 // Aggregate width and grid: the smallest ha whose aggregate grid has at most
 // target coarse unknowns and, where that costs at most a doubling of ha, at
-// most 63 aggregates per row (the bandwidth the banded setup handles).
+// most 63 aggregates per row (the bandwidth the banded setup handles).  The
+// default target, sqrt(2 N) within [16, 4096], keeps the coarse GEMV (nc^2
+// values per iteration) well below the fine iteration's bytes (~24 N).
 TwoLevel mf6hip_tl_shape(const Layout &L, int target) {
-  if (target <= 0) target = 4096;
+  if (target <= 0) {
+    target = (int)std::sqrt(2.0 * (double)L.N);
+    if (target < 16) target = 16;
+    if (target > 4096) target = 4096;
+  }
   TwoLevel T{};
   int ha = (int)std::ceil(std::sqrt((double)L.NRC / (double)target));
   if (ha < 1) ha = 1;
@@ -642,6 +648,31 @@ int mf6hip_tl_setup(Mf6Handle *h) {
                      std::to_string(info) + "); the system must be symmetric definite");
     return MF6HIP_EINVAL;
   }
+  return MF6HIP_OK;
+}
+
+// The banded coarse setup of one system whose FP64 planes are A (layout
+// L), into caller-owned arrays: cmask (2 HP), S3 (3 nc), Lc / Lr (64 nc),
+// scratch G (nc ld doubles) and the FP32 inverse A32 (nc ld).  For the
+// ensemble solver; aggregate grids wider than 63 are not supported there.
+// Returns the factorisation's info code through *info (0: definite).
+int mf6hip_tl_setup_slice(const Layout &L, const double *A, const TwoLevel &T,
+                          unsigned *cmask, double *S3, double *Lc, double *Lr, double *G,
+                          float *A32, int *dinfo, int *info, hipStream_t s) {
+  if (T.naj > 63) {
+    mf6hip_set_error("ensemble two-level: aggregate grid wider than 63");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  k_tl_cmask<<<tl_grid(2 * (size_t)L.HP), MF6_BLOCK, 0, s>>>(L, A, cmask);
+  k_tl_build<<<T.nc, MF6_BLOCK, 0, s>>>(L, A, cmask, T, S3);
+  k_tl_band_chol<<<1, TL_CHOL_THREADS, 0, s>>>(T, S3, Lc, Lr, dinfo);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (*info != 0) return MF6HIP_OK;
+  k_tl_band_inv<<<(T.nc + 3) / 4, 256, 0, s>>>(T, Lc, Lr, G);
+  k_tl_finish_rows<<<tl_grid((size_t)T.nc * T.ld), MF6_BLOCK, 0, s>>>(T, G, nullptr, A32);
+  HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }
 

@@ -153,6 +153,10 @@ struct Mf6Handle {
 };
 
 // two-level preconditioner (mf6hip_coarse.hip)
+TwoLevel mf6hip_tl_shape(const Layout &L, int target);
+int mf6hip_tl_setup_slice(const Layout &L, const double *A, const TwoLevel &T,
+                          unsigned *cmask, double *S3, double *Lc, double *Lr, double *G,
+                          float *A32, int *dinfo, int *info, hipStream_t s);
 int mf6hip_tl_setup(Mf6Handle *h);
 void mf6hip_tl_free(Mf6Handle *h);
 template <typename TV>

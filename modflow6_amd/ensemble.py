@@ -62,7 +62,11 @@ def fixed_system(case):
 class Ensemble:
     """ctypes handle of one device ensemble."""
 
-    def __init__(self, cases, log=None, keep_systems=False):
+    def __init__(self, cases, log=None, keep_systems=False, preconditioner=0,
+                 systems=None):
+        """preconditioner 0: ILU0; 2: ILU0 plus the aggregation coarse
+        correction (default coarse size per realisation).  systems: the
+        (amat, rhs, x0) of each realisation when already formulated."""
         self.L = L = _lib.load()
         self.cases = list(cases)
         self.systems = [] if keep_systems else None
@@ -72,13 +76,14 @@ class Ensemble:
         for c in self.cases:
             s = ImsLinearSettings(dvclose=c.ims.inner_dvclose, rclose=c.ims.inner_rclose,
                                   iter1=c.ims.inner_maximum, ilinmeth=1,
-                                  precision=_lib.MF6HIP_MIXED)
+                                  precision=_lib.MF6HIP_MIXED,
+                                  preconditioner=preconditioner)
             cs = s.to_c(c.nlay, c.nrow, c.ncol)
             _lib.check(L.mf6hip_batch_add(self.h, C.byref(cs)), "mf6hip_batch_add")
         _lib.check(L.mf6hip_batch_finalize(self.h), "mf6hip_batch_finalize")
         t0 = time.perf_counter()
         for i, c in enumerate(self.cases):
-            amat, rhs, x0 = fixed_system(c)
+            amat, rhs, x0 = systems[i] if systems is not None else fixed_system(c)
             if self.systems is not None:
                 self.systems.append((amat, rhs, x0))
             ia, _ = dis_structure(c.nlay, c.nrow, c.ncol)
@@ -115,9 +120,9 @@ class Ensemble:
 
 
 def coarse_size_for(case):
-    """Coarse size of the two-level preconditioner for one realisation: one
-    aggregate per 16 grid columns, at most 4096."""
-    return int(min(4096, max(16, case.nrow * case.ncol // 16)))
+    """Coarse size of the two-level preconditioner for one realisation: the
+    library's default, sqrt(2 N) within [16, 4096]."""
+    return 0
 
 
 def solve_one_by_one(cases, systems, log=None, precision=_lib.MF6HIP_MIXED,
@@ -133,7 +138,7 @@ def solve_one_by_one(cases, systems, log=None, precision=_lib.MF6HIP_MIXED,
         s = ImsLinearSettings(dvclose=c.ims.inner_dvclose, rclose=c.ims.inner_rclose,
                               iter1=c.ims.inner_maximum, ilinmeth=1,
                               precision=precision, preconditioner=preconditioner,
-                              coarse_size=coarse_size_for(c) if preconditioner == 2 else 0)
+                              coarse_size=coarse_size_for(c))
         ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
         sv = HipLinearSolver().initialize(ia, ja, s, c.nlay, c.nrow, c.ncol)
         x = x0.copy()

@@ -40,7 +40,8 @@ class ImsLinearSettings:
     kernel_variant: int = 0      # 0 fused CG (symmetric systems), 1 unfused
     preconditioner: int = 0      # 0 ILU0/MILU0, 1 ILU0 + learned correction (FCG),
                                  # 2 two-level (ILU0 + aggregation coarse correction)
-    coarse_size: int = 0         # preconditioner 2: most coarse unknowns (0 = 4096)
+    coarse_size: int = 0         # preconditioner 2: most coarse unknowns
+                                 # (0: sqrt(2 N) within [16, 4096])
 
     def to_c(self, nlay, nrow, ncol):
         return _lib.ImsSettingsC(
