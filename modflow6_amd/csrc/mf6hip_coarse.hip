@@ -243,12 +243,6 @@ __global__ void __launch_bounds__(TL_CHOL_THREADS) k_tl_band_chol(TwoLevel T,
   if (threadIdx.x == 0) info[0] = bad;
 }
 
-__device__ inline double wsum_all(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
 // lane `lane`'s value in every lane (two 32-bit readlanes; lane is uniform)
 __device__ inline double readlane64(double v, int lane) {
   const unsigned long long u = __double_as_longlong(v);
