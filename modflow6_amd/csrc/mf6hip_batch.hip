@@ -17,6 +17,14 @@
 // Per system the arithmetic is that of the single-system fused CG
 // (ims_base_cg, src/Solution/LinearMethods/ImsLinearBase.f90:30-240, on the
 // column red-black ordering), including the IMS convergence test.
+//
+// With preconditioner 2 every system also carries the two-level coarse
+// correction of the single-system solver (mf6hip_coarse.hip): its coarse
+// inverse is built when the system is staged, the black / red backward
+// kernels write its per-column residual sums, kb_tl_gather / kb_tl_gemv
+// form yc = Ainv P^T d over per-system tasks of four coarse rows, the
+// direction update adds P yc, the round kernel adds rc.yc to rho, and each
+// residual replacement restarts the directions.
 #include <cmath>
 #include <cstring>
 #include <string>
