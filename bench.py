@@ -288,6 +288,13 @@ def learned_workload(args):
     print(json.dumps(summary))
 
 
+# mixed-precision methods, one-level and two-level (additive *_2l,
+# deflated *_dl)
+TWO_LEVEL_MIXED = ("mixed", "mixed16", "mixed_2l", "mixed16_2l", "mixed_dl", "mixed16_dl")
+COARSE_FORM = {"_2l": " + aggregation coarse correction (additive)",
+               "_dl": " + aggregation coarse correction (deflated)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,8 +302,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="both",
                     help="comma list of fp64, mixed (FP32 factor), mixed16 (FP16 "
-                         "factor), fp64_2l, mixed_2l, mixed16_2l (two-level); both: "
-                         "fp64, mixed, mixed16, fp64_2l, mixed16_2l")
+                         "factor), fp64_2l, mixed_2l, mixed16_2l (two-level, additive), "
+                         "fp64_dl, mixed_dl, mixed16_dl (two-level, deflated); both: "
+                         "fp64, mixed, mixed16, fp64_2l, mixed16_2l, mixed16_dl")
     ap.add_argument("--coarse-size", type=int, default=0,
                     help="two-level: most coarse unknowns (0 = sqrt(2 N) within [16, 4096])")
     ap.add_argument("--shape", default="10,1000,1000")
@@ -353,13 +361,14 @@ def main():
     N = case.nodes
     nja = N + 2 * (nlay * (nrow * (ncol - 1) + (nrow - 1) * ncol) + (nlay - 1) * nrow * ncol)
 
-    methods = (["fp64", "mixed", "mixed16", "fp64_2l", "mixed16_2l"]
+    methods = (["fp64", "mixed", "mixed16", "fp64_2l", "mixed16_2l", "mixed16_dl"]
                if args.precision == "both" else args.precision.split(","))
     prec = {"fp64": _lib.MF6HIP_FP64, "mixed": _lib.MF6HIP_MIXED,
             "mixed16": _lib.MF6HIP_MIXED_FP16}
     res = {}
+    two_level = {"_2l": 2, "_dl": 3}
     for m in methods:
-        base = m[:-3] if m.endswith("_2l") else m
+        base = m[:-3] if m[-3:] in two_level else m
         if base not in prec:
             ap.error(f"unknown method {m}")
         # mixed16: ILU0 planes in FP16 (power-of-two scaled), SpMV and Krylov
@@ -368,7 +377,7 @@ def main():
         res[m] = run_precision(case, prec[base], args.steps, args.warmup, m,
                                replace=args.replace,
                                timing=not args.no_timing if base == "fp64" else True,
-                               preconditioner=2 if m.endswith("_2l") else 0,
+                               preconditioner=two_level.get(m[-3:], 0),
                                coarse_size=args.coarse_size)
 
     def mean(lst, key):
@@ -382,7 +391,7 @@ def main():
                           spmv_us=mean(recs, "spmv_ms") * 1e3,
                           true_residual_inf=true_residual(case, amat, rhs, heads),
                           converged=bool(recs[-1]["icnvg"]))
-    for mk in ("mixed", "mixed16", "fp64_2l", "mixed_2l", "mixed16_2l"):
+    for mk in TWO_LEVEL_MIXED + ("fp64_2l", "fp64_dl"):
         if "fp64" in res and mk in res:
             dh = float(np.max(np.abs(res[mk][1] - res["fp64"][1])))
             summary[mk]["max_head_diff_vs_fp64_m"] = dh
@@ -407,11 +416,9 @@ def main():
         m = summary[k]
         return (m["converged"] and m["true_residual_inf"] <= 1e-6 and
                 m.get("max_head_diff_vs_oracle_m", 0.0) <= 1e-5)
-    mixed = [k for k in ("mixed", "mixed16", "mixed_2l", "mixed16_2l")
-             if k in summary and valid(k)]
+    mixed = [k for k in TWO_LEVEL_MIXED if k in summary and valid(k)]
     head = (min(mixed, key=lambda k: summary[k]["s_per_solve"]) if mixed else
-            next(k for k in ("mixed", "mixed16", "mixed_2l", "mixed16_2l", "fp64",
-                             "fp64_2l") if k in summary))
+            next(k for k in TWO_LEVEL_MIXED + ("fp64", "fp64_2l", "fp64_dl") if k in summary))
     value = summary[head]["s_per_solve"]
 
     # roofline of the dominant kernel, k_f_spmv (CG SpMV q = A p): symmetric
@@ -473,7 +480,7 @@ def main():
         "data": "synthetic (seeded lognormal K, CHD/RCH/WEL)",
         "config": {"workload": f"steady confined GWF {nlay}x{nrow}x{ncol} DIS "
                                f"({N} cells, {nja} nonzeros), CG + column red-black ILU0"
-                               f"{' + aggregation coarse correction' if head.endswith('_2l') else ''}, "
+                               f"{COARSE_FORM.get(head[-3:], '')}, "
                                f"rclose = dvclose = 1e-6",
                    "solver": head},
         "methods": summary,
@@ -485,7 +492,7 @@ def main():
     if "fp64" in summary and "mixed16" in summary:
         line["speedup_mixed16_vs_fp64"] = (summary["fp64"]["s_per_solve"] /
                                            summary["mixed16"]["s_per_solve"])
-    for mk in ("fp64_2l", "mixed_2l", "mixed16_2l"):
+    for mk in ("fp64_2l", "mixed_2l", "mixed16_2l", "fp64_dl", "mixed_dl", "mixed16_dl"):
         if "fp64" in summary and mk in summary:
             line[f"speedup_{mk}_vs_fp64"] = (summary["fp64"]["s_per_solve"] /
                                              summary[mk]["s_per_solve"])

@@ -74,8 +74,10 @@ typedef struct mf6hip_ims_settings {
                            flexible CG (CG only); 2: two-level, ILU0 / MILU0
                            plus an additive coarse correction on column
                            aggregates (symmetric systems, CG, fused kernels,
-                           any precision) */
-  int coarse_size;      /* preconditioner 2: most coarse unknowns (aggregates
+                           any precision); 3: the same coarse space in the
+                           deflated form, z = M^-1 (r - A P yc) + P yc with
+                           yc = Ac^-1 P^T r (same conditions as 2) */
+  int coarse_size;      /* preconditioner 2, 3: most coarse unknowns (aggregates
                            of ha x ha columns, all layers); 0 = sqrt(2 N)
                            within [16, 4096]; at most 6144 */
 } mf6hip_ims_settings;

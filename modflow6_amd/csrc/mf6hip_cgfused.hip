@@ -135,14 +135,71 @@ enum { M_D_R = 0, M_D_B = 1, M_P = 2 };
 // precondition kernels
 // ---------------------------------------------------------------------------
 
-template <typename TM, typename TV, typename TP>
+// Deflated two-level preconditioner (preconditioner = 3): the sweeps run on
+// r - A v, v = P yc (yc the coarse correction of this iteration's residual).
+// For a coupled cell n of aggregate c,
+//   (A v)_n = s_n yc_c + sum over coupled horizontal neighbours m in another
+//             aggregate c' of A(n, m) (yc_c' - yc_c),
+// s_n = A(n, n) + the couplings of n to coupled cells.  Decoupled cells have
+// s = 0 and no stored edge couplings, so (A v)_n = 0 there, as in A v.
+// ew: couplings across the aggregate edges in j, [(k nrow + i) 2 naj + 2 aj
+// + e] (e = 0: to j - 1 at j = aj ha; e = 1: to j + 1 at j = aj ha + ha - 1);
+// ns: across the edges in i, [(k 2 nai + 2 ai + e) ncol + j].
+struct DeflPlanes {
+  const float *s = nullptr, *ew = nullptr, *ns = nullptr;
+};
+
+// Branch-free: a column off an edge reads the slot of its own aggregate row
+// (always in range) and weights it by a zero difference, so every lane
+// issues the same loads and none waits inside a divergent branch.
+template <typename TV>
+struct DeflCol {
+  TV y, dw, de, dn, ds;   // yc of the column's aggregate; yc of a neighbour minus it
+  int ew0, ns0;           // row offsets into ew / ns at layer 0
+
+  __device__ DeflCol() {}
+  __device__ DeflCol(const TV *__restrict__ yc, const TwoLevel &T, const Layout &L, int i,
+                     int j) {
+    const int ai = i / T.ha, aj = j / T.ha, c = ai * T.naj + aj;
+    const int ri = i - ai * T.ha, rj = j - aj * T.ha;
+    const bool w = rj == 0 && j > 0, e = rj == T.ha - 1 && j + 1 < L.ncol;
+    const bool n = ri == 0 && i > 0, so = ri == T.ha - 1 && i + 1 < L.nrow;
+    y = yc[c];
+    dw = yc[w ? c - 1 : c] - y;
+    de = yc[e ? c + 1 : c] - y;
+    dn = yc[n ? c - T.naj : c] - y;
+    ds = yc[so ? c + T.naj : c] - y;
+    ew0 = i * 2 * T.naj + 2 * aj;
+    ns0 = 2 * ai * L.ncol + j;
+  }
+  // (A v) at layer k, cell position pos
+  __device__ TV av(const float *__restrict__ sp, const float *__restrict__ ewp,
+                   const float *__restrict__ nsp, const TwoLevel &T, const Layout &L, int k,
+                   int pos) const {
+    const size_t ke = (size_t)k * L.nrow * 2 * T.naj + ew0;
+    const size_t kn = (size_t)k * 2 * T.nai * L.ncol + ns0;
+    TV r = (TV)sp[pos] * y;
+    r = r + (TV)ewp[ke] * dw;
+    r = r + (TV)ewp[ke + 1] * de;
+    r = r + (TV)nsp[kn] * dn;
+    r = r + (TV)nsp[kn + L.ncol] * ds;
+    return r;
+  }
+};
+
+template <typename TM, typename TV, typename TP, bool DEFL>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__restrict__ A,
                                                          const TP *__restrict__ piv,
                                                          TV *__restrict__ d,
                                                          const TV *__restrict__ q,
                                                          TV *__restrict__ zf, double *part,
                                                          MaxLoc *mpart, const KState *st,
-                                                         const FState *fs, int nblk) {
+                                                         const FState *fs, int nblk,
+                                                         const TV *__restrict__ yc,
+                                                         const float *__restrict__ dsp,
+                                                         const float *__restrict__ dew,
+                                                         const float *__restrict__ dns,
+                                                         TwoLevel T) {
   const int lb = xcd_block();
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
@@ -156,14 +213,20 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__r
     int l = col_l(L, 0, h2);
     TV zup = (TV)0;
     TV lup = (TV)0;   // A(n, up) * piv(up)
+    DeflCol<TV> dc;
+    if constexpr (DEFL) {
+      const int i = l / L.ncol;
+      dc = DeflCol<TV>(yc, T, L, i, l - i * L.ncol);
+    }
     for (int k = 0; k < L.nlay; k++) {
       int pos = k * L.HP + h2;
       TV dv = d[pos];
       dv = dv - alpha * q[pos];
+      TV tv = dv;
+      if constexpr (DEFL) tv = tv - dc.av(dsp, dew, dns, T, L, k, pos);
       d[pos] = dv;
       md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
       ss = ss + (double)dv * (double)dv;
-      TV tv = dv;
       if (k > 0) tv = tv - lup * zup;
       zf[pos] = tv;
       zup = tv;
@@ -185,7 +248,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwd(Layout L, const TM *__r
 // then the backward sweep (cell below) over the same column.  Staging the
 // forward values in z instead of registers keeps the register count low
 // and the kernel memory-parallel.
-template <typename TM, typename TV, typename TP>
+template <typename TM, typename TV, typename TP, bool DEFL>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__restrict__ A,
                                                        const TP *__restrict__ piv,
                                                        TV *__restrict__ d,
@@ -195,7 +258,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
                                                        MaxLoc *mpart, const KState *st,
                                                        const FState *fs, int nblk, TV sc,
                                                        const unsigned *__restrict__ cmask,
-                                                       double *__restrict__ colsum) {
+                                                       double *__restrict__ colsum,
+                                                       const TV *__restrict__ yc,
+                                                       const float *__restrict__ dsp,
+                                                       const float *__restrict__ dew,
+                                                       const float *__restrict__ dns,
+                                                       TwoLevel T) {
   const int lb = xcd_block();
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
@@ -208,29 +276,41 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_black(Layout L, const TM *__res
        h2 += gridDim.x * blockDim.x) {
     int l = col_l(L, 1, h2);
     int i = l / L.ncol, j = l - i * L.ncol;
-    int ib = i > 0 ? hn(l, -L.ncol) : -1;
-    int il = j > 0 ? hn(l, -1) : -1;
-    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    // neighbours off the grid read position 0 of their plane (in range) and
+    // their term is dropped by a select: every lane issues the same loads
+    // and none waits inside a branch
+    const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+    const int ib = hb ? hn(l, -L.ncol) : 0;
+    const int il = hl ? hn(l, -1) : 0;
+    const int ir = hr ? hn(l, 1) : 0;
+    const int ifr = hf ? hn(l, L.ncol) : 0;
     TV zup = (TV)0, lup = (TV)0;
     const unsigned cm = colsum ? cmask[L.HP + h2] : 0u;
     double cs = 0.0;
+    DeflCol<TV> dc;
+    if constexpr (DEFL) dc = DeflCol<TV>(yc, T, L, i, j);
     for (int k = 0; k < L.nlay; k++) {
       int pos = L.H + k * L.HP + h2;
       int rb = k * L.HP;
+      // L(n, r) = A(n, r) piv(r); A(n, back) = front(back), A(n, left) =
+      // right(left), A(n, right) = right(n), A(n, front) = front(n).  The
+      // neighbour terms come first so that their loads go out with d's.
+      const TV tb = (TV)((TV)A[5 * NP + rb + ib] * (TV)ldp(piv, rb + ib)) * zf[rb + ib];
+      const TV tl = (TV)((TV)A[4 * NP + rb + il] * (TV)ldp(piv, rb + il)) * zf[rb + il];
+      const TV tr = (TV)((TV)A[4 * NP + pos] * (TV)ldp(piv, rb + ir)) * zf[rb + ir];
+      const TV tf = (TV)((TV)A[5 * NP + pos] * (TV)ldp(piv, rb + ifr)) * zf[rb + ifr];
       TV dv = d[pos];
       dv = dv - alpha * q[pos];
+      TV tv = dv;
+      if constexpr (DEFL) tv = tv - dc.av(dsp, dew, dns, T, L, k, pos);
       d[pos] = dv;
-      if ((cm >> k) & 1u) cs = cs + (double)dv;
+      cs = ((cm >> k) & 1u) ? cs + (double)dv : cs;
       md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
       ss = ss + (double)dv * (double)dv;
-      TV tv = dv;
-      // L(n, r) = A(n, r) piv(r); A(n, back) = front(back), A(n, left) =
-      // right(left), A(n, right) = right(n), A(n, front) = front(n)
-      if (ib >= 0) tv = tv - (TV)((TV)A[5 * NP + rb + ib] * (TV)ldp(piv, rb + ib)) * zf[rb + ib];
-      if (il >= 0) tv = tv - (TV)((TV)A[4 * NP + rb + il] * (TV)ldp(piv, rb + il)) * zf[rb + il];
-      if (ir >= 0) tv = tv - (TV)((TV)A[4 * NP + pos] * (TV)ldp(piv, rb + ir)) * zf[rb + ir];
-      if (ifr >= 0) tv = tv - (TV)((TV)A[5 * NP + pos] * (TV)ldp(piv, rb + ifr)) * zf[rb + ifr];
+      tv = hb ? tv - tb : tv;
+      tv = hl ? tv - tl : tv;
+      tv = hr ? tv - tr : tv;
+      tv = hf ? tv - tf : tv;
       if (k > 0) tv = tv - lup * zup;
       z[pos] = tv;
       zup = tv;
@@ -274,10 +354,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
        h2 += gridDim.x * blockDim.x) {
     int l = col_l(L, 0, h2);
     int i = l / L.ncol, j = l - i * L.ncol;
-    int ib = i > 0 ? hn(l, -L.ncol) : -1;
-    int il = j > 0 ? hn(l, -1) : -1;
-    int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-    int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+    // off-grid neighbours: in-range position 0, term dropped by a select
+    const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+    const int ib = hb ? hn(l, -L.ncol) : 0;
+    const int il = hl ? hn(l, -1) : 0;
+    const int ir = hr ? hn(l, 1) : 0;
+    const int ifr = hf ? hn(l, L.ncol) : 0;
     TV zdn = (TV)0;
     const unsigned cm = colsum ? cmask[h2] : 0u;
     double cs = 0.0;
@@ -286,16 +368,20 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
       int bb = L.H + k * L.HP;
       TV tv = zf[pos] * sc;
       if (k < L.nlay - 1) tv = tv - (TV)A[6 * NP + pos] * zdn;
-      if (ib >= 0) tv = tv - (TV)A[5 * NP + bb + ib] * z[bb + ib];
-      if (il >= 0) tv = tv - (TV)A[4 * NP + bb + il] * z[bb + il];
-      if (ir >= 0) tv = tv - (TV)A[4 * NP + pos] * z[bb + ir];
-      if (ifr >= 0) tv = tv - (TV)A[5 * NP + pos] * z[bb + ifr];
+      const TV tb = (TV)A[5 * NP + bb + ib] * z[bb + ib];
+      const TV tl = (TV)A[4 * NP + bb + il] * z[bb + il];
+      const TV tr = (TV)A[4 * NP + pos] * z[bb + ir];
+      const TV tf = (TV)A[5 * NP + pos] * z[bb + ifr];
+      tv = hb ? tv - tb : tv;
+      tv = hl ? tv - tl : tv;
+      tv = hr ? tv - tr : tv;
+      tv = hf ? tv - tf : tv;
       tv = tv * (TV)ldp(piv, pos);
       z[pos] = tv;
       zdn = tv;
       const TV dv = d[pos];
       rho = rho + (double)dv * (double)tv;
-      if ((cm >> k) & 1u) cs = cs + (double)dv;
+      cs = ((cm >> k) & 1u) ? cs + (double)dv : cs;
     }
     if (colsum) colsum[l] = cs;
   }
@@ -452,11 +538,13 @@ static void launch_f_pupdate(const Layout &L, int nb, hipStream_t s, const TV *z
 // unrolled over a compile-time bound NLMAX in chunks of four layers whose
 // loads are all issued before their arithmetic (more loads in flight per
 // thread than the runtime-bounded loop gets).
-template <typename TM, typename TV, int NLMAX>
+template <typename TM, typename TV, int NLMAX, bool CS>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__restrict__ A,
                                                       const TV *__restrict__ p,
                                                       TV *__restrict__ q, double *part,
-                                                      const KState *st, int nblk) {
+                                                      const KState *st, int nblk,
+                                                      const unsigned *__restrict__ cmask,
+                                                      double *__restrict__ colsq) {
   constexpr int CH = 4;
   const int lb = xcd_block();
   __shared__ double lds[8];
@@ -484,6 +572,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
     TV pprev = (TV)0;
     TV pcur = p[own];
     TV aup = (TV)0;
+    unsigned cm = 0u;
+    if constexpr (CS) cm = cmask[c * L.HP + h2];
+    double cs = 0.0;
 #pragma unroll
     for (int k0 = 0; k0 < NLMAX; k0 += CH) {
       if (k0 >= L.nlay) break;
@@ -521,6 +612,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
           if (hf) tq = tq + a5[u] * pf[u];
           if (k < L.nlay - 1) tq = tq + a6[u] * pn[u];
           q[own + k * L.HP] = tq;
+          if constexpr (CS)
+            if ((cm >> k) & 1u) cs = cs + (double)tq;
           s = s + (double)pcur * (double)tq;
           aup = a6[u];
           pprev = pcur;
@@ -528,6 +621,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
         }
       }
     }
+    if constexpr (CS) colsq[c * L.HP + h2] = cs;
   }
   s = bsum(s, lds);
   if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
@@ -543,11 +637,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
 // the single-column kernel would leave the XCDs that receive the upper half
 // of the logical blocks idle); slots gridDim.x .. nblk-1 of its p.q partials
 // (read by k_f_alpha over nblk slots) are cleared.
-template <typename TV, int NLMAX>
+template <typename TV, int NLMAX, bool CS>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__restrict__ A,
                                                        const TV *__restrict__ p,
                                                        TV *__restrict__ q, double *part,
-                                                       const KState *st, int nblk) {
+                                                       const KState *st, int nblk,
+                                                       const unsigned *__restrict__ cmask,
+                                                       double *__restrict__ colsq) {
   using V2 = typename Vec2<TV>::type;
   constexpr int CH = 4;
   const int lb = xcd_block();
@@ -570,6 +666,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
     const int own = c * L.H + h2, oth = (1 - c) * L.H;
     V2 pcur = *reinterpret_cast<const V2 *>(p + own);
     V2 pprev{(TV)0, (TV)0}, aup{(TV)0, (TV)0};
+    unsigned cm0 = 0u, cm1 = 0u;
+    if constexpr (CS) {
+      cm0 = cmask[c * L.HP + h2];
+      cm1 = cmask[c * L.HP + h2 + 1];
+    }
+    double cs0 = 0.0, cs1 = 0.0;
 #pragma unroll
     for (int k0 = 0; k0 < NLMAX; k0 += CH) {
       if (k0 >= L.nlay) break;
@@ -629,6 +731,10 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
             t1 = t1 + a6[u].y * pn[u].y;
           }
           *reinterpret_cast<V2 *>(q + own + k * L.HP) = V2{t0, t1};
+          if constexpr (CS) {
+            if ((cm0 >> k) & 1u) cs0 = cs0 + (double)t0;
+            if ((cm1 >> k) & 1u) cs1 = cs1 + (double)t1;
+          }
           s = s + (double)pcur.x * (double)t0;
           s = s + (double)pcur.y * (double)t1;
           aup = a6[u];
@@ -637,6 +743,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
         }
       }
     }
+    if constexpr (CS) *reinterpret_cast<double2 *>(colsq + c * L.HP + h2) = double2{cs0, cs1};
   }
   s = bsum(s, lds);
   if (threadIdx.x == 0) {
@@ -645,29 +752,43 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
   }
 }
 
-template <typename TM, typename TV>
-static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
-                          TV *q, double *part, const KState *st) {
+// cmask / colsq given (the deflated two-level form): the SpMV also writes
+// each column's sum of q over its coupled cells, colsq[c HP + h2] (colour
+// layout, so that a pair's two sums are one store).
+template <typename TM, typename TV, bool CS>
+static void launch_f_spmv_t(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
+                            TV *q, double *part, const KState *st, const unsigned *cmask,
+                            double *colsq) {
   if (spmv_pairable(L)) {
     const int g2 = pair_grid(L, nb);
     if (L.nlay <= 4)
-      k_f_spmv2<TV, 4><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 4, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
     else if (L.nlay <= 8)
-      k_f_spmv2<TV, 8><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 8, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
     else if (L.nlay <= 16)
-      k_f_spmv2<TV, 16><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 16, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
     else
-      k_f_spmv2<TV, 32><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+      k_f_spmv2<TV, 32, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
     return;
   }
   if (L.nlay <= 4)
-    k_f_spmv<TM, TV, 4><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    k_f_spmv<TM, TV, 4, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
   else if (L.nlay <= 8)
-    k_f_spmv<TM, TV, 8><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    k_f_spmv<TM, TV, 8, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
   else if (L.nlay <= 16)
-    k_f_spmv<TM, TV, 16><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    k_f_spmv<TM, TV, 16, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
   else
-    k_f_spmv<TM, TV, 32><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb);
+    k_f_spmv<TM, TV, 32, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+}
+
+template <typename TM, typename TV>
+static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
+                          TV *q, double *part, const KState *st,
+                          const unsigned *cmask = nullptr, double *colsq = nullptr) {
+  if (colsq)
+    launch_f_spmv_t<TM, TV, true>(L, nb, s, A, p, q, part, st, cmask, colsq);
+  else
+    launch_f_spmv_t<TM, TV, false>(L, nb, s, A, p, q, part, st, nullptr, nullptr);
 }
 
 // x += apend_x p (the last iteration's head update), then no update pending
@@ -939,13 +1060,33 @@ static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV 
                       TV *z, KState *st, FState *fs, int nbc, int nb, TV sc = (TV)1) {
   const Layout &L = h->L;
   hipStream_t s = h->stream;
-  const bool tl = h->s.preconditioner == 2;
+  const bool tl = h->s.preconditioner >= 2;
   const unsigned *cm = tl ? h->tl_cmask : nullptr;
   double *cs = tl ? h->tl_colsum : nullptr;
-  k_f_red_fwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, h->d_part, h->d_mpart,
-                                                    st, fs, nb);
-  k_f_black<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part, h->d_mpart,
-                                                  st, fs, nb, sc, cm, cs);
+  // deflated form: the sweeps take r - A P yc
+  const TV *yc = nullptr;
+  DeflPlanes D;
+  if (h->s.preconditioner == 3) {
+    yc = sizeof(TV) == sizeof(double) ? (const TV *)h->tl_yc64 : (const TV *)h->tl_yc32;
+    D.s = h->tl_s32;
+    D.ew = h->tl_ew;
+    D.ns = h->tl_ns;
+  }
+  if (yc) {
+    k_f_red_fwd<TM, TV, TP, true><<<nbc, MF6_BLOCK, 0, s>>>(
+        L, A, piv, d, q, zf, h->d_part, h->d_mpart, st, fs, nb, yc, D.s, D.ew, D.ns, h->tl);
+    k_f_black<TM, TV, TP, true><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part,
+                                                          h->d_mpart, st, fs, nb, sc, cm, cs,
+                                                          yc, D.s, D.ew, D.ns, h->tl);
+  } else {
+    k_f_red_fwd<TM, TV, TP, false><<<nbc, MF6_BLOCK, 0, s>>>(
+        L, A, piv, d, q, zf, h->d_part, h->d_mpart, st, fs, nb, nullptr, nullptr, nullptr,
+        nullptr, h->tl);
+    k_f_black<TM, TV, TP, false><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, q, zf, z, h->d_part,
+                                                           h->d_mpart, st, fs, nb, sc, cm, cs,
+                                                           nullptr, nullptr, nullptr, nullptr,
+                                                           h->tl);
+  }
   k_f_red_bwd<TM, TV, TP><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb, sc,
                                                     cm, cs);
 }
@@ -965,11 +1106,17 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
   KState *st = h->d_state;
   FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
   double *pbuf[2] = {h->p, h->p2};
-  const bool tl = h->s.preconditioner == 2;
+  const bool tl = h->s.preconditioner >= 2, defl = h->s.preconditioner == 3;
   for (int round = 0; round < n; round++) {
+    // additive form: the coarse correction of the residual the sweeps
+    // produce; deflated form: of the residual before the sweeps, from the
+    // last sweeps' column sums and the SpMV's (P^T r - alpha P^T q)
+    if (defl)
+      mf6hip_tl_enqueue<double>(h, (const double *)nullptr, h->tl_yc64, h->d_part + P_RHO_C * nb,
+                                nb, st, &fs->apend_d);
     launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, h->z, st, fs, nbc,
                                       nb);
-    if (tl)
+    if (tl && !defl)
       mf6hip_tl_enqueue<double>(h, (const double *)nullptr, h->tl_yc64, h->d_part + P_RHO_C * nb,
                                 nb, st);
     k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
@@ -983,7 +1130,8 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
       slot = h->last_spmv_n++;
       (void)hipEventRecord(h->spmv_ev[2 * slot], s);
     }
-    launch_f_spmv<double, double>(L, nb, s, h->A, pout, h->q, h->d_part, st);
+    launch_f_spmv<double, double>(L, nb, s, h->A, pout, h->q, h->d_part, st,
+                                  defl ? h->tl_cmask : nullptr, defl ? h->tl_colsq : nullptr);
     if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
     k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
   }
@@ -1023,6 +1171,7 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
   FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
   HIPCHK(hipMemsetAsync(fs, 0, sizeof(FState), s));
   HIPCHK(hipMemsetAsync(h->q, 0, sizeof(double) * L.NP, s));
+  if (h->s.preconditioner == 3) mf6hip_tl_colsum<double>(h, h->d);
   // one grid size for every kernel of a round, so that each partial array
   // has the same number of slots for the round kernel to reduce
   const int nbc = gridc(L.HP0);
@@ -1072,7 +1221,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
   // one-level solve does, lets the FP32 recurrences stall against the
   // coarse correction), so replacements come more often.
   const int R = h->s.inner_fp32_max > 0 ? h->s.inner_fp32_max
-                                        : (h->s.preconditioner == 2 ? 32 : 64);
+                                        : (h->s.preconditioner >= 2 ? 32 : 64);
   HIPCHK(hipMemsetAsync(fs, 0, sizeof(FState), s));
   HIPCHK(hipMemsetAsync(h->q32, 0, sizeof(float) * L.NP, s));
   HIPCHK(hipMemsetAsync(h->e32, 0, sizeof(float) * L.NP, s));
@@ -1082,15 +1231,19 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
   int round = 0, last_replace = 0;
   int total_budget = h->s.iter1;
   bool done = false;
-  const bool tl = h->s.preconditioner == 2;
+  const bool tl = h->s.preconditioner >= 2, defl = h->s.preconditioner == 3;
+  if (defl) mf6hip_tl_colsum<float>(h, h->r32);
   *icnvg = 0;
   while (!done) {
     int batch = 8;
     while (true) {
       for (int it = 0; it < batch; it++, round++) {
+        if (defl)
+          mf6hip_tl_enqueue<float>(h, (const float *)nullptr, h->tl_yc32,
+                                   h->d_part + P_RHO_C * nb, nb, st, &fs->apend_d);
         launch_pc<TMP, float, TP>(h, Apc, piv32, h->r32, h->q32, h->zf32, h->z32, st, fs, nbc,
                                   nb, sc);
-        if (tl)
+        if (tl && !defl)
           mf6hip_tl_enqueue<float>(h, (const float *)nullptr, h->tl_yc32,
                                    h->d_part + P_RHO_C * nb, nb, st);
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
@@ -1104,7 +1257,8 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
           slot = h->last_spmv_n++;
           (void)hipEventRecord(h->spmv_ev[2 * slot], s);
         }
-        launch_f_spmv<float, float>(L, nb, s, h->A32, pout, h->q32, h->d_part, st);
+        launch_f_spmv<float, float>(L, nb, s, h->A32, pout, h->q32, h->d_part, st,
+                                    defl ? h->tl_cmask : nullptr, defl ? h->tl_colsq : nullptr);
         if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
         k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
       }
@@ -1126,6 +1280,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
     k_r_accumulate<<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->x, h->e32);
     k_r_residual<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->r32, h->d_mpart);
     k_r_after<<<1, MF6_BLOCK, 0, s>>>(st, fs, h->d_mpart, nb, rout, tl ? 1 : 0);
+    if (defl) mf6hip_tl_colsum<float>(h, h->r32);
     double hr[2];
     HIPCHK(hipMemcpyAsync(hr, rout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -32,6 +32,13 @@
 // and the direction update adds the prolongation: p = z + P yc + beta p.
 // rho = d.(z + P yc) = d.z + rc.yc, so the round kernel adds the partials of
 // rc.yc to those of d.z.
+// Deflated form (preconditioner = 3): z = M^-1 (r - A P yc) + P yc with
+// yc = Ac^-1 P^T r, the coarse solve before the sweeps.  P^T r of the
+// iteration's new residual is P^T r_old - alpha P^T q: the sweeps' column
+// sums of r_old and the SpMV's column sums of q (k_tl_gather with colsq).
+// k_tl_defl_setup stores what A P yc needs per cell (the row sum over the
+// coupled cells and the couplings across aggregate edges), k_tl_colsum the
+// column sums of a replaced residual.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -410,15 +417,25 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_tl_restrict(Layout L, const TV *_
 // rc[I] = sum of colsum over the columns of aggregate I (fixed order)
 __global__ void __launch_bounds__(MF6_BLOCK) k_tl_gather(Layout L, const double *__restrict__ colsum,
                                                         TwoLevel T, double *__restrict__ rc,
-                                                        const KState *st) {
+                                                        const KState *st,
+                                                        const double *__restrict__ colsq = nullptr,
+                                                        const double *alpha = nullptr) {
   __shared__ double lds[MF6_BLOCK / 64];
   if (st && st->stop) return;
+  // colsq given (colour layout, c HP + l / 2): P^T (r - alpha q); alpha = 0
+  // leaves colsq unread
+  const double a = colsq ? *alpha : 0.0;
   for (int I = blockIdx.x; I < T.nc; I += gridDim.x) {
     const int ai = I / T.naj, aj = I - ai * T.naj;
     double s = 0.0;
     for (int t = threadIdx.x; t < T.ha * T.ha; t += blockDim.x) {
       const int i = ai * T.ha + t / T.ha, j = aj * T.ha + t % T.ha;
-      if (i < L.nrow && j < L.ncol) s = s + colsum[i * L.ncol + j];
+      if (i < L.nrow && j < L.ncol) {
+        const int l = i * L.ncol + j;
+        double v = colsum[l];
+        if (a != 0.0) v = v - a * colsq[((i + j) & 1) * L.HP + (l >> 1)];
+        s = s + v;
+      }
     }
     s = block_sum(s, lds);
     if (threadIdx.x == 0) rc[I] = s;
@@ -505,6 +522,73 @@ __global__ void k_tl_prolong(Layout L, const unsigned *__restrict__ cmask, TwoLe
   }
 }
 
+// colsum[l] = sum of d over the coupled cells of grid column l
+template <typename TV>
+__global__ void k_tl_colsum(Layout L, const TV *__restrict__ d, const unsigned *__restrict__ cmask,
+                            double *__restrict__ colsum) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L.HP0 + L.HP1;
+       t += gridDim.x * blockDim.x) {
+    const int c = t >= L.HP0;
+    const int h2 = t - c * L.HP0;
+    const unsigned m = cmask[c * L.HP + h2];
+    double sum = 0.0;
+    for (int k = 0; k < L.nlay; k++)
+      if ((m >> k) & 1u) sum = sum + (double)d[c * L.H + k * L.HP + h2];
+    colsum[col_l(L, c, h2)] = sum;
+  }
+}
+
+// Per cell n (FP64 planes A): s[n] = A(n, n) + sum of A(n, m) over the
+// coupled neighbours m (0 on decoupled cells), and the couplings A(n, m) to
+// coupled neighbours across aggregate edges into ew / ns (layouts as the
+// fused CG reads them: ew [(k nrow + i) 2 naj + 2 aj + e], e = 0 to j - 1
+// at the aggregate's first column, 1 to j + 1 at its last; ns [(k 2 nai +
+// 2 ai + e) ncol + j], the same in i).
+__global__ void k_tl_defl_setup(Layout L, const double *__restrict__ A,
+                                const unsigned *__restrict__ cmask, TwoLevel T,
+                                float *__restrict__ sdiag, float *__restrict__ ew,
+                                float *__restrict__ ns) {
+  const size_t NP = (size_t)L.NP;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L.HP0 + L.HP1;
+       t += gridDim.x * blockDim.x) {
+    const int c = t >= L.HP0;
+    const int h2 = t - c * L.HP0;
+    const int l = col_l(L, c, h2);
+    const int i = l / L.ncol, j = l - i * L.ncol;
+    const int ai = i / T.ha, aj = j / T.ha;
+    const int ri = i - ai * T.ha, rj = j - aj * T.ha;
+    const bool hb = i > 0, hf = i < L.nrow - 1, hl = j > 0, hr = j < L.ncol - 1;
+    const unsigned m = cmask[c * L.HP + h2];
+    const unsigned *om = cmask + (1 - c) * L.HP;
+    const unsigned mb = hb ? om[hn(l, -L.ncol)] : 0u, mf = hf ? om[hn(l, L.ncol)] : 0u;
+    const unsigned ml = hl ? om[hn(l, -1)] : 0u, mr = hr ? om[hn(l, 1)] : 0u;
+    for (int k = 0; k < L.nlay; k++) {
+      const size_t pos = (size_t)c * L.H + (size_t)k * L.HP + h2;
+      const bool own = (m >> k) & 1u;
+      const double ab = ((mb >> k) & 1u) ? A[2 * NP + pos] : 0.0;
+      const double al = ((ml >> k) & 1u) ? A[3 * NP + pos] : 0.0;
+      const double ar = ((mr >> k) & 1u) ? A[4 * NP + pos] : 0.0;
+      const double af = ((mf >> k) & 1u) ? A[5 * NP + pos] : 0.0;
+      const double au = (k > 0 && ((m >> (k - 1)) & 1u)) ? A[1 * NP + pos] : 0.0;
+      const double ad = (k + 1 < L.nlay && ((m >> (k + 1)) & 1u)) ? A[6 * NP + pos] : 0.0;
+      double sum = A[pos];
+      sum = sum + au;
+      sum = sum + ab;
+      sum = sum + al;
+      sum = sum + ar;
+      sum = sum + af;
+      sum = sum + ad;
+      sdiag[pos] = own ? (float)sum : 0.0f;
+      const size_t e0 = ((size_t)k * L.nrow + i) * 2 * T.naj + 2 * aj;
+      const size_t n0 = ((size_t)k * 2 * T.nai + 2 * ai) * L.ncol + j;
+      if (rj == 0 && hl) ew[e0] = own ? (float)al : 0.0f;
+      if (rj == T.ha - 1 && hr) ew[e0 + 1] = own ? (float)ar : 0.0f;
+      if (ri == 0 && hb) ns[n0] = own ? (float)ab : 0.0f;
+      if (ri == T.ha - 1 && hf) ns[n0 + L.ncol] = own ? (float)af : 0.0f;
+    }
+  }
+}
+
 inline int tl_grid(size_t n) {
   size_t g = (n + MF6_BLOCK - 1) / MF6_BLOCK;
   if (g > 65535) g = 65535;
@@ -565,6 +649,22 @@ int mf6hip_tl_alloc(Mf6Handle *h) {
     mf6hip_set_error("hipMalloc failed (two-level preconditioner)");
     return MF6HIP_EHIP;
   }
+  if (h->s.preconditioner == 3) {
+    const TwoLevel &T = h->tl;
+    const size_t new_ = (size_t)L.nlay * L.nrow * 2 * T.naj;
+    const size_t nns = (size_t)L.nlay * 2 * T.nai * L.ncol;
+    if (hipMalloc((void **)&h->tl_s32, sizeof(float) * (size_t)L.NP) != hipSuccess ||
+        hipMalloc((void **)&h->tl_ew, sizeof(float) * new_) != hipSuccess ||
+        hipMalloc((void **)&h->tl_ns, sizeof(float) * nns) != hipSuccess ||
+        hipMalloc((void **)&h->tl_colsq, sizeof(double) * 2 * (size_t)L.HP) != hipSuccess) {
+      mf6hip_set_error("hipMalloc failed (two-level preconditioner)");
+      return MF6HIP_EHIP;
+    }
+    HIPCHK(hipMemsetAsync(h->tl_s32, 0, sizeof(float) * (size_t)L.NP, h->stream));
+    HIPCHK(hipMemsetAsync(h->tl_ew, 0, sizeof(float) * new_, h->stream));
+    HIPCHK(hipMemsetAsync(h->tl_ns, 0, sizeof(float) * nns, h->stream));
+    HIPCHK(hipMemsetAsync(h->tl_colsq, 0, sizeof(double) * 2 * (size_t)L.HP, h->stream));
+  }
   rocblas_handle rb = nullptr;
   if (rocblas_create_handle(&rb) != rocblas_status_success) {
     mf6hip_set_error("rocblas_create_handle failed");
@@ -582,7 +682,8 @@ void mf6hip_tl_free(Mf6Handle *h) {
   if (h->tl_blas) rocblas_destroy_handle((rocblas_handle)h->tl_blas);
   h->tl_blas = nullptr;
   void *bufs[] = {h->tl_cmask, h->tl_G, h->tl_rc, h->tl_yc64, h->tl_yc32, h->tl_info,
-                  h->tl_A64, h->tl_A32, h->tl_colsum, h->tl_S3, h->tl_Lc, h->tl_Lr};
+                  h->tl_A64, h->tl_A32, h->tl_colsum, h->tl_S3, h->tl_Lc, h->tl_Lr,
+                  h->tl_s32, h->tl_ew, h->tl_ns, h->tl_colsq};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tl_cmask = nullptr;
@@ -591,6 +692,8 @@ void mf6hip_tl_free(Mf6Handle *h) {
   h->tl_info = nullptr;
   h->tl_colsum = nullptr;
   h->tl_S3 = h->tl_Lc = h->tl_Lr = nullptr;
+  h->tl_s32 = h->tl_ew = h->tl_ns = nullptr;
+  h->tl_colsq = nullptr;
 }
 
 // Coarse operator of the current FP64 planes: mask, S = -Ac, its inverse.
@@ -601,6 +704,9 @@ int mf6hip_tl_setup(Mf6Handle *h) {
   hipStream_t s = h->stream;
   k_tl_cmask<<<tl_grid(2 * (size_t)L.HP), MF6_BLOCK, 0, s>>>(L, h->A, h->tl_cmask);
   k_tl_build<<<T.nc, MF6_BLOCK, 0, s>>>(L, h->A, h->tl_cmask, T, h->tl_S3);
+  if (h->s.preconditioner == 3)
+    k_tl_defl_setup<<<tl_grid((size_t)L.HP0 + L.HP1), MF6_BLOCK, 0, s>>>(
+        L, h->A, h->tl_cmask, T, h->tl_s32, h->tl_ew, h->tl_ns);
   HIPCHK(hipGetLastError());
   int info = 0;
   if (T.naj <= 63) {
@@ -672,16 +778,19 @@ int mf6hip_tl_setup_slice(const Layout &L, const double *A, const TwoLevel &T,
 
 // rc = P^T d (from the preconditioner kernels' column sums when d is null,
 // else from d itself) ; yc = Ainv rc ; partial rc.yc into part[0 .. nslot)
+// With alpha (a device pointer, the deflated form): rc = P^T (d - alpha q)
+// from the column sums of d and of q.
 template <typename TV>
 void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslot,
-                       const KState *st) {
+                       const KState *st, const double *alpha) {
   const TwoLevel &T = h->tl;
   hipStream_t s = h->stream;
   int gr = T.nc < 4096 ? T.nc : 4096;
   if (d)
     k_tl_restrict<TV><<<gr, MF6_BLOCK, 0, s>>>(h->L, d, h->tl_cmask, T, h->tl_rc, st);
   else
-    k_tl_gather<<<gr, MF6_BLOCK, 0, s>>>(h->L, h->tl_colsum, T, h->tl_rc, st);
+    k_tl_gather<<<gr, MF6_BLOCK, 0, s>>>(h->L, h->tl_colsum, T, h->tl_rc, st,
+                                         alpha ? h->tl_colsq : nullptr, alpha);
   int gg = (T.nc + 3) / 4;
   if (gg > nslot) gg = nslot;
   if (gg < 1) gg = 1;
@@ -695,9 +804,21 @@ void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslo
 }
 
 template void mf6hip_tl_enqueue<double>(Mf6Handle *, const double *, double *, double *, int,
-                                        const KState *);
+                                        const KState *, const double *);
 template void mf6hip_tl_enqueue<float>(Mf6Handle *, const float *, float *, double *, int,
-                                       const KState *);
+                                       const KState *, const double *);
+
+// column sums of a (replaced) residual d over the coupled cells, for the
+// deflated form's first coarse solve after it
+template <typename TV>
+void mf6hip_tl_colsum(Mf6Handle *h, const TV *d) {
+  const Layout &L = h->L;
+  k_tl_colsum<TV><<<tl_grid((size_t)L.HP0 + L.HP1), MF6_BLOCK, 0, h->stream>>>(
+      L, d, h->tl_cmask, h->tl_colsum);
+}
+
+template void mf6hip_tl_colsum<double>(Mf6Handle *, const double *);
+template void mf6hip_tl_colsum<float>(Mf6Handle *, const float *);
 
 // z += P Ainv P^T r on device vectors (FP64): the coarse part of M^-1 r for
 // the parity helper mf6hip_apply_op

@@ -148,6 +148,11 @@ struct Mf6Handle {
   double *tl_S3 = nullptr;       // compact rows of S = -Ac (diagonal, west, north)
   double *tl_Lc = nullptr;       // banded Cholesky factor of S by columns, nc x (naj + 1)
   double *tl_Lr = nullptr;       // the same factor by rows
+  // deflated form (preconditioner = 3): per-cell row sums over the coupled
+  // cells, couplings across aggregate edges, per-column sums of q (colour
+  // layout, c HP + h2)
+  float *tl_s32 = nullptr, *tl_ew = nullptr, *tl_ns = nullptr;
+  double *tl_colsq = nullptr;
   void *tl_blas = nullptr;    // rocblas_handle
   int spmv_calls = 0;
 };
@@ -161,7 +166,9 @@ int mf6hip_tl_setup(Mf6Handle *h);
 void mf6hip_tl_free(Mf6Handle *h);
 template <typename TV>
 void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslot,
-                       const KState *st);
+                       const KState *st, const double *alpha = nullptr);
+template <typename TV>
+void mf6hip_tl_colsum(Mf6Handle *h, const TV *d);
 int mf6hip_tl_apply64(Mf6Handle *h, const double *r, double *z);
 
 // solver entry points used by the GWF part

@@ -1334,7 +1334,7 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
   CK(ilu_factor(h));
   int sym = 0;
   if (h->s.ilinmeth == 1 && h->s.north == 0) CK(mf6hip_is_symmetric(h, &sym));
-  if (h->s.preconditioner == 2) {
+  if (h->s.preconditioner >= 2) {
     if (!sym) {
       mf6hip_set_error("the two-level preconditioner needs a symmetric system (CG)");
       return MF6HIP_EUNSUPPORTED;
@@ -1428,8 +1428,8 @@ static int check_settings(const mf6hip_ims_settings *s) {
   if (s->precision != MF6HIP_FP64 && s->ilinmeth != 1) { mf6hip_set_error("mixed precision is implemented for CG only"); return MF6HIP_EUNSUPPORTED; }
   if (s->ordering != MF6HIP_ORDER_RB) { mf6hip_set_error("unknown ordering"); return MF6HIP_EINVAL; }
   if (s->kernel_variant < 0 || s->kernel_variant > 1) { mf6hip_set_error("unknown kernel variant"); return MF6HIP_EINVAL; }
-  if (s->preconditioner < 0 || s->preconditioner > 2) { mf6hip_set_error("unknown preconditioner"); return MF6HIP_EINVAL; }
-  if (s->preconditioner == 2 && (s->ilinmeth != 1 || s->kernel_variant != 0 || s->north != 0 ||
+  if (s->preconditioner < 0 || s->preconditioner > 3) { mf6hip_set_error("unknown preconditioner"); return MF6HIP_EINVAL; }
+  if (s->preconditioner >= 2 && (s->ilinmeth != 1 || s->kernel_variant != 0 || s->north != 0 ||
                                  (s->precision != MF6HIP_FP64 && s->icnvgopt != 0))) {
     mf6hip_set_error("the two-level preconditioner runs in the fused CG (LINEAR_ACCELERATION CG, no orthogonalization; mixed precision with ICNVGOPT 0)");
     return MF6HIP_EUNSUPPORTED;
@@ -1667,7 +1667,8 @@ extern "C" int mf6hip_set_timing(void *handle, int enable, int sample) {
 // op 0: y = A x (SpMV of the last system), op 1: y = M^-1 x (ILU apply of the
 // last factor), op 2: y = inverse pivots (x ignored), op 3: y = M^-1 x of the
 // two-level preconditioner (ILU0 plus the coarse correction; needs
-// preconditioner = 2 and a system loaded by mf6hip_load_factor).
+// preconditioner = 2 or 3 and a system loaded by mf6hip_load_factor: the
+// additive operator, whose coarse part the deflated form shares).
 extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !y || (op != 2 && !x)) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
@@ -1683,8 +1684,8 @@ extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y)
   else if (op == 1) launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, tx, h->zf, ty, nullptr);
   else if (op == 2) HIPCHK(hipMemcpyAsync(ty, h->invpiv, sizeof(double) * L.NP, hipMemcpyDeviceToDevice, s));
   else if (op == 3) {
-    if (h->s.preconditioner != 2 || !h->tl_cmask) {
-      mf6hip_set_error("op 3 needs preconditioner = 2 and a loaded system");
+    if (h->s.preconditioner < 2 || !h->tl_cmask) {
+      mf6hip_set_error("op 3 needs preconditioner = 2 or 3 and a loaded system");
       return MF6HIP_EINVAL;
     }
     launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, tx, h->zf, ty, nullptr);
@@ -1707,7 +1708,7 @@ extern "C" int mf6hip_load_factor(void *handle, const double *amat) {
   HIPCHK(hipMemcpyAsync(h->d_amat_csr, amat, sizeof(double) * h->nja, hipMemcpyHostToDevice, s));
   k_csr_to_planes<<<grid_for(L.N), MF6_BLOCK, 0, s>>>(L, h->d_ia, h->d_amat_csr, h->A);
   CK(ilu_factor(h));
-  if (h->s.preconditioner == 2) CK(mf6hip_tl_setup(h));
+  if (h->s.preconditioner >= 2) CK(mf6hip_tl_setup(h));
   HIPCHK(hipStreamSynchronize(s));
   return MF6HIP_OK;
 }

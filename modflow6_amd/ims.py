@@ -39,8 +39,9 @@ class ImsLinearSettings:
     inner_fp32_rtol: float = 0.0
     kernel_variant: int = 0      # 0 fused CG (symmetric systems), 1 unfused
     preconditioner: int = 0      # 0 ILU0/MILU0, 1 ILU0 + learned correction (FCG),
-                                 # 2 two-level (ILU0 + aggregation coarse correction)
-    coarse_size: int = 0         # preconditioner 2: most coarse unknowns
+                                 # 2 two-level (ILU0 + aggregation coarse correction),
+                                 # 3 two-level, deflated form
+    coarse_size: int = 0         # preconditioner 2, 3: most coarse unknowns
                                  # (0: sqrt(2 N) within [16, 4096])
 
     def to_c(self, nlay, nrow, ncol):
