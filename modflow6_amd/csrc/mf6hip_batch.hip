@@ -193,10 +193,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Til
       if (h2 >= L.HP1) continue;
       const int l = col_l(L, 1, h2);
       const int i = l / L.ncol, j = l - i * L.ncol;
-      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
-      const int il = j > 0 ? hn(l, -1) : -1;
-      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      // off-grid neighbours read position 0 (in range); a select drops
+      // their term, so no lane waits inside a branch
+      const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+      const int ib = hb ? hn(l, -L.ncol) : 0;
+      const int il = hl ? hn(l, -1) : 0;
+      const int ir = hr ? hn(l, 1) : 0;
+      const int ifr = hf ? hn(l, L.ncol) : 0;
       float zup = 0.f, lup = 0.f;
       const unsigned cm = css ? cms[L.HP + h2] : 0u;
       double cs = 0.0;
@@ -210,10 +213,22 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_black(const BSys *sys, const Til
         md = maxloc_pick(md, MaxLoc{(double)dv, k * L.NRC + l});
         ss = ss + (double)dv * (double)dv;
         float tv = dv;
-        if (ib >= 0) tv = tv - (A[5 * NP + rb + ib] * pv[rb + ib]) * zr[rb + ib];
-        if (il >= 0) tv = tv - (A[4 * NP + rb + il] * pv[rb + il]) * zr[rb + il];
-        if (ir >= 0) tv = tv - (A[4 * NP + pos] * pv[rb + ir]) * zr[rb + ir];
-        if (ifr >= 0) tv = tv - (A[5 * NP + pos] * pv[rb + ifr]) * zr[rb + ifr];
+        {
+          const auto t_ = (A[5 * NP + rb + ib] * pv[rb + ib]) * zr[rb + ib];
+          tv = hb ? tv - t_ : tv;
+        }
+        {
+          const auto t_ = (A[4 * NP + rb + il] * pv[rb + il]) * zr[rb + il];
+          tv = hl ? tv - t_ : tv;
+        }
+        {
+          const auto t_ = (A[4 * NP + pos] * pv[rb + ir]) * zr[rb + ir];
+          tv = hr ? tv - t_ : tv;
+        }
+        {
+          const auto t_ = (A[5 * NP + pos] * pv[rb + ifr]) * zr[rb + ifr];
+          tv = hf ? tv - t_ : tv;
+        }
         if (k > 0) tv = tv - lup * zup;
         zz[pos] = tv;
         zup = tv;
@@ -265,10 +280,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const T
       if (h2 >= L.HP0) continue;
       const int l = col_l(L, 0, h2);
       const int i = l / L.ncol, j = l - i * L.ncol;
-      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
-      const int il = j > 0 ? hn(l, -1) : -1;
-      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      // off-grid neighbours read position 0 (in range); a select drops
+      // their term, so no lane waits inside a branch
+      const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+      const int ib = hb ? hn(l, -L.ncol) : 0;
+      const int il = hl ? hn(l, -1) : 0;
+      const int ir = hr ? hn(l, 1) : 0;
+      const int ifr = hf ? hn(l, L.ncol) : 0;
       float zdn = 0.f;
       const unsigned cm = css ? cms[h2] : 0u;
       double cs = 0.0;
@@ -277,10 +295,22 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_red_bwd(const BSys *sys, const T
         const int bb = L.H + k * L.HP;
         float tv = zr[pos];
         if (k < L.nlay - 1) tv = tv - A[6 * NP + pos] * zdn;
-        if (ib >= 0) tv = tv - A[5 * NP + bb + ib] * zz[bb + ib];
-        if (il >= 0) tv = tv - A[4 * NP + bb + il] * zz[bb + il];
-        if (ir >= 0) tv = tv - A[4 * NP + pos] * zz[bb + ir];
-        if (ifr >= 0) tv = tv - A[5 * NP + pos] * zz[bb + ifr];
+        {
+          const auto t_ = A[5 * NP + bb + ib] * zz[bb + ib];
+          tv = hb ? tv - t_ : tv;
+        }
+        {
+          const auto t_ = A[4 * NP + bb + il] * zz[bb + il];
+          tv = hl ? tv - t_ : tv;
+        }
+        {
+          const auto t_ = A[4 * NP + pos] * zz[bb + ir];
+          tv = hr ? tv - t_ : tv;
+        }
+        {
+          const auto t_ = A[5 * NP + pos] * zz[bb + ifr];
+          tv = hf ? tv - t_ : tv;
+        }
         tv = tv * pv[pos];
         zz[pos] = tv;
         zdn = tv;
@@ -369,10 +399,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_spmv(const BSys *sys, const Tile
       if (h2 >= ncols_of(L, cc)) continue;
       const int l = col_l(L, cc, h2);
       const int i = l / L.ncol, j = l - i * L.ncol;
-      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
-      const int il = j > 0 ? hn(l, -1) : -1;
-      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      // off-grid neighbours read position 0 (in range); a select drops
+      // their term, so no lane waits inside a branch
+      const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+      const int ib = hb ? hn(l, -L.ncol) : 0;
+      const int il = hl ? hn(l, -1) : 0;
+      const int ir = hr ? hn(l, 1) : 0;
+      const int ifr = hf ? hn(l, L.ncol) : 0;
       const int own = cc * L.H + h2;
       const int oth = (1 - cc) * L.H;
       float pprev = 0.f, pcur = p[own], aup = 0.f;
@@ -384,10 +417,22 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_spmv(const BSys *sys, const Tile
         float tq = 0.f;
         tq = tq + A[pos] * pcur;
         if (k > 0) tq = tq + aup * pprev;
-        if (ib >= 0) tq = tq + A[5 * NP + ob + ib] * p[ob + ib];
-        if (il >= 0) tq = tq + A[4 * NP + ob + il] * p[ob + il];
-        if (ir >= 0) tq = tq + A[4 * NP + pos] * p[ob + ir];
-        if (ifr >= 0) tq = tq + A[5 * NP + pos] * p[ob + ifr];
+        {
+          const auto t_ = A[5 * NP + ob + ib] * p[ob + ib];
+          tq = hb ? tq + t_ : tq;
+        }
+        {
+          const auto t_ = A[4 * NP + ob + il] * p[ob + il];
+          tq = hl ? tq + t_ : tq;
+        }
+        {
+          const auto t_ = A[4 * NP + pos] * p[ob + ir];
+          tq = hr ? tq + t_ : tq;
+        }
+        {
+          const auto t_ = A[5 * NP + pos] * p[ob + ifr];
+          tq = hf ? tq + t_ : tq;
+        }
         if (k < L.nlay - 1) tq = tq + adn * pnext;
         qq[pos] = tq;
         sum = sum + (double)pcur * (double)tq;
@@ -559,10 +604,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_residual(const BSys *sys, const 
       if (h2 >= ncols_of(L, cc)) continue;
       const int l = col_l(L, cc, h2);
       const int i = l / L.ncol, j = l - i * L.ncol;
-      const int ib = i > 0 ? hn(l, -L.ncol) : -1;
-      const int il = j > 0 ? hn(l, -1) : -1;
-      const int ir = j < L.ncol - 1 ? hn(l, 1) : -1;
-      const int ifr = i < L.nrow - 1 ? hn(l, L.ncol) : -1;
+      // off-grid neighbours read position 0 (in range); a select drops
+      // their term, so no lane waits inside a branch
+      const bool hb = i > 0, hl = j > 0, hr = j < L.ncol - 1, hf = i < L.nrow - 1;
+      const int ib = hb ? hn(l, -L.ncol) : 0;
+      const int il = hl ? hn(l, -1) : 0;
+      const int ir = hr ? hn(l, 1) : 0;
+      const int ifr = hf ? hn(l, L.ncol) : 0;
       const int own = cc * L.H + h2;
       const int oth = (1 - cc) * L.H;
       for (int k = 0; k < L.nlay; k++) {
@@ -571,10 +619,22 @@ __global__ void __launch_bounds__(MF6_BLOCK) kb_residual(const BSys *sys, const 
         double t2 = 0.0;
         t2 = t2 + A[pos] * xx[pos];
         if (k > 0) t2 = t2 + A[NP + pos] * xx[pos - L.HP];
-        if (ib >= 0) t2 = t2 + A[2 * NP + pos] * xx[ob + ib];
-        if (il >= 0) t2 = t2 + A[3 * NP + pos] * xx[ob + il];
-        if (ir >= 0) t2 = t2 + A[4 * NP + pos] * xx[ob + ir];
-        if (ifr >= 0) t2 = t2 + A[5 * NP + pos] * xx[ob + ifr];
+        {
+          const auto t_ = A[2 * NP + pos] * xx[ob + ib];
+          t2 = hb ? t2 + t_ : t2;
+        }
+        {
+          const auto t_ = A[3 * NP + pos] * xx[ob + il];
+          t2 = hl ? t2 + t_ : t2;
+        }
+        {
+          const auto t_ = A[4 * NP + pos] * xx[ob + ir];
+          t2 = hr ? t2 + t_ : t2;
+        }
+        {
+          const auto t_ = A[5 * NP + pos] * xx[ob + ifr];
+          t2 = hf ? t2 + t_ : t2;
+        }
         if (k < L.nlay - 1) t2 = t2 + A[6 * NP + pos] * xx[pos + L.HP];
         const double r = b[vo + pos] - t2;
         d32[vo + pos] = (float)r;
