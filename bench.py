@@ -4,7 +4,8 @@ steady confined GWF model (10 x 1000 x 1000 DIS, exponential-covariance
 ln K with sigma 2 / correlation 20 cells, CHD + RCH + 2000 wells), FP64
 ILU0-PCG vs the mixed-precision solves (FP32 ILU0 factor, and the FP16
 factor variant), each also with the two-level preconditioner (ILU0 plus an
-aggregation coarse correction; methods *_2l), on one MI355X; the headline
+aggregation coarse correction: additive, methods *_2l; deflated, methods
+*_dl), on one MI355X; the headline
 value is the fastest mixed solve that meets the accuracy bar (FP64 true
 residual <= 1e-6, heads within 1e-5 m of the oracle's sample), and the
 speedups are against FP64 ILU0-PCG.
@@ -80,18 +81,20 @@ def true_residual(case, amat, rhs, x):
     return float(np.max(np.abs(rhs - ax)))
 
 
-def pmc_traffic():
-    """HBM bytes per launch of the FP32 SpMV at 10 M cells from the committed
+def pmc_traffic(name="hbm_traffic_mixed_steady_10m.md"):
+    """HBM bytes per launch of the FP32 SpMV at 10 M cells from a committed
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by the factors
-    measured on a copy kernel of known size in the same runs)."""
+    measured on a copy kernel of known size in the same runs): the plain
+    SpMV, or (hbm_traffic_twolevel_steady_10m.md) the one that also writes
+    the column sums of q for the deflated two-level form."""
     import re
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed_steady_10m.md")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
     m = re.search(r"HBM traffic per launch = .* = ([0-9.]+) MB", open(path).read())
     if not m:
         return None, None
-    return float(m.group(1)) * 1e6, "profiles/hbm_traffic_mixed_steady_10m.md"
+    return float(m.group(1)) * 1e6, "profiles/" + name
 
 
 def cpu_baseline(case, budget_s=30.0):
@@ -432,6 +435,11 @@ def main():
         sp_bytes, sp_us = fp32_spmv_bytes, summary[head]["spmv_us"]
     else:
         sp_bytes, sp_us = fp64_spmv_bytes, summary[head]["spmv_us"]
+    # deflated two-level form: the SpMV also reads each grid column's
+    # coupled-cell mask (4 B) and writes its sum of q (8 B)
+    defl = head.endswith("_dl")
+    if defl:
+        sp_bytes += 12 * nrow * ncol
     achieved = sp_bytes / (sp_us * 1e-6) / 1e9 if sp_us > 0 else None
     # CSR-equivalent bytes of the same product (values + int32 column
     # indices + row pointers + x + y), FP64 and FP32 value forms
@@ -439,7 +447,8 @@ def main():
     csr32 = 8 * nja + 12 * N + 4
     traffic, traffic_src = None, None
     if f32 and N == 10_000_000:
-        traffic, traffic_src = pmc_traffic()
+        traffic, traffic_src = pmc_traffic("hbm_traffic_twolevel_steady_10m.md" if defl
+                                           else "hbm_traffic_mixed_steady_10m.md")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=traffic, traffic_source=traffic_src,
