@@ -149,13 +149,19 @@ struct DeflPlanes {
   const float *s = nullptr, *ew = nullptr, *ns = nullptr;
 };
 
-// Branch-free: a column off an edge reads the slot of its own aggregate row
+// Branch-free: a column off an edge reads a slot of its own aggregate row
 // (always in range) and weights it by a zero difference, so every lane
-// issues the same loads and none waits inside a divergent branch.
+// issues the same loads and none waits inside a divergent branch.  With
+// aggregates at least two columns wide a column is on at most one edge per
+// direction: one load per direction, slot and yc difference picked per
+// column.  One-column aggregates (every column on both edges) add the second
+// slot behind a grid-uniform flag.
 template <typename TV>
 struct DeflCol {
-  TV y, dw, de, dn, ds;   // yc of the column's aggregate; yc of a neighbour minus it
-  int ew0, ns0;           // row offsets into ew / ns at layer 0
+  TV y, dj, di, dj2, di2;   // yc of the column's aggregate; neighbour yc differences
+  int ej, ni;               // offsets into ew / ns at layer 0 of the picked slots
+  int ew0, ns0;             // offsets of the first slots (second pass, ha = 1)
+  bool two;                 // ha = 1
 
   __device__ DeflCol() {}
   __device__ DeflCol(const TV *__restrict__ yc, const TwoLevel &T, const Layout &L, int i,
@@ -164,25 +170,31 @@ struct DeflCol {
     const int ri = i - ai * T.ha, rj = j - aj * T.ha;
     const bool w = rj == 0 && j > 0, e = rj == T.ha - 1 && j + 1 < L.ncol;
     const bool n = ri == 0 && i > 0, so = ri == T.ha - 1 && i + 1 < L.nrow;
+    two = T.ha == 1;
     y = yc[c];
-    dw = yc[w ? c - 1 : c] - y;
-    de = yc[e ? c + 1 : c] - y;
-    dn = yc[n ? c - T.naj : c] - y;
-    ds = yc[so ? c + T.naj : c] - y;
     ew0 = i * 2 * T.naj + 2 * aj;
     ns0 = 2 * ai * L.ncol + j;
+    // w, else e (the first pass); e again when both (ha = 1, second pass)
+    dj = yc[w ? c - 1 : (e ? c + 1 : c)] - y;
+    ej = ew0 + (w ? 0 : 1);
+    dj2 = yc[(w && e) ? c + 1 : c] - y;
+    di = yc[n ? c - T.naj : (so ? c + T.naj : c)] - y;
+    ni = ns0 + (n ? 0 : L.ncol);
+    di2 = yc[(n && so) ? c + T.naj : c] - y;
   }
   // (A v) at layer k, cell position pos
   __device__ TV av(const float *__restrict__ sp, const float *__restrict__ ewp,
                    const float *__restrict__ nsp, const TwoLevel &T, const Layout &L, int k,
                    int pos) const {
-    const size_t ke = (size_t)k * L.nrow * 2 * T.naj + ew0;
-    const size_t kn = (size_t)k * 2 * T.nai * L.ncol + ns0;
+    const size_t ke = (size_t)k * L.nrow * 2 * T.naj;
+    const size_t kn = (size_t)k * 2 * T.nai * L.ncol;
     TV r = (TV)sp[pos] * y;
-    r = r + (TV)ewp[ke] * dw;
-    r = r + (TV)ewp[ke + 1] * de;
-    r = r + (TV)nsp[kn] * dn;
-    r = r + (TV)nsp[kn + L.ncol] * ds;
+    r = r + (TV)ewp[ke + ej] * dj;
+    r = r + (TV)nsp[kn + ni] * di;
+    if (two) {
+      r = r + (TV)ewp[ke + ew0 + 1] * dj2;
+      r = r + (TV)nsp[kn + ns0 + L.ncol] * di2;
+    }
     return r;
   }
 };

@@ -198,3 +198,26 @@ def test_deflated_outer_loop_matches_oracle(precision):
     print("outer/inner", [(r.kiter, r.inner) for r in log], "max |h - h_oracle|", err)
     assert log[-1].converged
     assert err < 10 * case.ims.inner_dvclose
+
+
+@pytest.mark.parametrize("pc", [2, 3])
+def test_one_column_aggregates(pc):
+    """coarse_size at least nrow x ncol: every aggregate is one grid column
+    (ha = 1), so each column sits on both edges in each direction; both
+    forms still reach the oracle's heads."""
+    case = configs.steady_small(nlay=2, nrow=6, ncol=8, nwel=2)
+    assert coarse_shape(case.nrow, case.ncol, 64)[0] == 1
+    sim, amat, rhs, x0 = oracle_system(case)
+    xo, its_o = _oracle_heads(sim, amat, rhs, x0)
+    for precision in (_lib.MF6HIP_FP64, _lib.MF6HIP_MIXED_FP16):
+        sv = device_solver(case, precision=precision, preconditioner=pc, coarse_size=64)
+        x = x0.copy()
+        try:
+            icnvg, its = sv.solve(1, amat.copy(), rhs.copy(), x)
+        finally:
+            sv.destroy()
+        err = np.max(np.abs(x - xo))
+        print("preconditioner", pc, "precision", precision, "iterations", its, "oracle", its_o,
+              "max |h - h_oracle|", err)
+        assert icnvg == 1, its
+        assert err <= 10 * case.ims.inner_dvclose
