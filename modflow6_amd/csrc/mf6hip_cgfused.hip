@@ -144,7 +144,8 @@ enum { M_D_R = 0, M_D_B = 1, M_P = 2 };
 // s = 0 and no stored edge couplings, so (A v)_n = 0 there, as in A v.
 // ew: couplings across the aggregate edges in j, [(k nrow + i) 2 naj + 2 aj
 // + e] (e = 0: to j - 1 at j = aj ha; e = 1: to j + 1 at j = aj ha + ha - 1);
-// ns: across the edges in i, [(k 2 nai + 2 ai + e) ncol + j].
+// ns: across the edges in i, [(k 2 nai + 2 ai + e) ncol + jc], jc the
+// column's place among its row's cells of its colour (colour 0 first).
 struct DeflPlanes {
   const float *s = nullptr, *ew = nullptr, *ns = nullptr;
 };
@@ -173,7 +174,10 @@ struct DeflCol {
     two = T.ha == 1;
     y = yc[c];
     ew0 = i * 2 * T.naj + 2 * aj;
-    ns0 = 2 * ai * L.ncol + j;
+    // ns rows hold a grid row's colour-0 columns, then its colour-1 columns
+    const int jc = ((i + j) & 1) == 0 ? (j >> 1)
+                                      : ((i & 1) ? (L.ncol >> 1) : ((L.ncol + 1) >> 1)) + (j >> 1);
+    ns0 = 2 * ai * L.ncol + jc;
     // w, else e (the first pass); e again when both (ha = 1, second pass)
     dj = yc[w ? c - 1 : (e ? c + 1 : c)] - y;
     ej = ew0 + (w ? 0 : 1);

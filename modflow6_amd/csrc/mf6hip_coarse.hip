@@ -538,12 +538,20 @@ __global__ void k_tl_colsum(Layout L, const TV *__restrict__ d, const unsigned *
   }
 }
 
+// Position of grid column (i, j) in a row of ns: colour by colour within the
+// row (the row's colour-0 cells first), so the lanes of one colour kernel
+// read consecutive words.
+__host__ __device__ inline int ns_col(const Layout &L, int i, int j) {
+  if (((i + j) & 1) == 0) return j >> 1;
+  return ((i & 1) ? (L.ncol >> 1) : ((L.ncol + 1) >> 1)) + (j >> 1);
+}
+
 // Per cell n (FP64 planes A): s[n] = A(n, n) + sum of A(n, m) over the
 // coupled neighbours m (0 on decoupled cells), and the couplings A(n, m) to
 // coupled neighbours across aggregate edges into ew / ns (layouts as the
 // fused CG reads them: ew [(k nrow + i) 2 naj + 2 aj + e], e = 0 to j - 1
 // at the aggregate's first column, 1 to j + 1 at its last; ns [(k 2 nai +
-// 2 ai + e) ncol + j], the same in i).
+// 2 ai + e) ncol + ns_col(i, j)], the same in i).
 __global__ void k_tl_defl_setup(Layout L, const double *__restrict__ A,
                                 const unsigned *__restrict__ cmask, TwoLevel T,
                                 float *__restrict__ sdiag, float *__restrict__ ew,
@@ -580,7 +588,7 @@ __global__ void k_tl_defl_setup(Layout L, const double *__restrict__ A,
       sum = sum + ad;
       sdiag[pos] = own ? (float)sum : 0.0f;
       const size_t e0 = ((size_t)k * L.nrow + i) * 2 * T.naj + 2 * aj;
-      const size_t n0 = ((size_t)k * 2 * T.nai + 2 * ai) * L.ncol + j;
+      const size_t n0 = ((size_t)k * 2 * T.nai + 2 * ai) * L.ncol + ns_col(L, i, j);
       if (rj == 0 && hl) ew[e0] = own ? (float)al : 0.0f;
       if (rj == T.ha - 1 && hr) ew[e0 + 1] = own ? (float)ar : 0.0f;
       if (ri == 0 && hb) ns[n0] = own ? (float)ab : 0.0f;
