@@ -39,11 +39,13 @@ def build_fortran():
 
 def build_oracle():
     """Compile the CPU oracle (oracle/liboracle.so) and, where the reference
-    sources are present, oracle/_ref/libmf6ref.so."""
+    sources are present, oracle/_ref/libmf6ref.so (IMS core) and
+    oracle/_ref/libmf6refgwf.so (GWF packages and the solution's outer-loop
+    procedures)."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-C", odir, "all"])
     if os.path.isdir("/root/reference/src"):
-        _run(["make", "-C", odir, "ref"])
+        _run(["make", "-C", odir, "ref", "refgwf"])
 
 
 def build_all():

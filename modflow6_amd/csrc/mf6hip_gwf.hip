@@ -354,8 +354,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, in
         snold = 1.0;
         snnew = 1.0;
       } else {
-        snold = sqsat(tp, bt, xold[p], 0.0);
-        snnew = sqsat(tp, bt, hcell, 0.0);
+        // STO's satomega: 1e-6 under NEWTON, else 0 (gwf-sto.f90:826-829)
+        snold = sqsat(tp, bt, xold[p], g.satomega);
+        snnew = sqsat(tp, bt, hcell, g.satomega);
       }
       double sc1 = g.ss[p] * (tp - bt) * g.area[p];
       double rho1 = sc1 * tled;
@@ -843,8 +844,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
         snold = 1.0;
         snnew = 1.0;
       } else {
-        snold = sqsat(tp, bt, xold[p], 0.0);
-        snnew = sqsat(tp, bt, hcell, 0.0);
+        snold = sqsat(tp, bt, xold[p], g.satomega);
+        snnew = sqsat(tp, bt, hcell, g.satomega);
       }
       const double sc1 = g.ss[p] * (tp - bt) * g.area[p];
       const double rho1 = sc1 * tled;
@@ -977,7 +978,7 @@ static GwfDev make_dev(Mf6Handle *h) {
   g.icelltype = h->icelltype; g.iconvert = h->iconvert; g.ibound = h->ibound;
   g.inewton = h->m.inewton; g.ivarcv = h->m.ivarcv; g.idewatcv = h->m.idewatcv;
   g.icellavg = h->m.icellavg; g.has_sto = h->m.has_sto;
-  g.satomega = h->m.inewton ? 1.0e-6 : 0.0;   // gwf-npf.f90:1394-1396
+  g.satomega = h->m.inewton ? 1.0e-6 : 0.0;   // NPF gwf-npf.f90:1394-1396, STO gwf-sto.f90:826-829
   g.fn = h->m.inewton;
   return g;
 }

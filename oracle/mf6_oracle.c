@@ -802,12 +802,13 @@ void orc_npf_fn(int nodes, const int *ia, const int *ja, const int *jas,
 }
 
 /* gwf-sto.f90:226-345 sto_fc (transient only; istor_coef, iorig_ss,
- * iconf_ss and integratechanges all 0, STO satomega = 0). */
+ * iconf_ss and integratechanges all 0).  satomega is STO's own: 1e-6 when the
+ * model uses NEWTON (source_options, gwf-sto.f90:826-829), else 0. */
 void orc_sto_fc(int nodes, const int *ia, const int *ibound,
                 const int *iconvert, const double *top, const double *bot,
                 const double *area, const double *ss, const double *sy,
-                double delt, const double *hold, const double *hnew,
-                double *amat, double *rhs)
+                double delt, double satomega, const double *hold,
+                const double *hnew, double *amat, double *rhs)
 {
   double tled = DONE / delt;
   for (int n = 0; n < nodes; n++) {
@@ -817,8 +818,8 @@ void orc_sto_fc(int nodes, const int *ia, const int *ibound,
     double snold, snnew;
     if (iconvert[n] == 0) { snold = DONE; snnew = DONE; }
     else {
-      snold = orc_sqsat(tp, bt, hold[n], 0.0);
-      snnew = orc_sqsat(tp, bt, hnew[n], 0.0);
+      snold = orc_sqsat(tp, bt, hold[n], satomega);
+      snnew = orc_sqsat(tp, bt, hnew[n], satomega);
     }
     double sc1 = orc_sscapacity(0, tp, bt, area[n], ss[n]);
     double rho1 = sc1 * tled;
@@ -940,8 +941,9 @@ void orc_npf_cq(int nodes, const int *ia, const int *ja, const int *jas,
 void orc_sto_cq(int nodes, const int *ia, const int *ibound,
                 const int *iconvert, const double *top, const double *bot,
                 const double *area, const double *ss, const double *sy,
-                double delt, const double *hold, const double *hnew,
-                double *flowja, double *strgss, double *strgsy)
+                double delt, double satomega, const double *hold,
+                const double *hnew, double *flowja, double *strgss,
+                double *strgsy)
 {
   for (int n = 0; n < nodes; n++) { strgss[n] = DZERO; strgsy[n] = DZERO; }
   double tled = DONE / delt;
@@ -951,8 +953,8 @@ void orc_sto_cq(int nodes, const int *ia, const int *ibound,
     double snold, snnew;
     if (iconvert[n] == 0) { snold = DONE; snnew = DONE; }
     else {
-      snold = orc_sqsat(tp, bt, hold[n], 0.0);
-      snnew = orc_sqsat(tp, bt, hnew[n], 0.0);
+      snold = orc_sqsat(tp, bt, hold[n], satomega);
+      snnew = orc_sqsat(tp, bt, hnew[n], satomega);
     }
     double sc1 = orc_sscapacity(0, tp, bt, area[n], ss[n]);
     double rho1 = sc1 * tled;

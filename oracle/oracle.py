@@ -100,7 +100,7 @@ def lib():
         L.orc_npf_fn.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, i32p,
                                  i32p, i, f64p, f64p, d, f64p, f64p, f64p]
         L.orc_sto_fc.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
-                                 f64p, d, f64p, f64p, f64p, f64p]
+                                 f64p, d, d, f64p, f64p, f64p, f64p]
         L.orc_sto_fn.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
                                  f64p, d, f64p, f64p, f64p]
         L.orc_bnd_fc.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
@@ -108,7 +108,7 @@ def lib():
                                  f64p, f64p, i32p, i32p, i, i, i, i, f64p,
                                  f64p, f64p, f64p, f64p, f64p, f64p]
         L.orc_sto_cq.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
-                                 f64p, d, f64p, f64p, f64p, f64p, f64p]
+                                 f64p, d, d, f64p, f64p, f64p, f64p, f64p]
         L.orc_bnd_cq.argtypes = [i, i32p, f64p, f64p, i32p, f64p, i32p, f64p,
                                  f64p]
         L.orc_csr_diagsum.argtypes = [i, i32p, f64p]
@@ -198,6 +198,49 @@ def ref_lib():
                                        i, i32p, i32p, f64p, d, d, d]
         _REF = R
     return _REF
+
+
+_REFGWF = None
+
+
+def ref_gwf_lib():
+    """Load oracle/_ref/libmf6refgwf.so (the reference's GWF packages and
+    NumericalSolution procedures, driven through oracle/ref_gwf_shim.f90), or
+    None when it has not been built."""
+    global _REFGWF
+    if _REFGWF is None:
+        path = os.path.join(HERE, "_ref", "libmf6refgwf.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        d, i = C.c_double, C.c_int
+        R.ref_gwf_dis.argtypes = [i, i, i, i32p, i32p, i32p, i32p, i32p, f64p,
+                                  f64p, f64p, f64p, f64p, f64p]
+        R.ref_npf_cf.argtypes = [i, i, d, i32p, i32p, f64p, f64p]
+        R.ref_npf_fill.argtypes = [i, i, i, i, i, i, i, i, d, i32p, i32p, f64p,
+                                   f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_npf_nur.restype = i
+        R.ref_npf_nur.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p,
+                                  C.POINTER(d), C.POINTER(i)]
+        R.ref_sto_fill.argtypes = [i, i, i, i, d, i, i, i, i, d, i32p, i32p,
+                                   f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_drn.argtypes = [i, i, i, i32p, f64p, f64p, i, f64p, i, i32p,
+                              f64p, f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_npf_cq.argtypes = [i, i, i, i, i, i, d, i32p, i32p, f64p, f64p,
+                                 f64p, f64p, f64p, f64p]
+        R.ref_sto_cq.argtypes = [i, i, d, i, d, i32p, i32p, f64p, f64p, f64p,
+                                 f64p, f64p, f64p, f64p]
+        R.ref_sln_new.argtypes = [i, i, d, d, d, d, d, d]
+        R.ref_sln_free.argtypes = []
+        R.ref_sln_underrelax.argtypes = [i, d, i, i32p, f64p, f64p, f64p,
+                                         f64p, f64p, f64p, f64p]
+        R.ref_sln_bt_xupdate.restype = i
+        R.ref_sln_bt_xupdate.argtypes = [i, i32p, f64p, f64p]
+        R.ref_gwf_ptc.restype = i
+        R.ref_gwf_ptc.argtypes = [i, i, f64p, f64p, f64p, i32p, i, i, i, i32p,
+                                  d, f64p, f64p]
+        _REFGWF = R
+    return _REFGWF
 
 
 # --------------------------------------------------------------------------
@@ -477,7 +520,9 @@ class GwfSimulation:
         self.icelltype = np.ascontiguousarray(c.icelltype, dtype=np.int32)
         self.ibound = np.ones(self.neq, dtype=np.int32)
         self.inewton = 1 if c.newton else 0
-        self.satomega = 1e-6 if c.newton else 0.0        # gwf-npf.f90:1394
+        # NPF and STO both take 1e-6 under NEWTON, else 0 (gwf-npf.f90:1394,
+        # gwf-sto.f90:826-829)
+        self.satomega = 1e-6 if c.newton else 0.0
         self.condsat = np.zeros(d.hwva.size)
         lib().orc_calc_condsat(self.neq, d.ia, d.ja, d.jas, d.ihc, d.cl1,
                                d.cl2, d.hwva, d.top, d.bot, self.k11,
@@ -592,7 +637,8 @@ class GwfSimulation:
         if self.case.has_sto and not self.iss:
             L.orc_sto_fc(self.neq, d.ia, self.ibound, self.iconvert, d.top,
                          d.bot, d.area, self.ss, self.sy, self.delt,
-                         self.xold, self.x, self.amat, self.rhs)
+                         self.satomega, self.xold, self.x, self.amat,
+                         self.rhs)
         for t, p in enumerate(self.pk):
             if p.kind == "CHD":
                 continue      # chd_fc is a no-op (gwf-chd.f90:238-250)
@@ -627,7 +673,8 @@ class GwfSimulation:
         if self.case.has_sto and not self.iss:
             L.orc_sto_cq(self.neq, d.ia, self.ibound, self.iconvert, d.top,
                          d.bot, d.area, self.ss, self.sy, self.delt,
-                         self.xold, self.x, flowja, strgss, strgsy)
+                         self.satomega, self.xold, self.x, flowja, strgss,
+                         strgsy)
         if self.case.has_sto:
             terms["STO-SS"] = self._accumulate(strgss)
             if np.any(np.asarray(self.iconvert) != 0):
