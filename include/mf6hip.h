@@ -345,9 +345,12 @@ int mf6hip_batch_add(void *batch, const mf6hip_ims_settings *settings);
 /* allocate device storage for every registered system */
 int mf6hip_batch_finalize(void *batch);
 /* system data in the reference layout (1-based DIS CSR, amat after the
- * sln_ls row fixes, rhs, start heads) */
-int mf6hip_batch_set_system(void *batch, int isys, const int *ia, int nja,
-                            const double *amat, const double *rhs,
+ * sln_ls row fixes, rhs, start heads).  ia / ja must be the DIS 7-point
+ * structure of the registered shape (diagonal first) and amat symmetric:
+ * checked on the host before any staging, MF6HIP_EINVAL /
+ * MF6HIP_EUNSUPPORTED otherwise.  Needs mf6hip_batch_finalize first. */
+int mf6hip_batch_set_system(void *batch, int isys, const int *ia, const int *ja,
+                            int nja, const double *amat, const double *rhs,
                             const double *x0);
 /* pick columns per thread and the persistent grid by timing short replays;
  * report receives a text summary */
@@ -370,7 +373,20 @@ int mf6hip_set_timing(void *handle, int enable, int sample);
  * op 1: y = M^-1 x, the ILU0/MILU0 apply (ims_base_ilu0a, src/Solution/
  *       LinearMethods/ImsLinearBase.f90:1049-1092) of the factor on the
  *       device, in the device ordering;
- * op 2: y = the inverse pivots of that factor (x unused).
+ * op 2: y = the inverse pivots of that factor (x unused);
+ * op 3: y = the additive two-level preconditioner applied to x
+ *       (preconditioner 2 or 3): ILU0 apply plus P (P^T A P)^-1 P^T x;
+ * op 4: y = A x through the fused CG's SpMV (FP64 planes, amux order);
+ * op 5: y = A x through the fused CG's FP32 SpMV, A and x rounded to FP32
+ *       (mixed-precision handles), y the FP32 result widened;
+ * op 6: y = the deflated two-level preconditioner applied to x (FP64,
+ *       preconditioner 3): yc = (P^T A P)^-1 P^T x,
+ *       y = M^-1 (x - A P yc) + P yc;
+ * op 7: y = M^-1 x through the fused CG's ILU0 sweeps (FP64, preconditioner
+ *       0): equal to op 1 for symmetric A;
+ * op 8: the same sweeps in the mixed handle's precision (FP32 planes and
+ *       pivots, or FP16 planes scaled by a power of two with FP32 pivots),
+ *       x rounded to FP32.
  * A and the factor are those of the last mf6hip_solve / mf6hip_load_factor. */
 int mf6hip_apply_op(void *handle, int op, const double *x, double *y);
 

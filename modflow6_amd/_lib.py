@@ -166,7 +166,7 @@ def load():
     L.mf6hip_batch_create.argtypes = [i, C.POINTER(vp)]
     L.mf6hip_batch_add.argtypes = [vp, C.POINTER(ImsSettingsC)]
     L.mf6hip_batch_finalize.argtypes = [vp]
-    L.mf6hip_batch_set_system.argtypes = [vp, i, ip, i, dp, dp, dp]
+    L.mf6hip_batch_set_system.argtypes = [vp, i, ip, ip, i, dp, dp, dp]
     L.mf6hip_batch_autotune.argtypes = [vp, C.c_char_p, i]
     L.mf6hip_batch_solve.argtypes = [vp, ip, ip, dp]
     L.mf6hip_batch_get_x.argtypes = [vp, i, dp]

@@ -931,15 +931,13 @@ int reset_states(BatchHandle *b) {
 
 }  // namespace
 
+// Create and add touch no device (the stream is created by finalize), so
+// registration and the host checks of mf6hip_batch_set_system also run
+// where no GPU is present.
 extern "C" int mf6hip_batch_create(int capacity, void **handle) {
   if (!handle || capacity < 1) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
   BatchHandle *b = new BatchHandle();
   b->cap = capacity;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete b;
-    mf6hip_set_error("hipStreamCreate failed");
-    return MF6HIP_EHIP;
-  }
   *handle = b;
   return MF6HIP_OK;
 }
@@ -1007,6 +1005,11 @@ extern "C" int mf6hip_batch_finalize(void *handle) {
   BatchHandle *b = (BatchHandle *)handle;
   if (!b || b->nsys < 1) { mf6hip_set_error("empty ensemble"); return MF6HIP_EINVAL; }
   if (b->finalized) return MF6HIP_OK;
+  if (!b->stream && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    b->stream = nullptr;
+    mf6hip_set_error("hipStreamCreate failed");
+    return MF6HIP_EHIP;
+  }
   BCK(balloc(&b->A64, b->ptot));
   BCK(balloc(&b->A32, b->ptot));
   const size_t v = b->vtot;
@@ -1062,14 +1065,20 @@ extern "C" int mf6hip_batch_finalize(void *handle) {
 }
 
 // System data in the reference layout (host): 1-based DIS CSR, amat after
-// the sln_ls row fixes, rhs, start heads.
-extern "C" int mf6hip_batch_set_system(void *handle, int isys, const int *ia, int nja,
-                                       const double *amat, const double *rhs, const double *x0) {
+// the sln_ls row fixes, rhs, start heads.  ia / ja must be the DIS 7-point
+// structure of the system's shape and amat symmetric (the batched CG reads
+// every coupling from the upper planes); both are checked on the host
+// before anything is staged.
+extern "C" int mf6hip_batch_set_system(void *handle, int isys, const int *ia, const int *ja,
+                                       int nja, const double *amat, const double *rhs,
+                                       const double *x0) {
   BatchHandle *b = (BatchHandle *)handle;
-  if (!b || !b->finalized || !ia || !amat || !rhs || !x0) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
+  if (!b || !ia || !ja || !amat || !rhs || !x0) { mf6hip_set_error("bad argument"); return MF6HIP_EINVAL; }
   if (isys < 0 || isys >= b->nsys) { mf6hip_set_error("system index out of range"); return MF6HIP_EINVAL; }
   const BSys &S = b->sys[isys];
-  if (ia[S.L.N] - 1 != nja) { mf6hip_set_error("IA / NJA mismatch"); return MF6HIP_EINVAL; }
+  BCK(mf6hip_check_structure(S.L, S.L.N, nja, ia, ja));
+  BCK(mf6hip_check_symmetric_csr(S.L, ia, amat));
+  if (!b->finalized) { mf6hip_set_error("ensemble not finalized"); return MF6HIP_EINVAL; }
   BCK(mf6hip_stage_system(S.L, ia, nja, amat, rhs, x0, b->set[isys].relax, b->A64 + S.aoff,
                           b->b64 + S.voff, b->x64 + S.voff, b->piv64 + S.voff, b->stream));
   HIPCHK(hipMemcpyAsync(b->x0 + S.voff, b->x64 + S.voff, sizeof(double) * S.L.NP,

@@ -125,6 +125,22 @@ struct FState {
   int restart;      // the next round starts a new direction (beta = 0)
 };
 
+// W consecutive values of one array, loaded and stored as one access
+// (16 bytes for 4 FP32 or 2 FP64 values)
+template <typename T, int W>
+struct alignas(sizeof(T) * W) VecW {
+  T v[W];
+};
+
+template <int W, typename T>
+__device__ inline VecW<T, W> ldw(const T *p) {
+  return *reinterpret_cast<const VecW<T, W> *>(p);
+}
+template <int W, typename T>
+__device__ inline void stw(T *p, const VecW<T, W> &v) {
+  *reinterpret_cast<VecW<T, W> *>(p) = v;
+}
+
 }  // namespace
 
 // partial slots in part[] / mpart[] (each nblk long)
@@ -190,9 +206,14 @@ struct DeflCol {
   __device__ TV av(const float *__restrict__ sp, const float *__restrict__ ewp,
                    const float *__restrict__ nsp, const TwoLevel &T, const Layout &L, int k,
                    int pos) const {
+    return av_s(sp[pos], ewp, nsp, T, L, k);
+  }
+  // the same with the cell's row sum s already loaded
+  __device__ TV av_s(float sv, const float *__restrict__ ewp, const float *__restrict__ nsp,
+                     const TwoLevel &T, const Layout &L, int k) const {
     const size_t ke = (size_t)k * L.nrow * 2 * T.naj;
     const size_t kn = (size_t)k * 2 * T.nai * L.ncol;
-    TV r = (TV)sp[pos] * y;
+    TV r = (TV)sv * y;
     r = r + (TV)ewp[ke + ej] * dj;
     r = r + (TV)nsp[kn + ni] * di;
     if (two) {
@@ -406,6 +427,321 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwd(Layout L, const TM *__r
 }
 
 // ---------------------------------------------------------------------------
+// The same three kernels with W columns of one colour per thread
+// ---------------------------------------------------------------------------
+//
+// For ncol a multiple of 2W (groups never straddle a grid row; HP is then a
+// multiple of W), each stream is one W-wide access.  Column m of colour c in
+// grid row i sits at h = i ncol/2 + m; its neighbours in the other colour
+// are at h - ncol/2 (back), h + ncol/2 (front), h - 1 + s (left) and h + s
+// (right), s = (i + c) mod 2.  A group's back and front values are one
+// aligned access each; its left and right values are the W-wide access at
+// h0 plus one edge value (h0 - 1 for s = 0, h0 + W for s = 1).  Per cell
+// the arithmetic is that of the single-column kernels, in the same order.
+// The grid is sized for the groups (a W-th of the columns); the colour
+// partials are then reduced over that grid by k_f_round.
+
+// geometry of a group of W columns of colour c starting at h0
+template <int W>
+struct GroupGeo {
+  int i, sf, ib, ifr, ee;
+  bool hb, hf, lok, rok;
+  __device__ GroupGeo(const Layout &L, int c, int h0) {
+    const int half = L.ncol >> 1;
+    i = h0 / half;
+    const int m0 = h0 - i * half;
+    sf = (i & 1) ^ c;
+    hb = i > 0;
+    hf = i < L.nrow - 1;
+    lok = sf == 1 || m0 > 0;          // the first column has a left neighbour
+    rok = sf == 0 || m0 + W < half;   // the last column has a right neighbour
+    // edge position, clamped into the layer (its term is dropped when the
+    // neighbour is off the grid)
+    ee = sf == 0 ? (h0 > 0 ? h0 - 1 : 0) : (h0 + W < L.HP ? h0 + W : L.HP - 1);
+    ib = hb ? h0 - half : h0;
+    ifr = hf ? h0 + half : h0;
+  }
+  // grid column (l) of the group's column e
+  __device__ int col(int h0, int e) const { return 2 * (h0 + e) + sf; }
+  // element e's left / right neighbour value from the W-wide load at h0 (w)
+  // and the edge value (x)
+  template <typename T>
+  __device__ T left(const VecW<T, W> &w, T x, int e) const {
+    return sf ? w.v[e] : (e > 0 ? w.v[e > 0 ? e - 1 : 0] : x);
+  }
+  template <typename T>
+  __device__ T right(const VecW<T, W> &w, T x, int e) const {
+    return sf ? (e < W - 1 ? w.v[e < W - 1 ? e + 1 : e] : x) : w.v[e];
+  }
+  __device__ bool has_left(int e) const { return e > 0 || lok; }
+  __device__ bool has_right(int e) const { return e < W - 1 || rok; }
+};
+
+template <typename TM, typename TV, typename TP, bool DEFL, int W>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwdw(Layout L, const TM *__restrict__ A,
+                                                          const TP *__restrict__ piv,
+                                                          TV *__restrict__ d,
+                                                          const TV *__restrict__ q,
+                                                          TV *__restrict__ zf, double *part,
+                                                          MaxLoc *mpart, const KState *st,
+                                                          const FState *fs, int nblk,
+                                                          const TV *__restrict__ yc,
+                                                          const float *__restrict__ dsp,
+                                                          const float *__restrict__ dew,
+                                                          const float *__restrict__ dns,
+                                                          TwoLevel T) {
+  const int lb = xcd_block();
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  const int NP = L.NP, HP = L.HP;
+  const TV alpha = (TV)fs->apend_d;
+  double ss = 0.0;
+  MaxLoc md{0.0, -1};
+  for (int t = lb * blockDim.x + threadIdx.x; t < HP / W; t += gridDim.x * blockDim.x) {
+    const int h0 = t * W;
+    const GroupGeo<W> G(L, 0, h0);
+    DeflCol<TV> dc[W];
+    if constexpr (DEFL) {
+#pragma unroll
+      for (int e = 0; e < W; e++) dc[e] = DeflCol<TV>(yc, T, L, G.i, G.col(h0, e) - G.i * L.ncol);
+    }
+    TV zup[W], lup[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) zup[e] = lup[e] = (TV)0;
+    for (int k = 0; k < L.nlay; k++) {
+      const int pos = k * HP + h0;
+      VecW<TV, W> dv = ldw<W>(d + pos);
+      const VecW<TV, W> qv = ldw<W>(q + pos);
+      const VecW<TM, W> a6 = ldw<W>(A + 6 * NP + pos);
+      const VecW<TP, W> pv = ldw<W>(piv + pos);
+      VecW<float, W> sv;
+      if constexpr (DEFL) sv = ldw<W>(dsp + pos);
+      VecW<TV, W> zo;
+#pragma unroll
+      for (int e = 0; e < W; e++) {
+        const TV x = dv.v[e] - alpha * qv.v[e];
+        TV tv = x;
+        if constexpr (DEFL) tv = tv - dc[e].av_s(sv.v[e], dew, dns, T, L, k);
+        dv.v[e] = x;
+        md = maxloc_pick(md, MaxLoc{(double)x, k * L.NRC + G.col(h0, e)});
+        ss = ss + (double)x * (double)x;
+        if (k > 0) tv = tv - lup[e] * zup[e];
+        zo.v[e] = tv;
+        zup[e] = tv;
+        lup[e] = (TV)((TV)a6.v[e] * (TV)pv.v[e]);
+      }
+      stw<W>(d + pos, dv);
+      stw<W>(zf + pos, zo);
+    }
+  }
+  ss = bsum(ss, lds);
+  if (threadIdx.x == 0) part[P_SS_R * nblk + lb] = ss;
+  md = bmax(md, mlds);
+  if (threadIdx.x == 0) mpart[M_D_R * nblk + lb] = md;
+}
+
+template <typename TM, typename TV, typename TP, bool DEFL, int W>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_blackw(Layout L, const TM *__restrict__ A,
+                                                        const TP *__restrict__ piv,
+                                                        TV *__restrict__ d,
+                                                        const TV *__restrict__ q,
+                                                        const TV *__restrict__ zf,
+                                                        TV *__restrict__ z, double *part,
+                                                        MaxLoc *mpart, const KState *st,
+                                                        const FState *fs, int nblk, TV sc,
+                                                        const unsigned *__restrict__ cmask,
+                                                        double *__restrict__ colsum,
+                                                        const TV *__restrict__ yc,
+                                                        const float *__restrict__ dsp,
+                                                        const float *__restrict__ dew,
+                                                        const float *__restrict__ dns,
+                                                        TwoLevel T) {
+  const int lb = xcd_block();
+  __shared__ double lds[8];
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  const int NP = L.NP, HP = L.HP;
+  const TV alpha = (TV)fs->apend_d;
+  double ss = 0.0, rho = 0.0;
+  MaxLoc md{0.0, -1};
+  for (int t = lb * blockDim.x + threadIdx.x; t < HP / W; t += gridDim.x * blockDim.x) {
+    const int h0 = t * W;
+    const GroupGeo<W> G(L, 1, h0);
+    VecW<unsigned, W> cm;
+    double cs[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      cm.v[e] = 0u;
+      cs[e] = 0.0;
+    }
+    if (colsum) cm = ldw<W>(cmask + HP + h0);
+    DeflCol<TV> dc[W];
+    if constexpr (DEFL) {
+#pragma unroll
+      for (int e = 0; e < W; e++) dc[e] = DeflCol<TV>(yc, T, L, G.i, G.col(h0, e) - G.i * L.ncol);
+    }
+    TV zup[W], lup[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) zup[e] = lup[e] = (TV)0;
+    for (int k = 0; k < L.nlay; k++) {
+      const int pos = L.H + k * HP + h0;
+      const int rb = k * HP;
+      // L(n, r) = A(n, r) piv(r) with A(n, back) = front(back), A(n, left) =
+      // right(left), A(n, right) = right(n), A(n, front) = front(n); the
+      // neighbour loads go out with d's
+      const VecW<TM, W> a5b = ldw<W>(A + 5 * NP + rb + G.ib);
+      const VecW<TP, W> pb = ldw<W>(piv + rb + G.ib);
+      const VecW<TV, W> zb = ldw<W>(zf + rb + G.ib);
+      const VecW<TM, W> a4w = ldw<W>(A + 4 * NP + rb + h0);
+      const TM a4x = A[4 * NP + rb + G.ee];
+      const VecW<TP, W> pw = ldw<W>(piv + rb + h0);
+      const TP px = piv[rb + G.ee];
+      const VecW<TV, W> zw = ldw<W>(zf + rb + h0);
+      const TV zx = zf[rb + G.ee];
+      const VecW<TP, W> pf = ldw<W>(piv + rb + G.ifr);
+      const VecW<TV, W> zfr = ldw<W>(zf + rb + G.ifr);
+      const VecW<TM, W> a4 = ldw<W>(A + 4 * NP + pos);
+      const VecW<TM, W> a5 = ldw<W>(A + 5 * NP + pos);
+      const VecW<TM, W> a6 = ldw<W>(A + 6 * NP + pos);
+      const VecW<TP, W> po = ldw<W>(piv + pos);
+      VecW<TV, W> dv = ldw<W>(d + pos);
+      const VecW<TV, W> qv = ldw<W>(q + pos);
+      VecW<float, W> sv;
+      if constexpr (DEFL) sv = ldw<W>(dsp + pos);
+      VecW<TV, W> zo;
+#pragma unroll
+      for (int e = 0; e < W; e++) {
+        const TV tb = (TV)((TV)a5b.v[e] * (TV)pb.v[e]) * zb.v[e];
+        const TV tl = (TV)((TV)G.left(a4w, a4x, e) * (TV)G.left(pw, px, e)) * G.left(zw, zx, e);
+        const TV tr = (TV)((TV)a4.v[e] * (TV)G.right(pw, px, e)) * G.right(zw, zx, e);
+        const TV tf = (TV)((TV)a5.v[e] * (TV)pf.v[e]) * zfr.v[e];
+        const TV x = dv.v[e] - alpha * qv.v[e];
+        TV tv = x;
+        if constexpr (DEFL) tv = tv - dc[e].av_s(sv.v[e], dew, dns, T, L, k);
+        dv.v[e] = x;
+        cs[e] = ((cm.v[e] >> k) & 1u) ? cs[e] + (double)x : cs[e];
+        md = maxloc_pick(md, MaxLoc{(double)x, k * L.NRC + G.col(h0, e)});
+        ss = ss + (double)x * (double)x;
+        tv = G.hb ? tv - tb : tv;
+        tv = G.has_left(e) ? tv - tl : tv;
+        tv = G.has_right(e) ? tv - tr : tv;
+        tv = G.hf ? tv - tf : tv;
+        if (k > 0) tv = tv - lup[e] * zup[e];
+        zo.v[e] = tv;
+        zup[e] = tv;
+        lup[e] = (TV)((TV)a6.v[e] * (TV)po.v[e]);
+      }
+      stw<W>(d + pos, dv);
+      stw<W>(z + pos, zo);
+    }
+    if (colsum) {
+#pragma unroll
+      for (int e = 0; e < W; e++) colsum[G.col(h0, e)] = cs[e];
+    }
+    TV zdn[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) zdn[e] = (TV)0;
+    for (int k = L.nlay - 1; k >= 0; k--) {
+      const int pos = L.H + k * HP + h0;
+      VecW<TV, W> zv = ldw<W>(z + pos);
+      const VecW<TM, W> a6 = ldw<W>(A + 6 * NP + pos);
+      const VecW<TP, W> po = ldw<W>(piv + pos);
+      const VecW<TV, W> dv = ldw<W>(d + pos);
+#pragma unroll
+      for (int e = 0; e < W; e++) {
+        TV tv = zv.v[e] * sc;
+        if (k < L.nlay - 1) tv = tv - (TV)a6.v[e] * zdn[e];
+        tv = tv * (TV)po.v[e];
+        zv.v[e] = tv;
+        zdn[e] = tv;
+        rho = rho + (double)dv.v[e] * (double)tv;
+      }
+      stw<W>(z + pos, zv);
+    }
+  }
+  ss = bsum(ss, lds);
+  if (threadIdx.x == 0) part[P_SS_B * nblk + lb] = ss;
+  rho = bsum(rho, lds);
+  if (threadIdx.x == 0) part[P_RHO_B * nblk + lb] = rho;
+  md = bmax(md, mlds);
+  if (threadIdx.x == 0) mpart[M_D_B * nblk + lb] = md;
+}
+
+template <typename TM, typename TV, typename TP, int W>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_red_bwdw(Layout L, const TM *__restrict__ A,
+                                                          const TP *__restrict__ piv,
+                                                          const TV *__restrict__ d,
+                                                          const TV *__restrict__ zf,
+                                                          TV *__restrict__ z, double *part,
+                                                          const KState *st, int nblk, TV sc,
+                                                          const unsigned *__restrict__ cmask,
+                                                          double *__restrict__ colsum) {
+  const int lb = xcd_block();
+  __shared__ double lds[8];
+  if (st->stop) return;
+  const int NP = L.NP, HP = L.HP;
+  double rho = 0.0;
+  for (int t = lb * blockDim.x + threadIdx.x; t < HP / W; t += gridDim.x * blockDim.x) {
+    const int h0 = t * W;
+    const GroupGeo<W> G(L, 0, h0);
+    VecW<unsigned, W> cm;
+    double cs[W];
+    TV zdn[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      cm.v[e] = 0u;
+      cs[e] = 0.0;
+      zdn[e] = (TV)0;
+    }
+    if (colsum) cm = ldw<W>(cmask + h0);
+    for (int k = L.nlay - 1; k >= 0; k--) {
+      const int pos = k * HP + h0;
+      const int bb = L.H + k * HP;
+      const VecW<TV, W> zfv = ldw<W>(zf + pos);
+      const VecW<TM, W> a6 = ldw<W>(A + 6 * NP + pos);
+      const VecW<TM, W> a5b = ldw<W>(A + 5 * NP + bb + G.ib);
+      const VecW<TV, W> zb = ldw<W>(z + bb + G.ib);
+      const VecW<TM, W> a4w = ldw<W>(A + 4 * NP + bb + h0);
+      const TM a4x = A[4 * NP + bb + G.ee];
+      const VecW<TV, W> zw = ldw<W>(z + bb + h0);
+      const TV zx = z[bb + G.ee];
+      const VecW<TV, W> zfr = ldw<W>(z + bb + G.ifr);
+      const VecW<TM, W> a4 = ldw<W>(A + 4 * NP + pos);
+      const VecW<TM, W> a5 = ldw<W>(A + 5 * NP + pos);
+      const VecW<TP, W> po = ldw<W>(piv + pos);
+      const VecW<TV, W> dv = ldw<W>(d + pos);
+      VecW<TV, W> zo;
+#pragma unroll
+      for (int e = 0; e < W; e++) {
+        TV tv = zfv.v[e] * sc;
+        if (k < L.nlay - 1) tv = tv - (TV)a6.v[e] * zdn[e];
+        const TV tb = (TV)a5b.v[e] * zb.v[e];
+        const TV tl = (TV)G.left(a4w, a4x, e) * G.left(zw, zx, e);
+        const TV tr = (TV)a4.v[e] * G.right(zw, zx, e);
+        const TV tf = (TV)a5.v[e] * zfr.v[e];
+        tv = G.hb ? tv - tb : tv;
+        tv = G.has_left(e) ? tv - tl : tv;
+        tv = G.has_right(e) ? tv - tr : tv;
+        tv = G.hf ? tv - tf : tv;
+        tv = tv * (TV)po.v[e];
+        zo.v[e] = tv;
+        zdn[e] = tv;
+        rho = rho + (double)dv.v[e] * (double)tv;
+        cs[e] = ((cm.v[e] >> k) & 1u) ? cs[e] + (double)dv.v[e] : cs[e];
+      }
+      stw<W>(z + pos, zo);
+    }
+    if (colsum) {
+#pragma unroll
+      for (int e = 0; e < W; e++) colsum[G.col(h0, e)] = cs[e];
+    }
+  }
+  rho = bsum(rho, lds);
+  if (threadIdx.x == 0) part[P_RHO_R * nblk + lb] = rho;
+}
+
+// ---------------------------------------------------------------------------
 // direction update with the deferred head update, then the SpMV
 // ---------------------------------------------------------------------------
 
@@ -460,16 +796,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate(Layout L, const TV *__r
   if (threadIdx.x == 0) mpart[M_P * nblk + lb] = mp;
 }
 
-// The same update with two columns of one colour per thread (positions
-// h2, h2 + 1, h2 even; 2-wide loads), on its own grid, for layouts whose
-// pairs never straddle a grid row; slots gridDim.x .. nblk-1 of its max |p|
-// partials are cleared.
-template <typename T> struct Vec2;
-template <> struct Vec2<float> { using type = float2; };
-template <> struct Vec2<double> { using type = double2; };
-
-template <typename TV, typename TX>
-__global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__restrict__ z,
+// The same update with W columns of one colour per thread (W-wide
+// accesses; ncol a multiple of 2W so that a group never straddles a grid
+// row), on its own grid; slots gridDim.x .. nblk-1 of its max |p| partials
+// are cleared.
+template <typename TV, typename TX, int W>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdatew(Layout L, const TV *__restrict__ z,
                                                           const TV *__restrict__ pin,
                                                           TV *__restrict__ pout,
                                                           TX *__restrict__ x, MaxLoc *mpart,
@@ -477,45 +809,54 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__
                                                           int nblk, const TV *__restrict__ yc,
                                                           const unsigned *__restrict__ cmask,
                                                           TwoLevel T) {
-  using V2 = typename Vec2<TV>::type;
-  using X2 = typename Vec2<TX>::type;
   const int lb = xcd_block();
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
   const bool first = st->iter == 0;
   const TV beta = (TV)st->beta;
   const TV ax = (TV)fs->apend_x;
+  const int HP = L.HP, half = L.ncol >> 1;
   MaxLoc mp{0.0, -1};
-  const int npair = L.HP0;
-  for (int t = lb * blockDim.x + threadIdx.x; t < npair; t += gridDim.x * blockDim.x) {
+  for (int t = lb * blockDim.x + threadIdx.x; t < 2 * (HP / W); t += gridDim.x * blockDim.x) {
     const int c = t & 1;
-    const int h2 = (t >> 1) * 2;
-    const int l0 = col_l(L, c, h2), l1 = col_l(L, c, h2 + 1);
-    const int own = c * L.H + h2;
-    const unsigned cm0 = yc ? cmask[c * L.HP + h2] : 0u;
-    const unsigned cm1 = yc ? cmask[c * L.HP + h2 + 1] : 0u;
-    const TV y0 = cm0 ? yc[tl_agg(T, L, l0)] : (TV)0;
-    const TV y1 = cm1 ? yc[tl_agg(T, L, l1)] : (TV)0;
+    const int h0 = (t >> 1) * W;
+    const int sf = ((h0 / half) & 1) ^ c;
+    const int own = c * L.H + h0;
+    VecW<unsigned, W> cm;
+    TV yv[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      cm.v[e] = 0u;
+      yv[e] = (TV)0;
+    }
+    if (yc) {
+      cm = ldw<W>(cmask + c * HP + h0);
+#pragma unroll
+      for (int e = 0; e < W; e++)
+        if (cm.v[e]) yv[e] = yc[tl_agg(T, L, 2 * (h0 + e) + sf)];
+    }
     for (int k = 0; k < L.nlay; k++) {
-      const int pos = own + k * L.HP;
-      V2 zv = *reinterpret_cast<const V2 *>(z + pos);
-      if ((cm0 >> k) & 1u) zv.x = zv.x + y0;
-      if ((cm1 >> k) & 1u) zv.y = zv.y + y1;
-      V2 pv;
+      const int pos = own + k * HP;
+      VecW<TV, W> zv = ldw<W>(z + pos);
+      VecW<TV, W> pv;
       if (first) {
-        pv = zv;
+#pragma unroll
+        for (int e = 0; e < W; e++) pv.v[e] = ((cm.v[e] >> k) & 1u) ? zv.v[e] + yv[e] : zv.v[e];
       } else {
-        const V2 po = *reinterpret_cast<const V2 *>(pin + pos);
-        pv.x = zv.x + beta * po.x;
-        pv.y = zv.y + beta * po.y;
-        X2 xv = *reinterpret_cast<const X2 *>(x + pos);
-        xv.x = (TX)(xv.x + (TX)(ax * po.x));
-        xv.y = (TX)(xv.y + (TX)(ax * po.y));
-        *reinterpret_cast<X2 *>(x + pos) = xv;
+        const VecW<TV, W> po = ldw<W>(pin + pos);
+        VecW<TX, W> xv = ldw<W>(x + pos);
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+          const TV zz = ((cm.v[e] >> k) & 1u) ? zv.v[e] + yv[e] : zv.v[e];
+          pv.v[e] = zz + beta * po.v[e];
+          xv.v[e] = (TX)(xv.v[e] + (TX)(ax * po.v[e]));
+        }
+        stw<W>(x + pos, xv);
       }
-      *reinterpret_cast<V2 *>(pout + pos) = pv;
-      mp = maxloc_pick(mp, MaxLoc{(double)pv.x, k * L.NRC + l0});
-      mp = maxloc_pick(mp, MaxLoc{(double)pv.y, k * L.NRC + l1});
+      stw<W>(pout + pos, pv);
+#pragma unroll
+      for (int e = 0; e < W; e++)
+        mp = maxloc_pick(mp, MaxLoc{(double)pv.v[e], k * L.NRC + 2 * (h0 + e) + sf});
     }
   }
   mp = bmax(mp, mlds);
@@ -526,15 +867,27 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_pupdate2(Layout L, const TV *__
   }
 }
 
-// pairs of one colour never straddle a grid row
-static inline bool spmv_pairable(const Layout &L) {
-  return !L.odd && (L.ncol / 2) % 2 == 0 && L.HP0 == L.HP1 && L.HP0 % 2 == 0;
+// Columns per thread of the grouped kernels for a layout: 16-byte accesses
+// (4 FP32 or 2 FP64 values) where the grid rows allow, else 2, else 0 (the
+// single-column kernels).
+template <typename TV>
+static inline int col_group(const Layout &L) {
+  constexpr int WMAX = 16 / (int)sizeof(TV);
+  if (L.odd) return 0;
+  if (L.ncol % (2 * WMAX) == 0) return WMAX;
+  if (WMAX > 2 && L.ncol % 4 == 0) return 2;
+  return 0;
 }
 
-static inline int pair_grid(const Layout &L, int nb) {
-  int g2 = (L.HP0 + MF6_BLOCK - 1) / MF6_BLOCK;
-  if (g2 > nb) g2 = nb;
-  return g2 < 1 ? 1 : g2;
+// grid of the colour kernels (one thread per group of one colour); k_f_round
+// reduces their partials over this many slots
+template <typename TV>
+static inline int colour_grid(const Layout &L) {
+  const int w = col_group<TV>(L);
+  int n = w ? L.HP / w : L.HP0;
+  int g = (n + MF6_BLOCK - 1) / MF6_BLOCK;
+  if (g > MF6_MAXBLK) g = MF6_MAXBLK;
+  return g < 1 ? 1 : g;
 }
 
 template <typename TV, typename TX>
@@ -542,12 +895,22 @@ static void launch_f_pupdate(const Layout &L, int nb, hipStream_t s, const TV *z
                              TV *pout, TX *x, MaxLoc *mpart, const KState *st,
                              const FState *fs, const TV *yc = nullptr,
                              const unsigned *cmask = nullptr, TwoLevel T = TwoLevel{}) {
-  if (spmv_pairable(L))
-    k_f_pupdate2<TV, TX><<<pair_grid(L, nb), MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st,
-                                                               fs, nb, yc, cmask, T);
-  else
+  constexpr int WMAX = 16 / (int)sizeof(TV);
+  const int w = col_group<TV>(L);
+  if (w) {
+    int g = (2 * (L.HP / w) + MF6_BLOCK - 1) / MF6_BLOCK;
+    if (g > nb) g = nb;
+    if (g < 1) g = 1;
+    if (w == WMAX)
+      k_f_pupdatew<TV, TX, WMAX><<<g, MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st, fs, nb,
+                                                         yc, cmask, T);
+    else
+      k_f_pupdatew<TV, TX, 2><<<g, MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st, fs, nb, yc,
+                                                      cmask, T);
+  } else {
     k_f_pupdate<TV, TX><<<nb, MF6_BLOCK, 0, s>>>(L, z, pin, pout, x, mpart, st, fs, nb, yc,
                                                  cmask, T);
+  }
 }
 
 // q = A p (amux order, symmetric planes), partial p.q.  The layer walk is
@@ -643,123 +1006,129 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv(Layout L, const TM *__rest
   if (threadIdx.x == 0) part[P_PQ * nblk + lb] = s;
 }
 
-// The same product with two columns of one colour per thread (positions h2,
-// h2 + 1, h2 even): the own-plane values and p of the pair are one 2-wide
-// load, which halves the load instructions of the streams.  Needs a layout
-// whose pairs never straddle a grid row: ncol and ncol / 2 even (then HP0 =
-// HP1 is even as well).  Same amux order per cell; the p.q partial adds the
-// pair's two cells per layer.
-// It runs on its own grid (half the columns of k_f_spmv: a grid sized for
-// the single-column kernel would leave the XCDs that receive the upper half
-// of the logical blocks idle); slots gridDim.x .. nblk-1 of its p.q partials
-// (read by k_f_alpha over nblk slots) are cleared.
-template <typename TV, int NLMAX, bool CS>
-__global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__restrict__ A,
+// The same product with W columns of one colour per thread (positions h0 ..
+// h0 + W - 1, h0 a multiple of W): every stream is one W-wide load, the own
+// planes and p as in the single-column kernel and the other colour's
+// neighbour values as well.  With ncol even, column m of colour c in grid
+// row i sits at position h = i ncol/2 + m and its neighbours in the other
+// colour at h - ncol/2 (back), h + ncol/2 (front), and h - 1 + s (left) /
+// h + s (right), s = (i + c) mod 2.  The W columns' back and front values are
+// therefore one aligned load each, and their left and right values lie in
+// the W-wide load at h0 plus one edge value (h0 - 1 for s = 0, h0 + W for
+// s = 1).  Needs ncol a multiple of 2W, so that a group never straddles a
+// grid row (HP is then a multiple of W).  Same amux order per cell; the p.q
+// partial adds the group's cells per layer.
+// It runs on its own grid (a W-th of the columns of k_f_spmv; a grid sized
+// for the single-column kernel would leave the XCDs that receive the upper
+// part of the logical blocks idle); slots gridDim.x .. nblk-1 of its p.q
+// partials (read by k_f_alpha over nblk slots) are cleared.
+template <typename TV, int W, int NLMAX, int CH, bool CS>
+__global__ void __launch_bounds__(MF6_BLOCK) k_f_spmvw(Layout L, const TV *__restrict__ A,
                                                        const TV *__restrict__ p,
                                                        TV *__restrict__ q, double *part,
                                                        const KState *st, int nblk,
                                                        const unsigned *__restrict__ cmask,
                                                        double *__restrict__ colsq) {
-  using V2 = typename Vec2<TV>::type;
-  constexpr int CH = 4;
+  using VT = VecW<TV, W>;
   const int lb = xcd_block();
   __shared__ double lds[8];
   if (st->stop) return;
-  const int NP = L.NP;
+  const int NP = L.NP, HP = L.HP, half = L.ncol >> 1;
+  const int ngrp = 2 * (HP / W);   // groups of both colours, interleaved
   double s = 0.0;
-  const int npair = L.HP0;   // HP0 / 2 pairs per colour, interleaved
-  for (int t = lb * blockDim.x + threadIdx.x; t < npair; t += gridDim.x * blockDim.x) {
+  for (int t = lb * blockDim.x + threadIdx.x; t < ngrp; t += gridDim.x * blockDim.x) {
     const int c = t & 1;
-    const int h2 = (t >> 1) * 2;
-    const int l0 = col_l(L, c, h2), l1 = col_l(L, c, h2 + 1);
-    const int i = l0 / L.ncol, j0 = l0 - i * L.ncol, j1 = l1 - i * L.ncol;
+    const int h0 = (t >> 1) * W;
+    const int i = h0 / half, m0 = h0 - i * half;
+    const int sf = (i & 1) ^ c;
     const bool hb = i > 0, hf = i < L.nrow - 1;
-    const bool hl0 = j0 > 0, hr0 = j0 < L.ncol - 1, hl1 = j1 > 0, hr1 = j1 < L.ncol - 1;
-    const int ib0 = hb ? hn(l0, -L.ncol) : 0, ib1 = hb ? hn(l1, -L.ncol) : 0;
-    const int if0 = hf ? hn(l0, L.ncol) : 0, if1 = hf ? hn(l1, L.ncol) : 0;
-    const int il0 = hl0 ? hn(l0, -1) : 0, il1 = hl1 ? hn(l1, -1) : 0;
-    const int ir0 = hr0 ? hn(l0, 1) : 0, ir1 = hr1 ? hn(l1, 1) : 0;
-    const int own = c * L.H + h2, oth = (1 - c) * L.H;
-    V2 pcur = *reinterpret_cast<const V2 *>(p + own);
-    V2 pprev{(TV)0, (TV)0}, aup{(TV)0, (TV)0};
-    unsigned cm0 = 0u, cm1 = 0u;
-    if constexpr (CS) {
-      cm0 = cmask[c * L.HP + h2];
-      cm1 = cmask[c * L.HP + h2 + 1];
+    const bool lok = sf == 1 || m0 > 0;          // the first column has a left neighbour
+    const bool rok = sf == 0 || m0 + W < half;   // the last column has a right neighbour
+    // edge value position, clamped into the layer (its term is dropped when
+    // the neighbour is off the grid)
+    const int ee = sf == 0 ? (h0 > 0 ? h0 - 1 : 0) : (h0 + W < HP ? h0 + W : HP - 1);
+    const int own = c * L.H + h0, oth = (1 - c) * L.H;
+    const int ib = hb ? h0 - half : h0, ifr = hf ? h0 + half : h0;
+    VT pcur = *reinterpret_cast<const VT *>(p + own);
+    VT pprev, aup;
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      pprev.v[e] = (TV)0;
+      aup.v[e] = (TV)0;
     }
-    double cs0 = 0.0, cs1 = 0.0;
+    VecW<unsigned, W> cm;
+    double cs[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      cm.v[e] = 0u;
+      cs[e] = 0.0;
+    }
+    if constexpr (CS) cm = *reinterpret_cast<const VecW<unsigned, W> *>(cmask + c * HP + h0);
 #pragma unroll
     for (int k0 = 0; k0 < NLMAX; k0 += CH) {
       if (k0 >= L.nlay) break;
-      V2 a0[CH], a4[CH], a5[CH], a6[CH], pn[CH];
-      TV gb0[CH], gb1[CH], pb0[CH], pb1[CH], gl0[CH], gl1[CH], pl0[CH], pl1[CH], pr0[CH],
-          pr1[CH], pf0[CH], pf1[CH];
+      VT a0[CH], a4[CH], a5[CH], a6[CH], pn[CH], po[CH], a4o[CH], a5b[CH], pb[CH], pf[CH];
+      TV pe[CH], a4e[CH];
 #pragma unroll
       for (int u = 0; u < CH; u++) {
         const int k = k0 + u;
         if (k < L.nlay) {
-          const int pos = own + k * L.HP, ob = oth + k * L.HP;
-          const bool dn = k + 1 < L.nlay;
-          a0[u] = *reinterpret_cast<const V2 *>(A + pos);
-          a4[u] = *reinterpret_cast<const V2 *>(A + 4 * NP + pos);
-          a5[u] = *reinterpret_cast<const V2 *>(A + 5 * NP + pos);
-          a6[u] = dn ? *reinterpret_cast<const V2 *>(A + 6 * NP + pos) : V2{(TV)0, (TV)0};
-          pn[u] = dn ? *reinterpret_cast<const V2 *>(p + pos + L.HP) : V2{(TV)0, (TV)0};
-          gb0[u] = hb ? A[5 * NP + ob + ib0] : (TV)0;
-          pb0[u] = hb ? p[ob + ib0] : (TV)0;
-          gb1[u] = hb ? A[5 * NP + ob + ib1] : (TV)0;
-          pb1[u] = hb ? p[ob + ib1] : (TV)0;
-          gl0[u] = hl0 ? A[4 * NP + ob + il0] : (TV)0;
-          pl0[u] = hl0 ? p[ob + il0] : (TV)0;
-          gl1[u] = hl1 ? A[4 * NP + ob + il1] : (TV)0;
-          pl1[u] = hl1 ? p[ob + il1] : (TV)0;
-          pr0[u] = hr0 ? p[ob + ir0] : (TV)0;
-          pr1[u] = hr1 ? p[ob + ir1] : (TV)0;
-          pf0[u] = hf ? p[ob + if0] : (TV)0;
-          pf1[u] = hf ? p[ob + if1] : (TV)0;
+          const int pos = own + k * HP, ob = oth + k * HP;
+          // the bottom layer's down plane is zero and its p below is not
+          // used: both loads stay in range (own position)
+          const int pdn = k + 1 < L.nlay ? pos + HP : pos;
+          a0[u] = *reinterpret_cast<const VT *>(A + pos);
+          a4[u] = *reinterpret_cast<const VT *>(A + 4 * NP + pos);
+          a5[u] = *reinterpret_cast<const VT *>(A + 5 * NP + pos);
+          a6[u] = *reinterpret_cast<const VT *>(A + 6 * NP + pos);
+          pn[u] = *reinterpret_cast<const VT *>(p + pdn);
+          po[u] = *reinterpret_cast<const VT *>(p + ob + h0);
+          pe[u] = p[ob + ee];
+          a4o[u] = *reinterpret_cast<const VT *>(A + 4 * NP + ob + h0);
+          a4e[u] = A[4 * NP + ob + ee];
+          a5b[u] = *reinterpret_cast<const VT *>(A + 5 * NP + ob + ib);
+          pb[u] = *reinterpret_cast<const VT *>(p + ob + ib);
+          pf[u] = *reinterpret_cast<const VT *>(p + ob + ifr);
         }
       }
 #pragma unroll
       for (int u = 0; u < CH; u++) {
         const int k = k0 + u;
         if (k < L.nlay) {
-          TV t0 = (TV)0, t1 = (TV)0;
-          t0 = t0 + a0[u].x * pcur.x;
-          t1 = t1 + a0[u].y * pcur.y;
-          if (k > 0) {
-            t0 = t0 + aup.x * pprev.x;
-            t1 = t1 + aup.y * pprev.y;
+          const bool dn = k + 1 < L.nlay;
+          VT qv;
+#pragma unroll
+          for (int e = 0; e < W; e++) {
+            // left: A(left, right) p(left); right: A(n, right) p(right)
+            const TV al = sf ? a4o[u].v[e] : (e > 0 ? a4o[u].v[e > 0 ? e - 1 : 0] : a4e[u]);
+            const TV pl = sf ? po[u].v[e] : (e > 0 ? po[u].v[e > 0 ? e - 1 : 0] : pe[u]);
+            const TV pr = sf ? (e < W - 1 ? po[u].v[e < W - 1 ? e + 1 : e] : pe[u]) : po[u].v[e];
+            TV tq = (TV)0;
+            tq = tq + a0[u].v[e] * pcur.v[e];
+            if (k > 0) tq = tq + aup.v[e] * pprev.v[e];
+            if (hb) tq = tq + a5b[u].v[e] * pb[u].v[e];
+            if (e > 0 || lok) tq = tq + al * pl;
+            if (e < W - 1 || rok) tq = tq + a4[u].v[e] * pr;
+            if (hf) tq = tq + a5[u].v[e] * pf[u].v[e];
+            if (dn) tq = tq + a6[u].v[e] * pn[u].v[e];
+            qv.v[e] = tq;
+            if constexpr (CS)
+              if ((cm.v[e] >> k) & 1u) cs[e] = cs[e] + (double)tq;
+            s = s + (double)pcur.v[e] * (double)tq;
           }
-          if (hb) {
-            t0 = t0 + gb0[u] * pb0[u];
-            t1 = t1 + gb1[u] * pb1[u];
-          }
-          if (hl0) t0 = t0 + gl0[u] * pl0[u];
-          if (hl1) t1 = t1 + gl1[u] * pl1[u];
-          if (hr0) t0 = t0 + a4[u].x * pr0[u];
-          if (hr1) t1 = t1 + a4[u].y * pr1[u];
-          if (hf) {
-            t0 = t0 + a5[u].x * pf0[u];
-            t1 = t1 + a5[u].y * pf1[u];
-          }
-          if (k + 1 < L.nlay) {
-            t0 = t0 + a6[u].x * pn[u].x;
-            t1 = t1 + a6[u].y * pn[u].y;
-          }
-          *reinterpret_cast<V2 *>(q + own + k * L.HP) = V2{t0, t1};
-          if constexpr (CS) {
-            if ((cm0 >> k) & 1u) cs0 = cs0 + (double)t0;
-            if ((cm1 >> k) & 1u) cs1 = cs1 + (double)t1;
-          }
-          s = s + (double)pcur.x * (double)t0;
-          s = s + (double)pcur.y * (double)t1;
+          *reinterpret_cast<VT *>(q + own + k * HP) = qv;
           aup = a6[u];
           pprev = pcur;
           pcur = pn[u];
         }
       }
     }
-    if constexpr (CS) *reinterpret_cast<double2 *>(colsq + c * L.HP + h2) = double2{cs0, cs1};
+    if constexpr (CS) {
+      VecW<double, W> cv;
+#pragma unroll
+      for (int e = 0; e < W; e++) cv.v[e] = cs[e];
+      *reinterpret_cast<VecW<double, W> *>(colsq + c * HP + h0) = cv;
+    }
   }
   s = bsum(s, lds);
   if (threadIdx.x == 0) {
@@ -768,23 +1137,43 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmv2(Layout L, const TV *__res
   }
 }
 
+// layers unrolled per chunk of the grouped SpMV (loads of a chunk are all
+// issued before its arithmetic)
+#define MF6_SPMV_CH 2
+
+template <typename TV, int W, bool CS>
+static void launch_f_spmvw(const Layout &L, int nb, hipStream_t s, const TV *A, const TV *p,
+                           TV *q, double *part, const KState *st, const unsigned *cmask,
+                           double *colsq) {
+  int g = (2 * (L.HP / W) + MF6_BLOCK - 1) / MF6_BLOCK;
+  if (g > nb) g = nb;
+  if (g < 1) g = 1;
+  constexpr int CH = MF6_SPMV_CH;
+  if (L.nlay <= 4)
+    k_f_spmvw<TV, W, 4, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+  else if (L.nlay <= 8)
+    k_f_spmvw<TV, W, 8, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+  else if (L.nlay <= 16)
+    k_f_spmvw<TV, W, 16, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+  else
+    k_f_spmvw<TV, W, 32, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+}
+
 // cmask / colsq given (the deflated two-level form): the SpMV also writes
 // each column's sum of q over its coupled cells, colsq[c HP + h2] (colour
-// layout, so that a pair's two sums are one store).
+// layout, so that a group's sums are one store).
 template <typename TM, typename TV, bool CS>
 static void launch_f_spmv_t(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
                             TV *q, double *part, const KState *st, const unsigned *cmask,
                             double *colsq) {
-  if (spmv_pairable(L)) {
-    const int g2 = pair_grid(L, nb);
-    if (L.nlay <= 4)
-      k_f_spmv2<TV, 4, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
-    else if (L.nlay <= 8)
-      k_f_spmv2<TV, 8, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
-    else if (L.nlay <= 16)
-      k_f_spmv2<TV, 16, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
-    else
-      k_f_spmv2<TV, 32, CS><<<g2, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+  constexpr int WMAX = 16 / (int)sizeof(TV);
+  const int w = col_group<TV>(L);
+  if (w == WMAX) {
+    launch_f_spmvw<TV, WMAX, CS>(L, nb, s, A, p, q, part, st, cmask, colsq);
+    return;
+  }
+  if (w == 2) {
+    launch_f_spmvw<TV, 2, CS>(L, nb, s, A, p, q, part, st, cmask, colsq);
     return;
   }
   if (L.nlay <= 4)
@@ -1064,6 +1453,31 @@ static inline int gridc(int n) {
   return g < 1 ? 1 : g;
 }
 
+// the grouped colour kernels (W columns per thread) on nbc blocks
+template <typename TM, typename TV, typename TP, int W>
+static void launch_pc_w(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV *q, TV *zf,
+                        TV *z, KState *st, FState *fs, int nbc, int nb, TV sc,
+                        const unsigned *cm, double *cs, const TV *yc, const DeflPlanes &D) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  if (yc) {
+    k_f_red_fwdw<TM, TV, TP, true, W><<<nbc, MF6_BLOCK, 0, s>>>(
+        L, A, piv, d, q, zf, h->d_part, h->d_mpart, st, fs, nb, yc, D.s, D.ew, D.ns, h->tl);
+    k_f_blackw<TM, TV, TP, true, W><<<nbc, MF6_BLOCK, 0, s>>>(
+        L, A, piv, d, q, zf, z, h->d_part, h->d_mpart, st, fs, nb, sc, cm, cs, yc, D.s, D.ew,
+        D.ns, h->tl);
+  } else {
+    k_f_red_fwdw<TM, TV, TP, false, W><<<nbc, MF6_BLOCK, 0, s>>>(
+        L, A, piv, d, q, zf, h->d_part, h->d_mpart, st, fs, nb, nullptr, nullptr, nullptr,
+        nullptr, h->tl);
+    k_f_blackw<TM, TV, TP, false, W><<<nbc, MF6_BLOCK, 0, s>>>(
+        L, A, piv, d, q, zf, z, h->d_part, h->d_mpart, st, fs, nb, sc, cm, cs, nullptr, nullptr,
+        nullptr, nullptr, h->tl);
+  }
+  k_f_red_bwdw<TM, TV, TP, W><<<nbc, MF6_BLOCK, 0, s>>>(L, A, piv, d, zf, z, h->d_part, st, nb,
+                                                       sc, cm, cs);
+}
+
 // The colour kernels run on nbc blocks (one thread per column of one
 // colour); their partials use the slot stride nb of the interleaved kernels.
 // A grid sized for both colours would leave the XCDs that receive the upper
@@ -1087,6 +1501,16 @@ static void launch_pc(Mf6Handle *h, const TM *A, const TP *piv, TV *d, const TV 
     D.s = h->tl_s32;
     D.ew = h->tl_ew;
     D.ns = h->tl_ns;
+  }
+  const int w = col_group<TV>(L);
+  constexpr int WMAX = 16 / (int)sizeof(TV);
+  if (w == WMAX) {
+    launch_pc_w<TM, TV, TP, WMAX>(h, A, piv, d, q, zf, z, st, fs, nbc, nb, sc, cm, cs, yc, D);
+    return;
+  }
+  if (w == 2) {
+    launch_pc_w<TM, TV, TP, 2>(h, A, piv, d, q, zf, z, st, fs, nbc, nb, sc, cm, cs, yc, D);
+    return;
   }
   if (yc) {
     k_f_red_fwd<TM, TV, TP, true><<<nbc, MF6_BLOCK, 0, s>>>(
@@ -1190,7 +1614,7 @@ int mf6hip_run_cg_fused(Mf6Handle *h) {
   if (h->s.preconditioner == 3) mf6hip_tl_colsum<double>(h, h->d);
   // one grid size for every kernel of a round, so that each partial array
   // has the same number of slots for the round kernel to reduce
-  const int nbc = gridc(L.HP0);
+  const int nbc = colour_grid<double>(L);
   const int nbs = gridc(2 * L.HP0);
   const int nb = nbs > nbc ? nbs : nbc;
   double *pbuf[2] = {h->p, h->p2};
@@ -1227,7 +1651,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
   hipStream_t s = h->stream;
   KState *st = h->d_state;
   FState *fs = reinterpret_cast<FState *>(h->d_state + 2);
-  const int nbc = gridc(L.HP0);
+  const int nbc = colour_grid<float>(L);
   const int nbs = gridc(2 * L.HP0);
   const int nb = nbs > nbc ? nbs : nbc;
   float *pbuf[2] = {h->p32, h->p232};
@@ -1352,5 +1776,106 @@ int mf6hip_is_symmetric(Mf6Handle *h, int *sym) {
   HIPCHK(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   *sym = hf == 0;
+  return MF6HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// parity helpers: the fused CG's operators on device vectors (positions)
+// ---------------------------------------------------------------------------
+
+template <typename TI, typename TO>
+__global__ void k_op_convert(int n, const TI *__restrict__ in, TO *__restrict__ out) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    out[e] = (TO)in[e];
+}
+
+// op 4: y = A x with the fused CG's SpMV on the FP64 planes;
+// op 5: y = A32 x32 with the fused CG's SpMV in FP32 (A and x rounded to
+//       FP32; needs a mixed-precision handle);
+// op 6: y = the deflated two-level preconditioner applied to x (FP64 handle,
+//       preconditioner 3): yc = (P^T A P)^-1 P^T x, y = M^-1 (x - A P yc) +
+//       P yc, M the ILU0 factor, through the fused CG's sweeps;
+// op 7: y = M^-1 x through the fused CG's sweeps (FP64, preconditioner 0);
+// op 8: the same in the mixed handle's precision (FP32 planes and pivots,
+//       or the scaled FP16 planes with FP32 pivots), x rounded to FP32.
+// x and y are device vectors in positions; y may not alias x.  A state slot
+// of its own (index 3) keeps the solve's states untouched.
+int mf6hip_fused_op(Mf6Handle *h, int op, const double *x, double *y) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  const int nbc = colour_grid<double>(L);
+  const int nbs = gridc(2 * L.HP0);
+  const int nb = nbs > nbc ? nbs : nbc;
+  KState *st = h->d_state + 3;
+  HIPCHK(hipMemsetAsync(st, 0, sizeof(KState), s));
+  if (op == 4) {
+    launch_f_spmv<double, double>(L, nb, s, h->A, x, y, h->d_part, st);
+  } else if (op == 5) {
+    if (!h->A32 || !h->r32 || !h->q32) {
+      mf6hip_set_error("op 5 needs a mixed-precision handle");
+      return MF6HIP_EINVAL;
+    }
+    k_op_convert<double, float><<<gridc(MF6_NSLOT * L.NP), MF6_BLOCK, 0, s>>>(
+        MF6_NSLOT * L.NP, h->A, h->A32);
+    k_op_convert<double, float><<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, x, h->r32);
+    launch_f_spmv<float, float>(L, nb, s, h->A32, h->r32, h->q32, h->d_part, st);
+    k_op_convert<float, double><<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->q32, y);
+  } else if (op == 6) {
+    if (h->s.preconditioner != 3 || h->s.precision != MF6HIP_FP64 || !h->tl_cmask ||
+        !h->tl_yc64) {
+      mf6hip_set_error("op 6 needs an FP64 handle with preconditioner 3 and a loaded system");
+      return MF6HIP_EINVAL;
+    }
+    // the sweeps take d - alpha q with alpha = 0 (pending update of a fresh
+    // state), q = 0 and column sums of q = 0
+    FState *fz = reinterpret_cast<FState *>(h->d_part + 7 * (size_t)MF6_MAXBLK + 64);
+    HIPCHK(hipMemsetAsync(fz, 0, sizeof(FState), s));
+    HIPCHK(hipMemcpyAsync(h->d, x, sizeof(double) * L.NP, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemsetAsync(h->q, 0, sizeof(double) * L.NP, s));
+    HIPCHK(hipMemsetAsync(h->tl_colsq, 0, sizeof(double) * 2 * (size_t)L.HP, s));
+    mf6hip_tl_colsum<double>(h, h->d);
+    mf6hip_tl_enqueue<double>(h, (const double *)nullptr, h->tl_yc64, h->d_part + P_RHO_C * nb,
+                              nb, st, &fz->apend_d);
+    launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, y, st, fz, nbc,
+                                      nb);
+    CK(mf6hip_tl_prolong64(h, y));
+  } else if (op == 7 || op == 8) {
+    if (h->s.preconditioner != 0) {
+      mf6hip_set_error("ops 7 and 8 need a handle with preconditioner 0");
+      return MF6HIP_EINVAL;
+    }
+    FState *fz = reinterpret_cast<FState *>(h->d_part + 7 * (size_t)MF6_MAXBLK + 64);
+    HIPCHK(hipMemsetAsync(fz, 0, sizeof(FState), s));
+    if (op == 7) {
+      HIPCHK(hipMemcpyAsync(h->d, x, sizeof(double) * L.NP, hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemsetAsync(h->q, 0, sizeof(double) * L.NP, s));
+      launch_pc<double, double, double>(h, h->A, h->invpiv, h->d, h->q, h->zf, y, st, fz, nbc,
+                                        nb);
+    } else {
+      if (h->s.precision == MF6HIP_FP64) {
+        mf6hip_set_error("op 8 needs a mixed-precision handle");
+        return MF6HIP_EINVAL;
+      }
+      const int nbc32 = colour_grid<float>(L);
+      const int nbs32 = gridc(2 * L.HP0);
+      const int nb32 = nbs32 > nbc32 ? nbs32 : nbc32;
+      k_op_convert<double, float><<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, x, h->r32);
+      HIPCHK(hipMemsetAsync(h->q32, 0, sizeof(float) * L.NP, s));
+      if (h->s.precision == MF6HIP_MIXED) {
+        k_op_convert<double, float><<<gridc(MF6_NSLOT * L.NP), MF6_BLOCK, 0, s>>>(
+            MF6_NSLOT * L.NP, h->A, h->A32);
+        launch_pc<float, float, float>(h, h->A32, h->invpiv32, h->r32, h->q32, h->zf32, h->z32,
+                                       st, fz, nbc32, nb32);
+      } else {
+        launch_pc<_Float16, float, float>(h, h->A16, h->invpiv32s, h->r32, h->q32, h->zf32,
+                                          h->z32, st, fz, nbc32, nb32, h->pc_scale);
+      }
+      k_op_convert<float, double><<<gridc(L.NP), MF6_BLOCK, 0, s>>>(L.NP, h->z32, y);
+    }
+  } else {
+    mf6hip_set_error("unknown fused operator");
+    return MF6HIP_EINVAL;
+  }
+  HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }

@@ -630,8 +630,24 @@ TwoLevel mf6hip_tl_shape(const Layout &L, int target) {
   return T;
 }
 
+// Allocates every buffer of the two-level preconditioner; tl_ready is set
+// only once all of them exist, and a failure frees what was allocated, so
+// the next call starts over instead of running on missing buffers.
+static int tl_alloc_all(Mf6Handle *h);
+
 int mf6hip_tl_alloc(Mf6Handle *h) {
-  if (h->tl_cmask) return MF6HIP_OK;
+  if (h->tl_ready) return MF6HIP_OK;
+  mf6hip_tl_free(h);
+  const int rc = tl_alloc_all(h);
+  if (rc != MF6HIP_OK) {
+    mf6hip_tl_free(h);
+    return rc;
+  }
+  h->tl_ready = true;
+  return MF6HIP_OK;
+}
+
+static int tl_alloc_all(Mf6Handle *h) {
   const Layout &L = h->L;
   h->tl = mf6hip_tl_shape(L, h->s.coarse_size);
   const size_t nn = (size_t)h->tl.nc * h->tl.ld;
@@ -687,6 +703,7 @@ int mf6hip_tl_alloc(Mf6Handle *h) {
 }
 
 void mf6hip_tl_free(Mf6Handle *h) {
+  h->tl_ready = false;
   if (h->tl_blas) rocblas_destroy_handle((rocblas_handle)h->tl_blas);
   h->tl_blas = nullptr;
   void *bufs[] = {h->tl_cmask, h->tl_G, h->tl_rc, h->tl_yc64, h->tl_yc32, h->tl_info,
@@ -827,6 +844,14 @@ void mf6hip_tl_colsum(Mf6Handle *h, const TV *d) {
 
 template void mf6hip_tl_colsum<double>(Mf6Handle *, const double *);
 template void mf6hip_tl_colsum<float>(Mf6Handle *, const float *);
+
+// z += P yc (the FP64 coarse correction of the last mf6hip_tl_enqueue)
+int mf6hip_tl_prolong64(Mf6Handle *h, double *z) {
+  k_tl_prolong<double><<<tl_grid(2 * (size_t)h->L.HP), MF6_BLOCK, 0, h->stream>>>(
+      h->L, h->tl_cmask, h->tl, h->tl_yc64, z);
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
 
 // z += P Ainv P^T r on device vectors (FP64): the coarse part of M^-1 r for
 // the parity helper mf6hip_apply_op

@@ -140,6 +140,7 @@ struct Mf6Handle {
   // inverse (FP64 for the FP64 solve, FP32 for the mixed ones), restricted
   // residual and coarse correction
   TwoLevel tl{};
+  bool tl_ready = false;         // every two-level buffer below is allocated
   unsigned *tl_cmask = nullptr;
   double *tl_G = nullptr, *tl_A64 = nullptr, *tl_rc = nullptr, *tl_yc64 = nullptr;
   float *tl_A32 = nullptr, *tl_yc32 = nullptr;
@@ -170,6 +171,13 @@ void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslo
 template <typename TV>
 void mf6hip_tl_colsum(Mf6Handle *h, const TV *d);
 int mf6hip_tl_apply64(Mf6Handle *h, const double *r, double *z);
+int mf6hip_tl_prolong64(Mf6Handle *h, double *z);
+// parity helpers on the fused CG's operators (ops 4-8 of mf6hip_apply_op)
+int mf6hip_fused_op(Mf6Handle *h, int op, const double *x, double *y);
+
+// host checks of a system in the reference's CSR layout (1-based ia / ja)
+int mf6hip_check_structure(const Layout &L, int neq, int nja, const int *ia, const int *ja);
+int mf6hip_check_symmetric_csr(const Layout &L, const int *ia, const double *amat);
 
 // solver entry points used by the GWF part
 int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg,

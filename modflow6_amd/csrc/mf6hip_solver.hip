@@ -1445,7 +1445,7 @@ static int check_settings(const mf6hip_ims_settings *s) {
 }
 
 // Verify ia/ja are the full DIS 7-point structure, diagonal first.
-static int check_structure(const Layout &L, int neq, int nja, const int *ia, const int *ja) {
+int mf6hip_check_structure(const Layout &L, int neq, int nja, const int *ia, const int *ja) {
   if (neq != L.N) { mf6hip_set_error("NEQ differs from nlay*nrow*ncol"); return MF6HIP_EINVAL; }
   long long e = 1;
   for (int n = 0; n < neq; n++) {
@@ -1468,6 +1468,45 @@ static int check_structure(const Layout &L, int neq, int nja, const int *ia, con
     e += c;
   }
   if (ia[neq] != e || nja != e - 1) { mf6hip_set_error("NJA does not match the DIS 7-point structure"); return MF6HIP_EINVAL; }
+  return MF6HIP_OK;
+}
+
+// Bitwise symmetry of CSR values on the DIS 7-point structure (checked by
+// mf6hip_check_structure): every coupling A(n, m) with m > n equals A(m, n).
+// In a DIS row the slots are diagonal, up, back, left, right, front, down,
+// each present when that neighbour exists.
+int mf6hip_check_symmetric_csr(const Layout &L, const int *ia, const double *amat) {
+  for (int n = 0; n < L.N; n++) {
+    const int k = n / L.NRC, r = n - k * L.NRC, i = r / L.ncol, j = r - i * L.ncol;
+    // entry index (0-based) of each present slot of row n
+    int e = ia[n] - 1;
+    int pos[MF6_NSLOT];
+    pos[S_DIAG] = e++;
+    pos[S_UP] = k > 0 ? e++ : -1;
+    pos[S_BACK] = i > 0 ? e++ : -1;
+    pos[S_LEFT] = j > 0 ? e++ : -1;
+    pos[S_RIGHT] = j < L.ncol - 1 ? e++ : -1;
+    pos[S_FRONT] = i < L.nrow - 1 ? e++ : -1;
+    pos[S_DOWN] = k < L.nlay - 1 ? e++ : -1;
+    // the lower slot of the neighbour that mirrors each upper slot: the
+    // right neighbour's left, the front neighbour's back, the lower cell's up
+    // (the neighbour's lower slot is its first entries after the diagonal)
+    const int m[3] = {n + 1, n + L.ncol, n + L.NRC};
+    const int us[3] = {S_RIGHT, S_FRONT, S_DOWN};
+    for (int t = 0; t < 3; t++) {
+      if (pos[us[t]] < 0) continue;
+      const int mm = m[t];
+      const int km = mm / L.NRC, rm = mm - km * L.NRC, im = rm / L.ncol;
+      // slots of row mm before its left / back / up slot
+      int em = ia[mm] - 1 + 1;   // after the diagonal
+      if (us[t] == S_RIGHT) em += (km > 0) + (im > 0);   // up, back come first
+      else if (us[t] == S_FRONT) em += (km > 0);         // up comes first
+      if (!(amat[pos[us[t]]] == amat[em])) {
+        mf6hip_set_error("the system is not symmetric at node " + std::to_string(n + 1));
+        return MF6HIP_EUNSUPPORTED;
+      }
+    }
+  }
   return MF6HIP_OK;
 }
 
@@ -1547,7 +1586,7 @@ extern "C" int mf6hip_create(int neq, int nja, const int *ia, const int *ja,
   if (Nll > (1LL << 30)) { delete h; mf6hip_set_error("grid too large for int32 indexing"); return MF6HIP_EUNSUPPORTED; }
   h->L = make_layout(settings->nlay, settings->nrow, settings->ncol);
   Layout &L = h->L;
-  int rc = check_structure(L, neq, nja, ia, ja);
+  int rc = mf6hip_check_structure(L, neq, nja, ia, ja);
   if (rc) { delete h; return rc; }
   h->neq = neq;
   h->nja = nja;
@@ -1668,7 +1707,9 @@ extern "C" int mf6hip_set_timing(void *handle, int enable, int sample) {
 // last factor), op 2: y = inverse pivots (x ignored), op 3: y = M^-1 x of the
 // two-level preconditioner (ILU0 plus the coarse correction; needs
 // preconditioner = 2 or 3 and a system loaded by mf6hip_load_factor: the
-// additive operator, whose coarse part the deflated form shares).
+// additive operator, whose coarse part the deflated form shares); ops 4-6:
+// the fused CG's operators (mf6hip_fused_op: FP64 SpMV, FP32 SpMV, deflated
+// two-level apply, ILU0 apply in FP64 and in the handle's mixed precision).
 extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !y || (op != 2 && !x)) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
@@ -1690,6 +1731,8 @@ extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y)
     }
     launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, tx, h->zf, ty, nullptr);
     CK(mf6hip_tl_apply64(h, tx, ty));
+  } else if (op >= 4 && op <= 8) {
+    CK(mf6hip_fused_op(h, op, tx, ty));
   } else { mf6hip_set_error("unknown op"); return MF6HIP_EINVAL; }
   k_pos_to_nat<double, double><<<g, MF6_BLOCK, 0, s>>>(L, ty, h->d_tmp_nat);
   HIPCHK(hipMemcpyAsync(y, h->d_tmp_nat, sizeof(double) * L.N, hipMemcpyDeviceToHost, s));

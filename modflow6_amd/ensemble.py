@@ -86,9 +86,9 @@ class Ensemble:
             amat, rhs, x0 = systems[i] if systems is not None else fixed_system(c)
             if self.systems is not None:
                 self.systems.append((amat, rhs, x0))
-            ia, _ = dis_structure(c.nlay, c.nrow, c.ncol)
-            _lib.check(L.mf6hip_batch_set_system(self.h, i, _lib.iptr(ia), amat.size,
-                                                 _lib.dptr(amat), _lib.dptr(rhs),
+            ia, ja = dis_structure(c.nlay, c.nrow, c.ncol)
+            _lib.check(L.mf6hip_batch_set_system(self.h, i, _lib.iptr(ia), _lib.iptr(ja),
+                                                 amat.size, _lib.dptr(amat), _lib.dptr(rhs),
                                                  _lib.dptr(x0)), "mf6hip_batch_set_system")
             if log and ((i + 1) % 32 == 0 or i + 1 == len(self.cases)):
                 log(f"ensemble: {i + 1}/{len(self.cases)} systems staged, "

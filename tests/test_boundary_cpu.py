@@ -181,3 +181,44 @@ def test_configs_are_deterministic():
         assert np.array_equal(pa.nodes, pb.nodes) and np.array_equal(pa.v1, pb.v1)
     t = configs.transient_newton(nlay=2, nrow=16, ncol=16, nwel=4)
     assert t.newton and t.has_sto and t.ims.ilinmeth == 2
+
+
+def test_batch_set_system_checks_structure_and_symmetry():
+    """The ensemble's set_system checks its CSR on the host before staging:
+    a JA that is not the DIS structure and a nonsymmetric matrix are
+    rejected (the batched CG reads every coupling from the upper planes, so
+    either would otherwise be solved as a different system); a valid system
+    passes the checks and then needs the (device) finalize."""
+    from tests.helpers import dis_matrix
+    L = _lib.load()
+    shape = (2, 3, 4)
+    h = C.c_void_p()
+    assert L.mf6hip_batch_create(2, C.byref(h)) == 0
+    try:
+        cs = _settings(precision=1).to_c(*shape)
+        assert L.mf6hip_batch_add(h, C.byref(cs)) == 0
+        ia, ja, a = dis_matrix(shape, seed=3)
+        n = ia.size - 1
+        rhs, x0 = np.zeros(n), np.zeros(n)
+
+        def set_sys(ia_, ja_, a_):
+            return L.mf6hip_batch_set_system(h, 0, _lib.iptr(ia_), _lib.iptr(ja_), a_.size,
+                                             _lib.dptr(a_), _lib.dptr(rhs), _lib.dptr(x0))
+
+        bad = ja.copy()
+        bad[[1, 2]] = bad[[2, 1]]
+        assert set_sys(ia, bad, a) == 1 and "JA does not match" in _lib.last_error()
+        bad_ia = ia.copy()
+        bad_ia[5] += 1
+        assert set_sys(bad_ia, ja, a) == 1 and "does not match" in _lib.last_error()
+        for entry in range(a.size):               # every off-diagonal, one at a time
+            r = np.searchsorted(ia - 1, entry, side="right") - 1
+            if entry == ia[r] - 1:
+                continue
+            ns = a.copy()
+            ns[entry] += 0.5
+            assert set_sys(ia, ja, ns) == 3, entry
+            assert "not symmetric" in _lib.last_error()
+        assert set_sys(ia, ja, a) == 1 and "not finalized" in _lib.last_error()
+    finally:
+        L.mf6hip_batch_destroy(h)

@@ -8,8 +8,9 @@ import pytest
 from modflow6_amd import _lib, configs
 from modflow6_amd.solution import GwfSolution
 from oracle import oracle as O
-from tests.helpers import (apply_op, bits_equal, device_solver, load_factor,
-                           oracle_permuted_factor, oracle_system, ulp_diff)
+from tests.helpers import (apply_op, bits_equal, device_solver, dis_matrix, f32_amux,
+                           half_scale_exp, load_factor, oracle_permuted_factor, oracle_system,
+                           rb_ilu_apply, ulp_diff)
 
 pytestmark = pytest.mark.gpu
 
@@ -118,3 +119,72 @@ def test_row_fixes_bit_exact():
     assert bits_equal(amat_d, amat), f"amat max ulp {ulp_diff(amat_d, amat)}"
     assert bits_equal(rhs_d, rhs), f"rhs max ulp {ulp_diff(rhs_d, rhs)}"
     gs.destroy()
+
+
+# Fused-CG SpMV (ops 4 and 5): shapes that take each kernel form.  FP32
+# groups 4 columns per thread when ncol % 8 == 0, 2 when ncol % 4 == 0;
+# FP64 groups 2 when ncol % 4 == 0; other shapes take the single-column
+# kernel.  Layer counts cover each unrolled bound (4, 8, 16, 32).
+FUSED_SHAPES = [(3, 100, 100), (2, 16, 24), (10, 40, 48), (1, 51, 51), (2, 7, 9),
+                (4, 12, 10), (5, 30, 40), (17, 8, 16), (3, 9, 8), (6, 3, 8)]
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES)
+def test_fused_spmv_bit_exact(shape):
+    nlay, nrow, ncol = shape
+    ia, ja, a = dis_matrix(shape, seed=nlay * 1000 + ncol)
+    n = ia.size - 1
+    case = configs.steady_small(nlay=nlay, nrow=nrow, ncol=ncol, nwel=0)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n)
+    yo = np.zeros(n)
+    O.lib().orc_amux(n, x, yo, a, ja - 1, ia - 1)
+    sv = device_solver(case)
+    try:
+        load_factor(sv, a)
+        y = apply_op(sv, 4, x)
+        assert bits_equal(y, yo), f"FP64 max ulp {ulp_diff(y, yo)}"
+    finally:
+        sv.destroy()
+    sv = device_solver(case, precision=_lib.MF6HIP_MIXED)
+    try:
+        load_factor(sv, a)
+        y32 = apply_op(sv, 5, x)
+        want = f32_amux(ia, ja, a, x)
+        assert bits_equal(y32, want), f"FP32 max ulp {ulp_diff(y32, want)}"
+    finally:
+        sv.destroy()
+
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES)
+def test_fused_ilu_sweeps_bit_exact(shape):
+    """The fused CG's ILU0 sweeps (grouped kernels where ncol allows, else
+    the single-column ones): in FP64 (op 7) equal to the oracle's ilu0a on
+    the permuted matrix (through op 1, itself bit-exact against it); in FP32
+    (op 8, MIXED) and with the scaled FP16 planes (op 8, MIXED_FP16) equal
+    to the numpy colour-sweep restatement in that precision, which the CPU
+    suite pins to ilu0a in FP64."""
+    nlay, nrow, ncol = shape
+    ia, ja, a = dis_matrix(shape, seed=nlay * 100 + nrow + ncol)
+    case = configs.steady_small(nlay=nlay, nrow=nrow, ncol=ncol, nwel=0)
+    x = np.random.default_rng(8).standard_normal(case.nodes)
+    sv = device_solver(case)
+    try:
+        load_factor(sv, a)
+        z1 = apply_op(sv, 1, x)
+        z7 = apply_op(sv, 7, x)
+        piv = apply_op(sv, 2)
+        assert bits_equal(z7, z1), f"FP64 max ulp {ulp_diff(z7, z1)}"
+    finally:
+        sv.destroy()
+    k = half_scale_exp(shape, ia, ja, a)
+    for precision, hk in ((_lib.MF6HIP_MIXED, None), (_lib.MF6HIP_MIXED_FP16, k)):
+        sv = device_solver(case, precision=precision)
+        try:
+            load_factor(sv, a)
+            z8 = apply_op(sv, 8, x)
+        finally:
+            sv.destroy()
+        want = rb_ilu_apply(shape, ia, ja, a, piv, x, dtype=np.float32, half_scale_exp=hk)
+        assert bits_equal(z8, want), f"precision {precision}: max ulp {ulp_diff(z8, want)}"

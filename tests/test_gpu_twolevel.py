@@ -104,6 +104,9 @@ def _oracle_heads(sim, amat, rhs, x0):
 SOLVE_CASES = {
     "steady_small": lambda: configs.steady_small(nlay=3, nrow=20, ncol=24, nwel=6),
     "steady_60": lambda: configs.steady_10m(nlay=4, nrow=60, ncol=64, nwel=12),
+    # odd ncol: the single-column kernels and the odd-ncol place of a column
+    # in the i-edge coupling rows of the deflated form
+    "steady_odd": lambda: configs.steady_small(nlay=2, nrow=13, ncol=17, nwel=3),
 }
 
 
@@ -221,3 +224,134 @@ def test_one_column_aggregates(pc):
               "max |h - h_oracle|", err)
         assert icnvg == 1, its
         assert err <= 10 * case.ims.inner_dvclose
+
+
+# ---------------------------------------------------------------------------
+# The operators themselves against a sparse numpy build (no dense Q)
+# ---------------------------------------------------------------------------
+
+def dyadic_dis_matrix(shape, seed, chd_columns=True):
+    """A symmetric, negative definite DIS-structured matrix whose entries
+    (and every partial sum of a row) are exact in FP32: off-diagonals k/8,
+    k in 1..24 (5 % zero), diagonals minus the row's sum minus j/8.  With
+    chd_columns the first and last grid columns are decoupled identity-like
+    rows (CHD cells after the sln_ls fixes)."""
+    from modflow6_amd.ims import dis_structure
+    nlay, nrow, ncol = shape
+    ia, ja = dis_structure(nlay, nrow, ncol)
+    ia0 = ia.astype(np.int64) - 1
+    ja0 = ja.astype(np.int64) - 1
+    n = nlay * nrow * ncol
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n), np.diff(ia0))
+    up = ja0 > rows
+    vals = rng.integers(1, 25, up.sum()) / 8.0
+    vals[rng.random(up.sum()) < 0.05] = 0.0
+    a = np.zeros(ia0[-1])
+    a[up] = vals
+    if chd_columns:
+        jcol = np.arange(n) % ncol
+        fixed = (jcol == 0) | (jcol == ncol - 1)
+        a[up & (fixed[rows] | fixed[ja0])] = 0.0
+    import scipy.sparse as sp
+    U = sp.csr_matrix((a, ja0, ia0), shape=(n, n))
+    S = (U + U.T).tocsr()
+    S.sort_indices()
+    lo = ja0 < rows
+    a[lo] = np.asarray(S[rows[lo], ja0[lo]]).ravel()
+    diag = ia0[:-1]
+    a[diag] = -(np.bincount(rows, weights=a, minlength=n) + rng.integers(0, 9, n) / 8.0) - 0.125
+    return ia, ja, a
+
+
+def sparse_coarse(shape, ia, ja, a, target):
+    """A, P and P^T A P as scipy sparse matrices (natural node order);
+    aggregates of ha x ha grid columns, P restricted to coupled rows, empty
+    aggregates given a -1 diagonal."""
+    import scipy.sparse as sp
+    nlay, nrow, ncol = shape
+    n = nlay * nrow * ncol
+    A = sp.csr_matrix((a, ja - 1, ia - 1), shape=(n, n))
+    off = A - sp.diags(A.diagonal())
+    coupled = np.asarray((off != 0).sum(axis=1)).ravel() > 0
+    ha, nai, naj = coarse_shape(nrow, ncol, target)
+    k, i, j = np.unravel_index(np.arange(n), (nlay, nrow, ncol))
+    agg = (i // ha) * naj + j // ha
+    nc = nai * naj
+    r = np.flatnonzero(coupled)
+    P = sp.csr_matrix((np.ones(r.size), (r, agg[r])), shape=(n, nc))
+    Ac = (P.T @ A @ P).tocsc()
+    empty = np.asarray(P.sum(axis=0)).ravel() == 0
+    Ac = Ac + sp.diags(np.where(empty, -1.0, 0.0))
+    return A, P, Ac.tocsc(), ha, naj
+
+
+DEFL_CASES = {
+    "3x20x24": ((3, 20, 24), 40),
+    "odd_2x13x17": ((2, 13, 17), 12),
+    "one_column_2x6x8": ((2, 6, 8), 64),
+    "4x30x40": ((4, 30, 40), 100),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DEFL_CASES))
+def test_deflated_operator_matches_sparse_build(name):
+    """op 6 (the deflated preconditioner of the fused CG: yc = Ac^-1 P^T r,
+    z = M^-1 (r - A P yc) + P yc) against yc from a sparse solve of
+    P^T A P, A P yc from the sparse product, and M^-1 from the device's ILU0
+    apply (op 1, bit-exact against the oracle's ilu0a).  The matrix entries
+    are exact in FP32, so the FP32 row sums and edge couplings the sweeps
+    read are exact and the only differences are the FP64 roundings of the
+    two coarse solves: tolerance 1e-9 of the largest |z|.  A wrong place
+    in the edge-coupling rows moves z by O(1)."""
+    import scipy.sparse.linalg as spla
+    shape, target = DEFL_CASES[name]
+    ia, ja, a = dyadic_dis_matrix(shape, seed=sum(shape))
+    assert np.array_equal(a.astype(np.float32).astype(np.float64), a)
+    A, P, Ac, ha, naj = sparse_coarse(shape, ia, ja, a, target)
+    if name.startswith("one_column"):
+        assert ha == 1
+    nlay, nrow, ncol = shape
+    case = configs.steady_small(nlay=nlay, nrow=nrow, ncol=ncol, nwel=0)
+    sv = device_solver(case, preconditioner=3, coarse_size=target)
+    try:
+        load_factor(sv, a)
+        rng = np.random.default_rng(4)
+        for _ in range(3):
+            r = rng.standard_normal(case.nodes)
+            yc = spla.spsolve(Ac, P.T @ r)
+            w = r - A @ (P @ yc)
+            want = apply_op(sv, 1, w) + P @ yc
+            got = apply_op(sv, 6, r)
+            err = np.max(np.abs(got - want))
+            print(name, "ha", ha, "max |z - z_sparse|", err, "scale", np.max(np.abs(want)))
+            assert err <= 1e-9 * np.max(np.abs(want))
+    finally:
+        sv.destroy()
+
+
+def test_coarse_operator_63_wide_aggregate_grid():
+    """The additive coarse part (op 3 - op 1) on a 63 x 63 aggregate grid
+    (nc = 3969, the widest the banded coarse Cholesky takes, as at the 10M
+    cell headline): P (P^T A P)^-1 P^T x from a sparse solve, without a dense
+    Q.  Tolerance 1e-9 of the largest value."""
+    import scipy.sparse.linalg as spla
+    shape, target = (2, 252, 252), 3969
+    ia, ja, a = dyadic_dis_matrix(shape, seed=9)
+    A, P, Ac, ha, naj = sparse_coarse(shape, ia, ja, a, target)
+    assert (ha, naj, P.shape[1]) == (4, 63, 3969)
+    nlay, nrow, ncol = shape
+    case = configs.steady_small(nlay=nlay, nrow=nrow, ncol=ncol, nwel=0)
+    sv = device_solver(case, preconditioner=2, coarse_size=target)
+    try:
+        load_factor(sv, a)
+        rng = np.random.default_rng(6)
+        for _ in range(2):
+            x = rng.standard_normal(case.nodes)
+            got = apply_op(sv, 3, x) - apply_op(sv, 1, x)
+            want = P @ spla.spsolve(Ac, P.T @ x)
+            err = np.max(np.abs(got - want))
+            print("63-wide: max |coarse - sparse|", err, "scale", np.max(np.abs(want)))
+            assert err <= 1e-9 * np.max(np.abs(want))
+    finally:
+        sv.destroy()

@@ -196,3 +196,44 @@ def test_small_model_golden_heads_current():
     icn, it, _, _, _ = sim.imslinear.apply(sim.amat, sim.rhs, sim.x, kstp=1)
     assert icn == 1 and it == int(g["iterations"])
     assert np.array_equal(sim.x.view(np.int64), g["heads"].view(np.int64))
+
+
+def test_f32_amux_restatement_reduces_to_amux():
+    """The FP32 amux restatement the fused-SpMV GPU test checks against
+    equals the oracle's amux on values FP32 holds exactly (small dyadic
+    numbers: every product and partial sum is exact in both precisions),
+    and differs from it on general values (it does round to FP32)."""
+    from tests.helpers import bits_equal, dis_matrix, f32_amux
+    ia, ja, a = dis_matrix((3, 5, 8), seed=1)
+    n = ia.size - 1
+    ad = np.round(a * 8) / 8
+    xd = np.round(np.random.default_rng(2).standard_normal(n) * 4) / 4
+    yo = np.zeros(n)
+    O.lib().orc_amux(n, xd, yo, ad, ja - 1, ia - 1)
+    assert bits_equal(f32_amux(ia, ja, ad, xd), yo)
+    x = np.random.default_rng(3).standard_normal(n)
+    O.lib().orc_amux(n, x, yo, a, ja - 1, ia - 1)
+    y32 = f32_amux(ia, ja, a, x)
+    assert not bits_equal(y32, yo)
+    assert np.max(np.abs(y32 - yo)) < 1e-5 * np.max(np.abs(yo))
+
+
+@pytest.mark.parametrize("shape", [(3, 5, 8), (2, 7, 9), (4, 6, 6), (1, 9, 4)])
+@pytest.mark.parametrize("relax", [0.0, 0.97])
+def test_rb_ilu_apply_restatement_is_ilu0a(shape, relax):
+    """The column red-black ILU0 apply restated in numpy colour sweeps (the
+    spec the FP32 / FP16 fused-sweep GPU test checks against) equals, in
+    FP64, the oracle's ilu0a on the permuted matrix bit for bit -- both
+    with a plain and a modified (MILU0) factor, on even and odd ncol."""
+    from tests.helpers import bits_equal, dis_matrix, oracle_rb_factor, oracle_rb_ilu0a, rb_ilu_apply
+    ia, ja, a = dis_matrix(shape, seed=sum(shape))
+    perm, ims, piv, stop = oracle_rb_factor(shape, ia, ja, a, relax)
+    assert stop == -1
+    r = np.random.default_rng(7).standard_normal(ia.size - 1)
+    want = oracle_rb_ilu0a(perm, ims, r)
+    got = rb_ilu_apply(shape, ia, ja, a, piv, r)
+    assert bits_equal(got, want)
+    # FP32 rounding moves it (the restatement does round), but not far
+    got32 = rb_ilu_apply(shape, ia, ja, a, piv, r, dtype=np.float32)
+    assert not bits_equal(got32, want)
+    assert np.max(np.abs(got32 - want)) < 1e-4 * np.max(np.abs(want))
