@@ -150,36 +150,49 @@ def cpu_baseline(case, budget_s=30.0):
 def transient_workload(args):
     """The transient Newton workload (5 x 500 x 500, 10 periods x 5 steps,
     BiCGSTAB + ILU0, NEWTON UNDER_RELAXATION) run end to end on the device:
-    every time step's formulate, linear solves and outer updates.  A step of
-    this workload is one time step; the metric is seconds per simulation."""
-    from modflow6_amd import configs
+    every time step's formulate, linear solves and outer updates, once with
+    FP64 BiCGSTAB (the reference algorithm) and once with the mixed-precision
+    BiCGSTAB (FP32 ILU0 + SpMV inner solves inside FP64 refinement; FP64
+    true residual, Newton update and under-relaxation).  A step of this
+    workload is one time step; the metric is seconds per simulation."""
+    from modflow6_amd import _lib, configs
     from modflow6_amd.solution import GwfSolution
     case = configs.transient_newton()
-    gs = GwfSolution(case)
-    t0 = time.perf_counter()
-    nsteps = 0
-    for kper, (perlen, nstp, steady) in enumerate(case.periods, start=1):
-        gs.kper = kper
-        gs.kstp = 0
-        for _ in range(nstp):
-            gs.run_timestep(perlen / nstp, steady)
-            nsteps += 1
-            log(f"time step {nsteps}: {len(gs.log[-1])} outer, "
-                f"{sum(r.inner for r in gs.log[-1])} inner, {time.perf_counter() - t0:.2f} s")
-    secs = time.perf_counter() - t0
-    heads = gs.heads()
-    gs.destroy()
-    out = dict(workload=case.name, fp64_s_per_simulation=secs, time_steps=nsteps,
-               outer=[len(s) for s in gs.log],
-               inner=[int(sum(r.inner for r in s)) for s in gs.log],
-               converged=all(s[-1].converged for s in gs.log),
-               mixed="not applicable: the mixed-precision solve is CG only; "
-                     "this workload is nonsymmetric (BiCGSTAB)")
     gold = os.path.join(ROOT, "tests", "golden", "transient_newton_heads_sample.npz")
-    if os.path.exists(gold):
-        g = np.load(gold)
-        out["max_head_diff_vs_oracle_m"] = float(np.max(np.abs(heads[g["cells"]] -
-                                                              g["heads_last_step"])))
+    g = np.load(gold) if os.path.exists(gold) else None
+    out = dict(workload=case.name)
+    for label, prec in (("fp64", _lib.MF6HIP_FP64), ("mixed", _lib.MF6HIP_MIXED),
+                        ("mixed16", _lib.MF6HIP_MIXED_FP16)):
+        if args.precision not in ("both", "all", label):
+            continue
+        gs = GwfSolution(case, precision=prec)
+        t0 = time.perf_counter()
+        nsteps = 0
+        for kper, (perlen, nstp, steady) in enumerate(case.periods, start=1):
+            gs.kper = kper
+            gs.kstp = 0
+            for _ in range(nstp):
+                gs.run_timestep(perlen / nstp, steady)
+                nsteps += 1
+                log(f"{label} time step {nsteps}: {len(gs.log[-1])} outer, "
+                    f"{sum(r.inner for r in gs.log[-1])} inner, "
+                    f"{time.perf_counter() - t0:.2f} s")
+        secs = time.perf_counter() - t0
+        heads = gs.heads()
+        gs.destroy()
+        rec = dict(s_per_simulation=secs, time_steps=nsteps,
+                   outer=[len(s) for s in gs.log],
+                   inner=[int(sum(r.inner for r in s)) for s in gs.log],
+                   converged=all(s[-1].converged for s in gs.log))
+        if g is not None:
+            rec["max_head_diff_vs_oracle_m"] = float(np.max(np.abs(heads[g["cells"]] -
+                                                                  g["heads_last_step"])))
+        out[label] = rec
+    if "fp64" in out:
+        for m in ("mixed", "mixed16"):
+            if m in out:
+                out[f"speedup_{m}_vs_fp64"] = out["fp64"]["s_per_simulation"] / \
+                    out[m]["s_per_simulation"]
     ref = os.path.join(ROOT, "tests", "golden", "transient_newton_oracle.json")
     if os.path.exists(ref):
         meta = json.load(open(ref))

@@ -39,13 +39,17 @@ def build_fortran():
 
 def build_oracle():
     """Compile the CPU oracle (oracle/liboracle.so) and, where the reference
-    sources are present, oracle/_ref/libmf6ref.so (IMS core) and
+    sources are present, oracle/_ref/libmf6ref.so (IMS core),
     oracle/_ref/libmf6refgwf.so (GWF packages and the solution's outer-loop
-    procedures)."""
+    procedures) and oracle/_ref/libmf6hipplugin.so (the reference-side
+    LinearSolverBaseType plugin with its test driver)."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-C", odir, "all"])
     if os.path.isdir("/root/reference/src"):
         _run(["make", "-C", odir, "ref", "refgwf"])
+        # the reference-side LinearSolverBaseType plugin (links libmf6hip.so)
+        if os.path.exists(os.path.join(HERE, "libmf6hip.so")):
+            _run(["make", "-C", odir, "plugin"])
 
 
 def build_all():

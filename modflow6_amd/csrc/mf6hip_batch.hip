@@ -756,11 +756,26 @@ struct BatchHandle {
   double *A64 = nullptr, *b64 = nullptr, *x64 = nullptr, *x0 = nullptr, *piv64 = nullptr;
   float *A32 = nullptr, *piv32 = nullptr, *d32 = nullptr, *z32 = nullptr, *zf32 = nullptr,
         *q32 = nullptr, *p0 = nullptr, *p1 = nullptr, *e32 = nullptr;
+  // the tile set in use (pointers into tsets[cpt - 1])
   BSys *dsys = nullptr;
   Tile *dtr = nullptr, *dtb = nullptr, *dti = nullptr;
   int ntr = 0, ntb = 0, nti = 0, T = 0;
   double *part = nullptr;
   MaxLoc *mpart = nullptr;
+  // per columns-per-thread value (1, 2): its tile lists, the systems' tile
+  // offsets, the partial arrays, and the captured iteration graphs per grid
+  // size; built once, kept until destroy, so a graph stays valid for reuse
+  struct TileSet {
+    bool built = false;
+    BSys *dsys = nullptr;
+    Tile *dtr = nullptr, *dtb = nullptr, *dti = nullptr;
+    int ntr = 0, ntb = 0, nti = 0, T = 0;
+    double *part = nullptr;
+    MaxLoc *mpart = nullptr;
+    std::vector<int> grids;
+    std::vector<hipGraphExec_t> execs;
+  } tsets[2];
+  int *h_act = nullptr;                // pinned: active flags staged to the device
   KState *st = nullptr;
   BFState *fs = nullptr;
   int *active = nullptr;
@@ -781,8 +796,7 @@ struct BatchHandle {
   int ntg = 0;
   double tl_setup_ms = 0.0;   // coarse setups at staging, summed
   hipStream_t stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;      // the graph of the selected (cpt, grid)
   bool finalized = false;
   double last_ms = 0.0;
   std::vector<int> iters, icnvg;
@@ -791,11 +805,17 @@ struct BatchHandle {
 
 namespace {
 
-void batch_free_graph(BatchHandle *b) {
-  if (b->gexec) hipGraphExecDestroy(b->gexec);
-  if (b->graph) hipGraphDestroy(b->graph);
+// every tile set and captured graph (destroy only)
+void batch_free_tilesets(BatchHandle *b) {
+  for (auto &t : b->tsets) {
+    for (hipGraphExec_t e : t.execs)
+      if (e) hipGraphExecDestroy(e);
+    void *bufs[] = {t.dsys, t.dtr, t.dtb, t.dti, t.part, t.mpart};
+    for (void *p : bufs)
+      if (p) hipFree(p);
+    t = BatchHandle::TileSet{};
+  }
   b->gexec = nullptr;
-  b->graph = nullptr;
 }
 
 template <typename T>
@@ -811,48 +831,61 @@ int balloc(T **p, size_t n) {
   return MF6HIP_OK;
 }
 
-// tile lists for a columns-per-thread value
+// tile lists for a columns-per-thread value: built on the first request,
+// then only selected (the device arrays and the graphs captured with them
+// stay valid)
 int build_tiles(BatchHandle *b, int cpt) {
-  std::vector<Tile> tr, tb, ti;
-  const int w = MF6_BLOCK * cpt;
-  for (int s = 0; s < b->nsys; s++) {
-    BSys &S = b->sys[s];
-    S.tr0 = (int)tr.size();
-    for (int h = 0; h < S.L.HP0; h += w) tr.push_back(Tile{s, h});
-    S.ntr = (int)tr.size() - S.tr0;
-    S.tb0 = (int)tb.size();
-    for (int h = 0; h < S.L.HP1; h += w) tb.push_back(Tile{s, h});
-    S.ntb = (int)tb.size() - S.tb0;
-    S.ti0 = (int)ti.size();
-    for (int h = 0; h < 2 * S.L.HP0; h += w) ti.push_back(Tile{s, h});
-    S.nti = (int)ti.size() - S.ti0;
+  BatchHandle::TileSet &ts = b->tsets[cpt - 1];
+  if (!ts.built) {
+    std::vector<Tile> tr, tb, ti;
+    std::vector<BSys> sys = b->sys;
+    const int w = MF6_BLOCK * cpt;
+    for (int s = 0; s < b->nsys; s++) {
+      BSys &S = sys[s];
+      S.tr0 = (int)tr.size();
+      for (int h = 0; h < S.L.HP0; h += w) tr.push_back(Tile{s, h});
+      S.ntr = (int)tr.size() - S.tr0;
+      S.tb0 = (int)tb.size();
+      for (int h = 0; h < S.L.HP1; h += w) tb.push_back(Tile{s, h});
+      S.ntb = (int)tb.size() - S.tb0;
+      S.ti0 = (int)ti.size();
+      for (int h = 0; h < 2 * S.L.HP0; h += w) ti.push_back(Tile{s, h});
+      S.nti = (int)ti.size() - S.ti0;
+    }
+    int T = (int)tr.size();
+    if ((int)tb.size() > T) T = (int)tb.size();
+    if ((int)ti.size() > T) T = (int)ti.size();
+    if (b->ntg > T) T = b->ntg;
+    BCK(balloc(&ts.dsys, sys.size()));
+    BCK(balloc(&ts.dtr, tr.size()));
+    BCK(balloc(&ts.dtb, tb.size()));
+    BCK(balloc(&ts.dti, ti.size()));
+    BCK(balloc(&ts.part, 6 * (size_t)T));
+    BCK(balloc(&ts.mpart, 4 * (size_t)T));
+    HIPCHK(hipMemcpy(ts.dtr, tr.data(), sizeof(Tile) * tr.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ts.dtb, tb.data(), sizeof(Tile) * tb.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ts.dti, ti.data(), sizeof(Tile) * ti.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ts.dsys, sys.data(), sizeof(BSys) * sys.size(), hipMemcpyHostToDevice));
+    ts.ntr = (int)tr.size();
+    ts.ntb = (int)tb.size();
+    ts.nti = (int)ti.size();
+    ts.T = T;
+    ts.built = true;
   }
-  b->ntr = (int)tr.size();
-  b->ntb = (int)tb.size();
-  b->nti = (int)ti.size();
-  int T = b->ntr;
-  if (b->ntb > T) T = b->ntb;
-  if (b->nti > T) T = b->nti;
-  if (b->ntg > T) T = b->ntg;
-  if (b->dtr) hipFree(b->dtr);
-  if (b->dtb) hipFree(b->dtb);
-  if (b->dti) hipFree(b->dti);
-  if (b->part) hipFree(b->part);
-  if (b->mpart) hipFree(b->mpart);
-  b->dtr = b->dtb = b->dti = nullptr;
-  b->part = nullptr;
-  b->mpart = nullptr;
-  b->T = T;
-  BCK(balloc(&b->dtr, tr.size()));
-  BCK(balloc(&b->dtb, tb.size()));
-  BCK(balloc(&b->dti, ti.size()));
-  BCK(balloc(&b->part, 6 * (size_t)T));
-  BCK(balloc(&b->mpart, 4 * (size_t)T));
-  HIPCHK(hipMemcpy(b->dtr, tr.data(), sizeof(Tile) * tr.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->dtb, tb.data(), sizeof(Tile) * tb.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->dti, ti.data(), sizeof(Tile) * ti.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->dsys, b->sys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice));
+  b->dsys = ts.dsys;
+  b->dtr = ts.dtr;
+  b->dtb = ts.dtb;
+  b->dti = ts.dti;
+  b->ntr = ts.ntr;
+  b->ntb = ts.ntb;
+  b->nti = ts.nti;
+  b->T = ts.T;
+  b->part = ts.part;
+  b->mpart = ts.mpart;
   b->cpt = cpt;
+  b->gexec = nullptr;
+  for (size_t g = 0; g < ts.grids.size(); g++)
+    if (ts.grids[g] == b->grid) b->gexec = ts.execs[g];
   return MF6HIP_OK;
 }
 
@@ -895,14 +928,28 @@ void launch_replacement(BatchHandle *b) {
                                          b->tl ? 1 : 0);
 }
 
+// The graph of the selected tile set and grid: captured (kernel launches
+// only) and instantiated on the first request, then reused.
 int capture(BatchHandle *b) {
-  batch_free_graph(b);
+  BatchHandle::TileSet &ts = b->tsets[b->cpt - 1];
+  for (size_t g = 0; g < ts.grids.size(); g++)
+    if (ts.grids[g] == b->grid) {
+      b->gexec = ts.execs[g];
+      return MF6HIP_OK;
+    }
+  hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < b->R; i++) launch_iteration(b);
   launch_replacement(b);
-  hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
-  if (e != hipSuccess) { mf6hip_set_error("graph capture failed"); return MF6HIP_EHIP; }
-  HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (e != hipSuccess || !graph) { mf6hip_set_error("graph capture failed"); return MF6HIP_EHIP; }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) { mf6hip_set_error("graph instantiation failed"); return MF6HIP_EHIP; }
+  ts.grids.push_back(b->grid);
+  ts.execs.push_back(exec);
+  b->gexec = exec;
   return MF6HIP_OK;
 }
 
@@ -921,8 +968,8 @@ int reset_states(BatchHandle *b) {
   HIPCHK(hipMemsetAsync(b->fs, 0, sizeof(BFState) * b->nsys, b->stream));
   HIPCHK(hipMemsetAsync(b->q32, 0, sizeof(float) * b->vtot, b->stream));
   HIPCHK(hipMemsetAsync(b->e32, 0, sizeof(float) * b->vtot, b->stream));
-  std::vector<int> act(b->nsys, 1);
-  HIPCHK(hipMemcpyAsync(b->active, act.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice, b->stream));
+  for (int s = 0; s < b->nsys; s++) b->h_act[s] = 1;
+  HIPCHK(hipMemcpyAsync(b->active, b->h_act, sizeof(int) * b->nsys, hipMemcpyHostToDevice, b->stream));
   launch_replacement(b);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -1032,6 +1079,7 @@ extern "C" int mf6hip_batch_finalize(void *handle) {
   BCK(balloc(&b->rtrue, b->nsys));
   HIPCHK(hipHostMalloc((void **)&b->h_st, sizeof(KState) * b->nsys, hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&b->h_rtrue, sizeof(double) * b->nsys, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&b->h_act, sizeof(int) * b->nsys, hipHostMallocDefault));
   if (b->tl) {
     std::vector<GTask> tg;
     for (int s = 0; s < b->nsys; s++) {
@@ -1157,10 +1205,9 @@ extern "C" int mf6hip_batch_autotune(void *handle, char *report, int report_len)
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  BCK(build_tiles(b, bc));
   b->grid = bg;
+  BCK(build_tiles(b, bc));
   HIPCHK(hipMemcpy(b->x64, xsave, sizeof(double) * b->vtot, hipMemcpyDeviceToDevice));
-  batch_free_graph(b);
   rep += "chosen cpt " + std::to_string(bc) + " grid " + std::to_string(bg);
   if (b->tl) rep += "; coarse setups at staging " + std::to_string(b->tl_setup_ms) + " ms";
   b->tune = rep;
@@ -1186,7 +1233,8 @@ extern "C" int mf6hip_batch_solve(void *handle, int *icnvg, int *iters, double *
                         b->stream));
   BCK(reset_states(b));
   BCK(capture(b));
-  std::vector<int> done(b->nsys, 0), act(b->nsys, 1);
+  std::vector<int> done(b->nsys, 0);
+  int *act = b->h_act;   // pinned; all 1 after reset_states
   for (int s = 0; s < b->nsys; s++) { icnvg[s] = 0; iters[s] = 0; }
   int ndone = 0, replays = 0;
   const int max_replays = 1000000;
@@ -1221,7 +1269,7 @@ extern "C" int mf6hip_batch_solve(void *handle, int *icnvg, int *iters, double *
     }
     if (changed) {
       HIPCHK(hipMemcpyAsync(b->st, b->h_st, sizeof(KState) * b->nsys, hipMemcpyHostToDevice, b->stream));
-      HIPCHK(hipMemcpyAsync(b->active, act.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice, b->stream));
+      HIPCHK(hipMemcpyAsync(b->active, act, sizeof(int) * b->nsys, hipMemcpyHostToDevice, b->stream));
     }
   }
   for (int s = 0; s < b->nsys; s++)
@@ -1253,16 +1301,16 @@ extern "C" int mf6hip_batch_destroy(void *handle) {
   BatchHandle *b = (BatchHandle *)handle;
   if (!b) return MF6HIP_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
-  batch_free_graph(b);
+  batch_free_tilesets(b);
   void *bufs[] = {b->A64, b->b64, b->x64, b->x0, b->piv64, b->A32, b->piv32, b->d32, b->z32, b->zf32,
-                  b->q32, b->p0, b->p1, b->e32, b->dsys, b->dtr, b->dtb, b->dti, b->part,
-                  b->mpart, b->st, b->fs, b->active, b->rtrue, b->dtg, b->tl_cmask,
+                  b->q32, b->p0, b->p1, b->e32, b->st, b->fs, b->active, b->rtrue, b->dtg, b->tl_cmask,
                   b->tl_colsum, b->tl_rc, b->tl_yc, b->tl_A32, b->tl_S3, b->tl_Lc, b->tl_Lr,
                   b->tl_G, b->tl_dinfo};
   for (void *p : bufs)
     if (p) hipFree(p);
   if (b->h_st) hipHostFree(b->h_st);
   if (b->h_rtrue) hipHostFree(b->h_rtrue);
+  if (b->h_act) hipHostFree(b->h_act);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return MF6HIP_OK;

@@ -1241,7 +1241,7 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
                                                         const double *part,
                                                         const MaxLoc *mpart, int nblk,
                                                         int ncol, double *hist, int *hloc,
-                                                        int ncoarse = 0) {
+                                                        int ncoarse = 0, int nonsym = 0) {
   __shared__ double lds[4][MF6_RBLOCK / 64];
   __shared__ MaxLoc mlds[2][MF6_RBLOCK / 64];
   // the state is read before the partials (and the stop flag is only acted
@@ -1330,6 +1330,12 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
     st->beta = 0.0;
     fs->restart = 0;
   }
+  // The deflated preconditioner z = M^-1 (r - A Q r) + Q r (A-DEF1) is not
+  // symmetric, so r.z is not guaranteed positive.  A nonpositive rho (this
+  // one or the last) restarts the direction, p = z: its step alpha =
+  // r.z / p.Ap is then the exact minimiser of the A-norm error along p, so
+  // the error does not grow.
+  if (nonsym && (rho_new <= 0.0 || (it > 0 && S.rho <= 0.0))) st->beta = 0.0;
 }
 
 __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,
@@ -1560,7 +1566,7 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
       mf6hip_tl_enqueue<double>(h, (const double *)nullptr, h->tl_yc64, h->d_part + P_RHO_C * nb,
                                 nb, st);
     k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                      h->d_hloc, tl ? nb : 0);
+                                      h->d_hloc, tl ? nb : 0, defl ? 1 : 0);
     double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
     launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs,
                                      tl ? h->tl_yc64 : nullptr, h->tl_cmask, h->tl);
@@ -1687,7 +1693,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
           mf6hip_tl_enqueue<float>(h, (const float *)nullptr, h->tl_yc32,
                                    h->d_part + P_RHO_C * nb, nb, st);
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                          h->d_hloc, tl ? nb : 0);
+                                          h->d_hloc, tl ? nb : 0, defl ? 1 : 0);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
         launch_f_pupdate<float, float>(L, nb, s, h->z32, pin, pout, h->e32, h->d_mpart, st, fs,
                                        tl ? h->tl_yc32 : nullptr, h->tl_cmask, h->tl);

@@ -77,6 +77,7 @@ struct Mf6Handle {
   double *d_amax = nullptr;
   float *r32 = nullptr, *e32 = nullptr, *p32 = nullptr, *p232 = nullptr,
         *q32 = nullptr, *z32 = nullptr, *zf32 = nullptr;
+  float *dhat32 = nullptr, *qhat32 = nullptr, *t32 = nullptr;   // mixed BiCGSTAB
   // reductions / state
   double *d_part = nullptr;   // [8][MF6_MAXBLK]
   MaxLoc *d_mpart = nullptr;  // [4][MF6_MAXBLK]

@@ -610,22 +610,24 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_cg_update(Layout L, TX *__restric
 }
 
 // BiCGSTAB: p = d (first) or d + beta (p - omega0 v)
-__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_p(int n, const double *__restrict__ d,
-                                                      double *__restrict__ p,
-                                                      const double *__restrict__ v,
+template <typename TV>
+__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_p(int n, const TV *__restrict__ d,
+                                                      TV *__restrict__ p,
+                                                      const TV *__restrict__ v,
                                                       const KState *st) {
   if (st->stop) return;
   const bool first = st->iter == 0;
-  const double beta = st->beta, omega0 = st->omega0;
+  const TV beta = (TV)st->beta, omega0 = (TV)st->omega0;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
     p[e] = first ? d[e] : d[e] + beta * (p[e] - omega0 * v[e]);
 }
 
 // BiCGSTAB: v = A phat with partial dhat.v
-__global__ void __launch_bounds__(MF6_BLOCK) k_spmv_dot(Layout L, const double *__restrict__ A,
-                                                        const double *__restrict__ x,
-                                                        double *__restrict__ y,
-                                                        const double *__restrict__ w,
+template <typename TM, typename TV>
+__global__ void __launch_bounds__(MF6_BLOCK) k_spmv_dot(Layout L, const TM *__restrict__ A,
+                                                        const TV *__restrict__ x,
+                                                        TV *__restrict__ y,
+                                                        const TV *__restrict__ w,
                                                         double *__restrict__ part,
                                                         const KState *st) {
   __shared__ double lds[8];
@@ -635,30 +637,32 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_spmv_dot(Layout L, const double *
        p += gridDim.x * blockDim.x) {
     int c, k, i, j, l;
     if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
-    double t = row_dot(L, A, x, p, c, k, i, j, l);
+    TV t = row_dot(L, A, x, p, c, k, i, j, l);
     y[p] = t;
-    s = s + w[p] * t;
+    s = s + (double)w[p] * (double)t;
   }
   s = block_sum(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // BiCGSTAB: q = d - alpha v
-__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_q(int n, const double *__restrict__ d,
-                                                      const double *__restrict__ v,
-                                                      double *__restrict__ q,
+template <typename TV>
+__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_q(int n, const TV *__restrict__ d,
+                                                      const TV *__restrict__ v,
+                                                      TV *__restrict__ q,
                                                       const KState *st) {
   if (st->stop) return;
-  const double alpha = st->alpha;
+  const TV alpha = (TV)st->alpha;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
     q[e] = d[e] - alpha * v[e];
 }
 
 // BiCGSTAB: t = A qhat with partial t.q and t.t
-__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_t(Layout L, const double *__restrict__ A,
-                                                      const double *__restrict__ qhat,
-                                                      double *__restrict__ t,
-                                                      const double *__restrict__ q,
+template <typename TM, typename TV>
+__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_t(Layout L, const TM *__restrict__ A,
+                                                      const TV *__restrict__ qhat,
+                                                      TV *__restrict__ t,
+                                                      const TV *__restrict__ q,
                                                       double *__restrict__ part,
                                                       const KState *st, int nblk) {
   __shared__ double lds[8];
@@ -668,10 +672,10 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_t(Layout L, const double *__
        p += gridDim.x * blockDim.x) {
     int c, k, i, j, l;
     if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
-    double tv = row_dot(L, A, qhat, p, c, k, i, j, l);
+    TV tv = row_dot(L, A, qhat, p, c, k, i, j, l);
     t[p] = tv;
-    s1 = s1 + tv * q[p];
-    s2 = s2 + tv * tv;
+    s1 = s1 + (double)tv * (double)q[p];
+    s2 = s2 + (double)tv * (double)tv;
   }
   s1 = block_sum(s1, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s1;
@@ -680,19 +684,20 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_t(Layout L, const double *__
 }
 
 // BiCGSTAB update: x += alpha phat + omega qhat ; d = q - omega t
-__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_update(Layout L, double *__restrict__ x,
-                                                           const double *__restrict__ phat,
-                                                           const double *__restrict__ qhat,
-                                                           double *__restrict__ d,
-                                                           const double *__restrict__ q,
-                                                           const double *__restrict__ t,
+template <typename TV, typename TX>
+__global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_update(Layout L, TX *__restrict__ x,
+                                                           const TV *__restrict__ phat,
+                                                           const TV *__restrict__ qhat,
+                                                           TV *__restrict__ d,
+                                                           const TV *__restrict__ q,
+                                                           const TV *__restrict__ t,
                                                            double *__restrict__ part,
                                                            MaxLoc *__restrict__ mpart,
                                                            const KState *st, int nblk) {
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
-  const double alpha = st->alpha, omega = st->omega;
+  const TV alpha = (TV)st->alpha, omega = (TV)st->omega;
   double s = 0.0;
   MaxLoc mx{0.0, -1}, mr{0.0, -1};
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.NP;
@@ -700,13 +705,13 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_update(Layout L, double *__r
     int c, k, i, j, l;
     if (!pos_to_cell(L, e, c, k, i, j, l)) continue;
     int nat = natural(L, k, i, j);
-    double tv = alpha * phat[e] + omega * qhat[e];
-    x[e] = x[e] + tv;
-    mx = maxloc_pick(mx, MaxLoc{tv, nat});
+    TV tv = alpha * phat[e] + omega * qhat[e];
+    x[e] = (TX)(x[e] + (TX)tv);
+    mx = maxloc_pick(mx, MaxLoc{(double)tv, nat});
     tv = q[e] - omega * t[e];
     d[e] = tv;
-    mr = maxloc_pick(mr, MaxLoc{tv, nat});
-    s = s + tv * tv;
+    mr = maxloc_pick(mr, MaxLoc{(double)tv, nat});
+    s = s + (double)tv * (double)tv;
   }
   s = block_sum(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -1172,18 +1177,18 @@ static int run_bcgs(Mf6Handle *h) {
     for (int it = 0; it < nb_it; it++) {
       k_dot<double><<<nb, MF6_BLOCK, 0, s>>>(L.NP, h->dhat, h->d, h->d_part, st);
       k_s_bcgs_rho<<<1, MF6_BLOCK, 0, s>>>(st, h->d_part, nb);
-      k_bcgs_p<<<g, MF6_BLOCK, 0, s>>>(L.NP, h->d, h->p, h->v, st);
+      k_bcgs_p<double><<<g, MF6_BLOCK, 0, s>>>(L.NP, h->d, h->p, h->v, st);
       launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, h->p, h->zf, h->phat, st);
       int slot;
       timed_spmv_begin(h, slot);
-      k_spmv_dot<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->phat, h->v, h->dhat, h->d_part, st);
+      k_spmv_dot<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->phat, h->v, h->dhat, h->d_part, st);
       timed_spmv_end(h, slot);
       k_s_bcgs_alpha<<<1, MF6_BLOCK, 0, s>>>(st, h->d_part, nb);
-      k_bcgs_q<<<g, MF6_BLOCK, 0, s>>>(L.NP, h->d, h->v, h->q, st);
+      k_bcgs_q<double><<<g, MF6_BLOCK, 0, s>>>(L.NP, h->d, h->v, h->q, st);
       launch_ilu_apply<double, double, double>(h, h->A, h->invpiv, h->q, h->zf, h->qhat, st);
-      k_bcgs_t<<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->qhat, h->t, h->q, h->d_part, st, nb);
+      k_bcgs_t<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->qhat, h->t, h->q, h->d_part, st, nb);
       k_s_bcgs_omega<<<1, MF6_BLOCK, 0, s>>>(st, h->d_part, nb);
-      k_bcgs_update<<<nb, MF6_BLOCK, 0, s>>>(L, h->x, h->phat, h->qhat, h->d, h->q, h->t,
+      k_bcgs_update<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->x, h->phat, h->qhat, h->d, h->q, h->t,
                                              h->d_part, h->d_mpart, st, nb);
       k_s_iter_end<<<1, MF6_BLOCK, 0, s>>>(st, h->d_part, h->d_mpart, nb, h->d_hist,
                                            h->d_hloc, 1);
@@ -1320,6 +1325,111 @@ static int run_mixed_cg(Mf6Handle *h, const TP *piv32) {
   return MF6HIP_OK;
 }
 
+// inner BiCGSTAB end: stop when max|r| <= rclose (FP32 inner target), or at
+// the breakdowns ims_base_bcgs exits on (ImsLinearBase.f90:475-505): rho,
+// alpha or omega unchanged to within DSAME, rho * omega == 0
+__global__ void k_s_inner_end_bcgs(KState *st, const MaxLoc *mpart, int nb, int itmax) {
+  __shared__ MaxLoc mlds[8];
+  if (st->stop) return;
+  MaxLoc mx = reduce_mparts(mpart, nb, mlds);
+  MaxLoc mr = reduce_mparts(mpart + nb, nb, mlds);
+  if (threadIdx.x != 0) return;
+  st->iter = st->iter + 1;
+  st->deltax = mx.v;
+  st->xloc = mx.loc;
+  st->rmax = mr.v;
+  st->rloc = mr.loc;
+  if (fabs(mr.v) <= st->rclose) { st->icnvg = 1; st->stop = 1; return; }
+  if (is_close_d(st->rho, st->rho0) || is_close_d(st->alpha, st->alpha0) ||
+      is_close_d(st->omega, st->omega0) || st->rho * st->omega == 0.0) {
+    st->stop = 1;
+    return;
+  }
+  st->rho0 = st->rho;
+  st->alpha0 = st->alpha;
+  st->omega0 = st->omega;
+  if (st->iter >= itmax) st->stop = 1;
+}
+
+// Mixed precision BiCGSTAB (nonsymmetric Newton systems): FP32 ILU0 +
+// FP32 SpMV BiCGSTAB (ims_base_bcgs, ImsLinearBase.f90:249-549, ISCL = 0)
+// inner solves inside FP64 iterative refinement.  Each refinement step
+// restarts the inner BiCGSTAB from the FP64 true residual b - A x (FP32
+// copy, also the shadow residual dhat) and reduces it by inner_fp32_rtol
+// (default 1e-4) or to half of RCLOSE; the correction is added to the FP64
+// heads.  Convergence is the FP64 true residual within RCLOSE with the last
+// inner step's max |dx| within DVCLOSE, as for the mixed CG.
+template <typename TP>
+static int run_mixed_bcgs(Mf6Handle *h, const TP *piv32) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  int nb = h->nblk;
+  int g = grid_for(L.N);
+  KState *ist = h->d_state + 1;
+  KState *hs = h->h_state;
+  double rtol = h->s.inner_fp32_rtol > 0 ? h->s.inner_fp32_rtol : 1e-4;
+  int inner_max = h->s.inner_fp32_max > 0 ? h->s.inner_fp32_max : h->s.iter1;
+  int total = 0, icnvg = 0;
+  double last_dv = 1e300;
+  float *r = h->r32, *p = h->p32, *v = h->p232, *q = h->q32, *phat = h->z32,
+        *qhat = h->qhat32, *t = h->t32, *dhat = h->dhat32, *e = h->e32;
+  HIPCHK(hipMemsetAsync(e, 0, sizeof(float) * L.NP, s));
+  HIPCHK(hipMemsetAsync(v, 0, sizeof(float) * L.NP, s));
+  MaxLoc *mres = (MaxLoc *)(h->d_part + 7 * MF6_MAXBLK);
+  for (int refine = 0; total < h->s.iter1; refine++) {
+    k_residual<double, double><<<g, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->d, nullptr);
+    k_maxloc_ss<<<nb, MF6_BLOCK, 0, s>>>(L, h->d, h->d_part, h->d_mpart, 0.0);
+    k_s_maxloc<<<1, MF6_BLOCK, 0, s>>>(h->d_mpart, nb, mres);
+    MaxLoc hm;
+    HIPCHK(hipMemcpyAsync(&hm, mres, sizeof(MaxLoc), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (fabs(hm.v) <= h->s.rclose && fabs(last_dv) <= h->s.dvclose) { icnvg = 1; break; }
+    if (hm.v == 0.0) { icnvg = 1; break; }
+    k_convert<double, float><<<g, MF6_BLOCK, 0, s>>>(L.NP, h->d, r);
+    HIPCHK(hipMemcpyAsync(dhat, r, sizeof(float) * L.NP, hipMemcpyDeviceToDevice, s));
+    k_s_inner_init<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_part, h->d_mpart, nb, rtol,
+                                           0.5 * h->s.rclose);
+    int batch = 4, done = 0;
+    while (done < inner_max && total + done < h->s.iter1) {
+      for (int it = 0; it < batch; it++) {
+        k_dot<float><<<nb, MF6_BLOCK, 0, s>>>(L.NP, dhat, r, h->d_part, ist);
+        k_s_bcgs_rho<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_part, nb);
+        k_bcgs_p<float><<<g, MF6_BLOCK, 0, s>>>(L.NP, r, p, v, ist);
+        launch_ilu_apply<float, float, TP>(h, h->A32, piv32, p, h->zf32, phat, ist);
+        int slot;
+        timed_spmv_begin(h, slot);
+        k_spmv_dot<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->A32, phat, v, dhat, h->d_part,
+                                                          ist);
+        timed_spmv_end(h, slot);
+        k_s_bcgs_alpha<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_part, nb);
+        k_bcgs_q<float><<<g, MF6_BLOCK, 0, s>>>(L.NP, r, v, q, ist);
+        launch_ilu_apply<float, float, TP>(h, h->A32, piv32, q, h->zf32, qhat, ist);
+        k_bcgs_t<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, h->A32, qhat, t, q, h->d_part, ist,
+                                                        nb);
+        k_s_bcgs_omega<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_part, nb);
+        k_bcgs_update<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, e, phat, qhat, r, q, t,
+                                                             h->d_part, h->d_mpart, ist, nb);
+        k_s_inner_end_bcgs<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_mpart, nb, inner_max);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(hs + 1, ist, sizeof(KState), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      done += batch;
+      if (hs[1].stop) break;
+      if (batch < 32) batch *= 2;
+    }
+    int inner = hs[1].iter;
+    total += inner;
+    last_dv = hs[1].deltax;
+    k_ir_correct<<<g, MF6_BLOCK, 0, s>>>(L, h->x, e);
+    if (inner == 0) break;
+  }
+  hs->iter = total;
+  hs->icnvg = icnvg;
+  hs->deltax = last_dv;
+  return MF6HIP_OK;
+}
+
 // Solve the device-resident system (h->A, h->b, h->x): factor, residual,
 // Krylov.  Equivalent of imslinear_ap without scaling / reordering.
 int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_iter,
@@ -1350,6 +1460,11 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
       CK(mf6hip_run_cg_reliable(h, &ic, &its));
       h->h_state->icnvg = ic;
       h->h_state->iter = its;
+    } else if (h->s.ilinmeth == 2) {
+      int rc;
+      if (h->s.precision == MF6HIP_MIXED) rc = run_mixed_bcgs<float>(h, h->invpiv32);
+      else rc = run_mixed_bcgs<unsigned short>(h, h->invpiv16);
+      if (rc) return rc;
     } else {
       int rc;
       if (h->s.precision == MF6HIP_MIXED) rc = run_mixed_cg<float>(h, h->invpiv32);
@@ -1425,7 +1540,6 @@ static int check_settings(const mf6hip_ims_settings *s) {
   if (s->north < 0) { mf6hip_set_error("NUMBER_ORTHOGONALIZATIONS must be >= 0"); return MF6HIP_EINVAL; }
   if (s->iter1 < 0) { mf6hip_set_error("INNER_MAXIMUM must be >= 0"); return MF6HIP_EINVAL; }
   if (s->precision < 0 || s->precision > 2) { mf6hip_set_error("unknown precision mode"); return MF6HIP_EINVAL; }
-  if (s->precision != MF6HIP_FP64 && s->ilinmeth != 1) { mf6hip_set_error("mixed precision is implemented for CG only"); return MF6HIP_EUNSUPPORTED; }
   if (s->ordering != MF6HIP_ORDER_RB) { mf6hip_set_error("unknown ordering"); return MF6HIP_EINVAL; }
   if (s->kernel_variant < 0 || s->kernel_variant > 1) { mf6hip_set_error("unknown kernel variant"); return MF6HIP_EINVAL; }
   if (s->preconditioner < 0 || s->preconditioner > 3) { mf6hip_set_error("unknown preconditioner"); return MF6HIP_EINVAL; }
@@ -1546,6 +1660,11 @@ static int alloc_solver(Mf6Handle *h) {
     }
     CK(dalloc(&h->r32, N));
     CK(dalloc(&h->e32, N));
+    if (h->s.ilinmeth == 2) {
+      CK(dalloc(&h->dhat32, N));
+      CK(dalloc(&h->qhat32, N));
+      CK(dalloc(&h->t32, N));
+    }
     CK(dalloc(&h->p32, N));
     CK(dalloc(&h->p232, N));
     CK(dalloc(&h->q32, N));
@@ -1566,8 +1685,9 @@ static int alloc_solver(Mf6Handle *h) {
     size_t dn[] = {MF6_NSLOT * N, N, N, N, N, N, N, N, N, N, N, N, N, N, N, N, N, N};
     for (int t = 0; t < (int)(sizeof(dz) / sizeof(dz[0])); t++)
       if (dz[t]) HIPCHK(hipMemset(dz[t], 0, sizeof(double) * dn[t]));
-    float *fz[] = {h->A32, h->invpiv32, h->r32, h->e32, h->p32, h->p232, h->q32, h->z32, h->zf32};
-    size_t fn[] = {MF6_NSLOT * N, N, N, N, N, N, N, N, N};
+    float *fz[] = {h->A32, h->invpiv32, h->r32, h->e32, h->p32, h->p232, h->q32, h->z32, h->zf32,
+                   h->dhat32, h->qhat32, h->t32};
+    size_t fn[] = {MF6_NSLOT * N, N, N, N, N, N, N, N, N, N, N, N};
     for (int t = 0; t < (int)(sizeof(fz) / sizeof(fz[0])); t++)
       if (fz[t]) HIPCHK(hipMemset(fz[t], 0, sizeof(float) * fn[t]));
     if (h->invpiv16) HIPCHK(hipMemset(h->invpiv16, 0, sizeof(unsigned short) * N));
@@ -1663,7 +1783,7 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
                   h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax, h->sat,
-                  h->flowja, h->strg, h->bsim, h->chd_flag};
+                  h->flowja, h->strg, h->bsim, h->chd_flag, h->dhat32, h->qhat32, h->t32};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);

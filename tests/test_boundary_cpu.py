@@ -127,7 +127,7 @@ def _create(settings, nlay=2, nrow=3, ncol=4, ia=None, ja=None, neq=None):
     (dict(relax=1.5), 1, "RELAXATION_FACTOR"),
     (dict(rclose=0.0), 1, "INNER_RCLOSE"),
     (dict(north=-1), 1, "ORTHOGONALIZATIONS"),
-    (dict(precision=1, ilinmeth=2), 3, "CG only"),
+    (dict(precision=1, ilinmeth=2, preconditioner=2), 3, "two-level"),
     (dict(kernel_variant=7), 1, "kernel variant"),
 ])
 def test_create_rejects_unsupported_settings(kw, code, msg):

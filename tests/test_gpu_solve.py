@@ -163,3 +163,56 @@ def test_fused_cg_graph_replay_matches_eager_launches():
         assert [r.inner for r in gs.log[0]] == inner_e
         assert np.array_equal(hg.view(np.int64), he.view(np.int64))
     gs.destroy()
+
+
+@pytest.mark.parametrize("precision", [_lib.MF6HIP_MIXED, _lib.MF6HIP_MIXED_FP16])
+def test_mixed_bicgstab_newton_system(precision):
+    """Mixed-precision BiCGSTAB (FP32 ILU0 + SpMV BiCGSTAB inside FP64
+    refinement) on a nonsymmetric Newton system: heads within 10 x
+    INNER_DVCLOSE of the oracle's FP64 BiCGSTAB + ILU0 (the reference
+    algorithm), FP64 true residual within INNER_RCLOSE."""
+    case = configs.transient_newton(seed=5, nlay=3, nrow=30, ncol=32, nwel=10,
+                                    nper=1, nstp=2, perlen=10.0)
+    sim, amat, rhs, x0 = oracle_system(case)
+    xn = x0.copy()
+    ims = O.ImsLinear(sim.dis.ia, sim.dis.ja, sim.settings)
+    icn_n, it_n, *_ = ims.apply(amat.copy(), rhs.copy(), xn, kstp=1)
+    assert icn_n == 1
+    sv = device_solver(case, precision=precision)
+    xd = x0.copy()
+    try:
+        icn_d, it_d = sv.solve(1, amat.copy(), rhs.copy(), xd, kstp=1)
+    finally:
+        sv.destroy()
+    d = sim.dis
+    r = rhs.copy()
+    ax = np.zeros_like(r)
+    O.lib().orc_amux(sim.neq, xd, ax, amat, d.ja, d.ia)
+    rtrue = np.max(np.abs(r - ax))
+    err = np.max(np.abs(xd - xn))
+    print(f"mixed BiCGSTAB precision {precision}: iterations {it_d} (oracle FP64 {it_n}), "
+          f"max |h - h_oracle| {err:.3e}, max |b - A x| {rtrue:.3e}")
+    assert icn_d == 1
+    assert err < 10 * case.ims.inner_dvclose
+    assert rtrue <= case.ims.inner_rclose
+
+
+@pytest.mark.parametrize("precision", [_lib.MF6HIP_MIXED, _lib.MF6HIP_MIXED_FP16])
+def test_mixed_bicgstab_transient_simulation(precision):
+    """A whole transient unconfined Newton run (NEWTON UNDER_RELAXATION,
+    drains, storage) with the mixed BiCGSTAB on the device-resident model:
+    every step converges and the heads stay within 10 x the simulation
+    tolerance of the oracle's FP64 run, as the FP64 device run does."""
+    case = configs.transient_newton(seed=3, nlay=3, nrow=40, ncol=40, nwel=16,
+                                    nper=2, nstp=3, perlen=20.0)
+    ho = O.GwfSimulation(case).run()
+    gs = GwfSolution(case, precision=precision)
+    try:
+        hd = gs.run()
+    finally:
+        gs.destroy()
+    err = np.max(np.abs(hd - ho))
+    print("mixed BiCGSTAB outer iterations per step", [len(s) for s in gs.log],
+          "max |h - h_oracle|", err)
+    assert all(s[-1].converged for s in gs.log)
+    assert err < 10 * case.ims.inner_dvclose * 10
