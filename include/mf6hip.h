@@ -287,6 +287,66 @@ int mf6hip_gwf_outer_update(void *handle, int kiter, mf6hip_underrelax *ur,
                             int *inewtonur, double *dxold_max, double *hncg,
                             int *lrch);
 
+/* --- XMI-style outer loop (srcbmi/mf6xmi.F90:220-340) -------------------
+ *
+ * prepare_solve / solve / finalize_solve of the coupling interface for the
+ * handle's one GWF model: each solve call is one NumericalSolution outer
+ * iteration (NumericalSolution.f90:1482-1837) run on the device entry points
+ * above -- backtracking, formulate, pseudo-transient continuation (steady
+ * Newton models), sln_ls, the OUTER_DVCLOSE test, under-relaxation and
+ * Newton under-relaxation -- so a coupler loops
+ *     prepare_solve; do { solve(&c) } while (!c && k < OUTER_MAXIMUM);
+ *     finalize_solve
+ * and can exchange heads with mf6hip_gwf_get_x_device / set_x_device between
+ * the calls, as it reads and writes SLN_1/X through get_value_ptr
+ * (srcbmi/mf6bmi.f90:707). */
+
+/* IMS NONLINEAR block (sln_ar, NumericalSolution.f90:790-900 with the
+ * complexity presets of sln_setouter :2623-2671 already applied). */
+typedef struct mf6hip_nonlinear {
+  double outer_dvclose;     /* OUTER_DVCLOSE */
+  int outer_maximum;        /* OUTER_MAXIMUM */
+  int nonmeth;              /* UNDER_RELAXATION: 0 NONE, 1 SIMPLE, 2 COOLEY, 3 DBD */
+  double gamma, theta, akappa, amomentum;   /* UNDER_RELAXATION_GAMMA / THETA /
+                                               KAPPA / MOMENTUM */
+  int backtracking_number;  /* BACKTRACKING_NUMBER (0: none) */
+  double backtracking_tolerance, backtracking_reduction_factor,
+      backtracking_residual_limit;
+  int iallowptc;            /* 1 PTC allowed, 0 NO_PTC ALL, -1 NO_PTC FIRST */
+  int iptcopt;              /* DEV_PTC_OPTION */
+  double ptcexp, ptcdel0;   /* DEV_PTC_EXPONENT, DEV_PTC_DEL0 */
+} mf6hip_nonlinear;
+
+/* What one solve call did (the ConvergenceSummary entries of an outer
+ * iteration plus the NONLINEAR actions). */
+typedef struct mf6hip_xmi_iteration {
+  int kiter;          /* outer iteration (1-based within the time step) */
+  int inner;          /* inner (linear) iterations */
+  int icnvg_inner;    /* linear solve converged */
+  double hncg;        /* signed largest head change */
+  int lrch;           /* its 1-based node */
+  int converged;      /* outer test passed */
+  int nur;            /* Newton under-relaxation moved a head */
+  int nbt;            /* backtracking steps */
+  int iptc;           /* pseudo-transient continuation terms added */
+} mf6hip_xmi_iteration;
+
+/* Set the NONLINEAR block (once, before the first prepare_solve); resets
+ * the Cooley / PTC / backtracking state. */
+int mf6hip_xmi_set_nonlinear(void *handle, const mf6hip_nonlinear *nl);
+/* xmi_prepare_solve: start time step kstp of period kper (iss = 1 steady,
+ * delt its length): gwf_ad + chd_ad, outer iteration counter reset. */
+int mf6hip_xmi_prepare_solve(void *handle, int kper, int kstp, int iss, double delt);
+/* xmi_solve: one outer iteration; *has_converged = 1 when it met the outer
+ * convergence test. */
+int mf6hip_xmi_solve(void *handle, int *has_converged);
+/* The record of the last solve call. */
+int mf6hip_xmi_last_iteration(void *handle, mf6hip_xmi_iteration *rec);
+/* xmi_finalize_solve: whether the time step converged and how many outer
+ * iterations it took (either pointer may be NULL).  Non-convergence is not
+ * an error (BMI_SUCCESS in the reference). */
+int mf6hip_xmi_finalize_solve(void *handle, int *converged, int *outer_iterations);
+
 /* Timing of the last mf6hip_gwf_linear_solve on the device stream (HIP
  * events): total milliseconds and the average duration of its dominant
  * kernel (the SpMV) with its launch count. */

@@ -40,6 +40,8 @@ EXPORTS = [
     "mf6hip_batch_create", "mf6hip_batch_add", "mf6hip_batch_finalize",
     "mf6hip_batch_set_system", "mf6hip_batch_autotune", "mf6hip_batch_solve",
     "mf6hip_batch_get_x", "mf6hip_batch_destroy",
+    "mf6hip_xmi_set_nonlinear", "mf6hip_xmi_prepare_solve", "mf6hip_xmi_solve",
+    "mf6hip_xmi_last_iteration", "mf6hip_xmi_finalize_solve",
 ]
 
 
@@ -102,6 +104,26 @@ class UnderRelaxC(C.Structure):
                 ("theta", C.c_double), ("akappa", C.c_double),
                 ("amomentum", C.c_double), ("bigch", C.c_double),
                 ("bigchold", C.c_double), ("relaxold", C.c_double)]
+
+
+class NonlinearC(C.Structure):
+    """mf6hip_nonlinear: the IMS NONLINEAR block."""
+    _fields_ = [("outer_dvclose", C.c_double), ("outer_maximum", C.c_int),
+                ("nonmeth", C.c_int), ("gamma", C.c_double), ("theta", C.c_double),
+                ("akappa", C.c_double), ("amomentum", C.c_double),
+                ("backtracking_number", C.c_int),
+                ("backtracking_tolerance", C.c_double),
+                ("backtracking_reduction_factor", C.c_double),
+                ("backtracking_residual_limit", C.c_double),
+                ("iallowptc", C.c_int), ("iptcopt", C.c_int),
+                ("ptcexp", C.c_double), ("ptcdel0", C.c_double)]
+
+
+class XmiIterationC(C.Structure):
+    """mf6hip_xmi_iteration: what one mf6hip_xmi_solve call did."""
+    _fields_ = [("kiter", C.c_int), ("inner", C.c_int), ("icnvg_inner", C.c_int),
+                ("hncg", C.c_double), ("lrch", C.c_int), ("converged", C.c_int),
+                ("nur", C.c_int), ("nbt", C.c_int), ("iptc", C.c_int)]
 
 
 class Mf6HipError(RuntimeError):
@@ -171,6 +193,11 @@ def load():
     L.mf6hip_batch_solve.argtypes = [vp, ip, ip, dp]
     L.mf6hip_batch_get_x.argtypes = [vp, i, dp]
     L.mf6hip_batch_destroy.argtypes = [vp]
+    L.mf6hip_xmi_set_nonlinear.argtypes = [vp, C.POINTER(NonlinearC)]
+    L.mf6hip_xmi_prepare_solve.argtypes = [vp, i, i, i, d]
+    L.mf6hip_xmi_solve.argtypes = [vp, ip]
+    L.mf6hip_xmi_last_iteration.argtypes = [vp, C.POINTER(XmiIterationC)]
+    L.mf6hip_xmi_finalize_solve.argtypes = [vp, ip, ip]
     for nm in EXPORTS:
         getattr(L, nm).restype = i
     _LIB = L

@@ -157,6 +157,20 @@ struct Mf6Handle {
   double *tl_colsq = nullptr;
   void *tl_blas = nullptr;    // rocblas_handle
   int spmv_calls = 0;
+  // XMI-style outer loop (mf6hip_xmi.hip): NONLINEAR settings, the time
+  // step being solved, the outer iteration counter and the state the
+  // NONLINEAR options keep between outer iterations
+  struct Xmi {
+    mf6hip_nonlinear nl{};
+    bool nl_set = false;
+    bool prepared = false;
+    int kper = 0, kstp = 0, iss = 1, kiter = 0, converged = 0;
+    double delt = 1.0;
+    mf6hip_underrelax ur{};
+    mf6hip_ptc ptc{};
+    double res_prev = 0.0, res_new = 0.0;
+    mf6hip_xmi_iteration last{};
+  } xmi;
 };
 
 // two-level preconditioner (mf6hip_coarse.hip)

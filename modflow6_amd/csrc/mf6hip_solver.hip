@@ -1274,7 +1274,7 @@ static int run_mixed_cg(Mf6Handle *h, const TP *piv32) {
   double last_dv = 1e300;
   HIPCHK(hipMemsetAsync(h->e32, 0, sizeof(float) * L.NP, s));
   MaxLoc *mres = (MaxLoc *)(h->d_part + 7 * MF6_MAXBLK);
-  for (int refine = 0; total < h->s.iter1; refine++) {
+  while (total < h->s.iter1) {
     // true FP64 residual and its max
     k_residual<double, double><<<g, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->d, nullptr);
     k_maxloc_ss<<<nb, MF6_BLOCK, 0, s>>>(L, h->d, h->d_part, h->d_mpart, 0.0);
@@ -1376,7 +1376,7 @@ static int run_mixed_bcgs(Mf6Handle *h, const TP *piv32) {
   HIPCHK(hipMemsetAsync(e, 0, sizeof(float) * L.NP, s));
   HIPCHK(hipMemsetAsync(v, 0, sizeof(float) * L.NP, s));
   MaxLoc *mres = (MaxLoc *)(h->d_part + 7 * MF6_MAXBLK);
-  for (int refine = 0; total < h->s.iter1; refine++) {
+  while (total < h->s.iter1) {
     k_residual<double, double><<<g, MF6_BLOCK, 0, s>>>(L, h->A, h->x, h->b, h->d, nullptr);
     k_maxloc_ss<<<nb, MF6_BLOCK, 0, s>>>(L, h->d, h->d_part, h->d_mpart, 0.0);
     k_s_maxloc<<<1, MF6_BLOCK, 0, s>>>(h->d_mpart, nb, mres);

@@ -366,3 +366,60 @@ class GwfSolution:
 
     def destroy(self):
         self.solver.destroy()
+
+
+def nonlinear_settings(case):
+    """mf6hip_nonlinear of a case's IMS NONLINEAR block."""
+    ims = case.ims
+    return _lib.NonlinearC(ims.outer_dvclose, ims.outer_maximum, ims.nonmeth,
+                           ims.under_relaxation_gamma, ims.under_relaxation_theta,
+                           ims.under_relaxation_kappa, ims.under_relaxation_momentum,
+                           ims.backtracking_number, ims.backtracking_tolerance,
+                           ims.backtracking_reduction_factor,
+                           ims.backtracking_residual_limit, ims.iallowptc,
+                           ims.ptc_option, ims.ptc_exponent, ims.ptc_del0)
+
+
+class XmiDriver:
+    """A coupler's loop over the XMI-style entry points (srcbmi/mf6xmi.F90
+    :220-340): per time step mf6hip_xmi_prepare_solve, mf6hip_xmi_solve
+    until it reports convergence or OUTER_MAXIMUM, mf6hip_xmi_finalize_solve.
+    The outer iterations run in the library (C++ over the device entry
+    points); only scalars reach Python."""
+
+    def __init__(self, gs: GwfSolution):
+        self.gs = gs
+        self.L = gs.L
+        self.nl = nonlinear_settings(gs.case)
+        _lib.check(self.L.mf6hip_xmi_set_nonlinear(gs.h, C.byref(self.nl)),
+                   "mf6hip_xmi_set_nonlinear")
+        self.log: List[List[OuterRecord]] = []
+
+    def run_timestep(self, kper, kstp, delt, steady):
+        L, h = self.L, self.gs.h
+        _lib.check(L.mf6hip_xmi_prepare_solve(h, kper, kstp, 1 if steady else 0, float(delt)),
+                   "mf6hip_xmi_prepare_solve")
+        recs = []
+        for _ in range(self.nl.outer_maximum):
+            hc = C.c_int(0)
+            _lib.check(L.mf6hip_xmi_solve(h, C.byref(hc)), "mf6hip_xmi_solve")
+            r = _lib.XmiIterationC()
+            _lib.check(L.mf6hip_xmi_last_iteration(h, C.byref(r)), "mf6hip_xmi_last_iteration")
+            recs.append(OuterRecord(r.kiter, r.inner, r.icnvg_inner, r.hncg, r.lrch,
+                                    bool(r.converged), nur=bool(r.nur), nbt=r.nbt,
+                                    iptc=r.iptc))
+            if hc.value:
+                break
+        conv, nout = C.c_int(0), C.c_int(0)
+        _lib.check(L.mf6hip_xmi_finalize_solve(h, C.byref(conv), C.byref(nout)),
+                   "mf6hip_xmi_finalize_solve")
+        assert nout.value == len(recs)
+        self.log.append(recs)
+        return recs
+
+    def run(self):
+        """Every stress period and time step of the case (equal steps)."""
+        for kper, (perlen, nstp, steady) in enumerate(self.gs.case.periods, start=1):
+            for kstp in range(1, nstp + 1):
+                self.run_timestep(kper, kstp, perlen / nstp, steady)
+        return self.gs.heads()

@@ -222,3 +222,17 @@ def test_batch_set_system_checks_structure_and_symmetry():
         assert set_sys(ia, ja, a) == 1 and "not finalized" in _lib.last_error()
     finally:
         L.mf6hip_batch_destroy(h)
+
+
+def test_xmi_entry_points_validate_without_a_device():
+    """The XMI-style entry points check their handle and arguments before any
+    device work: NULL handles and settings are rejected."""
+    L = _lib.load()
+    nl = _lib.NonlinearC(1e-6, 25, 0, 1.0, 1.0, 0.0, 0.0, 0, 0.0, 0.0, 0.0, 1, 0, 1.0, 0.0)
+    assert L.mf6hip_xmi_set_nonlinear(None, C.byref(nl)) == 1
+    assert L.mf6hip_xmi_prepare_solve(None, 1, 1, 1, 1.0) == 1
+    hc = C.c_int(0)
+    assert L.mf6hip_xmi_solve(None, C.byref(hc)) == 1
+    assert L.mf6hip_xmi_finalize_solve(None, None, None) == 1
+    r = _lib.XmiIterationC()
+    assert L.mf6hip_xmi_last_iteration(None, C.byref(r)) == 1
