@@ -97,11 +97,28 @@ def pmc_traffic(name="hbm_traffic_mixed_steady_10m.md"):
     return float(m.group(1)) * 1e6, "profiles/" + name
 
 
-def cpu_baseline(case, budget_s=30.0):
+def host_cpu():
+    """CPU model and logical CPU count of the host the baseline runs on."""
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(model=model, nproc=os.cpu_count(),
+                affinity=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                else None)
+
+
+def cpu_baseline(case, budget_s=20.0):
     """The reference's own ims_base_cg (oracle/_ref, flang -O2, 1 core) on
     the full system: factor + initial residual, then CG iterations timed
-    over a bounded sample; projected to one solve with the natural-order
-    iteration count recorded for this seeded system."""
+    over two bounded samples (a short one, then one of about budget_s
+    seconds); the solve is projected to the natural-order iteration count
+    recorded for this seeded system from the long sample, and the short
+    sample's per-iteration time bounds how much the rate drifts."""
     from oracle import oracle as O
     import ctypes as C
     R = O.ref_lib()
@@ -133,9 +150,10 @@ def cpu_baseline(case, budget_s=30.0):
         return dict(value=t_full, t_iter=t_full / max(it_full, 1), t_pre=0.0,
                     n_sample=it_full, n_natural=it_full, full=True)
     t_a, _ = run(1)
-    n_s = max(2, int(budget_s / max(t_a, 1e-3) / 2))
-    n_s = min(n_s, 12)
-    t_b, it_b = run(1 + n_s)
+    t_s, it_s = run(9)
+    t_iter_short = (t_s - t_a) / max(it_s - 1, 1)
+    n_l = int(min(400, max(16, budget_s / max(t_iter_short, 1e-4))))
+    t_b, it_b = run(1 + n_l)
     t_iter = (t_b - t_a) / max(it_b - 1, 1)
     t_pre = t_a - t_iter
     ref = os.path.join(ROOT, "tests", "golden", "steady_10m_oracle.json")
@@ -143,8 +161,9 @@ def cpu_baseline(case, budget_s=30.0):
     if os.path.exists(ref) and case.nodes == 10_000_000:
         n_nat = json.load(open(ref))["iterations"]
     proj = t_pre + t_iter * n_nat if n_nat else None
-    return dict(value=proj, t_iter=t_iter, t_pre=t_pre, n_sample=it_b,
-                n_natural=n_nat)
+    return dict(value=proj, t_iter=t_iter, t_iter_short=t_iter_short, t_pre=t_pre,
+                n_sample=it_b, n_short=it_s, n_natural=n_nat,
+                cpu_s=t_a + t_s + t_b)
 
 
 def transient_workload(args):
@@ -307,6 +326,11 @@ def learned_workload(args):
 # mixed-precision methods, one-level and two-level (additive *_2l,
 # deflated *_dl)
 TWO_LEVEL_MIXED = ("mixed", "mixed16", "mixed_2l", "mixed16_2l", "mixed_dl", "mixed16_dl")
+# arithmetic types of each method family: FP64 throughout; FP32 ILU0
+# factor, SpMV and Krylov vectors inside FP64 refinement; the same with the
+# ILU0 factor's couplings stored in FP16 (scaled by a power of two)
+DTYPE = {"fp64": "f64", "mixed": "f32-in-f64",
+         "mixed16": "f16-factor/f32-krylov-in-f64"}
 COARSE_FORM = {"_2l": " + aggregation coarse correction (additive)",
                "_dl": " + aggregation coarse correction (deflated)"}
 
@@ -320,7 +344,7 @@ def main():
                     help="comma list of fp64, mixed (FP32 factor), mixed16 (FP16 "
                          "factor), fp64_2l, mixed_2l, mixed16_2l (two-level, additive), "
                          "fp64_dl, mixed_dl, mixed16_dl (two-level, deflated); both: "
-                         "fp64, mixed, mixed16, fp64_2l, mixed16_2l, mixed16_dl")
+                         "fp64, mixed, mixed16, fp64_2l, fp64_dl, mixed16_2l, mixed16_dl")
     ap.add_argument("--coarse-size", type=int, default=0,
                     help="two-level: most coarse unknowns (0 = sqrt(2 N) within [16, 4096])")
     ap.add_argument("--shape", default="10,1000,1000")
@@ -377,7 +401,7 @@ def main():
     N = case.nodes
     nja = N + 2 * (nlay * (nrow * (ncol - 1) + (nrow - 1) * ncol) + (nlay - 1) * nrow * ncol)
 
-    methods = (["fp64", "mixed", "mixed16", "fp64_2l", "mixed16_2l", "mixed16_dl"]
+    methods = (["fp64", "mixed", "mixed16", "fp64_2l", "fp64_dl", "mixed16_2l", "mixed16_dl"]
                if args.precision == "both" else args.precision.split(","))
     prec = {"fp64": _lib.MF6HIP_FP64, "mixed": _lib.MF6HIP_MIXED,
             "mixed16": _lib.MF6HIP_MIXED_FP16}
@@ -462,15 +486,21 @@ def main():
     if f32 and N == 10_000_000:
         traffic, traffic_src = pmc_traffic("hbm_traffic_twolevel_steady_10m.md" if defl
                                            else "hbm_traffic_mixed_steady_10m.md")
+    # the SpMV kernel the layout launches: W columns of one colour per thread
+    # (16-byte accesses: W = 4 in FP32, 2 in FP64) when ncol is a multiple
+    # of 2W, W = 2 when ncol is a multiple of 4, else one column per thread
+    wmax = 4 if f32 else 2
+    w = 0 if ncol % 2 else (wmax if ncol % (2 * wmax) == 0 else (2 if ncol % 4 == 0 else 0))
+    kname = (f"k_f_spmvw<{'float' if f32 else 'double'}, {w}>" if w else "k_f_spmv")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=traffic, traffic_source=traffic_src,
-                    kernel=("k_f_spmv2" if ncol % 4 == 0 and (nrow * ncol // 2) % 2 == 0
-                            else "k_f_spmv"), bytes_per_launch=sp_bytes,
+                    kernel=kname, bytes_per_launch=sp_bytes,
                     avg_launch_us=sp_us,
-                    csr_equivalent_bytes=csr32 if f32 else csr64,
-                    csr_equivalent_GBs=((csr32 if f32 else csr64) /
-                                        (sp_us * 1e-6) / 1e9) if sp_us > 0 else None)
+                    # byte model of the same product in CSR (values, int32
+                    # column indices and row pointers, x, y): a size
+                    # comparison, not a measured or achievable bandwidth
+                    csr_equivalent_bytes=csr32 if f32 else csr64)
     if "fp64" in summary:
         f = summary["fp64"]["spmv_us"]
         roofline["fp64_spmv_GBs"] = fp64_spmv_bytes / (f * 1e-6) / 1e9 if f > 0 else None
@@ -484,12 +514,16 @@ def main():
                     smp = (f"reference ims_base_cg (flang -O2): the whole solve, "
                            f"{cb['n_sample']} CG iterations")
                 else:
-                    smp = (f"reference ims_base_cg (flang -O2) on the full system: "
-                           f"factor+residual {cb['t_pre']:.2f} s, {cb['n_sample']} CG "
-                           f"iterations at {cb['t_iter'] * 1e3:.1f} ms each, projected "
-                           f"to {cb['n_natural']} natural-order iterations")
+                    smp = (f"reference ims_base_cg (flang -O2, one thread) on the full "
+                           f"system: factor+residual {cb['t_pre']:.2f} s, then {cb['n_sample']} "
+                           f"CG iterations at {cb['t_iter'] * 1e3:.1f} ms each "
+                           f"({cb['n_short']} at {cb['t_iter_short'] * 1e3:.1f} ms in a "
+                           f"short sample), projected to the {cb['n_natural']} natural-order "
+                           f"iterations of this seeded system; {cb['cpu_s']:.0f} s of CPU "
+                           f"work in all")
                 cpu = dict(value=cb["value"], unit="s", cores=1, kind="reference",
-                           sample=smp, t_iter_s=cb["t_iter"])
+                           sample=smp, t_iter_s=cb["t_iter"],
+                           t_iter_short_s=cb.get("t_iter_short"), host=host_cpu())
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed:", e)
 
@@ -498,7 +532,7 @@ def main():
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": value * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-in-f64" if f32 else "f64",
+        "dtype": DTYPE.get(head.split("_")[0], "f64"),
         "data": "synthetic (seeded lognormal K, CHD/RCH/WEL)",
         "config": {"workload": f"steady confined GWF {nlay}x{nrow}x{ncol} DIS "
                                f"({N} cells, {nja} nonzeros), CG + column red-black ILU0"
@@ -520,6 +554,12 @@ def main():
                                              summary[mk]["s_per_solve"])
     if "fp64" in summary and head != "fp64":
         line["speedup_headline_vs_fp64"] = summary["fp64"]["s_per_solve"] / value
+    # like for like: the same preconditioner (form and coarse space) in FP64
+    same = "fp64" + head[len(head.split("_")[0]):]
+    if same in summary and same != head:
+        line["speedup_headline_vs_fp64_same_preconditioner"] = (
+            summary[same]["s_per_solve"] / value)
+        line["fp64_same_preconditioner"] = same
     print(json.dumps(line))
 
 

@@ -173,6 +173,23 @@ int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
                            const int *nodelist, const double *v1,
                            const double *v2);
 
+/* Package options of the reference that change a package's formulation.
+ * The device path implements none of them; mf6hip_gwf_set_package_ex
+ * rejects a nonzero one with MF6HIP_EUNSUPPORTED before any other check. */
+typedef struct mf6hip_package_options {
+  double flowred;     /* WEL AUTO_FLOW_REDUCE fraction (0: off); wel_cf /
+                         wel_fn, src/Model/GroundWaterFlow/gwf-wel.f90:378-424 */
+  int idrndepth;      /* DRN AUXDEPTHNAME given (drain depth smoothing,
+                         gwf-drn.f90:340-470) */
+  int has_auxmult;    /* AUXMULTNAME given (the host applies the multiplier
+                         to v1 / v2 instead) */
+} mf6hip_package_options;
+
+/* mf6hip_gwf_set_package with the package's options (NULL: none). */
+int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
+                              const int *nodelist, const double *v1,
+                              const double *v2, const mf6hip_package_options *opt);
+
 /* Heads in / out (natural order, host memory). */
 int mf6hip_gwf_set_x(void *handle, const double *x);
 int mf6hip_gwf_get_x(void *handle, double *x);

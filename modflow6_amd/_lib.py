@@ -25,7 +25,8 @@ TERM = {"STO-SS": 0, "STO-SY": 1, "WEL": 2, "DRN": 3, "RCH": 4, "CHD": 5}
 # every entry point declared in include/mf6hip.h
 EXPORTS = [
     "mf6hip_create", "mf6hip_solve", "mf6hip_destroy", "mf6hip_last_error",
-    "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_x",
+    "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_package_ex",
+    "mf6hip_gwf_set_x",
     "mf6hip_gwf_get_x", "mf6hip_gwf_set_x_device", "mf6hip_gwf_get_x_device",
     "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
     "mf6hip_gwf_linear_solve", "mf6hip_gwf_outer_update",
@@ -106,6 +107,11 @@ class UnderRelaxC(C.Structure):
                 ("bigchold", C.c_double), ("relaxold", C.c_double)]
 
 
+class PackageOptionsC(C.Structure):
+    """mf6hip_package_options: package options the device path rejects."""
+    _fields_ = [("flowred", C.c_double), ("idrndepth", C.c_int), ("has_auxmult", C.c_int)]
+
+
 class NonlinearC(C.Structure):
     """mf6hip_nonlinear: the IMS NONLINEAR block."""
     _fields_ = [("outer_dvclose", C.c_double), ("outer_maximum", C.c_int),
@@ -154,6 +160,7 @@ def load():
     L.mf6hip_last_error.argtypes = [C.c_char_p, i]
     L.mf6hip_gwf_create.argtypes = [vp, C.POINTER(GwfModelC)]
     L.mf6hip_gwf_set_package.argtypes = [vp, i, i, ip, dp, dp]
+    L.mf6hip_gwf_set_package_ex.argtypes = [vp, i, i, ip, dp, dp, C.POINTER(PackageOptionsC)]
     L.mf6hip_gwf_set_x.argtypes = [vp, dp]
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
     L.mf6hip_gwf_set_x_device.argtypes = [vp, vp]

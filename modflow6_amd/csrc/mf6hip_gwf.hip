@@ -1070,6 +1070,33 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   return MF6HIP_OK;
 }
 
+// Package options that change the formulation: checked before anything
+// else, so a host that parses them learns at once what the device path does
+// not model.
+extern "C" int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
+                                         const int *nodelist, const double *v1,
+                                         const double *v2,
+                                         const mf6hip_package_options *opt) {
+  if (opt) {
+    if (pkgtype == MF6HIP_PKG_WEL && opt->flowred != 0.0) {
+      mf6hip_set_error("WEL AUTO_FLOW_REDUCE (head-dependent well rates, wel_cf / wel_fn, "
+                       "gwf-wel.f90:378-424) is not supported on the device path");
+      return MF6HIP_EUNSUPPORTED;
+    }
+    if (pkgtype == MF6HIP_PKG_DRN && opt->idrndepth != 0) {
+      mf6hip_set_error("DRN AUXDEPTHNAME (drain depth smoothing, drn_cf / drn_fn, "
+                       "gwf-drn.f90:340-470) is not supported on the device path");
+      return MF6HIP_EUNSUPPORTED;
+    }
+    if (opt->has_auxmult != 0) {
+      mf6hip_set_error("AUXMULTNAME: multiply the package values on the host before "
+                       "mf6hip_gwf_set_package_ex");
+      return MF6HIP_EUNSUPPORTED;
+    }
+  }
+  return mf6hip_gwf_set_package(handle, pkgtype, nbound, nodelist, v1, v2);
+}
+
 extern "C" int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
                                       const int *nodelist, const double *v1,
                                       const double *v2) {
@@ -1476,7 +1503,13 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
   for (int t : pkorder) need += (size_t)h->pk[t].nbound;
   need += 3 * (size_t)h->pk[MF6HIP_PKG_CHD].nbound;
   CK(galloc(&h->flowja, (size_t)h->nja));
-  CK(galloc(&h->strg, 2 * (size_t)L.NP));
+  if (!h->strg) {
+    // the storage rates are summed over every position: the padding slots
+    // of an odd nrow * ncol layout, which k_budget_cells never writes, must
+    // hold zero
+    CK(galloc(&h->strg, 2 * (size_t)L.NP));
+    HIPCHK(hipMemsetAsync(h->strg, 0, sizeof(double) * 2 * (size_t)L.NP, s));
+  }
   CK(galloc(&h->chd_flag, (size_t)L.NP));
   if (need > h->bsim_n) {
     if (h->bsim) hipFree(h->bsim);

@@ -100,7 +100,49 @@ module mf6hip_module
     real(c_double) :: relaxold
   end type mf6hip_underrelax
 
+  !> mf6hip_package_options: package options the device path rejects
+  type, bind(C), public :: mf6hip_package_options
+    real(c_double) :: flowred
+    integer(c_int) :: idrndepth
+    integer(c_int) :: has_auxmult
+  end type mf6hip_package_options
+
+  !> mf6hip_nonlinear: the IMS NONLINEAR block for the XMI-style outer loop
+  type, bind(C), public :: mf6hip_nonlinear
+    real(c_double) :: outer_dvclose
+    integer(c_int) :: outer_maximum
+    integer(c_int) :: nonmeth
+    real(c_double) :: gamma
+    real(c_double) :: theta
+    real(c_double) :: akappa
+    real(c_double) :: amomentum
+    integer(c_int) :: backtracking_number
+    real(c_double) :: backtracking_tolerance
+    real(c_double) :: backtracking_reduction_factor
+    real(c_double) :: backtracking_residual_limit
+    integer(c_int) :: iallowptc
+    integer(c_int) :: iptcopt
+    real(c_double) :: ptcexp
+    real(c_double) :: ptcdel0
+  end type mf6hip_nonlinear
+
+  !> mf6hip_xmi_iteration: what one mf6hip_xmi_solve call did
+  type, bind(C), public :: mf6hip_xmi_iteration
+    integer(c_int) :: kiter
+    integer(c_int) :: inner
+    integer(c_int) :: icnvg_inner
+    real(c_double) :: hncg
+    integer(c_int) :: lrch
+    integer(c_int) :: converged
+    integer(c_int) :: nur
+    integer(c_int) :: nbt
+    integer(c_int) :: iptc
+  end type mf6hip_xmi_iteration
+
   public :: mf6hip_create, mf6hip_solve, mf6hip_destroy, mf6hip_last_error
+  public :: mf6hip_gwf_set_package_ex, mf6hip_xmi_set_nonlinear, &
+            mf6hip_xmi_prepare_solve, mf6hip_xmi_solve, mf6hip_xmi_last_iteration, &
+            mf6hip_xmi_finalize_solve
   public :: mf6hip_gwf_create, mf6hip_gwf_set_package, mf6hip_gwf_set_x, &
             mf6hip_gwf_get_x, mf6hip_gwf_advance, mf6hip_gwf_formulate, &
             mf6hip_gwf_residual_norms, mf6hip_gwf_backtrack, &
@@ -272,6 +314,59 @@ module mf6hip_module
       type(c_ptr), value :: ur
       integer(c_int), intent(out) :: inewtonur, lrch
       real(c_double), intent(out) :: dxold_max, hncg
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_set_package_ex(handle, pkgtype, nbound, nodelist, v1, &
+                                       v2, opt) result(rc) &
+      bind(C, name='mf6hip_gwf_set_package_ex')
+      import :: c_int, c_double, c_ptr, mf6hip_package_options
+      type(c_ptr), value :: handle
+      integer(c_int), value :: pkgtype, nbound
+      integer(c_int), intent(in) :: nodelist(*)
+      real(c_double), intent(in) :: v1(*), v2(*)
+      type(mf6hip_package_options), intent(in) :: opt
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_xmi_set_nonlinear(handle, nl) result(rc) &
+      bind(C, name='mf6hip_xmi_set_nonlinear')
+      import :: c_int, c_ptr, mf6hip_nonlinear
+      type(c_ptr), value :: handle
+      type(mf6hip_nonlinear), intent(in) :: nl
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_xmi_prepare_solve(handle, kper, kstp, iss, delt) result(rc) &
+      bind(C, name='mf6hip_xmi_prepare_solve')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: kper, kstp, iss
+      real(c_double), value :: delt
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_xmi_solve(handle, has_converged) result(rc) &
+      bind(C, name='mf6hip_xmi_solve')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), intent(out) :: has_converged
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_xmi_last_iteration(handle, rec) result(rc) &
+      bind(C, name='mf6hip_xmi_last_iteration')
+      import :: c_int, c_ptr, mf6hip_xmi_iteration
+      type(c_ptr), value :: handle
+      type(mf6hip_xmi_iteration), intent(out) :: rec
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_xmi_finalize_solve(handle, converged, outer_iterations) result(rc) &
+      bind(C, name='mf6hip_xmi_finalize_solve')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), intent(out) :: converged, outer_iterations
       integer(c_int) :: rc
     end function
   end interface

@@ -41,9 +41,13 @@ STRUCTS = {
     "mf6hip_gwf_model": _lib.GwfModelC,
     "mf6hip_ptc": _lib.PtcC,
     "mf6hip_underrelax": _lib.UnderRelaxC,
+    "mf6hip_package_options": _lib.PackageOptionsC,
+    "mf6hip_nonlinear": _lib.NonlinearC,
+    "mf6hip_xmi_iteration": _lib.XmiIterationC,
 }
 FORTRAN_TYPES = ["mf6hip_ims_settings", "mf6hip_gwf_model", "mf6hip_ptc",
-                 "mf6hip_underrelax"]
+                 "mf6hip_underrelax", "mf6hip_package_options", "mf6hip_nonlinear",
+                 "mf6hip_xmi_iteration"]
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
@@ -236,3 +240,19 @@ def test_xmi_entry_points_validate_without_a_device():
     assert L.mf6hip_xmi_finalize_solve(None, None, None) == 1
     r = _lib.XmiIterationC()
     assert L.mf6hip_xmi_last_iteration(None, C.byref(r)) == 1
+
+
+@pytest.mark.parametrize("pkg,opt,msg", [
+    (_lib.PKG["WEL"], dict(flowred=0.1), "AUTO_FLOW_REDUCE"),
+    (_lib.PKG["DRN"], dict(idrndepth=1), "AUXDEPTHNAME"),
+    (_lib.PKG["RCH"], dict(has_auxmult=1), "AUXMULTNAME"),
+])
+def test_set_package_rejects_unsupported_options(pkg, opt, msg):
+    """Package options the device path does not model are rejected before
+    any other work (no handle is needed to see it)."""
+    L = _lib.load()
+    o = _lib.PackageOptionsC(**opt)
+    assert L.mf6hip_gwf_set_package_ex(None, pkg, 0, None, None, None, C.byref(o)) == 3
+    assert msg in _lib.last_error()
+    # no options: the plain package path (here: no handle)
+    assert L.mf6hip_gwf_set_package_ex(None, pkg, 0, None, None, None, None) == 1

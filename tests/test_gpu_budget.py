@@ -155,3 +155,18 @@ def test_device_run_writes_head_and_budget_files(tmp_path):
     # the last save is the budget of the final heads, record for record
     got = open(tmp_path / "m.cbc", "rb").read()
     assert got[-len(again):] == again
+
+
+def test_budget_of_odd_layout_after_dirty_device_memory():
+    """Regression: the storage term is summed over every position of the
+    layout, and an odd nrow * ncol layout (the 51 x 51 bump model) has
+    padding slots the budget kernel never writes.  Fill and release a large
+    device allocation first, so that the model's buffers are likely to land
+    on memory holding nonzero values: the terms must still match the
+    oracle (no storage term in this steady model)."""
+    import torch
+    junk = torch.full((64 * 1024 * 1024,), 12345.0, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    test_device_budget_matches_oracle("bump")
