@@ -77,9 +77,10 @@ typedef struct mf6hip_ims_settings {
                            any precision); 3: the same coarse space in the
                            deflated form, z = M^-1 (r - A P yc) + P yc with
                            yc = Ac^-1 P^T r (A-DEF1; same conditions as 2).
-                           This operator is not symmetric: r.z can be
-                           nonpositive, and the CG then restarts its
-                           direction (p = z) instead of forming beta */
+                           This operator is not symmetric: r.z can change
+                           sign between iterations, and the CG then
+                           restarts its direction (p = z) instead of
+                           forming beta */
   int coarse_size;      /* preconditioner 2, 3: most coarse unknowns (aggregates
                            of ha x ha columns, all layers); 0 = sqrt(2 N)
                            within [16, 4096]; at most 6144 */

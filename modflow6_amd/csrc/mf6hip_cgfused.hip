@@ -1331,11 +1331,13 @@ __global__ void __launch_bounds__(MF6_RBLOCK) k_f_round(KState *st, FState *fs,
     fs->restart = 0;
   }
   // The deflated preconditioner z = M^-1 (r - A Q r) + Q r (A-DEF1) is not
-  // symmetric, so r.z is not guaranteed positive.  A nonpositive rho (this
-  // one or the last) restarts the direction, p = z: its step alpha =
-  // r.z / p.Ap is then the exact minimiser of the A-norm error along p, so
-  // the error does not grow.
-  if (nonsym && (rho_new <= 0.0 || (it > 0 && S.rho <= 0.0))) st->beta = 0.0;
+  // symmetric, so r.z need not keep the sign it has for a definite
+  // preconditioner (negative here: the IMS matrices are negative definite,
+  // and so is their ILU0 factor).  When rho changes sign from one iteration
+  // to the next (beta <= 0) the direction restarts, p = z: its step alpha =
+  // r.z / p.Ap is then the exact minimiser of the energy-norm error along p,
+  // so the error does not grow.
+  if (nonsym && it > 0 && rho_new * S.rho <= 0.0) st->beta = 0.0;
 }
 
 __global__ void __launch_bounds__(MF6_RBLOCK) k_f_alpha(KState *st, FState *fs,

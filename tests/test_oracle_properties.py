@@ -237,3 +237,85 @@ def test_rb_ilu_apply_restatement_is_ilu0a(shape, relax):
     got32 = rb_ilu_apply(shape, ia, ja, a, piv, r, dtype=np.float32)
     assert not bits_equal(got32, want)
     assert np.max(np.abs(got32 - want)) < 1e-4 * np.max(np.abs(want))
+
+
+def _deflated_cg(A, b, x0, minv, P, Ac_solve, rule, dvclose=1e-6, rclose=1e-6, itmax=2000):
+    """CG with the A-DEF1 preconditioner z = M^-1 (r - A Q r) + Q r,
+    Q = P Ac^-1 P^T, in the fused CG's recurrence (rho = r.z, beta =
+    rho / rho_old, alpha = rho / p.Ap, the IMS test on max |alpha p| and
+    max |r|).  rule: None (plain recurrence), "flip" (restart, beta = 0,
+    when rho changes sign: the device rule) or "nonpositive" (restart when
+    rho <= 0)."""
+    x = x0.copy()
+    r = b - A @ x
+    p = np.zeros_like(r)
+    rho_old = None
+    restarts = 0
+    for it in range(1, itmax + 1):
+        yc = Ac_solve(P.T @ r)
+        z = minv(r - A @ (P @ yc)) + P @ yc
+        rho = float(r @ z)
+        beta = 0.0 if rho_old is None else rho / rho_old
+        if rho_old is not None and rule == "flip" and rho * rho_old <= 0.0:
+            beta, restarts = 0.0, restarts + 1
+        if rule == "nonpositive" and rho <= 0.0:
+            beta, restarts = 0.0, restarts + 1
+        p = z + beta * p
+        q = A @ p
+        alpha = rho / float(p @ q)
+        x = x + alpha * p
+        r = r - alpha * q
+        rho_old = rho
+        if np.max(np.abs(alpha * p)) <= dvclose and np.max(np.abs(r)) <= rclose:
+            return x, it, restarts
+    return x, itmax, restarts
+
+
+def test_deflated_cg_restart_rule_follows_the_sign_convention():
+    """The IMS matrices are negative definite (negative diagonal), and so is
+    their ILU0 factor: r.z is normally negative.  The device restarts the
+    deflated CG's direction only when rho changes sign.  On a steady system
+    that rule leaves the plain recurrence's iteration count and heads
+    essentially unchanged, while restarting on rho <= 0 (a rule written for
+    positive definite systems) would restart every iteration."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from tests.helpers import oracle_rb_factor, oracle_system, rb_ilu_apply
+    case = configs.steady_small(nlay=3, nrow=20, ncol=24, nwel=6)
+    sim, amat, rhs, x0 = oracle_system(case)
+    shape = (case.nlay, case.nrow, case.ncol)
+    d = sim.dis
+    n = sim.neq
+    ia1 = (d.ia + 1).astype(np.int32)
+    ja1 = (d.ja + 1).astype(np.int32)
+    A = sp.csr_matrix((amat, d.ja, d.ia), shape=(n, n))
+    off = A - sp.diags(A.diagonal())
+    coupled = np.asarray((off != 0).sum(axis=1)).ravel() > 0
+    assert np.all(A.diagonal()[coupled] < 0)      # CHD rows are decoupled unit rows
+    perm, ims, piv, stop = oracle_rb_factor(shape, ia1, ja1, amat)
+
+    def minv(v):
+        return rb_ilu_apply(shape, ia1, ja1, amat, piv, v)
+
+    # aggregates of 4 x 4 grid columns over the coupled rows, as the device
+    k, i, j = np.unravel_index(np.arange(n), shape)
+    naj = -(-case.ncol // 4)
+    agg = (i // 4) * naj + j // 4
+    rows = np.flatnonzero(coupled)
+    P = sp.csr_matrix((np.ones(rows.size), (rows, agg[rows])),
+                      shape=(n, (-(-case.nrow // 4)) * naj))
+    lu = spla.splu((P.T @ A @ P).tocsc())
+    xo = x0.copy()
+    icn, its_o, *_ = O.ImsLinear(d.ia, d.ja, sim.settings).apply(amat.copy(), rhs.copy(), xo)
+    res = {rule: _deflated_cg(A, rhs, x0, minv, P, lu.solve, rule)
+           for rule in (None, "flip", "nonpositive")}
+    for rule, (x, its, nr) in res.items():
+        print("rule", rule, "iterations", its, "restarts", nr,
+              "max |h - h_oracle|", np.max(np.abs(x - xo)), "oracle ILU0-PCG", its_o)
+    x_plain, its_plain, _ = res[None]
+    x_flip, its_flip, nr_flip = res["flip"]
+    assert np.max(np.abs(x_flip - xo)) < 10 * case.ims.inner_dvclose
+    assert its_flip <= its_plain + 3 and its_flip < its_o
+    x_np, its_np, nr_np = res["nonpositive"]
+    assert nr_np >= its_np - 1          # fires on (nearly) every iteration
+    assert nr_flip < nr_np
