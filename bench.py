@@ -220,6 +220,70 @@ def transient_workload(args):
     print(json.dumps(out))
 
 
+def ensemble_cpu_baseline(cases, systems, iters, every=16, sample_its=20):
+    """The reference's ims_base_cg (oracle/_ref, flang -O2, one thread per
+    solve) on the ensemble: every `every`-th realisation is timed for its
+    factor + initial residual and `sample_its` CG iterations, a full solve
+    of it projected with the iteration count the device solve of that
+    realisation took; the other realisations are projected from the sample's
+    per-cell iteration time and their own device iteration counts.  The
+    ensemble's CPU figure is the summed serial time divided over the host's
+    cores, i.e. as many concurrent serial solves as there are cores (perfect
+    load balance assumed)."""
+    from oracle import oracle as O
+    import ctypes as C
+    R = O.ref_lib()
+    if R is None:
+        return None
+    t_all = time.perf_counter()
+    per_cell, pre_cell, sample = [], [], []
+    for i in range(0, len(cases), every):
+        c = cases[i]
+        amat, rhs, x0 = systems[i]
+        ia, ja = dis_structure_1based(c)
+        n = c.nodes
+
+        def run(itmax):
+            x = x0.copy()
+            b = rhs.copy()
+            hv = np.zeros(max(itmax, 1)); hi = np.zeros(max(itmax, 1), np.int32)
+            ic, it, l2 = C.c_int(), C.c_int(), C.c_double()
+            t0 = time.perf_counter()
+            R.ref_ims_ap(n, ja.size, ia, ja, amat, b, x, 1, 1, 0, 0, itmax, 1e-6, 1e-6,
+                         0.0, 1, C.byref(ic), C.byref(it), C.byref(l2), hv, hv.copy(), hi,
+                         hi.copy())
+            return time.perf_counter() - t0, it.value
+        t1, _ = run(1)
+        t2, it2 = run(1 + sample_its)
+        t_iter = (t2 - t1) / max(it2 - 1, 1)
+        t_pre = max(t1 - t_iter, 0.0)
+        per_cell.append(t_iter / n)
+        pre_cell.append(t_pre / n)
+        sample.append(dict(system=i, cells=n, t_iter_s=t_iter, t_pre_s=t_pre,
+                           projected_s=t_pre + t_iter * int(iters[i])))
+    pc, qc = float(np.median(per_cell)), float(np.median(pre_cell))
+    serial = 0.0
+    sampled = {d["system"]: d["projected_s"] for d in sample}
+    for i, c in enumerate(cases):
+        serial += sampled.get(i, (qc + pc * int(iters[i])) * c.nodes)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(value=serial / cores, unit="s", cores=cores, kind="reference",
+                sample=f"reference ims_base_cg (flang -O2, one thread per solve) timed on "
+                       f"{len(sample)} of {len(cases)} realisations (one in {every}: factor + "
+                       f"residual, {sample_its} CG iterations each), each solve projected with "
+                       f"the iteration count of its device solve, the rest from the sample's "
+                       f"median time per cell and iteration; {serial:.0f} s of serial work "
+                       f"spread over {cores} concurrent processes; "
+                       f"{time.perf_counter() - t_all:.0f} s of CPU work measured",
+                serial_s=serial, host=host_cpu())
+
+
+def dis_structure_1based(case):
+    from modflow6_amd.ims import dis_structure
+    ia, ja = dis_structure(case.nlay, case.nrow, case.ncol)
+    return np.ascontiguousarray(ia, dtype=np.int32), np.ascontiguousarray(ja, dtype=np.int32)
+
+
 def ensemble_workload(args):
     """256 steady realisations of uneven size solved together (one
     mixed-precision PCG per kernel launch over every realisation, HIP-graph
@@ -289,6 +353,12 @@ def ensemble_workload(args):
         ax = np.bincount(rows, weights=amat * tl_heads[i][ja - 1], minlength=c.nodes)
         tl_resid.append(float(np.max(np.abs(rhs - ax))))
     ms = float(np.mean(runs))
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = ensemble_cpu_baseline(cases, ens.systems, iters)
+        except Exception as e:  # the baseline is reported, never required
+            log("ensemble cpu baseline failed:", e)
     print(json.dumps(dict(workload=f"ensemble of {len(cases)} steady realisations "
                                    f"(nlay 2-8, n 64-512), {cells} cells",
                           ms_per_ensemble_solve=ms, cells_per_s=cells / (ms * 1e-3),
@@ -307,7 +377,8 @@ def ensemble_workload(args):
                           two_level_one_by_one_s=tl_s,
                           two_level_iterations_mean=float(np.mean(tl_its)),
                           two_level_iterations_max=int(np.max(tl_its)),
-                          two_level_max_true_residual_checked=max(tl_resid) if tl_resid else None)))
+                          two_level_max_true_residual_checked=max(tl_resid) if tl_resid else None,
+                          cpu_baseline=cpu)))
 
 
 def learned_workload(args):
@@ -316,7 +387,10 @@ def learned_workload(args):
                                            epochs=args.epochs, scale_up=args.scale_up,
                                            scale_up_iters=args.scale_up_iters,
                                            nn_every=[int(v) for v in args.nn_every.split(",")],
-                                           log=log)
+                                           log=log,
+                                           channels=tuple(int(v) for v in
+                                                          args.unet_channels.split(",")),
+                                           per_field=args.samples_per_field or None)
     if args.params_out:
         os.makedirs(os.path.dirname(os.path.abspath(args.params_out)), exist_ok=True)
         np.save(args.params_out, params)
@@ -362,8 +436,15 @@ def main():
                     help="steady: the headline 10M-cell solve (default); transient: the "
                          "transient Newton simulation; learned: train and evaluate the "
                          "learned preconditioner")
-    ap.add_argument("--ntrain", type=int, default=64)
-    ap.add_argument("--ntest", type=int, default=16)
+    ap.add_argument("--ntrain", type=int, default=512,
+                    help="learned: training fields (the workload's 512)")
+    ap.add_argument("--ntest", type=int, default=64,
+                    help="learned: unseen evaluation fields (the workload's 64)")
+    ap.add_argument("--unet-channels", default="16,32,64,64",
+                    help="learned: channels per U-Net level (the workload's 4-level "
+                         "16-32-64-64)")
+    ap.add_argument("--samples-per-field", type=int, default=1,
+                    help="learned: PCG iterates sampled per training field (0 = all five)")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--nn-every", default="1,2,4",
                     help="learned: network application periods evaluated (first is headline)")

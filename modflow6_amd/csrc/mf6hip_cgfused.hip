@@ -1139,7 +1139,9 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmvw(Layout L, const TV *__res
 
 // layers unrolled per chunk of the grouped SpMV (loads of a chunk are all
 // issued before its arithmetic)
+#ifndef MF6_SPMV_CH
 #define MF6_SPMV_CH 2
+#endif
 
 template <typename TV, int W, bool CS>
 static void launch_f_spmvw(const Layout &L, int nb, hipStream_t s, const TV *A, const TV *p,

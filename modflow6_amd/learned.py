@@ -29,6 +29,9 @@ from .ims import HipLinearSolver, ImsLinearSettings, dis_structure
 # six levels: the coarsest grid of a 256 x 256 layer is 8 x 8, so the
 # receptive field spans the model
 CHANNELS = (16, 32, 64, 64, 64, 64)
+# the learned-preconditioner workload's stated network: four levels of
+# 16, 32, 64, 64 channels
+WORKLOAD_CHANNELS = (16, 32, 64, 64)
 
 
 def standardise(a):
@@ -243,14 +246,20 @@ def _settings(case, **kw):
     return ImsLinearSettings(**base)
 
 
-def make_dataset(seeds, shape=(16, 256, 256), iterates=(0, 25, 50, 100, 200), log=print):
-    """Training samples from FP64 ILU0-PCG iterates on seeded fields."""
+def make_dataset(seeds, shape=(16, 256, 256), iterates=(0, 25, 50, 100, 200), log=print,
+                 per_field=None):
+    """Training samples from FP64 ILU0-PCG iterates on seeded fields.
+    per_field = m takes m of the iterates of each field (a rotation through
+    the list by field index), so that many fields fit in memory."""
     out = []
     t0 = time.perf_counter()
     for i, seed in enumerate(seeds):
         case = learned_case(seed, *shape)
         sy = device_system(case, _settings(case))
-        out.extend(training_pairs(sy, iterates))
+        its = iterates
+        if per_field is not None and per_field < len(iterates):
+            its = tuple(iterates[(i + q) % len(iterates)] for q in range(per_field))
+        out.extend(training_pairs(sy, its))
         sy.solver.destroy()
         if (i + 1) % 8 == 0 or i + 1 == len(seeds):
             log(f"dataset: {i + 1}/{len(seeds)} fields, {len(out)} samples, "
@@ -294,14 +303,15 @@ def solve_pair(case, net=None, fcg_m=8, iter1=None, nn_every=1):
 
 
 def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, lr=1e-3,
-                 fcg_m=8, scale_up=True, scale_up_iters=20000, nn_every=1, log=print):
+                 fcg_m=8, scale_up=True, scale_up_iters=20000, nn_every=1, log=print,
+                 channels=CHANNELS, per_field=None):
     """Train on `ntrain` seeded fields, evaluate on `ntest` unseen seeds, and
     apply the trained network to the 10 x 1000 x 1000 steady model."""
     t0 = time.perf_counter()
     train_seeds = list(range(1000, 1000 + ntrain))
     test_seeds = list(range(5000, 5000 + ntest))
-    data = make_dataset(train_seeds, shape, log=log)
-    net = UNet(*shape, nin=3, channels=CHANNELS, seed=0)
+    data = make_dataset(train_seeds, shape, log=log, per_field=per_field)
+    net = UNet(*shape, nin=3, channels=channels, seed=0)
     net.set_data(data)
     nsteps = max(1, epochs * data.shape[0] // batch)
     t1 = time.perf_counter()
@@ -327,6 +337,7 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, 
                 for k, v in r.items() if k.startswith("fcg_learned")))
     net.destroy()
     summary = dict(
+        channels=list(channels), levels=len(channels),
         train_fields=ntrain, test_fields=ntest, samples=int(data.shape[0]),
         shape=list(shape), epochs=epochs, batch=batch, train_steps=nsteps, train_s=train_s,
         nn_every=nn_every,
@@ -348,7 +359,7 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, 
                 all_converged=all(e[k]["converged"] for e in evals))
     if scale_up:
         case = configs.steady_10m()
-        big = UNet(case.nlay, case.nrow, case.ncol, nin=3, channels=CHANNELS, params=params)
+        big = UNet(case.nlay, case.nrow, case.ncol, nin=3, channels=channels, params=params)
         r = solve_pair(case, big, fcg_m, iter1=scale_up_iters, nn_every=nn_every)
         big.destroy()
         summary["scale_up_10m"] = r
