@@ -479,6 +479,29 @@ int mf6hip_load_factor(void *handle, const double *amat);
  * layout, after the sln_ls row fixes); either pointer may be NULL. */
 int mf6hip_get_system(void *handle, double *amat_out, double *rhs_out);
 
+/* ---------------------------------------------------------------------
+ * Residual norms without allocation.  No counterpart in the reference: a
+ * variant of mf6hip_gwf_residual_norms for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight coupling loops, hosts
+ * that forbid hipMalloc).  It computes r = A x - b (sln_calc_residual,
+ * NumericalSolution.f90:2966-2982) of a system the caller holds on the
+ * device in the reference's CSR layout (1-based ia/ja, diagonal first),
+ * over rows with active[n] >= 1 (every row when active is NULL), and
+ * writes, asynchronously on `stream` (NULL: the null stream), four
+ * doubles to the device array `out`: ||r||_2 (sln_l2norm, :2855-2871),
+ * ||b||_2 over the same rows, the signed r of largest magnitude and its
+ * 1-based row (0 when no row is active).
+ * It calls no allocator: its per-block partial sums live in a fixed device
+ * array of 4 x 2048 doubles (64 KiB) that each GPU the library runs on
+ * holds until the process ends.  neq must be in [1, 2^30] (MF6HIP_EINVAL
+ * otherwise).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two threads) race and must be
+ * serialised by the caller. */
+int mf6hip_csr_residual_norms_noalloc(int neq, int nja, const int *ia, const int *ja,
+                                      const double *amat, const double *rhs,
+                                      const double *x, const int *active,
+                                      double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,7 @@ EXPORTS = [
     "mf6hip_batch_get_x", "mf6hip_batch_destroy",
     "mf6hip_xmi_set_nonlinear", "mf6hip_xmi_prepare_solve", "mf6hip_xmi_solve",
     "mf6hip_xmi_last_iteration", "mf6hip_xmi_finalize_solve",
+    "mf6hip_csr_residual_norms_noalloc",
 ]
 
 
@@ -205,6 +206,7 @@ def load():
     L.mf6hip_xmi_solve.argtypes = [vp, ip]
     L.mf6hip_xmi_last_iteration.argtypes = [vp, C.POINTER(XmiIterationC)]
     L.mf6hip_xmi_finalize_solve.argtypes = [vp, ip, ip]
+    L.mf6hip_csr_residual_norms_noalloc.argtypes = [i, i] + [vp] * 8
     for nm in EXPORTS:
         getattr(L, nm).restype = i
     _LIB = L

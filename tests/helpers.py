@@ -260,3 +260,38 @@ def oracle_rb_ilu0a(perm, ims, r):
     pz = np.zeros_like(r)
     O.lib().orc_ilu0a(ims.nja, ims.neq, ims.apc, ims.iapc, ims.japc, pr, pz)
     return pz[perm]
+
+
+class HipBuffers:
+    """Device buffers for tests that hand device pointers to the C-ABI,
+    through the HIP runtime the library itself is bound to: after the
+    library is loaded, the process's libamdhip64.so.7 (loading the runtime
+    by another path would put a second runtime in the process)."""
+
+    def __init__(self):
+        _lib.load()
+        self.L = C.CDLL("libamdhip64.so.7")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+        self.bufs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), max(a.nbytes, 1)) == 0
+        assert self.L.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0     # host to device
+        self.bufs.append(p)
+        return p
+
+    def get(self, p, n, dtype=np.float64):
+        out = np.empty(n, dtype=dtype)
+        assert self.L.hipDeviceSynchronize() == 0
+        assert self.L.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.bufs:
+            self.L.hipFree(p)
+        self.bufs = []
