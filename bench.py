@@ -217,6 +217,7 @@ def transient_workload(args):
         meta = json.load(open(ref))
         out["oracle_outer"] = meta["outer_per_step"]
         out["cpu_oracle_s_one_core"] = meta["run_s_one_core"]
+        out["cpu_oracle_host"] = meta.get("host")
     print(json.dumps(out))
 
 

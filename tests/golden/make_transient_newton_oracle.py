@@ -2,8 +2,9 @@
 oracle's run of the transient Newton workload (configs.transient_newton(),
 5 x 500 x 500, 10 periods x 5 steps, BiCGSTAB + ILU0, NEWTON
 UNDER_RELAXATION, default seed): outer iterations per time step and the
-heads of 4096 seeded cells after the first and the last time step.  Tens of
-minutes of one CPU core; run once, offline."""
+heads of 4096 seeded cells after the first and the last time step.  About
+five minutes of one CPU core; run once, offline.  Rerun after any change to
+the oracle's GWF formulate or outer loop: the device tests compare with it."""
 import json
 import os
 import sys
@@ -16,6 +17,17 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from modflow6_amd import configs  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+
+
+def host_cpu():
+    """CPU model of the host the oracle ran on (its one-core time depends on it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -42,7 +54,7 @@ def main():
                    outer_per_step=[len(s) for s in sim.log],
                    inner_per_step=[int(sum(r["inner"] for r in s)) for s in sim.log],
                    converged=[bool(s[-1]["converged"]) for s in sim.log],
-                   run_s_one_core=time.time() - t0,
+                   run_s_one_core=time.time() - t0, host=host_cpu(),
                    head_min=float(sim.x.min()), head_max=float(sim.x.max())),
               open(os.path.join(HERE, "transient_newton_oracle.json"), "w"), indent=1)
     np.savez(os.path.join(HERE, "transient_newton_heads_sample.npz"), cells=cells,
