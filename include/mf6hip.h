@@ -468,6 +468,11 @@ int mf6hip_set_timing(void *handle, int enable, int sample);
  * op 8: the same sweeps in the mixed handle's precision (FP32 planes and
  *       pivots, or FP16 planes scaled by a power of two with FP32 pivots),
  *       x rounded to FP32.
+ * op 9: (preconditioner 3) the restriction P^T x over the coupled cells two
+ *       ways: y[0, nc) as the deflated iteration forms it from column sums
+ *       (P^T (r - alpha q) with r = q = x and alpha = 0.75, i.e. 0.25 P^T x),
+ *       y[nc, 2 nc) directly; the rest of y is zero (nc = the number of
+ *       aggregates; needs 2 nc <= N).
  * A and the factor are those of the last mf6hip_solve / mf6hip_load_factor. */
 int mf6hip_apply_op(void *handle, int op, const double *x, double *y);
 

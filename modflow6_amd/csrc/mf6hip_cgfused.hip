@@ -1570,7 +1570,7 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
       mf6hip_tl_enqueue<double>(h, (const double *)nullptr, h->tl_yc64, h->d_part + P_RHO_C * nb,
                                 nb, st);
     k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                      h->d_hloc, tl ? nb : 0, defl ? 1 : 0);
+                                      h->d_hloc, tl ? mf6hip_tl_nslot(h, nb) : 0, defl ? 1 : 0);
     double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
     launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs,
                                      tl ? h->tl_yc64 : nullptr, h->tl_cmask, h->tl);
@@ -1697,7 +1697,7 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
           mf6hip_tl_enqueue<float>(h, (const float *)nullptr, h->tl_yc32,
                                    h->d_part + P_RHO_C * nb, nb, st);
         k_f_round<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb, nbc, h->d_hist,
-                                          h->d_hloc, tl ? nb : 0, defl ? 1 : 0);
+                                          h->d_hloc, tl ? mf6hip_tl_nslot(h, nb) : 0, defl ? 1 : 0);
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
         launch_f_pupdate<float, float>(L, nb, s, h->z32, pin, pout, h->e32, h->d_mpart, st, fs,
                                        tl ? h->tl_yc32 : nullptr, h->tl_cmask, h->tl);

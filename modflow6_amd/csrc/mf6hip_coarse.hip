@@ -442,6 +442,59 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_tl_gather(Layout L, const double 
   }
 }
 
+// The same sums by aggregate-row strips: block (ai, chunk) takes TL_GCH
+// consecutive grid columns (whole aggregates) of the strip's ha grid rows,
+// one column per lane and every fourth row per wavefront, so each grid
+// row's sums are read as whole 64-column segments (and each colour's as two
+// interleaved ones) rather than as ha-wide pieces.  Each lane adds its rows
+// in order, the four wavefronts' column totals are added in order, and the
+// aggregate sums add ha consecutive column totals (fixed order).
+constexpr int TL_GCH = 64;
+__global__ void __launch_bounds__(4 * TL_GCH) k_tl_gather_rows(Layout L,
+                                                              const double *__restrict__ colsum,
+                                                              TwoLevel T, double *__restrict__ rc,
+                                                              const KState *st,
+                                                              const double *__restrict__ colsq,
+                                                              const double *alpha) {
+  __shared__ double acc[4][TL_GCH];
+  if (st && st->stop) return;
+  const double a = colsq ? *alpha : 0.0;
+  const int per = (TL_GCH / T.ha) * T.ha;   // columns of whole aggregates per block
+  const int nag = per / T.ha;
+  const int nchunk = (L.ncol + per - 1) / per;
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int blk = blockIdx.x; blk < T.nai * nchunk; blk += gridDim.x) {
+    const int ai = blk / nchunk, ch = blk - ai * nchunk;
+    const int j = ch * per + ln;
+    const int i0 = ai * T.ha;
+    const int i1 = i0 + T.ha < L.nrow ? i0 + T.ha : L.nrow;
+    double sum = 0.0;
+    if (ln < per && j < L.ncol) {
+#pragma unroll 4
+      for (int i = i0 + w; i < i1; i += 4) {
+        const int l = i * L.ncol + j;
+        double v = colsum[l];
+        if (a != 0.0) v = v - a * colsq[((i + j) & 1) * L.HP + (l >> 1)];
+        sum = sum + v;
+      }
+    }
+    acc[w][ln] = sum;
+    __syncthreads();
+    if ((int)threadIdx.x < nag) {
+      const int aj = ch * nag + (int)threadIdx.x;
+      if (aj < T.naj) {
+        double t = 0.0;
+        for (int q = 0; q < T.ha; q++) {
+          const int cq = threadIdx.x * T.ha + q;
+          t = t + (((acc[0][cq] + acc[1][cq]) + acc[2][cq]) + acc[3][cq]);
+        }
+        rc[ai * T.naj + aj] = t;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <typename TA> struct Row4;
 template <> struct Row4<float> {
   float4 v;
@@ -464,6 +517,7 @@ template <> struct Row4<double> {
 // row loads; rc is staged in LDS (ld doubles, zero beyond nc; the row
 // entries beyond nc are zero as well).  part[b] = this block's sum of
 // rc[I] yc[I]; slots gridDim.x .. nslot-1 are cleared.
+constexpr int TL_GEMV_THREADS = MF6_BLOCK;
 template <typename TA, typename TV>
 __global__ void __launch_bounds__(MF6_BLOCK) k_tl_gemv(TwoLevel T, const TA *__restrict__ Ainv,
                                                       const double *__restrict__ rc,
@@ -535,6 +589,24 @@ __global__ void k_tl_colsum(Layout L, const TV *__restrict__ d, const unsigned *
     for (int k = 0; k < L.nlay; k++)
       if ((m >> k) & 1u) sum = sum + (double)d[c * L.H + k * L.HP + h2];
     colsum[col_l(L, c, h2)] = sum;
+  }
+}
+
+// the same column sums in the colour layout (c HP + l / 2) the SpMV writes
+// the column sums of q in, for the gather check of mf6hip_apply_op
+template <typename TV>
+__global__ void k_tl_colsum_colour(Layout L, const TV *__restrict__ d,
+                                   const unsigned *__restrict__ cmask,
+                                   double *__restrict__ colsq) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L.HP0 + L.HP1;
+       t += gridDim.x * blockDim.x) {
+    const int c = t >= L.HP0;
+    const int h2 = t - c * L.HP0;
+    const unsigned m = cmask[c * L.HP + h2];
+    double sum = 0.0;
+    for (int k = 0; k < L.nlay; k++)
+      if ((m >> k) & 1u) sum = sum + (double)d[c * L.H + k * L.HP + h2];
+    colsq[c * L.HP + h2] = sum;
   }
 }
 
@@ -801,6 +873,33 @@ int mf6hip_tl_setup_slice(const Layout &L, const double *A, const TwoLevel &T,
   return MF6HIP_OK;
 }
 
+// blocks of the coarse GEMV (each writes one partial of rc.yc)
+static int tl_gemv_grid(const TwoLevel &T, int nslot) {
+  const int rows = TL_GEMV_THREADS / 64;
+  int gg = (T.nc + rows - 1) / rows;
+  if (gg > nslot) gg = nslot;
+  return gg < 1 ? 1 : gg;
+}
+
+int mf6hip_tl_nslot(const Mf6Handle *h, int nslot) { return tl_gemv_grid(h->tl, nslot); }
+
+// rc = P^T (colsum - alpha colsq) into tl_rc (alpha null: P^T colsum)
+static void launch_tl_gather(Mf6Handle *h, const KState *st, const double *alpha) {
+  const TwoLevel &T = h->tl;
+  const Layout &L = h->L;
+  const double *cq = alpha ? h->tl_colsq : nullptr;
+  if (T.ha <= TL_GCH) {
+    const int per = (TL_GCH / T.ha) * T.ha;
+    int gr = T.nai * ((L.ncol + per - 1) / per);
+    if (gr > 4096) gr = 4096;
+    k_tl_gather_rows<<<gr, 4 * TL_GCH, 0, h->stream>>>(L, h->tl_colsum, T, h->tl_rc, st, cq,
+                                                       alpha);
+  } else {
+    const int gr = T.nc < 4096 ? T.nc : 4096;
+    k_tl_gather<<<gr, MF6_BLOCK, 0, h->stream>>>(L, h->tl_colsum, T, h->tl_rc, st, cq, alpha);
+  }
+}
+
 // rc = P^T d (from the preconditioner kernels' column sums when d is null,
 // else from d itself) ; yc = Ainv rc ; partial rc.yc into part[0 .. nslot)
 // With alpha (a device pointer, the deflated form): rc = P^T (d - alpha q)
@@ -814,18 +913,15 @@ void mf6hip_tl_enqueue(Mf6Handle *h, const TV *d, TV *yc, double *part, int nslo
   if (d)
     k_tl_restrict<TV><<<gr, MF6_BLOCK, 0, s>>>(h->L, d, h->tl_cmask, T, h->tl_rc, st);
   else
-    k_tl_gather<<<gr, MF6_BLOCK, 0, s>>>(h->L, h->tl_colsum, T, h->tl_rc, st,
-                                         alpha ? h->tl_colsq : nullptr, alpha);
-  int gg = (T.nc + 3) / 4;
-  if (gg > nslot) gg = nslot;
-  if (gg < 1) gg = 1;
+    launch_tl_gather(h, st, alpha);
+  const int gg = tl_gemv_grid(T, nslot);
   const size_t shm = sizeof(double) * (size_t)T.ld;
   if (sizeof(TV) == sizeof(double))
-    k_tl_gemv<double, TV><<<gg, MF6_BLOCK, shm, s>>>(T, h->tl_A64, h->tl_rc, yc, part, nslot,
-                                                     st);
+    k_tl_gemv<double, TV><<<gg, TL_GEMV_THREADS, shm, s>>>(T, h->tl_A64, h->tl_rc, yc, part,
+                                                           nslot, st);
   else
-    k_tl_gemv<float, TV><<<gg, MF6_BLOCK, shm, s>>>(T, h->tl_A32, h->tl_rc, yc, part, nslot,
-                                                    st);
+    k_tl_gemv<float, TV><<<gg, TL_GEMV_THREADS, shm, s>>>(T, h->tl_A32, h->tl_rc, yc, part,
+                                                          nslot, st);
 }
 
 template void mf6hip_tl_enqueue<double>(Mf6Handle *, const double *, double *, double *, int,
@@ -849,6 +945,33 @@ template void mf6hip_tl_colsum<float>(Mf6Handle *, const float *);
 int mf6hip_tl_prolong64(Mf6Handle *h, double *z) {
   k_tl_prolong<double><<<tl_grid(2 * (size_t)h->L.HP), MF6_BLOCK, 0, h->stream>>>(
       h->L, h->tl_cmask, h->tl, h->tl_yc64, z);
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
+
+// Restriction P^T x over the coupled cells two ways, for the parity helper
+// mf6hip_apply_op (op 9): out[0, nc) by the deflated iteration's route, the
+// gather of column sums with the colour-layout column sums of x as q and
+// alpha = 0.75 (so P^T (x - 0.75 x)); out[nc, 2 nc) by k_tl_restrict.  x is
+// a device vector in the position layout; out a device array of 2 nc.
+int mf6hip_tl_gather_check(Mf6Handle *h, const double *x, double *out) {
+  if (!h->tl_cmask || !h->tl_colsq || !h->tl_colsum) {
+    mf6hip_set_error("op 9 needs preconditioner = 3 and a loaded system");
+    return MF6HIP_EINVAL;
+  }
+  const Layout &L = h->L;
+  const TwoLevel &T = h->tl;
+  hipStream_t s = h->stream;
+  static const double alpha_host = 0.75;
+  double *alpha = h->d_part + 7 * (size_t)MF6_MAXBLK + 16;
+  HIPCHK(hipMemcpyAsync(alpha, &alpha_host, sizeof(double), hipMemcpyHostToDevice, s));
+  mf6hip_tl_colsum<double>(h, x);
+  k_tl_colsum_colour<double><<<tl_grid((size_t)L.HP0 + L.HP1), MF6_BLOCK, 0, s>>>(
+      L, x, h->tl_cmask, h->tl_colsq);
+  launch_tl_gather(h, nullptr, alpha);
+  HIPCHK(hipMemcpyAsync(out, h->tl_rc, sizeof(double) * T.nc, hipMemcpyDeviceToDevice, s));
+  const int gr = T.nc < 4096 ? T.nc : 4096;
+  k_tl_restrict<double><<<gr, MF6_BLOCK, 0, s>>>(L, x, h->tl_cmask, T, out + T.nc, nullptr);
   HIPCHK(hipGetLastError());
   return MF6HIP_OK;
 }

@@ -187,6 +187,9 @@ template <typename TV>
 void mf6hip_tl_colsum(Mf6Handle *h, const TV *d);
 int mf6hip_tl_apply64(Mf6Handle *h, const double *r, double *z);
 int mf6hip_tl_prolong64(Mf6Handle *h, double *z);
+int mf6hip_tl_gather_check(Mf6Handle *h, const double *x, double *out);
+// partial slots of rc.yc the coarse GEMV writes (of nslot cleared)
+int mf6hip_tl_nslot(const Mf6Handle *h, int nslot);
 // parity helpers on the fused CG's operators (ops 4-8 of mf6hip_apply_op)
 int mf6hip_fused_op(Mf6Handle *h, int op, const double *x, double *y);
 

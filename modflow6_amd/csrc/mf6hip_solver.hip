@@ -1853,8 +1853,16 @@ extern "C" int mf6hip_apply_op(void *handle, int op, const double *x, double *y)
     CK(mf6hip_tl_apply64(h, tx, ty));
   } else if (op >= 4 && op <= 8) {
     CK(mf6hip_fused_op(h, op, tx, ty));
+  } else if (op == 9) {
+    if (h->s.preconditioner != 3 || 2 * h->tl.nc > L.N) {
+      mf6hip_set_error("op 9 needs preconditioner = 3 and at least twice as many cells as "
+                       "aggregates");
+      return MF6HIP_EINVAL;
+    }
+    HIPCHK(hipMemsetAsync(h->d_tmp_nat, 0, sizeof(double) * L.N, s));
+    CK(mf6hip_tl_gather_check(h, tx, h->d_tmp_nat));
   } else { mf6hip_set_error("unknown op"); return MF6HIP_EINVAL; }
-  k_pos_to_nat<double, double><<<g, MF6_BLOCK, 0, s>>>(L, ty, h->d_tmp_nat);
+  if (op != 9) k_pos_to_nat<double, double><<<g, MF6_BLOCK, 0, s>>>(L, ty, h->d_tmp_nat);
   HIPCHK(hipMemcpyAsync(y, h->d_tmp_nat, sizeof(double) * L.N, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipGetLastError());

@@ -355,3 +355,45 @@ def test_coarse_operator_63_wide_aggregate_grid():
             assert err <= 1e-9 * np.max(np.abs(want))
     finally:
         sv.destroy()
+
+
+GATHER_CASES = {
+    "3x20x24": ((3, 20, 24), 40),
+    "odd_2x13x17": ((2, 13, 17), 12),
+    "one_column_2x6x8": ((2, 6, 8), 64),
+    # 1000 columns, ha = 16: rows of column sums not aligned to cache lines,
+    # the last aggregate of each row cut by the edge, four column chunks
+    "2x40x1000": ((2, 40, 1000), 189),
+}
+
+
+@pytest.mark.parametrize("name", sorted(GATHER_CASES))
+def test_deflated_restriction_gather(name):
+    """op 9: the restriction the deflated iteration forms from column sums
+    (P^T (r - alpha q) from the sweeps' natural-order sums of r and the
+    SpMV's colour-order sums of q; here r = q = x and alpha = 0.75) against
+    the direct restriction over the coupled cells (k_tl_restrict) and
+    against P^T x from the sparse P (coupled rows only).  Tolerance 1e-12
+    of the largest |P^T x| (the three sum in different orders)."""
+    shape, target = GATHER_CASES[name]
+    ia, ja, a = dyadic_dis_matrix(shape, seed=sum(shape) + 1)
+    A, P, Ac, ha, naj = sparse_coarse(shape, ia, ja, a, target)
+    nc = P.shape[1]
+    nlay, nrow, ncol = shape
+    case = configs.steady_small(nlay=nlay, nrow=nrow, ncol=ncol, nwel=0)
+    sv = device_solver(case, preconditioner=3, coarse_size=target)
+    try:
+        load_factor(sv, a)
+        rng = np.random.default_rng(5)
+        for _ in range(2):
+            x = rng.standard_normal(case.nodes)
+            y = apply_op(sv, 9, x)
+            want = P.T @ x
+            scale = np.max(np.abs(want))
+            print(name, "ha", ha, "nc", nc, "gather err", np.max(np.abs(y[:nc] - 0.25 * want)),
+                  "restrict err", np.max(np.abs(y[nc:2 * nc] - want)), "scale", scale)
+            assert np.max(np.abs(y[nc:2 * nc] - want)) <= 1e-12 * scale
+            assert np.max(np.abs(y[:nc] - 0.25 * want)) <= 1e-12 * scale
+            assert not np.any(y[2 * nc:])
+    finally:
+        sv.destroy()
