@@ -27,6 +27,13 @@
 extern "C" {
 #endif
 
+/* Status codes of every entry point (0 = success, as the reference's BMI
+ * functions return). */
+#define MF6HIP_OK 0
+#define MF6HIP_EINVAL 1        /* invalid argument or input the call cannot take */
+#define MF6HIP_EHIP 2          /* a HIP runtime call failed */
+#define MF6HIP_EUNSUPPORTED 3  /* an option or input the device path does not model */
+
 /* --- IMS settings ------------------------------------------------------ */
 
 /* precision of the device solve (opt-in; stock .ims files get MF6HIP_FP64) */
@@ -190,6 +197,22 @@ typedef struct mf6hip_package_options {
 int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
                               const int *nodelist, const double *v1,
                               const double *v2, const mf6hip_package_options *opt);
+
+/* Horizontal flow barriers (the HFB package): a new set replaces the last
+ * one, as hfb_rp does when a PERIOD block is read (gwf-hfb.f90:149-201:
+ * condsat_reset, read_data, condsat_modify :770-832).  noden / nodem are
+ * 1-based node numbers of horizontally adjacent cells, hydchr the barriers'
+ * hydraulic characteristics (> 0: conductance per unit area, combined in
+ * series with the face's conductance; <= 0: -hydchr multiplies it).  Under
+ * NEWTON, or when both cells are confined, the barrier is folded into the
+ * face's saturated conductance, as the reference does; every later
+ * formulate and budget sees it.  nhfb = 0 removes the barriers.
+ * MF6HIP_EINVAL for cells that are not horizontally connected (check_data,
+ * :711-766); MF6HIP_EUNSUPPORTED for a barrier next to a convertible cell in
+ * a model without NEWTON (the reference applies those in hfb_fc at every
+ * formulate, :211-348, which the device path does not model). */
+int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, const int *nodem,
+                       const double *hydchr);
 
 /* Heads in / out (natural order, host memory). */
 int mf6hip_gwf_set_x(void *handle, const double *x);

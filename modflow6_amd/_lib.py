@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmf6hip.so")
 
 MF6HIP_FP64 = 0
+# status codes (include/mf6hip.h)
+MF6HIP_OK, MF6HIP_EINVAL, MF6HIP_EHIP, MF6HIP_EUNSUPPORTED = 0, 1, 2, 3
 MF6HIP_MIXED = 1
 MF6HIP_MIXED_FP16 = 2
 MF6HIP_ORDER_RB = 0
@@ -26,6 +28,7 @@ TERM = {"STO-SS": 0, "STO-SY": 1, "WEL": 2, "DRN": 3, "RCH": 4, "CHD": 5}
 EXPORTS = [
     "mf6hip_create", "mf6hip_solve", "mf6hip_destroy", "mf6hip_last_error",
     "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_package_ex",
+    "mf6hip_gwf_set_hfb",
     "mf6hip_gwf_set_x",
     "mf6hip_gwf_get_x", "mf6hip_gwf_set_x_device", "mf6hip_gwf_get_x_device",
     "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
@@ -162,6 +165,7 @@ def load():
     L.mf6hip_gwf_create.argtypes = [vp, C.POINTER(GwfModelC)]
     L.mf6hip_gwf_set_package.argtypes = [vp, i, i, ip, dp, dp]
     L.mf6hip_gwf_set_package_ex.argtypes = [vp, i, i, ip, dp, dp, C.POINTER(PackageOptionsC)]
+    L.mf6hip_gwf_set_hfb.argtypes = [vp, i, ip, ip, dp]
     L.mf6hip_gwf_set_x.argtypes = [vp, dp]
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
     L.mf6hip_gwf_set_x_device.argtypes = [vp, vp]

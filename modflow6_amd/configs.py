@@ -115,6 +115,9 @@ class GwfCase:
     ss: Optional[np.ndarray] = None
     sy: Optional[np.ndarray] = None
     meta: Dict = field(default_factory=dict)
+    # HFB package: (noden, nodem, hydchr), 0-based node numbers of
+    # horizontally adjacent cells (gwf-hfb.f90 read_data)
+    hfb: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None
 
     @property
     def nodes(self):

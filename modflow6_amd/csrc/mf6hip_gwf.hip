@@ -254,6 +254,56 @@ __global__ void k_condsat(GwfDev g, double *__restrict__ condsat) {
   }
 }
 
+// Horizontal flow barriers folded into condsat (gwf-hfb.f90:789-832
+// condsat_modify), in barrier order (a second barrier on a face combines
+// with the first's result): the face's condsat is saved, then replaced,
+// under NEWTON or when both cells are confined, by the series combination
+// with the barrier, cond * condhfb / (cond + condhfb), condhfb = hydchr *
+// width * (mean saturated thickness), or by -cond * hydchr for hydchr <= 0
+// (a multiplier).  One thread: the list is short and the order matters.
+// nn / mm are 0-based natural node numbers of horizontally adjacent cells.
+__global__ void k_hfb_apply(GwfDev g, int nhfb, const int *__restrict__ nn,
+                            const int *__restrict__ mm, const double *__restrict__ hydchr,
+                            double *__restrict__ condsat, int *__restrict__ face,
+                            double *__restrict__ sav) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const Layout &L = g.L;
+  for (int b = 0; b < nhfb; b++) {
+    const int n = nn[b], m = mm[b];
+    const int a = n < m ? n : m, c = n < m ? m : n;
+    const int r = a % L.NRC, i = r / L.ncol, j = r - i * L.ncol, k = a / L.NRC;
+    const int dir = c - a == 1 ? 0 : 1;   // right or front neighbour
+    const int pa = cell_pos(L, k, i, j);
+    const int pc = dir == 0 ? cell_pos(L, k, i, j + 1) : cell_pos(L, k, i + 1, j);
+    const int f = dir * L.NP + pa;
+    face[b] = f;
+    double cond = condsat[f];
+    sav[b] = cond;
+    const int pn = n == a ? pa : pc, pm = n == a ? pc : pa;
+    if (g.inewton == 1 || (g.icelltype[pn] == 0 && g.icelltype[pm] == 0)) {
+      const double topn = g.top[pn], topm = g.top[pm], botn = g.bot[pn], botm = g.bot[pm];
+      // DIS horizontal connections have ihc = 1: the mean thickness
+      const double faheight = 0.5 * ((topn - botn) + (topm - botm));
+      if (hydchr[b] > 0.0) {
+        const double fawidth = conn_geom(g, dir, pa, pc, i, j).hwva;
+        const double condhfb = hydchr[b] * fawidth * faheight;
+        cond = cond * condhfb / (cond + condhfb);
+      } else {
+        cond = -cond * hydchr[b];
+      }
+      condsat[f] = cond;
+    }
+  }
+}
+
+// condsat_reset (gwf-hfb.f90:770-781): every face back to the value saved
+// before its barrier, in barrier order
+__global__ void k_hfb_reset(int nhfb, const int *__restrict__ face,
+                            const double *__restrict__ sav, double *__restrict__ condsat) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int b = 0; b < nhfb; b++) condsat[face[b]] = sav[b];
+}
+
 // bnd_cf of the WEL / DRN / RCH packages
 __global__ void k_bnd_cf(int kind, int nb, const int *__restrict__ nodes,
                          const double *__restrict__ v1, const double *__restrict__ v2,
@@ -1015,6 +1065,7 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
     return MF6HIP_EUNSUPPORTED;
   }
   h->m = *m;
+  h->h_icelltype.assign(m->icelltype, m->icelltype + L.N);
   CK(upload_pos(h, &h->top, m->top));
   CK(upload_pos(h, &h->bot, m->bot));
   CK(upload_pos(h, &h->k11, m->k11));
@@ -1067,6 +1118,93 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   k_condsat<<<ggrid(L.N), MF6_BLOCK, 0, h->stream>>>(g, h->condsat);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
+  return MF6HIP_OK;
+}
+
+// A new set of horizontal flow barriers (hfb_rp's condsat_reset, read_data
+// and condsat_modify, gwf-hfb.f90:149-201): noden / nodem are 1-based node
+// numbers, hydchr the barriers' hydraulic characteristics.  nhfb = 0 removes
+// the barriers.  The checks of check_data (gwf-hfb.f90:711-766) apply: the
+// cells must be connected, horizontally.  Without NEWTON a barrier next to a
+// convertible cell is applied by hfb_fc at every formulate instead
+// (gwf-hfb.f90:211-348), which the device path does not model: rejected.
+extern "C" int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, const int *nodem,
+                                  const double *hydchr) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no GWF model on this handle"); return MF6HIP_EINVAL; }
+  if (nhfb < 0 || (nhfb > 0 && (!noden || !nodem || !hydchr))) {
+    mf6hip_set_error("null argument or negative nhfb");
+    return MF6HIP_EINVAL;
+  }
+  const Layout &L = h->L;
+  std::vector<int> nn(nhfb), mm(nhfb);
+  for (int b = 0; b < nhfb; b++) {
+    const int n = noden[b] - 1, m = nodem[b] - 1;
+    if (n < 0 || n >= L.N || m < 0 || m >= L.N) {
+      mf6hip_set_error("HFB no. " + std::to_string(b + 1) + ": node number out of range");
+      return MF6HIP_EINVAL;
+    }
+    const int kn = n / L.NRC, km = m / L.NRC;
+    const int rn = n % L.NRC, rm = m % L.NRC;
+    const int in = rn / L.ncol, jn = rn % L.ncol, im = rm / L.ncol, jm = rm % L.ncol;
+    const bool horiz = kn == km && ((in == im && (jn - jm == 1 || jm - jn == 1)) ||
+                                    (jn == jm && (in - im == 1 || im - in == 1)));
+    const bool vert = rn == rm && (kn - km == 1 || km - kn == 1);
+    if (!horiz) {
+      mf6hip_set_error("HFB no. " + std::to_string(b + 1) +
+                       (vert ? " is between two cells not horizontally connected: "
+                             : " is between two unconnected cells: ") +
+                       std::to_string(n + 1) + " and " + std::to_string(m + 1));
+      return MF6HIP_EINVAL;
+    }
+    if (!h->m.inewton && (h->h_icelltype[n] != 0 || h->h_icelltype[m] != 0)) {
+      mf6hip_set_error("HFB no. " + std::to_string(b + 1) +
+                       ": a barrier next to a convertible cell without NEWTON is applied by "
+                       "hfb_fc at every formulate (gwf-hfb.f90:211-348), which the device "
+                       "path does not model");
+      return MF6HIP_EUNSUPPORTED;
+    }
+    nn[b] = n;
+    mm[b] = m;
+  }
+  hipStream_t s = h->stream;
+  if (h->hfb_n > 0) k_hfb_reset<<<1, 64, 0, s>>>(h->hfb_n, h->d_hfb_face, h->d_hfb_sav, h->condsat);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  h->hfb_n = 0;
+  if (h->d_hfb_face) { hipFree(h->d_hfb_face); h->d_hfb_face = nullptr; }
+  if (h->d_hfb_sav) { hipFree(h->d_hfb_sav); h->d_hfb_sav = nullptr; }
+  if (nhfb == 0) return MF6HIP_OK;
+  int *dnn = nullptr, *dmm = nullptr;
+  double *dhy = nullptr;
+  CK(galloc(&h->d_hfb_face, (size_t)nhfb));
+  CK(galloc(&h->d_hfb_sav, (size_t)nhfb));
+  if (hipMalloc((void **)&dnn, sizeof(int) * nhfb) != hipSuccess ||
+      hipMalloc((void **)&dmm, sizeof(int) * nhfb) != hipSuccess ||
+      hipMalloc((void **)&dhy, sizeof(double) * nhfb) != hipSuccess) {
+    if (dnn) hipFree(dnn);
+    if (dmm) hipFree(dmm);
+    if (dhy) hipFree(dhy);
+    mf6hip_set_error("hipMalloc failed (HFB lists)");
+    return MF6HIP_EHIP;
+  }
+  hipError_t e = hipMemcpy(dnn, nn.data(), sizeof(int) * nhfb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dmm, mm.data(), sizeof(int) * nhfb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dhy, hydchr, sizeof(double) * nhfb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    k_hfb_apply<<<1, 64, 0, s>>>(make_dev(h), nhfb, dnn, dmm, dhy, h->condsat, h->d_hfb_face,
+                                 h->d_hfb_sav);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  hipFree(dnn);
+  hipFree(dmm);
+  hipFree(dhy);
+  if (e != hipSuccess) {
+    mf6hip_set_error(std::string("HFB: ") + hipGetErrorString(e));
+    return MF6HIP_EHIP;
+  }
+  h->hfb_n = nhfb;
   return MF6HIP_OK;
 }
 

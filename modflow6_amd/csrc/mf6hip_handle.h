@@ -6,12 +6,6 @@
 #include "../../include/mf6hip.h"
 #include "mf6hip_common.h"
 
-// Status codes (0 = success, as the reference's BMI functions).
-#define MF6HIP_OK 0
-#define MF6HIP_EINVAL 1
-#define MF6HIP_EHIP 2
-#define MF6HIP_EUNSUPPORTED 3
-
 void mf6hip_set_error(const std::string &msg);
 
 #define HIPCHK(call)                                                        \
@@ -100,6 +94,13 @@ struct Mf6Handle {
   int *ibound = nullptr;      // positions; CHD cells negative
   int *ibound0 = nullptr;     // as uploaded (before CHD)
   double *condsat = nullptr;  // 3 planes: right, front, down (per position)
+  // horizontal flow barriers folded into condsat (gwf-hfb.f90 condsat_modify):
+  // the condsat index of each barrier's face and its value before that
+  // barrier, for the reset before the next set (condsat_reset)
+  int hfb_n = 0;
+  int *d_hfb_face = nullptr;
+  double *d_hfb_sav = nullptr;
+  std::vector<int> h_icelltype;   // natural order, host copy for the checks
   double *sat = nullptr;      // npf sat of the last formulate (per position)
   // budget (gwf_cq + gwf_bd) work arrays, allocated on first use
   double *flowja = nullptr;   // nja, CSR order

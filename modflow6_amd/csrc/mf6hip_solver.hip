@@ -1783,7 +1783,8 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
                   h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax, h->sat,
-                  h->flowja, h->strg, h->bsim, h->chd_flag, h->dhat32, h->qhat32, h->t32};
+                  h->flowja, h->strg, h->bsim, h->chd_flag, h->dhat32, h->qhat32, h->t32,
+                  h->d_hfb_face, h->d_hfb_sav};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);

@@ -8,6 +8,10 @@ module mf6hip_module
   implicit none
   private
 
+  integer(c_int), parameter, public :: MF6HIP_OK = 0
+  integer(c_int), parameter, public :: MF6HIP_EINVAL = 1
+  integer(c_int), parameter, public :: MF6HIP_EHIP = 2
+  integer(c_int), parameter, public :: MF6HIP_EUNSUPPORTED = 3
   integer(c_int), parameter, public :: MF6HIP_FP64 = 0
   integer(c_int), parameter, public :: MF6HIP_MIXED = 1
   integer(c_int), parameter, public :: MF6HIP_MIXED_FP16 = 2
@@ -140,7 +144,7 @@ module mf6hip_module
   end type mf6hip_xmi_iteration
 
   public :: mf6hip_create, mf6hip_solve, mf6hip_destroy, mf6hip_last_error
-  public :: mf6hip_gwf_set_package_ex, mf6hip_xmi_set_nonlinear, &
+  public :: mf6hip_gwf_set_package_ex, mf6hip_gwf_set_hfb, mf6hip_xmi_set_nonlinear, &
             mf6hip_xmi_prepare_solve, mf6hip_xmi_solve, mf6hip_xmi_last_iteration, &
             mf6hip_xmi_finalize_solve
   public :: mf6hip_gwf_create, mf6hip_gwf_set_package, mf6hip_gwf_set_x, &
@@ -326,6 +330,16 @@ module mf6hip_module
       integer(c_int), intent(in) :: nodelist(*)
       real(c_double), intent(in) :: v1(*), v2(*)
       type(mf6hip_package_options), intent(in) :: opt
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_set_hfb(handle, nhfb, noden, nodem, hydchr) result(rc) &
+      bind(C, name='mf6hip_gwf_set_hfb')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: nhfb
+      integer(c_int), intent(in) :: noden(*), nodem(*)
+      real(c_double), intent(in) :: hydchr(*)
       integer(c_int) :: rc
     end function
 

@@ -108,6 +108,8 @@ class GwfSolution:
                                                 nodes.size, _lib.iptr(nodes),
                                                 _lib.dptr(v1), _lib.dptr(v2)),
                        f"mf6hip_gwf_set_package({p.kind})")
+        if c.hfb is not None:
+            self.set_hfb(*c.hfb)
         x0 = np.ascontiguousarray(c.strt, dtype=np.float64)
         _lib.check(L.mf6hip_gwf_set_x(self.h, _lib.dptr(x0)), "mf6hip_gwf_set_x")
         if timing:
@@ -363,6 +365,16 @@ class GwfSolution:
                                             _lib.dptr(rhs)),
                    "mf6hip_get_system")
         return amat, rhs
+
+    def set_hfb(self, noden, nodem, hydchr):
+        """A new set of horizontal flow barriers (0-based node numbers of
+        horizontally adjacent cells, hydraulic characteristics): hfb_rp's
+        condsat_reset + condsat_modify (gwf-hfb.f90:149-201, :770-832)."""
+        n = np.ascontiguousarray(np.asarray(noden) + 1, dtype=np.int32)
+        m = np.ascontiguousarray(np.asarray(nodem) + 1, dtype=np.int32)
+        hy = np.ascontiguousarray(hydchr, dtype=np.float64)
+        _lib.check(self.L.mf6hip_gwf_set_hfb(self.h, n.size, _lib.iptr(n), _lib.iptr(m),
+                                             _lib.dptr(hy)), "mf6hip_gwf_set_hfb")
 
     def destroy(self):
         self.solver.destroy()

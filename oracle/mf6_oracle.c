@@ -658,6 +658,51 @@ typedef struct {
   double satomega;                           /* NPF: 1e-6 under Newton */
 } orc_npf;
 
+/* gwf-hfb.f90:789-832 condsat_modify: barrier b joins noden[b] and
+ * nodem[b] (0-based) through idxloc[b], the ja position of nodem in row
+ * noden (check_data, :711-766).  In barrier order (a second barrier on a
+ * face combines with the first's result), csatsav[b] keeps the face's
+ * condsat before the barrier; under NEWTON or when both cells are confined
+ * the face's condsat becomes cond condhfb / (cond + condhfb), condhfb =
+ * hydchr fawidth faheight, or -cond hydchr for hydchr <= 0. */
+void orc_hfb_condsat_modify(int nhfb, const int *noden, const int *nodem,
+                            const int *idxloc, const double *hydchr, int inewton,
+                            const int *icelltype, const int *jas, const int *ihc,
+                            const double *hwva, const double *top, const double *bot,
+                            double *condsat, double *csatsav)
+{
+  for (int b = 0; b < nhfb; b++) {
+    int js = jas[idxloc[b]];
+    double cond = condsat[js];
+    csatsav[b] = cond;
+    int n = noden[b], m = nodem[b];
+    if (inewton == 1 || (icelltype[n] == 0 && icelltype[m] == 0)) {
+      double topn = top[n], topm = top[m], botn = bot[n], botm = bot[m];
+      double faheight;
+      if (ihc[js] == 2)
+        faheight = (topn < topm ? topn : topm) - (botn > botm ? botn : botm);
+      else
+        faheight = DHALF * ((topn - botn) + (topm - botm));
+      if (hydchr[b] > DZERO) {
+        double fawidth = hwva[js];
+        double condhfb = hydchr[b] * fawidth * faheight;
+        cond = cond * condhfb / (cond + condhfb);
+      } else {
+        cond = -cond * hydchr[b];
+      }
+      condsat[js] = cond;
+    }
+  }
+}
+
+/* gwf-hfb.f90:770-781 condsat_reset: each barrier's face back to the value
+ * saved before it, in barrier order */
+void orc_hfb_condsat_reset(int nhfb, const int *idxloc, const int *jas,
+                           const double *csatsav, double *condsat)
+{
+  for (int b = 0; b < nhfb; b++) condsat[jas[idxloc[b]]] = csatsav[b];
+}
+
 /* gwf-npf.f90:1950-2037 calc_condsat (upperOnly, saturation 1, no
  * THICKSTRT) for every node. */
 void orc_calc_condsat(int nodes, const int *ia, const int *ja, const int *jas,

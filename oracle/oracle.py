@@ -93,6 +93,9 @@ def lib():
         L.orc_syterms.argtypes = [d] * 6 + [C.POINTER(d), C.POINTER(d)]
         L.orc_calc_condsat.argtypes = [i, i32p, i32p, i32p, i32p, f64p, f64p,
                                        f64p, f64p, f64p, f64p, f64p, i, f64p]
+        L.orc_hfb_condsat_modify.argtypes = [i, i32p, i32p, i32p, f64p, i, i32p, i32p,
+                                             i32p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_hfb_condsat_reset.argtypes = [i, i32p, i32p, f64p, f64p]
         L.orc_npf_cf.argtypes = [i, i, d, i32p, i32p, f64p, f64p, f64p, f64p]
         L.orc_npf_fc.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
                                  f64p, f64p, i32p, i32p, i, i, i, i, f64p,
@@ -226,6 +229,8 @@ def ref_gwf_lib():
                                    f64p, f64p, f64p, f64p, f64p, f64p]
         R.ref_drn.argtypes = [i, i, i, i32p, f64p, f64p, i, f64p, i, i32p,
                               f64p, f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_hfb_fc.argtypes = [i, i, i, i, i32p, i32p, i32p, f64p, i32p, i32p, f64p,
+                                 f64p, f64p]
         R.ref_npf_cq.argtypes = [i, i, i, i, i, i, d, i32p, i32p, f64p, f64p,
                                  f64p, f64p, f64p, f64p]
         R.ref_sto_cq.argtypes = [i, i, d, i, d, i32p, i32p, f64p, f64p, f64p,
@@ -527,6 +532,9 @@ class GwfSimulation:
         lib().orc_calc_condsat(self.neq, d.ia, d.ja, d.jas, d.ihc, d.cl1,
                                d.cl2, d.hwva, d.top, d.bot, self.k11,
                                self.k33, 0, self.condsat)
+        self.hfb = None
+        if c.hfb is not None:
+            self.set_hfb(*c.hfb)
         self.sat = np.ones(self.neq)                      # preprocess :1850
         self.x = np.ascontiguousarray(c.strt, dtype=np.float64).copy()
         self.xold = self.x.copy()
@@ -569,6 +577,41 @@ class GwfSimulation:
         self.ptc = PtcState(ims.iallowptc, ims.ptc_option, ims.ptc_del0,
                             ims.ptc_exponent, 0.0, 0.0)
         self.work = np.zeros(self.neq)
+
+    def set_hfb(self, noden, nodem, hydchr):
+        """hfb_rp (gwf-hfb.f90:149-201): condsat_reset of the last set, then
+        check_data's positions (:711-766) and condsat_modify (:789-832) of
+        the new one.  0-based node numbers.  A barrier next to a convertible
+        cell without NEWTON (applied by hfb_fc at every formulate) is not
+        restated: ValueError."""
+        d = self.dis
+        if self.hfb is not None:
+            _, idx0, sav0 = self.hfb
+            lib().orc_hfb_condsat_reset(idx0.size, idx0, d.jas, sav0, self.condsat)
+            self.hfb = None
+        n = np.ascontiguousarray(noden, dtype=np.int32)
+        m = np.ascontiguousarray(nodem, dtype=np.int32)
+        hy = np.ascontiguousarray(hydchr, dtype=np.float64)
+        idx = np.empty(n.size, dtype=np.int32)
+        for b in range(n.size):
+            row = d.ja[d.ia[n[b]] + 1:d.ia[n[b] + 1]]
+            hit = np.nonzero(row == m[b])[0]
+            if hit.size == 0:
+                raise ValueError(f"HFB no. {b + 1} is between two unconnected cells")
+            ipos = int(d.ia[n[b]]) + 1 + int(hit[0])
+            if d.ihc[d.jas[ipos]] == 0:
+                raise ValueError(f"HFB no. {b + 1} is between two cells not horizontally "
+                                 "connected")
+            if self.inewton == 0 and (self.icelltype[n[b]] != 0 or
+                                      self.icelltype[m[b]] != 0):
+                raise ValueError(f"HFB no. {b + 1}: convertible cell without NEWTON "
+                                 "(hfb_fc) is not restated")
+            idx[b] = ipos
+        sav = np.zeros(n.size)
+        lib().orc_hfb_condsat_modify(n.size, n, m, idx, hy, self.inewton, self.icelltype,
+                                     d.jas.astype(np.int32), d.ihc.astype(np.int32),
+                                     d.hwva, d.top, d.bot, self.condsat, sav)
+        self.hfb = (n, idx, sav)
 
     def _ibotnode(self):
         # gwf-npf.f90:1905-1939: lowest vertically connected node

@@ -27,6 +27,7 @@ module mf6ref_gwf_shim
   use GwfNpfModule, only: GwfNpfType
   use GwfStoModule, only: GwfStoType
   use DrnModule, only: DrnType
+  use GwfHfbModule, only: GwfHfbType
   use NumericalSolutionModule, only: NumericalSolutionType
   use GwfModule, only: GwfModelType
   use TdisModule, only: delt
@@ -437,6 +438,64 @@ contains
                 drn%icubic_scaling, drn%imover, drn%naux)
     deallocate (drn)
   end subroutine ref_drn
+
+  ! ------------------------------------------------------------------
+  ! HFB
+  ! ------------------------------------------------------------------
+
+  !> gwf-hfb.f90:211 hfb_fc (no XT3D, no viscosity) on amat: barriers between
+  !! noden / nodem (1-based) at the ja positions idxloc (1-based, as
+  !! check_data stores them); condsav returns what hfb_fc saves for hfb_cq.
+  subroutine ref_hfb_fc(nodes, nja, nhfb, inewton, noden, nodem, idxloc, &
+                        hydchr, icelltype, ibound, hnew, amat, condsav) &
+    bind(C, name='ref_hfb_fc')
+    integer(c_int), value :: nodes, nja, nhfb, inewton
+    integer(c_int), intent(in) :: noden(nhfb), nodem(nhfb), idxloc(nhfb)
+    real(c_double), intent(in) :: hydchr(nhfb)
+    integer(c_int), intent(in) :: icelltype(nodes), ibound(nodes)
+    real(c_double), intent(in) :: hnew(nodes)
+    real(c_double), intent(inout) :: amat(nja)
+    real(c_double), intent(out) :: condsav(nhfb)
+    type(GwfHfbType), pointer :: hfb
+    real(DP), allocatable :: r(:), h(:)
+    allocate (r(nodes), h(nodes))
+    r = 0.0_DP
+    h = hnew
+    allocate (hfb)
+    hfb%dis => dis
+    allocate (hfb%inewton, hfb%nhfb, hfb%ivsc)
+    hfb%inewton = inewton
+    hfb%nhfb = nhfb
+    hfb%ivsc = 0
+    allocate (hfb%xt3d)
+    allocate (hfb%noden(nhfb), hfb%nodem(nhfb), hfb%idxloc(nhfb), &
+              hfb%hydchr(nhfb), hfb%condsav(nhfb))
+    hfb%noden = noden
+    hfb%nodem = nodem
+    hfb%idxloc = idxloc
+    hfb%hydchr = hydchr
+    hfb%condsav = 0.0_DP
+    allocate (hfb%ibound(nodes), hfb%icelltype(nodes))
+    hfb%ibound = ibound
+    hfb%icelltype = icelltype
+    hfb%ihc => dis%con%ihc
+    hfb%ia => dis%con%ia
+    hfb%ja => dis%con%ja
+    hfb%jas => dis%con%jas
+    hfb%isym => dis%con%isym
+    hfb%hwva => dis%con%hwva
+    hfb%top => dis%top
+    hfb%bot => dis%bot
+    spm%amat = amat
+    call hfb%hfb_fc(1, mat, idxglo, r, h)
+    amat = spm%amat
+    condsav = hfb%condsav
+    deallocate (hfb%noden, hfb%nodem, hfb%idxloc, hfb%hydchr, hfb%condsav, &
+                hfb%ibound, hfb%icelltype)
+    deallocate (hfb%xt3d)
+    deallocate (hfb%inewton, hfb%nhfb, hfb%ivsc)
+    deallocate (hfb)
+  end subroutine ref_hfb_fc
 
   ! ------------------------------------------------------------------
   ! NumericalSolution: under-relaxation and backtracking

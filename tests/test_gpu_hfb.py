@@ -1,0 +1,171 @@
+"""Horizontal flow barriers on the device (mf6hip_gwf_set_hfb) against the
+oracle's HFB (pinned to the reference in tests/test_oracle_hfb.py): the
+formulated system bit for bit with barriers folded into the faces'
+saturated conductances (confined cells; NEWTON with convertible cells and
+storage), the heads of a solved step within 10 x dvclose, the budget's face
+flows bit for bit, a new barrier set replacing the last one, and the
+rejections."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from modflow6_amd import _lib, configs
+from modflow6_amd.solution import GwfSolution
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _barriers(case, seed, nb=40):
+    """nb barriers on random horizontal faces, both orientations, one face
+    twice, characteristics of both signs."""
+    rng = np.random.default_rng(seed)
+    nrc = case.nrow * case.ncol
+    noden, nodem = [], []
+    while len(noden) < nb:
+        n = int(rng.integers(case.nodes))
+        k, r = divmod(n, nrc)
+        i, j = divmod(r, case.ncol)
+        opts = []
+        if j + 1 < case.ncol:
+            opts.append(n + 1)
+        if j > 0:
+            opts.append(n - 1)
+        if i + 1 < case.nrow:
+            opts.append(n + case.ncol)
+        if i > 0:
+            opts.append(n - case.ncol)
+        m = opts[int(rng.integers(len(opts)))]
+        noden.append(n)
+        nodem.append(m)
+    noden.append(nodem[0])
+    nodem.append(noden[0])
+    hy = rng.uniform(1e-5, 1e-2, len(noden))
+    hy[3] = 0.0
+    hy[5] = -0.4
+    return np.array(noden), np.array(nodem), hy
+
+
+def _confined():
+    c = configs.steady_small(nlay=3, nrow=20, ncol=24, nwel=6)
+    return dataclasses.replace(c, hfb=_barriers(c, 1))
+
+
+def _newton():
+    c = configs.transient_newton(nlay=3, nrow=24, ncol=30, nwel=8, nper=1, nstp=2)
+    return dataclasses.replace(c, hfb=_barriers(c, 2))
+
+
+CASES = {"confined": _confined, "newton": _newton}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_formulate_with_barriers_bit_exact(name):
+    case = CASES[name]()
+    sim = O.GwfSimulation(case)
+    perlen, nstp, steady = case.periods[0]
+    sim.kper, sim.kstp = 1, 1
+    sim.advance(perlen / nstp, steady)
+    sim.formulate(1)
+    gs = GwfSolution(case)
+    try:
+        gs.kper, gs.kstp = 1, 1
+        gs.prepare_timestep(perlen / nstp, steady)
+        amat, rhs = gs.formulate(1, want_system=True)
+    finally:
+        gs.destroy()
+    plain = O.GwfSimulation(dataclasses.replace(case, hfb=None))
+    assert not np.array_equal(plain.condsat, sim.condsat)
+    bad = np.flatnonzero(amat.view(np.int64) != sim.amat.view(np.int64))
+    assert bad.size == 0, (bad[:8], amat[bad[:8]], sim.amat[bad[:8]])
+    assert np.array_equal(rhs.view(np.int64), sim.rhs.view(np.int64))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_step_with_barriers_heads_and_budget(name):
+    case = CASES[name]()
+    sim = O.GwfSimulation(case)
+    perlen, nstp, steady = case.periods[0]
+    sim.kper = 1
+    conv, _ = sim.run_timestep(perlen / nstp, steady)
+    assert conv
+    gs = GwfSolution(case)
+    try:
+        gs.kper = 1
+        gs.run_timestep(perlen / nstp, steady)
+        hd = gs.heads()
+        # the device's budget of the oracle's heads (formulated at the
+        # oracle's last outer heads, so both hold the same sat): the same
+        # face flows
+        _lib.check(gs.L.mf6hip_gwf_set_x(gs.h, _lib.dptr(np.ascontiguousarray(sim.xtemp))),
+                   "set_x")
+        gs.formulate(2)
+        _lib.check(gs.L.mf6hip_gwf_set_x(gs.h, _lib.dptr(np.ascontiguousarray(sim.x))), "set_x")
+        db = gs.budget()
+    finally:
+        gs.destroy()
+    tol = 10 * max(case.ims.inner_dvclose, case.ims.outer_dvclose)
+    print(name, "max |dh|", np.max(np.abs(hd - sim.x)), "tolerance", tol)
+    assert np.max(np.abs(hd - sim.x)) < tol
+    ob = sim.budget()
+    assert np.array_equal(db["flowja"].view(np.int64), ob["flowja"].view(np.int64))
+
+
+def test_new_barrier_set_replaces_the_last():
+    """A second set (hfb_rp of a later period) resets the faces first: the
+    system equals the oracle's after the same two sets, and an empty set
+    gives back the system without barriers except on a face that had two
+    barriers (the reference's reset restores the value between them)."""
+    case = _confined()
+    n2, m2, h2 = _barriers(case, 7, nb=15)
+    gs = GwfSolution(case)
+    sim = O.GwfSimulation(case)
+    try:
+        gs.kper, gs.kstp = 1, 1
+        gs.prepare_timestep(1.0, True)
+        gs.set_hfb(n2, m2, h2)
+        sim.set_hfb(n2, m2, h2)
+        sim.kper, sim.kstp = 1, 1
+        sim.advance(1.0, True)
+        sim.formulate(1)
+        amat, _ = gs.formulate(1, want_system=True)
+        assert np.array_equal(amat.view(np.int64), sim.amat.view(np.int64))
+        gs.set_hfb(np.array([], dtype=int), np.array([], dtype=int), np.array([]))
+        sim.set_hfb(np.array([], dtype=int), np.array([], dtype=int), np.array([]))
+        sim.formulate(1)
+        amat, _ = gs.formulate(1, want_system=True)
+        assert np.array_equal(amat.view(np.int64), sim.amat.view(np.int64))
+    finally:
+        gs.destroy()
+
+
+def test_set_hfb_rejections():
+    case = configs.steady_small(nlay=2, nrow=6, ncol=7, nwel=2)
+    gs = GwfSolution(case)
+    try:
+        L = gs.L
+
+        def call(n, m, hy):
+            n = np.ascontiguousarray(np.asarray(n) + 1, dtype=np.int32)
+            m = np.ascontiguousarray(np.asarray(m) + 1, dtype=np.int32)
+            hy = np.ascontiguousarray(hy, dtype=np.float64)
+            return L.mf6hip_gwf_set_hfb(gs.h, n.size, _lib.iptr(n), _lib.iptr(m), _lib.dptr(hy))
+        assert call([0], [2], [0.1]) == _lib.MF6HIP_EINVAL
+        assert "unconnected" in _lib.last_error()
+        assert call([0], [42], [0.1]) == _lib.MF6HIP_EINVAL
+        assert "not horizontally" in _lib.last_error()
+        assert call([0], [84], [0.1]) == _lib.MF6HIP_EINVAL
+    finally:
+        gs.destroy()
+    conv = dataclasses.replace(case, icelltype=np.ones(case.nodes, dtype=np.int32))
+    gs = GwfSolution(conv)
+    try:
+        n = np.array([1], dtype=np.int32)
+        m = np.array([2], dtype=np.int32)
+        hy = np.array([0.1])
+        rc = gs.L.mf6hip_gwf_set_hfb(gs.h, 1, _lib.iptr(n), _lib.iptr(m), _lib.dptr(hy))
+        assert rc == _lib.MF6HIP_EUNSUPPORTED
+        assert "hfb_fc" in _lib.last_error()
+    finally:
+        gs.destroy()
