@@ -198,11 +198,15 @@ def steady_small(seed=20260101, nlay=3, nrow=100, ncol=100, nwel=20):
 
 
 def steady_10m(seed=20260102, nlay=10, nrow=1000, ncol=1000, nwel=2000,
-               corr_len=20.0, sigma=2.0):
+               corr_len=20.0, sigma=2.0, k11=None):
+    """k11 given: that K field instead of the seeded one (the wells then
+    take the first draws of the seed's generator)."""
     rng = np.random.default_rng(seed)
     nodes = nlay * nrow * ncol
-    k11 = fields.correlated_lognormal((nlay, nrow, ncol), np.log(10.0),
-                                      sigma, corr_len, rng).reshape(nodes)
+    if k11 is None:
+        k11 = fields.correlated_lognormal((nlay, nrow, ncol), np.log(10.0),
+                                          sigma, corr_len, rng).reshape(nodes)
+    k11 = np.asarray(k11, dtype=np.float64).reshape(nodes)
     botm = np.stack([np.full((nrow, ncol), -10.0 * (k + 1))
                      for k in range(nlay)])
     pk = [_wells(rng, nlay, nrow, ncol, nwel, -200.0),

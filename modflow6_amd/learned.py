@@ -47,24 +47,59 @@ def features(case):
     return np.ascontiguousarray(f, dtype=np.float32)
 
 
-def learned_case(seed, nlay=16, nrow=256, ncol=256):
-    """Steady confined GWF model of the learned-preconditioner workload:
-    lognormal K with sigma on [0.5, 2.5], exponential correlation length on
-    [4, 64] cells, K33 / K on [0.01, 1]; delr = delc = 50 m, 10 m layers;
-    CHD 10 / 0 m on the first / last column and RCH 1e-4 m/d."""
+def learned_field(seed, nlay=16, nrow=256, ncol=256):
+    """The seeded draws of one learned-workload field: (K, sigma, correlation
+    length, K33 / K)."""
     rng = np.random.default_rng(seed)
     sigma = rng.uniform(0.5, 2.5)
     corr = rng.uniform(4.0, 64.0)
     aniso = 10.0 ** rng.uniform(-2.0, 0.0)
     k = fields.correlated_lognormal((nlay, nrow, ncol), np.log(10.0), sigma,
                                     corr, rng).reshape(-1)
-    case = configs.steady_10m(seed=seed, nlay=nlay, nrow=nrow, ncol=ncol, nwel=1)
+    return k, sigma, corr, aniso
+
+
+def learned_case(seed, nlay=16, nrow=256, ncol=256, field=None):
+    """Steady confined GWF model of the learned-preconditioner workload:
+    lognormal K with sigma on [0.5, 2.5], exponential correlation length on
+    [4, 64] cells, K33 / K on [0.01, 1]; delr = delc = 50 m, 10 m layers;
+    CHD 10 / 0 m on the first / last column and RCH 1e-4 m/d.  field: the
+    seed's learned_field when already drawn."""
+    k, sigma, corr, aniso = field if field is not None else learned_field(seed, nlay, nrow,
+                                                                          ncol)
+    case = configs.steady_10m(seed=seed, nlay=nlay, nrow=nrow, ncol=ncol, nwel=1, k11=k)
     case.packages = [p for p in case.packages if p.kind != "WEL"]
     case.k11 = k
     case.k33 = aniso * k
     case.name = f"learned_{nlay}x{nrow}x{ncol}_{seed}"
     case.meta = dict(seed=seed, sigma=sigma, corr_len=corr, aniso=aniso)
     return case
+
+
+def _field_job(args):
+    return learned_field(*args)
+
+
+def learned_cases(seeds, shape, pool=None):
+    """learned_case of every seed in seed order, the K fields drawn by `pool`
+    (CPU processes, see field_pool) when given."""
+    if pool is None:
+        for s in seeds:
+            yield learned_case(s, *shape)
+        return
+    for s, f in zip(seeds, pool.imap(_field_job, [(s, *shape) for s in seeds])):
+        yield learned_case(s, *shape, field=f)
+
+
+def field_pool():
+    """A pool of spawned CPU processes for learned_field (they never touch
+    the GPU).  Create it before the process first uses the GPU: its workers
+    start here, so no process is started afterwards."""
+    import multiprocessing as mp
+    import os
+    n = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else (os.cpu_count() or 1))
+    return mp.get_context("spawn").Pool(n) if n > 1 else None
 
 
 class UNet:
@@ -247,14 +282,13 @@ def _settings(case, **kw):
 
 
 def make_dataset(seeds, shape=(16, 256, 256), iterates=(0, 25, 50, 100, 200), log=print,
-                 per_field=None):
+                 per_field=None, pool=None):
     """Training samples from FP64 ILU0-PCG iterates on seeded fields.
     per_field = m takes m of the iterates of each field (a rotation through
     the list by field index), so that many fields fit in memory."""
     out = []
     t0 = time.perf_counter()
-    for i, seed in enumerate(seeds):
-        case = learned_case(seed, *shape)
+    for i, (seed, case) in enumerate(zip(seeds, learned_cases(seeds, shape, pool))):
         sy = device_system(case, _settings(case))
         its = iterates
         if per_field is not None and per_field < len(iterates):
@@ -310,7 +344,22 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, 
     t0 = time.perf_counter()
     train_seeds = list(range(1000, 1000 + ntrain))
     test_seeds = list(range(5000, 5000 + ntest))
-    data = make_dataset(train_seeds, shape, log=log, per_field=per_field)
+    pool = field_pool()
+    try:
+        summary, params = _run_workload(pool, train_seeds, test_seeds, shape, epochs, batch,
+                                        lr, fcg_m, scale_up, scale_up_iters, nn_every, log,
+                                        channels, per_field, t0)
+    finally:
+        if pool is not None:
+            pool.close()
+            pool.join()
+    return summary, params
+
+
+def _run_workload(pool, train_seeds, test_seeds, shape, epochs, batch, lr, fcg_m, scale_up,
+                  scale_up_iters, nn_every, log, channels, per_field, t0):
+    ntrain, ntest = len(train_seeds), len(test_seeds)
+    data = make_dataset(train_seeds, shape, log=log, per_field=per_field, pool=pool)
     net = UNet(*shape, nin=3, channels=channels, seed=0)
     net.set_data(data)
     nsteps = max(1, epochs * data.shape[0] // batch)
@@ -328,8 +377,8 @@ def run_workload(ntrain=64, ntest=16, shape=(16, 256, 256), epochs=30, batch=4, 
     train_s = time.perf_counter() - t1
     params = net.get_params()
     evals = []
-    for seed in test_seeds:
-        r = solve_pair(learned_case(seed, *shape), net, fcg_m, nn_every=nn_every)
+    for seed, case in zip(test_seeds, learned_cases(test_seeds, shape, pool)):
+        r = solve_pair(case, net, fcg_m, nn_every=nn_every)
         evals.append(r)
         log(f"test seed {seed}: PCG {r['pcg']['iterations']} it {r['pcg']['s']:.3f} s, "
             f"FCG+ILU0 {r['fcg_ilu0']['iterations']} it, " + ", ".join(
