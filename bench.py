@@ -221,7 +221,7 @@ def transient_workload(args):
     print(json.dumps(out))
 
 
-def ensemble_cpu_baseline(cases, systems, iters, every=16, sample_its=20):
+def ensemble_cpu_baseline(cases, systems, iters, every=16, sample_its=100):
     """The reference's ims_base_cg (oracle/_ref, flang -O2, one thread per
     solve) on the ensemble: every `every`-th realisation is timed for its
     factor + initial residual and `sample_its` CG iterations, a full solve
