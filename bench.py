@@ -430,6 +430,9 @@ def main():
     ap.add_argument("--no-timing", action="store_true",
                     help="no per-launch SpMV events (the FP64 fused CG then replays "
                          "captured graphs; no SpMV time is reported)")
+    ap.add_argument("--relax", type=float, default=0.0,
+                    help="RELAXATION_FACTOR of the IMS file: > 0 factors MILU0 instead of "
+                         "ILU0 in every method")
     ap.add_argument("--replace", type=int, default=0,
                     help="mixed: FP32 iterations between residual replacements (0 = 64)")
     ap.add_argument("--workload", default="steady",
@@ -479,6 +482,7 @@ def main():
     else:
         case = configs.steady_10m(nlay=nlay, nrow=nrow, ncol=ncol,
                                   nwel=max(1, 2000 * nrow * ncol // 1_000_000))
+    case.ims.relaxation_factor = args.relax
     log(f"case {case.name}: {case.nodes} cells, built in {time.perf_counter() - t0:.1f} s")
     N = case.nodes
     nja = N + 2 * (nlay * (nrow * (ncol - 1) + (nrow - 1) * ncol) + (nlay - 1) * nrow * ncol)

@@ -211,14 +211,31 @@ struct DeflCol {
   // the same with the cell's row sum s already loaded
   __device__ TV av_s(float sv, const float *__restrict__ ewp, const float *__restrict__ nsp,
                      const TwoLevel &T, const Layout &L, int k) const {
+    return av_e(sv, edges(ewp, nsp, T, L, k));
+  }
+  // the edge couplings of layer k (loaded ahead of their use by the
+  // chunked sweeps), and (A v) from them
+  struct Edge {
+    float ew, ns, ew2, ns2;
+  };
+  __device__ Edge edges(const float *__restrict__ ewp, const float *__restrict__ nsp,
+                        const TwoLevel &T, const Layout &L, int k) const {
     const size_t ke = (size_t)k * L.nrow * 2 * T.naj;
     const size_t kn = (size_t)k * 2 * T.nai * L.ncol;
+    Edge g;
+    g.ew = ewp[ke + ej];
+    g.ns = nsp[kn + ni];
+    g.ew2 = two ? ewp[ke + ew0 + 1] : 0.0f;
+    g.ns2 = two ? nsp[kn + ns0 + L.ncol] : 0.0f;
+    return g;
+  }
+  __device__ TV av_e(float sv, const Edge &g) const {
     TV r = (TV)sv * y;
-    r = r + (TV)ewp[ke + ej] * dj;
-    r = r + (TV)nsp[kn + ni] * di;
+    r = r + (TV)g.ew * dj;
+    r = r + (TV)g.ns * di;
     if (two) {
-      r = r + (TV)ewp[ke + ew0 + 1] * dj2;
-      r = r + (TV)nsp[kn + ns0 + L.ncol] * di2;
+      r = r + (TV)g.ew2 * dj2;
+      r = r + (TV)g.ns2 * di2;
     }
     return r;
   }
@@ -477,7 +494,11 @@ struct GroupGeo {
   __device__ bool has_right(int e) const { return e < W - 1 || rok; }
 };
 
-template <typename TM, typename TV, typename TP, bool DEFL, int W>
+// The layers are walked in chunks of CH whose loads (d, q, the down plane,
+// the pivots and, deflated, the row sums and edge couplings) are all issued
+// before the chunk's arithmetic; the arithmetic and its order are those of
+// the one-layer-at-a-time walk.
+template <typename TM, typename TV, typename TP, bool DEFL, int W, int CH = 2>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwdw(Layout L, const TM *__restrict__ A,
                                                           const TP *__restrict__ piv,
                                                           TV *__restrict__ d,
@@ -509,30 +530,51 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwdw(Layout L, const TM *__
     TV zup[W], lup[W];
 #pragma unroll
     for (int e = 0; e < W; e++) zup[e] = lup[e] = (TV)0;
-    for (int k = 0; k < L.nlay; k++) {
-      const int pos = k * HP + h0;
-      VecW<TV, W> dv = ldw<W>(d + pos);
-      const VecW<TV, W> qv = ldw<W>(q + pos);
-      const VecW<TM, W> a6 = ldw<W>(A + 6 * NP + pos);
-      const VecW<TP, W> pv = ldw<W>(piv + pos);
-      VecW<float, W> sv;
-      if constexpr (DEFL) sv = ldw<W>(dsp + pos);
-      VecW<TV, W> zo;
+    for (int k0 = 0; k0 < L.nlay; k0 += CH) {
+      VecW<TV, W> dv[CH], qv[CH];
+      VecW<TM, W> a6[CH];
+      VecW<TP, W> pv[CH];
+      VecW<float, W> sv[CH];
+      typename DeflCol<TV>::Edge eg[CH][W];
 #pragma unroll
-      for (int e = 0; e < W; e++) {
-        const TV x = dv.v[e] - alpha * qv.v[e];
-        TV tv = x;
-        if constexpr (DEFL) tv = tv - dc[e].av_s(sv.v[e], dew, dns, T, L, k);
-        dv.v[e] = x;
-        md = maxloc_pick(md, MaxLoc{(double)x, k * L.NRC + G.col(h0, e)});
-        ss = ss + (double)x * (double)x;
-        if (k > 0) tv = tv - lup[e] * zup[e];
-        zo.v[e] = tv;
-        zup[e] = tv;
-        lup[e] = (TV)((TV)a6.v[e] * (TV)pv.v[e]);
+      for (int u = 0; u < CH; u++) {
+        const int k = k0 + u;
+        if (k < L.nlay) {
+          const int pos = k * HP + h0;
+          dv[u] = ldw<W>(d + pos);
+          qv[u] = ldw<W>(q + pos);
+          a6[u] = ldw<W>(A + 6 * NP + pos);
+          pv[u] = ldw<W>(piv + pos);
+          if constexpr (DEFL) {
+            sv[u] = ldw<W>(dsp + pos);
+#pragma unroll
+            for (int e = 0; e < W; e++) eg[u][e] = dc[e].edges(dew, dns, T, L, k);
+          }
+        }
       }
-      stw<W>(d + pos, dv);
-      stw<W>(zf + pos, zo);
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int k = k0 + u;
+        if (k < L.nlay) {
+          const int pos = k * HP + h0;
+          VecW<TV, W> zo;
+#pragma unroll
+          for (int e = 0; e < W; e++) {
+            const TV x = dv[u].v[e] - alpha * qv[u].v[e];
+            TV tv = x;
+            if constexpr (DEFL) tv = tv - dc[e].av_e(sv[u].v[e], eg[u][e]);
+            dv[u].v[e] = x;
+            md = maxloc_pick(md, MaxLoc{(double)x, k * L.NRC + G.col(h0, e)});
+            ss = ss + (double)x * (double)x;
+            if (k > 0) tv = tv - lup[e] * zup[e];
+            zo.v[e] = tv;
+            zup[e] = tv;
+            lup[e] = (TV)((TV)a6[u].v[e] * (TV)pv[u].v[e]);
+          }
+          stw<W>(d + pos, dv[u]);
+          stw<W>(zf + pos, zo);
+        }
+      }
     }
   }
   ss = bsum(ss, lds);
@@ -608,7 +650,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_blackw(Layout L, const TM *__re
       VecW<TV, W> dv = ldw<W>(d + pos);
       const VecW<TV, W> qv = ldw<W>(q + pos);
       VecW<float, W> sv;
-      if constexpr (DEFL) sv = ldw<W>(dsp + pos);
+      typename DeflCol<TV>::Edge eg[W];
+      if constexpr (DEFL) {
+        sv = ldw<W>(dsp + pos);
+#pragma unroll
+        for (int e = 0; e < W; e++) eg[e] = dc[e].edges(dew, dns, T, L, k);
+      }
       VecW<TV, W> zo;
 #pragma unroll
       for (int e = 0; e < W; e++) {
@@ -618,7 +665,7 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_blackw(Layout L, const TM *__re
         const TV tf = (TV)((TV)a5.v[e] * (TV)pf.v[e]) * zfr.v[e];
         const TV x = dv.v[e] - alpha * qv.v[e];
         TV tv = x;
-        if constexpr (DEFL) tv = tv - dc[e].av_s(sv.v[e], dew, dns, T, L, k);
+        if constexpr (DEFL) tv = tv - dc[e].av_e(sv.v[e], eg[e]);
         dv.v[e] = x;
         cs[e] = ((cm.v[e] >> k) & 1u) ? cs[e] + (double)x : cs[e];
         md = maxloc_pick(md, MaxLoc{(double)x, k * L.NRC + G.col(h0, e)});
