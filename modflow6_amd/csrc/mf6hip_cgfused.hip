@@ -494,11 +494,15 @@ struct GroupGeo {
   __device__ bool has_right(int e) const { return e < W - 1 || rok; }
 };
 
-// The layers are walked in chunks of CH whose loads (d, q, the down plane,
-// the pivots and, deflated, the row sums and edge couplings) are all issued
-// before the chunk's arithmetic; the arithmetic and its order are those of
-// the one-layer-at-a-time walk.
-template <typename TM, typename TV, typename TP, bool DEFL, int W, int CH = 2>
+// The layers are walked in chunks of CH (MF6_SWEEP_CH, default 2) whose
+// loads (d, q, the down plane, the pivots and, deflated, the row sums and
+// edge couplings) are all issued before the chunk's arithmetic; the
+// arithmetic and its order are those of the one-layer-at-a-time walk.  On
+// the 10 x 1000 x 1000 steady solve chunks of 4 are slower.
+#ifndef MF6_SWEEP_CH
+#define MF6_SWEEP_CH 2
+#endif
+template <typename TM, typename TV, typename TP, bool DEFL, int W, int CH = MF6_SWEEP_CH>
 __global__ void __launch_bounds__(MF6_BLOCK) k_f_red_fwdw(Layout L, const TM *__restrict__ A,
                                                           const TP *__restrict__ piv,
                                                           TV *__restrict__ d,
