@@ -85,11 +85,24 @@ class ImsOptions:
 class Package:
     """Boundary package period data (0-based node numbers).
     kind: WEL (v1 = q), DRN (v1 = elev, v2 = cond), RCH (v1 = rate),
-    CHD (v1 = head)."""
+    CHD (v1 = head).  Options: WEL flowred, the AUTO_FLOW_REDUCE value as
+    read (None: off); DRN depth, each drain's AUXDEPTHNAME value (None: no
+    depth auxiliary), and cubic, DEV_CUBIC_SCALING."""
     kind: str
     nodes: np.ndarray
     v1: np.ndarray
     v2: Optional[np.ndarray] = None
+    flowred: Optional[float] = None
+    depth: Optional[np.ndarray] = None
+    cubic: bool = False
+
+
+def wel_flowred(value):
+    """AUTO_FLOW_REDUCE as wel_options stores it (gwf-wel.f90:209-217): a
+    value <= 0 becomes 0.1, one above 1 becomes 1."""
+    if value <= 0.0:
+        return 0.1
+    return 1.0 if value > 1.0 else float(value)
 
 
 @dataclass

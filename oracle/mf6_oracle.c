@@ -499,6 +499,35 @@ double orc_sqsat_deriv(double top, double bot, double x, double eps)
   return y;
 }
 
+/* src/Utilities/SmoothingFunctions.f90:412-460 sQSaturation (cubic
+ * smoothing from bot to top, coefficients c1 / b^3 and c2 / b^2; the
+ * reference's defaults are c1 = -2, c2 = 3).  The reference's x**3.0 and
+ * x**2.0 compile to (x x) x and x x, which these products reproduce. */
+double orc_sqsat_cubic(double top, double bot, double x, double c1, double c2)
+{
+  double w = x - bot;
+  double b = top - bot;
+  double s = w / b;
+  double cof1 = c1 / ((b * b) * b);
+  double cof2 = c2 / (b * b);
+  if (s < DZERO) return DZERO;
+  if (s < DONE) return cof1 * ((w * w) * w) + cof2 * (w * w);
+  return DONE;
+}
+
+/* SmoothingFunctions.f90:466-516 sQSaturationDerivative */
+double orc_sqsat_cubic_deriv(double top, double bot, double x, double c1, double c2)
+{
+  double w = x - bot;
+  double b = top - bot;
+  double s = w / b;
+  double cof1 = c1 * 3.0 / ((b * b) * b);
+  double cof2 = c2 * 2.0 / (b * b);
+  if (s < DZERO) return DZERO;
+  if (s < DONE) return cof1 * (w * w) + cof2 * w;
+  return DZERO;
+}
+
 /* GwfConductanceUtils.f90:290-305 logmean */
 static double orc_logmean(double d1, double d2)
 {
@@ -1089,15 +1118,53 @@ void orc_rate_accumulator(int n, const double *flow, double *rin, double *rout)
   }
 }
 
-/* gwf-wel.f90:296-332 wel_cf without AUTO_FLOW_REDUCE */
+/* gwf-wel.f90:296-332 wel_cf.  With AUTO_FLOW_REDUCE (iflowred /= 0;
+ * flowred as wel_options :205-217 leaves it, in (0, 1]) a withdrawal in a
+ * convertible cell is scaled by sQSaturation over the bottom flowred of the
+ * cell's thickness. */
 void orc_wel_cf(int nbound, const int *nodelist, const double *q,
-                const int *ibound, double *hcof, double *brhs)
+                const int *ibound, int iflowred, double flowred,
+                const int *icelltype, const double *top, const double *bot,
+                const double *xnew, double *hcof, double *brhs)
 {
   for (int i = 0; i < nbound; i++) {
     int node = nodelist[i];
     hcof[i] = DZERO;
     if (ibound[node] <= 0) { brhs[i] = DZERO; continue; }
-    brhs[i] = -q[i];
+    double qi = q[i];
+    if (iflowred != 0 && qi < DZERO && icelltype[node] != 0) {
+      double tp = top[node], bt = bot[node];
+      double thick = tp - bt;
+      tp = bt + flowred * thick;
+      qi = qi * orc_sqsat_cubic(tp, bt, xnew[node], -2.0, 3.0);
+    }
+    brhs[i] = -qi;
+  }
+}
+
+/* gwf-wel.f90:378-424 wel_fn: the Newton term of a reduced withdrawal,
+ * d(q sQSaturation)/dh = q sQSaturationDerivative, on the diagonal and the
+ * rhs (brhs: wel_cf's rhs). */
+void orc_wel_fn(int nbound, const int *nodelist, const double *q,
+                const double *brhs, const int *ibound, int iflowred,
+                double flowred, const int *icelltype, const double *top,
+                const double *bot, const double *xnew, const int *ia,
+                double *amat, double *rhs)
+{
+  for (int i = 0; i < nbound; i++) {
+    int node = nodelist[i];
+    if (ibound[node] <= 0) continue;
+    if (iflowred == 0 || icelltype[node] == 0) continue;
+    double qi = -brhs[i];
+    if (qi < DZERO) {
+      double tp = top[node], bt = bot[node];
+      double thick = tp - bt;
+      tp = bt + flowred * thick;
+      double drterm = orc_sqsat_cubic_deriv(tp, bt, xnew[node], -2.0, 3.0);
+      drterm = drterm * q[i];
+      amat[ia[node]] = amat[ia[node]] + drterm;
+      rhs[node] = rhs[node] + drterm * xnew[node];
+    }
   }
 }
 
@@ -1116,20 +1183,72 @@ void orc_rch_cf(int nbound, const int *nodelist, const double *recharge,
   }
 }
 
-/* gwf-drn.f90:340-373 drn_cf and get_drain_factor :534-574 with no depth
- * auxiliary (factor is a step at the drain elevation). */
+/* gwf-drn.f90:501-531 get_drain_elevations: with a depth d /= 0 the
+ * conductance is scaled between elev and elev + d */
+static void drn_elevations(int idepth, const double *depth, const double *elev,
+                           int i, double *drndepth, double *drntop, double *drnbot)
+{
+  double e = elev[i];
+  *drndepth = idepth != 0 ? depth[i] : DZERO;
+  if (*drndepth != DZERO) {
+    double el = e + *drndepth;
+    *drntop = el > e ? el : e;
+    *drnbot = el < e ? el : e;
+  } else {
+    *drntop = e;
+    *drnbot = e;
+  }
+}
+
+/* gwf-drn.f90:340-373 drn_cf with get_drain_factor :534-574: a step at the
+ * drain elevation, or with a depth sQSaturation (c1 = -1, c2 = 2;
+ * icubic: DEV_CUBIC_SCALING, by default on under NEWTON, :130-135) or
+ * sQuadraticSaturation with eps = 0 (linear) between the drain bottom and
+ * top. */
 void orc_drn_cf(int nbound, const int *nodelist, const double *elev,
-                const double *cond, const int *ibound, const double *xnew,
-                double *hcof, double *brhs)
+                const double *cond, int idepth, const double *depth, int icubic,
+                const int *ibound, const double *xnew, double *hcof, double *brhs)
 {
   for (int i = 0; i < nbound; i++) {
     int node = nodelist[i];
     if (ibound[node] <= 0) { hcof[i] = DZERO; brhs[i] = DZERO; continue; }
     double cdrn = cond[i];
-    double drnbot = elev[i];
-    double fact = xnew[node] <= drnbot ? DZERO : DONE;
+    double drndepth, drntop, drnbot, fact;
+    drn_elevations(idepth, depth, elev, i, &drndepth, &drntop, &drnbot);
+    double x = xnew[node];
+    if (drndepth != DZERO) {
+      if (icubic != 0) fact = orc_sqsat_cubic(drntop, drnbot, x, -DONE, 2.0);
+      else fact = orc_sqsat(drntop, drnbot, x, DZERO);
+    } else {
+      fact = x <= drnbot ? DZERO : DONE;
+    }
     brhs[i] = -fact * cdrn * drnbot;
     hcof[i] = -fact * cdrn;
+  }
+}
+
+/* gwf-drn.f90:418-470 drn_fn: with a depth auxiliary, the derivative of the
+ * cubic scaling (whichever scaling drn_cf used) times cdrn (drnbot - h) on
+ * the diagonal and the rhs. */
+void orc_drn_fn(int nbound, const int *nodelist, const double *elev,
+                const double *cond, int idepth, const double *depth,
+                const int *ibound, const double *xnew, const int *ia,
+                double *amat, double *rhs)
+{
+  if (idepth == 0) return;
+  for (int i = 0; i < nbound; i++) {
+    int node = nodelist[i];
+    if (ibound[node] <= 0) continue;
+    double cdrn = cond[i];
+    double x = xnew[node];
+    double drndepth, drntop, drnbot;
+    drn_elevations(idepth, depth, elev, i, &drndepth, &drntop, &drnbot);
+    if (drndepth != DZERO) {
+      double drterm = orc_sqsat_cubic_deriv(drntop, drnbot, x, -DONE, 2.0);
+      drterm = drterm * cdrn * (drnbot - x);
+      amat[ia[node]] = amat[ia[node]] + drterm;
+      rhs[node] = rhs[node] + drterm * x;
+    }
   }
 }
 

@@ -75,7 +75,8 @@ def lib():
                                  f64p, C.POINTER(C.c_int),
                                  C.POINTER(C.c_double), C.POINTER(C.c_int),
                                  C.c_void_p]
-        for nm, nargs in (("orc_sqsat", 4), ("orc_sqsat_deriv", 4)):
+        for nm, nargs in (("orc_sqsat", 4), ("orc_sqsat_deriv", 4),
+                          ("orc_sqsat_cubic", 5), ("orc_sqsat_cubic_deriv", 5)):
             getattr(L, nm).restype = d
             getattr(L, nm).argtypes = [d] * nargs
         L.orc_condmean.restype = d
@@ -118,9 +119,15 @@ def lib():
         L.orc_chd_rate.argtypes = [i, i32p, i32p, i32p, i32p, f64p, f64p,
                                    f64p, f64p]
         L.orc_rate_accumulator.argtypes = [i, f64p, C.POINTER(d), C.POINTER(d)]
-        L.orc_wel_cf.argtypes = [i, i32p, f64p, i32p, f64p, f64p]
+        L.orc_wel_cf.argtypes = [i, i32p, f64p, i32p, i, d, i32p, f64p, f64p,
+                                 f64p, f64p, f64p]
+        L.orc_wel_fn.argtypes = [i, i32p, f64p, f64p, i32p, i, d, i32p, f64p,
+                                 f64p, f64p, i32p, f64p, f64p]
         L.orc_rch_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
-        L.orc_drn_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p, f64p]
+        L.orc_drn_cf.argtypes = [i, i32p, f64p, f64p, i, f64p, i, i32p, f64p,
+                                 f64p, f64p]
+        L.orc_drn_fn.argtypes = [i, i32p, f64p, f64p, i, f64p, i32p, f64p, i32p,
+                                 f64p, f64p]
         L.orc_sln_ls_fixes.argtypes = [i, i32p, i32p, i32p, i, f64p, f64p,
                                        f64p, f64p]
         L.orc_get_dxmax.restype = i
@@ -187,6 +194,10 @@ def ref_lib():
         R.ref_sqsat.argtypes = [d] * 4
         R.ref_sqsat_deriv.restype = d
         R.ref_sqsat_deriv.argtypes = [d] * 4
+        R.ref_sqsat_cubic.restype = d
+        R.ref_sqsat_cubic.argtypes = [d] * 5
+        R.ref_sqsat_cubic_deriv.restype = d
+        R.ref_sqsat_cubic_deriv.argtypes = [d] * 5
         R.ref_ssterms.argtypes = [i, i, i, d, d, d, d, d, d, d, d,
                                   C.POINTER(d), C.POINTER(d)]
         R.ref_syterms.argtypes = [d] * 6 + [C.POINTER(d), C.POINTER(d)]
@@ -229,6 +240,8 @@ def ref_gwf_lib():
                                    f64p, f64p, f64p, f64p, f64p, f64p]
         R.ref_drn.argtypes = [i, i, i, i32p, f64p, f64p, i, f64p, i, i32p,
                               f64p, f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_wel.argtypes = [i, i, i, i32p, f64p, i, d, i32p, i32p, f64p,
+                              f64p, f64p, f64p, f64p, f64p, f64p]
         R.ref_hfb_fc.argtypes = [i, i, i, i, i32p, i32p, i32p, f64p, i32p, i32p, f64p,
                                  f64p, f64p]
         R.ref_npf_cq.argtypes = [i, i, i, i, i, i, d, i32p, i32p, f64p, f64p,
@@ -505,6 +518,14 @@ def permute_csr(ia, ja, a, perm):
 _PKG_ORDER = {"WEL": 0, "DRN": 1, "RCH": 2, "CHD": 3}   # gwf.f90:119-124
 
 
+def wel_flowred(value):
+    """gwf-wel.f90:209-217 (wel_options): AUTO_FLOW_REDUCE <= 0 becomes 0.1,
+    above 1 becomes 1."""
+    if value <= 0.0:
+        return 0.1
+    return 1.0 if value > 1.0 else float(value)
+
+
 class GwfSimulation:
     """Restated MF6 run of a :class:`modflow6_amd.configs.GwfCase`.
 
@@ -658,20 +679,11 @@ class GwfSimulation:
         L.orc_npf_cf(self.neq, self.inewton, self.satomega, self.icelltype,
                      self.ibound, d.top, d.bot, self.x, self.sat)
         for t, p in enumerate(self.pk):
-            nd = self.pk_nodes[t]
-            if p.kind == "WEL":
-                L.orc_wel_cf(nd.size, nd, np.ascontiguousarray(p.v1),
-                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
-            elif p.kind == "RCH":
-                L.orc_rch_cf(nd.size, nd, np.ascontiguousarray(p.v1), d.area,
-                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
-            elif p.kind == "DRN":
-                L.orc_drn_cf(nd.size, nd, np.ascontiguousarray(p.v1),
-                             np.ascontiguousarray(p.v2), self.ibound, self.x,
-                             self.pk_hcof[t], self.pk_rhs[t])
-            elif p.kind == "CHD":
+            if p.kind == "CHD":
                 self.pk_hcof[t][:] = 0.0
                 self.pk_rhs[t][:] = 0.0
+            else:
+                self._bnd_cf(t, p)
         # gwf_fc
         L.orc_npf_fc(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
                      d.hwva, self.condsat, self.ibound, self.icelltype,
@@ -696,6 +708,45 @@ class GwfSimulation:
                 L.orc_sto_fn(self.neq, d.ia, self.ibound, self.iconvert,
                              d.top, d.bot, d.area, self.ss, self.sy,
                              self.delt, self.x, self.amat, self.rhs)
+            # bnd_fn per package (gwf.f90:545-553): WEL AUTO_FLOW_REDUCE
+            # and DRN depth scaling
+            for t, p in enumerate(self.pk):
+                nd = self.pk_nodes[t]
+                if p.kind == "WEL" and p.flowred is not None:
+                    L.orc_wel_fn(nd.size, nd, np.ascontiguousarray(p.v1, dtype=np.float64),
+                                 self.pk_rhs[t], self.ibound, 1, wel_flowred(p.flowred),
+                                 self.icelltype, d.top, d.bot, self.x, d.ia,
+                                 self.amat, self.rhs)
+                elif p.kind == "DRN" and p.depth is not None:
+                    L.orc_drn_fn(nd.size, nd, np.ascontiguousarray(p.v1, dtype=np.float64),
+                                 np.ascontiguousarray(p.v2, dtype=np.float64), 1,
+                                 np.ascontiguousarray(p.depth, dtype=np.float64),
+                                 self.ibound, self.x, d.ia, self.amat, self.rhs)
+
+    def _bnd_cf(self, t, p):
+        """bnd_cf of package t (WEL, DRN or RCH) into pk_hcof / pk_rhs."""
+        L = lib()
+        d = self.dis
+        nd = self.pk_nodes[t]
+        v1 = np.ascontiguousarray(p.v1, dtype=np.float64)
+        if p.kind == "WEL":
+            on = p.flowred is not None
+            L.orc_wel_cf(nd.size, nd, v1, self.ibound, 1 if on else 0,
+                         wel_flowred(p.flowred) if on else 0.0, self.icelltype,
+                         d.top, d.bot, self.x, self.pk_hcof[t], self.pk_rhs[t])
+        elif p.kind == "RCH":
+            L.orc_rch_cf(nd.size, nd, v1, d.area, self.ibound, self.pk_hcof[t],
+                         self.pk_rhs[t])
+        elif p.kind == "DRN":
+            idepth = 1 if p.depth is not None else 0
+            depth = (np.ascontiguousarray(p.depth, dtype=np.float64) if idepth
+                     else np.zeros(nd.size))
+            # cubic scaling by default under NEWTON, or DEV_CUBIC_SCALING
+            # (gwf-drn.f90:130-135, :207)
+            icubic = 1 if (p.cubic or self.inewton) else 0
+            L.orc_drn_cf(nd.size, nd, v1, np.ascontiguousarray(p.v2, dtype=np.float64),
+                         idepth, depth, icubic, self.ibound, self.x,
+                         self.pk_hcof[t], self.pk_rhs[t])
 
     # -- gwf_cq + gwf_bd (gwf.f90:741-800): flowja (positive into the cell;
     #    the diagonal ends as the cell's flow residual) and the model budget
@@ -727,17 +778,9 @@ class GwfSimulation:
             nd = self.pk_nodes[t]
             if p.kind == "CHD":
                 continue
-            # gwf_cq calls bnd_cf first (heads-dependent terms, DRN)
-            if p.kind == "WEL":
-                L.orc_wel_cf(nd.size, nd, np.ascontiguousarray(p.v1),
-                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
-            elif p.kind == "RCH":
-                L.orc_rch_cf(nd.size, nd, np.ascontiguousarray(p.v1), d.area,
-                             self.ibound, self.pk_hcof[t], self.pk_rhs[t])
-            elif p.kind == "DRN":
-                L.orc_drn_cf(nd.size, nd, np.ascontiguousarray(p.v1),
-                             np.ascontiguousarray(p.v2), self.ibound, self.x,
-                             self.pk_hcof[t], self.pk_rhs[t])
+            # gwf_cq calls bnd_cf first (heads-dependent terms: DRN, and
+            # WEL with AUTO_FLOW_REDUCE)
+            self._bnd_cf(t, p)
             sv = np.zeros(nd.size)
             L.orc_bnd_cq(nd.size, nd, self.pk_hcof[t], self.pk_rhs[t],
                          self.ibound, self.x, d.ia, flowja, sv)

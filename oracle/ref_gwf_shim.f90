@@ -27,6 +27,8 @@ module mf6ref_gwf_shim
   use GwfNpfModule, only: GwfNpfType
   use GwfStoModule, only: GwfStoType
   use DrnModule, only: DrnType
+  use BndModule, only: BndType
+  use WelModule, only: wel_create
   use GwfHfbModule, only: GwfHfbType
   use NumericalSolutionModule, only: NumericalSolutionType
   use GwfModule, only: GwfModelType
@@ -438,6 +440,103 @@ contains
                 drn%icubic_scaling, drn%imover, drn%naux)
     deallocate (drn)
   end subroutine ref_drn
+
+  ! ------------------------------------------------------------------
+  ! WEL
+  ! ------------------------------------------------------------------
+
+  !> gwf-wel.f90:296 wel_cf, then :340 wel_fc and :378 wel_fn on amat /
+  !! rhs, and BoundaryPackage.f90:545 bnd_cq (simvals, flowja diagonal).
+  !! iflowred / flowred: AUTO_FLOW_REDUCE as wel_options leaves them;
+  !! icelltype the NPF cell types the package points at.  WelType is
+  !! private to its module, so the package is made by wel_create (:73) and
+  !! filled through the memory manager: the model, NPF (ICELLTYPE) and
+  !! input-context variables its allocate_scalars / allocate_arrays
+  !! (:135-178) read are created first, under a new model name per call.
+  subroutine ref_wel(nodes, nja, nbound, nodelist, q, iflowred, flowred, &
+                     icelltype, ibound, xnew, hcof, brhs, amat, rhs, flowja, &
+                     simvals) bind(C, name='ref_wel')
+    use MemoryManagerModule, only: mem_allocate, mem_setptr
+    use MemoryHelperModule, only: create_mem_path
+    use SimVariablesModule, only: idm_context
+    use CharacterStringModule, only: CharacterStringType
+    use ConstantsModule, only: LENBOUNDNAME, LENMODELNAME, LENMEMPATH
+    integer(c_int), value :: nodes, nja, nbound, iflowred
+    real(c_double), value :: flowred
+    integer(c_int), intent(in) :: nodelist(nbound), ibound(nodes)
+    integer(c_int), intent(in) :: icelltype(nodes)
+    real(c_double), intent(in) :: q(nbound), xnew(nodes)
+    real(c_double), intent(inout) :: hcof(nbound), brhs(nbound)
+    real(c_double), intent(inout) :: amat(nja), rhs(nodes), flowja(nja), &
+                                    simvals(nbound)
+    class(BndType), pointer :: pk
+    integer(I4B), pointer :: iv
+    real(DP), pointer :: rv
+    integer(I4B), dimension(:, :), pointer, contiguous :: cid
+    type(CharacterStringType), dimension(:), pointer, contiguous :: bnm
+    real(DP), dimension(:, :), pointer, contiguous :: aux
+    real(DP), dimension(:), pointer, contiguous :: qp
+    integer(I4B), dimension(:), pointer, contiguous :: ict, ib
+    real(DP), dimension(:), pointer, contiguous :: xn
+    character(len=LENMODELNAME) :: mname
+    character(len=LENMEMPATH) :: mpath, inpath
+    integer(I4B), save :: ncall = 0
+    real(DP), allocatable :: r(:), f(:)
+    allocate (r(nodes), f(nja))
+    r = rhs
+    f = flowja
+    ncall = ncall + 1
+    write (mname, '(a,i0)') 'SHIMWEL', ncall
+    mpath = create_mem_path(mname)
+    inpath = create_mem_path(mname, 'WEL', idm_context)
+    call mem_allocate(iv, 'INEWTON', mpath)
+    iv = 1
+    call mem_allocate(iv, 'IPRPAK', mpath)
+    iv = 0
+    call mem_allocate(iv, 'IPRFLOW', mpath)
+    iv = 0
+    call mem_allocate(iv, 'IPAKCB', mpath)
+    iv = 0
+    call mem_allocate(iv, 'IPER', inpath)
+    iv = 1
+    call mem_allocate(cid, 3, nbound, 'CELLID', inpath)
+    call mem_allocate(bnm, LENBOUNDNAME, nbound, 'BOUNDNAME', inpath)
+    call mem_allocate(aux, 0, nbound, 'AUXVAR', inpath)
+    call mem_allocate(qp, nbound, 'Q', inpath)
+    call mem_allocate(ict, nodes, 'ICELLTYPE', create_mem_path(mname, 'NPF'))
+    ict = icelltype
+    call wel_create(pk, 1, 1, 0, 0, mname, 'WEL', inpath)
+    pk%maxbound = nbound
+    pk%ncolbnd = 1
+    call pk%allocate_arrays()
+    pk%nbound = nbound
+    pk%nodelist(1:nbound) = nodelist
+    qp = q
+    call mem_setptr(iv, 'IFLOWRED', pk%memoryPath)
+    iv = iflowred
+    call mem_setptr(rv, 'FLOWRED', pk%memoryPath)
+    rv = flowred
+    allocate (ib(nodes), xn(nodes))
+    ib = ibound
+    xn = xnew
+    pk%ibound => ib
+    pk%xnew => xn
+    pk%dis => dis
+    call pk%bnd_cf()
+    hcof = pk%hcof(1:nbound)
+    brhs = pk%rhs(1:nbound)
+    spm%amat = amat
+    call pk%bnd_fc(r, dis%con%ia, idxglo, mat)
+    call pk%bnd_fn(r, dis%con%ia, idxglo, mat)
+    amat = spm%amat
+    rhs = r
+    pk%simvals = 0.0_DP
+    call pk%bnd_cq(xnew, f)
+    flowja = f
+    simvals = pk%simvals(1:nbound)
+    nullify (pk%ibound, pk%xnew)
+    deallocate (ib, xn)
+  end subroutine ref_wel
 
   ! ------------------------------------------------------------------
   ! HFB

@@ -20,7 +20,8 @@ module mf6ref_shim
   use MathUtilModule, only: is_close
   use GwfConductanceUtilsModule, only: hcond, vcond, condmean
   use SmoothingModule, only: sQuadraticSaturation, &
-                             sQuadraticSaturationDerivative
+                             sQuadraticSaturationDerivative, sQSaturation, &
+                             sQSaturationDerivative
   use GwfStorageUtilsModule, only: SsTerms, SyTerms, SsCapacity, SyCapacity
   use InputOutputModule, only: ulasav, ubdsv1, ubdsv06, ubdsvd
   implicit none
@@ -264,6 +265,21 @@ contains
     real(c_double) :: y
     y = sQuadraticSaturationDerivative(top, bot, x, eps)
   end function ref_sqsat_deriv
+
+  ! SmoothingFunctions.f90:412 sQSaturation / :466 sQSaturationDerivative
+  function ref_sqsat_cubic(top, bot, x, c1, c2) result(y) &
+    bind(C, name='ref_sqsat_cubic')
+    real(c_double), value :: top, bot, x, c1, c2
+    real(c_double) :: y
+    y = sQSaturation(top, bot, x, c1, c2)
+  end function ref_sqsat_cubic
+
+  function ref_sqsat_cubic_deriv(top, bot, x, c1, c2) result(y) &
+    bind(C, name='ref_sqsat_cubic_deriv')
+    real(c_double), value :: top, bot, x, c1, c2
+    real(c_double) :: y
+    y = sQSaturationDerivative(top, bot, x, c1, c2)
+  end function ref_sqsat_cubic_deriv
 
   ! GwfStorageUtils.f90:32 SsTerms, :98 SyTerms, :143 SsCapacity, :169
   subroutine ref_ssterms(iconvert, iorig_ss, iconf_ss, top, bot, rho1, &

@@ -13,7 +13,10 @@ calls the type-bound procedures:
 * sto_fc / sto_fn / sto_cq (gwf-sto.f90:226, :353, :447), with STO's
   satomega as source_options sets it (1e-6 under NEWTON, :826-829)
 * drn_cf + drn_fc + drn_fn + bnd_cq (gwf-drn.f90:340, :377, :418,
-  BoundaryPackage.f90:545)
+  BoundaryPackage.f90:545), with and without a depth auxiliary
+* wel_cf + wel_fc + wel_fn + bnd_cq (gwf-wel.f90:296, :340, :378), with and
+  without AUTO_FLOW_REDUCE (WelType is private to its module: the shim makes
+  it with wel_create and fills it through the memory manager)
 * sln_underrelax (NumericalSolution.f90:2989): SIMPLE, COOLEY and
   delta-bar-delta over several outer iterations
 * sln_backtracking_xupdate (:2783)
@@ -200,7 +203,12 @@ def test_sto_steady_is_noop(grid):
     assert eqbits(a2, g.amat0) and eqbits(r2, g.rhs0)
 
 
-def test_drn_cf_fc_cq(grid):
+@pytest.mark.parametrize("idepth,icubic", [(0, 0), (1, 0), (1, 1)])
+def test_drn_cf_fc_fn_cq(grid, idepth, icubic):
+    """drn_cf + drn_fc + drn_fn + bnd_cq: a step at the drain elevation, or
+    with a depth auxiliary (AUXDEPTHNAME; positive, negative and zero
+    depths) the linear (sQuadraticSaturation, eps = 0) or cubic
+    (sQSaturation, DEV_CUBIC_SCALING) scaling and the cubic Newton term."""
     g, d = grid, grid.d
     rng = g.rng
     nb = g.n // 3
@@ -208,21 +216,74 @@ def test_drn_cf_fc_cq(grid):
     elev = g.hnew[nodes] + rng.uniform(-2.0, 2.0, nb)
     elev[: nb // 6] = g.hnew[nodes[: nb // 6]]        # xnew == drain elevation
     cond = rng.uniform(1.0, 500.0, nb)
+    depth = rng.uniform(-3.0, 3.0, nb)
+    depth[nb // 6: nb // 4] = 0.0
     h1, b1 = np.zeros(nb), np.zeros(nb)
-    L.orc_drn_cf(nb, nodes, elev, cond, g.ibound, g.hnew, h1, b1)
+    L.orc_drn_cf(nb, nodes, elev, cond, idepth, depth, icubic, g.ibound, g.hnew, h1, b1)
     a1, r1 = g.amat0.copy(), g.rhs0.copy()
     L.orc_bnd_fc(nb, nodes, h1, b1, d.ia, a1, r1)
+    L.orc_drn_fn(nb, nodes, elev, cond, idepth, depth, g.ibound, g.hnew, d.ia, a1, r1)
     f1, sv1 = g.amat0.copy(), np.zeros(nb)
     L.orc_bnd_cq(nb, nodes, h1, b1, g.ibound, g.hnew, d.ia, f1, sv1)
     h2, b2 = np.zeros(nb), np.zeros(nb)
     a2, r2 = g.amat0.copy(), g.rhs0.copy()
     f2, sv2 = g.amat0.copy(), np.zeros(nb)
-    R.ref_drn(g.n, d.nja, nb, one(nodes), elev, cond, 0, np.zeros(nb), 0,
+    R.ref_drn(g.n, d.nja, nb, one(nodes), elev, cond, idepth, depth, icubic,
               g.ibound, g.hnew, h2, b2, a2, r2, f2, sv2)
     assert eqbits(h1, h2) and eqbits(b1, b2)
     assert eqbits(a1, a2) and eqbits(r1, r2)
     assert eqbits(f1, f2) and eqbits(sv1, sv2)
     assert np.count_nonzero(h1) > 0 and np.count_nonzero(h1) < nb
+    if idepth:
+        # some drains inside their scaling interval: fractional factors
+        frac = -h1 / np.where(g.ibound[nodes] > 0, cond, 1.0)
+        assert np.count_nonzero((frac > 0.0) & (frac < 1.0)) > 0
+
+
+@pytest.mark.parametrize("iflowred,flowred", [(0, 0.0), (1, 0.05), (1, 0.4), (1, 1.0)])
+def test_wel_cf_fc_fn_cq(grid, iflowred, flowred):
+    """wel_cf + wel_fc + wel_fn + bnd_cq (gwf-wel.f90:296-424) with and
+    without AUTO_FLOW_REDUCE: withdrawals in convertible cells scaled by
+    sQSaturation over the bottom flowred of the cell, injections and
+    confined cells untouched, the Newton term on the diagonal and rhs."""
+    g, d = grid, grid.d
+    rng = g.rng
+    nb = g.n // 3
+    nodes = np.ascontiguousarray(rng.choice(g.n, nb, replace=False), dtype=np.int32)
+    q = rng.uniform(-400.0, 150.0, nb)
+    q[: nb // 10] = 0.0
+    # a fifth of the wells with heads inside the reduction interval
+    h = g.hnew.copy()
+    band = nodes[nb // 10: nb // 10 + nb // 5]
+    fr = flowred if iflowred else 0.3
+    h[band] = d.bot[band] + rng.uniform(0.05, 0.95, band.size) * fr * (d.top[band] - d.bot[band])
+    h1, b1 = np.zeros(nb), np.zeros(nb)
+    L.orc_wel_cf(nb, nodes, q, g.ibound, iflowred, flowred, g.icelltype, d.top, d.bot,
+                 h, h1, b1)
+    a1, r1 = g.amat0.copy(), g.rhs0.copy()
+    L.orc_bnd_fc(nb, nodes, h1, b1, d.ia, a1, r1)
+    L.orc_wel_fn(nb, nodes, q, b1, g.ibound, iflowred, flowred, g.icelltype, d.top,
+                 d.bot, h, d.ia, a1, r1)
+    f1, sv1 = g.amat0.copy(), np.zeros(nb)
+    L.orc_bnd_cq(nb, nodes, h1, b1, g.ibound, h, d.ia, f1, sv1)
+    h2, b2 = np.zeros(nb), np.zeros(nb)
+    a2, r2 = g.amat0.copy(), g.rhs0.copy()
+    f2, sv2 = g.amat0.copy(), np.zeros(nb)
+    R.ref_wel(g.n, d.nja, nb, one(nodes), q, iflowred, flowred, g.icelltype, g.ibound,
+              h, h2, b2, a2, r2, f2, sv2)
+    assert eqbits(h1, h2) and eqbits(b1, b2)
+    assert eqbits(a1, a2) and eqbits(r1, r2)
+    assert eqbits(f1, f2) and eqbits(sv1, sv2)
+    act = g.ibound[nodes] > 0
+    if iflowred:
+        # reduced withdrawals: some between zero and the full rate, and the
+        # Newton term changed the matrix
+        red = act & (q < 0) & (g.icelltype[nodes] != 0)
+        frac = -b1[red] / q[red]
+        assert np.count_nonzero((frac > 0.0) & (frac < 1.0)) > 0
+        assert not eqbits(a1, g.amat0)
+    else:
+        assert eqbits(b1[act], -q[act]) and eqbits(a1, g.amat0)
 
 
 @pytest.mark.parametrize("inewton", [0, 1])

@@ -60,6 +60,14 @@ def test_scalar_helpers():
             if top > bot:
                 assert eqbits(L.orc_sqsat_deriv(top, bot, x, eps),
                               R.ref_sqsat_deriv(top, bot, x, eps))
+        # sQSaturation / sQSaturationDerivative (cubic smoothing of WEL
+        # AUTO_FLOW_REDUCE and DRN depth scaling), both coefficient pairs
+        if top > bot:
+            for c1, c2 in ((-2.0, 3.0), (-1.0, 2.0)):
+                assert eqbits(L.orc_sqsat_cubic(top, bot, x, c1, c2),
+                              R.ref_sqsat_cubic(top, bot, x, c1, c2))
+                assert eqbits(L.orc_sqsat_cubic_deriv(top, bot, x, c1, c2),
+                              R.ref_sqsat_cubic_deriv(top, bot, x, c1, c2))
     for _ in range(3000):
         k1, k2 = rng.lognormal(0, 2, 2) * rng.choice([1, 1, 0], 2)
         t1, t2 = rng.uniform(0, 20, 2)
