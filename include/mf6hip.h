@@ -176,24 +176,39 @@ int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *model);
  * WEL: v1 = q.  DRN: v1 = elevation, v2 = conductance.  RCH: v1 = recharge
  * rate (nodelist = receiving cells).  CHD: v1 = head.  Packages are applied
  * in MF6HIP_PKG order regardless of call order; a second call with the same
- * type replaces that package's data. */
+ * type replaces that package's data and options (this call: none). */
 int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
                            const int *nodelist, const double *v1,
                            const double *v2);
 
-/* Package options of the reference that change a package's formulation.
- * The device path implements none of them; mf6hip_gwf_set_package_ex
- * rejects a nonzero one with MF6HIP_EUNSUPPORTED before any other check. */
+/* Package options of the reference that change a package's formulation
+ * (mf6hip_gwf_set_package_ex).
+ *  - WEL AUTO_FLOW_REDUCE (iflowred = 1, flowred the value as read: <= 0
+ *    becomes 0.1, above 1 becomes 1, as wel_options stores it,
+ *    src/Model/GroundWaterFlow/gwf-wel.f90:205-217): withdrawals in
+ *    convertible cells are scaled by sQSaturation over the bottom flowred
+ *    of the cell, with the Newton term under NEWTON (wel_cf / wel_fn,
+ *    :296-332, :378-424).
+ *  - DRN AUXDEPTHNAME (idrndepth = 1, drndepth the named auxiliary column,
+ *    nbound values): conductance scaled between elev and elev + depth
+ *    (get_drain_elevations / get_drain_factor, gwf-drn.f90:501-574),
+ *    linearly or, under NEWTON or with DEV_CUBIC_SCALING (icubic_scaling =
+ *    1), by sQSaturation; drn_fn adds the cubic Newton term (:418-470).
+ *  - AUXMULTNAME is rejected (MF6HIP_EUNSUPPORTED): the host applies the
+ *    multiplier to v1 / v2 itself. */
 typedef struct mf6hip_package_options {
-  double flowred;     /* WEL AUTO_FLOW_REDUCE fraction (0: off); wel_cf /
-                         wel_fn, src/Model/GroundWaterFlow/gwf-wel.f90:378-424 */
-  int idrndepth;      /* DRN AUXDEPTHNAME given (drain depth smoothing,
-                         gwf-drn.f90:340-470) */
-  int has_auxmult;    /* AUXMULTNAME given (the host applies the multiplier
-                         to v1 / v2 instead) */
+  double flowred;          /* WEL AUTO_FLOW_REDUCE value as read */
+  int iflowred;            /* WEL: AUTO_FLOW_REDUCE given */
+  int idrndepth;           /* DRN: AUXDEPTHNAME given (drndepth holds it) */
+  int icubic_scaling;      /* DRN: DEV_CUBIC_SCALING given */
+  int has_auxmult;         /* AUXMULTNAME given (rejected) */
+  const double *drndepth;  /* DRN: nbound drain depths, host memory */
 } mf6hip_package_options;
 
-/* mf6hip_gwf_set_package with the package's options (NULL: none). */
+/* mf6hip_gwf_set_package with the package's options (NULL: none).  The
+ * options are checked before the package data are replaced: AUXMULTNAME
+ * gives MF6HIP_EUNSUPPORTED; an option of another package type, a missing
+ * drndepth (nbound > 0) or a NaN flowred MF6HIP_EINVAL. */
 int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
                               const int *nodelist, const double *v1,
                               const double *v2, const mf6hip_package_options *opt);

@@ -112,8 +112,11 @@ class UnderRelaxC(C.Structure):
 
 
 class PackageOptionsC(C.Structure):
-    """mf6hip_package_options: package options the device path rejects."""
-    _fields_ = [("flowred", C.c_double), ("idrndepth", C.c_int), ("has_auxmult", C.c_int)]
+    """mf6hip_package_options: WEL AUTO_FLOW_REDUCE, DRN AUXDEPTHNAME /
+    DEV_CUBIC_SCALING, AUXMULTNAME (rejected)."""
+    _fields_ = [("flowred", C.c_double), ("iflowred", C.c_int), ("idrndepth", C.c_int),
+                ("icubic_scaling", C.c_int), ("has_auxmult", C.c_int),
+                ("drndepth", C.POINTER(C.c_double))]
 
 
 class NonlinearC(C.Structure):

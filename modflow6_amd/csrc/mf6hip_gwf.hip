@@ -89,6 +89,29 @@ __device__ inline double sqsat_deriv(double top, double bot, double x, double ep
   return y / b;
 }
 
+// SmoothingFunctions.f90:412-460 sQSaturation (cubic smoothing from bot
+// to top with coefficients c1 / b^3, c2 / b^2); the reference's x**3.0 and
+// x**2.0 compile to (x x) x and x x
+__device__ inline double sqsat_cubic(double top, double bot, double x, double c1, double c2) {
+  const double w = x - bot, b = top - bot;
+  const double s = w / b;
+  const double cof1 = c1 / ((b * b) * b), cof2 = c2 / (b * b);
+  if (s < 0.0) return 0.0;
+  if (s < 1.0) return cof1 * ((w * w) * w) + cof2 * (w * w);
+  return 1.0;
+}
+
+// SmoothingFunctions.f90:466-516 sQSaturationDerivative
+__device__ inline double sqsat_cubic_deriv(double top, double bot, double x, double c1,
+                                           double c2) {
+  const double w = x - bot, b = top - bot;
+  const double s = w / b;
+  const double cof1 = c1 * 3.0 / ((b * b) * b), cof2 = c2 * 2.0 / (b * b);
+  if (s < 0.0) return 0.0;
+  if (s < 1.0) return cof1 * (w * w) + cof2 * w;
+  return 0.0;
+}
+
 __device__ inline double logmean(double d1, double d2) {
   double drat = d2 / d1;
   if (drat <= 0.995 || drat >= 1.005) return (d2 - d1) / log(drat);
@@ -304,31 +327,71 @@ __global__ void k_hfb_reset(int nhfb, const int *__restrict__ face,
   for (int b = 0; b < nhfb; b++) condsat[face[b]] = sav[b];
 }
 
-// bnd_cf of the WEL / DRN / RCH packages
-__global__ void k_bnd_cf(int kind, int nb, const int *__restrict__ nodes,
-                         const double *__restrict__ v1, const double *__restrict__ v2,
-                         Layout L, const int *__restrict__ ibound,
-                         const double *__restrict__ x, const double *__restrict__ area,
+// One package's data and options as the kernels see them.
+struct PkgDev {
+  int kind, nb;
+  const int *nodes;
+  const double *v1, *v2, *depth;
+  int iflowred;
+  double flowred;
+  int idepth, icubic;   // icubic: cubic drain scaling (NEWTON or DEV_CUBIC_SCALING)
+};
+
+// get_drain_elevations (gwf-drn.f90:501-531): with a depth d /= 0 the
+// conductance is scaled between elev and elev + d
+__device__ inline void drn_elevations(const PkgDev &P, int e, double &drndepth, double &drntop,
+                                      double &drnbot) {
+  const double el = P.v1[e];
+  drndepth = P.idepth ? P.depth[e] : 0.0;
+  if (drndepth != 0.0) {
+    const double t = el + drndepth;
+    drntop = t > el ? t : el;
+    drnbot = t < el ? t : el;
+  } else {
+    drntop = el;
+    drnbot = el;
+  }
+}
+
+// bnd_cf of the WEL / DRN / RCH packages: wel_cf (gwf-wel.f90:296-332, with
+// AUTO_FLOW_REDUCE), drn_cf with get_drain_factor (gwf-drn.f90:340-373,
+// :534-574), rch_cf
+__global__ void k_bnd_cf(PkgDev P, GwfDev g, const double *__restrict__ x,
                          double *__restrict__ hcof, double *__restrict__ brhs) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
-    int p = pos_of_natural(L, nodes[e]);
-    if (kind == MF6HIP_PKG_WEL) {
+  const Layout &L = g.L;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < P.nb; e += gridDim.x * blockDim.x) {
+    int p = pos_of_natural(L, P.nodes[e]);
+    if (P.kind == MF6HIP_PKG_WEL) {
       hcof[e] = 0.0;
-      brhs[e] = ibound[p] <= 0 ? 0.0 : -v1[e];
-    } else if (kind == MF6HIP_PKG_DRN) {
-      if (ibound[p] <= 0) {
+      double q = P.v1[e];
+      if (P.iflowred != 0 && q < 0.0 && g.icelltype[p] != 0) {
+        const double bt = g.bot[p];
+        const double tp = bt + P.flowred * (g.top[p] - bt);
+        q = q * sqsat_cubic(tp, bt, x[p], -2.0, 3.0);
+      }
+      brhs[e] = g.ibound[p] <= 0 ? 0.0 : -q;
+    } else if (P.kind == MF6HIP_PKG_DRN) {
+      if (g.ibound[p] <= 0) {
         hcof[e] = 0.0;
         brhs[e] = 0.0;
       } else {
-        double cdrn = v2[e], drnbot = v1[e];
-        double fact = x[p] <= drnbot ? 0.0 : 1.0;
+        double cdrn = P.v2[e];
+        double drndepth, drntop, drnbot, fact;
+        drn_elevations(P, e, drndepth, drntop, drnbot);
+        const double h = x[p];
+        if (drndepth != 0.0) {
+          fact = P.icubic ? sqsat_cubic(drntop, drnbot, h, -1.0, 2.0)
+                          : sqsat(drntop, drnbot, h, 0.0);
+        } else {
+          fact = h <= drnbot ? 0.0 : 1.0;
+        }
         brhs[e] = -fact * cdrn * drnbot;
         hcof[e] = -fact * cdrn;
       }
-    } else if (kind == MF6HIP_PKG_RCH) {
+    } else if (P.kind == MF6HIP_PKG_RCH) {
       hcof[e] = 0.0;
-      double rr = -v1[e] * area[p];
-      brhs[e] = ibound[p] <= 0 ? 0.0 : rr;
+      double rr = -P.v1[e] * g.area[p];
+      brhs[e] = g.ibound[p] <= 0 ? 0.0 : rr;
     } else {
       hcof[e] = 0.0;
       brhs[e] = 0.0;
@@ -339,6 +402,7 @@ __global__ void k_bnd_cf(int kind, int nb, const int *__restrict__ nodes,
 struct BndDev {
   const int *off, *ent;
   const double *hcof[5], *brhs[5];
+  PkgDev pk[5];
 };
 
 // sln_buildsystem for one model: one thread per row
@@ -508,6 +572,38 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, in
             drterm = -rho2 * tthk * derv;
             diag = diag + (drterm + rho2);
             b = b - rterm + drterm * h + rho2 * bt;
+          }
+        }
+      }
+      // --- bnd_fn in bndlist order (gwf.f90:545-553): wel_fn
+      // (gwf-wel.f90:378-424) for AUTO_FLOW_REDUCE withdrawals in
+      // convertible cells, drn_fn (gwf-drn.f90:418-470) for drains with a
+      // depth
+      if (g.ibound[p] > 0) {
+        for (int e = bd.off[p]; e < bd.off[p + 1]; e++) {
+          const int code = bd.ent[e];
+          const int pk = code >> 28, idx = code & 0x0fffffff;
+          const PkgDev &P = bd.pk[pk];
+          if (pk == MF6HIP_PKG_WEL) {
+            if (P.iflowred == 0 || g.icelltype[p] == 0) continue;
+            const double q = -bd.brhs[pk][idx];
+            if (q < 0.0) {
+              const double bt = g.bot[p];
+              const double tp = bt + P.flowred * (g.top[p] - bt);
+              double drterm = sqsat_cubic_deriv(tp, bt, hcell, -2.0, 3.0);
+              drterm = drterm * P.v1[idx];
+              diag = diag + drterm;
+              b = b + drterm * hcell;
+            }
+          } else if (pk == MF6HIP_PKG_DRN && P.idepth) {
+            double drndepth, drntop, drnbot;
+            drn_elevations(P, idx, drndepth, drntop, drnbot);
+            if (drndepth != 0.0) {
+              double drterm = sqsat_cubic_deriv(drntop, drnbot, hcell, -1.0, 2.0);
+              drterm = drterm * P.v2[idx] * (drnbot - hcell);
+              diag = diag + drterm;
+              b = b + drterm * hcell;
+            }
           }
         }
       }
@@ -1208,31 +1304,56 @@ extern "C" int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, cons
   return MF6HIP_OK;
 }
 
-// Package options that change the formulation: checked before anything
-// else, so a host that parses them learns at once what the device path does
-// not model.
+// Package options that change the formulation (wel_options,
+// gwf-wel.f90:185-230; drn_options, gwf-drn.f90:187-241): validated before
+// the package data are replaced, stored with them after.
 extern "C" int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
                                          const int *nodelist, const double *v1,
                                          const double *v2,
                                          const mf6hip_package_options *opt) {
   if (opt) {
-    if (pkgtype == MF6HIP_PKG_WEL && opt->flowred != 0.0) {
-      mf6hip_set_error("WEL AUTO_FLOW_REDUCE (head-dependent well rates, wel_cf / wel_fn, "
-                       "gwf-wel.f90:378-424) is not supported on the device path");
-      return MF6HIP_EUNSUPPORTED;
-    }
-    if (pkgtype == MF6HIP_PKG_DRN && opt->idrndepth != 0) {
-      mf6hip_set_error("DRN AUXDEPTHNAME (drain depth smoothing, drn_cf / drn_fn, "
-                       "gwf-drn.f90:340-470) is not supported on the device path");
-      return MF6HIP_EUNSUPPORTED;
-    }
     if (opt->has_auxmult != 0) {
       mf6hip_set_error("AUXMULTNAME: multiply the package values on the host before "
                        "mf6hip_gwf_set_package_ex");
       return MF6HIP_EUNSUPPORTED;
     }
+    if (opt->iflowred != 0 && pkgtype != MF6HIP_PKG_WEL) {
+      mf6hip_set_error("AUTO_FLOW_REDUCE is a WEL option");
+      return MF6HIP_EINVAL;
+    }
+    if ((opt->idrndepth != 0 || opt->icubic_scaling != 0) && pkgtype != MF6HIP_PKG_DRN) {
+      mf6hip_set_error("AUXDEPTHNAME / DEV_CUBIC_SCALING are DRN options");
+      return MF6HIP_EINVAL;
+    }
+    if (opt->idrndepth != 0 && nbound > 0 && !opt->drndepth) {
+      mf6hip_set_error("DRN AUXDEPTHNAME needs the depths (drndepth)");
+      return MF6HIP_EINVAL;
+    }
+    if (opt->iflowred != 0 && !(opt->flowred == opt->flowred)) {
+      mf6hip_set_error("AUTO_FLOW_REDUCE value is not a number");
+      return MF6HIP_EINVAL;
+    }
   }
-  return mf6hip_gwf_set_package(handle, pkgtype, nbound, nodelist, v1, v2);
+  CK(mf6hip_gwf_set_package(handle, pkgtype, nbound, nodelist, v1, v2));
+  if (!opt) return MF6HIP_OK;
+  Mf6Handle *h = (Mf6Handle *)handle;
+  DevPackage &pk = h->pk[pkgtype];
+  if (opt->iflowred != 0) {
+    // wel_options: a value <= 0 becomes 0.1, one above 1 becomes 1
+    double f = opt->flowred;
+    pk.iflowred = 1;
+    pk.flowred = f <= 0.0 ? 0.1 : (f > 1.0 ? 1.0 : f);
+  }
+  if (opt->idrndepth != 0) {
+    pk.idepth = 1;
+    if (nbound > 0) {
+      CK(galloc(&pk.depth, (size_t)nbound));
+      HIPCHK(hipMemcpy(pk.depth, opt->drndepth, sizeof(double) * nbound,
+                       hipMemcpyHostToDevice));
+    }
+  }
+  pk.icubic = opt->icubic_scaling != 0 ? 1 : 0;
+  return MF6HIP_OK;
 }
 
 extern "C" int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
@@ -1245,7 +1366,7 @@ extern "C" int mf6hip_gwf_set_package(void *handle, int pkgtype, int nbound,
   if (pkgtype == MF6HIP_PKG_DRN && nbound > 0 && !v2) { mf6hip_set_error("DRN needs conductance (v2)"); return MF6HIP_EINVAL; }
   if (nbound >= (1 << 28)) { mf6hip_set_error("too many bounds"); return MF6HIP_EUNSUPPORTED; }
   DevPackage &pk = h->pk[pkgtype];
-  void *old[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs};
+  void *old[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs, pk.depth};
   for (void *b : old)
     if (b) hipFree(b);
   pk = DevPackage();
@@ -1382,6 +1503,24 @@ extern "C" int mf6hip_gwf_advance(void *handle, int iss, double delt) {
   return MF6HIP_OK;
 }
 
+// the kernels' view of package t (drains scale cubically under NEWTON or
+// with DEV_CUBIC_SCALING, gwf-drn.f90:130-135)
+static PkgDev pkg_dev(const Mf6Handle *h, int t) {
+  const DevPackage &pk = h->pk[t];
+  PkgDev P;
+  P.kind = t;
+  P.nb = pk.nbound;
+  P.nodes = pk.nodes;
+  P.v1 = pk.v1;
+  P.v2 = pk.v2;
+  P.depth = pk.depth;
+  P.iflowred = pk.iflowred;
+  P.flowred = pk.flowred;
+  P.idepth = pk.idepth;
+  P.icubic = (pk.icubic || h->m.inewton) ? 1 : 0;
+  return P;
+}
+
 extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double *amat_out,
                                     double *rhs_out) {
   (void)kiter;
@@ -1394,8 +1533,7 @@ extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double
   for (int t = MF6HIP_PKG_WEL; t <= MF6HIP_PKG_RCH; t++) {
     DevPackage &pk = h->pk[t];
     if (pk.nbound > 0)
-      k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(t, pk.nbound, pk.nodes, pk.v1, pk.v2, L,
-                                                      h->ibound, h->x, h->area, pk.hcof,
+      k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(pkg_dev(h, t), make_dev(h), h->x, pk.hcof,
                                                       pk.brhs);
   }
   BndDev bd;
@@ -1404,6 +1542,7 @@ extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double
   for (int t = 0; t < 5; t++) {
     bd.hcof[t] = h->pk[t].hcof;
     bd.brhs[t] = h->pk[t].brhs;
+    bd.pk[t] = pkg_dev(h, t);
   }
   GwfDev g = make_dev(h);
   g.fn = h->m.inewton && inewton;
@@ -1669,8 +1808,7 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
     h->bsim_seg_off[q] = (size_t)(seg - h->bsim);
     h->bsim_seg_len[q] = pk.nbound;
     if (pk.nbound > 0) {
-      k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(t, pk.nbound, pk.nodes, pk.v1, pk.v2, L,
-                                                      h->ibound, h->x, h->area, pk.hcof,
+      k_bnd_cf<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(pkg_dev(h, t), make_dev(h), h->x, pk.hcof,
                                                       pk.brhs);
       k_bnd_rate<<<ggrid(pk.nbound), MF6_BLOCK, 0, s>>>(pk.nbound, pk.nodes, L, h->ibound, h->x,
                                                         pk.hcof, pk.brhs, seg);

@@ -26,6 +26,12 @@ struct DevPackage {
   double *v2 = nullptr;
   double *hcof = nullptr;  // bnd_cf results
   double *brhs = nullptr;
+  // options (mf6hip_package_options): WEL AUTO_FLOW_REDUCE, DRN depth
+  int iflowred = 0;
+  double flowred = 0.0;    // as wel_options stores it, (0, 1]
+  int idepth = 0;
+  int icubic = 0;          // DEV_CUBIC_SCALING given
+  double *depth = nullptr; // DRN depths (idepth)
 };
 
 struct Mf6Handle {

@@ -1791,7 +1791,7 @@ extern "C" int mf6hip_destroy(void *handle) {
   mf6hip_tl_free(h);
   for (int t = 0; t < 5; t++) {
     DevPackage &pk = h->pk[t];
-    void *pb[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs};
+    void *pb[] = {pk.nodes, pk.v1, pk.v2, pk.hcof, pk.brhs, pk.depth};
     for (void *b : pb)
       if (b) hipFree(b);
   }

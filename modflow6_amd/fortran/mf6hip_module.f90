@@ -104,11 +104,16 @@ module mf6hip_module
     real(c_double) :: relaxold
   end type mf6hip_underrelax
 
-  !> mf6hip_package_options: package options the device path rejects
+  !> mf6hip_package_options: WEL AUTO_FLOW_REDUCE, DRN AUXDEPTHNAME /
+  !! DEV_CUBIC_SCALING (drndepth: c_loc of the depth column), AUXMULTNAME
+  !! (rejected)
   type, bind(C), public :: mf6hip_package_options
     real(c_double) :: flowred
+    integer(c_int) :: iflowred
     integer(c_int) :: idrndepth
+    integer(c_int) :: icubic_scaling
     integer(c_int) :: has_auxmult
+    type(c_ptr) :: drndepth
   end type mf6hip_package_options
 
   !> mf6hip_nonlinear: the IMS NONLINEAR block for the XMI-style outer loop

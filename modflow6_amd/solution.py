@@ -104,10 +104,25 @@ class GwfSolution:
             v1 = np.ascontiguousarray(p.v1, dtype=np.float64)
             v2 = (np.ascontiguousarray(p.v2, dtype=np.float64)
                   if p.v2 is not None else None)
-            _lib.check(L.mf6hip_gwf_set_package(self.h, _lib.PKG[p.kind],
-                                                nodes.size, _lib.iptr(nodes),
-                                                _lib.dptr(v1), _lib.dptr(v2)),
-                       f"mf6hip_gwf_set_package({p.kind})")
+            opt = None
+            if p.flowred is not None or p.depth is not None or p.cubic:
+                # WEL AUTO_FLOW_REDUCE (the value as read), DRN AUXDEPTHNAME
+                # and DEV_CUBIC_SCALING
+                opt = _lib.PackageOptionsC()
+                if p.flowred is not None:
+                    opt.iflowred = 1
+                    opt.flowred = float(p.flowred)
+                if p.depth is not None:
+                    depth = np.ascontiguousarray(p.depth, dtype=np.float64)
+                    self._keep.append(depth)
+                    opt.idrndepth = 1
+                    opt.drndepth = _lib.dptr(depth)
+                opt.icubic_scaling = 1 if p.cubic else 0
+            _lib.check(L.mf6hip_gwf_set_package_ex(self.h, _lib.PKG[p.kind],
+                                                   nodes.size, _lib.iptr(nodes),
+                                                   _lib.dptr(v1), _lib.dptr(v2),
+                                                   C.byref(opt) if opt is not None else None),
+                       f"mf6hip_gwf_set_package_ex({p.kind})")
         if c.hfb is not None:
             self.set_hfb(*c.hfb)
         x0 = np.ascontiguousarray(c.strt, dtype=np.float64)

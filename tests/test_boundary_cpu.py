@@ -242,17 +242,23 @@ def test_xmi_entry_points_validate_without_a_device():
     assert L.mf6hip_xmi_last_iteration(None, C.byref(r)) == 1
 
 
-@pytest.mark.parametrize("pkg,opt,msg", [
-    (_lib.PKG["WEL"], dict(flowred=0.1), "AUTO_FLOW_REDUCE"),
-    (_lib.PKG["DRN"], dict(idrndepth=1), "AUXDEPTHNAME"),
-    (_lib.PKG["RCH"], dict(has_auxmult=1), "AUXMULTNAME"),
+@pytest.mark.parametrize("pkg,opt,nbound,rc,msg", [
+    (_lib.PKG["RCH"], dict(has_auxmult=1), 0, 3, "AUXMULTNAME"),
+    (_lib.PKG["DRN"], dict(iflowred=1, flowred=0.2), 0, 1, "WEL option"),
+    (_lib.PKG["WEL"], dict(idrndepth=1), 0, 1, "DRN options"),
+    (_lib.PKG["RCH"], dict(icubic_scaling=1), 0, 1, "DRN options"),
+    (_lib.PKG["DRN"], dict(idrndepth=1), 3, 1, "needs the depths"),
+    (_lib.PKG["WEL"], dict(iflowred=1, flowred=float("nan")), 0, 1, "not a number"),
+    (_lib.PKG["WEL"], dict(iflowred=1, flowred=0.2), 0, 1, "no model"),
 ])
-def test_set_package_rejects_unsupported_options(pkg, opt, msg):
-    """Package options the device path does not model are rejected before
-    any other work (no handle is needed to see it)."""
+def test_set_package_option_checks(pkg, opt, nbound, rc, msg):
+    """Package options are checked before the package data are replaced
+    (no handle is needed to see it): AUXMULTNAME is not modelled, options
+    of another package type and missing depths are errors; valid options go
+    on to the plain package path (here: no handle)."""
     L = _lib.load()
     o = _lib.PackageOptionsC(**opt)
-    assert L.mf6hip_gwf_set_package_ex(None, pkg, 0, None, None, None, C.byref(o)) == 3
+    assert L.mf6hip_gwf_set_package_ex(None, pkg, nbound, None, None, None, C.byref(o)) == rc
     assert msg in _lib.last_error()
     # no options: the plain package path (here: no handle)
     assert L.mf6hip_gwf_set_package_ex(None, pkg, 0, None, None, None, None) == 1
