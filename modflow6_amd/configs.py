@@ -131,6 +131,11 @@ class GwfCase:
     # HFB package: (noden, nodem, hydchr), 0-based node numbers of
     # horizontally adjacent cells (gwf-hfb.f90 read_data)
     hfb: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None
+    # GNC package of this model, explicit form: (noden, nodem, nodesj,
+    # alphasj), 0-based connected cells n, m and per GNC numjs contributing
+    # cells (-1: none) with their weights, shapes (nexg,), (nexg,),
+    # (nexg, numjs), (nexg, numjs) (GhostNode.f90 read_data)
+    gnc: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]] = None
 
     @property
     def nodes(self):

@@ -965,6 +965,112 @@ void orc_bnd_fc(int nbound, const int *nodelist, const double *hcof,
   }
 }
 
+/* Ghost-node correction of one model in explicit form (GhostNode.f90).
+ * GNC g corrects the flow between noden[g] and nodem[g] (0-based, connected
+ * at CSR position ipos[g] of row n) with the heads of nodesj[g*numjs + i]
+ * (-1: none) weighted by alphasj[g*numjs + i]. */
+
+/* GhostNode.f90:280-324 gnc_fc (explicit) with gnc_fmsav :251-273: cond is
+ * the matrix value at (n, m) when gnc_fc runs (the npf conductance), saved
+ * for the budget; rhs(n) -= alpha cond (x(n) - x(j)), rhs(m) += the same. */
+void orc_gnc_fc(int nexg, int numjs, const int *noden, const int *nodem,
+                const int *nodesj, const double *alphasj, const int *ipos,
+                const int *ibound, const double *x, const double *amat,
+                double *cond, double *rhs)
+{
+  for (int g = 0; g < nexg; g++) cond[g] = amat[ipos[g]];
+  for (int g = 0; g < nexg; g++) {
+    int n = noden[g], m = nodem[g];
+    if (ibound[n] == 0 || ibound[m] == 0) continue;
+    double c = cond[g];
+    for (int i = 0; i < numjs; i++) {
+      int j = nodesj[g * numjs + i];
+      if (j < 0) continue;
+      double alpha = alphasj[g * numjs + i];
+      if (alpha == DZERO) continue;
+      double aterm = alpha * c;
+      double rterm = aterm * (x[n] - x[j]);
+      rhs[n] = rhs[n] - rterm;
+      rhs[m] = rhs[m] + rterm;
+    }
+  }
+}
+
+/* GhostNode.f90:340-443 gnc_fn of one model: the derivative of the
+ * correction with respect to the upstream head of a convertible upstream
+ * cell (horizontal connections, or vertical ones with VARIABLECV), with the
+ * saturated conductance of the face. */
+void orc_gnc_fn(int nexg, int numjs, const int *noden, const int *nodem,
+                const int *nodesj, const double *alphasj, const int *ia,
+                const int *ipos, const int *isym, const int *jas,
+                const int *ihc, const double *condsat, int ivarcv,
+                const int *icelltype, const double *top, const double *bot,
+                const int *ibound, const double *x, double *amat, double *rhs)
+{
+  for (int g = 0; g < nexg; g++) {
+    int n = noden[g], m = nodem[g];
+    if (ibound[n] == 0 || ibound[m] == 0) continue;
+    int js = jas[ipos[g]];
+    int hc = ihc[js];
+    double csat = condsat[js];
+    if (hc == 0 && ivarcv == 0) continue;
+    int iups = x[m] > x[n] ? 1 : 0;
+    int up = iups == 0 ? n : m;
+    double topup = top[up], botup = bot[up], xup = x[up];
+    if (icelltype[up] == 0) continue;
+    for (int i = 0; i < numjs; i++) {
+      int j = nodesj[g * numjs + i];
+      if (j < 0) continue;
+      if (ibound[j] == 0) continue;
+      double alpha = alphasj[g * numjs + i];
+      if (alpha == DZERO) continue;
+      double consterm = csat * alpha * (x[n] - x[j]);
+      double derv = orc_sqsat_deriv(topup, botup, xup, DEM6);
+      double term = consterm * derv;
+      if (iups == 0) {
+        amat[ia[n]] = amat[ia[n]] + term;
+        if (ibound[m] > 0) amat[isym[ipos[g]]] = amat[isym[ipos[g]]] - term;
+        rhs[n] = rhs[n] + term * x[n];
+        rhs[m] = rhs[m] - term * x[n];
+      } else {
+        amat[ia[m]] = amat[ia[m]] - term;
+        if (ibound[n] > 0) amat[ipos[g]] = amat[ipos[g]] + term;
+        rhs[n] = rhs[n] + term * x[m];
+        rhs[m] = rhs[m] - term * x[m];
+      }
+    }
+  }
+}
+
+/* GhostNode.f90:478-503 gnc_cq with deltaQgnc :507-543: the correction flow
+ * (sum_j alpha_j) x(n) - sum_j alpha_j x(j), times the saved cond, added to
+ * flowja at (n, m) and taken from (m, n). */
+void orc_gnc_cq(int nexg, int numjs, const int *noden, const int *nodem,
+                const int *nodesj, const double *alphasj, const int *ipos,
+                const int *isym, const int *ibound, const double *x,
+                const double *cond, double *flowja)
+{
+  for (int g = 0; g < nexg; g++) {
+    int n = noden[g], m = nodem[g];
+    double dq = DZERO;
+    if (ibound[n] != 0 && ibound[m] != 0) {
+      double sigalj = DZERO, hd = DZERO;
+      for (int i = 0; i < numjs; i++) {
+        int j = nodesj[g * numjs + i];
+        if (j < 0) continue;
+        if (ibound[j] == 0) continue;
+        double alpha = alphasj[g * numjs + i];
+        sigalj = sigalj + alpha;
+        hd = hd + alpha * x[j];
+      }
+      double aterm = sigalj * x[n] - hd;
+      dq = aterm * cond[g];
+    }
+    flowja[ipos[g]] = flowja[ipos[g]] + dq;
+    flowja[isym[ipos[g]]] = flowja[isym[ipos[g]]] - dq;
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* GWF budget: gwf_cq + gwf_bd (gwf.f90:741-800)                        */
 /* ------------------------------------------------------------------ */

@@ -121,6 +121,13 @@ def lib():
         L.orc_rate_accumulator.argtypes = [i, f64p, C.POINTER(d), C.POINTER(d)]
         L.orc_wel_cf.argtypes = [i, i32p, f64p, i32p, i, d, i32p, f64p, f64p,
                                  f64p, f64p, f64p]
+        L.orc_gnc_fc.argtypes = [i, i, i32p, i32p, i32p, f64p, i32p, i32p, f64p,
+                                 f64p, f64p, f64p]
+        L.orc_gnc_fn.argtypes = [i, i, i32p, i32p, i32p, f64p, i32p, i32p, i32p,
+                                 i32p, i32p, f64p, i, i32p, f64p, f64p, i32p,
+                                 f64p, f64p, f64p]
+        L.orc_gnc_cq.argtypes = [i, i, i32p, i32p, i32p, f64p, i32p, i32p, i32p,
+                                 f64p, f64p, f64p]
         L.orc_wel_fn.argtypes = [i, i32p, f64p, f64p, i32p, i, d, i32p, f64p,
                                  f64p, f64p, i32p, f64p, f64p]
         L.orc_rch_cf.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
@@ -240,6 +247,8 @@ def ref_gwf_lib():
                                    f64p, f64p, f64p, f64p, f64p, f64p]
         R.ref_drn.argtypes = [i, i, i, i32p, f64p, f64p, i, f64p, i, i32p,
                               f64p, f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_gnc.argtypes = [i, i, i, i, i32p, i32p, i32p, f64p, i32p, f64p,
+                              i, i, i32p, f64p, f64p, f64p, f64p, f64p]
         R.ref_wel.argtypes = [i, i, i, i32p, f64p, i, d, i32p, i32p, f64p,
                               f64p, f64p, f64p, f64p, f64p, f64p]
         R.ref_hfb_fc.argtypes = [i, i, i, i, i32p, i32p, i32p, f64p, i32p, i32p, f64p,
@@ -556,6 +565,9 @@ class GwfSimulation:
         self.hfb = None
         if c.hfb is not None:
             self.set_hfb(*c.hfb)
+        self.gnc = None
+        if c.gnc is not None:
+            self.set_gnc(*c.gnc)
         self.sat = np.ones(self.neq)                      # preprocess :1850
         self.x = np.ascontiguousarray(c.strt, dtype=np.float64).copy()
         self.xold = self.x.copy()
@@ -634,6 +646,25 @@ class GwfSimulation:
                                      d.hwva, d.top, d.bot, self.condsat, sav)
         self.hfb = (n, idx, sav)
 
+    def set_gnc(self, noden, nodem, nodesj, alphasj):
+        """The model's GNC package in explicit form (GhostNode.f90): n and m
+        must be connected (gnc_mc :172-246); nodesj -1 marks an unused
+        slot."""
+        d = self.dis
+        n = np.ascontiguousarray(noden, dtype=np.int32)
+        m = np.ascontiguousarray(nodem, dtype=np.int32)
+        js = np.ascontiguousarray(np.asarray(nodesj).reshape(n.size, -1), dtype=np.int32)
+        al = np.ascontiguousarray(np.asarray(alphasj, dtype=np.float64).reshape(js.shape))
+        ipos = np.zeros(n.size, dtype=np.int32)
+        for g in range(n.size):
+            row = d.ja[d.ia[n[g]] + 1:d.ia[n[g] + 1]]
+            hit = np.flatnonzero(row == m[g])
+            if hit.size == 0:
+                raise ValueError(f"GNC {g + 1}: cell {n[g] + 1} is not connected to "
+                                 f"cell {m[g] + 1}")
+            ipos[g] = d.ia[n[g]] + 1 + hit[0]
+        self.gnc = (n, m, js, al, ipos, np.zeros(n.size))
+
     def _ibotnode(self):
         # gwf-npf.f90:1905-1939: lowest vertically connected node
         d = self.dis
@@ -689,6 +720,10 @@ class GwfSimulation:
                      d.hwva, self.condsat, self.ibound, self.icelltype,
                      self.inewton, 0, 1, 0, self.k11, self.k33, d.top, d.bot,
                      self.sat, self.x, self.amat)
+        if self.gnc is not None:
+            gn, gm, gj, ga, gp, gc = self.gnc
+            L.orc_gnc_fc(gn.size, gj.shape[1], gn, gm, gj, ga, gp, self.ibound, self.x,
+                         self.amat, gc, self.rhs)
         if self.case.has_sto and not self.iss:
             L.orc_sto_fc(self.neq, d.ia, self.ibound, self.iconvert, d.top,
                          d.bot, d.area, self.ss, self.sy, self.delt,
@@ -704,6 +739,11 @@ class GwfSimulation:
             L.orc_npf_fn(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc,
                          self.condsat, self.ibound, self.icelltype, 0, d.top,
                          d.bot, self.satomega, self.x, self.amat, self.rhs)
+            if self.gnc is not None:
+                gn, gm, gj, ga, gp, gc = self.gnc
+                L.orc_gnc_fn(gn.size, gj.shape[1], gn, gm, gj, ga, d.ia, gp, d.isym, d.jas,
+                             d.ihc, self.condsat, 0, self.icelltype, d.top, d.bot,
+                             self.ibound, self.x, self.amat, self.rhs)
             if self.case.has_sto and not self.iss:
                 L.orc_sto_fn(self.neq, d.ia, self.ibound, self.iconvert,
                              d.top, d.bot, d.area, self.ss, self.sy,
@@ -761,6 +801,10 @@ class GwfSimulation:
                      d.hwva, self.condsat, self.ibound, self.icelltype,
                      self.inewton, 0, 1, 0, self.k11, self.k33, d.top, d.bot,
                      self.sat, self.x, flowja)
+        if self.gnc is not None:
+            gn, gm, gj, ga, gp, gc = self.gnc
+            L.orc_gnc_cq(gn.size, gj.shape[1], gn, gm, gj, ga, gp, d.isym, self.ibound,
+                         self.x, gc, flowja)
         terms = {}
         strgss = np.zeros(self.neq)
         strgsy = np.zeros(self.neq)

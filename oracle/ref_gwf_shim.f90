@@ -30,6 +30,7 @@ module mf6ref_gwf_shim
   use BndModule, only: BndType
   use WelModule, only: wel_create
   use GwfHfbModule, only: GwfHfbType
+  use GhostNodeModule, only: GhostNodeType
   use NumericalSolutionModule, only: NumericalSolutionType
   use GwfModule, only: GwfModelType
   use TdisModule, only: delt
@@ -537,6 +538,80 @@ contains
     nullify (pk%ibound, pk%xnew)
     deallocate (ib, xn)
   end subroutine ref_wel
+
+  ! ------------------------------------------------------------------
+  ! GNC (one model, explicit)
+  ! ------------------------------------------------------------------
+
+  !> GhostNode.f90:172 gnc_mc (matrix positions), :280 gnc_fc with
+  !! gnc_fmsav :251, under NEWTON :340 gnc_fn, then :478 gnc_cq.  nodesj /
+  !! alphasj hold numjs entries per GNC (node 0: none); cond returns the
+  !! conductances gnc_fmsav saved.
+  subroutine ref_gnc(nodes, nja, nexg, numjs, noden, nodem, nodesj, alphasj, &
+                     ibound, x, inewton, ivarcv, icelltype, condsat, amat, &
+                     rhs, cond, flowja) bind(C, name='ref_gnc')
+    integer(c_int), value :: nodes, nja, nexg, numjs, inewton, ivarcv
+    integer(c_int), intent(in) :: noden(nexg), nodem(nexg), nodesj(numjs, nexg)
+    real(c_double), intent(in) :: alphasj(numjs, nexg)
+    integer(c_int), intent(in) :: ibound(nodes), icelltype(nodes)
+    real(c_double), intent(in) :: x(nodes), condsat(*)
+    real(c_double), intent(inout) :: amat(nja), rhs(nodes), cond(nexg), &
+                                    flowja(nja)
+    type(GhostNodeType), pointer :: gnc
+    type(GwfModelType), pointer :: gwf
+    integer(I4B), allocatable :: ict(:)
+    real(DP), allocatable :: f(:)
+    allocate (gwf)
+    allocate (gwf%moffset)
+    gwf%moffset = 0
+    gwf%dis => dis
+    allocate (gwf%ibound(nodes), gwf%x(nodes), gwf%rhs(nodes))
+    gwf%ibound = ibound
+    gwf%x = x
+    gwf%rhs = rhs
+    allocate (gnc)
+    allocate (gnc%smgnc, gnc%implicit, gnc%i2kn, gnc%nexg, gnc%numjs)
+    gnc%smgnc = .true.
+    gnc%implicit = .false.
+    gnc%i2kn = .false.
+    gnc%nexg = nexg
+    gnc%numjs = numjs
+    gnc%m1 => gwf
+    gnc%m2 => gwf
+    allocate (gnc%nodem1(nexg), gnc%nodem2(nexg), gnc%nodesj(numjs, nexg), &
+              gnc%alphasj(numjs, nexg), gnc%cond(nexg), gnc%idxglo(nexg), &
+              gnc%idxsymglo(nexg), gnc%idiagn(nexg), gnc%idiagm(nexg), &
+              gnc%jposinrown(numjs, nexg), gnc%jposinrowm(numjs, nexg))
+    gnc%nodem1 = noden
+    gnc%nodem2 = nodem
+    gnc%nodesj = nodesj
+    gnc%alphasj = alphasj
+    gnc%cond = 0.0_DP
+    call gnc%gnc_mc(mat)
+    spm%amat = amat
+    call gnc%gnc_fc(1, mat)
+    if (inewton /= 0) then
+      allocate (ict(nodes))
+      ict = icelltype
+      call gnc%gnc_fn(1, mat, condsat(1:dis%njas), ivarcv_opt=ivarcv, &
+                      ictm1_opt=ict, ictm2_opt=ict)
+      deallocate (ict)
+    end if
+    amat = spm%amat
+    rhs = gwf%rhs
+    cond = gnc%cond
+    allocate (f(nja))
+    f = flowja
+    call gnc%gnc_cq(f)
+    flowja = f
+    deallocate (gnc%nodem1, gnc%nodem2, gnc%nodesj, gnc%alphasj, gnc%cond, &
+                gnc%idxglo, gnc%idxsymglo, gnc%idiagn, gnc%idiagm, &
+                gnc%jposinrown, gnc%jposinrowm)
+    deallocate (gnc%smgnc, gnc%implicit, gnc%i2kn, gnc%nexg, gnc%numjs)
+    deallocate (gnc)
+    deallocate (gwf%ibound, gwf%x, gwf%rhs, gwf%moffset)
+    deallocate (gwf)
+  end subroutine ref_gnc
 
   ! ------------------------------------------------------------------
   ! HFB

@@ -14,6 +14,8 @@ calls the type-bound procedures:
   satomega as source_options sets it (1e-6 under NEWTON, :826-829)
 * drn_cf + drn_fc + drn_fn + bnd_cq (gwf-drn.f90:340, :377, :418,
   BoundaryPackage.f90:545), with and without a depth auxiliary
+* gnc_fc (explicit) / gnc_fn / gnc_cq of one model (GhostNode.f90:280,
+  :340, :478), the reference's GhostNodeType with positions from its gnc_mc
 * wel_cf + wel_fc + wel_fn + bnd_cq (gwf-wel.f90:296, :340, :378), with and
   without AUTO_FLOW_REDUCE (WelType is private to its module: the shim makes
   it with wel_create and fills it through the memory manager)
@@ -284,6 +286,65 @@ def test_wel_cf_fc_fn_cq(grid, iflowred, flowred):
         assert not eqbits(a1, g.amat0)
     else:
         assert eqbits(b1[act], -q[act]) and eqbits(a1, g.amat0)
+
+
+def _gncs(g, rng, nexg, numjs=3):
+    """nexg ghost-node corrections on random connections (either direction,
+    some vertical), contributing cells among the neighbours of n and
+    elsewhere, unused slots (-1) and zero weights."""
+    d = g.d
+    noden, nodem, js, al = [], [], [], []
+    while len(noden) < nexg:
+        n = int(rng.integers(g.n))
+        row = d.ja[d.ia[n] + 1:d.ia[n + 1]]
+        if row.size == 0:
+            continue
+        m = int(row[rng.integers(row.size)])
+        cand = np.concatenate([row, rng.integers(0, g.n, 2)])
+        j = cand[rng.integers(cand.size, size=numjs)].astype(np.int32)
+        j[rng.random(numjs) < 0.2] = -1
+        a = rng.uniform(-0.3, 0.6, numjs)
+        a[rng.random(numjs) < 0.15] = 0.0
+        noden.append(n)
+        nodem.append(m)
+        js.append(j)
+        al.append(a)
+    return (np.array(noden, dtype=np.int32), np.array(nodem, dtype=np.int32),
+            np.ascontiguousarray(js, dtype=np.int32), np.ascontiguousarray(al))
+
+
+@pytest.mark.parametrize("inewton", [0, 1])
+def test_gnc_fc_fn_cq(grid, inewton):
+    """gnc_fc (explicit, with gnc_fmsav's conductances read from the matrix),
+    gnc_fn and gnc_cq (GhostNode.f90:251-543) of one model against the
+    reference's GhostNodeType (positions from its own gnc_mc)."""
+    g, d = grid, grid.d
+    rng = g.rng
+    nexg, numjs = g.n // 4, 3
+    gn, gm, gj, ga = _gncs(g, rng, nexg, numjs)
+    ipos = np.zeros(nexg, dtype=np.int32)
+    for k in range(nexg):
+        row = d.ja[d.ia[gn[k]] + 1:d.ia[gn[k] + 1]]
+        ipos[k] = d.ia[gn[k]] + 1 + np.flatnonzero(row == gm[k])[0]
+    a1, r1 = g.amat0.copy(), g.rhs0.copy()
+    c1 = np.zeros(nexg)
+    L.orc_gnc_fc(nexg, numjs, gn, gm, gj, ga, ipos, g.ibound, g.hnew, a1, c1, r1)
+    if inewton:
+        L.orc_gnc_fn(nexg, numjs, gn, gm, gj, ga, d.ia, ipos, d.isym, d.jas, d.ihc,
+                     g.condsat, 0, g.icelltype, d.top, d.bot, g.ibound, g.hnew, a1, r1)
+    f1 = g.amat0.copy()
+    L.orc_gnc_cq(nexg, numjs, gn, gm, gj, ga, ipos, d.isym, g.ibound, g.hnew, c1, f1)
+    a2, r2 = g.amat0.copy(), g.rhs0.copy()
+    c2, f2 = np.zeros(nexg), g.amat0.copy()
+    R.ref_gnc(g.n, d.nja, nexg, numjs, one(gn), one(gm),
+              np.ascontiguousarray(gj + 1, dtype=np.int32), ga, g.ibound, g.hnew,
+              inewton, 0, g.icelltype, g.condsat, a2, r2, c2, f2)
+    assert eqbits(c1, c2)
+    assert eqbits(r1, r2) and eqbits(a1, a2)
+    assert eqbits(f1, f2)
+    assert not eqbits(r1, g.rhs0) and not eqbits(f1, g.amat0)
+    if inewton:
+        assert not eqbits(a1, g.amat0)
 
 
 @pytest.mark.parametrize("inewton", [0, 1])
