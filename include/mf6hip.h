@@ -229,6 +229,23 @@ int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
 int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, const int *nodem,
                        const double *hydchr);
 
+/* Ghost-node correction of the model (the GNC package of one model,
+ * src/Exchange/GhostNode.f90): a new set replaces the last one, as gnc_df /
+ * read_data / gnc_mc (:84-246) set it up.  GNC g corrects the flow between
+ * the connected cells noden[g] and nodem[g] (1-based) with the heads of
+ * numjs contributing cells nodesj[g * numjs + i] (0: none) weighted by
+ * alphasj[g * numjs + i].  Every formulate adds gnc_fc's explicit rhs terms
+ * (:280-324, conductances read from the matrix after npf_fc as gnc_fmsav
+ * :251-273 does) and, under NEWTON, gnc_fn (:340-443); every budget adds
+ * gnc_cq's correction flows (:478-543).  nexg = 0 removes them.
+ * MF6HIP_EUNSUPPORTED for implicit != 0: the implicit form puts the
+ * contributing cells into the matrix rows of n and m (gnc_ac :134-161),
+ * outside a DIS grid's 7-point rows.  MF6HIP_EINVAL for cells that are not
+ * connected or out of range. */
+int mf6hip_gwf_set_gnc(void *handle, int nexg, int numjs, const int *noden,
+                       const int *nodem, const int *nodesj, const double *alphasj,
+                       int implicit);
+
 /* Heads in / out (natural order, host memory). */
 int mf6hip_gwf_set_x(void *handle, const double *x);
 int mf6hip_gwf_get_x(void *handle, double *x);

@@ -28,7 +28,7 @@ TERM = {"STO-SS": 0, "STO-SY": 1, "WEL": 2, "DRN": 3, "RCH": 4, "CHD": 5}
 EXPORTS = [
     "mf6hip_create", "mf6hip_solve", "mf6hip_destroy", "mf6hip_last_error",
     "mf6hip_gwf_create", "mf6hip_gwf_set_package", "mf6hip_gwf_set_package_ex",
-    "mf6hip_gwf_set_hfb",
+    "mf6hip_gwf_set_hfb", "mf6hip_gwf_set_gnc",
     "mf6hip_gwf_set_x",
     "mf6hip_gwf_get_x", "mf6hip_gwf_set_x_device", "mf6hip_gwf_get_x_device",
     "mf6hip_gwf_advance", "mf6hip_gwf_formulate",
@@ -169,6 +169,7 @@ def load():
     L.mf6hip_gwf_set_package.argtypes = [vp, i, i, ip, dp, dp]
     L.mf6hip_gwf_set_package_ex.argtypes = [vp, i, i, ip, dp, dp, C.POINTER(PackageOptionsC)]
     L.mf6hip_gwf_set_hfb.argtypes = [vp, i, ip, ip, dp]
+    L.mf6hip_gwf_set_gnc.argtypes = [vp, i, i, ip, ip, ip, dp, i]
     L.mf6hip_gwf_set_x.argtypes = [vp, dp]
     L.mf6hip_gwf_get_x.argtypes = [vp, dp]
     L.mf6hip_gwf_set_x_device.argtypes = [vp, vp]

@@ -405,8 +405,20 @@ struct BndDev {
   PkgDev pk[5];
 };
 
+// The model's explicit ghost-node corrections as the kernels see them
+// (the handle's d_gnc_* arrays).
+struct GncDev {
+  int n, numjs;
+  const int *off, *ent;     // per position: g * 2 + role (role 1: the row is m)
+  const int *np, *mp, *s;   // positions of n and m; direction index of m from n
+  const int *jp;            // numjs contributing positions per GNC (-1: none)
+  const double *a;          // their weights
+  double *cond;             // the conductance gnc_fmsav saves
+};
+
 // sln_buildsystem for one model: one thread per row
-__global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, int iss, double delt,
+__global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, GncDev G, int iss,
+                                                         double delt,
                                                          const double *__restrict__ x,
                                                          const double *__restrict__ xold,
                                                          double *__restrict__ A,
@@ -459,6 +471,28 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, in
       diag = diag + (-cond);
     }
     double b = 0.0;
+    // --- gnc_fc, explicit (GhostNode.f90:280-324), with gnc_fmsav
+    // (:251-273): the conductance is the matrix value at (n, m) after
+    // npf_fc; rhs(n) -= alpha cond (x(n) - x(j)), rhs(m) += the same
+    for (int e = G.off ? G.off[p] : 0, e1 = G.off ? G.off[p + 1] : 0; e < e1; e++) {
+      const int code = G.ent[e];
+      const int gi = code >> 1, role = code & 1;
+      const double cond = off[role == 0 ? G.s[gi] : 5 - G.s[gi]];
+      if (role == 0) G.cond[gi] = cond;
+      const int pn = G.np[gi], pm = G.mp[gi];
+      if (g.ibound[pn] == 0 || g.ibound[pm] == 0) continue;
+      const double xn = x[pn];
+      for (int jj = 0; jj < G.numjs; jj++) {
+        const int pj = G.jp[gi * G.numjs + jj];
+        if (pj < 0) continue;
+        const double alpha = G.a[gi * G.numjs + jj];
+        if (alpha == 0.0) continue;
+        const double aterm = alpha * cond;
+        const double rterm = aterm * (xn - x[pj]);
+        if (role == 0) b = b - rterm;
+        else b = b + rterm;
+      }
+    }
     // --- sto_fc
     if (g.has_sto && !iss && g.ibound[p] >= 1) {
       double tled = 1.0 / delt;
@@ -550,6 +584,51 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, in
           } else {                // row m (upstream)
             b = b - term * hb;
             diag = diag + (-term);
+          }
+        }
+      }
+      // --- gnc_fn (GhostNode.f90:340-443): the correction's derivative
+      // with respect to a convertible upstream cell's head, with the face's
+      // saturated conductance (horizontal faces, vertical ones with
+      // VARIABLECV)
+      for (int e = G.off ? G.off[p] : 0, e1 = G.off ? G.off[p + 1] : 0; e < e1; e++) {
+        const int code = G.ent[e];
+        const int gi = code >> 1, role = code & 1;
+        const int pn = G.np[gi], pm = G.mp[gi];
+        if (g.ibound[pn] == 0 || g.ibound[pm] == 0) continue;
+        const int sn = G.s[gi];
+        if ((sn == 0 || sn == 5) && g.ivarcv == 0) continue;
+        const double csat = sn < 3 ? g.condsat[dirs[sn] * N + pm] : g.condsat[dirs[sn] * N + pn];
+        const double xn = x[pn], xm = x[pm];
+        const int iups = xm > xn ? 1 : 0;
+        const int up = iups ? pm : pn;
+        if (g.icelltype[up] == 0) continue;
+        const double topup = g.top[up], botup = g.bot[up], xup = x[up];
+        for (int jj = 0; jj < G.numjs; jj++) {
+          const int pj = G.jp[gi * G.numjs + jj];
+          if (pj < 0) continue;
+          if (g.ibound[pj] == 0) continue;
+          const double alpha = G.a[gi * G.numjs + jj];
+          if (alpha == 0.0) continue;
+          const double consterm = csat * alpha * (xn - x[pj]);
+          const double derv = sqsat_deriv(topup, botup, xup, 1.0e-6);
+          const double term = consterm * derv;
+          if (iups == 0) {
+            if (role == 0) {
+              diag = diag + term;
+              b = b + term * xn;
+            } else {
+              if (g.ibound[pm] > 0) off[5 - sn] = off[5 - sn] - term;
+              b = b - term * xn;
+            }
+          } else {
+            if (role == 1) {
+              diag = diag - term;
+              b = b - term * xm;
+            } else {
+              if (g.ibound[pn] > 0) off[sn] = off[sn] + term;
+              b = b + term * xm;
+            }
           }
         }
       }
@@ -932,7 +1011,7 @@ struct BndRates {
 // (csr_diagsum) and, for a CHD cell, its CHD rate (calc_chd_rate: minus the
 // inflow from its neighbours).  strg receives the STO-SS / STO-SY rates.
 __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
-    GwfDev g, BndRates br, const int *__restrict__ ia, const int *__restrict__ chd_cell,
+    GwfDev g, BndRates br, GncDev G, const int *__restrict__ ia, const int *__restrict__ chd_cell,
     int iss, double delt, const double *__restrict__ x, const double *__restrict__ xold,
     double *__restrict__ flowja, double *__restrict__ strg) {
   const Layout &L = g.L;
@@ -977,8 +1056,34 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
                                           g.sat[m], g.condsat[dir * N + p]);
         q[s] = cond * (x[m] - hcell);
       }
-      flowja[ipos++] = q[s];
     }
+    // --- gnc_cq (GhostNode.f90:478-503, deltaQgnc :507-543): the
+    // correction flow with the conductance saved at the last formulate,
+    // added at (n, m) and taken from (m, n), in GNC order
+    for (int e = G.off ? G.off[p] : 0, e1 = G.off ? G.off[p + 1] : 0; e < e1; e++) {
+      const int code = G.ent[e];
+      const int gi = code >> 1, role = code & 1;
+      const int pn = G.np[gi], pm = G.mp[gi];
+      double dq = 0.0;
+      if (g.ibound[pn] != 0 && g.ibound[pm] != 0) {
+        double sigalj = 0.0, hd = 0.0;
+        for (int jj = 0; jj < G.numjs; jj++) {
+          const int pj = G.jp[gi * G.numjs + jj];
+          if (pj < 0) continue;
+          if (g.ibound[pj] == 0) continue;
+          const double alpha = G.a[gi * G.numjs + jj];
+          sigalj = sigalj + alpha;
+          hd = hd + alpha * x[pj];
+        }
+        const double aterm = sigalj * x[pn] - hd;
+        dq = aterm * G.cond[gi];
+      }
+      if (role == 0) q[G.s[gi]] = q[G.s[gi]] + dq;
+      else q[5 - G.s[gi]] = q[5 - G.s[gi]] - dq;
+    }
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+      if (pres[s]) flowja[ipos++] = q[s];
     double diag = 0.0;
     // --- sto_cq
     double rss = 0.0, rsy = 0.0;
@@ -1503,6 +1608,130 @@ extern "C" int mf6hip_gwf_advance(void *handle, int iss, double delt) {
   return MF6HIP_OK;
 }
 
+// the kernels' view of the model's ghost-node corrections (none: null
+// tables)
+static GncDev gnc_dev(const Mf6Handle *h) {
+  GncDev G;
+  G.n = h->gnc_n;
+  G.numjs = h->gnc_numjs;
+  const bool on = h->gnc_n > 0;
+  G.off = on ? h->d_gnc_off : nullptr;
+  G.ent = h->d_gnc_ent;
+  G.np = h->d_gnc_np;
+  G.mp = h->d_gnc_mp;
+  G.s = h->d_gnc_s;
+  G.jp = h->d_gnc_jp;
+  G.a = h->d_gnc_a;
+  G.cond = h->d_gnc_cond;
+  return G;
+}
+
+static void gnc_free(Mf6Handle *h) {
+  void *bufs[] = {h->d_gnc_np, h->d_gnc_mp, h->d_gnc_s, h->d_gnc_jp,
+                  h->d_gnc_a, h->d_gnc_cond, h->d_gnc_off, h->d_gnc_ent};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  h->d_gnc_np = h->d_gnc_mp = h->d_gnc_s = h->d_gnc_jp = nullptr;
+  h->d_gnc_off = h->d_gnc_ent = nullptr;
+  h->d_gnc_a = h->d_gnc_cond = nullptr;
+  h->gnc_n = h->gnc_numjs = 0;
+}
+
+// The model's GNC package (GhostNode.f90 gnc_df / read_data / gnc_mc): a new
+// set replaces the last one.  Explicit form only: the implicit form adds
+// the contributing cells to rows n and m of the matrix (gnc_ac :134-161),
+// which a DIS grid's 7-point rows do not hold.
+extern "C" int mf6hip_gwf_set_gnc(void *handle, int nexg, int numjs, const int *noden,
+                                  const int *nodem, const int *nodesj, const double *alphasj,
+                                  int implicit) {
+  Mf6Handle *h = (Mf6Handle *)handle;
+  if (!h || !h->has_model) { mf6hip_set_error("no model on this handle"); return MF6HIP_EINVAL; }
+  if (nexg < 0 || numjs < 0 || (nexg > 0 && (numjs < 1 || !noden || !nodem || !nodesj ||
+                                             !alphasj))) {
+    mf6hip_set_error("bad GNC arrays");
+    return MF6HIP_EINVAL;
+  }
+  if (implicit != 0 && nexg > 0) {
+    mf6hip_set_error("implicit GNC adds the contributing cells to the matrix rows of n and m "
+                     "(GhostNode.f90:134-161), outside a DIS grid's 7-point rows: use the "
+                     "EXPLICIT option");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  const Layout &L = h->L;
+  if ((long long)nexg * numjs >= (1LL << 30) || nexg >= (1 << 29)) {
+    mf6hip_set_error("too many GNC entries");
+    return MF6HIP_EUNSUPPORTED;
+  }
+  std::vector<int> np(nexg), mp(nexg), sd(nexg), jp((size_t)nexg * numjs);
+  std::vector<int> cnt(L.NP + 1, 0);
+  for (int g = 0; g < nexg; g++) {
+    const int n = noden[g] - 1, m = nodem[g] - 1;
+    if (n < 0 || n >= L.N || m < 0 || m >= L.N) {
+      mf6hip_set_error("GNC " + std::to_string(g + 1) + ": node out of range");
+      return MF6HIP_EINVAL;
+    }
+    const int kn = n / L.NRC, rn = n - kn * L.NRC, in = rn / L.ncol, jn = rn - in * L.ncol;
+    const int km = m / L.NRC, rm = m - km * L.NRC, im = rm / L.ncol, jm = rm - im * L.ncol;
+    int dir = -1;   // m seen from n: up, back, left, right, front, down
+    if (rn == rm && km == kn - 1) dir = 0;
+    else if (kn == km && jn == jm && im == in - 1) dir = 1;
+    else if (kn == km && in == im && jm == jn - 1) dir = 2;
+    else if (kn == km && in == im && jm == jn + 1) dir = 3;
+    else if (kn == km && jn == jm && im == in + 1) dir = 4;
+    else if (rn == rm && km == kn + 1) dir = 5;
+    if (dir < 0) {
+      mf6hip_set_error("GNC " + std::to_string(g + 1) + ": cell " + std::to_string(n + 1) +
+                       " is not connected to cell " + std::to_string(m + 1));
+      return MF6HIP_EINVAL;
+    }
+    np[g] = pos_of_natural(L, n);
+    mp[g] = pos_of_natural(L, m);
+    sd[g] = dir;
+    for (int q = 0; q < numjs; q++) {
+      const int jn1 = nodesj[(size_t)g * numjs + q];
+      if (jn1 < 0 || jn1 > L.N) {
+        mf6hip_set_error("GNC " + std::to_string(g + 1) + ": contributing node out of range");
+        return MF6HIP_EINVAL;
+      }
+      jp[(size_t)g * numjs + q] = jn1 == 0 ? -1 : pos_of_natural(L, jn1 - 1);
+    }
+    cnt[np[g] + 1]++;
+    cnt[mp[g] + 1]++;
+  }
+  for (int p = 0; p < L.NP; p++) cnt[p + 1] += cnt[p];
+  std::vector<int> ent(2 * (size_t)(nexg > 0 ? nexg : 1));
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int g = 0; g < nexg; g++) {
+      ent[fill[np[g]]++] = g * 2;
+      ent[fill[mp[g]]++] = g * 2 + 1;
+    }
+  }
+  CK(hipStreamSynchronize(h->stream) == hipSuccess ? MF6HIP_OK : MF6HIP_EHIP);
+  gnc_free(h);
+  if (nexg == 0) return MF6HIP_OK;
+  CK(galloc(&h->d_gnc_np, (size_t)nexg));
+  CK(galloc(&h->d_gnc_mp, (size_t)nexg));
+  CK(galloc(&h->d_gnc_s, (size_t)nexg));
+  CK(galloc(&h->d_gnc_jp, (size_t)nexg * numjs));
+  CK(galloc(&h->d_gnc_a, (size_t)nexg * numjs));
+  CK(galloc(&h->d_gnc_cond, (size_t)nexg));
+  CK(galloc(&h->d_gnc_off, (size_t)L.NP + 1));
+  CK(galloc(&h->d_gnc_ent, 2 * (size_t)nexg));
+  HIPCHK(hipMemcpy(h->d_gnc_np, np.data(), sizeof(int) * nexg, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_gnc_mp, mp.data(), sizeof(int) * nexg, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_gnc_s, sd.data(), sizeof(int) * nexg, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_gnc_jp, jp.data(), sizeof(int) * jp.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_gnc_a, alphasj, sizeof(double) * (size_t)nexg * numjs,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(h->d_gnc_cond, 0, sizeof(double) * nexg));
+  HIPCHK(hipMemcpy(h->d_gnc_off, cnt.data(), sizeof(int) * (L.NP + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_gnc_ent, ent.data(), sizeof(int) * 2 * nexg, hipMemcpyHostToDevice));
+  h->gnc_n = nexg;
+  h->gnc_numjs = numjs;
+  return MF6HIP_OK;
+}
+
 // the kernels' view of package t (drains scale cubically under NEWTON or
 // with DEV_CUBIC_SCALING, gwf-drn.f90:130-135)
 static PkgDev pkg_dev(const Mf6Handle *h, int t) {
@@ -1546,8 +1775,8 @@ extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double
   }
   GwfDev g = make_dev(h);
   g.fn = h->m.inewton && inewton;
-  k_formulate<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, bd, h->iss, h->delt, h->x, h->xold, h->A,
-                                               h->b);
+  k_formulate<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, bd, gnc_dev(h), h->iss, h->delt, h->x,
+                                               h->xold, h->A, h->b);
   HIPCHK(hipGetLastError());
   if (amat_out || rhs_out) CK(mf6hip_copy_system(h, amat_out, rhs_out));
   return MF6HIP_OK;
@@ -1824,8 +2053,9 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
   if (chd.nbound > 0)
     k_mark_cells<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, L, h->chd_flag);
   GwfDev g = make_dev(h);
-  k_budget_cells<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, br, h->d_ia, h->chd_flag, h->iss, h->delt,
-                                                  h->x, h->xold, h->flowja, h->strg);
+  k_budget_cells<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, br, gnc_dev(h), h->d_ia, h->chd_flag,
+                                                  h->iss, h->delt, h->x, h->xold, h->flowja,
+                                                  h->strg);
   if (chd.nbound > 0)
     k_chd_inout<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, h->d_ia, L,
                                                        h->ibound, h->flowja, chd_in, chd_out,

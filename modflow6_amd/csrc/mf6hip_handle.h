@@ -106,6 +106,15 @@ struct Mf6Handle {
   int hfb_n = 0;
   int *d_hfb_face = nullptr;
   double *d_hfb_sav = nullptr;
+  // ghost-node correction (explicit GNC of the model): per GNC the positions
+  // of n and m, the direction index of m seen from n (0..5: up, back, left,
+  // right, front, down), numjs contributing positions (-1: none) and
+  // weights, the conductance gnc_fmsav saved; per position the GNCs that
+  // touch its row (g * 2 + role, role 1: the row is m), in GNC order
+  int gnc_n = 0, gnc_numjs = 0;
+  int *d_gnc_np = nullptr, *d_gnc_mp = nullptr, *d_gnc_s = nullptr, *d_gnc_jp = nullptr;
+  double *d_gnc_a = nullptr, *d_gnc_cond = nullptr;
+  int *d_gnc_off = nullptr, *d_gnc_ent = nullptr;
   std::vector<int> h_icelltype;   // natural order, host copy for the checks
   double *sat = nullptr;      // npf sat of the last formulate (per position)
   // budget (gwf_cq + gwf_bd) work arrays, allocated on first use

@@ -1784,7 +1784,8 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
                   h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax, h->sat,
                   h->flowja, h->strg, h->bsim, h->chd_flag, h->dhat32, h->qhat32, h->t32,
-                  h->d_hfb_face, h->d_hfb_sav};
+                  h->d_hfb_face, h->d_hfb_sav, h->d_gnc_np, h->d_gnc_mp, h->d_gnc_s,
+                  h->d_gnc_jp, h->d_gnc_a, h->d_gnc_cond, h->d_gnc_off, h->d_gnc_ent};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);

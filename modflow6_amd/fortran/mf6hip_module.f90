@@ -149,7 +149,8 @@ module mf6hip_module
   end type mf6hip_xmi_iteration
 
   public :: mf6hip_create, mf6hip_solve, mf6hip_destroy, mf6hip_last_error
-  public :: mf6hip_gwf_set_package_ex, mf6hip_gwf_set_hfb, mf6hip_xmi_set_nonlinear, &
+  public :: mf6hip_gwf_set_package_ex, mf6hip_gwf_set_hfb, mf6hip_gwf_set_gnc, &
+            mf6hip_xmi_set_nonlinear, &
             mf6hip_xmi_prepare_solve, mf6hip_xmi_solve, mf6hip_xmi_last_iteration, &
             mf6hip_xmi_finalize_solve
   public :: mf6hip_gwf_create, mf6hip_gwf_set_package, mf6hip_gwf_set_x, &
@@ -345,6 +346,17 @@ module mf6hip_module
       integer(c_int), value :: nhfb
       integer(c_int), intent(in) :: noden(*), nodem(*)
       real(c_double), intent(in) :: hydchr(*)
+      integer(c_int) :: rc
+    end function
+
+    function mf6hip_gwf_set_gnc(handle, nexg, numjs, noden, nodem, nodesj, &
+                                alphasj, implicit) result(rc) &
+      bind(C, name='mf6hip_gwf_set_gnc')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: nexg, numjs, implicit
+      integer(c_int), intent(in) :: noden(*), nodem(*), nodesj(*)
+      real(c_double), intent(in) :: alphasj(*)
       integer(c_int) :: rc
     end function
 

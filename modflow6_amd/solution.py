@@ -125,6 +125,8 @@ class GwfSolution:
                        f"mf6hip_gwf_set_package_ex({p.kind})")
         if c.hfb is not None:
             self.set_hfb(*c.hfb)
+        if c.gnc is not None:
+            self.set_gnc(*c.gnc)
         x0 = np.ascontiguousarray(c.strt, dtype=np.float64)
         _lib.check(L.mf6hip_gwf_set_x(self.h, _lib.dptr(x0)), "mf6hip_gwf_set_x")
         if timing:
@@ -380,6 +382,26 @@ class GwfSolution:
                                             _lib.dptr(rhs)),
                    "mf6hip_get_system")
         return amat, rhs
+
+    def set_gnc(self, noden, nodem, nodesj, alphasj, implicit=False):
+        """The model's GNC package (explicit form), 0-based cells and
+        contributing cells (-1: none), (nexg, numjs) weights; replaces the
+        last set (mf6hip_gwf_set_gnc)."""
+        n = np.ascontiguousarray(np.asarray(noden) + 1, dtype=np.int32)
+        m = np.ascontiguousarray(np.asarray(nodem) + 1, dtype=np.int32)
+        if n.size == 0:
+            _lib.check(self.L.mf6hip_gwf_set_gnc(self.h, 0, 0, None, None, None, None,
+                                                 1 if implicit else 0),
+                       "mf6hip_gwf_set_gnc")
+            return
+        js = np.asarray(nodesj).reshape(n.size, -1)
+        numjs = js.shape[1]
+        j1 = np.ascontiguousarray(js + 1, dtype=np.int32)
+        al = np.ascontiguousarray(np.asarray(alphasj, dtype=np.float64).reshape(js.shape))
+        _lib.check(self.L.mf6hip_gwf_set_gnc(self.h, n.size, numjs, _lib.iptr(n), _lib.iptr(m),
+                                             _lib.iptr(j1), _lib.dptr(al),
+                                             1 if implicit else 0),
+                   "mf6hip_gwf_set_gnc")
 
     def set_hfb(self, noden, nodem, hydchr):
         """A new set of horizontal flow barriers (0-based node numbers of
