@@ -153,7 +153,9 @@ typedef struct mf6hip_gwf_model {
   int inewtonur;            /* GWF NEWTON UNDER_RELAXATION */
   int ivarcv;               /* NPF VARIABLECV */
   int idewatcv;             /* NPF VARIABLECV DEWATERED */
-  int icellavg;             /* NPF ALTERNATIVE_CELL_AVERAGING (0 = harmonic) */
+  int icellavg;             /* NPF ALTERNATIVE_CELL_AVERAGING: 0 harmonic,
+                               1 LOGARITHMIC, 2 AMT-LMK, 3 AMT-HMK
+                               (gwf-npf.f90 condmean cases) */
   int has_sto;              /* STO package present */
   const int *iconvert;      /* nodes (STO ICONVERT), may be NULL without STO */
   const double *ss;         /* nodes (STO SS) */

@@ -136,6 +136,12 @@ class GwfCase:
     # cells (-1: none) with their weights, shapes (nexg,), (nexg,),
     # (nexg, numjs), (nexg, numjs) (GhostNode.f90 read_data)
     gnc: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]] = None
+    # NPF options: VARIABLECV (ivarcv), with DEWATERED (idewatcv), and
+    # ALTERNATIVE_CELL_AVERAGING (icellavg: 0 harmonic, 1 LOGARITHMIC,
+    # 2 AMT-LMK, 3 AMT-HMK)
+    ivarcv: int = 0
+    idewatcv: int = 0
+    icellavg: int = 0
 
     @property
     def nodes(self):

@@ -1261,9 +1261,10 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
     mf6hip_set_error("STO arrays missing");
     return MF6HIP_EINVAL;
   }
-  if (m->ivarcv) {
-    mf6hip_set_error("VARIABLECV is not supported on the device path");
-    return MF6HIP_EUNSUPPORTED;
+  if (m->icellavg < 0 || m->icellavg > 3) {
+    mf6hip_set_error("icellavg must be 0 (harmonic), 1 (LOGARITHMIC), 2 (AMT-LMK) or 3 "
+                     "(AMT-HMK)");
+    return MF6HIP_EINVAL;
   }
   h->m = *m;
   h->h_icelltype.assign(m->icelltype, m->icelltype + L.N);

@@ -91,9 +91,9 @@ class GwfSolution:
         m.ibound = i(np.ones(c.nodes, dtype=np.int32))
         m.inewton = 1 if c.newton else 0
         m.inewtonur = 1 if c.newton_ur else 0
-        m.ivarcv = 0
-        m.idewatcv = 0
-        m.icellavg = 0
+        m.ivarcv = int(c.ivarcv)
+        m.idewatcv = int(c.idewatcv)
+        m.icellavg = int(c.icellavg)
         m.has_sto = 1 if c.has_sto else 0
         if c.has_sto:
             m.iconvert, m.ss, m.sy = i(c.iconvert), d(c.ss), d(c.sy)

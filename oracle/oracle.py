@@ -558,10 +558,14 @@ class GwfSimulation:
         # NPF and STO both take 1e-6 under NEWTON, else 0 (gwf-npf.f90:1394,
         # gwf-sto.f90:826-829)
         self.satomega = 1e-6 if c.newton else 0.0
+        # NPF options: VARIABLECV (DEWATERED), ALTERNATIVE_CELL_AVERAGING
+        self.ivarcv = int(c.ivarcv)
+        self.idewatcv = int(c.idewatcv)
+        self.icellavg = int(c.icellavg)
         self.condsat = np.zeros(d.hwva.size)
         lib().orc_calc_condsat(self.neq, d.ia, d.ja, d.jas, d.ihc, d.cl1,
                                d.cl2, d.hwva, d.top, d.bot, self.k11,
-                               self.k33, 0, self.condsat)
+                               self.k33, self.icellavg, self.condsat)
         self.hfb = None
         if c.hfb is not None:
             self.set_hfb(*c.hfb)
@@ -718,7 +722,8 @@ class GwfSimulation:
         # gwf_fc
         L.orc_npf_fc(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
                      d.hwva, self.condsat, self.ibound, self.icelltype,
-                     self.inewton, 0, 1, 0, self.k11, self.k33, d.top, d.bot,
+                     self.inewton, self.ivarcv, self.idewatcv, self.icellavg,
+                     self.k11, self.k33, d.top, d.bot,
                      self.sat, self.x, self.amat)
         if self.gnc is not None:
             gn, gm, gj, ga, gp, gc = self.gnc
@@ -737,12 +742,12 @@ class GwfSimulation:
                          self.amat, self.rhs)
         if self.inewton and inewton:
             L.orc_npf_fn(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc,
-                         self.condsat, self.ibound, self.icelltype, 0, d.top,
+                         self.condsat, self.ibound, self.icelltype, self.ivarcv, d.top,
                          d.bot, self.satomega, self.x, self.amat, self.rhs)
             if self.gnc is not None:
                 gn, gm, gj, ga, gp, gc = self.gnc
                 L.orc_gnc_fn(gn.size, gj.shape[1], gn, gm, gj, ga, d.ia, gp, d.isym, d.jas,
-                             d.ihc, self.condsat, 0, self.icelltype, d.top, d.bot,
+                             d.ihc, self.condsat, self.ivarcv, self.icelltype, d.top, d.bot,
                              self.ibound, self.x, self.amat, self.rhs)
             if self.case.has_sto and not self.iss:
                 L.orc_sto_fn(self.neq, d.ia, self.ibound, self.iconvert,
@@ -799,7 +804,8 @@ class GwfSimulation:
         flowja = np.zeros(d.nja)
         L.orc_npf_cq(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
                      d.hwva, self.condsat, self.ibound, self.icelltype,
-                     self.inewton, 0, 1, 0, self.k11, self.k33, d.top, d.bot,
+                     self.inewton, self.ivarcv, self.idewatcv, self.icellavg,
+                     self.k11, self.k33, d.top, d.bot,
                      self.sat, self.x, flowja)
         if self.gnc is not None:
             gn, gm, gj, ga, gp, gc = self.gnc
