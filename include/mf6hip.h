@@ -160,6 +160,9 @@ typedef struct mf6hip_gwf_model {
   const int *iconvert;      /* nodes (STO ICONVERT), may be NULL without STO */
   const double *ss;         /* nodes (STO SS) */
   const double *sy;         /* nodes (STO SY) */
+  const double *k22;        /* nodes: NPF K22, the conductivity of row-direction
+                               faces (hy_eff without angles, gwf-npf.f90:
+                               2280-2355); NULL: K11 */
 } mf6hip_gwf_model;
 
 /* package types, in the bndlist order MF6 fixes by type (gwf.f90:119-124,

@@ -91,7 +91,8 @@ class GwfModelC(C.Structure):
                 ("icellavg", C.c_int), ("has_sto", C.c_int),
                 ("iconvert", C.POINTER(C.c_int)),
                 ("ss", C.POINTER(C.c_double)),
-                ("sy", C.POINTER(C.c_double))]
+                ("sy", C.POINTER(C.c_double)),
+                ("k22", C.POINTER(C.c_double))]
 
 
 class PtcC(C.Structure):

@@ -142,6 +142,9 @@ class GwfCase:
     ivarcv: int = 0
     idewatcv: int = 0
     icellavg: int = 0
+    # NPF K22 (nodes,): the conductivity of row-direction (i) faces; None:
+    # K11
+    k22: Optional[np.ndarray] = None
 
     @property
     def nodes(self):

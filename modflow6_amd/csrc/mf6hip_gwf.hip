@@ -178,7 +178,7 @@ __device__ inline double vcond(int ibdn, int ibdm, int ictn, int ictm, int ivarc
 
 struct GwfDev {
   Layout L;
-  const double *top, *bot, *area, *k11, *k33, *ss, *sy, *delr, *delc, *condsat;
+  const double *top, *bot, *area, *k11, *k22, *k33, *ss, *sy, *delr, *delc, *condsat;
   double *sat;   // npf_cf's sat, written by the formulate (read by the budget)
   const int *icelltype, *iconvert, *ibound;
   int inewton, ivarcv, idewatcv, icellavg, has_sto;
@@ -219,6 +219,13 @@ __device__ inline Conn conn_geom(const GwfDev &g, int dir, int pa, int pb, int i
   return c;
 }
 
+// horizontal conductivity of a face direction: hy_eff (gwf-npf.f90:
+// 2280-2355) without rotation angles is K22 on row-direction faces (dir 1)
+// and K11 on column-direction ones
+__device__ inline double hk(const GwfDev &g, int dir, int p) {
+  return dir == 1 ? g.k22[p] : g.k11[p];
+}
+
 // npf_fc conductance of the pair (a < b) as the reference computes it
 __device__ inline double pair_cond(const GwfDev &g, int dir, int pa, int pb, int ia_,
                                    int ja_, double ha, double hb, double csat) {
@@ -230,7 +237,7 @@ __device__ inline double pair_cond(const GwfDev &g, int dir, int pa, int pb, int
                  g.top[pb], g.bot[pa], g.bot[pb], c.hwva);
   }
   return hcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.inewton,
-               g.icellavg, csat, ha, hb, sata, satb, g.k11[pa], g.k11[pb], g.top[pa],
+               g.icellavg, csat, ha, hb, sata, satb, hk(g, dir, pa), hk(g, dir, pb), g.top[pa],
                g.top[pb], g.bot[pa], g.bot[pb], c.cl1, c.cl2, c.hwva);
 }
 
@@ -246,7 +253,7 @@ __device__ inline double pair_cond_sat(const GwfDev &g, int dir, int pa, int pb,
                  g.top[pb], g.bot[pa], g.bot[pb], c.hwva);
   }
   return hcond(g.ibound[pa], g.ibound[pb], g.icelltype[pa], g.icelltype[pb], g.inewton,
-               g.icellavg, csat, ha, hb, sata, satb, g.k11[pa], g.k11[pb], g.top[pa],
+               g.icellavg, csat, ha, hb, sata, satb, hk(g, dir, pa), hk(g, dir, pb), g.top[pa],
                g.top[pb], g.bot[pa], g.bot[pb], c.cl1, c.cl2, c.hwva);
 }
 
@@ -268,8 +275,8 @@ __global__ void k_condsat(GwfDev g, double *__restrict__ condsat) {
           cs = vcond(1, 1, 1, 1, 1, 1, 1.0, botn, botm, g.k33[p], g.k33[pb], 1.0, 1.0, topn,
                      topm, botn, botm, cn.hwva);
         } else {
-          cs = hcond(1, 1, 1, 1, 0, g.icellavg, 1.0, topn, topm, 1.0, 1.0, g.k11[p],
-                     g.k11[pb], topn, topm, botn, botm, cn.cl1, cn.cl2, cn.hwva);
+          cs = hcond(1, 1, 1, 1, 0, g.icellavg, 1.0, topn, topm, 1.0, 1.0, hk(g, dir, p),
+                     hk(g, dir, pb), topn, topm, botn, botm, cn.cl1, cn.cl2, cn.hwva);
         }
       }
       condsat[dir * L.NP + p] = cs;
@@ -1223,7 +1230,8 @@ __global__ void k_rate_final(const double *part, int nb, double *out) {
 static GwfDev make_dev(Mf6Handle *h) {
   GwfDev g;
   g.L = h->L;
-  g.top = h->top; g.bot = h->bot; g.area = h->area; g.k11 = h->k11; g.k33 = h->k33;
+  g.top = h->top; g.bot = h->bot; g.area = h->area; g.k11 = h->k11; g.k22 = h->k22;
+  g.k33 = h->k33;
   g.ss = h->ss; g.sy = h->sy; g.delr = h->delr; g.delc = h->delc; g.condsat = h->condsat;
   g.sat = h->sat;
   g.icelltype = h->icelltype; g.iconvert = h->iconvert; g.ibound = h->ibound;
@@ -1271,6 +1279,7 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   CK(upload_pos(h, &h->top, m->top));
   CK(upload_pos(h, &h->bot, m->bot));
   CK(upload_pos(h, &h->k11, m->k11));
+  CK(upload_pos(h, &h->k22, m->k22 ? m->k22 : m->k11));   // K22: K11 when not given
   CK(upload_pos(h, &h->k33, m->k33));
   CK(upload_pos(h, &h->icelltype, m->icelltype));
   CK(upload_pos(h, &h->ibound0, m->ibound));

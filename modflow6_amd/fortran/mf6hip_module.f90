@@ -76,6 +76,7 @@ module mf6hip_module
     type(c_ptr) :: iconvert
     type(c_ptr) :: ss
     type(c_ptr) :: sy
+    type(c_ptr) :: k22
   end type mf6hip_gwf_model
 
   !> mf6hip_ptc: pseudo-transient continuation control and state

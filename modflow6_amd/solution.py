@@ -87,6 +87,8 @@ class GwfSolution:
         m.delr, m.delc = d(c.delr), d(c.delc)
         m.top, m.bot = d(top), d(bot)
         m.k11, m.k33 = d(c.k11), d(c.k33)
+        if c.k22 is not None:
+            m.k22 = d(c.k22)
         m.icelltype = i(c.icelltype)
         m.ibound = i(np.ones(c.nodes, dtype=np.int32))
         m.inewton = 1 if c.newton else 0

@@ -737,8 +737,8 @@ void orc_hfb_condsat_reset(int nhfb, const int *idxloc, const int *jas,
 void orc_calc_condsat(int nodes, const int *ia, const int *ja, const int *jas,
                       const int *ihc, const double *cl1, const double *cl2,
                       const double *hwva, const double *top, const double *bot,
-                      const double *k11, const double *k33, int icellavg,
-                      double *condsat)
+                      const double *k11, const double *k22, int ncol,
+                      const double *k33, int icellavg, double *condsat)
 {
   for (int node = 0; node < nodes; node++) {
     for (int ii = ia[node] + 1; ii < ia[node + 1]; ii++) {
@@ -749,8 +749,8 @@ void orc_calc_condsat(int nodes, const int *ia, const int *ja, const int *jas,
       double satn = DONE, satm = DONE;
       double topn = top[n], botn = bot[n], topm = top[m], botm = bot[m];
       int ihcj = ihc[jj];
-      double hyn = ihcj == 0 ? k33[n] : k11[n];
-      double hym = ihcj == 0 ? k33[m] : k11[m];
+      double hyn = ihcj == 0 ? k33[n] : (abs(m - n) == ncol ? k22[n] : k11[n]);
+      double hym = ihcj == 0 ? k33[m] : (abs(m - n) == ncol ? k22[m] : k11[m]);
       double hn = topn, hm = topm;
       double csat;
       if (ihcj == 0) {
@@ -793,7 +793,8 @@ void orc_npf_fc(int nodes, const int *ia, const int *ja, const int *jas,
                 const double *cl2, const double *hwva, const double *condsat,
                 const int *ibound, const int *icelltype, int inewton,
                 int ivarcv, int idewatcv, int icellavg, const double *k11,
-                const double *k33, const double *top, const double *bot,
+                const double *k22, int ncol, const double *k33,
+                const double *top, const double *bot,
                 const double *sat, const double *hnew, double *amat)
 {
   for (int n = 0; n < nodes; n++) {
@@ -802,8 +803,8 @@ void orc_npf_fc(int nodes, const int *ia, const int *ja, const int *jas,
       if (m < n) continue;
       int jj = jas[ii];
       int ihcj = ihc[jj];
-      double hyn = ihcj == 0 ? k33[n] : k11[n];
-      double hym = ihcj == 0 ? k33[m] : k11[m];
+      double hyn = ihcj == 0 ? k33[n] : (abs(m - n) == ncol ? k22[n] : k11[n]);
+      double hym = ihcj == 0 ? k33[m] : (abs(m - n) == ncol ? k22[m] : k11[m]);
       double cond;
       if (ihcj == 0) {
         cond = orc_vcond(ibound[n], ibound[m], icelltype[n], icelltype[m],
@@ -1084,7 +1085,8 @@ void orc_npf_cq(int nodes, const int *ia, const int *ja, const int *jas,
                 const double *cl2, const double *hwva, const double *condsat,
                 const int *ibound, const int *icelltype, int inewton,
                 int ivarcv, int idewatcv, int icellavg, const double *k11,
-                const double *k33, const double *top, const double *bot,
+                const double *k22, int ncol, const double *k33,
+                const double *top, const double *bot,
                 const double *sat, const double *hnew, double *flowja)
 {
   for (int n = 0; n < nodes; n++) {
@@ -1093,8 +1095,8 @@ void orc_npf_cq(int nodes, const int *ia, const int *ja, const int *jas,
       if (m < n) continue;
       int jj = jas[ii];
       int ihcj = ihc[jj];
-      double hyn = ihcj == 0 ? k33[n] : k11[n];
-      double hym = ihcj == 0 ? k33[m] : k11[m];
+      double hyn = ihcj == 0 ? k33[n] : (abs(m - n) == ncol ? k22[n] : k11[n]);
+      double hym = ihcj == 0 ? k33[m] : (abs(m - n) == ncol ? k22[m] : k11[m]);
       double cond;
       if (ihcj == 0) {
         cond = orc_vcond(ibound[n], ibound[m], icelltype[n], icelltype[m],

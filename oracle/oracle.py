@@ -93,14 +93,14 @@ def lib():
                                   C.POINTER(d), C.POINTER(d)]
         L.orc_syterms.argtypes = [d] * 6 + [C.POINTER(d), C.POINTER(d)]
         L.orc_calc_condsat.argtypes = [i, i32p, i32p, i32p, i32p, f64p, f64p,
-                                       f64p, f64p, f64p, f64p, f64p, i, f64p]
+                                       f64p, f64p, f64p, f64p, f64p, i, f64p, i, f64p]
         L.orc_hfb_condsat_modify.argtypes = [i, i32p, i32p, i32p, f64p, i, i32p, i32p,
                                              i32p, f64p, f64p, f64p, f64p, f64p]
         L.orc_hfb_condsat_reset.argtypes = [i, i32p, i32p, f64p, f64p]
         L.orc_npf_cf.argtypes = [i, i, d, i32p, i32p, f64p, f64p, f64p, f64p]
         L.orc_npf_fc.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
                                  f64p, f64p, i32p, i32p, i, i, i, i, f64p,
-                                 f64p, f64p, f64p, f64p, f64p, f64p]
+                                 f64p, i, f64p, f64p, f64p, f64p, f64p, f64p]
         L.orc_npf_fn.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, i32p,
                                  i32p, i, f64p, f64p, d, f64p, f64p, f64p]
         L.orc_sto_fc.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
@@ -110,7 +110,7 @@ def lib():
         L.orc_bnd_fc.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
         L.orc_npf_cq.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
                                  f64p, f64p, i32p, i32p, i, i, i, i, f64p,
-                                 f64p, f64p, f64p, f64p, f64p, f64p]
+                                 f64p, i, f64p, f64p, f64p, f64p, f64p, f64p]
         L.orc_sto_cq.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
                                  f64p, d, d, f64p, f64p, f64p, f64p, f64p]
         L.orc_bnd_cq.argtypes = [i, i32p, f64p, f64p, i32p, f64p, i32p, f64p,
@@ -551,6 +551,10 @@ class GwfSimulation:
         d = self.dis
         self.neq = d.nodes
         self.k11 = np.ascontiguousarray(c.k11, dtype=np.float64)
+        # K22 (row-direction faces, hy_eff gwf-npf.f90:2280-2355 without
+        # angles): K11 when not given
+        self.k22 = np.ascontiguousarray(c.k22 if c.k22 is not None else c.k11,
+                                        dtype=np.float64)
         self.k33 = np.ascontiguousarray(c.k33, dtype=np.float64)
         self.icelltype = np.ascontiguousarray(c.icelltype, dtype=np.int32)
         self.ibound = np.ones(self.neq, dtype=np.int32)
@@ -565,7 +569,7 @@ class GwfSimulation:
         self.condsat = np.zeros(d.hwva.size)
         lib().orc_calc_condsat(self.neq, d.ia, d.ja, d.jas, d.ihc, d.cl1,
                                d.cl2, d.hwva, d.top, d.bot, self.k11,
-                               self.k33, self.icellavg, self.condsat)
+                               self.k22, d.ncol, self.k33, self.icellavg, self.condsat)
         self.hfb = None
         if c.hfb is not None:
             self.set_hfb(*c.hfb)
@@ -723,7 +727,7 @@ class GwfSimulation:
         L.orc_npf_fc(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
                      d.hwva, self.condsat, self.ibound, self.icelltype,
                      self.inewton, self.ivarcv, self.idewatcv, self.icellavg,
-                     self.k11, self.k33, d.top, d.bot,
+                     self.k11, self.k22, d.ncol, self.k33, d.top, d.bot,
                      self.sat, self.x, self.amat)
         if self.gnc is not None:
             gn, gm, gj, ga, gp, gc = self.gnc
@@ -805,7 +809,7 @@ class GwfSimulation:
         L.orc_npf_cq(self.neq, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2,
                      d.hwva, self.condsat, self.ibound, self.icelltype,
                      self.inewton, self.ivarcv, self.idewatcv, self.icellavg,
-                     self.k11, self.k33, d.top, d.bot,
+                     self.k11, self.k22, d.ncol, self.k33, d.top, d.bot,
                      self.sat, self.x, flowja)
         if self.gnc is not None:
             gn, gm, gj, ga, gp, gc = self.gnc

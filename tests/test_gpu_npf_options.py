@@ -1,9 +1,10 @@
 """NPF options on the device against the oracle's NPF (pinned to the
 reference's npf_fc / npf_fn / npf_cq with VARIABLECV, DEWATERED and the
 alternative cell averagings in tests/test_oracle_gwf_vs_reference.py and
-tests/test_oracle_vs_reference.py): VARIABLECV with and without DEWATERED,
-under NEWTON and without, and ALTERNATIVE_CELL_AVERAGING LOGARITHMIC /
-AMT-LMK / AMT-HMK: the formulated system and the budget's face flows bit for
+tests/test_oracle_vs_reference.py; K22 in tests/test_oracle_npf_k22.py):
+VARIABLECV with and without DEWATERED, under NEWTON and without,
+ALTERNATIVE_CELL_AVERAGING LOGARITHMIC / AMT-LMK / AMT-HMK, and K22 on the
+row-direction faces: the formulated system and the budget's face flows bit for
 bit (to 1e-14 relative where a logarithmic mean calls log()), and the heads
 of solved steps within 10 x dvclose."""
 import dataclasses
@@ -39,6 +40,12 @@ def _averaging(icellavg):
 # 226-305)
 LOG_MEANS = {"logarithmic", "amt_lmk"}
 
+def _k22(base, seed=8):
+    c = base()
+    rng = np.random.default_rng(seed)
+    return dataclasses.replace(c, k22=np.asarray(c.k11) * rng.lognormal(-1.0, 0.7, c.nodes))
+
+
 CASES = {
     "varcv_newton": lambda: _convertible(True, 1, 0),
     "varcv_dewatered_newton": lambda: _convertible(True, 1, 1),
@@ -47,6 +54,8 @@ CASES = {
     "logarithmic": lambda: _averaging(1),
     "amt_lmk": lambda: _averaging(2),
     "amt_hmk": lambda: _averaging(3),
+    "k22_confined": lambda: _k22(lambda: _averaging(0)),
+    "k22_newton": lambda: _k22(lambda: _convertible(True, 0, 0)),
 }
 
 
@@ -66,7 +75,8 @@ def test_formulate_and_budget_bit_exact(name):
         db = gs.budget()
     finally:
         gs.destroy()
-    base = O.GwfSimulation(dataclasses.replace(case, ivarcv=0, idewatcv=0, icellavg=0))
+    base = O.GwfSimulation(dataclasses.replace(case, ivarcv=0, idewatcv=0, icellavg=0,
+                                               k22=None))
     base.kper, base.kstp = 1, 1
     base.advance(perlen / nstp, steady)
     base.formulate(1)
@@ -87,7 +97,7 @@ def test_formulate_and_budget_bit_exact(name):
     assert np.array_equal(db["flowja"].view(np.int64), ob["flowja"].view(np.int64))
 
 
-@pytest.mark.parametrize("name", ["varcv_dewatered_newton", "logarithmic"])
+@pytest.mark.parametrize("name", ["varcv_dewatered_newton", "logarithmic", "k22_newton"])
 def test_step_heads(name):
     case = CASES[name]()
     sim = O.GwfSimulation(case)

@@ -82,7 +82,8 @@ class Grid:
         self.k33 = np.ascontiguousarray(0.1 * s.k11)
         self.condsat = np.zeros(d.hwva.size)
         L.orc_calc_condsat(n, d.ia, d.ja, d.jas, d.ihc, d.cl1, d.cl2, d.hwva,
-                           d.top, d.bot, self.k11, self.k33, 0, self.condsat)
+                           d.top, d.bot, self.k11, self.k11, d.ncol, self.k33, 0,
+                           self.condsat)
         self.sat = np.ascontiguousarray(rng.uniform(0.0, 1.0, n))
         self.amat0 = rng.standard_normal(d.nja)
         self.rhs0 = rng.standard_normal(n)
@@ -126,7 +127,7 @@ def test_npf_fc(grid, inewton, ivarcv, idewatcv):
     r2 = g.rhs0.copy()
     L.orc_npf_fc(g.n, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2, d.hwva,
                  g.condsat, g.ibound, g.icelltype, inewton, ivarcv, idewatcv, 0,
-                 g.k11, g.k33, d.top, d.bot, sat, g.hnew, a1)
+                 g.k11, g.k11, d.ncol, g.k33, d.top, d.bot, sat, g.hnew, a1)
     R.ref_npf_fill(0, g.n, d.nja, 1, inewton, ivarcv, idewatcv, 0, 1e-6,
                    g.icelltype, g.ibound, g.k11, g.k33, g.condsat, sat, g.hnew,
                    a2, r2)
@@ -358,7 +359,7 @@ def test_npf_cq(grid, inewton):
     f1, f2 = g.amat0.copy(), g.amat0.copy()
     L.orc_npf_cq(g.n, d.ia, d.ja, d.jas, d.isym, d.ihc, d.cl1, d.cl2, d.hwva,
                  g.condsat, g.ibound, g.icelltype, inewton, 0, 1, 0, g.k11,
-                 g.k33, d.top, d.bot, sat, h, f1)
+                 g.k11, d.ncol, g.k33, d.top, d.bot, sat, h, f1)
     R.ref_npf_cq(g.n, d.nja, inewton, 0, 1, 0, 1e-6, g.icelltype, g.ibound,
                  g.k11, g.k33, g.condsat, sat, h, f2)
     assert eqbits(f1, f2)
