@@ -28,6 +28,7 @@
 // FP64 heads and the FP32 residual is replaced by the FP64 true residual
 // b - A x, keeping the search direction (residual replacement, so the
 // Krylov space is not restarted).
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 #include <cmath>
@@ -1194,22 +1195,38 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_f_spmvw(Layout L, const TV *__res
 #define MF6_SPMV_CH 2
 #endif
 
+// A launch of MF6_BLOCK threads; with e0 / e1 the events are recorded at the
+// kernel's own start and end (hipExtLaunchKernelGGL), so their interval is
+// the kernel's execution, as a profiler's kernel trace measures it.
+template <typename... KArgs, typename... Args>
+static void launch_timed(void (*kern)(KArgs...), int grid, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1, Args... args) {
+  if (e0)
+    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(MF6_BLOCK), 0, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(MF6_BLOCK), 0, s, args...);
+}
+
 template <typename TV, int W, bool CS>
 static void launch_f_spmvw(const Layout &L, int nb, hipStream_t s, const TV *A, const TV *p,
                            TV *q, double *part, const KState *st, const unsigned *cmask,
-                           double *colsq) {
+                           double *colsq, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   int g = (2 * (L.HP / W) + MF6_BLOCK - 1) / MF6_BLOCK;
   if (g > nb) g = nb;
   if (g < 1) g = 1;
   constexpr int CH = MF6_SPMV_CH;
   if (L.nlay <= 4)
-    k_f_spmvw<TV, W, 4, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmvw<TV, W, 4, CH, CS>, g, s, e0, e1, L, A, p, q, part, st, nb, cmask,
+                 colsq);
   else if (L.nlay <= 8)
-    k_f_spmvw<TV, W, 8, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmvw<TV, W, 8, CH, CS>, g, s, e0, e1, L, A, p, q, part, st, nb, cmask,
+                 colsq);
   else if (L.nlay <= 16)
-    k_f_spmvw<TV, W, 16, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmvw<TV, W, 16, CH, CS>, g, s, e0, e1, L, A, p, q, part, st, nb, cmask,
+                 colsq);
   else
-    k_f_spmvw<TV, W, 32, CH, CS><<<g, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmvw<TV, W, 32, CH, CS>, g, s, e0, e1, L, A, p, q, part, st, nb, cmask,
+                 colsq);
 }
 
 // cmask / colsq given (the deflated two-level form): the SpMV also writes
@@ -1218,35 +1235,38 @@ static void launch_f_spmvw(const Layout &L, int nb, hipStream_t s, const TV *A, 
 template <typename TM, typename TV, bool CS>
 static void launch_f_spmv_t(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
                             TV *q, double *part, const KState *st, const unsigned *cmask,
-                            double *colsq) {
+                            double *colsq, hipEvent_t e0, hipEvent_t e1) {
   constexpr int WMAX = 16 / (int)sizeof(TV);
   const int w = col_group<TV>(L);
   if (w == WMAX) {
-    launch_f_spmvw<TV, WMAX, CS>(L, nb, s, A, p, q, part, st, cmask, colsq);
+    launch_f_spmvw<TV, WMAX, CS>(L, nb, s, A, p, q, part, st, cmask, colsq, e0, e1);
     return;
   }
   if (w == 2) {
-    launch_f_spmvw<TV, 2, CS>(L, nb, s, A, p, q, part, st, cmask, colsq);
+    launch_f_spmvw<TV, 2, CS>(L, nb, s, A, p, q, part, st, cmask, colsq, e0, e1);
     return;
   }
   if (L.nlay <= 4)
-    k_f_spmv<TM, TV, 4, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmv<TM, TV, 4, CS>, nb, s, e0, e1, L, A, p, q, part, st, nb, cmask, colsq);
   else if (L.nlay <= 8)
-    k_f_spmv<TM, TV, 8, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmv<TM, TV, 8, CS>, nb, s, e0, e1, L, A, p, q, part, st, nb, cmask, colsq);
   else if (L.nlay <= 16)
-    k_f_spmv<TM, TV, 16, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmv<TM, TV, 16, CS>, nb, s, e0, e1, L, A, p, q, part, st, nb, cmask,
+                 colsq);
   else
-    k_f_spmv<TM, TV, 32, CS><<<nb, MF6_BLOCK, 0, s>>>(L, A, p, q, part, st, nb, cmask, colsq);
+    launch_timed(k_f_spmv<TM, TV, 32, CS>, nb, s, e0, e1, L, A, p, q, part, st, nb, cmask,
+                 colsq);
 }
 
 template <typename TM, typename TV>
 static void launch_f_spmv(const Layout &L, int nb, hipStream_t s, const TM *A, const TV *p,
                           TV *q, double *part, const KState *st,
-                          const unsigned *cmask = nullptr, double *colsq = nullptr) {
+                          const unsigned *cmask = nullptr, double *colsq = nullptr,
+                          hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (colsq)
-    launch_f_spmv_t<TM, TV, true>(L, nb, s, A, p, q, part, st, cmask, colsq);
+    launch_f_spmv_t<TM, TV, true>(L, nb, s, A, p, q, part, st, cmask, colsq, e0, e1);
   else
-    launch_f_spmv_t<TM, TV, false>(L, nb, s, A, p, q, part, st, nullptr, nullptr);
+    launch_f_spmv_t<TM, TV, false>(L, nb, s, A, p, q, part, st, nullptr, nullptr, e0, e1);
 }
 
 // x += apend_x p (the last iteration's head update), then no update pending
@@ -1625,15 +1645,16 @@ static void enqueue_fused64(Mf6Handle *h, int n, int nbc, int nb) {
     double *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
     launch_f_pupdate<double, double>(L, nb, s, h->z, pin, pout, h->x, h->d_mpart, st, fs,
                                      tl ? h->tl_yc64 : nullptr, h->tl_cmask, h->tl);
-    int slot = -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
         2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
-      slot = h->last_spmv_n++;
-      (void)hipEventRecord(h->spmv_ev[2 * slot], s);
+      const int slot = h->last_spmv_n++;
+      e0 = h->spmv_ev[2 * slot];
+      e1 = h->spmv_ev[2 * slot + 1];
     }
     launch_f_spmv<double, double>(L, nb, s, h->A, pout, h->q, h->d_part, st,
-                                  defl ? h->tl_cmask : nullptr, defl ? h->tl_colsq : nullptr);
-    if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
+                                  defl ? h->tl_cmask : nullptr, defl ? h->tl_colsq : nullptr,
+                                  e0, e1);
     k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
   }
 }
@@ -1752,15 +1773,16 @@ static int run_cg_reliable(Mf6Handle *h, const TMP *Apc, const TP *piv32, float 
         float *pin = pbuf[round & 1], *pout = pbuf[(round + 1) & 1];
         launch_f_pupdate<float, float>(L, nb, s, h->z32, pin, pout, h->e32, h->d_mpart, st, fs,
                                        tl ? h->tl_yc32 : nullptr, h->tl_cmask, h->tl);
-        int slot = -1;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
         if (h->timing && (h->spmv_calls++ % h->spmv_sample) == 0 &&
             2 * (h->last_spmv_n + 1) <= (int)h->spmv_ev.size()) {
-          slot = h->last_spmv_n++;
-          (void)hipEventRecord(h->spmv_ev[2 * slot], s);
+          const int slot = h->last_spmv_n++;
+          e0 = h->spmv_ev[2 * slot];
+          e1 = h->spmv_ev[2 * slot + 1];
         }
         launch_f_spmv<float, float>(L, nb, s, h->A32, pout, h->q32, h->d_part, st,
-                                    defl ? h->tl_cmask : nullptr, defl ? h->tl_colsq : nullptr);
-        if (slot >= 0) (void)hipEventRecord(h->spmv_ev[2 * slot + 1], s);
+                                    defl ? h->tl_cmask : nullptr, defl ? h->tl_colsq : nullptr,
+                                    e0, e1);
         k_f_alpha<<<1, MF6_RBLOCK, 0, s>>>(st, fs, h->d_part, h->d_mpart, nb);
       }
       HIPCHK(hipGetLastError());
