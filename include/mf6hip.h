@@ -160,9 +160,15 @@ typedef struct mf6hip_gwf_model {
   const int *iconvert;      /* nodes (STO ICONVERT), may be NULL without STO */
   const double *ss;         /* nodes (STO SS) */
   const double *sy;         /* nodes (STO SY) */
-  const double *k22;        /* nodes: NPF K22, the conductivity of row-direction
-                               faces (hy_eff without angles, gwf-npf.f90:
-                               2280-2355); NULL: K11 */
+  const double *k22;        /* nodes: NPF K22; NULL: K11 */
+  const double *angle1;     /* nodes: NPF ANGLE1, degrees as read; NULL: none */
+  const double *angle2;     /* nodes: NPF ANGLE2 (needs ANGLE1 and ANGLE3) */
+  const double *angle3;     /* nodes: NPF ANGLE3 (needs ANGLE1 and ANGLE2) */
+  /* The faces' conductivities follow hy_eff (gwf-npf.f90:2280-2355) with the
+   * DIS connection normals: column- and row-direction faces K11 unless K22
+   * is given, then the rotated tensor's value along the face normal
+   * (HGeoUtil.f90:29-108 hyeff, harmonic); vertical faces K33 unless ANGLE2
+   * is given.  Computed once on the host at mf6hip_gwf_create. */
 } mf6hip_gwf_model;
 
 /* package types, in the bndlist order MF6 fixes by type (gwf.f90:119-124,

@@ -92,7 +92,10 @@ class GwfModelC(C.Structure):
                 ("iconvert", C.POINTER(C.c_int)),
                 ("ss", C.POINTER(C.c_double)),
                 ("sy", C.POINTER(C.c_double)),
-                ("k22", C.POINTER(C.c_double))]
+                ("k22", C.POINTER(C.c_double)),
+                ("angle1", C.POINTER(C.c_double)),
+                ("angle2", C.POINTER(C.c_double)),
+                ("angle3", C.POINTER(C.c_double))]
 
 
 class PtcC(C.Structure):

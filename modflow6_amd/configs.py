@@ -143,8 +143,12 @@ class GwfCase:
     idewatcv: int = 0
     icellavg: int = 0
     # NPF K22 (nodes,): the conductivity of row-direction (i) faces; None:
-    # K11
+    # K11.  ANGLE1..3 (nodes,), degrees as read: rotate the K tensor (K22
+    # given: horizontal faces; ANGLE2 given: vertical faces)
     k22: Optional[np.ndarray] = None
+    angle1: Optional[np.ndarray] = None
+    angle2: Optional[np.ndarray] = None
+    angle3: Optional[np.ndarray] = None
 
     @property
     def nodes(self):

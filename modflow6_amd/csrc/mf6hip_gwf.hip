@@ -1252,6 +1252,65 @@ static int upload_pos(Mf6Handle *h, T **dst, const T *src) {
   return MF6HIP_OK;
 }
 
+// sin and cos of one angle from one libm sincos call, as the reference's
+// compiled hyeff gets them (its compiler merges each sin / cos pair; sincos
+// differs from separate sin and cos calls in the last bit for some
+// arguments)
+static void host_sincos(double a, double *s, double *c) { ::sincos(a, s, c); }
+
+// HGeoUtil.f90:29-108 hyeff (harmonic, iavgmeth 0): the conductivity along
+// the unit vector vg of the tensor k11, k22, k33 rotated by ang1..3
+// (radians); ve**2 of the reference compiles to ve * ve
+static double hyeff_host(double k11, double k22, double k33, double ang1, double ang2,
+                         double ang3, double vg1, double vg2, double vg3) {
+  double s1, c1, s2, c2, s3, c3;
+  host_sincos(ang1, &s1, &c1);
+  host_sincos(ang2, &s2, &c2);
+  host_sincos(ang3, &s3, &c3);
+  const double r11 = c1 * c2, r12 = c1 * s2 * s3 - s1 * c3, r13 = -c1 * s2 * c3 - s1 * s3;
+  const double r21 = s1 * c2, r22 = s1 * s2 * s3 + c1 * c3, r23 = -s1 * s2 * c3 + c1 * s3;
+  const double r31 = s2, r32 = -c2 * s3, r33 = c2 * c3;
+  const double ve1 = r11 * vg1 + r21 * vg2 + r31 * vg3;
+  const double ve2 = r12 * vg1 + r22 * vg2 + r32 * vg3;
+  const double ve3 = r13 * vg1 + r23 * vg2 + r33 * vg3;
+  double dnum = 1.0, d1 = ve1 * ve1, d2 = ve2 * ve2, d3 = ve3 * ve3;
+  if (ve1 != 0.0) { dnum = dnum * k11; d2 = d2 * k11; d3 = d3 * k11; }
+  if (ve2 != 0.0) { dnum = dnum * k22; d1 = d1 * k22; d3 = d3 * k22; }
+  if (ve3 != 0.0) { dnum = dnum * k33; d1 = d1 * k33; d2 = d2 * k33; }
+  const double denom = d1 + d2 + d3;
+  return denom > 0.0 ? dnum / denom : 0.0;
+}
+
+// The conductivities of column-direction (kx), row-direction (ky) and
+// vertical (kz) faces, hy_eff (gwf-npf.f90:2280-2355) with the DIS face
+// normals along the axes (hyeff depends on them only through squares and
+// zero tests); angles converted as prepcheck does (:1686-1731, DPIO180 =
+// atan(1) / 45, Constants.f90:130).
+static int directional_k(const mf6hip_gwf_model *m, int n, std::vector<double> &kx,
+                         std::vector<double> &ky, std::vector<double> &kz) {
+  const bool i1 = m->angle1, i2 = m->angle2, i3 = m->angle3;
+  if ((i2 && !i1) || (i3 && !i1) || (i2 != i3)) {
+    mf6hip_set_error("NPF angles: ANGLE2 and ANGLE3 need ANGLE1 and each other");
+    return MF6HIP_EINVAL;
+  }
+  const double dpio180 = std::atan(1.0) / 45.0;
+  kx.assign(m->k11, m->k11 + n);
+  ky.assign(m->k11, m->k11 + n);
+  kz.assign(m->k33, m->k33 + n);
+  const double *k22 = m->k22 ? m->k22 : m->k11;
+  for (int c = 0; c < n; c++) {
+    const double a1 = i1 ? m->angle1[c] * dpio180 : 0.0;
+    const double a2 = i2 ? m->angle2[c] * dpio180 : 0.0;
+    const double a3 = i3 ? m->angle3[c] * dpio180 : 0.0;
+    if (m->k22) {
+      kx[c] = hyeff_host(m->k11[c], k22[c], m->k33[c], a1, a2, a3, 1.0, 0.0, 0.0);
+      ky[c] = hyeff_host(m->k11[c], k22[c], m->k33[c], a1, a2, a3, 0.0, 1.0, 0.0);
+    }
+    if (i2) kz[c] = hyeff_host(m->k11[c], k22[c], m->k33[c], a1, a2, a3, 0.0, 0.0, 1.0);
+  }
+  return MF6HIP_OK;
+}
+
 extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   Mf6Handle *h = (Mf6Handle *)handle;
   if (!h || !m) { mf6hip_set_error("null argument"); return MF6HIP_EINVAL; }
@@ -1278,9 +1337,13 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   h->h_icelltype.assign(m->icelltype, m->icelltype + L.N);
   CK(upload_pos(h, &h->top, m->top));
   CK(upload_pos(h, &h->bot, m->bot));
-  CK(upload_pos(h, &h->k11, m->k11));
-  CK(upload_pos(h, &h->k22, m->k22 ? m->k22 : m->k11));   // K22: K11 when not given
-  CK(upload_pos(h, &h->k33, m->k33));
+  {
+    std::vector<double> kx, ky, kz;
+    CK(directional_k(m, L.N, kx, ky, kz));
+    CK(upload_pos(h, &h->k11, kx.data()));
+    CK(upload_pos(h, &h->k22, ky.data()));
+    CK(upload_pos(h, &h->k33, kz.data()));
+  }
   CK(upload_pos(h, &h->icelltype, m->icelltype));
   CK(upload_pos(h, &h->ibound0, m->ibound));
   {

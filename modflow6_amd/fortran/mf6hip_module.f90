@@ -77,6 +77,9 @@ module mf6hip_module
     type(c_ptr) :: ss
     type(c_ptr) :: sy
     type(c_ptr) :: k22
+    type(c_ptr) :: angle1
+    type(c_ptr) :: angle2
+    type(c_ptr) :: angle3
   end type mf6hip_gwf_model
 
   !> mf6hip_ptc: pseudo-transient continuation control and state

@@ -89,6 +89,9 @@ class GwfSolution:
         m.k11, m.k33 = d(c.k11), d(c.k33)
         if c.k22 is not None:
             m.k22 = d(c.k22)
+        for nm in ("angle1", "angle2", "angle3"):
+            if getattr(c, nm) is not None:
+                setattr(m, nm, d(getattr(c, nm)))
         m.icelltype = i(c.icelltype)
         m.ibound = i(np.ones(c.nodes, dtype=np.int32))
         m.inewton = 1 if c.newton else 0

@@ -19,6 +19,7 @@
  * GwfStorageUtils.f90, sparsekit amux, blas ddot/dnrm2) compiled from
  * /root/reference by oracle/Makefile into oracle/_ref.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -730,6 +731,47 @@ void orc_hfb_condsat_reset(int nhfb, const int *idxloc, const int *jas,
                            const double *csatsav, double *condsat)
 {
   for (int b = 0; b < nhfb; b++) condsat[jas[idxloc[b]]] = csatsav[b];
+}
+
+/* src/Utilities/HGeoUtil.f90:29-108 hyeff: the conductivity along the unit
+ * vector vg of the tensor with principal values k11, k22, k33 rotated by
+ * ang1, ang2, ang3 (radians); iavgmeth 0 harmonic (the NPF default), 1
+ * arithmetic.  The reference's ve**2 compiles to ve * ve. */
+double orc_hyeff(double k11, double k22, double k33, double ang1, double ang2,
+                 double ang3, double vg1, double vg2, double vg3, int iavgmeth)
+{
+  /* each sin / cos pair from one sincos call: the reference's compiled
+   * hyeff merges them so, and sincos can differ from separate sin and cos
+   * calls in the last bit */
+  double s1, c1, s2, c2, s3, c3;
+  sincos(ang1, &s1, &c1);
+  sincos(ang2, &s2, &c2);
+  sincos(ang3, &s3, &c3);
+  double r11 = c1 * c2;
+  double r12 = c1 * s2 * s3 - s1 * c3;
+  double r13 = -c1 * s2 * c3 - s1 * s3;
+  double r21 = s1 * c2;
+  double r22 = s1 * s2 * s3 + c1 * c3;
+  double r23 = -s1 * s2 * c3 + c1 * s3;
+  double r31 = s2;
+  double r32 = -c2 * s3;
+  double r33 = c2 * c3;
+  double ve1 = r11 * vg1 + r21 * vg2 + r31 * vg3;
+  double ve2 = r12 * vg1 + r22 * vg2 + r32 * vg3;
+  double ve3 = r13 * vg1 + r23 * vg2 + r33 * vg3;
+  double k = DZERO;
+  if (iavgmeth == 0) {
+    double dnum = DONE;
+    double d1 = ve1 * ve1, d2 = ve2 * ve2, d3 = ve3 * ve3;
+    if (ve1 != DZERO) { dnum = dnum * k11; d2 = d2 * k11; d3 = d3 * k11; }
+    if (ve2 != DZERO) { dnum = dnum * k22; d1 = d1 * k22; d3 = d3 * k22; }
+    if (ve3 != DZERO) { dnum = dnum * k33; d1 = d1 * k33; d2 = d2 * k33; }
+    double denom = d1 + d2 + d3;
+    if (denom > DZERO) k = dnum / denom;
+  } else if (iavgmeth == 1) {
+    k = ve1 * ve1 * k11 + ve2 * ve2 * k22 + ve3 * ve3 * k33;
+  }
+  return k;
 }
 
 /* gwf-npf.f90:1950-2037 calc_condsat (upperOnly, saturation 1, no

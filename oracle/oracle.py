@@ -10,6 +10,7 @@ library; Python only sequences the calls the way the reference does.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 
@@ -75,6 +76,8 @@ def lib():
                                  f64p, C.POINTER(C.c_int),
                                  C.POINTER(C.c_double), C.POINTER(C.c_int),
                                  C.c_void_p]
+        L.orc_hyeff.restype = d
+        L.orc_hyeff.argtypes = [d] * 9 + [i]
         for nm, nargs in (("orc_sqsat", 4), ("orc_sqsat_deriv", 4),
                           ("orc_sqsat_cubic", 5), ("orc_sqsat_cubic_deriv", 5)):
             getattr(L, nm).restype = d
@@ -247,6 +250,10 @@ def ref_gwf_lib():
                                    f64p, f64p, f64p, f64p, f64p, f64p]
         R.ref_drn.argtypes = [i, i, i, i32p, f64p, f64p, i, f64p, i, i32p,
                               f64p, f64p, f64p, f64p, f64p, f64p, f64p]
+        R.ref_hyeff.restype = d
+        R.ref_hyeff.argtypes = [d] * 9 + [i]
+        R.ref_dpio180.restype = d
+        R.ref_dpio180.argtypes = [d]
         R.ref_gnc.argtypes = [i, i, i, i, i32p, i32p, i32p, f64p, i32p, f64p,
                               i, i, i32p, f64p, f64p, f64p, f64p, f64p]
         R.ref_wel.argtypes = [i, i, i, i32p, f64p, i, d, i32p, i32p, f64p,
@@ -527,6 +534,42 @@ def permute_csr(ia, ja, a, perm):
 _PKG_ORDER = {"WEL": 0, "DRN": 1, "RCH": 2, "CHD": 3}   # gwf.f90:119-124
 
 
+# Constants.f90:130 DPIO180 = atan(1) / 45, the NPF angle conversion
+DPIO180 = math.atan(1.0) / 45.0
+
+
+def npf_directional_k(c):
+    """hy_eff (gwf-npf.f90:2280-2355) per cell for the three face directions
+    of a DIS grid, whose connection normals are the axes (connection_normal,
+    Dis.f90:1039-1087; hyeff depends on the normal only through squares and
+    zero tests, so its sign does not matter): column-direction faces (1, 0,
+    0) and row-direction faces (0, 1, 0) use K11 unless K22 is given, then
+    hyeff with ANGLE1 (ANGLE2, ANGLE3 when given); vertical faces use K33
+    unless ANGLE2 is given.  Angles are in degrees as read (prepcheck
+    :1686-1731 converts them)."""
+    L = lib()
+    k11 = np.ascontiguousarray(c.k11, dtype=np.float64)
+    k22 = np.ascontiguousarray(c.k22 if c.k22 is not None else c.k11, dtype=np.float64)
+    k33 = np.ascontiguousarray(c.k33, dtype=np.float64)
+    ang = [None if a is None else np.asarray(a, dtype=np.float64) * DPIO180
+           for a in (c.angle1, c.angle2, c.angle3)]
+    i1, i2, i3 = (a is not None for a in ang)
+    z = np.zeros(k11.size)
+    a1, a2, a3 = (a if a is not None else z for a in ang)
+    hx, hy, hz = k11.copy(), k11.copy(), k33.copy()
+    if c.k22 is not None:
+        for n in range(k11.size):
+            b1 = a1[n] if i1 else 0.0
+            b2 = a2[n] if (i1 and i2) else 0.0
+            b3 = a3[n] if (i1 and i2 and i3) else 0.0
+            hx[n] = L.orc_hyeff(k11[n], k22[n], k33[n], b1, b2, b3, 1.0, 0.0, 0.0, 0)
+            hy[n] = L.orc_hyeff(k11[n], k22[n], k33[n], b1, b2, b3, 0.0, 1.0, 0.0, 0)
+    if i2:
+        for n in range(k11.size):
+            hz[n] = L.orc_hyeff(k11[n], k22[n], k33[n], a1[n], a2[n], a3[n] if i3 else 0.0,
+                                0.0, 0.0, 1.0, 0)
+    return hx, hy, hz
+
 def wel_flowred(value):
     """gwf-wel.f90:209-217 (wel_options): AUTO_FLOW_REDUCE <= 0 becomes 0.1,
     above 1 becomes 1."""
@@ -550,12 +593,9 @@ class GwfSimulation:
                                    c.top2d, c.botm)
         d = self.dis
         self.neq = d.nodes
-        self.k11 = np.ascontiguousarray(c.k11, dtype=np.float64)
-        # K22 (row-direction faces, hy_eff gwf-npf.f90:2280-2355 without
-        # angles): K11 when not given
-        self.k22 = np.ascontiguousarray(c.k22 if c.k22 is not None else c.k11,
-                                        dtype=np.float64)
-        self.k33 = np.ascontiguousarray(c.k33, dtype=np.float64)
+        # the conductivities of column-direction, row-direction and vertical
+        # faces (hy_eff, gwf-npf.f90:2280-2355)
+        self.k11, self.k22, self.k33 = npf_directional_k(c)
         self.icelltype = np.ascontiguousarray(c.icelltype, dtype=np.int32)
         self.ibound = np.ones(self.neq, dtype=np.int32)
         self.inewton = 1 if c.newton else 0

@@ -540,6 +540,28 @@ contains
   end subroutine ref_wel
 
   ! ------------------------------------------------------------------
+  ! HGeoUtil
+  ! ------------------------------------------------------------------
+
+  !> HGeoUtil.f90:29 hyeff (the effective conductivity along a direction)
+  function ref_hyeff(k11, k22, k33, ang1, ang2, ang3, vg1, vg2, vg3, &
+                     iavgmeth) result(k) bind(C, name='ref_hyeff')
+    use HGeoUtilModule, only: hyeff
+    real(c_double), value :: k11, k22, k33, ang1, ang2, ang3, vg1, vg2, vg3
+    integer(c_int), value :: iavgmeth
+    real(c_double) :: k
+    k = hyeff(k11, k22, k33, ang1, ang2, ang3, vg1, vg2, vg3, iavgmeth)
+  end function ref_hyeff
+
+  !> Constants.f90:130 DPIO180 times a value (the NPF angle conversion)
+  function ref_dpio180(a) result(r) bind(C, name='ref_dpio180')
+    use ConstantsModule, only: DPIO180
+    real(c_double), value :: a
+    real(c_double) :: r
+    r = a * DPIO180
+  end function ref_dpio180
+
+  ! ------------------------------------------------------------------
   ! GNC (one model, explicit)
   ! ------------------------------------------------------------------
 

@@ -3,8 +3,10 @@ reference's npf_fc / npf_fn / npf_cq with VARIABLECV, DEWATERED and the
 alternative cell averagings in tests/test_oracle_gwf_vs_reference.py and
 tests/test_oracle_vs_reference.py; K22 in tests/test_oracle_npf_k22.py):
 VARIABLECV with and without DEWATERED, under NEWTON and without,
-ALTERNATIVE_CELL_AVERAGING LOGARITHMIC / AMT-LMK / AMT-HMK, and K22 on the
-row-direction faces: the formulated system and the budget's face flows bit for
+ALTERNATIVE_CELL_AVERAGING LOGARITHMIC / AMT-LMK / AMT-HMK, K22 on the
+row-direction faces, and ANGLE1 (with ANGLE2 / ANGLE3) rotating the
+conductivity tensor (hyeff computed on the host at create, as the oracle
+does, so the faces match bit for bit): the formulated system and the budget's face flows bit for
 bit (to 1e-14 relative where a logarithmic mean calls log()), and the heads
 of solved steps within 10 x dvclose."""
 import dataclasses
@@ -46,6 +48,16 @@ def _k22(base, seed=8):
     return dataclasses.replace(c, k22=np.asarray(c.k11) * rng.lognormal(-1.0, 0.7, c.nodes))
 
 
+def _angles(base, vertical, seed=9):
+    c = _k22(base, seed)
+    rng = np.random.default_rng(seed)
+    a1 = rng.uniform(-90.0, 90.0, c.nodes)
+    if not vertical:
+        return dataclasses.replace(c, angle1=a1)
+    return dataclasses.replace(c, angle1=a1, angle2=rng.uniform(-30.0, 30.0, c.nodes),
+                               angle3=rng.uniform(-30.0, 30.0, c.nodes))
+
+
 CASES = {
     "varcv_newton": lambda: _convertible(True, 1, 0),
     "varcv_dewatered_newton": lambda: _convertible(True, 1, 1),
@@ -56,6 +68,8 @@ CASES = {
     "amt_hmk": lambda: _averaging(3),
     "k22_confined": lambda: _k22(lambda: _averaging(0)),
     "k22_newton": lambda: _k22(lambda: _convertible(True, 0, 0)),
+    "angle1_confined": lambda: _angles(lambda: _averaging(0), False),
+    "angles_newton": lambda: _angles(lambda: _convertible(True, 0, 0), True),
 }
 
 
@@ -76,7 +90,8 @@ def test_formulate_and_budget_bit_exact(name):
     finally:
         gs.destroy()
     base = O.GwfSimulation(dataclasses.replace(case, ivarcv=0, idewatcv=0, icellavg=0,
-                                               k22=None))
+                                               k22=None, angle1=None, angle2=None,
+                                               angle3=None))
     base.kper, base.kstp = 1, 1
     base.advance(perlen / nstp, steady)
     base.formulate(1)
@@ -97,7 +112,8 @@ def test_formulate_and_budget_bit_exact(name):
     assert np.array_equal(db["flowja"].view(np.int64), ob["flowja"].view(np.int64))
 
 
-@pytest.mark.parametrize("name", ["varcv_dewatered_newton", "logarithmic", "k22_newton"])
+@pytest.mark.parametrize("name", ["varcv_dewatered_newton", "logarithmic", "k22_newton",
+                                  "angles_newton"])
 def test_step_heads(name):
     case = CASES[name]()
     sim = O.GwfSimulation(case)

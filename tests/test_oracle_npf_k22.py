@@ -6,6 +6,7 @@ transposition: the row faces of a grid equal the column faces of its
 transpose with K11 and K22 swapped, so K22 must reach exactly the row faces
 with the formula K11 uses on the column faces."""
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 
@@ -72,3 +73,30 @@ def test_k22_rows_equal_columns_of_the_transpose():
     # and K22 matters: with K22 = K11 the row faces change
     f1 = _faces(nlay, nrow, ncol, delr, delc, top2d, botm, k11.ravel(), k11.ravel(), k33.ravel())
     assert f1[(0, ncol)] != f[(0, ncol)] and f1[(0, 1)] == f[(0, 1)]
+
+
+R = O.ref_gwf_lib()
+
+
+@pytest.mark.skipif(R is None, reason="oracle/_ref not built")
+def test_hyeff_matches_reference():
+    """hyeff (HGeoUtil.f90:29-108) bit for bit against the reference, for
+    the DIS face normals and random directions, both averagings, angles
+    converted with the reference's DPIO180."""
+    rng = np.random.default_rng(2)
+    axes = [(1.0, 0.0, 0.0), (-1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, -1.0, 0.0),
+            (0.0, 0.0, 1.0), (0.0, 0.0, -1.0)]
+    for _ in range(600):
+        k = rng.lognormal(0, 1.5, 3)
+        deg = rng.uniform(-180, 180, 3) * (rng.random(3) < 0.8)
+        ang = [R.ref_dpio180(a) for a in deg]
+        assert all(a == d * O.DPIO180 for a, d in zip(ang, deg))
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        for vg in axes + [tuple(v)]:
+            for iavg in (0, 1):
+                a = L.orc_hyeff(*k, *ang, *vg, iavg)
+                b = R.ref_hyeff(*k, *ang, *vg, iavg)
+                assert np.float64(a).view(np.int64) == np.float64(b).view(np.int64)
+    # no rotation: the axes give K11, K22, K33 exactly
+    assert L.orc_hyeff(2.0, 3.0, 5.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0) == 3.0
