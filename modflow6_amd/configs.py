@@ -142,6 +142,11 @@ class GwfCase:
     ivarcv: int = 0
     idewatcv: int = 0
     icellavg: int = 0
+    # STO options: STORAGECOEFFICIENT (ss is the storage coefficient),
+    # SS_CONFINED_ONLY, and the original specific storage formulation
+    istor_coef: int = 0
+    iconf_ss: int = 0
+    iorig_ss: int = 0
     # NPF K22 (nodes,): the conductivity of row-direction (i) faces; None:
     # K11.  ANGLE1..3 (nodes,), degrees as read: rotate the K tensor (K22
     # given: horizontal faces; ANGLE2 given: vertical faces)

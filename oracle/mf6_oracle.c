@@ -924,6 +924,7 @@ void orc_npf_fn(int nodes, const int *ia, const int *ja, const int *jas,
 void orc_sto_fc(int nodes, const int *ia, const int *ibound,
                 const int *iconvert, const double *top, const double *bot,
                 const double *area, const double *ss, const double *sy,
+                int istor_coef, int iconf_ss, int iorig_ss,
                 double delt, double satomega, const double *hold,
                 const double *hnew, double *amat, double *rhs)
 {
@@ -938,11 +939,11 @@ void orc_sto_fc(int nodes, const int *ia, const int *ibound,
       snold = orc_sqsat(tp, bt, hold[n], satomega);
       snnew = orc_sqsat(tp, bt, hnew[n], satomega);
     }
-    double sc1 = orc_sscapacity(0, tp, bt, area[n], ss[n]);
+    double sc1 = orc_sscapacity(istor_coef, tp, bt, area[n], ss[n]);
     double rho1 = sc1 * tled;
     double rho1old = rho1;
     double aterm, rhsterm;
-    orc_ssterms(iconvert[n], 0, 0, tp, bt, rho1, rho1old, snnew, snold,
+    orc_ssterms(iconvert[n], iorig_ss, iconf_ss, tp, bt, rho1, rho1old, snnew, snold,
                 hnew[n], hold[n], &aterm, &rhsterm);
     amat[idiag] = amat[idiag] + aterm;
     rhs[n] = rhs[n] + rhsterm;
@@ -963,6 +964,7 @@ void orc_sto_fc(int nodes, const int *ia, const int *ibound,
 void orc_sto_fn(int nodes, const int *ia, const int *ibound,
                 const int *iconvert, const double *top, const double *bot,
                 const double *area, const double *ss, const double *sy,
+                int istor_coef, int iconf_ss, int iorig_ss,
                 double delt, const double *hnew, double *amat, double *rhs)
 {
   double tled = DONE / delt;
@@ -973,16 +975,20 @@ void orc_sto_fn(int nodes, const int *ia, const int *ibound,
     double tthk = tp - bt;
     double h = hnew[n];
     double snnew = orc_sqsat(tp, bt, h, DEM6);
-    double sc1 = orc_sscapacity(0, tp, bt, area[n], ss[n]);
+    double sc1 = orc_sscapacity(istor_coef, tp, bt, area[n], ss[n]);
     double sc2 = orc_sycapacity(area[n], sy[n]);
     double rho1 = sc1 * tled;
     double rho2 = sc2 * tled;
     if (iconvert[n] != 0) {
       double derv = orc_sqsat_deriv(tp, bt, h, DEM6);
       /* iconf_ss == 0, iorig_ss == 0 */
-      double drterm = -rho1 * derv * (h - bt) + rho1 * tthk * snnew * derv;
-      amat[idiag] = amat[idiag] + drterm;
-      rhs[n] = rhs[n] + drterm * h;
+      double drterm;
+      if (iconf_ss == 0) {
+        if (iorig_ss == 0) drterm = -rho1 * derv * (h - bt) + rho1 * tthk * snnew * derv;
+        else drterm = -(rho1 * derv * h);
+        amat[idiag] = amat[idiag] + drterm;
+        rhs[n] = rhs[n] + drterm * h;
+      }
       if (snnew < DONE) {
         if (snnew > DZERO) {
           double rterm = -rho2 * tthk * snnew;
@@ -1165,6 +1171,7 @@ void orc_npf_cq(int nodes, const int *ia, const int *ja, const int *jas,
 void orc_sto_cq(int nodes, const int *ia, const int *ibound,
                 const int *iconvert, const double *top, const double *bot,
                 const double *area, const double *ss, const double *sy,
+                int istor_coef, int iconf_ss, int iorig_ss,
                 double delt, double satomega, const double *hold,
                 const double *hnew, double *flowja, double *strgss,
                 double *strgsy)
@@ -1180,11 +1187,11 @@ void orc_sto_cq(int nodes, const int *ia, const int *ibound,
       snold = orc_sqsat(tp, bt, hold[n], satomega);
       snnew = orc_sqsat(tp, bt, hnew[n], satomega);
     }
-    double sc1 = orc_sscapacity(0, tp, bt, area[n], ss[n]);
+    double sc1 = orc_sscapacity(istor_coef, tp, bt, area[n], ss[n]);
     double rho1 = sc1 * tled;
     double rho1old = rho1;
     double aterm, rhsterm;
-    orc_ssterms(iconvert[n], 0, 0, tp, bt, rho1, rho1old, snnew, snold,
+    orc_ssterms(iconvert[n], iorig_ss, iconf_ss, tp, bt, rho1, rho1old, snnew, snold,
                 hnew[n], hold[n], &aterm, &rhsterm);
     double rate = aterm * hnew[n] - rhsterm;
     strgss[n] = rate;

@@ -107,15 +107,15 @@ def lib():
         L.orc_npf_fn.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, i32p,
                                  i32p, i, f64p, f64p, d, f64p, f64p, f64p]
         L.orc_sto_fc.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
-                                 f64p, d, d, f64p, f64p, f64p, f64p]
+                                 f64p, i, i, i, d, d, f64p, f64p, f64p, f64p]
         L.orc_sto_fn.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
-                                 f64p, d, f64p, f64p, f64p]
+                                 f64p, i, i, i, d, f64p, f64p, f64p]
         L.orc_bnd_fc.argtypes = [i, i32p, f64p, f64p, i32p, f64p, f64p]
         L.orc_npf_cq.argtypes = [i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
                                  f64p, f64p, i32p, i32p, i, i, i, i, f64p,
                                  f64p, i, f64p, f64p, f64p, f64p, f64p, f64p]
         L.orc_sto_cq.argtypes = [i, i32p, i32p, i32p, f64p, f64p, f64p, f64p,
-                                 f64p, d, d, f64p, f64p, f64p, f64p, f64p]
+                                 f64p, i, i, i, d, d, f64p, f64p, f64p, f64p, f64p]
         L.orc_bnd_cq.argtypes = [i, i32p, f64p, f64p, i32p, f64p, i32p, f64p,
                                  f64p]
         L.orc_csr_diagsum.argtypes = [i, i32p, f64p]
@@ -262,7 +262,7 @@ def ref_gwf_lib():
                                  f64p, f64p]
         R.ref_npf_cq.argtypes = [i, i, i, i, i, i, d, i32p, i32p, f64p, f64p,
                                  f64p, f64p, f64p, f64p]
-        R.ref_sto_cq.argtypes = [i, i, d, i, d, i32p, i32p, f64p, f64p, f64p,
+        R.ref_sto_cq.argtypes = [i, i, d, i, i, i, i, d, i32p, i32p, f64p, f64p, f64p,
                                  f64p, f64p, f64p, f64p]
         R.ref_sln_new.argtypes = [i, i, d, d, d, d, d, d]
         R.ref_sln_free.argtypes = []
@@ -603,6 +603,11 @@ class GwfSimulation:
         # gwf-sto.f90:826-829)
         self.satomega = 1e-6 if c.newton else 0.0
         # NPF options: VARIABLECV (DEWATERED), ALTERNATIVE_CELL_AVERAGING
+        # STO options: STORAGECOEFFICIENT, SS_CONFINED_ONLY and the original
+        # specific storage formulation (gwf-sto.f90:32-34)
+        self.istor_coef = int(c.istor_coef)
+        self.iconf_ss = int(c.iconf_ss)
+        self.iorig_ss = int(c.iorig_ss)
         self.ivarcv = int(c.ivarcv)
         self.idewatcv = int(c.idewatcv)
         self.icellavg = int(c.icellavg)
@@ -775,7 +780,8 @@ class GwfSimulation:
                          self.amat, gc, self.rhs)
         if self.case.has_sto and not self.iss:
             L.orc_sto_fc(self.neq, d.ia, self.ibound, self.iconvert, d.top,
-                         d.bot, d.area, self.ss, self.sy, self.delt,
+                         d.bot, d.area, self.ss, self.sy,
+                         self.istor_coef, self.iconf_ss, self.iorig_ss, self.delt,
                          self.satomega, self.xold, self.x, self.amat,
                          self.rhs)
         for t, p in enumerate(self.pk):
@@ -796,6 +802,7 @@ class GwfSimulation:
             if self.case.has_sto and not self.iss:
                 L.orc_sto_fn(self.neq, d.ia, self.ibound, self.iconvert,
                              d.top, d.bot, d.area, self.ss, self.sy,
+                         self.istor_coef, self.iconf_ss, self.iorig_ss,
                              self.delt, self.x, self.amat, self.rhs)
             # bnd_fn per package (gwf.f90:545-553): WEL AUTO_FLOW_REDUCE
             # and DRN depth scaling
@@ -860,7 +867,8 @@ class GwfSimulation:
         strgsy = np.zeros(self.neq)
         if self.case.has_sto and not self.iss:
             L.orc_sto_cq(self.neq, d.ia, self.ibound, self.iconvert, d.top,
-                         d.bot, d.area, self.ss, self.sy, self.delt,
+                         d.bot, d.area, self.ss, self.sy,
+                         self.istor_coef, self.iconf_ss, self.iorig_ss, self.delt,
                          self.satomega, self.xold, self.x, flowja, strgss,
                          strgsy)
         if self.case.has_sto:

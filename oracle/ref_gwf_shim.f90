@@ -355,10 +355,10 @@ contains
 
   !> gwf-sto.f90:447 sto_cq: per-cell STO-SS / STO-SY rates, added to the
   !! flowja diagonal
-  subroutine ref_sto_cq(nodes, nja, deltv, iss, satomega, iconvert, ibound, &
-                        ss, sy, hold, hnew, flowja, strgss, strgsy) &
-    bind(C, name='ref_sto_cq')
-    integer(c_int), value :: nodes, nja, iss
+  subroutine ref_sto_cq(nodes, nja, deltv, iss, istor_coef, iconf_ss, iorig_ss, &
+                        satomega, iconvert, ibound, ss, sy, hold, hnew, flowja, &
+                        strgss, strgsy) bind(C, name='ref_sto_cq')
+    integer(c_int), value :: nodes, nja, iss, istor_coef, iconf_ss, iorig_ss
     real(c_double), value :: deltv, satomega
     integer(c_int), intent(in) :: iconvert(nodes), ibound(nodes)
     real(c_double), intent(in) :: ss(nodes), sy(nodes), hold(nodes), &
@@ -369,7 +369,8 @@ contains
     real(DP), allocatable :: f(:)
     allocate (f(nja))
     f = flowja
-    sto => new_sto(deltv, iss, 0, 0, 0, satomega, iconvert, ibound, ss, sy)
+    sto => new_sto(deltv, iss, istor_coef, iconf_ss, iorig_ss, satomega, iconvert, &
+                   ibound, ss, sy)
     call sto%sto_cq(f, hnew, hold)
     flowja = f
     strgss = sto%strgss

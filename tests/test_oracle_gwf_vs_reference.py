@@ -151,7 +151,11 @@ def test_npf_fn(grid, ivarcv, satomega):
 
 
 @pytest.mark.parametrize("delt,satomega", [(30.0 / 5, 1e-6), (1.0, 0.0), (0.37, 1e-6)])
-def test_sto_fc_fn_cq(grid, delt, satomega):
+@pytest.mark.parametrize("istor,iconf,iorig", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1),
+                                               (1, 1, 0)])
+def test_sto_fc_fn_cq(grid, delt, satomega, istor, iconf, iorig):
+    """sto_fc / sto_fn / sto_cq with STORAGECOEFFICIENT, SS_CONFINED_ONLY and
+    the original specific storage formulation."""
     g, d = grid, grid.d
     rng = g.rng
     iconvert = np.ascontiguousarray(rng.integers(0, 2, g.n), dtype=np.int32)
@@ -162,11 +166,11 @@ def test_sto_fc_fn_cq(grid, delt, satomega):
         r1, r2 = g.rhs0.copy(), g.rhs0.copy()
         if iflag == 0:
             L.orc_sto_fc(g.n, d.ia, g.ibound, iconvert, d.top, d.bot, d.area,
-                         ss, sy, delt, satomega, g.hold, g.hnew, a1, r1)
+                         ss, sy, istor, iconf, iorig, delt, satomega, g.hold, g.hnew, a1, r1)
         else:
             L.orc_sto_fn(g.n, d.ia, g.ibound, iconvert, d.top, d.bot, d.area,
-                         ss, sy, delt, g.hnew, a1, r1)
-        R.ref_sto_fill(iflag, g.n, d.nja, 2, delt, 0, 0, 0, 0, satomega,
+                         ss, sy, istor, iconf, iorig, delt, g.hnew, a1, r1)
+        R.ref_sto_fill(iflag, g.n, d.nja, 2, delt, 0, istor, iconf, iorig, satomega,
                        iconvert, g.ibound, ss, sy, g.hold, g.hnew, a2, r2)
         assert eqbits(a1, a2), iflag
         assert eqbits(r1, r2), iflag
@@ -174,8 +178,9 @@ def test_sto_fc_fn_cq(grid, delt, satomega):
     f1, f2 = g.amat0.copy(), g.amat0.copy()
     s1, y1, s2, y2 = (np.zeros(g.n) for _ in range(4))
     L.orc_sto_cq(g.n, d.ia, g.ibound, iconvert, d.top, d.bot, d.area, ss, sy,
-                 delt, satomega, g.hold, g.hnew, f1, s1, y1)
-    R.ref_sto_cq(g.n, d.nja, delt, 0, satomega, iconvert, g.ibound, ss, sy,
+                 istor, iconf, iorig, delt, satomega, g.hold, g.hnew, f1, s1, y1)
+    R.ref_sto_cq(g.n, d.nja, delt, 0, istor, iconf, iorig, satomega, iconvert, g.ibound,
+                 ss, sy,
                  g.hold, g.hnew, f2, s2, y2)
     assert eqbits(f1, f2) and eqbits(s1, s2) and eqbits(y1, y2)
     assert np.count_nonzero(y1) > 0
