@@ -164,6 +164,9 @@ typedef struct mf6hip_gwf_model {
   const double *angle1;     /* nodes: NPF ANGLE1, degrees as read; NULL: none */
   const double *angle2;     /* nodes: NPF ANGLE2 (needs ANGLE1 and ANGLE3) */
   const double *angle3;     /* nodes: NPF ANGLE3 (needs ANGLE1 and ANGLE2) */
+  int istor_coef;           /* STO STORAGECOEFFICIENT (gwf-sto.f90:32) */
+  int iconf_ss;             /* STO SS_CONFINED_ONLY (:33) */
+  int iorig_ss;             /* STO original specific storage formulation (:34) */
   /* The faces' conductivities follow hy_eff (gwf-npf.f90:2280-2355) with the
    * DIS connection normals: column- and row-direction faces K11 unless K22
    * is given, then the rotated tensor's value along the face normal

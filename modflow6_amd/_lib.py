@@ -95,7 +95,8 @@ class GwfModelC(C.Structure):
                 ("k22", C.POINTER(C.c_double)),
                 ("angle1", C.POINTER(C.c_double)),
                 ("angle2", C.POINTER(C.c_double)),
-                ("angle3", C.POINTER(C.c_double))]
+                ("angle3", C.POINTER(C.c_double)),
+                ("istor_coef", C.c_int), ("iconf_ss", C.c_int), ("iorig_ss", C.c_int)]
 
 
 class PtcC(C.Structure):

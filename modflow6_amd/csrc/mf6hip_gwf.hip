@@ -89,6 +89,30 @@ __device__ inline double sqsat_deriv(double top, double bot, double x, double ep
   return y / b;
 }
 
+// GwfStorageUtils.f90:32-96 SsTerms with STO's flags (iorig_ss: the original
+// formulation; iconf_ss: SS_CONFINED_ONLY); rho1 from SsCapacity
+// (:143-160: ss times the thickness, or times 1 with STORAGECOEFFICIENT)
+__device__ inline void ss_terms(int iconvert, int iorig_ss, int iconf_ss, double top,
+                                double bot, double rho1, double rho1old, double snnew,
+                                double snold, double hold, double &aterm, double &rhsterm) {
+  aterm = -rho1 * snnew;
+  rhsterm = 0.0;
+  if (iconvert != 0 && iorig_ss == 0) {
+    if (iconf_ss == 0) {
+      const double tthk = top - bot;
+      const double zold = bot + 0.5 * tthk * snold;
+      const double znew = bot + 0.5 * tthk * snnew;
+      rhsterm = -rho1old * snold * (hold - zold) - rho1 * snnew * znew;
+    } else {
+      if (snold == 1.0) rhsterm = rhsterm - rho1old * (hold - top);
+      if (snnew == 1.0) rhsterm = rhsterm - rho1 * top;
+      else aterm = 0.0;
+    }
+  } else {
+    rhsterm = -rho1old * snold * hold;
+  }
+}
+
 // SmoothingFunctions.f90:412-460 sQSaturation (cubic smoothing from bot
 // to top with coefficients c1 / b^3, c2 / b^2); the reference's x**3.0 and
 // x**2.0 compile to (x x) x and x x
@@ -182,6 +206,7 @@ struct GwfDev {
   double *sat;   // npf_cf's sat, written by the formulate (read by the budget)
   const int *icelltype, *iconvert, *ibound;
   int inewton, ivarcv, idewatcv, icellavg, has_sto;
+  int istor_coef, iconf_ss, iorig_ss;   // STO options
   int fn;   // add the Newton terms (npf_fn, sto_fn): gwf_fc inwtflag
   double satomega;
 };
@@ -513,19 +538,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, Gn
         snold = sqsat(tp, bt, xold[p], g.satomega);
         snnew = sqsat(tp, bt, hcell, g.satomega);
       }
-      double sc1 = g.ss[p] * (tp - bt) * g.area[p];
+      double sc1 = g.ss[p] * (g.istor_coef == 0 ? tp - bt : 1.0) * g.area[p];
       double rho1 = sc1 * tled;
       double rho1old = rho1;
-      double aterm = -rho1 * snnew;
-      double rhsterm = 0.0;
-      if (g.iconvert[p] != 0) {
-        double tthk = tp - bt;
-        double zold = bt + 0.5 * tthk * snold;
-        double znew = bt + 0.5 * tthk * snnew;
-        rhsterm = -rho1old * snold * (xold[p] - zold) - rho1 * snnew * znew;
-      } else {
-        rhsterm = -rho1old * snold * xold[p];
-      }
+      double aterm, rhsterm;
+      ss_terms(g.iconvert[p], g.iorig_ss, g.iconf_ss, tp, bt, rho1, rho1old, snnew, snold,
+               xold[p], aterm, rhsterm);
       diag = diag + aterm;
       b = b + rhsterm;
       if (g.iconvert[p] != 0) {
@@ -646,12 +664,16 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, Gn
         double tthk = tp - bt;
         double h = hcell;
         double snnew = sqsat(tp, bt, h, 1.0e-6);
-        double rho1 = g.ss[p] * (tp - bt) * g.area[p] * tled;
+        double rho1 = g.ss[p] * (g.istor_coef == 0 ? tp - bt : 1.0) * g.area[p] * tled;
         double rho2 = g.sy[p] * g.area[p] * tled;
         double derv = sqsat_deriv(tp, bt, h, 1.0e-6);
-        double drterm = -rho1 * derv * (h - bt) + rho1 * tthk * snnew * derv;
-        diag = diag + drterm;
-        b = b + drterm * h;
+        double drterm;
+        if (g.iconf_ss == 0) {   // gwf-sto.f90:414-423
+          if (g.iorig_ss == 0) drterm = -rho1 * derv * (h - bt) + rho1 * tthk * snnew * derv;
+          else drterm = -(rho1 * derv * h);
+          diag = diag + drterm;
+          b = b + drterm * h;
+        }
         if (snnew < 1.0) {
           if (snnew > 0.0) {
             double rterm = -rho2 * tthk * snnew;
@@ -1105,19 +1127,12 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
         snold = sqsat(tp, bt, xold[p], g.satomega);
         snnew = sqsat(tp, bt, hcell, g.satomega);
       }
-      const double sc1 = g.ss[p] * (tp - bt) * g.area[p];
+      const double sc1 = g.ss[p] * (g.istor_coef == 0 ? tp - bt : 1.0) * g.area[p];
       const double rho1 = sc1 * tled;
       const double rho1old = rho1;
-      const double aterm = -rho1 * snnew;
-      double rhsterm;
-      if (g.iconvert[p] != 0) {
-        const double tthk = tp - bt;
-        const double zold = bt + 0.5 * tthk * snold;
-        const double znew = bt + 0.5 * tthk * snnew;
-        rhsterm = -rho1old * snold * (xold[p] - zold) - rho1 * snnew * znew;
-      } else {
-        rhsterm = -rho1old * snold * xold[p];
-      }
+      double aterm, rhsterm;
+      ss_terms(g.iconvert[p], g.iorig_ss, g.iconf_ss, tp, bt, rho1, rho1old, snnew, snold,
+               xold[p], aterm, rhsterm);
       rss = aterm * hcell - rhsterm;
       diag = diag + rss;
       if (g.iconvert[p] != 0) {
@@ -1237,6 +1252,7 @@ static GwfDev make_dev(Mf6Handle *h) {
   g.icelltype = h->icelltype; g.iconvert = h->iconvert; g.ibound = h->ibound;
   g.inewton = h->m.inewton; g.ivarcv = h->m.ivarcv; g.idewatcv = h->m.idewatcv;
   g.icellavg = h->m.icellavg; g.has_sto = h->m.has_sto;
+  g.istor_coef = h->m.istor_coef; g.iconf_ss = h->m.iconf_ss; g.iorig_ss = h->m.iorig_ss;
   g.satomega = h->m.inewton ? 1.0e-6 : 0.0;   // NPF gwf-npf.f90:1394-1396, STO gwf-sto.f90:826-829
   g.fn = h->m.inewton;
   return g;

@@ -80,6 +80,9 @@ module mf6hip_module
     type(c_ptr) :: angle1
     type(c_ptr) :: angle2
     type(c_ptr) :: angle3
+    integer(c_int) :: istor_coef
+    integer(c_int) :: iconf_ss
+    integer(c_int) :: iorig_ss
   end type mf6hip_gwf_model
 
   !> mf6hip_ptc: pseudo-transient continuation control and state

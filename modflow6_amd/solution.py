@@ -89,6 +89,8 @@ class GwfSolution:
         m.k11, m.k33 = d(c.k11), d(c.k33)
         if c.k22 is not None:
             m.k22 = d(c.k22)
+        m.istor_coef, m.iconf_ss, m.iorig_ss = (int(c.istor_coef), int(c.iconf_ss),
+                                                int(c.iorig_ss))
         for nm in ("angle1", "angle2", "angle3"):
             if getattr(c, nm) is not None:
                 setattr(m, nm, d(getattr(c, nm)))

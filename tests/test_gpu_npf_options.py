@@ -6,7 +6,9 @@ VARIABLECV with and without DEWATERED, under NEWTON and without,
 ALTERNATIVE_CELL_AVERAGING LOGARITHMIC / AMT-LMK / AMT-HMK, K22 on the
 row-direction faces, and ANGLE1 (with ANGLE2 / ANGLE3) rotating the
 conductivity tensor (hyeff computed on the host at create, as the oracle
-does, so the faces match bit for bit): the formulated system and the budget's face flows bit for
+does, so the faces match bit for bit), and the STO options
+STORAGECOEFFICIENT, SS_CONFINED_ONLY and the original specific storage
+formulation (sto_fc / sto_fn / sto_cq, pinned with the same flags): the formulated system and the budget's face flows bit for
 bit (to 1e-14 relative where a logarithmic mean calls log()), and the heads
 of solved steps within 10 x dvclose."""
 import dataclasses
@@ -70,6 +72,12 @@ CASES = {
     "k22_newton": lambda: _k22(lambda: _convertible(True, 0, 0)),
     "angle1_confined": lambda: _angles(lambda: _averaging(0), False),
     "angles_newton": lambda: _angles(lambda: _convertible(True, 0, 0), True),
+    "sto_storagecoefficient": lambda: dataclasses.replace(_convertible(True, 0, 0),
+                                                          istor_coef=1),
+    "sto_confined_only": lambda: dataclasses.replace(_convertible(True, 0, 0), iconf_ss=1),
+    "sto_original": lambda: dataclasses.replace(_convertible(True, 0, 0), iorig_ss=1),
+    "sto_confined_only_plain": lambda: dataclasses.replace(_convertible(False, 0, 0),
+                                                           iconf_ss=1),
 }
 
 
@@ -91,11 +99,12 @@ def test_formulate_and_budget_bit_exact(name):
         gs.destroy()
     base = O.GwfSimulation(dataclasses.replace(case, ivarcv=0, idewatcv=0, icellavg=0,
                                                k22=None, angle1=None, angle2=None,
-                                               angle3=None))
+                                               angle3=None, istor_coef=0, iconf_ss=0,
+                                               iorig_ss=0))
     base.kper, base.kstp = 1, 1
     base.advance(perlen / nstp, steady)
     base.formulate(1)
-    assert not np.array_equal(base.amat, sim.amat)
+    assert not (np.array_equal(base.amat, sim.amat) and np.array_equal(base.rhs, sim.rhs))
     ob = sim.budget()
     if name in LOG_MEANS:
         # logarithmic means call log(): the device's and the host's libm
@@ -113,7 +122,8 @@ def test_formulate_and_budget_bit_exact(name):
 
 
 @pytest.mark.parametrize("name", ["varcv_dewatered_newton", "logarithmic", "k22_newton",
-                                  "angles_newton"])
+                                  "angles_newton", "sto_storagecoefficient",
+                                  "sto_confined_only"])
 def test_step_heads(name):
     case = CASES[name]()
     sim = O.GwfSimulation(case)
