@@ -147,7 +147,7 @@ typedef struct mf6hip_gwf_model {
   const double *bot;        /* nodes: cell bottom */
   const double *k11;        /* nodes: NPF K */
   const double *k33;        /* nodes: NPF K33 (K when not given) */
-  const int *icelltype;     /* nodes: NPF ICELLTYPE (after negative->1) */
+  const int *icelltype;     /* nodes: NPF ICELLTYPE as read (see ithickstrt) */
   const int *ibound;        /* nodes: IDOMAIN/ibound before CHD (1 or 0) */
   int inewton;              /* GWF NEWTON */
   int inewtonur;            /* GWF NEWTON UNDER_RELAXATION */
@@ -167,6 +167,11 @@ typedef struct mf6hip_gwf_model {
   int istor_coef;           /* STO STORAGECOEFFICIENT (gwf-sto.f90:32) */
   int iconf_ss;             /* STO SS_CONFINED_ONLY (:33) */
   int iorig_ss;             /* STO original specific storage formulation (:34) */
+  int ithickstrt;           /* NPF THICKSTRT: a negative ICELLTYPE marks a confined
+                               cell with the saturated thickness of its starting
+                               head (gwf-npf.f90:1836-1886); without it a
+                               negative ICELLTYPE means convertible */
+  const double *strt;       /* nodes: IC STRT (needed with THICKSTRT) */
   /* The faces' conductivities follow hy_eff (gwf-npf.f90:2280-2355) with the
    * DIS connection normals: column- and row-direction faces K11 unless K22
    * is given, then the rotated tensor's value along the face normal

@@ -96,7 +96,8 @@ class GwfModelC(C.Structure):
                 ("angle1", C.POINTER(C.c_double)),
                 ("angle2", C.POINTER(C.c_double)),
                 ("angle3", C.POINTER(C.c_double)),
-                ("istor_coef", C.c_int), ("iconf_ss", C.c_int), ("iorig_ss", C.c_int)]
+                ("istor_coef", C.c_int), ("iconf_ss", C.c_int), ("iorig_ss", C.c_int),
+                ("ithickstrt", C.c_int), ("strt", C.POINTER(C.c_double))]
 
 
 class PtcC(C.Structure):

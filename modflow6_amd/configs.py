@@ -147,6 +147,10 @@ class GwfCase:
     istor_coef: int = 0
     iconf_ss: int = 0
     iorig_ss: int = 0
+    # NPF THICKSTRT: cells with a negative ICELLTYPE are confined with the
+    # saturated thickness of their starting head (without it a negative
+    # ICELLTYPE means convertible)
+    thickstrt: bool = False
     # NPF K22 (nodes,): the conductivity of row-direction (i) faces; None:
     # K11.  ANGLE1..3 (nodes,), degrees as read: rotate the K tensor (K22
     # given: horizontal faces; ANGLE2 given: vertical faces)

@@ -54,7 +54,7 @@ static int galloc(T **p, size_t n) {
 // stateless helpers (reference statement order)
 // ---------------------------------------------------------------------------
 
-__device__ inline double sqsat(double top, double bot, double x, double eps) {
+__host__ __device__ inline double sqsat(double top, double bot, double x, double eps) {
   double teps = eps, y;
   double b = top - bot;
   if (b > 0.0) {
@@ -204,6 +204,7 @@ struct GwfDev {
   Layout L;
   const double *top, *bot, *area, *k11, *k22, *k33, *ss, *sy, *delr, *delc, *condsat;
   double *sat;   // npf_cf's sat, written by the formulate (read by the budget)
+  const double *sat0, *hn0;   // calc_condsat's initial sat and heads (THICKSTRT)
   const int *icelltype, *iconvert, *ibound;
   int inewton, ivarcv, idewatcv, icellavg, has_sto;
   int istor_coef, iconf_ss, iorig_ss;   // STO options
@@ -296,12 +297,16 @@ __global__ void k_condsat(GwfDev g, double *__restrict__ condsat) {
         int pb = cell_pos(L, kb, ib, jb);
         Conn cn = conn_geom(g, dir, p, pb, i, j);
         double topn = g.top[p], botn = g.bot[p], topm = g.top[pb], botm = g.bot[pb];
+        // calc_initial_sat and THICKSTRT's heads (gwf-npf.f90:1997-2006):
+        // 1 and the cell top except for THICKSTRT cells
+        const double satn = g.sat0[p], satm = g.sat0[pb];
         if (cn.ihc == 0) {
-          cs = vcond(1, 1, 1, 1, 1, 1, 1.0, botn, botm, g.k33[p], g.k33[pb], 1.0, 1.0, topn,
+          cs = vcond(1, 1, 1, 1, 1, 1, 1.0, botn, botm, g.k33[p], g.k33[pb], satn, satm, topn,
                      topm, botn, botm, cn.hwva);
         } else {
-          cs = hcond(1, 1, 1, 1, 0, g.icellavg, 1.0, topn, topm, 1.0, 1.0, hk(g, dir, p),
-                     hk(g, dir, pb), topn, topm, botn, botm, cn.cl1, cn.cl2, cn.hwva);
+          cs = hcond(1, 1, 1, 1, 0, g.icellavg, 1.0, g.hn0[p], g.hn0[pb], satn, satm,
+                     hk(g, dir, p), hk(g, dir, pb), topn, topm, botn, botm, cn.cl1, cn.cl2,
+                     cn.hwva);
         }
       }
       condsat[dir * L.NP + p] = cs;
@@ -1249,6 +1254,7 @@ static GwfDev make_dev(Mf6Handle *h) {
   g.k33 = h->k33;
   g.ss = h->ss; g.sy = h->sy; g.delr = h->delr; g.delc = h->delc; g.condsat = h->condsat;
   g.sat = h->sat;
+  g.sat0 = h->sat0; g.hn0 = h->hn0;
   g.icelltype = h->icelltype; g.iconvert = h->iconvert; g.ibound = h->ibound;
   g.inewton = h->m.inewton; g.ivarcv = h->m.ivarcv; g.idewatcv = h->m.idewatcv;
   g.icellavg = h->m.icellavg; g.has_sto = h->m.has_sto;
@@ -1349,8 +1355,11 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
                      "(AMT-HMK)");
     return MF6HIP_EINVAL;
   }
+  if (m->ithickstrt && !m->strt) {
+    mf6hip_set_error("THICKSTRT needs the starting heads (strt)");
+    return MF6HIP_EINVAL;
+  }
   h->m = *m;
-  h->h_icelltype.assign(m->icelltype, m->icelltype + L.N);
   CK(upload_pos(h, &h->top, m->top));
   CK(upload_pos(h, &h->bot, m->bot));
   {
@@ -1360,7 +1369,29 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
     CK(upload_pos(h, &h->k22, ky.data()));
     CK(upload_pos(h, &h->k33, kz.data()));
   }
-  CK(upload_pos(h, &h->icelltype, m->icelltype));
+  {
+    // THICKSTRT (gwf-npf.f90:1836-1886): cells entered with a negative
+    // ICELLTYPE become confined, with the saturated thickness of their
+    // starting head in calc_condsat; without it a negative ICELLTYPE means
+    // convertible.  Every cell is active here (the model's ibound is 1 or 0
+    // only through IDOMAIN, and THICKSTRT cells with IDOMAIN 0 keep sat 1).
+    std::vector<int> ict(m->icelltype, m->icelltype + L.N);
+    std::vector<double> sat0(L.N, 1.0), hn0(m->top, m->top + L.N);
+    for (int n = 0; n < L.N; n++) {
+      if (ict[n] >= 0) continue;
+      if (!m->ithickstrt) { ict[n] = 1; continue; }
+      ict[n] = 0;
+      hn0[n] = m->strt[n];
+      if (m->ibound[n] == 0) continue;
+      const double tp = m->top[n], bt = m->bot[n], hs = m->strt[n];
+      if (m->inewton) sat0[n] = sqsat(tp, bt, hs, 1.0e-6);   // thksat :775-794
+      else sat0[n] = hs >= tp ? 1.0 : (hs - bt) / (tp - bt);
+    }
+    h->h_icelltype = ict;
+    CK(upload_pos(h, &h->icelltype, ict.data()));
+    CK(upload_pos(h, &h->sat0, sat0.data()));
+    CK(upload_pos(h, &h->hn0, hn0.data()));
+  }
   CK(upload_pos(h, &h->ibound0, m->ibound));
   {
     std::vector<double> area(L.N);

@@ -94,6 +94,7 @@ struct Mf6Handle {
   bool has_model = false;
   mf6hip_gwf_model m{};
   double *top = nullptr, *bot = nullptr, *area = nullptr, *k11 = nullptr, *k22 = nullptr,
+         *sat0 = nullptr, *hn0 = nullptr,
          *k33 = nullptr, *ss = nullptr, *sy = nullptr;
   double *delr = nullptr, *delc = nullptr;
   int *icelltype = nullptr, *iconvert = nullptr;

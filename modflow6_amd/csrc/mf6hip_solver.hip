@@ -1779,7 +1779,7 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->qhat, h->A32, h->invpiv32, h->invpiv16, h->r32, h->e32, h->p32,
                   h->p232, h->q32, h->z32, h->zf32, h->d_part, h->d_mpart, h->d_state,
                   h->d_hist, h->d_hloc, h->d_tmp_nat, h->d_amat_csr, h->top, h->bot,
-                  h->area, h->k11, h->k22, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
+                  h->area, h->k11, h->k22, h->sat0, h->hn0, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,
                   h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax, h->sat,

@@ -83,6 +83,8 @@ module mf6hip_module
     integer(c_int) :: istor_coef
     integer(c_int) :: iconf_ss
     integer(c_int) :: iorig_ss
+    integer(c_int) :: ithickstrt
+    type(c_ptr) :: strt
   end type mf6hip_gwf_model
 
   !> mf6hip_ptc: pseudo-transient continuation control and state

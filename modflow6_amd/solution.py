@@ -91,6 +91,9 @@ class GwfSolution:
             m.k22 = d(c.k22)
         m.istor_coef, m.iconf_ss, m.iorig_ss = (int(c.istor_coef), int(c.iconf_ss),
                                                 int(c.iorig_ss))
+        if c.thickstrt:
+            m.ithickstrt = 1
+            m.strt = d(c.strt)
         for nm in ("angle1", "angle2", "angle3"):
             if getattr(c, nm) is not None:
                 setattr(m, nm, d(getattr(c, nm)))

@@ -780,7 +780,8 @@ void orc_calc_condsat(int nodes, const int *ia, const int *ja, const int *jas,
                       const int *ihc, const double *cl1, const double *cl2,
                       const double *hwva, const double *top, const double *bot,
                       const double *k11, const double *k22, int ncol,
-                      const double *k33, int icellavg, double *condsat)
+                      const double *k33, int icellavg, const double *sat0,
+                      const double *hn0, double *condsat)
 {
   for (int node = 0; node < nodes; node++) {
     for (int ii = ia[node] + 1; ii < ia[node + 1]; ii++) {
@@ -788,12 +789,14 @@ void orc_calc_condsat(int nodes, const int *ia, const int *ja, const int *jas,
       if (m < node) continue;
       int n = node;
       int jj = jas[ii];
-      double satn = DONE, satm = DONE;
+      /* calc_initial_sat and THICKSTRT's heads (:1997-2006): sat0 / hn0
+       * are 1 and the cell top except for THICKSTRT cells */
+      double satn = sat0[n], satm = sat0[m];
       double topn = top[n], botn = bot[n], topm = top[m], botm = bot[m];
       int ihcj = ihc[jj];
       double hyn = ihcj == 0 ? k33[n] : (abs(m - n) == ncol ? k22[n] : k11[n]);
       double hym = ihcj == 0 ? k33[m] : (abs(m - n) == ncol ? k22[m] : k11[m]);
-      double hn = topn, hm = topm;
+      double hn = hn0[n], hm = hn0[m];
       double csat;
       if (ihcj == 0) {
         csat = orc_vcond(1, 1, 1, 1, 0, 1, 1, DONE, botn, botm, hyn, hym,

@@ -96,7 +96,8 @@ def lib():
                                   C.POINTER(d), C.POINTER(d)]
         L.orc_syterms.argtypes = [d] * 6 + [C.POINTER(d), C.POINTER(d)]
         L.orc_calc_condsat.argtypes = [i, i32p, i32p, i32p, i32p, f64p, f64p,
-                                       f64p, f64p, f64p, f64p, f64p, i, f64p, i, f64p]
+                                       f64p, f64p, f64p, f64p, f64p, i, f64p, i, f64p,
+                                       f64p, f64p]
         L.orc_hfb_condsat_modify.argtypes = [i, i32p, i32p, i32p, f64p, i, i32p, i32p,
                                              i32p, f64p, f64p, f64p, f64p, f64p]
         L.orc_hfb_condsat_reset.argtypes = [i, i32p, i32p, f64p, f64p]
@@ -611,10 +612,30 @@ class GwfSimulation:
         self.ivarcv = int(c.ivarcv)
         self.idewatcv = int(c.idewatcv)
         self.icellavg = int(c.icellavg)
+        # THICKSTRT (gwf-npf.f90:1836-1886, calc_initial_sat :2049-2058):
+        # cells entered with a negative ICELLTYPE become confined with the
+        # saturated thickness of their starting head; without the option a
+        # negative ICELLTYPE means convertible
+        ic = np.asarray(c.icelltype, dtype=np.int32)
+        flag = (ic < 0) & bool(c.thickstrt)
+        self.icelltype = np.ascontiguousarray(np.where(ic < 0, 0 if c.thickstrt else 1, ic),
+                                              dtype=np.int32)
+        strt = np.asarray(c.strt, dtype=np.float64)
+        sat0 = np.ones(self.neq)
+        hn0 = np.ascontiguousarray(d.top, dtype=np.float64).copy()
+        for n in np.flatnonzero(flag):
+            hn0[n] = strt[n]
+            if self.inewton:
+                sat0[n] = lib().orc_sqsat(d.top[n], d.bot[n], strt[n], self.satomega)
+            elif strt[n] >= d.top[n]:
+                sat0[n] = 1.0
+            else:
+                sat0[n] = (strt[n] - d.bot[n]) / (d.top[n] - d.bot[n])
         self.condsat = np.zeros(d.hwva.size)
         lib().orc_calc_condsat(self.neq, d.ia, d.ja, d.jas, d.ihc, d.cl1,
                                d.cl2, d.hwva, d.top, d.bot, self.k11,
-                               self.k22, d.ncol, self.k33, self.icellavg, self.condsat)
+                               self.k22, d.ncol, self.k33, self.icellavg, sat0, hn0,
+                               self.condsat)
         self.hfb = None
         if c.hfb is not None:
             self.set_hfb(*c.hfb)

@@ -8,7 +8,8 @@ row-direction faces, and ANGLE1 (with ANGLE2 / ANGLE3) rotating the
 conductivity tensor (hyeff computed on the host at create, as the oracle
 does, so the faces match bit for bit), and the STO options
 STORAGECOEFFICIENT, SS_CONFINED_ONLY and the original specific storage
-formulation (sto_fc / sto_fn / sto_cq, pinned with the same flags): the formulated system and the budget's face flows bit for
+formulation (sto_fc / sto_fn / sto_cq, pinned with the same flags), and
+THICKSTRT: the formulated system and the budget's face flows bit for
 bit (to 1e-14 relative where a logarithmic mean calls log()), and the heads
 of solved steps within 10 x dvclose."""
 import dataclasses
@@ -60,6 +61,16 @@ def _angles(base, vertical, seed=9):
                                angle3=rng.uniform(-30.0, 30.0, c.nodes))
 
 
+def _thickstrt(newton, seed=10):
+    """A third of the cells entered with ICELLTYPE -1 under THICKSTRT:
+    confined, with the saturated thickness of their starting head."""
+    c = _convertible(newton, 0, 0, seed)
+    rng = np.random.default_rng(seed)
+    ic = np.asarray(c.icelltype).copy()
+    ic[rng.random(c.nodes) < 0.33] = -1
+    return dataclasses.replace(c, icelltype=ic, thickstrt=True)
+
+
 CASES = {
     "varcv_newton": lambda: _convertible(True, 1, 0),
     "varcv_dewatered_newton": lambda: _convertible(True, 1, 1),
@@ -78,6 +89,8 @@ CASES = {
     "sto_original": lambda: dataclasses.replace(_convertible(True, 0, 0), iorig_ss=1),
     "sto_confined_only_plain": lambda: dataclasses.replace(_convertible(False, 0, 0),
                                                            iconf_ss=1),
+    "thickstrt_newton": lambda: _thickstrt(True),
+    "thickstrt_plain": lambda: _thickstrt(False),
 }
 
 
@@ -100,7 +113,7 @@ def test_formulate_and_budget_bit_exact(name):
     base = O.GwfSimulation(dataclasses.replace(case, ivarcv=0, idewatcv=0, icellavg=0,
                                                k22=None, angle1=None, angle2=None,
                                                angle3=None, istor_coef=0, iconf_ss=0,
-                                               iorig_ss=0))
+                                               iorig_ss=0, thickstrt=False))
     base.kper, base.kstp = 1, 1
     base.advance(perlen / nstp, steady)
     base.formulate(1)
@@ -123,7 +136,7 @@ def test_formulate_and_budget_bit_exact(name):
 
 @pytest.mark.parametrize("name", ["varcv_dewatered_newton", "logarithmic", "k22_newton",
                                   "angles_newton", "sto_storagecoefficient",
-                                  "sto_confined_only"])
+                                  "sto_confined_only", "thickstrt_newton"])
 def test_step_heads(name):
     case = CASES[name]()
     sim = O.GwfSimulation(case)

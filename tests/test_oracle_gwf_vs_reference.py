@@ -83,7 +83,7 @@ class Grid:
         self.condsat = np.zeros(d.hwva.size)
         L.orc_calc_condsat(n, d.ia, d.ja, d.jas, d.ihc, d.cl1, d.cl2, d.hwva,
                            d.top, d.bot, self.k11, self.k11, d.ncol, self.k33, 0,
-                           self.condsat)
+                           np.ones(n), np.ascontiguousarray(d.top), self.condsat)
         self.sat = np.ascontiguousarray(rng.uniform(0.0, 1.0, n))
         self.amat0 = rng.standard_normal(d.nja)
         self.rhs0 = rng.standard_normal(n)

@@ -73,7 +73,8 @@ def test_hfb_combination_matches_reference_hfb_fc(seed):
     k11 = np.ascontiguousarray(case.k11, dtype=np.float64)
     condsat = np.zeros(d.hwva.size)
     L.orc_calc_condsat(n, d.ia, d.ja, d.jas, d.ihc, d.cl1, d.cl2, d.hwva, d.top, d.bot,
-                       k11, k11, d.ncol, 0.1 * k11, 0, condsat)
+                       k11, k11, d.ncol, 0.1 * k11, 0, np.ones(n), np.ascontiguousarray(d.top),
+                       condsat)
     noden, nodem, idx, hy = _barriers(d, rng, 6)
     # the matrix hfb_fc starts from: npf conductances off the diagonal
     rows = np.repeat(np.arange(n), np.diff(d.ia))

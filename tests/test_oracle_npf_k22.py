@@ -18,7 +18,8 @@ def _faces(nlay, nrow, ncol, delr, delc, top2d, botm, k11, k22, k33):
     cs = np.zeros(d.hwva.size)
     L.orc_calc_condsat(d.nodes, d.ia, d.ja, d.jas, d.ihc, d.cl1, d.cl2, d.hwva, d.top, d.bot,
                        np.ascontiguousarray(k11), np.ascontiguousarray(k22), ncol,
-                       np.ascontiguousarray(k33), 0, cs)
+                       np.ascontiguousarray(k33), 0, np.ones(d.nodes),
+                       np.ascontiguousarray(d.top), cs)
     out = {}
     for n in range(d.nodes):
         for ii in range(d.ia[n] + 1, d.ia[n + 1]):
@@ -100,3 +101,30 @@ def test_hyeff_matches_reference():
                 assert np.float64(a).view(np.int64) == np.float64(b).view(np.int64)
     # no rotation: the axes give K11, K22, K33 exactly
     assert L.orc_hyeff(2.0, 3.0, 5.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0) == 3.0
+
+
+def _thick_line(thickstrt, newton=False):
+    from modflow6_amd.configs import GwfCase, ImsOptions, Package
+    ncol = 6
+    return GwfCase(
+        name="line_thickstrt", nlay=1, nrow=1, ncol=ncol, delr=np.full(ncol, 10.0),
+        delc=np.full(1, 20.0), top2d=np.zeros((1, ncol)), botm=np.full((1, 1, ncol), -10.0),
+        k11=np.full(ncol, 3.0), k33=np.full(ncol, 3.0),
+        icelltype=np.full(ncol, -1, dtype=np.int32), strt=np.full(ncol, -4.0),
+        packages=[Package("CHD", np.array([0, ncol - 1]), np.array([-4.0, -5.0]))],
+        ims=ImsOptions(), periods=[(1.0, 1, True)], newton=newton, thickstrt=thickstrt)
+
+
+def test_thickstrt_closed_form():
+    """THICKSTRT (gwf-npf.f90:1836-1886): ICELLTYPE -1 cells become confined
+    with the saturated thickness of their starting head, here 6 of 10 m, so
+    each face conducts K w b / L = 3 * 20 * 6 / 10; without the option they
+    are convertible and calc_condsat uses the full 10 m."""
+    s = O.GwfSimulation(_thick_line(True))
+    assert np.all(s.icelltype == 0)
+    h = s.dis.ihc[s.dis.jas] != 0
+    assert np.allclose(s.condsat, 3.0 * 20.0 * 6.0 / 10.0, rtol=1e-15)
+    s2 = O.GwfSimulation(_thick_line(False))
+    assert np.all(s2.icelltype == 1)
+    assert np.allclose(s2.condsat, 3.0 * 20.0 * 10.0 / 10.0, rtol=1e-15)
+    assert h.any()
