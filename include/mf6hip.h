@@ -240,11 +240,12 @@ int mf6hip_gwf_set_package_ex(void *handle, int pkgtype, int nbound,
  * series with the face's conductance; <= 0: -hydchr multiplies it).  Under
  * NEWTON, or when both cells are confined, the barrier is folded into the
  * face's saturated conductance, as the reference does; every later
- * formulate and budget sees it.  nhfb = 0 removes the barriers.
- * MF6HIP_EINVAL for cells that are not horizontally connected (check_data,
- * :711-766); MF6HIP_EUNSUPPORTED for a barrier next to a convertible cell in
- * a model without NEWTON (the reference applies those in hfb_fc at every
- * formulate, :211-348, which the device path does not model). */
+ * formulate and budget sees it.  Without NEWTON a barrier next to a
+ * convertible cell is applied at every formulate as hfb_fc does (:211-348,
+ * the face's conductance combined with the barrier over the cells' wetted
+ * thickness) and in the budget as hfb_cq (:355-450).  nhfb = 0 removes the
+ * barriers.  MF6HIP_EINVAL for cells that are not horizontally connected
+ * (check_data, :711-766). */
 int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, const int *nodem,
                        const double *hydchr);
 

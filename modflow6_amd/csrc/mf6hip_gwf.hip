@@ -442,6 +442,14 @@ struct BndDev {
   PkgDev pk[5];
 };
 
+// Barriers that hfb_fc / hfb_cq apply (the handle's d_hfbd_* arrays).
+struct HfbDev {
+  const int *off, *ent;     // per position: b * 2 + role (role 1: the row is m)
+  const int *np, *mp, *s;   // positions of n and m; direction index of m from n
+  const double *hy, *w;     // hydchr, face width
+  double *sav;              // hfb_fc's condsav (written by the row of n)
+};
+
 // The model's explicit ghost-node corrections as the kernels see them
 // (the handle's d_gnc_* arrays).
 struct GncDev {
@@ -454,7 +462,8 @@ struct GncDev {
 };
 
 // sln_buildsystem for one model: one thread per row
-__global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, GncDev G, int iss,
+__global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, GncDev G, HfbDev F,
+                                                         int iss,
                                                          double delt,
                                                          const double *__restrict__ x,
                                                          const double *__restrict__ xold,
@@ -506,6 +515,35 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_formulate(GwfDev g, BndDev bd, Gn
       }
       off[s] = off[s] + cond;
       diag = diag + (-cond);
+    }
+    // --- hfb_fc without NEWTON (gwf-hfb.f90:211-348): each barrier next to
+    // a convertible cell, in barrier order, replaces the face's conductance
+    // aterm by its series combination with the barrier (wetted thickness:
+    // tops clipped at the heads of convertible cells) and adds aterm - cond
+    // to the diagonal
+    for (int e = F.off ? F.off[p] : 0, e1 = F.off ? F.off[p + 1] : 0; e < e1; e++) {
+      const int code = F.ent[e];
+      const int bi = code >> 1, role = code & 1;
+      const int pn = F.np[bi], pm = F.mp[bi];
+      if (g.ibound[pn] == 0 || g.ibound[pm] == 0) continue;
+      if (g.icelltype[pn] != 1 && g.icelltype[pm] != 1) continue;
+      const int s = role == 0 ? F.s[bi] : 5 - F.s[bi];
+      const double aterm = off[s];
+      double topn = g.top[pn], topm = g.top[pm];
+      const double botn = g.bot[pn], botm = g.bot[pm];
+      if (g.icelltype[pn] == 1 && x[pn] < topn) topn = x[pn];
+      if (g.icelltype[pm] == 1 && x[pm] < topm) topm = x[pm];
+      const double faheight = 0.5 * ((topn - botn) + (topm - botm));
+      double cond;
+      if (F.hy[bi] > 0.0) {
+        const double condhfb = F.hy[bi] * 1.0 * F.w[bi] * faheight;
+        cond = aterm * condhfb / (aterm + condhfb);
+      } else {
+        cond = -aterm * F.hy[bi];
+      }
+      if (role == 0) F.sav[bi] = cond;
+      diag = diag + (aterm - cond);
+      off[s] = cond;
     }
     double b = 0.0;
     // --- gnc_fc, explicit (GhostNode.f90:280-324), with gnc_fmsav
@@ -1045,7 +1083,8 @@ struct BndRates {
 // (csr_diagsum) and, for a CHD cell, its CHD rate (calc_chd_rate: minus the
 // inflow from its neighbours).  strg receives the STO-SS / STO-SY rates.
 __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
-    GwfDev g, BndRates br, GncDev G, const int *__restrict__ ia, const int *__restrict__ chd_cell,
+    GwfDev g, BndRates br, GncDev G, HfbDev F, const int *__restrict__ ia,
+    const int *__restrict__ chd_cell,
     int iss, double delt, const double *__restrict__ x, const double *__restrict__ xold,
     double *__restrict__ flowja, double *__restrict__ strg) {
   const Layout &L = g.L;
@@ -1090,6 +1129,19 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_budget_cells(
                                           g.sat[m], g.condsat[dir * N + p]);
         q[s] = cond * (x[m] - hcell);
       }
+    }
+    // --- hfb_cq without NEWTON (gwf-hfb.f90:355-450): across a barrier next
+    // to a convertible cell the flow is hfb_fc's condsav times the head
+    // difference (set, in barrier order)
+    for (int e = F.off ? F.off[p] : 0, e1 = F.off ? F.off[p + 1] : 0; e < e1; e++) {
+      const int code = F.ent[e];
+      const int bi = code >> 1, role = code & 1;
+      const int pn = F.np[bi], pm = F.mp[bi];
+      if (g.ibound[pn] == 0 || g.ibound[pm] == 0) continue;
+      if (g.icelltype[pn] != 1 && g.icelltype[pm] != 1) continue;
+      const double qnm = F.sav[bi] * (x[pm] - x[pn]);
+      if (role == 0) q[F.s[bi]] = qnm;
+      else q[5 - F.s[bi]] = -qnm;
     }
     // --- gnc_cq (GhostNode.f90:478-503, deltaQgnc :507-543): the
     // correction flow with the conductance saved at the last formulate,
@@ -1410,6 +1462,8 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
   CK(galloc(&h->delc, (size_t)L.nrow));
   HIPCHK(hipMemcpy(h->delr, m->delr, sizeof(double) * L.ncol, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->delc, m->delc, sizeof(double) * L.nrow, hipMemcpyHostToDevice));
+  h->h_delr.assign(m->delr, m->delr + L.ncol);
+  h->h_delc.assign(m->delc, m->delc + L.nrow);
   CK(galloc(&h->ibound, (size_t)L.NP));
   HIPCHK(hipMemcpy(h->ibound, h->ibound0, sizeof(int) * L.NP, hipMemcpyDeviceToDevice));
   CK(galloc(&h->condsat, 3 * (size_t)L.NP));
@@ -1449,6 +1503,77 @@ extern "C" int mf6hip_gwf_create(void *handle, const mf6hip_gwf_model *m) {
 // cells must be connected, horizontally.  Without NEWTON a barrier next to a
 // convertible cell is applied by hfb_fc at every formulate instead
 // (gwf-hfb.f90:211-348), which the device path does not model: rejected.
+static void hfbd_free(Mf6Handle *h) {
+  void *bufs[] = {h->d_hfbd_np, h->d_hfbd_mp, h->d_hfbd_s, h->d_hfbd_hy, h->d_hfbd_w,
+                  h->d_hfbd_sav, h->d_hfbd_off, h->d_hfbd_ent};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  h->d_hfbd_np = h->d_hfbd_mp = h->d_hfbd_s = h->d_hfbd_off = h->d_hfbd_ent = nullptr;
+  h->d_hfbd_hy = h->d_hfbd_w = h->d_hfbd_sav = nullptr;
+  h->hfbd_n = 0;
+}
+
+// The barriers (list positions dn of the new set) that hfb_fc / hfb_cq
+// apply: per barrier the positions of n and m, the direction index of m
+// seen from n, hydchr and the face width; per position the barriers of its
+// row (b * 2 + role, role 1: the row is m) in barrier order.
+static int hfb_dynamic(Mf6Handle *h, int nhfb, const std::vector<int> &nn,
+                       const std::vector<int> &mm, const double *hydchr,
+                       const std::vector<int> &dn) {
+  (void)nhfb;
+  const Layout &L = h->L;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  hfbd_free(h);
+  const int nd = (int)dn.size();
+  if (nd == 0) return MF6HIP_OK;
+  std::vector<int> np(nd), mp(nd), sd(nd), cnt(L.NP + 1, 0);
+  std::vector<double> hy(nd), w(nd);
+  for (int q = 0; q < nd; q++) {
+    const int n = nn[dn[q]], m = mm[dn[q]];
+    const int in = (n % L.NRC) / L.ncol, jn = (n % L.NRC) % L.ncol;
+    const int im = (m % L.NRC) / L.ncol, jm = (m % L.NRC) % L.ncol;
+    int dir;   // back, left, right, front
+    if (im == in - 1) dir = 1;
+    else if (jm == jn - 1) dir = 2;
+    else if (jm == jn + 1) dir = 3;
+    else dir = 4;
+    np[q] = pos_of_natural(L, n);
+    mp[q] = pos_of_natural(L, m);
+    sd[q] = dir;
+    hy[q] = hydchr[dn[q]];
+    w[q] = (dir == 2 || dir == 3) ? h->h_delc[in] : h->h_delr[jn];   // the face width
+    cnt[np[q] + 1]++;
+    cnt[mp[q] + 1]++;
+  }
+  for (int p = 0; p < L.NP; p++) cnt[p + 1] += cnt[p];
+  std::vector<int> ent(2 * (size_t)nd);
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int q = 0; q < nd; q++) {
+      ent[fill[np[q]]++] = q * 2;
+      ent[fill[mp[q]]++] = q * 2 + 1;
+    }
+  }
+  CK(galloc(&h->d_hfbd_np, (size_t)nd));
+  CK(galloc(&h->d_hfbd_mp, (size_t)nd));
+  CK(galloc(&h->d_hfbd_s, (size_t)nd));
+  CK(galloc(&h->d_hfbd_hy, (size_t)nd));
+  CK(galloc(&h->d_hfbd_w, (size_t)nd));
+  CK(galloc(&h->d_hfbd_sav, (size_t)nd));
+  CK(galloc(&h->d_hfbd_off, (size_t)L.NP + 1));
+  CK(galloc(&h->d_hfbd_ent, 2 * (size_t)nd));
+  HIPCHK(hipMemcpy(h->d_hfbd_np, np.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_hfbd_mp, mp.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_hfbd_s, sd.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_hfbd_hy, hy.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_hfbd_w, w.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(h->d_hfbd_sav, 0, sizeof(double) * nd));
+  HIPCHK(hipMemcpy(h->d_hfbd_off, cnt.data(), sizeof(int) * (L.NP + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_hfbd_ent, ent.data(), sizeof(int) * 2 * nd, hipMemcpyHostToDevice));
+  h->hfbd_n = nd;
+  return MF6HIP_OK;
+}
+
 extern "C" int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, const int *nodem,
                                   const double *hydchr) {
   Mf6Handle *h = (Mf6Handle *)handle;
@@ -1478,16 +1603,30 @@ extern "C" int mf6hip_gwf_set_hfb(void *handle, int nhfb, const int *noden, cons
                        std::to_string(n + 1) + " and " + std::to_string(m + 1));
       return MF6HIP_EINVAL;
     }
-    if (!h->m.inewton && (h->h_icelltype[n] != 0 || h->h_icelltype[m] != 0)) {
-      mf6hip_set_error("HFB no. " + std::to_string(b + 1) +
-                       ": a barrier next to a convertible cell without NEWTON is applied by "
-                       "hfb_fc at every formulate (gwf-hfb.f90:211-348), which the device "
-                       "path does not model");
-      return MF6HIP_EUNSUPPORTED;
-    }
     nn[b] = n;
     mm[b] = m;
   }
+  // Without NEWTON a barrier next to a convertible cell (ICELLTYPE 1) is
+  // applied by hfb_fc at every formulate and hfb_cq in the budget
+  // (gwf-hfb.f90:211-450); the others are folded into condsat as
+  // condsat_modify does (:789-832).  Faces hold barriers of one kind only.
+  std::vector<int> fn, fm, dn;
+  std::vector<double> fh;
+  for (int b = 0; b < nhfb; b++) {
+    const bool dyn = !h->m.inewton && (h->h_icelltype[nn[b]] == 1 || h->h_icelltype[mm[b]] == 1);
+    if (dyn) {
+      dn.push_back(b);
+    } else {
+      fn.push_back(nn[b]);
+      fm.push_back(mm[b]);
+      fh.push_back(hydchr[b]);
+    }
+  }
+  CK(hfb_dynamic(h, nhfb, nn, mm, hydchr, dn));
+  nhfb = (int)fn.size();
+  nn = fn;
+  mm = fm;
+  hydchr = fh.data();
   hipStream_t s = h->stream;
   if (h->hfb_n > 0) k_hfb_reset<<<1, 64, 0, s>>>(h->hfb_n, h->d_hfb_face, h->d_hfb_sav, h->condsat);
   HIPCHK(hipGetLastError());
@@ -1728,6 +1867,21 @@ extern "C" int mf6hip_gwf_advance(void *handle, int iss, double delt) {
   return MF6HIP_OK;
 }
 
+// the kernels' view of the barriers hfb_fc / hfb_cq apply (none: null
+// tables)
+static HfbDev hfb_dev(const Mf6Handle *h) {
+  HfbDev F;
+  F.off = h->hfbd_n > 0 ? h->d_hfbd_off : nullptr;
+  F.ent = h->d_hfbd_ent;
+  F.np = h->d_hfbd_np;
+  F.mp = h->d_hfbd_mp;
+  F.s = h->d_hfbd_s;
+  F.hy = h->d_hfbd_hy;
+  F.w = h->d_hfbd_w;
+  F.sav = h->d_hfbd_sav;
+  return F;
+}
+
 // the kernels' view of the model's ghost-node corrections (none: null
 // tables)
 static GncDev gnc_dev(const Mf6Handle *h) {
@@ -1895,7 +2049,7 @@ extern "C" int mf6hip_gwf_formulate(void *handle, int kiter, int inewton, double
   }
   GwfDev g = make_dev(h);
   g.fn = h->m.inewton && inewton;
-  k_formulate<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, bd, gnc_dev(h), h->iss, h->delt, h->x,
+  k_formulate<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, bd, gnc_dev(h), hfb_dev(h), h->iss, h->delt, h->x,
                                                h->xold, h->A, h->b);
   HIPCHK(hipGetLastError());
   if (amat_out || rhs_out) CK(mf6hip_copy_system(h, amat_out, rhs_out));
@@ -2173,7 +2327,8 @@ extern "C" int mf6hip_gwf_budget(void *handle, double *flowja_out, double *terms
   if (chd.nbound > 0)
     k_mark_cells<<<ggrid(chd.nbound), MF6_BLOCK, 0, s>>>(chd.nbound, chd.nodes, L, h->chd_flag);
   GwfDev g = make_dev(h);
-  k_budget_cells<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, br, gnc_dev(h), h->d_ia, h->chd_flag,
+  k_budget_cells<<<ggrid(L.N), MF6_BLOCK, 0, s>>>(g, br, gnc_dev(h), hfb_dev(h), h->d_ia,
+                                                  h->chd_flag,
                                                   h->iss, h->delt, h->x, h->xold, h->flowja,
                                                   h->strg);
   if (chd.nbound > 0)

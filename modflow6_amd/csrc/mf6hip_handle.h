@@ -107,6 +107,15 @@ struct Mf6Handle {
   int hfb_n = 0;
   int *d_hfb_face = nullptr;
   double *d_hfb_sav = nullptr;
+  // barriers next to a convertible cell without NEWTON (hfb_fc / hfb_cq at
+  // every formulate and budget): positions of n and m, direction index of m
+  // seen from n, hydchr, face width, hfb_fc's condsav; per position its
+  // barriers (b * 2 + role) in barrier order
+  int hfbd_n = 0;
+  int *d_hfbd_np = nullptr, *d_hfbd_mp = nullptr, *d_hfbd_s = nullptr;
+  double *d_hfbd_hy = nullptr, *d_hfbd_w = nullptr, *d_hfbd_sav = nullptr;
+  int *d_hfbd_off = nullptr, *d_hfbd_ent = nullptr;
+  std::vector<double> h_delr, h_delc;
   // ghost-node correction (explicit GNC of the model): per GNC the positions
   // of n and m, the direction index of m seen from n (0..5: up, back, left,
   // right, front, down), numjs contributing positions (-1: none) and

@@ -1785,7 +1785,9 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->unet_feat, h->fcg_store, h->A16, h->invpiv32s, h->d_amax, h->sat,
                   h->flowja, h->strg, h->bsim, h->chd_flag, h->dhat32, h->qhat32, h->t32,
                   h->d_hfb_face, h->d_hfb_sav, h->d_gnc_np, h->d_gnc_mp, h->d_gnc_s,
-                  h->d_gnc_jp, h->d_gnc_a, h->d_gnc_cond, h->d_gnc_off, h->d_gnc_ent};
+                  h->d_gnc_jp, h->d_gnc_a, h->d_gnc_cond, h->d_gnc_off, h->d_gnc_ent,
+                  h->d_hfbd_np, h->d_hfbd_mp, h->d_hfbd_s, h->d_hfbd_hy, h->d_hfbd_w,
+                  h->d_hfbd_sav, h->d_hfbd_off, h->d_hfbd_ent};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);

@@ -774,6 +774,62 @@ double orc_hyeff(double k11, double k22, double k33, double ang1, double ang2,
   return k;
 }
 
+/* gwf-hfb.f90:211-348 hfb_fc without NEWTON or XT3D: each barrier next to a
+ * convertible cell (ICELLTYPE 1), in barrier order, replaces the matrix
+ * value aterm at (n, m) and (m, n) by its series combination with the
+ * barrier, cond = aterm condhfb / (aterm + condhfb), condhfb = hydchr w h
+ * with h the mean wetted thickness of the two cells (tops clipped at the
+ * heads of convertible cells), or -hydchr aterm for hydchr <= 0, and adds
+ * aterm - cond to both diagonals; condsav keeps cond for hfb_cq.  idxloc:
+ * the CSR position of (n, m) in row n. */
+void orc_hfb_fc(int nhfb, const int *noden, const int *nodem, const int *idxloc,
+                const double *hydchr, const int *ibound, const int *icelltype,
+                const int *ia, const int *isym, const int *jas, const double *hwva,
+                const double *top, const double *bot, const double *hnew,
+                double *amat, double *condsav)
+{
+  for (int b = 0; b < nhfb; b++) {
+    int ipos = idxloc[b];
+    double aterm = amat[ipos];
+    int n = noden[b], m = nodem[b];
+    if (ibound[n] == 0 || ibound[m] == 0) continue;
+    if (icelltype[n] != 1 && icelltype[m] != 1) continue;
+    double topn = top[n], topm = top[m], botn = bot[n], botm = bot[m];
+    if (icelltype[n] == 1 && hnew[n] < topn) topn = hnew[n];
+    if (icelltype[m] == 1 && hnew[m] < topm) topm = hnew[m];
+    double faheight = DHALF * ((topn - botn) + (topm - botm));
+    double cond;
+    if (hydchr[b] > DZERO) {
+      double fawidth = hwva[jas[ipos]];
+      double condhfb = hydchr[b] * DONE * fawidth * faheight;
+      cond = aterm * condhfb / (aterm + condhfb);
+    } else {
+      cond = -aterm * hydchr[b];
+    }
+    condsav[b] = cond;
+    amat[ia[n]] = amat[ia[n]] + (aterm - cond);
+    amat[ipos] = cond;
+    amat[ia[m]] = amat[ia[m]] + (aterm - cond);
+    amat[isym[ipos]] = cond;
+  }
+}
+
+/* gwf-hfb.f90:355-450 hfb_cq without NEWTON: the flow across such a barrier
+ * is condsav (h(m) - h(n)) at (n, m) and its negative at (m, n) */
+void orc_hfb_cq(int nhfb, const int *noden, const int *nodem, const int *idxloc,
+                const int *isym, const int *ibound, const int *icelltype,
+                const double *hnew, const double *condsav, double *flowja)
+{
+  for (int b = 0; b < nhfb; b++) {
+    int n = noden[b], m = nodem[b];
+    if (ibound[n] == 0 || ibound[m] == 0) continue;
+    if (icelltype[n] != 1 && icelltype[m] != 1) continue;
+    double qnm = condsav[b] * (hnew[m] - hnew[n]);
+    flowja[idxloc[b]] = qnm;
+    flowja[isym[idxloc[b]]] = -qnm;
+  }
+}
+
 /* gwf-npf.f90:1950-2037 calc_condsat (upperOnly, saturation 1, no
  * THICKSTRT) for every node. */
 void orc_calc_condsat(int nodes, const int *ia, const int *ja, const int *jas,

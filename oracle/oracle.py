@@ -125,6 +125,10 @@ def lib():
         L.orc_rate_accumulator.argtypes = [i, f64p, C.POINTER(d), C.POINTER(d)]
         L.orc_wel_cf.argtypes = [i, i32p, f64p, i32p, i, d, i32p, f64p, f64p,
                                  f64p, f64p, f64p]
+        L.orc_hfb_fc.argtypes = [i, i32p, i32p, i32p, f64p, i32p, i32p, i32p, i32p,
+                                 i32p, f64p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_hfb_cq.argtypes = [i, i32p, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
+                                 f64p]
         L.orc_gnc_fc.argtypes = [i, i, i32p, i32p, i32p, f64p, i32p, i32p, f64p,
                                  f64p, f64p, f64p]
         L.orc_gnc_fn.argtypes = [i, i, i32p, i32p, i32p, f64p, i32p, i32p, i32p,
@@ -255,6 +259,8 @@ def ref_gwf_lib():
         R.ref_hyeff.argtypes = [d] * 9 + [i]
         R.ref_dpio180.restype = d
         R.ref_dpio180.argtypes = [d]
+        R.ref_hfb_cq.argtypes = [i, i, i, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
+                                 f64p]
         R.ref_gnc.argtypes = [i, i, i, i, i32p, i32p, i32p, f64p, i32p, f64p,
                               i, i, i32p, f64p, f64p, f64p, f64p, f64p]
         R.ref_wel.argtypes = [i, i, i, i32p, f64p, i, d, i32p, i32p, f64p,
@@ -689,11 +695,11 @@ class GwfSimulation:
         """hfb_rp (gwf-hfb.f90:149-201): condsat_reset of the last set, then
         check_data's positions (:711-766) and condsat_modify (:789-832) of
         the new one.  0-based node numbers.  A barrier next to a convertible
-        cell without NEWTON (applied by hfb_fc at every formulate) is not
-        restated: ValueError."""
+        cell without NEWTON is left to hfb_fc at every formulate (:211-348)
+        and hfb_cq (:355-450)."""
         d = self.dis
         if self.hfb is not None:
-            _, idx0, sav0 = self.hfb
+            idx0, sav0 = self.hfb[1], self.hfb[2]
             lib().orc_hfb_condsat_reset(idx0.size, idx0, d.jas, sav0, self.condsat)
             self.hfb = None
         n = np.ascontiguousarray(noden, dtype=np.int32)
@@ -709,16 +715,13 @@ class GwfSimulation:
             if d.ihc[d.jas[ipos]] == 0:
                 raise ValueError(f"HFB no. {b + 1} is between two cells not horizontally "
                                  "connected")
-            if self.inewton == 0 and (self.icelltype[n[b]] != 0 or
-                                      self.icelltype[m[b]] != 0):
-                raise ValueError(f"HFB no. {b + 1}: convertible cell without NEWTON "
-                                 "(hfb_fc) is not restated")
             idx[b] = ipos
         sav = np.zeros(n.size)
         lib().orc_hfb_condsat_modify(n.size, n, m, idx, hy, self.inewton, self.icelltype,
                                      d.jas.astype(np.int32), d.ihc.astype(np.int32),
                                      d.hwva, d.top, d.bot, self.condsat, sav)
-        self.hfb = (n, idx, sav)
+        # (noden, idxloc, csatsav, nodem, hydchr, condsav of hfb_fc)
+        self.hfb = (n, idx, sav, m, hy, np.zeros(n.size))
 
     def set_gnc(self, noden, nodem, nodesj, alphasj):
         """The model's GNC package in explicit form (GhostNode.f90): n and m
@@ -795,6 +798,10 @@ class GwfSimulation:
                      self.inewton, self.ivarcv, self.idewatcv, self.icellavg,
                      self.k11, self.k22, d.ncol, self.k33, d.top, d.bot,
                      self.sat, self.x, self.amat)
+        if self.hfb is not None and self.inewton == 0:
+            hn, hidx, _, hm, hy, hsav = self.hfb
+            L.orc_hfb_fc(hn.size, hn, hm, hidx, hy, self.ibound, self.icelltype, d.ia,
+                         d.isym, d.jas, d.hwva, d.top, d.bot, self.x, self.amat, hsav)
         if self.gnc is not None:
             gn, gm, gj, ga, gp, gc = self.gnc
             L.orc_gnc_fc(gn.size, gj.shape[1], gn, gm, gj, ga, gp, self.ibound, self.x,
@@ -879,6 +886,10 @@ class GwfSimulation:
                      self.inewton, self.ivarcv, self.idewatcv, self.icellavg,
                      self.k11, self.k22, d.ncol, self.k33, d.top, d.bot,
                      self.sat, self.x, flowja)
+        if self.hfb is not None and self.inewton == 0:
+            hn, hidx, _, hm, hy, hsav = self.hfb
+            L.orc_hfb_cq(hn.size, hn, hm, hidx, d.isym, self.ibound, self.icelltype,
+                         self.x, hsav, flowja)
         if self.gnc is not None:
             gn, gm, gj, ga, gp, gc = self.gnc
             L.orc_gnc_cq(gn.size, gj.shape[1], gn, gm, gj, ga, gp, d.isym, self.ibound,

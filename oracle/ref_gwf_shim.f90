@@ -694,6 +694,44 @@ contains
     deallocate (hfb)
   end subroutine ref_hfb_fc
 
+  !> gwf-hfb.f90:355 hfb_cq without NEWTON: the flows across barriers next
+  !! to a convertible cell from the conductances hfb_fc saved (condsav)
+  subroutine ref_hfb_cq(nodes, nja, nhfb, noden, nodem, idxloc, icelltype, &
+                        ibound, hnew, condsav, flowja) bind(C, name='ref_hfb_cq')
+    integer(c_int), value :: nodes, nja, nhfb
+    integer(c_int), intent(in) :: noden(nhfb), nodem(nhfb), idxloc(nhfb)
+    integer(c_int), intent(in) :: icelltype(nodes), ibound(nodes)
+    real(c_double), intent(in) :: hnew(nodes), condsav(nhfb)
+    real(c_double), intent(inout) :: flowja(nja)
+    type(GwfHfbType), pointer :: hfb
+    real(DP), allocatable :: h(:), f(:)
+    allocate (h(nodes), f(nja))
+    h = hnew
+    f = flowja
+    allocate (hfb)
+    hfb%dis => dis
+    allocate (hfb%inewton, hfb%nhfb, hfb%ivsc)
+    hfb%inewton = 0
+    hfb%nhfb = nhfb
+    hfb%ivsc = 0
+    allocate (hfb%xt3d)
+    allocate (hfb%noden(nhfb), hfb%nodem(nhfb), hfb%idxloc(nhfb), hfb%condsav(nhfb))
+    hfb%noden = noden
+    hfb%nodem = nodem
+    hfb%idxloc = idxloc
+    hfb%condsav = condsav
+    allocate (hfb%ibound(nodes), hfb%icelltype(nodes))
+    hfb%ibound = ibound
+    hfb%icelltype = icelltype
+    call hfb%hfb_cq(h, f)
+    flowja = f
+    deallocate (hfb%noden, hfb%nodem, hfb%idxloc, hfb%condsav, hfb%ibound, &
+                hfb%icelltype)
+    deallocate (hfb%xt3d)
+    deallocate (hfb%inewton, hfb%nhfb, hfb%ivsc)
+    deallocate (hfb)
+  end subroutine ref_hfb_cq
+
   ! ------------------------------------------------------------------
   ! NumericalSolution: under-relaxation and backtracking
   ! ------------------------------------------------------------------

@@ -2,9 +2,10 @@
 oracle's HFB (pinned to the reference in tests/test_oracle_hfb.py): the
 formulated system bit for bit with barriers folded into the faces'
 saturated conductances (confined cells; NEWTON with convertible cells and
-storage), the heads of a solved step within 10 x dvclose, the budget's face
-flows bit for bit, a new barrier set replacing the last one, and the
-rejections."""
+storage), between convertible cells without NEWTON (hfb_fc at every
+formulate, hfb_cq in the budget), the heads of a solved step within 10 x
+dvclose, the budget's face flows bit for bit, a new barrier set replacing the
+last one, and the rejections."""
 import dataclasses
 
 import numpy as np
@@ -57,7 +58,19 @@ def _newton():
     return dataclasses.replace(c, hfb=_barriers(c, 2))
 
 
-CASES = {"confined": _confined, "newton": _newton}
+def _convertible_plain():
+    """Convertible cells without NEWTON: every barrier is applied by hfb_fc
+    at each formulate (heads spread through the cells' thickness)."""
+    c = configs.transient_newton(nlay=3, nrow=24, ncol=30, nwel=8, nper=1, nstp=2)
+    rng = np.random.default_rng(12)
+    bot = c.botm.reshape(-1)
+    top = np.concatenate([c.top2d.reshape(-1), c.botm[:-1].reshape(-1)])
+    strt = bot + rng.uniform(0.2, 0.98, bot.size) * (top - bot)
+    c = dataclasses.replace(c, newton=False, newton_ur=False, strt=strt)
+    return dataclasses.replace(c, hfb=_barriers(c, 3))
+
+
+CASES = {"confined": _confined, "newton": _newton, "convertible_plain": _convertible_plain}
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -73,16 +86,24 @@ def test_formulate_with_barriers_bit_exact(name):
         gs.kper, gs.kstp = 1, 1
         gs.prepare_timestep(perlen / nstp, steady)
         amat, rhs = gs.formulate(1, want_system=True)
+        db = gs.budget()
     finally:
         gs.destroy()
+    # face flows of the same heads (hfb_cq with hfb_fc's conductances for the
+    # barriers next to convertible cells without NEWTON)
+    ob = sim.budget()
+    assert np.array_equal(db["flowja"].view(np.int64), ob["flowja"].view(np.int64))
     plain = O.GwfSimulation(dataclasses.replace(case, hfb=None))
-    assert not np.array_equal(plain.condsat, sim.condsat)
+    plain.kper, plain.kstp = 1, 1
+    plain.advance(perlen / nstp, steady)
+    plain.formulate(1)
+    assert not np.array_equal(plain.amat, sim.amat)
     bad = np.flatnonzero(amat.view(np.int64) != sim.amat.view(np.int64))
     assert bad.size == 0, (bad[:8], amat[bad[:8]], sim.amat[bad[:8]])
     assert np.array_equal(rhs.view(np.int64), sim.rhs.view(np.int64))
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("name", ["confined", "newton"])
 def test_step_with_barriers_heads_and_budget(name):
     case = CASES[name]()
     sim = O.GwfSimulation(case)
@@ -158,6 +179,7 @@ def test_set_hfb_rejections():
         assert call([0], [84], [0.1]) == _lib.MF6HIP_EINVAL
     finally:
         gs.destroy()
+    # convertible cells without NEWTON: accepted (hfb_fc at every formulate)
     conv = dataclasses.replace(case, icelltype=np.ones(case.nodes, dtype=np.int32))
     gs = GwfSolution(conv)
     try:
@@ -165,7 +187,6 @@ def test_set_hfb_rejections():
         m = np.array([2], dtype=np.int32)
         hy = np.array([0.1])
         rc = gs.L.mf6hip_gwf_set_hfb(gs.h, 1, _lib.iptr(n), _lib.iptr(m), _lib.dptr(hy))
-        assert rc == _lib.MF6HIP_EUNSUPPORTED
-        assert "hfb_fc" in _lib.last_error()
+        assert rc == _lib.MF6HIP_OK
     finally:
         gs.destroy()

@@ -173,3 +173,40 @@ def test_hfb_rejects_unconnected_and_vertical_pairs():
         sim.set_hfb(np.array([0]), np.array([2]), np.array([0.1]))
     with pytest.raises(ValueError, match="not horizontally"):
         sim.set_hfb(np.array([0]), np.array([42]), np.array([0.1]))
+
+
+@pytest.mark.skipif(R is None, reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed", [5, 6])
+def test_hfb_fc_cq_convertible_without_newton(seed):
+    """hfb_fc and hfb_cq between convertible cells without NEWTON
+    (gwf-hfb.f90:211-450) against the reference's, with heads below and
+    above the cell tops, inactive and constant-head cells, mixed cell types
+    and two barriers on one face."""
+    rng = np.random.default_rng(seed)
+    case = configs.steady_small(seed=seed, nlay=2, nrow=6, ncol=7, nwel=2)
+    d = O.dis_connections(case.nlay, case.nrow, case.ncol, case.delr, case.delc,
+                          case.top2d, case.botm)
+    n = d.nodes
+    R.ref_gwf_dis(n, d.nja, d.hwva.size, one(d.ia), one(d.ja), one(d.jas), d.ihc,
+                  one(d.isym), d.hwva, d.cl1, d.cl2, d.top, d.bot, d.area)
+    noden, nodem, idx, hy = _barriers(d, rng, 10)
+    icell = np.ascontiguousarray(rng.integers(0, 2, n), dtype=np.int32)
+    ib = np.ones(n, dtype=np.int32)
+    ib[rng.choice(n, 4, replace=False)] = -1
+    ib[rng.choice(n, 3, replace=False)] = 0
+    hnew = np.ascontiguousarray(d.bot + rng.uniform(0.1, 1.3, n) * (d.top - d.bot))
+    amat = rng.uniform(0.5, 50.0, d.nja)
+    a1, a2 = amat.copy(), amat.copy()
+    s1, s2 = np.zeros(noden.size), np.zeros(noden.size)
+    L.orc_hfb_fc(noden.size, noden, nodem, idx, hy, ib, icell, d.ia, d.isym, d.jas, d.hwva,
+                 d.top, d.bot, hnew, a1, s1)
+    R.ref_hfb_fc(n, d.nja, noden.size, 0, one(noden), one(nodem), one(idx), hy, icell, ib,
+                 hnew, a2, s2)
+    assert eqbits(a1, a2) and eqbits(s1, s2)
+    assert not eqbits(a1, amat)
+    f1 = rng.standard_normal(d.nja)
+    f2 = f1.copy()
+    L.orc_hfb_cq(noden.size, noden, nodem, idx, d.isym, ib, icell, hnew, s1, f1)
+    R.ref_hfb_cq(n, d.nja, noden.size, one(noden), one(nodem), one(idx), icell, ib, hnew,
+                 s1, f2)
+    assert eqbits(f1, f2)
