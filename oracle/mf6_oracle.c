@@ -350,7 +350,8 @@ void orc_bcgs(int *icnvg, int itmax, int *innerit, int neq, int nja,
               const double *a0, const int *iapc, const int *japc,
               const double *apc, double *x, const double *b, double *d,
               double *p, double *q, double *t, double *v, double *dhat,
-              double *phat, double *qhat, orc_iter_rec *hist)
+              double *phat, double *qhat, const double *dscale,
+              orc_iter_rec *hist)
 {
   double alpha = DZERO, alpha0 = DZERO, beta = DZERO, rho = DZERO;
   double rho0 = DZERO, omega = DZERO, omega0 = DZERO;
@@ -382,9 +383,11 @@ void orc_bcgs(int *icnvg, int itmax, int *innerit, int neq, int nja,
     for (int n = 0; n < neq; n++) {
       double tv = alpha * phat[n] + omega * qhat[n];
       x[n] = x[n] + tv;
+      if (dscale) tv = tv * dscale[n];   /* ImsLinearBase.f90:452-454 */
       if (fabs(tv) > fabs(deltax)) { deltax = tv; xloc = n; }
       tv = q[n] - omega * t[n];
       d[n] = tv;
+      if (dscale) tv = tv / dscale[n];   /* ImsLinearBase.f90:467-469 */
       if (fabs(tv) > fabs(rmax)) { rmax = tv; rloc = n; }
       l2norm = l2norm + tv * tv;
     }
@@ -412,20 +415,85 @@ void orc_bcgs(int *icnvg, int itmax, int *innerit, int neq, int nja,
   if (*icnvg < 0) *icnvg = 0;
 }
 
-/* ImsLinear.f90:617-750 imslinear_ap without scaling (ISCL = 0) and
- * reordering (IORD = 0).  work must hold 9*neq doubles (BCGS) or 4*neq (CG);
- * iwork neq ints; apc nja doubles; iapc neq+1, japc nja ints (already filled
- * by orc_pccrs).  Returns the number of inner iterations; *l2norm0_out gets
- * the initial residual norm, *ilu_stop the early-exit row of the factor. */
-int orc_ims_ap(int neq, int nja, const int *ia, const int *ja,
-               const double *amat, const double *rhs, double *x, int ilinmeth,
-               int ipc, int icnvgopt, int north, int iter1, double dvclose,
-               double rclose, double relax, int kstp, double *apc,
-               const int *iapc, const int *japc, int *iwork, double *work,
-               int *icnvg, double *l2norm0_out, int *ilu_stop,
-               orc_iter_rec *hist)
+/* ImsLinearBase.f90:619-754 ims_base_scale.  iopt 0 scales: iscl 1 takes
+ * dscale = dscale2 = 1/sqrt|a_nn| and a_nm = (c_n a_nm) c_m; iscl 2 scales
+ * each row to unit L2 norm (dscale, 1 for a zero row), then each column of
+ * the row-scaled matrix (dscale2, the squares summed row by row in CSR
+ * order).  Both then set x = x / dscale2 and b = b dscale.  iopt 1 undoes
+ * it: a_nm = ((1/c_n) a_nm) (1/c_m), x = x dscale2, b = b / dscale. */
+void orc_ims_scale(int iopt, int iscl, int neq, const int *ia, const int *ja,
+                   double *amat, double *x, double *b, double *dscale,
+                   double *dscale2)
+{
+  if (iopt == 0) {
+    if (iscl == 1) {
+      for (int n = 0; n < neq; n++) {
+        double c1 = DONE / sqrt(fabs(amat[ia[n]]));
+        dscale[n] = c1;
+        dscale2[n] = c1;
+      }
+      for (int n = 0; n < neq; n++) {
+        double c1 = dscale[n];
+        for (int i = ia[n]; i < ia[n + 1]; i++)
+          amat[i] = c1 * amat[i] * dscale2[ja[i]];
+      }
+    } else if (iscl == 2) {
+      for (int n = 0; n < neq; n++) {
+        double c1 = DZERO;
+        for (int i = ia[n]; i < ia[n + 1]; i++) c1 = c1 + amat[i] * amat[i];
+        c1 = sqrt(c1);
+        c1 = (c1 == DZERO) ? DONE : DONE / c1;
+        dscale[n] = c1;
+        for (int i = ia[n]; i < ia[n + 1]; i++) amat[i] = c1 * amat[i];
+      }
+      for (int n = 0; n < neq; n++) dscale2[n] = DZERO;
+      for (int n = 0; n < neq; n++)
+        for (int i = ia[n]; i < ia[n + 1]; i++)
+          dscale2[ja[i]] = dscale2[ja[i]] + amat[i] * amat[i];
+      for (int n = 0; n < neq; n++) {
+        double c2 = dscale2[n];
+        dscale2[n] = (c2 == DZERO) ? DONE : DONE / sqrt(c2);
+      }
+      for (int n = 0; n < neq; n++)
+        for (int i = ia[n]; i < ia[n + 1]; i++)
+          amat[i] = dscale2[ja[i]] * amat[i];
+    }
+    for (int n = 0; n < neq; n++) {
+      x[n] = x[n] / dscale2[n];
+      b[n] = b[n] * dscale[n];
+    }
+  } else {
+    for (int n = 0; n < neq; n++) {
+      double c1 = dscale[n];
+      for (int i = ia[n]; i < ia[n + 1]; i++)
+        amat[i] = (DONE / c1) * amat[i] * (DONE / dscale2[ja[i]]);
+      x[n] = x[n] * dscale2[n];
+      b[n] = b[n] / c1;
+    }
+  }
+}
+
+/* ImsLinear.f90:617-750 imslinear_ap without reordering (IORD = 0): epfact,
+ * scaling (iscl 1 or 2: A, x and b in place, undone at the end with the
+ * reference's rounding; dscale and dscale2 neq doubles each), pcu, residual,
+ * L2 norm, then CG or BCGS (BCGS measures its dx and residual maxima in
+ * unscaled units, ImsLinearBase.f90:452-469).  work must hold 9*neq doubles
+ * (BCGS) or 4*neq (CG); iwork neq ints; apc nja doubles; iapc neq+1, japc
+ * nja ints (already filled by orc_pccrs).  Returns the number of inner
+ * iterations; *l2norm0_out gets the initial residual norm, *ilu_stop the
+ * early-exit row of the factor. */
+int orc_ims_ap_scl(int neq, int nja, const int *ia, const int *ja,
+                   double *amat, double *rhs, double *x, int ilinmeth,
+                   int ipc, int icnvgopt, int north, int iter1, double dvclose,
+                   double rclose, double relax, int kstp, int iscl,
+                   double *dscale, double *dscale2, double *apc,
+                   const int *iapc, const int *japc, int *iwork, double *work,
+                   int *icnvg, double *l2norm0_out, int *ilu_stop,
+                   orc_iter_rec *hist)
 {
   double epfact = orc_epfact(icnvgopt, kstp);
+  if (iscl != 0)
+    orc_ims_scale(0, iscl, neq, ia, ja, amat, x, rhs, dscale, dscale2);
   int stop = orc_pcu(nja, neq, ipc, relax, amat, ia, ja, apc, iapc, japc,
                      iwork, work);
   if (ilu_stop) *ilu_stop = stop;
@@ -450,9 +518,26 @@ int orc_ims_ap(int neq, int nja, const int *ia, const int *ja,
     for (int n = 0; n < neq; n++) { t[n] = 0; v[n] = 0; dhat[n] = 0; phat[n] = 0; qhat[n] = 0; }
     orc_bcgs(icnvg, itmax, &innerit, neq, nja, icnvgopt, north, dvclose,
              rclose, l2norm0, epfact, ia, ja, amat, iapc, japc, apc, x, rhs, d,
-             p, q, t, v, dhat, phat, qhat, hist);
+             p, q, t, v, dhat, phat, qhat, iscl != 0 ? dscale : NULL, hist);
   }
+  if (iscl != 0)
+    orc_ims_scale(1, iscl, neq, ia, ja, amat, x, rhs, dscale, dscale2);
   return innerit;
+}
+
+/* orc_ims_ap_scl without scaling (ISCL = 0): A and b are left as given. */
+int orc_ims_ap(int neq, int nja, const int *ia, const int *ja,
+               const double *amat, const double *rhs, double *x, int ilinmeth,
+               int ipc, int icnvgopt, int north, int iter1, double dvclose,
+               double rclose, double relax, int kstp, double *apc,
+               const int *iapc, const int *japc, int *iwork, double *work,
+               int *icnvg, double *l2norm0_out, int *ilu_stop,
+               orc_iter_rec *hist)
+{
+  return orc_ims_ap_scl(neq, nja, ia, ja, (double *)amat, (double *)rhs, x,
+                        ilinmeth, ipc, icnvgopt, north, iter1, dvclose, rclose,
+                        relax, kstp, 0, NULL, NULL, apc, iapc, japc, iwork,
+                        work, icnvg, l2norm0_out, ilu_stop, hist);
 }
 
 /* ------------------------------------------------------------------ */

@@ -76,6 +76,13 @@ def lib():
                                  f64p, C.POINTER(C.c_int),
                                  C.POINTER(C.c_double), C.POINTER(C.c_int),
                                  C.c_void_p]
+        L.orc_ims_scale.argtypes = [i, i, i, i32p, i32p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_ims_ap_scl.restype = i
+        L.orc_ims_ap_scl.argtypes = [i, i, i32p, i32p, f64p, f64p, f64p, i, i, i,
+                                     i, i, d, d, d, i, i, f64p, f64p, f64p, i32p,
+                                     i32p, i32p, f64p, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                     C.c_void_p]
         L.orc_hyeff.restype = d
         L.orc_hyeff.argtypes = [d] * 9 + [i]
         for nm, nargs in (("orc_sqsat", 4), ("orc_sqsat_deriv", 4),
@@ -194,6 +201,11 @@ def ref_lib():
                                  i, i, d, d, d, i, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(d), f64p, f64p,
                                  i32p, i32p]
+        R.ref_ims_ap_scl.argtypes = [i, i, i32p, i32p, f64p, f64p, f64p, i, i, i,
+                                     i, i, d, d, d, i, i, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(d), f64p, f64p,
+                                     i32p, i32p]
+        R.ref_ims_scale.argtypes = [i, i, i, i, i32p, i32p, f64p, f64p, f64p, f64p, f64p]
         R.ref_testcnvg.argtypes = [i, C.POINTER(C.c_int), i, d, d, d, d, d, d]
         R.ref_epfact.restype = d
         R.ref_epfact.argtypes = [i, i]
@@ -419,6 +431,7 @@ class ImsSettings:
     ilinmeth: int = 1
     north: int = 0
     relax: float = 0.0
+    iscl: int = 0      # SCALING_METHOD: 0 none, 1 symmetric, 2 L2 norm
 
     @property
     def ipc(self):
@@ -449,10 +462,20 @@ class ImsLinear:
         l2 = C.c_double(0.0)
         stop = C.c_int(-1)
         hist = (IterRec * max(1, s.iter1))() if history else None
-        innerit = lib().orc_ims_ap(
+        iscl = getattr(s, "iscl", 0)
+        if iscl:
+            # SCALING_METHOD scales A and b in place and unscales them after
+            # the solve (ImsLinear.f90:646, :741), as the reference does
+            ds1, ds2 = np.ones(self.neq), np.ones(self.neq)
+        else:
+            # unscaled, A and b are read only: a private copy keeps the
+            # caller's arrays untouched either way
+            amat, rhs = np.array(amat, dtype=np.float64), np.array(rhs, dtype=np.float64)
+            ds1 = ds2 = np.ones(1)
+        innerit = lib().orc_ims_ap_scl(
             self.neq, self.nja, self.ia, self.ja, amat, rhs, x, s.ilinmeth,
             s.ipc, s.icnvgopt, s.north, s.iter1, s.dvclose, s.rclose, s.relax,
-            kstp, self.apc, self.iapc, self.japc, self.iwork, self.work,
+            kstp, iscl, ds1, ds2, self.apc, self.iapc, self.japc, self.iwork, self.work,
             C.byref(icnvg), C.byref(l2), C.byref(stop),
             C.cast(hist, C.c_void_p) if hist is not None else None)
         recs = None

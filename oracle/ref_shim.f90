@@ -15,7 +15,7 @@ module mf6ref_shim
   use IMSLinearBaseModule, only: ims_base_pccrs, ims_base_pcu, &
                                  ims_base_ilu0a, ims_base_cg, ims_base_bcgs, &
                                  ims_base_residual, ims_base_testcnvg, &
-                                 ims_base_epfact
+                                 ims_base_epfact, ims_base_scale
   use ConvergenceSummaryModule, only: ConvergenceSummaryType
   use MathUtilModule, only: is_close
   use GwfConductanceUtilsModule, only: hcond, vcond, condmean
@@ -119,9 +119,45 @@ contains
     real(c_double), intent(out) :: l2norm0
     real(c_double), intent(out) :: hdv(iter1), hr(iter1)
     integer(c_int), intent(out) :: hldv(iter1), hlr(iter1)
+    real(DP), allocatable :: a(:)
+    allocate (a(nja))
+    a = amat
+    call ref_ims_ap_scl(neq, nja, ia, ja, a, rhs, x, ilinmeth, ipc, &
+                        icnvgopt, north, iter1, dvclose, rclose, relax, &
+                        kstp, 0, icnvg, innerit, l2norm0, hdv, hr, hldv, hlr)
+  end subroutine ref_ims_ap
+
+  ! ImsLinearBase.f90:619-754 ims_base_scale, as called by imslinear_ap.
+  subroutine ref_ims_scale(iopt, iscl, neq, nja, ia, ja, amat, x, b, &
+                           dscale, dscale2) bind(C, name='ref_ims_scale')
+    integer(c_int), value :: iopt, iscl, neq, nja
+    integer(c_int), intent(in) :: ia(neq + 1), ja(nja)
+    real(c_double), intent(inout) :: amat(nja), x(neq), b(neq)
+    real(c_double), intent(inout) :: dscale(neq), dscale2(neq)
+    call ims_base_scale(iopt, iscl, neq, nja, ia, ja, amat, x, b, dscale, &
+                        dscale2)
+  end subroutine ref_ims_scale
+
+  ! ref_ims_ap with SCALING_METHOD iscl: ImsLinear.f90:646-652 scales A, x
+  ! and b before pcu, ims_base_bcgs receives ISCL and DSCALE, and
+  ! ImsLinear.f90:741-747 unscales after the solve (A and b in place).
+  subroutine ref_ims_ap_scl(neq, nja, ia, ja, amat, rhs, x, ilinmeth, ipc, &
+                            icnvgopt, north, iter1, dvclose, rclose, relax, &
+                            kstp, iscl, icnvg, innerit, l2norm0, hdv, hr, &
+                            hldv, hlr) bind(C, name='ref_ims_ap_scl')
+    integer(c_int), value :: neq, nja, ilinmeth, ipc, icnvgopt, north, iter1
+    real(c_double), value :: dvclose, rclose, relax
+    integer(c_int), value :: kstp, iscl
+    integer(c_int), intent(in) :: ia(neq + 1), ja(nja)
+    real(c_double), intent(inout) :: amat(nja)
+    real(c_double), intent(inout) :: rhs(neq), x(neq)
+    integer(c_int), intent(out) :: icnvg, innerit
+    real(c_double), intent(out) :: l2norm0
+    real(c_double), intent(out) :: hdv(iter1), hr(iter1)
+    integer(c_int), intent(out) :: hldv(iter1), hlr(iter1)
     ! local
     integer(I4B), allocatable :: iapc(:), japc(:), iw(:), jlu(:), jw(:)
-    real(DP), allocatable :: apc(:), w(:), wlu(:), dscale(:)
+    real(DP), allocatable :: apc(:), w(:), wlu(:), dscale(:), dscale2(:)
     real(DP), allocatable :: d(:), p(:), q(:), z(:), t(:), v(:), dhat(:), &
                              phat(:), qhat(:)
     integer(I4B) :: convmodstart(2), nconv, itmax, i
@@ -131,7 +167,7 @@ contains
     real(DP), external :: dnrm2
 
     allocate (iapc(neq + 1), japc(nja), iw(neq), jlu(1), jw(1))
-    allocate (apc(nja), w(neq), wlu(1), dscale(neq))
+    allocate (apc(nja), w(neq), wlu(1), dscale(neq), dscale2(neq))
     allocate (d(neq), p(neq), q(neq), z(neq), t(neq), v(neq), dhat(neq), &
               phat(neq), qhat(neq))
     nconv = max(2, iter1)
@@ -154,11 +190,16 @@ contains
     w = 0.0_DP
     apc = 0.0_DP
     dscale = 1.0_DP
+    dscale2 = 1.0_DP
     d = 0.0_DP; p = 0.0_DP; q = 0.0_DP; z = 0.0_DP
     t = 0.0_DP; v = 0.0_DP; dhat = 0.0_DP; phat = 0.0_DP; qhat = 0.0_DP
 
     call ims_base_pccrs(neq, nja, ia, ja, iapc, japc)
     epfact = ims_base_epfact(icnvgopt, kstp)
+    if (iscl /= 0) then
+      call ims_base_scale(0, iscl, neq, nja, ia, ja, amat, x, rhs, dscale, &
+                          dscale2)
+    end if
     call ims_base_pcu(6, nja, neq, neq, nja, ipc, relax, amat, ia, ja, apc, &
                       iapc, japc, iw, w, 0, 0.0_DP, 1, 1, 1, jlu, jw, wlu)
     icnvg = 0
@@ -177,10 +218,14 @@ contains
                        1, iw, jlu, nconv, 1, convmodstart, caccel, summary)
     else
       call ims_base_bcgs(icnvg, itmax, innerit, neq, nja, neq, nja, ipc, &
-                         icnvgopt, north, 0, dscale, dvclose, rclose, &
+                         icnvgopt, north, iscl, dscale, dvclose, rclose, &
                          l2norm0, epfact, ia, ja, amat, iapc, japc, apc, x, &
                          rhs, d, p, q, t, v, dhat, phat, qhat, 1, iw, jlu, &
                          nconv, 1, convmodstart, caccel, summary)
+    end if
+    if (iscl /= 0) then
+      call ims_base_scale(1, iscl, neq, nja, ia, ja, amat, x, rhs, dscale, &
+                          dscale2)
     end if
     do i = 1, min(innerit, iter1)
       hdv(i) = summary%convdvmax(1, i)
@@ -192,7 +237,7 @@ contains
                 summary%itinner, summary%convlocdv, summary%convdvmax, &
                 summary%convlocr, summary%convrmax)
     deallocate (summary)
-  end subroutine ref_ims_ap
+  end subroutine ref_ims_ap_scl
 
   ! ImsLinearBase.f90:1101 ims_base_testcnvg
   subroutine ref_testcnvg(icnvgopt, icnvg, iiter, dvmax, rmax, rmax0, &

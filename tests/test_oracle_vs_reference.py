@@ -242,3 +242,77 @@ def test_bicgstab_trajectory_bit_exact(kstp):
     case = configs.transient_newton(seed=8, nlay=3, nrow=24, ncol=20, nwel=8, nper=1,
                                     nstp=2)
     _compare_ims(case, kstp=kstp)
+
+
+@pytest.mark.parametrize("iscl", [1, 2])
+def test_ims_scale_bit_exact(iscl):
+    """ims_base_scale (ImsLinearBase.f90:619-754): the scaled A, x, b and both
+    scaling vectors, then the unscaled ones, bit for bit; with iscl 1 the
+    scaled diagonal is +-1 and with iscl 2 every column has unit L2 norm
+    (within rounding)."""
+    case = configs.steady_small(nlay=3, nrow=14, ncol=17, nwel=5)
+    sim, amat, rhs, x0 = oracle_system(case)
+    d = sim.dis
+    ia1, ja1 = (d.ia + 1).astype(np.int32), (d.ja + 1).astype(np.int32)
+    ia, ja = d.ia.astype(np.int32), d.ja.astype(np.int32)
+    n = sim.neq
+    a1, x1, b1 = amat.copy(), x0.copy(), rhs.copy()
+    a2, x2, b2 = amat.copy(), x0.copy(), rhs.copy()
+    s1, t1, s2, t2 = np.ones(n), np.ones(n), np.ones(n), np.ones(n)
+    O.lib().orc_ims_scale(0, iscl, n, ia, ja, a1, x1, b1, s1, t1)
+    R.ref_ims_scale(0, iscl, n, d.nja, ia1, ja1, a2, x2, b2, s2, t2)
+    for u, v in ((a1, a2), (x1, x2), (b1, b2), (s1, s2), (t1, t2)):
+        assert eqbits(u, v)
+    assert not eqbits(a1, amat)
+    if iscl == 1:
+        assert np.allclose(np.abs(a1[d.ia[:-1]]), 1.0, rtol=1e-14, atol=0)
+    else:
+        col = np.zeros(n)
+        np.add.at(col, d.ja, a1 * a1)
+        assert np.allclose(col, 1.0, rtol=1e-13, atol=0)
+    O.lib().orc_ims_scale(1, iscl, n, ia, ja, a1, x1, b1, s1, t1)
+    R.ref_ims_scale(1, iscl, n, d.nja, ia1, ja1, a2, x2, b2, s2, t2)
+    for u, v in ((a1, a2), (x1, x2), (b1, b2)):
+        assert eqbits(u, v)
+    assert np.allclose(a1, amat, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("iscl,ilinmeth", [(1, 1), (2, 1), (1, 2), (2, 2)])
+def test_ims_ap_scaled_bit_exact(iscl, ilinmeth):
+    """imslinear_ap with SCALING_METHOD (ImsLinear.f90:646-747): the whole
+    scaled CG / BiCGSTAB trajectory (BiCGSTAB's dx and residual maxima in
+    unscaled units), the heads and the unscaled A and b, bit for bit."""
+    case = configs.steady_small(nlay=3, nrow=30, ncol=24, nwel=8)
+    sim, amat, rhs, x0 = oracle_system(case)
+    s = sim.settings
+    s.iscl, s.ilinmeth = iscl, ilinmeth
+    d = sim.dis
+    ims = O.ImsLinear(d.ia, d.ja, s)
+    a1, b1, x1 = amat.copy(), rhs.copy(), x0.copy()
+    icn1, it1, l21, _, recs = ims.apply(a1, b1, x1, history=True)
+    itmax = s.iter1
+    hdv, hr = np.zeros(itmax), np.zeros(itmax)
+    hl1, hl2 = np.zeros(itmax, np.int32), np.zeros(itmax, np.int32)
+    icn2, it2, l22 = C.c_int(), C.c_int(), C.c_double()
+    a2, b2, x2 = amat.copy(), rhs.copy(), x0.copy()
+    R.ref_ims_ap_scl(sim.neq, d.nja, (d.ia + 1).astype(np.int32),
+                     (d.ja + 1).astype(np.int32), a2, b2, x2, ilinmeth, s.ipc, 0, 0,
+                     itmax, s.dvclose, s.rclose, 0.0, 1, iscl, C.byref(icn2),
+                     C.byref(it2), C.byref(l22), hdv, hr, hl1, hl2)
+    assert it1 > 10
+    assert icn1 == 1 or iscl == 2      # L2-norm scaling may not close here
+    assert (icn1, it1) == (icn2.value, it2.value)
+    assert eqbits(l21, l22.value)
+    for u, v in ((x1, x2), (a1, a2), (b1, b2)):
+        assert eqbits(u, v)
+    assert eqbits([r[0] for r in recs], hdv[:it1])
+    assert eqbits([r[1] for r in recs], hr[:it1])
+    assert np.array_equal(np.array([r[4] for r in recs]) + 1, hl1[:it1])
+    assert np.array_equal(np.array([r[5] for r in recs]) + 1, hl2[:it1])
+    if not icn1:
+        return
+    # the scaled solve reaches the unscaled one's heads
+    ims0 = O.ImsLinear(d.ia, d.ja, O.ImsSettings(s.dvclose, s.rclose, ilinmeth=ilinmeth))
+    x3 = x0.copy()
+    ims0.apply(amat, rhs, x3)
+    assert np.max(np.abs(x3 - x1)) < 10 * s.dvclose
