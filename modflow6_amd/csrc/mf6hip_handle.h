@@ -84,6 +84,7 @@ struct Mf6Handle {
   KState *d_state = nullptr;
   KState *h_state = nullptr;  // pinned mirror
   double *d_hist = nullptr;   // [4][hist_cap]  dvmax rmax alpha omega
+  double *d_dscale = nullptr; // [2][NP] SCALING_METHOD row and column factors
   int *d_hloc = nullptr;      // [2][hist_cap]
   int hist_cap = 0;
   // scratch for host transfers in natural order

@@ -693,7 +693,8 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_update(Layout L, TX *__restr
                                                            const TV *__restrict__ t,
                                                            double *__restrict__ part,
                                                            MaxLoc *__restrict__ mpart,
-                                                           const KState *st, int nblk) {
+                                                           const KState *st, int nblk,
+                                                           const double *__restrict__ dscale) {
   __shared__ double lds[8];
   __shared__ MaxLoc mlds[8];
   if (st->stop) return;
@@ -707,11 +708,15 @@ __global__ void __launch_bounds__(MF6_BLOCK) k_bcgs_update(Layout L, TX *__restr
     int nat = natural(L, k, i, j);
     TV tv = alpha * phat[e] + omega * qhat[e];
     x[e] = (TX)(x[e] + (TX)tv);
-    mx = maxloc_pick(mx, MaxLoc{(double)tv, nat});
+    double dv = (double)tv;
+    if (dscale) dv = dv * dscale[e];   // ImsLinearBase.f90:452-454
+    mx = maxloc_pick(mx, MaxLoc{dv, nat});
     tv = q[e] - omega * t[e];
     d[e] = tv;
-    mr = maxloc_pick(mr, MaxLoc{(double)tv, nat});
-    s = s + (double)tv * (double)tv;
+    double rv = (double)tv;
+    if (dscale) rv = rv / dscale[e];   // ImsLinearBase.f90:467-469
+    mr = maxloc_pick(mr, MaxLoc{rv, nat});
+    s = s + rv * rv;
   }
   s = block_sum(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -1189,7 +1194,8 @@ static int run_bcgs(Mf6Handle *h) {
       k_bcgs_t<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->A, h->qhat, h->t, h->q, h->d_part, st, nb);
       k_s_bcgs_omega<<<1, MF6_BLOCK, 0, s>>>(st, h->d_part, nb);
       k_bcgs_update<double, double><<<nb, MF6_BLOCK, 0, s>>>(L, h->x, h->phat, h->qhat, h->d, h->q, h->t,
-                                             h->d_part, h->d_mpart, st, nb);
+                                             h->d_part, h->d_mpart, st, nb,
+                                             h->s.iscl ? h->d_dscale : nullptr);
       k_s_iter_end<<<1, MF6_BLOCK, 0, s>>>(st, h->d_part, h->d_mpart, nb, h->d_hist,
                                            h->d_hloc, 1);
       if (h->s.north > 0) {
@@ -1408,7 +1414,7 @@ static int run_mixed_bcgs(Mf6Handle *h, const TP *piv32) {
                                                         nb);
         k_s_bcgs_omega<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_part, nb);
         k_bcgs_update<float, float><<<nb, MF6_BLOCK, 0, s>>>(L, e, phat, qhat, r, q, t,
-                                                             h->d_part, h->d_mpart, ist, nb);
+                                                             h->d_part, h->d_mpart, ist, nb, nullptr);
         k_s_inner_end_bcgs<<<1, MF6_BLOCK, 0, s>>>(ist, h->d_mpart, nb, inner_max);
       }
       HIPCHK(hipGetLastError());
@@ -1432,6 +1438,128 @@ static int run_mixed_bcgs(Mf6Handle *h, const TP *piv32) {
 
 // Solve the device-resident system (h->A, h->b, h->x): factor, residual,
 // Krylov.  Equivalent of imslinear_ap without scaling / reordering.
+// ---------------------------------------------------------------------------
+// IMS SCALING_METHOD (ImsLinearBase.f90:619-754 ims_base_scale) on the
+// planes.  Slot s of row n (0 diagonal, 1 up, 2 back, 3 left, 4 right,
+// 5 front, 6 down; CSR order) holds A(n, m) for the neighbour m in that
+// direction; the same column entry seen from row m is m's slot 7 - s.
+// ---------------------------------------------------------------------------
+
+__device__ inline bool scl_has(const Layout &L, int k, int i, int j, int s) {
+  switch (s) {
+    case 0: return true;
+    case 1: return k > 0;
+    case 2: return i > 0;
+    case 3: return j > 0;
+    case 4: return j < L.ncol - 1;
+    case 5: return i < L.nrow - 1;
+    default: return k < L.nlay - 1;
+  }
+}
+
+__device__ inline int scl_nb(const Layout &L, int p, int c, int k, int l, int s) {
+  const int o = (1 - c) * L.H + k * L.HP;
+  switch (s) {
+    case 0: return p;
+    case 1: return p - L.HP;
+    case 2: return o + hn(l, -L.ncol);
+    case 3: return o + hn(l, -1);
+    case 4: return o + hn(l, 1);
+    case 5: return o + hn(l, L.ncol);
+    default: return p + L.HP;
+  }
+}
+
+// iscl 1: ds1 = ds2 = 1 / sqrt|a_nn|.  iscl 2: ds1 = 1 / (row L2 norm), 1
+// for a zero row, and the row scaled by it in place.
+__global__ void k_scale_rows(Layout L, int iscl, double *__restrict__ A,
+                             double *__restrict__ ds1, double *__restrict__ ds2) {
+  const int N = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    if (iscl == 1) {
+      double c1 = 1.0 / sqrt(fabs(A[p]));
+      ds1[p] = c1;
+      ds2[p] = c1;
+    } else {
+      double c1 = 0.0;
+      for (int s = 0; s < MF6_NSLOT; s++)
+        if (scl_has(L, k, i, j, s)) c1 = c1 + A[s * N + p] * A[s * N + p];
+      c1 = sqrt(c1);
+      c1 = c1 == 0.0 ? 1.0 : 1.0 / c1;
+      ds1[p] = c1;
+      for (int s = 0; s < MF6_NSLOT; s++)
+        if (scl_has(L, k, i, j, s)) A[s * N + p] = c1 * A[s * N + p];
+    }
+  }
+}
+
+// iscl 2: ds2 = 1 / (column L2 norm of the row-scaled A), 1 for a zero
+// column; the squares are summed in the reference's order, row by row
+// (rows n - NRC, n - ncol, n - 1, n, n + 1, n + ncol, n + NRC).
+__global__ void k_scale_cols(Layout L, const double *__restrict__ A, double *__restrict__ ds2) {
+  const int N = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    double c2 = 0.0;
+    for (int t = 0; t < MF6_NSLOT; t++) {
+      const int s = t < 3 ? t + 1 : (t == 3 ? 0 : t);
+      if (!scl_has(L, k, i, j, s)) continue;
+      const double v = s == 0 ? A[p] : A[(7 - s) * N + scl_nb(L, p, c, k, l, s)];
+      c2 = c2 + v * v;
+    }
+    ds2[p] = c2 == 0.0 ? 1.0 : 1.0 / sqrt(c2);
+  }
+}
+
+// iopt 0: A = (ds1 A) ds2 (iscl 1) or ds2(col) A (iscl 2, rows already
+// scaled), x = x / ds2, b = b ds1.  iopt 1: A = ((1 / ds1) A) (1 / ds2),
+// x = x ds2, b = b / ds1.
+__global__ void k_scale_apply(Layout L, int iopt, int iscl, double *__restrict__ A,
+                              double *__restrict__ x, double *__restrict__ b,
+                              const double *__restrict__ ds1, const double *__restrict__ ds2) {
+  const int N = L.NP;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+    int c, k, i, j, l;
+    if (!pos_to_cell(L, p, c, k, i, j, l)) continue;
+    const double c1 = ds1[p];
+    for (int s = 0; s < MF6_NSLOT; s++) {
+      if (!scl_has(L, k, i, j, s)) continue;
+      const double c2 = ds2[scl_nb(L, p, c, k, l, s)];
+      double a = A[s * N + p];
+      if (iopt == 0) a = iscl == 1 ? c1 * a * c2 : c2 * a;
+      else a = (1.0 / c1) * a * (1.0 / c2);
+      A[s * N + p] = a;
+    }
+    if (iopt == 0) {
+      x[p] = x[p] / ds2[p];
+      b[p] = b[p] * c1;
+    } else {
+      x[p] = x[p] * ds2[p];
+      b[p] = b[p] / c1;
+    }
+  }
+}
+
+// Scales (iopt 0) or unscales (iopt 1) A, x and b on the device, as
+// imslinear_ap does around the solve (ImsLinear.f90:646-652, :741-747).
+static int ims_scale(Mf6Handle *h, int iopt) {
+  const Layout &L = h->L;
+  hipStream_t s = h->stream;
+  const int g = grid_for(L.N);
+  if (!h->d_dscale) CK(dalloc(&h->d_dscale, 2 * (size_t)L.NP));
+  double *ds1 = h->d_dscale, *ds2 = h->d_dscale + L.NP;
+  if (iopt == 0) {
+    k_scale_rows<<<g, MF6_BLOCK, 0, s>>>(L, h->s.iscl, h->A, ds1, ds2);
+    if (h->s.iscl == 2) k_scale_cols<<<g, MF6_BLOCK, 0, s>>>(L, h->A, ds2);
+  }
+  k_scale_apply<<<g, MF6_BLOCK, 0, s>>>(L, iopt, h->s.iscl, h->A, h->x, h->b, ds1, ds2);
+  HIPCHK(hipGetLastError());
+  return MF6HIP_OK;
+}
+
 int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_iter,
                         mf6hip_cnvg_out *out) {
   (void)kiter;
@@ -1441,6 +1569,7 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
   h->last_spmv_n = 0;
   h->spmv_calls = 0;
   HIPCHK(hipEventRecord(h->ev0, s));
+  if (h->s.iscl != 0) CK(ims_scale(h, 0));
   CK(ilu_factor(h));
   int sym = 0;
   if (h->s.ilinmeth == 1 && h->s.north == 0) CK(mf6hip_is_symmetric(h, &sym));
@@ -1483,6 +1612,7 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
       else CK(run_bcgs(h));
     }
   }
+  if (h->s.iscl != 0) CK(ims_scale(h, 1));
   HIPCHK(hipEventRecord(h->ev1, s));
   HIPCHK(hipEventSynchronize(h->ev1));
   float ms = 0;
@@ -1531,7 +1661,11 @@ int mf6hip_device_solve(Mf6Handle *h, int kstp, int kiter, int *icnvg, int *in_i
 // ---------------------------------------------------------------------------
 
 static int check_settings(const mf6hip_ims_settings *s) {
-  if (s->iscl != 0) { mf6hip_set_error("SCALING_METHOD is not supported on the device path"); return MF6HIP_EUNSUPPORTED; }
+  if (s->iscl < 0 || s->iscl > 2) { mf6hip_set_error("SCALING_METHOD must be 0 (none), 1 (symmetric) or 2 (L2 norm)"); return MF6HIP_EINVAL; }
+  if (s->iscl != 0 && (s->precision != MF6HIP_FP64 || s->preconditioner != 0)) {
+    mf6hip_set_error("SCALING_METHOD runs with FP64 precision and the ILU0/MILU0 preconditioner");
+    return MF6HIP_EUNSUPPORTED;
+  }
   if (s->iord != 0) { mf6hip_set_error("REORDERING_METHOD is not supported on the device path"); return MF6HIP_EUNSUPPORTED; }
   if (s->level > 0 || s->droptol > 0.0) { mf6hip_set_error("ILUT/MILUT (PRECONDITIONER_LEVELS / DROP_TOLERANCE) is not supported on the device path"); return MF6HIP_EUNSUPPORTED; }
   if (s->ilinmeth != 1 && s->ilinmeth != 2) { mf6hip_set_error("LINEAR_ACCELERATION must be CG (1) or BICGSTAB (2)"); return MF6HIP_EINVAL; }
@@ -1778,7 +1912,7 @@ extern "C" int mf6hip_destroy(void *handle) {
                   h->d, h->p, h->p2, h->q, h->z, h->zf, h->t, h->v, h->dhat, h->phat,
                   h->qhat, h->A32, h->invpiv32, h->invpiv16, h->r32, h->e32, h->p32,
                   h->p232, h->q32, h->z32, h->zf32, h->d_part, h->d_mpart, h->d_state,
-                  h->d_hist, h->d_hloc, h->d_tmp_nat, h->d_amat_csr, h->top, h->bot,
+                  h->d_hist, h->d_dscale, h->d_hloc, h->d_tmp_nat, h->d_amat_csr, h->top, h->bot,
                   h->area, h->k11, h->k22, h->sat0, h->hn0, h->k33, h->ss, h->sy, h->delr, h->delc, h->icelltype,
                   h->iconvert, h->ibound, h->ibound0, h->condsat, h->xold, h->dxold,
                   h->wsave, h->hchold, h->deold, h->ibotnode, h->bnd_off, h->bnd_ent,

@@ -123,7 +123,7 @@ def _create(settings, nlay=2, nrow=3, ncol=4, ia=None, ja=None, neq=None):
 
 
 @pytest.mark.parametrize("kw,code,msg", [
-    (dict(iscl=1), 3, "SCALING_METHOD"),
+    (dict(iscl=1, preconditioner=2), 3, "SCALING_METHOD"),
     (dict(iord=1), 3, "REORDERING_METHOD"),
     (dict(level=1), 3, "ILUT"),
     (dict(droptol=1e-3), 3, "ILUT"),
