@@ -60,7 +60,7 @@ typedef struct mf6hip_ims_settings {
   int icnvgopt;     /* 0 default, 1 STRICT, 2 L2NORM_RCLOSE, 3 RELATIVE, 4 */
   int iter1;        /* INNER_MAXIMUM */
   int ilinmeth;     /* 1 CG, 2 BICGSTAB */
-  int iscl;         /* SCALING_METHOD: must be 0 on the device path */
+  int iscl;         /* SCALING_METHOD: 0 none, 1 symmetric, 2 L2 norm (FP64, ILU0/MILU0 only) */
   int iord;         /* REORDERING_METHOD: must be 0 on the device path */
   int north;        /* NUMBER_ORTHOGONALIZATIONS */
   double relax;     /* RELAXATION_FACTOR: 0 -> ILU0, > 0 -> MILU0 */
